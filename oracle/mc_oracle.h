@@ -1,0 +1,177 @@
+/*
+ * MoCHII photon-packet transport + cell-balance: FP64 CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a from-scratch C restatement of the
+ * reference algorithm (seoncafe/MoCHII, Fortran 90) for the one hot path the
+ * HIP library implements.  It exists to check the HIP path: only the tests,
+ * the smoke check and the CPU-baseline leg of the benchmark load it.  The
+ * product library never links or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Parity status of
+ * each piece (what pins it) is listed in the header comment of the .c file
+ * that holds it.
+ */
+#ifndef MC_ORACLE_H
+#define MC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- constants (src/define.f90:12-88) ---------------------------------- */
+#define ORC_PI      3.141592653589793238462643383279502884197
+#define ORC_TWOPI   6.283185307179586476925286766559005768394
+#define ORC_FOURPI  12.56637061435917295385057353311801153679
+#define ORC_EV2ERG  1.602176634e-12
+#define ORC_HPLANCK 6.62607015e-27
+#define ORC_KBOLTZ  1.380649e-16
+#define ORC_PC2CM   3.0856776e18
+#define ORC_ETH_HI   13.598
+#define ORC_ETH_HEI  24.587
+#define ORC_ETH_HEII 54.416
+#define ORC_TAU_HUGE 745.2
+#define ORC_MAXSRC   16
+#define ORC_QMC_MAXDIM 12
+
+/* ---- QMC (src/qmc_mod.f90) ---------------------------------------------- */
+typedef struct {
+    uint32_t V[32][ORC_QMC_MAXDIM];   /* direction numbers V_d(k), k=1..32 */
+    uint32_t key[ORC_QMC_MAXDIM][2];  /* Owen keys per dimension and stream */
+} orc_qmc;
+
+void     orc_qmc_setup(orc_qmc *q, int seed);
+uint32_t orc_sobol_code(const orc_qmc *q, uint64_t idx, int d);   /* d 0-based */
+uint32_t orc_owen_scramble(uint32_t x, uint32_t seed);
+uint32_t orc_hash32(uint32_t x);
+void     orc_qmc_uniforms(const orc_qmc *q, uint64_t idx, int n, int stream, double *u);
+void     orc_qmc_raw(const orc_qmc *q, uint64_t idx, int n, double *u);
+
+/* ---- energy sampler (src/energy_sampler_mod.f90) ------------------------ */
+typedef struct {
+    int n;
+    double *energy, *density, *cdf;
+    double total_integral;
+} orc_sampler;
+
+int    orc_build_tabulated(orc_sampler *s, const double *e, const double *f, int n);
+int    orc_build_planck(orc_sampler *s, double T, double emin, double emax,
+                        double rtol, const double *thr, int nthr);
+double orc_sample_energy(const orc_sampler *s, double u);
+double orc_energy_cdf_value(const orc_sampler *s, double E);
+double orc_planck_shape(double E, double T);
+void   orc_sampler_free(orc_sampler *s);
+
+/* ---- cross sections and rate coefficients ------------------------------- */
+double orc_sigma_hydrogenic(double E, double Eth, double Z);
+double orc_sigma_vfky96(double E, double Eth, double E0, double s0, double ya,
+                        double P, double yw, double y0, double y1);
+double orc_sigma_HI(double E);
+double orc_sigma_HeI(double E);
+double orc_sigma_HeII(double E);
+double orc_alphaA_HII(double T);
+double orc_alphaB_HII(double T);
+double orc_alphaA_HeII(double T);
+double orc_alphaB_HeII(double T);
+double orc_alphaA_HeIII(double T);
+double orc_alphaB_HeIII(double T);
+double orc_ci_HI(double T);
+double orc_ci_HeI(double T);
+double orc_ci_HeII(double T);
+
+/* ---- counter-based RNG for the post-launch draws ------------------------ */
+double orc_rng_uniform(uint32_t seed, int stream, uint64_t id, uint32_t k);
+
+/* ---- model context ------------------------------------------------------- */
+enum { ORC_GRID_CAR = 0, ORC_GRID_AMR = 1 };
+enum { ORC_GEOM_POINT = 0, ORC_GEOM_EXT_SPH = 1, ORC_GEOM_EXT_REC = 2 };
+
+typedef struct {
+    /* launch */
+    int64_t nphotons;
+    int     qmc_seed;
+    double  eion_min, eion_max, source_cdf_tol;
+    int     nnu_ion, align_edges;
+    int     nsource;                 /* internal point sources */
+    double  tstar[ORC_MAXSRC], lum[ORC_MAXSRC];
+    double  sx[ORC_MAXSRC], sy[ORC_MAXSRC], sz[ORC_MAXSRC];
+    int     ext_on;                  /* external isotropic field */
+    int     ext_geometry;            /* ORC_GEOM_EXT_SPH / _REC */
+    double  ext_tstar, ext_intensity, rmax;
+    /* grid */
+    int     grid_type, car_walk_amr;
+    int     nx, ny, nz;
+    double  xmax, ymax, zmax;        /* box is [-xmax,xmax] etc. */
+    double  distance2cm;
+    /* gas */
+    double  He_abund, te_fixed, ion_relax, te_min, te_max;
+    int     caseA;
+    /* dust */
+    int     add_dust, dust_scatter;
+    double  lambda_ref;
+} orc_params;
+
+typedef struct {
+    orc_params p;
+    orc_qmc    qmc;
+    /* band */
+    int     nnu_band;
+    double *eedge;                   /* nnu_band+1 */
+    int     ncomp, multi_src;
+    int     comp_kind[ORC_MAXSRC + 1];
+    double  src_cdf[ORC_MAXSRC + 1];
+    orc_sampler samp[ORC_MAXSRC + 1];
+    double  Ltot;
+    int     qmc_ndim;
+    /* grid */
+    int     nleaf, ncells, car;
+    double  xmin, xmaxb, ymin, ymaxb, zmin, zmaxb, xrange, yrange, zrange, dcell;
+    int    *parent, *children, *level, *ileaf, *icell_of_leaf, *neighbor;
+    double *cx, *cy, *cz, *ch;
+    /* leaf state */
+    double *nH, *xHI, *xHeI, *xHeII, *ne, *Te, *rhokap;
+    /* dust optics table (sorted by wavelength) */
+    int     dn;
+    double *dlam, *dalb, *dg, *dcext, dcref;
+    /* tallies (leaf-major: hhe[il*6+k], jt[il*nnu+inu]) */
+    double *hhe, *jt, *dust_abs;
+    int64_t seg_direct, seg_scattered;
+    double  e_emit, e_abs, e_esc;
+    /* rates */
+    double *gHI, *gHeI, *gHeII, *hHI, *hHeI, *hHeII;
+} orc_ctx;
+
+orc_ctx *orc_create(const orc_params *p);
+void     orc_destroy(orc_ctx *c);
+int      orc_setup_band(orc_ctx *c);
+int      orc_grid_car(orc_ctx *c);
+int      orc_grid_octree(orc_ctx *c, int nleaf, const double *xl, const double *yl,
+                         const double *zl, const int *lev, double half);
+int      orc_state_init(orc_ctx *c, const double *nH, double xHI0, double xHeI0,
+                        double xHeII0);
+int      orc_set_dust_table(orc_ctx *c, int n, const double *lam, const double *alb,
+                            const double *g, const double *cext);
+void     orc_dust_at(const orc_ctx *c, double E, double *sabs, double *ssca, double *g);
+int      orc_bin_of(const orc_ctx *c, double E);
+
+/* one transport pass over global packet ids [first, last) (1-based ids),
+ * with nthreads worker threads; tallies are zeroed first */
+int      orc_transport_pass(orc_ctx *c, int64_t first, int64_t last, int nthreads);
+void     orc_rates_compute(orc_ctx *c);
+void     orc_equilibrium_update(orc_ctx *c, double *max_dx, double *dx_vol);
+void     orc_solve_ion_cell(double gH, double gHe1, double gHe2, double nH, double T,
+                            int caseA, double yHe, double *xHI, double *xHeI,
+                            double *xHeII, double *ne);
+void     orc_residual_grid(orc_ctx *c, double *eps_max, double *eps_vol);
+
+/* diagnostics: leaves crossed by the direct walk of one ray (no tally) */
+int      orc_trace_ray(orc_ctx *c, double x, double y, double z, double kx,
+                       double ky, double kz, int *cells, double *segs, int maxn);
+/* diagnostics: launch state of one stellar packet */
+int      orc_launch(orc_ctx *c, int64_t id, double *out9);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,371 @@
+"""ctypes binding to the FP64 CPU oracle (test infrastructure only).
+
+Loaded by the tests, the smoke check and the CPU-baseline leg of bench.py.
+The product package never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libmochii_ref.so")
+
+MAXSRC = 16
+GRID_CAR, GRID_AMR = 0, 1
+GEOM_POINT, GEOM_EXT_SPH, GEOM_EXT_REC = 0, 1, 2
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("nphotons", C.c_int64), ("qmc_seed", C.c_int),
+        ("eion_min", C.c_double), ("eion_max", C.c_double), ("source_cdf_tol", C.c_double),
+        ("nnu_ion", C.c_int), ("align_edges", C.c_int),
+        ("nsource", C.c_int),
+        ("tstar", C.c_double * MAXSRC), ("lum", C.c_double * MAXSRC),
+        ("sx", C.c_double * MAXSRC), ("sy", C.c_double * MAXSRC), ("sz", C.c_double * MAXSRC),
+        ("ext_on", C.c_int), ("ext_geometry", C.c_int),
+        ("ext_tstar", C.c_double), ("ext_intensity", C.c_double), ("rmax", C.c_double),
+        ("grid_type", C.c_int), ("car_walk_amr", C.c_int),
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("xmax", C.c_double), ("ymax", C.c_double), ("zmax", C.c_double),
+        ("distance2cm", C.c_double),
+        ("He_abund", C.c_double), ("te_fixed", C.c_double), ("ion_relax", C.c_double),
+        ("te_min", C.c_double), ("te_max", C.c_double),
+        ("caseA", C.c_int),
+        ("add_dust", C.c_int), ("dust_scatter", C.c_int),
+        ("lambda_ref", C.c_double),
+    ]
+
+
+class OrcCtx(C.Structure):
+    """Leading fields of orc_ctx that Python reads directly."""
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"oracle library not built: {LIB_PATH} (run make -C oracle)")
+    lib = C.CDLL(LIB_PATH)
+    lib.orc_create.restype = C.c_void_p
+    lib.orc_create.argtypes = [C.POINTER(OrcParams)]
+    lib.orc_destroy.argtypes = [C.c_void_p]
+    for name in ("orc_setup_band", "orc_grid_car"):
+        getattr(lib, name).argtypes = [C.c_void_p]
+        getattr(lib, name).restype = C.c_int
+    lib.orc_grid_octree.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _ip, C.c_double]
+    lib.orc_state_init.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, C.c_double]
+    lib.orc_set_dust_table.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]
+    lib.orc_dust_at.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp]
+    lib.orc_bin_of.argtypes = [C.c_void_p, C.c_double]
+    lib.orc_bin_of.restype = C.c_int
+    lib.orc_transport_pass.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int]
+    lib.orc_rates_compute.argtypes = [C.c_void_p]
+    lib.orc_equilibrium_update.argtypes = [C.c_void_p, _dp, _dp]
+    lib.orc_residual_grid.argtypes = [C.c_void_p, _dp, _dp]
+    lib.orc_solve_ion_cell.argtypes = [C.c_double] * 5 + [C.c_int, C.c_double, _dp, _dp, _dp, _dp]
+    lib.orc_trace_ray.argtypes = [C.c_void_p] + [C.c_double] * 6 + [_ip, _dp, C.c_int]
+    lib.orc_trace_ray.restype = C.c_int
+    lib.orc_launch.argtypes = [C.c_void_p, C.c_int64, _dp]
+    lib.orc_qmc_uniforms.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int, _dp]
+    lib.orc_qmc_raw.argtypes = [C.c_void_p, C.c_uint64, C.c_int, _dp]
+    lib.orc_qmc_setup.argtypes = [C.c_void_p, C.c_int]
+    lib.orc_sobol_code.argtypes = [C.c_void_p, C.c_uint64, C.c_int]
+    lib.orc_sobol_code.restype = C.c_uint32
+    lib.orc_rng_uniform.argtypes = [C.c_uint32, C.c_int, C.c_uint64, C.c_uint32]
+    lib.orc_rng_uniform.restype = C.c_double
+    for name in ("orc_sigma_HI", "orc_sigma_HeI", "orc_sigma_HeII", "orc_alphaA_HII",
+                 "orc_alphaB_HII", "orc_alphaA_HeII", "orc_alphaB_HeII", "orc_alphaA_HeIII",
+                 "orc_alphaB_HeIII", "orc_ci_HI", "orc_ci_HeI", "orc_ci_HeII",
+                 "orc_planck_shape"):
+        f = getattr(lib, name)
+        f.restype = C.c_double
+        f.argtypes = [C.c_double] * (2 if name == "orc_planck_shape" else 1)
+    lib.orc_build_planck.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                     C.c_double, _dp, C.c_int]
+    lib.orc_build_tabulated.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
+    lib.orc_sample_energy.argtypes = [C.c_void_p, C.c_double]
+    lib.orc_sample_energy.restype = C.c_double
+    lib.orc_energy_cdf_value.argtypes = [C.c_void_p, C.c_double]
+    lib.orc_energy_cdf_value.restype = C.c_double
+    lib.orc_sampler_free.argtypes = [C.c_void_p]
+    return lib
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = _load()
+    return _LIB
+
+
+def ptr(a, ct=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+# ---- QMC ----------------------------------------------------------------
+class Qmc:
+    SIZE = 32 * 12 * 4 + 12 * 2 * 4
+
+    def __init__(self, seed):
+        self.buf = C.create_string_buffer(self.SIZE)
+        lib().orc_qmc_setup(self.buf, seed)
+
+    def uniforms(self, idx, n, stream=1):
+        u = np.zeros(n)
+        lib().orc_qmc_uniforms(self.buf, idx, n, stream, ptr(u))
+        return u
+
+    def raw(self, idx, n):
+        u = np.zeros(n)
+        lib().orc_qmc_raw(self.buf, idx, n, ptr(u))
+        return u
+
+    def code(self, idx, d):
+        return lib().orc_sobol_code(self.buf, idx, d)
+
+
+# ---- samplers -------------------------------------------------------------
+class _SamplerStruct(C.Structure):
+    _fields_ = [("n", C.c_int), ("energy", _dp), ("density", _dp), ("cdf", _dp),
+                ("total_integral", C.c_double)]
+
+
+class Sampler:
+    def __init__(self):
+        self.s = _SamplerStruct()
+
+    @classmethod
+    def planck(cls, T, emin, emax, rtol=1e-8, thresholds=()):
+        o = cls()
+        thr = np.ascontiguousarray(thresholds, dtype=np.float64)
+        rc = lib().orc_build_planck(C.byref(o.s), T, emin, emax, rtol,
+                                    ptr(thr) if thr.size else None, thr.size)
+        if rc:
+            raise RuntimeError(f"orc_build_planck rc={rc}")
+        return o
+
+    @classmethod
+    def tabulated(cls, e, f):
+        o = cls()
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        rc = lib().orc_build_tabulated(C.byref(o.s), ptr(e), ptr(f), e.size)
+        if rc:
+            raise RuntimeError(f"orc_build_tabulated rc={rc}")
+        return o
+
+    def arrays(self):
+        n = self.s.n
+        return (np.ctypeslib.as_array(self.s.energy, (n,)).copy(),
+                np.ctypeslib.as_array(self.s.density, (n,)).copy(),
+                np.ctypeslib.as_array(self.s.cdf, (n,)).copy(), self.s.total_integral)
+
+    def sample(self, u):
+        return lib().orc_sample_energy(C.byref(self.s), u)
+
+    def cdf(self, e):
+        return lib().orc_energy_cdf_value(C.byref(self.s), e)
+
+    def __del__(self):
+        try:
+            lib().orc_sampler_free(C.byref(self.s))
+        except Exception:
+            pass
+
+
+# ---- model ----------------------------------------------------------------
+class Model:
+    """One oracle context: params, band, grid, leaf state, tallies."""
+
+    def __init__(self, params: dict, octree=None, nH=None, dust_table=None):
+        p = OrcParams()
+        for k, v in params.items():
+            if k in ("tstar", "lum", "sx", "sy", "sz"):
+                arr = getattr(p, k)
+                for i, x in enumerate(v):
+                    arr[i] = x
+            else:
+                setattr(p, k, v)
+        self.p = p
+        self.params = dict(params)
+        L = lib()
+        self.c = L.orc_create(C.byref(p))
+        rc = L.orc_setup_band(self.c)
+        if rc:
+            raise RuntimeError(f"orc_setup_band rc={rc}")
+        if octree is None:
+            L.orc_grid_car(self.c)
+        else:
+            xl, yl, zl, lev, half = octree
+            self._oct = [np.ascontiguousarray(a, dtype=np.float64) for a in (xl, yl, zl)]
+            self._lev = np.ascontiguousarray(lev, dtype=np.int32)
+            L.orc_grid_octree(self.c, self._lev.size, ptr(self._oct[0]), ptr(self._oct[1]),
+                              ptr(self._oct[2]), ptr(self._lev, C.c_int), half)
+        self.nleaf = C.cast(self.c, C.POINTER(_CtxHead)).contents.nleaf
+        if dust_table is not None:
+            lam, alb, g, cext = [np.ascontiguousarray(a, dtype=np.float64) for a in dust_table]
+            L.orc_set_dust_table(self.c, lam.size, ptr(lam), ptr(alb), ptr(g), ptr(cext))
+        if nH is not None:
+            self.state_init(nH)
+
+    # raw pointer access to the context arrays
+    def _head(self):
+        return C.cast(self.c, C.POINTER(_CtxHead)).contents
+
+    def arr(self, name, n=None):
+        h = self._head()
+        n = self.nleaf if n is None else n
+        return np.ctypeslib.as_array(getattr(h, name), (n,))
+
+    def state_init(self, nH, rhokap=None):
+        nH = np.ascontiguousarray(nH, dtype=np.float64)
+        P = self.params
+        lib().orc_state_init(self.c, ptr(nH), P.get("xHI_init", 1.0), P.get("xHeI_init", 1.0),
+                             P.get("xHeII_init", 0.0))
+        if rhokap is not None:
+            self.arr("rhokap")[:] = rhokap
+
+    def transport_pass(self, first=1, last=None, nthreads=1):
+        if last is None:
+            last = self.p.nphotons + 1
+        lib().orc_transport_pass(self.c, first, last, nthreads)
+
+    def rates_compute(self):
+        lib().orc_rates_compute(self.c)
+
+    def equilibrium_update(self):
+        a, b = C.c_double(), C.c_double()
+        lib().orc_equilibrium_update(self.c, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def residual_grid(self):
+        a, b = C.c_double(), C.c_double()
+        lib().orc_residual_grid(self.c, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def tallies(self):
+        h = self._head()
+        nl, nnu = self.nleaf, h.nnu_band
+        return dict(
+            hhe=np.ctypeslib.as_array(h.hhe, (nl * 6,)).reshape(nl, 6).copy(),
+            jt=np.ctypeslib.as_array(h.jt, (nl * nnu,)).reshape(nl, nnu).copy(),
+            dust_abs=np.ctypeslib.as_array(h.dust_abs, (nl,)).copy(),
+            seg_direct=h.seg_direct, seg_scattered=h.seg_scattered,
+            ledger=(h.e_emit, h.e_abs, h.e_esc))
+
+    def rates(self):
+        return {k: self.arr(k).copy() for k in ("gHI", "gHeI", "gHeII", "hHI", "hHeI", "hHeII")}
+
+    def state(self):
+        return {k: self.arr(k).copy() for k in ("nH", "xHI", "xHeI", "xHeII", "ne", "Te")}
+
+    def set_state(self, **kw):
+        for k, v in kw.items():
+            self.arr(k)[:] = v
+
+    def band(self):
+        h = self._head()
+        return dict(nnu_band=h.nnu_band,
+                    eedge=np.ctypeslib.as_array(h.eedge, (h.nnu_band + 1,)).copy(),
+                    Ltot=h.Ltot, qmc_ndim=h.qmc_ndim, ncomp=h.ncomp)
+
+    def bin_of(self, E):
+        return lib().orc_bin_of(self.c, E)
+
+    def dust_at(self, E):
+        a, b, g = C.c_double(), C.c_double(), C.c_double()
+        lib().orc_dust_at(self.c, E, C.byref(a), C.byref(b), C.byref(g))
+        return a.value, b.value, g.value
+
+    def trace_ray(self, x, y, z, kx, ky, kz, maxn=100000):
+        cells = np.zeros(maxn, dtype=np.int32)
+        segs = np.zeros(maxn)
+        n = lib().orc_trace_ray(self.c, x, y, z, kx, ky, kz, ptr(cells, C.c_int), ptr(segs), maxn)
+        return cells[:n].copy(), segs[:n].copy()
+
+    def launch(self, pid):
+        out = np.zeros(9)
+        lib().orc_launch(self.c, pid, ptr(out))
+        return out
+
+    def __del__(self):
+        try:
+            if self.c:
+                lib().orc_destroy(self.c)
+                self.c = None
+        except Exception:
+            pass
+
+
+class _CtxHead(C.Structure):
+    """Mirror of orc_ctx (mc_oracle.h) up to the rate arrays."""
+    _fields_ = [
+        ("p", OrcParams),
+        ("qmc", C.c_uint32 * (32 * 12 + 12 * 2)),
+        ("nnu_band", C.c_int), ("eedge", _dp),
+        ("ncomp", C.c_int), ("multi_src", C.c_int),
+        ("comp_kind", C.c_int * (MAXSRC + 1)),
+        ("src_cdf", C.c_double * (MAXSRC + 1)),
+        ("samp", _SamplerStruct * (MAXSRC + 1)),
+        ("Ltot", C.c_double), ("qmc_ndim", C.c_int),
+        ("nleaf", C.c_int), ("ncells", C.c_int), ("car", C.c_int),
+        ("xmin", C.c_double), ("xmaxb", C.c_double), ("ymin", C.c_double), ("ymaxb", C.c_double),
+        ("zmin", C.c_double), ("zmaxb", C.c_double), ("xrange", C.c_double),
+        ("yrange", C.c_double), ("zrange", C.c_double), ("dcell", C.c_double),
+        ("parent", _ip), ("children", _ip), ("level", _ip), ("ileaf", _ip),
+        ("icell_of_leaf", _ip), ("neighbor", _ip),
+        ("cx", _dp), ("cy", _dp), ("cz", _dp), ("ch", _dp),
+        ("nH", _dp), ("xHI", _dp), ("xHeI", _dp), ("xHeII", _dp), ("ne", _dp), ("Te", _dp),
+        ("rhokap", _dp),
+        ("dn", C.c_int), ("dlam", _dp), ("dalb", _dp), ("dg", _dp), ("dcext", _dp),
+        ("dcref", C.c_double),
+        ("hhe", _dp), ("jt", _dp), ("dust_abs", _dp),
+        ("seg_direct", C.c_int64), ("seg_scattered", C.c_int64),
+        ("e_emit", C.c_double), ("e_abs", C.c_double), ("e_esc", C.c_double),
+        ("gHI", _dp), ("gHeI", _dp), ("gHeII", _dp), ("hHI", _dp), ("hHeI", _dp), ("hHeII", _dp),
+    ]
+
+
+def solve_ion_cell(gH, gHe1, gHe2, nH, T, caseA, yHe, xHI, xHeI, xHeII):
+    a, b, c, ne = C.c_double(xHI), C.c_double(xHeI), C.c_double(xHeII), C.c_double()
+    lib().orc_solve_ion_cell(gH, gHe1, gHe2, nH, T, int(caseA), yHe, C.byref(a), C.byref(b),
+                             C.byref(c), C.byref(ne))
+    return a.value, b.value, c.value, ne.value
+
+
+def rng_uniform(seed, stream, pid, k):
+    return lib().orc_rng_uniform(seed, stream, pid, k)
+
+
+# ---- the reference's own modules compiled from source (optional) ----------
+def ref_lib():
+    """The reference Fortran modules built into oracle/_ref, or None."""
+    if not os.path.exists(REF_PATH):
+        return None
+    L = C.CDLL(REF_PATH)
+    L.ref_qmc_setup.argtypes = [C.c_int]
+    L.ref_qmc_uniforms.argtypes = [C.c_int64, C.c_int, C.c_int, _dp]
+    L.ref_qmc_raw.argtypes = [C.c_int64, C.c_int, _dp]
+    L.ref_build_planck.argtypes = [C.c_double] * 4 + [_dp, C.c_int]
+    L.ref_build_planck.restype = C.c_int
+    L.ref_build_tabulated.argtypes = [_dp, _dp, C.c_int]
+    L.ref_build_tabulated.restype = C.c_int
+    L.ref_sampler_get.argtypes = [_dp, _dp, _dp, _dp]
+    L.ref_sample_energy.argtypes = [C.c_double]
+    L.ref_sample_energy.restype = C.c_double
+    L.ref_cdf_value.argtypes = [C.c_double]
+    L.ref_cdf_value.restype = C.c_double
+    L.ref_sigma.argtypes = [C.c_int, C.c_double]
+    L.ref_sigma.restype = C.c_double
+    L.ref_rate.argtypes = [C.c_int, C.c_double]
+    L.ref_rate.restype = C.c_double
+    L.ref_gaunt_ff.argtypes = [C.c_float, C.c_float, C.POINTER(C.c_float),
+                               C.POINTER(C.c_float), C.c_int]
+    return L

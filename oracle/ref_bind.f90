@@ -1,0 +1,121 @@
+! C-callable entry points into the reference's own Fortran modules
+! (qmc_mod, energy_sampler_mod, photo_xsec, recomb_mod, gaunt), compiled from
+! their sources under the reference checkout by oracle/Makefile.  Test
+! infrastructure only: the tests use it to check the C oracle against the
+! reference code itself.
+module ref_bind
+  use, intrinsic :: iso_c_binding
+  use define, only : wp, par
+  use qmc_mod, only : qmc_setup, qmc_uniforms_stream, qmc_uniforms_raw
+  use energy_sampler_mod, only : energy_sampler_type, build_planck_sampler, &
+       build_tabulated_sampler, sample_energy_cdf, energy_cdf_value
+  use photo_xsec, only : sigma_HI, sigma_HeI, sigma_HeII
+  use recomb_mod
+  implicit none
+  type(energy_sampler_type), save :: smp
+
+  interface
+     subroutine getGauntFF(z, log10Te, xlf, g, iflag)
+        implicit none
+        integer, intent(out) :: iflag
+        real, intent(in)  :: log10Te, z
+        real, dimension(:), intent(in)  :: xlf
+        real, dimension(size(xlf)), intent(out) :: g
+     end subroutine getGauntFF
+  end interface
+
+contains
+
+  subroutine ref_qmc_setup(seed) bind(C, name='ref_qmc_setup')
+    integer(c_int), value :: seed
+    call qmc_setup(int(seed))
+  end subroutine ref_qmc_setup
+
+  subroutine ref_qmc_uniforms(idx, n, stream, u) bind(C, name='ref_qmc_uniforms')
+    integer(c_int64_t), value :: idx
+    integer(c_int), value :: n, stream
+    real(c_double), intent(out) :: u(n)
+    call qmc_uniforms_stream(int(idx, 8), u, int(stream))
+  end subroutine ref_qmc_uniforms
+
+  subroutine ref_qmc_raw(idx, n, u) bind(C, name='ref_qmc_raw')
+    integer(c_int64_t), value :: idx
+    integer(c_int), value :: n
+    real(c_double), intent(out) :: u(n)
+    call qmc_uniforms_raw(int(idx, 8), u)
+  end subroutine ref_qmc_raw
+
+  function ref_build_planck(T, emin, emax, rtol, thr, nthr) result(n) &
+       bind(C, name='ref_build_planck')
+    real(c_double), value :: T, emin, emax, rtol
+    integer(c_int), value :: nthr
+    real(c_double), intent(in) :: thr(nthr)
+    integer(c_int) :: n
+    call build_planck_sampler(smp, T, emin, emax, rtol, thr)
+    n = size(smp%energy)
+  end function ref_build_planck
+
+  function ref_build_tabulated(e, f, n) result(m) bind(C, name='ref_build_tabulated')
+    integer(c_int), value :: n
+    real(c_double), intent(in) :: e(n), f(n)
+    integer(c_int) :: m
+    call build_tabulated_sampler(smp, e, f)
+    m = size(smp%energy)
+  end function ref_build_tabulated
+
+  subroutine ref_sampler_get(e, d, c, total) bind(C, name='ref_sampler_get')
+    real(c_double), intent(out) :: e(*), d(*), c(*), total
+    integer :: n
+    n = size(smp%energy)
+    e(1:n) = smp%energy;  d(1:n) = smp%density;  c(1:n) = smp%cdf
+    total = smp%total_integral
+  end subroutine ref_sampler_get
+
+  real(c_double) function ref_sample_energy(u) bind(C, name='ref_sample_energy')
+    real(c_double), value :: u
+    ref_sample_energy = sample_energy_cdf(smp, u)
+  end function ref_sample_energy
+
+  real(c_double) function ref_cdf_value(e) bind(C, name='ref_cdf_value')
+    real(c_double), value :: e
+    ref_cdf_value = energy_cdf_value(smp, e)
+  end function ref_cdf_value
+
+  real(c_double) function ref_sigma(which, e) bind(C, name='ref_sigma')
+    integer(c_int), value :: which
+    real(c_double), value :: e
+    select case (which)
+    case (0); ref_sigma = sigma_HI(e)
+    case (1); ref_sigma = sigma_HeI(e)
+    case default; ref_sigma = sigma_HeII(e)
+    end select
+  end function ref_sigma
+
+  real(c_double) function ref_rate(which, T) bind(C, name='ref_rate')
+    integer(c_int), value :: which
+    real(c_double), value :: T
+    par%recomb_model = 'badnell_milne'
+    par%ci_model = 'voronov'
+    select case (which)
+    case (0); ref_rate = alphaA_HII(T)
+    case (1); ref_rate = alphaB_HII(T)
+    case (2); ref_rate = alphaA_HeII(T)
+    case (3); ref_rate = alphaB_HeII(T)
+    case (4); ref_rate = alphaA_HeIII(T)
+    case (5); ref_rate = alphaB_HeIII(T)
+    case (6); ref_rate = ci_HI(T)
+    case (7); ref_rate = ci_HeI(T)
+    case default; ref_rate = ci_HeII(T)
+    end select
+  end function ref_rate
+
+  subroutine ref_gaunt_ff(z, log10te, xlf, g, n) bind(C, name='ref_gaunt_ff')
+    real(c_float), value :: z, log10te
+    integer(c_int), value :: n
+    real(c_float), intent(in) :: xlf(n)
+    real(c_float), intent(out) :: g(n)
+    integer :: iflag
+    call getGauntFF(z, log10te, xlf, g, iflag)
+  end subroutine ref_gaunt_ff
+
+end module ref_bind
