@@ -1,0 +1,177 @@
+/*
+ * mochii_gpu.h -- C-ABI of the MI355X (gfx950) photon-packet transport and
+ * cell-balance library.
+ *
+ * The reference (seoncafe/MoCHII) is a Fortran 90 program whose hot path is a
+ * set of module procedures with module-global state.  This ABI replaces them
+ * at PASS granularity (one call = one transport pass or one grid-wide cell
+ * update); the per-packet procedures (gen_ion_photon_qmc, transport_ion_packet,
+ * score_ion_path, ...) run inside the GPU kernels.  Each entry point names the
+ * reference procedure it stands in for.
+ *
+ * Conventions
+ *   - plain C types only: pointers, sizes, doubles; no runtime-library types;
+ *   - every call returns int: 0 = ok, < 0 = error; mc_last_error() describes
+ *     the last error of the calling thread (the reference aborts the job via
+ *     fatal_error / MPI_ABORT instead, src/utility.f90:382-391);
+ *   - the library owns all device-resident state between calls; host arrays
+ *     are copied in/out only by the *_set / *_get calls;
+ *   - leaf arrays are indexed by the reference's 1-based leaf number il-1;
+ *     2-D tallies are returned in the reference's column-major layout, i.e.
+ *     hhe(6,nleaf) is 6 contiguous values per leaf, jt_ion(nnu_band,nleaf)
+ *     is nnu_band contiguous values per leaf;
+ *   - calls are synchronous on return and must come from one host thread
+ *     per context (the reference runs one single-threaded process per rank).
+ */
+#ifndef MOCHII_GPU_H
+#define MOCHII_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MC_MAXSRC 16
+
+/* precision forms */
+#define MC_FP64  0   /* all-FP64 transport (reference arithmetic) */
+#define MC_MIXED 1   /* FP32 cell-local geometry + opacity, FP64 tallies */
+
+/* grid types (par%grid_type) */
+#define MC_GRID_CAR 0
+#define MC_GRID_AMR 1
+
+/* external-field geometry (par%ext_geometry) */
+#define MC_EXT_SPH 1
+#define MC_EXT_REC 2
+
+/* leaf-state fields for mc_state_get / mc_state_set
+ * (src/gas_state_mod.f90:23-28, rhokap from src/octree_mod.f90:50) */
+#define MC_NH     0
+#define MC_XHI    1
+#define MC_XHEI   2
+#define MC_XHEII  3
+#define MC_NE     4
+#define MC_TE     5
+#define MC_RHOKAP 6
+
+/* Subset of the reference's par (src/define.f90:189-861) this path reads.
+ * Fortran character switches become enums/ints; logicals become int (0/1). */
+typedef struct mc_params {
+    int64_t nphotons;          /* par%nphotons                               */
+    int32_t qmc_seed;          /* par%qmc_seed (Owen scramble seed)           */
+    int32_t precision;         /* MC_FP64 | MC_MIXED                          */
+    double  eion_min, eion_max;/* par%eion_min, par%eion_max [eV]             */
+    double  source_cdf_tol;    /* par%source_cdf_tol                          */
+    int32_t nnu_ion;           /* par%nnu_ion (diagnostic J bins)             */
+    int32_t ion_align_edges;   /* par%ion_align_edges                         */
+    int32_t nsource;           /* par%nsource (internal point sources)        */
+    int32_t ext_on;            /* external field on (par%ext_intensity > 0)   */
+    double  tstar[MC_MAXSRC];  /* Planck T per source (par%tstar/src_tstar)   */
+    double  src_lum[MC_MAXSRC];/* band luminosity per source [erg/s]          */
+    double  src_x[MC_MAXSRC], src_y[MC_MAXSRC], src_z[MC_MAXSRC];
+    int32_t ext_geometry;      /* MC_EXT_SPH | MC_EXT_REC (par%ext_geometry)   */
+    int32_t car_walk_amr;      /* par%car_walk == 'amr' on a car grid          */
+    double  ext_tstar, ext_intensity, rmax;
+    int32_t grid_type;         /* MC_GRID_CAR | MC_GRID_AMR                    */
+    int32_t nx, ny, nz;        /* car grid dimensions                          */
+    double  xmax, ymax, zmax;  /* box half sizes [code units]                  */
+    double  distance2cm;       /* code length -> cm                            */
+    double  He_abund;          /* n_He/n_H                                     */
+    double  te_fixed, ion_relax, te_min, te_max, gas_tol, gas_tol_te;
+    int32_t case_A;            /* par%case_ab == 'A'                           */
+    int32_t solve_te;          /* par%solve_te                                 */
+    int32_t conv_vol;          /* par%conv_crit == 'vol'                       */
+    int32_t gas_niter;         /* par%gas_niter                                */
+    double  xHI_init, xHeI_init, xHeII_init;
+    int32_t ion_add_dust;      /* par%ion_add_dust                             */
+    int32_t ion_dust_scatter;  /* par%ion_dust_scatter                         */
+    double  lambda_ref;        /* par%lambda_ref [um]                          */
+} mc_params;
+
+typedef struct mc_ctx mc_ctx;
+
+/* Last error message of the calling thread ("" if none). */
+const char *mc_last_error(void);
+
+/* Set-up: ion_setup (src/ion_band_mod.f90:122-258) + ionizing_field_setup
+ * (src/ionizing_field_mod.f90:54-64) + jtally_ion_setup
+ * (src/jtally_mod.f90:197-210) + ion_score_setup (src/ion_score_mod.f90:71-109).
+ * For grid_type MC_GRID_CAR this also builds the namelist car grid
+ * (src/grid_mod_amr.f90:85-108, amr_build_car src/octree_mod.f90:648-674). */
+int mc_init(const mc_params *p, mc_ctx **ctx);
+int mc_finalize(mc_ctx *ctx);
+
+/* Octree from a leaf list: amr_build_tree + amr_build_neighbors
+ * (src/octree_mod.f90:347-497) as called from grid_create_amr
+ * (src/grid_mod_amr.f90:216-219).  Leaf coordinates are box-centred. */
+int mc_grid_octree(mc_ctx *ctx, int nleaf, const double *xleaf, const double *yleaf,
+                   const double *zleaf, const int32_t *level, double half);
+
+/* gas_state_setup (src/gas_state_mod.f90:41-90): nH per leaf; xHI may be
+ * NULL (then par%xHI_init); He fractions from par%xHeI_init/xHeII_init. */
+int mc_gas_state_setup(mc_ctx *ctx, const double *nH, const double *xHI);
+
+/* Leaf-state access (the module arrays of gas_state_mod / amr_grid%rhokap). */
+int mc_state_set(mc_ctx *ctx, int field, const double *src);
+int mc_state_get(mc_ctx *ctx, int field, double *dst);
+
+/* ion_dust_read (src/gas_opacity_mod.f90:69-205): (lambda[um], albedo, <cos>,
+ * C_ext/H) rows in any order; the library sorts them. */
+int mc_ion_dust_table(mc_ctx *ctx, int n, const double *lam, const double *albedo,
+                      const double *gcos, const double *cext);
+
+/* gas_opacity_fill (src/gas_opacity_mod.f90:299-332): refresh the per-leaf
+ * opacity inputs from the current state. */
+int mc_gas_opacity_fill(mc_ctx *ctx);
+
+/* ionizing_field_and_rates (src/ionizing_field_mod.f90:72-170): one transport
+ * pass of all par%nphotons stellar packets, reduction, gas_rates_compute and
+ * the energy-closure check (returns an error if closure > 5e-13, as
+ * ion_score_report_energy_closure aborts, src/ion_score_mod.f90:364-387).
+ * with_peel is accepted for signature parity and must be 0 (peel-off imaging
+ * is not on this path). */
+int mc_ionizing_field_and_rates(mc_ctx *ctx, int with_peel);
+
+/* gas_equilibrium_update (src/ion_balance_mod.f90:352-434). */
+int mc_gas_equilibrium_update(mc_ctx *ctx, double *max_dx, double *dx_vol);
+
+/* hydrogen_balance_residual_grid (src/ion_balance_mod.f90:313-343). */
+int mc_hydrogen_balance_residual_grid(mc_ctx *ctx, double *eps_max, double *eps_vol);
+
+/* Tallies of the last pass in the reference's layouts: hhe(6,nleaf)
+ * (ion_score_mod hhe_rate_path), jt_ion(nnu_band,nleaf) (jtally_mod), dust_abs
+ * (nleaf), segment counts {direct, scattered} (any pointer may be NULL). */
+int mc_tallies_get(mc_ctx *ctx, double *hhe, double *jt_ion, double *dust_abs,
+                   int64_t *segments);
+
+/* Rates of the last pass: gamma_HI, gamma_HeI, gamma_HeII, heat_HI, heat_HeI,
+ * heat_HeII (src/gas_rates_mod.f90:36-37), 6 arrays of nleaf back to back. */
+int mc_rates_get(mc_ctx *ctx, double *rates6);
+
+/* Energy ledger of the last pass {emitted, absorbed, escaped} [erg/s]
+ * (ion_score_energy_totals, src/ion_score_mod.f90:357-362). */
+int mc_energy_ledger(mc_ctx *ctx, double *out3);
+
+/* Sizes: nleaf, nnu_band; band total ion_Ltot; band edges ion_eedge(nnu+1)
+ * (eedge may be NULL). */
+int mc_info(mc_ctx *ctx, int32_t *nleaf, int32_t *nnu_band, double *Ltot, double *eedge);
+
+/* Switch the transport precision form (MC_FP64 | MC_MIXED) between passes. */
+int mc_set_precision(mc_ctx *ctx, int precision);
+
+/* Device times of the last pass/update [ms]: {launch, walk, rates, solve}
+ * measured with events on the library's stream. */
+int mc_timing(mc_ctx *ctx, double *out4);
+
+/* Diagnostic: leaves crossed by the direct walk of one ray to the box edge
+ * (geometry only), in the given precision form.  Writes up to maxn leaf
+ * numbers (1-based) to cells and their count to *n. */
+int mc_trace_ray(mc_ctx *ctx, int precision, const double *pos3, const double *dir3,
+                 int32_t *cells, int maxn, int32_t *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

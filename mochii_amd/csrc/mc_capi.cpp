@@ -1,0 +1,500 @@
+// C-ABI of the transport / cell-balance library (declared in
+// include/mochii_gpu.h).  Owns the device allocations of a context.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "mc_internal.h"
+
+static thread_local std::string g_err;
+
+void mc_set_error(const std::string &msg) { g_err = msg; }
+
+extern "C" const char *mc_last_error(void) { return g_err.c_str(); }
+
+#define CHECK_CTX(c)                                 \
+    do {                                             \
+        if (!(c)) {                                  \
+            mc_set_error("null context");            \
+            return -1;                               \
+        }                                            \
+    } while (0)
+
+template <class T>
+static int dalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    MC_HIP(hipMalloc((void **)p, sizeof(T) * n));
+    return 0;
+}
+
+static int upload_tables(mc_ctx *c)
+{
+    DevTables t;
+    std::memset(&t, 0, sizeof(t));
+    mc_qmc_tables(c->p.qmc_seed, t.V, t.key);
+    for (int i = 0; i <= c->nnu; ++i) t.eedge[i] = c->eedge[i];
+    for (int i = 0; i < c->ncomp; ++i) {
+        t.src_cdf[i] = c->src_cdf[i];
+        t.comp_kind[i] = c->comp_kind[i];
+    }
+    for (int i = 0; i < c->p.nsource; ++i) {
+        t.src_pos[i][0] = c->p.src_x[i];
+        t.src_pos[i][1] = c->p.src_y[i];
+        t.src_pos[i][2] = c->p.src_z[i];
+    }
+    for (size_t i = 0; i < c->samp_off.size(); ++i) t.samp_off[i] = c->samp_off[i];
+    t.nnu = c->nnu;
+    t.ncomp = c->ncomp;
+    t.multi = c->multi;
+    t.nsource = c->p.nsource;
+    t.qmc_ndim = c->qmc_ndim;
+    t.ext_geometry = c->p.ext_geometry;
+    t.Lpacket = c->Ltot / (double)c->p.nphotons;
+    t.rmax = c->p.rmax;
+    MC_HIP(hipMemcpy(c->d_tab, &t, sizeof(t), hipMemcpyHostToDevice));
+    const size_t nk = c->samp_e.size();
+    std::vector<double> s(3 * nk);
+    std::copy(c->samp_e.begin(), c->samp_e.end(), s.begin());
+    std::copy(c->samp_f.begin(), c->samp_f.end(), s.begin() + nk);
+    std::copy(c->samp_c.begin(), c->samp_c.end(), s.begin() + 2 * nk);
+    if (c->d_samp) hipFree(c->d_samp);
+    if (dalloc(&c->d_samp, 3 * nk)) return -100;
+    MC_HIP(hipMemcpy(c->d_samp, s.data(), sizeof(double) * 3 * nk, hipMemcpyHostToDevice));
+    return 0;
+}
+
+static void free_leaf_arrays(mc_ctx *c)
+{
+    DevState &s = c->st;
+    for (void *p : {(void *)s.nH, (void *)s.xHI, (void *)s.xHeI, (void *)s.xHeII, (void *)s.ne,
+                    (void *)s.Te, (void *)s.rhokap, (void *)s.opac, (void *)s.hhe, (void *)s.jt,
+                    (void *)s.rates, (void *)s.partials})
+        if (p) hipFree(p);
+    s.nH = s.xHI = s.xHeI = s.xHeII = s.ne = s.Te = s.rhokap = nullptr;
+    s.opac = nullptr;
+    s.hhe = s.jt = s.rates = s.partials = nullptr;
+}
+
+static int alloc_leaf_arrays(mc_ctx *c)
+{
+    free_leaf_arrays(c);
+    const size_t n = (size_t)c->nleaf;
+    DevState &s = c->st;
+    if (dalloc(&s.nH, n) || dalloc(&s.xHI, n) || dalloc(&s.xHeI, n) || dalloc(&s.xHeII, n) ||
+        dalloc(&s.ne, n) || dalloc(&s.Te, n) || dalloc(&s.rhokap, n) || dalloc(&s.opac, n) ||
+        dalloc(&s.hhe, n * 8) || dalloc(&s.jt, n * (size_t)c->nnu) || dalloc(&s.rates, n * 6) ||
+        dalloc(&s.partials, ((n + 255) / 256) * 3 + 3))
+        return -100;
+    MC_HIP(hipMemset(s.rhokap, 0, sizeof(double) * n));
+    MC_HIP(hipMemset(s.hhe, 0, sizeof(double) * n * 8));
+    MC_HIP(hipMemset(s.jt, 0, sizeof(double) * n * c->nnu));
+    MC_HIP(hipMemset(s.rates, 0, sizeof(double) * n * 6));
+    c->grid.nleaf = c->nleaf;
+    c->state_ready = false;
+    return 0;
+}
+
+extern "C" int mc_init(const mc_params *p, mc_ctx **out)
+{
+    if (!p || !out) {
+        mc_set_error("mc_init: null argument");
+        return -1;
+    }
+    *out = nullptr;
+    if (p->nphotons < 1) {
+        mc_set_error("nphotons must be >= 1");
+        return -1;
+    }
+    if (p->precision != MC_FP64 && p->precision != MC_MIXED) {
+        mc_set_error("precision must be MC_FP64 or MC_MIXED");
+        return -1;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        mc_set_error("no HIP device available");
+        return -2;
+    }
+    mc_ctx *c = new mc_ctx();
+    c->p = *p;
+    if (mc_setup_band(c) || mc_setup_samplers(c)) {
+        delete c;
+        return -1;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        mc_set_error("hipStreamCreate failed");
+        delete c;
+        return -100;
+    }
+    for (auto &e : c->ev) hipEventCreate(&e);
+    if (dalloc(&c->d_tab, 1) || upload_tables(c)) {
+        mc_finalize(c);
+        return -100;
+    }
+    if (dalloc(&c->st.ledger, 3) || dalloc(&c->st.segs, 2) || dalloc(&c->st.counter, 1)) {
+        mc_finalize(c);
+        return -100;
+    }
+    // launch-record buffers: batches of up to 4M packets
+    const int64_t cap = std::min<int64_t>(p->nphotons, (int64_t)1 << 22);
+    c->pk.cap = cap;
+    if (dalloc(&c->pk.x, cap) || dalloc(&c->pk.y, cap) || dalloc(&c->pk.z, cap) ||
+        dalloc(&c->pk.kx, cap) || dalloc(&c->pk.ky, cap) || dalloc(&c->pk.kz, cap) ||
+        dalloc(&c->pk.E, cap) || dalloc(&c->pk.sig, 6 * cap) || dalloc(&c->pk.inu, cap) ||
+        dalloc(&c->pk.leaf, cap)) {
+        mc_finalize(c);
+        return -100;
+    }
+    std::memset(&c->grid, 0, sizeof(c->grid));
+    if (p->grid_type == MC_GRID_CAR) {
+        const double dx = 2.0 * p->xmax / p->nx, dy = 2.0 * p->ymax / p->ny,
+                     dz = 2.0 * p->zmax / p->nz;
+        if (p->nx < 1 || p->ny < 1 || p->nz < 1 || std::fabs(dy - dx) > 1e-9 * dx ||
+            std::fabs(dz - dx) > 1e-9 * dx) {
+            mc_set_error("car grid needs cubic cells: 2*xmax/nx = 2*ymax/ny = 2*zmax/nz");
+            mc_finalize(c);
+            return -1;
+        }
+        c->car = 1;
+        c->xmin = -p->xmax; c->xmaxb = p->xmax;
+        c->ymin = -p->ymax; c->ymaxb = p->ymax;
+        c->zmin = -p->zmax; c->zmaxb = p->zmax;
+        c->dcell = (c->xmaxb - c->xmin) / p->nx;
+        c->nleaf = c->ncells = p->nx * p->ny * p->nz;
+        DevGrid &g = c->grid;
+        g.car = 1;
+        g.car_walk_amr = p->car_walk_amr;
+        g.nx = p->nx; g.ny = p->ny; g.nz = p->nz;
+        g.xmin = c->xmin; g.ymin = c->ymin; g.zmin = c->zmin;
+        g.xmax = c->xmaxb; g.ymax = c->ymaxb; g.zmax = c->zmaxb;
+        g.dcell = c->dcell;
+        if (alloc_leaf_arrays(c)) {
+            mc_finalize(c);
+            return -100;
+        }
+    } else if (p->grid_type != MC_GRID_AMR) {
+        mc_set_error("unknown grid_type");
+        mc_finalize(c);
+        return -1;
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" int mc_finalize(mc_ctx *c)
+{
+    if (!c) return 0;
+    free_leaf_arrays(c);
+    for (void *p : {(void *)c->d_tab, (void *)c->d_samp, (void *)c->d_dust, (void *)c->st.ledger,
+                    (void *)c->st.segs, (void *)c->st.counter, (void *)c->pk.x, (void *)c->pk.y,
+                    (void *)c->pk.z, (void *)c->pk.kx, (void *)c->pk.ky, (void *)c->pk.kz,
+                    (void *)c->pk.E, (void *)c->pk.sig, (void *)c->pk.inu, (void *)c->pk.leaf,
+                    (void *)c->d_cx, (void *)c->d_cy, (void *)c->d_cz, (void *)c->d_ch,
+                    (void *)c->d_ileaf, (void *)c->d_children, (void *)c->d_neighbor,
+                    (void *)c->d_cell_of_leaf, (void *)c->d_leaf_geom})
+        if (p) hipFree(p);
+    for (auto &e : c->ev)
+        if (e) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+extern "C" int mc_grid_octree(mc_ctx *c, int nleaf, const double *xl, const double *yl,
+                              const double *zl, const int32_t *lev, double half)
+{
+    CHECK_CTX(c);
+    if (c->p.grid_type != MC_GRID_AMR) {
+        mc_set_error("mc_grid_octree: context was created for a car grid");
+        return -1;
+    }
+    if (nleaf < 1 || !xl || !yl || !zl || !lev || !(half > 0.0)) {
+        mc_set_error("mc_grid_octree: bad arguments");
+        return -1;
+    }
+    if (mc_build_octree_host(c, nleaf, xl, yl, zl, lev, half)) return -1;
+    const size_t nc = (size_t)c->ncells;
+    for (void *p : {(void *)c->d_cx, (void *)c->d_cy, (void *)c->d_cz, (void *)c->d_ch,
+                    (void *)c->d_ileaf, (void *)c->d_children, (void *)c->d_neighbor,
+                    (void *)c->d_cell_of_leaf, (void *)c->d_leaf_geom})
+        if (p) hipFree(p);
+    if (dalloc(&c->d_cx, nc) || dalloc(&c->d_cy, nc) || dalloc(&c->d_cz, nc) ||
+        dalloc(&c->d_ch, nc) || dalloc(&c->d_ileaf, nc) || dalloc(&c->d_children, nc * 8) ||
+        dalloc(&c->d_neighbor, nc * 6) || dalloc(&c->d_cell_of_leaf, (size_t)nleaf) ||
+        dalloc(&c->d_leaf_geom, (size_t)nleaf))
+        return -100;
+    MC_HIP(hipMemcpy(c->d_cx, c->hcx.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(c->d_cy, c->hcy.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(c->d_cz, c->hcz.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(c->d_ch, c->hch.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(c->d_ileaf, c->hileaf.data(), sizeof(int32_t) * nc, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(c->d_children, c->hchildren.data(), sizeof(int32_t) * nc * 8,
+                     hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(c->d_neighbor, c->hneighbor.data(), sizeof(int32_t) * nc * 6,
+                     hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(c->d_cell_of_leaf, c->hcell_of_leaf.data(), sizeof(int32_t) * nleaf,
+                     hipMemcpyHostToDevice));
+    std::vector<float4> lg(nleaf);
+    for (int i = 0; i < nleaf; ++i) {
+        const int k = c->hcell_of_leaf[i];
+        lg[i] = make_float4((float)c->hcx[k], (float)c->hcy[k], (float)c->hcz[k], (float)c->hch[k]);
+    }
+    MC_HIP(hipMemcpy(c->d_leaf_geom, lg.data(), sizeof(float4) * nleaf, hipMemcpyHostToDevice));
+    DevGrid &g = c->grid;
+    g.car = 0;
+    g.car_walk_amr = 0;
+    g.nleaf = nleaf;
+    g.xmin = c->xmin; g.ymin = c->ymin; g.zmin = c->zmin;
+    g.xmax = c->xmaxb; g.ymax = c->ymaxb; g.zmax = c->zmaxb;
+    g.dcell = 0.0;
+    g.cx = c->d_cx; g.cy = c->d_cy; g.cz = c->d_cz; g.ch = c->d_ch;
+    g.ileaf = c->d_ileaf;
+    g.children = c->d_children;
+    g.neighbor = c->d_neighbor;
+    g.cell_of_leaf = c->d_cell_of_leaf;
+    g.leaf_geom = c->d_leaf_geom;
+    return alloc_leaf_arrays(c);
+}
+
+static double *field_ptr(mc_ctx *c, int field)
+{
+    switch (field) {
+    case MC_NH: return c->st.nH;
+    case MC_XHI: return c->st.xHI;
+    case MC_XHEI: return c->st.xHeI;
+    case MC_XHEII: return c->st.xHeII;
+    case MC_NE: return c->st.ne;
+    case MC_TE: return c->st.Te;
+    case MC_RHOKAP: return c->st.rhokap;
+    default: return nullptr;
+    }
+}
+
+extern "C" int mc_gas_state_setup(mc_ctx *c, const double *nH, const double *xHI)
+{
+    CHECK_CTX(c);
+    if (!c->st.nH) {
+        mc_set_error("mc_gas_state_setup: no grid yet");
+        return -1;
+    }
+    if (!nH) {
+        mc_set_error("mc_gas_state_setup: nH is required");
+        return -1;
+    }
+    const int n = c->nleaf;
+    const mc_params &p = c->p;
+    std::vector<double> xh(n), x1(n, p.xHeI_init), x2(n, p.xHeII_init), ne(n), te(n, p.te_fixed);
+    for (int i = 0; i < n; ++i) {
+        xh[i] = xHI ? xHI[i] : p.xHI_init;
+        const double x3 = std::max(0.0, 1.0 - x1[i] - x2[i]);
+        ne[i] = nH[i] * ((1.0 - xh[i]) + p.He_abund * (x2[i] + 2.0 * x3));
+    }
+    MC_HIP(hipMemcpy(c->st.nH, nH, sizeof(double) * n, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(c->st.xHI, xh.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(c->st.xHeI, x1.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(c->st.xHeII, x2.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(c->st.ne, ne.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(c->st.Te, te.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    c->state_ready = true;
+    return mc_run_opacity_fill(c);
+}
+
+extern "C" int mc_state_set(mc_ctx *c, int field, const double *src)
+{
+    CHECK_CTX(c);
+    double *d = field_ptr(c, field);
+    if (!d || !src) {
+        mc_set_error("mc_state_set: bad field or null source");
+        return -1;
+    }
+    MC_HIP(hipMemcpy(d, src, sizeof(double) * c->nleaf, hipMemcpyHostToDevice));
+    return mc_run_opacity_fill(c);
+}
+
+extern "C" int mc_state_get(mc_ctx *c, int field, double *dst)
+{
+    CHECK_CTX(c);
+    double *d = field_ptr(c, field);
+    if (!d || !dst) {
+        mc_set_error("mc_state_get: bad field or null destination");
+        return -1;
+    }
+    MC_HIP(hipMemcpy(dst, d, sizeof(double) * c->nleaf, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int mc_ion_dust_table(mc_ctx *c, int n, const double *lam, const double *alb,
+                                 const double *g, const double *cext)
+{
+    CHECK_CTX(c);
+    if (n < 2 || !lam || !alb || !g || !cext) {
+        mc_set_error("mc_ion_dust_table: need n >= 2 rows");
+        return -1;
+    }
+    // stable sort by wavelength (ion_dust_read's insertion sort keeps equal
+    // wavelengths in input order)
+    std::vector<int> ord(n);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return lam[a] < lam[b]; });
+    c->dlam.resize(n); c->dalb.resize(n); c->dg.resize(n); c->dcext.resize(n);
+    for (int i = 0; i < n; ++i) {
+        c->dlam[i] = lam[ord[i]]; c->dalb[i] = alb[ord[i]];
+        c->dg[i] = g[ord[i]]; c->dcext[i] = cext[ord[i]];
+    }
+    // C_ext at lambda_ref, log-log interpolation clamped at the table ends
+    const double x0 = c->p.lambda_ref;
+    if (x0 <= c->dlam[0]) c->dcref = c->dcext[0];
+    else if (x0 >= c->dlam[n - 1]) c->dcref = c->dcext[n - 1];
+    else {
+        int i = 0;
+        while (i < n - 1 && !(x0 < c->dlam[i + 1])) ++i;
+        const double w = std::log(x0 / c->dlam[i]) / std::log(c->dlam[i + 1] / c->dlam[i]);
+        c->dcref = std::exp(std::log(std::max(c->dcext[i], 2.2250738585072014e-308)) * (1.0 - w) +
+                            std::log(std::max(c->dcext[i + 1], 2.2250738585072014e-308)) * w);
+    }
+    std::vector<double> h(4 * (size_t)n);
+    std::copy(c->dlam.begin(), c->dlam.end(), h.begin());
+    std::copy(c->dalb.begin(), c->dalb.end(), h.begin() + n);
+    std::copy(c->dg.begin(), c->dg.end(), h.begin() + 2 * n);
+    std::copy(c->dcext.begin(), c->dcext.end(), h.begin() + 3 * n);
+    if (c->d_dust) hipFree(c->d_dust);
+    if (dalloc(&c->d_dust, 4 * (size_t)n)) return -100;
+    MC_HIP(hipMemcpy(c->d_dust, h.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice));
+    c->dn = n;
+    return 0;
+}
+
+extern "C" int mc_gas_opacity_fill(mc_ctx *c)
+{
+    CHECK_CTX(c);
+    return mc_run_opacity_fill(c);
+}
+
+extern "C" int mc_ionizing_field_and_rates(mc_ctx *c, int with_peel)
+{
+    CHECK_CTX(c);
+    if (with_peel) {
+        mc_set_error("peel-off imaging is not part of this library");
+        return -1;
+    }
+    if (!c->state_ready) {
+        mc_set_error("mc_ionizing_field_and_rates: gas state not set up");
+        return -1;
+    }
+    if (c->p.ion_add_dust && c->dn < 2) {
+        mc_set_error("ion_add_dust needs mc_ion_dust_table first");
+        return -1;
+    }
+    int rc = mc_run_pass(c);
+    if (rc) return rc;
+    rc = mc_run_rates(c);
+    if (rc) return rc;
+    const double closure = std::fabs(c->ledger[1] + c->ledger[2] - c->ledger[0]) /
+                           std::max(c->ledger[0], 2.2250738585072014e-308);
+    if (closure > 5.0e-13) {
+        mc_set_error("energy closure " + std::to_string(closure) + " exceeds 5e-13");
+        return -4;
+    }
+    return 0;
+}
+
+extern "C" int mc_gas_equilibrium_update(mc_ctx *c, double *max_dx, double *dx_vol)
+{
+    CHECK_CTX(c);
+    if (!max_dx || !dx_vol) {
+        mc_set_error("mc_gas_equilibrium_update: null output");
+        return -1;
+    }
+    return mc_run_equilibrium(c, max_dx, dx_vol);
+}
+
+extern "C" int mc_hydrogen_balance_residual_grid(mc_ctx *c, double *eps_max, double *eps_vol)
+{
+    CHECK_CTX(c);
+    if (!eps_max || !eps_vol) {
+        mc_set_error("mc_hydrogen_balance_residual_grid: null output");
+        return -1;
+    }
+    return mc_run_residual(c, eps_max, eps_vol);
+}
+
+extern "C" int mc_tallies_get(mc_ctx *c, double *hhe, double *jt, double *dust, int64_t *segs)
+{
+    CHECK_CTX(c);
+    const size_t n = (size_t)c->nleaf;
+    if (hhe || dust) {
+        std::vector<double> h(n * 8);
+        MC_HIP(hipMemcpy(h.data(), c->st.hhe, sizeof(double) * n * 8, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            if (hhe)
+                for (int k = 0; k < 6; ++k) hhe[i * 6 + k] = h[i * 8 + k];
+            if (dust) dust[i] = h[i * 8 + 6];
+        }
+    }
+    if (jt) MC_HIP(hipMemcpy(jt, c->st.jt, sizeof(double) * n * c->nnu, hipMemcpyDeviceToHost));
+    if (segs) {
+        segs[0] = c->segs[0];
+        segs[1] = c->segs[1];
+    }
+    return 0;
+}
+
+extern "C" int mc_rates_get(mc_ctx *c, double *rates6)
+{
+    CHECK_CTX(c);
+    if (!rates6) {
+        mc_set_error("mc_rates_get: null output");
+        return -1;
+    }
+    MC_HIP(hipMemcpy(rates6, c->st.rates, sizeof(double) * 6 * c->nleaf, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int mc_energy_ledger(mc_ctx *c, double *out3)
+{
+    CHECK_CTX(c);
+    for (int k = 0; k < 3; ++k) out3[k] = c->ledger[k];
+    return 0;
+}
+
+extern "C" int mc_info(mc_ctx *c, int32_t *nleaf, int32_t *nnu, double *Ltot, double *eedge)
+{
+    CHECK_CTX(c);
+    if (nleaf) *nleaf = c->nleaf;
+    if (nnu) *nnu = c->nnu;
+    if (Ltot) *Ltot = c->Ltot;
+    if (eedge)
+        for (int i = 0; i <= c->nnu; ++i) eedge[i] = c->eedge[i];
+    return 0;
+}
+
+extern "C" int mc_set_precision(mc_ctx *c, int precision)
+{
+    CHECK_CTX(c);
+    if (precision != MC_FP64 && precision != MC_MIXED) {
+        mc_set_error("precision must be MC_FP64 or MC_MIXED");
+        return -1;
+    }
+    c->p.precision = precision;
+    return 0;
+}
+
+extern "C" int mc_timing(mc_ctx *c, double *out4)
+{
+    CHECK_CTX(c);
+    for (int k = 0; k < 4; ++k) out4[k] = c->times[k];
+    return 0;
+}
+
+extern "C" int mc_trace_ray(mc_ctx *c, int precision, const double *pos, const double *dir,
+                            int32_t *cells, int maxn, int32_t *n)
+{
+    CHECK_CTX(c);
+    if (!pos || !dir || !cells || !n || maxn < 1) {
+        mc_set_error("mc_trace_ray: bad arguments");
+        return -1;
+    }
+    return mc_run_trace(c, precision, pos, dir, cells, maxn, n);
+}

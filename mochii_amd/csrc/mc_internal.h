@@ -1,0 +1,143 @@
+// Internal declarations of the gfx950 transport / cell-balance library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/mochii_gpu.h"
+
+#define MC_MAXBINS 64
+#define MC_QMC_DIM 12
+
+// Physical constants (src/define.f90:12-88 of the reference).
+#define MC_PI      3.141592653589793238462643383279502884197
+#define MC_TWOPI   6.283185307179586476925286766559005768394
+#define MC_FOURPI  12.56637061435917295385057353311801153679
+#define MC_EV2ERG  1.602176634e-12
+#define MC_KBOLTZ  1.380649e-16
+#define MC_ETH_HI   13.598
+#define MC_ETH_HEI  24.587
+#define MC_ETH_HEII 54.416
+#define MC_TAU_HUGE 745.2
+
+// Small launch/band tables shared by every kernel of a context
+// (read through the scalar/constant cache).
+struct DevTables {
+    uint32_t V[32][MC_QMC_DIM];      // Sobol direction numbers
+    uint32_t key[MC_QMC_DIM][2];     // Owen keys (stellar, diffuse)
+    double   eedge[MC_MAXBINS + 1];  // band edges
+    double   src_cdf[MC_MAXSRC + 1];
+    double   src_pos[MC_MAXSRC + 1][3];
+    int32_t  comp_kind[MC_MAXSRC + 1];
+    int32_t  samp_off[MC_MAXSRC + 2];// offsets of each component's knots
+    int32_t  nnu, ncomp, multi, nsource, qmc_ndim, ext_geometry;
+    double   Lpacket;
+    double   rmax;
+};
+
+// Grid description passed by value to kernels.
+struct DevGrid {
+    int32_t car;                      // 1 = uniform Cartesian (DDA), 0 = octree
+    int32_t car_walk_amr;             // car grid walked with the octree face-exit walk
+    int32_t nx, ny, nz, nleaf;
+    double  xmin, ymin, zmin, xmax, ymax, zmax, dcell;
+    // octree (SoA, 0-based cell and leaf indices, -1 = none)
+    const double  *cx, *cy, *cz, *ch;       // ncells
+    const int32_t *ileaf;                   // ncells: leaf or -1
+    const int32_t *children;                // ncells*8 or -1
+    const int32_t *neighbor;                // ncells*6 or -1
+    const int32_t *cell_of_leaf;            // nleaf
+    const float4  *leaf_geom;               // nleaf: centre(3) + half width (FP32)
+};
+
+// Packet launch records (SoA), written by the launch kernel.
+struct PacketBuf {
+    double *x, *y, *z, *kx, *ky, *kz;
+    double *E;
+    double *sig;       // 6 per packet: sHI, sHeI, sHeII, dabs, dsca, g
+    int32_t *inu;      // 0-based band bin
+    int32_t *leaf;     // launch leaf (0-based, -1 outside)
+    int64_t cap;
+};
+
+// Per-leaf state and tallies on the device.
+struct DevState {
+    double *nH, *xHI, *xHeI, *xHeII, *ne, *Te, *rhokap;
+    float4 *opac;      // FP32 opacity inputs per leaf (mixed form)
+    double *hhe;       // nleaf * 8: 6 rate sums, dust_abs, pad
+    double *jt;        // nleaf * nnu
+    double *rates;     // 6 * nleaf
+    double *ledger;    // 3
+    unsigned long long *segs;   // 2
+    unsigned long long *counter;// work counter for persistent kernels
+    double *partials;  // per-block metric partials
+};
+
+struct mc_ctx {
+    mc_params p;
+    // band + sources (host)
+    int nnu;
+    std::vector<double> eedge;
+    int ncomp, multi;
+    int comp_kind[MC_MAXSRC + 1];
+    double src_cdf[MC_MAXSRC + 1];
+    std::vector<double> samp_e, samp_f, samp_c;
+    std::vector<int> samp_off;
+    double Ltot;
+    int qmc_ndim;
+    // dust table (sorted)
+    std::vector<double> dlam, dalb, dg, dcext;
+    double dcref;
+    // grid (host copy of octree arrays)
+    int nleaf, ncells, car;
+    double xmin, xmaxb, ymin, ymaxb, zmin, zmaxb, dcell;
+    std::vector<double> hcx, hcy, hcz, hch;
+    std::vector<int32_t> hileaf, hchildren, hneighbor, hcell_of_leaf, hlevel, hparent;
+    // device
+    hipStream_t stream;
+    hipEvent_t ev[8];
+    DevTables *d_tab;
+    double *d_samp;          // 3 * nknots (energy, density, cdf)
+    double *d_dust;          // 4 * dn (lam, alb, g, cext)
+    int dn;
+    DevGrid grid;
+    double *d_cx, *d_cy, *d_cz, *d_ch;
+    int32_t *d_ileaf, *d_children, *d_neighbor, *d_cell_of_leaf;
+    float4 *d_leaf_geom;
+    DevState st;
+    PacketBuf pk;
+    int nblocks_part;
+    bool state_ready;
+    // last pass results
+    double ledger[3];
+    int64_t segs[2];
+    double times[4];
+};
+
+// host set-up (mc_setup.cpp)
+int mc_setup_band(mc_ctx *c);
+int mc_setup_samplers(mc_ctx *c);
+int mc_build_octree_host(mc_ctx *c, int nleaf, const double *xl, const double *yl,
+                         const double *zl, const int32_t *lev, double half);
+void mc_qmc_tables(int seed, uint32_t V[32][MC_QMC_DIM], uint32_t key[MC_QMC_DIM][2]);
+void mc_set_error(const std::string &msg);
+
+// device work (mc_kernels.hip)
+int mc_run_pass(mc_ctx *c);
+int mc_run_rates(mc_ctx *c);
+int mc_run_equilibrium(mc_ctx *c, double *max_dx, double *dx_vol);
+int mc_run_residual(mc_ctx *c, double *eps_max, double *eps_vol);
+int mc_run_opacity_fill(mc_ctx *c);
+int mc_run_trace(mc_ctx *c, int precision, const double *pos, const double *dir, int32_t *cells,
+                 int maxn, int32_t *n);
+
+#define MC_HIP(call)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            mc_set_error(std::string(#call) + ": " + hipGetErrorString(e_));  \
+            return -100;                                                      \
+        }                                                                     \
+    } while (0)

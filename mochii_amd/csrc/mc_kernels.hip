@@ -1,0 +1,1091 @@
+// gfx950 kernels of one transport pass and the cell updates.
+//
+//   k_launch      one thread per packet: Owen-scrambled Sobol point, emitter,
+//                 energy by CDF inversion, band bin, cross sections at the
+//                 packet energy -> SoA launch records
+//   k_walk<P,G>   persistent walk, one packet per lane; a lane whose packet
+//                 ends takes the next record at once (wave-aggregated counter),
+//                 so waves stay full until the batch drains.  Edge walk with
+//                 the analytic path estimator, then (dust scattering on) the
+//                 forced interaction, albedo weight, roulette, Henyey-Greenstein
+//                 and tau-target flights with the Lucy estimator.
+//                 P = FP64 (reference arithmetic) or MIXED (FP32 cell-local
+//                 geometry and opacity, FP64 tallies).  G = car DDA or octree.
+//   k_rates       leaf-volume normalisation of the tallies
+//   k_equilibrium per-leaf H/He balance at fixed T, relaxation, metrics,
+//                 and the FP32 opacity inputs for the next pass
+#include <cfloat>
+#include <cmath>
+
+#include "mc_internal.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr double kWmin = 1.0e-4;
+constexpr double kPsurv = 0.1;
+
+struct KArgs {
+    const DevTables *tab;
+    const double *samp;      // energy | density | cdf, each nknots
+    int32_t nknots;
+    const double *dust;      // lam | alb | g | cext, each dn
+    int32_t dn;
+    double dcref;
+    int32_t add_dust, dust_scatter;
+    double He_abund, d2cm;
+    uint32_t seed;
+    DevGrid g;
+    DevState s;
+    PacketBuf pk;
+    int64_t first_id;        // 1-based id of record 0
+    int64_t npk;             // records in this batch
+};
+
+// ---------------------------------------------------------------------------
+// launch helpers
+__device__ __forceinline__ uint32_t owen_scramble(uint32_t x, uint32_t key)
+{
+    uint32_t y = __builtin_bitreverse32(x) + key;
+    y ^= y * 0x6C50B47Cu;
+    y ^= y * 0xB82F1E52u;
+    y ^= y * 0xC7AFE638u;
+    y ^= y * 0x8D22F6E6u;
+    return __builtin_bitreverse32(y);
+}
+
+__device__ __forceinline__ double qmc_u(const DevTables *T, uint32_t idx, int d, int stream)
+{
+    uint32_t code = 0;
+    for (uint32_t m = idx; m; m &= m - 1) code ^= T->V[__builtin_ctz(m)][d];
+    code = owen_scramble(code, T->key[d][stream]);
+    return ((double)code + 0.5) * (1.0 / 4294967296.0);
+}
+
+// inverse of the piecewise-linear CDF of knots [lo, hi)
+__device__ double sample_energy(const double *samp, int nk, int lo0, int hi0, double u)
+{
+    const double *e = samp, *f = samp + nk, *c = samp + 2 * nk;
+    const double t = fmin(fmax(u, 0.0), 1.0);
+    if (t <= 0.0) return e[lo0];
+    if (t >= 1.0) return e[hi0 - 1];
+    int lo = lo0, hi = hi0 - 1;
+    while (hi - lo > 1) {
+        const int mid = lo + (hi - lo) / 2;
+        if (c[mid] <= t) lo = mid; else hi = mid;
+    }
+    const double area = c[hi] - c[lo];
+    if (area <= 0.0) return e[hi];
+    const double dx = e[hi] - e[lo], y0 = f[lo], y1 = f[hi];
+    const double local = (t - c[lo]) / area * (0.5 * (y0 + y1) * dx);
+    const double slope = (y1 - y0) / dx;
+    const double sc = fmax(fmax(fabs(y0), fabs(y1)), DBL_MIN);
+    double off;
+    if (fabs(slope * dx) <= 64.0 * DBL_EPSILON * sc) {
+        off = y0 > 0.0 ? local / y0 : 0.0;
+    } else {
+        const double disc = sqrt(fmax(0.0, y0 * y0 + 2.0 * slope * local));
+        off = 2.0 * local / (y0 + disc);
+    }
+    return e[lo] + fmin(fmax(off, 0.0), dx);
+}
+
+__device__ double sigma_hydrogenic(double E, double Eth, double Z)
+{
+    const double s0 = 6.30e-18 / (Z * Z);
+    if (E < Eth) return 0.0;
+    if (E == Eth) return s0;
+    const double eps = sqrt(E / Eth - 1.0), r = Eth / E;
+    return s0 * (((r * r) * r) * r) * exp(4.0 - 4.0 * atan(eps) / eps) /
+           (1.0 - exp(-2.0 * MC_PI / eps));
+}
+
+__device__ double sigma_HeI(double E)
+{
+    if (E < MC_ETH_HEI) return 0.0;
+    const double x = E / 13.61 - 0.4434, z = sqrt(x * x + 2.136 * 2.136);
+    const double Q = 5.5 - 0.5 * 3.188, xm = x - 1.0;
+    const double F = (xm * xm + 2.039 * 2.039) * pow(z, -Q) * pow(1.0 + sqrt(z / 1.469), -3.188);
+    return 949.2 * F * 1.0e-18;
+}
+
+// dust optics at E from the sorted (lambda, albedo, g, C_ext) table
+__device__ void dust_at(const KArgs &a, double E, double &sabs, double &ssca, double &g)
+{
+    const int n = a.dn;
+    const double *lam = a.dust, *alb = a.dust + n, *gc = a.dust + 2 * n, *ce = a.dust + 3 * n;
+    const double x0 = 1.23984 / E;
+    double al, cx, gg;
+    if (x0 <= lam[0]) {
+        al = alb[0]; cx = ce[0]; gg = gc[0];
+    } else if (x0 >= lam[n - 1]) {
+        al = alb[n - 1]; cx = ce[n - 1]; gg = gc[n - 1];
+    } else {
+        int lo = 0, hi = n - 1;   // largest i with lam[i] <= x0
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (lam[mid] <= x0) lo = mid; else hi = mid;
+        }
+        const double w = (x0 - lam[lo]) / (lam[lo + 1] - lam[lo]);
+        al = alb[lo] * (1.0 - w) + alb[lo + 1] * w;
+        gg = gc[lo] * (1.0 - w) + gc[lo + 1] * w;
+        const double wl = log(x0 / lam[lo]) / log(lam[lo + 1] / lam[lo]);
+        cx = exp(log(fmax(ce[lo], DBL_MIN)) * (1.0 - wl) + log(fmax(ce[lo + 1], DBL_MIN)) * wl);
+    }
+    sabs = (1.0 - al) * cx / a.dcref;
+    ssca = al * cx / a.dcref;
+    g = gg;
+}
+
+__device__ __forceinline__ int car_index(const DevGrid &g, double x, double y, double z)
+{
+    const int ix = min(max((int)((x - g.xmin) / g.dcell), 0), g.nx - 1);
+    const int iy = min(max((int)((y - g.ymin) / g.dcell), 0), g.ny - 1);
+    const int iz = min(max((int)((z - g.zmin) / g.dcell), 0), g.nz - 1);
+    return ix + g.nx * (iy + g.ny * iz);
+}
+
+// leaf containing (x,y,z), -1 outside (octree descent or raster index)
+__device__ int find_leaf(const DevGrid &g, double x, double y, double z)
+{
+    if (x < g.xmin || x > g.xmax || y < g.ymin || y > g.ymax || z < g.zmin || z > g.zmax)
+        return -1;
+    if (g.car) return car_index(g, x, y, z);
+    int c = 0;
+    for (;;) {
+        const int l = g.ileaf[c];
+        if (l >= 0) return l;
+        const int o = (x >= g.cx[c] ? 1 : 0) + (y >= g.cy[c] ? 2 : 0) + (z >= g.cz[c] ? 4 : 0);
+        c = g.children[c * 8 + o];
+        if (c < 0) return -1;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_launch(KArgs a)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.npk) return;
+    const DevTables *T = a.tab;
+    const int64_t id = a.first_id + i;
+    const uint32_t q = (uint32_t)(id - 1);   // index bits beyond 32 carry no direction number
+    double u[7];
+    for (int d = 0; d < T->qmc_ndim; ++d) u[d] = qmc_u(T, q, d, 0);
+    int comp = 0;
+    double umu, uphi, ue;
+    if (!T->multi && T->nsource == 1) {
+        ue = u[0]; umu = u[1]; uphi = u[2];
+    } else {
+        if (T->multi) {
+            comp = T->ncomp - 1;
+            for (int c = 0; c < T->ncomp; ++c)
+                if (u[0] <= T->src_cdf[c]) { comp = c; break; }
+        }
+        ue = u[1]; umu = u[2]; uphi = u[3];
+    }
+    double x, y, z, kx, ky, kz;
+    const int kind = T->comp_kind[comp];
+    if (kind >= 1) {
+        x = T->src_pos[kind - 1][0]; y = T->src_pos[kind - 1][1]; z = T->src_pos[kind - 1][2];
+        const double cost = 2.0 * umu - 1.0, sint = sqrt(1.0 - cost * cost);
+        double sp, cp;
+        sincos(MC_TWOPI * uphi, &sp, &cp);
+        kx = sint * cp; ky = sint * sp; kz = cost;
+    } else if (T->ext_geometry == MC_EXT_SPH) {
+        // uniform entry point on the bounding sphere, Lambert inward direction
+        const DevGrid &g = a.g;
+        const double xc = 0.5 * (g.xmin + g.xmax), yc = 0.5 * (g.ymin + g.ymax),
+                     zc = 0.5 * (g.zmin + g.zmax);
+        const double cost = 2.0 * u[4] - 1.0, sint = sqrt(1.0 - cost * cost);
+        double s1, c1;
+        sincos(MC_TWOPI * u[5], &s1, &c1);
+        const double rx = sint * c1, ry = sint * s1, rz = cost;
+        x = xc + T->rmax * rx; y = yc + T->rmax * ry; z = zc + T->rmax * rz;
+        const double nx = -rx, ny = -ry, nz = -rz;
+        double tx, ty, tz;
+        if (fabs(nx) <= fabs(ny) && fabs(nx) <= fabs(nz)) { tx = 0.0; ty = nz; tz = -ny; }
+        else if (fabs(ny) <= fabs(nz)) { tx = -nz; ty = 0.0; tz = nx; }
+        else { tx = ny; ty = -nx; tz = 0.0; }
+        const double inv = 1.0 / sqrt(tx * tx + ty * ty + tz * tz);
+        tx *= inv; ty *= inv; tz *= inv;
+        const double bx = ny * tz - nz * ty, by = nz * tx - nx * tz, bz = nx * ty - ny * tx;
+        const double mu = sqrt(umu), sm = sqrt(1.0 - mu * mu);
+        double s2, c2;
+        sincos(MC_TWOPI * uphi, &s2, &c2);
+        kx = -mu * rx + sm * (c2 * tx + s2 * bx);
+        ky = -mu * ry + sm * (c2 * ty + s2 * by);
+        kz = -mu * rz + sm * (c2 * tz + s2 * bz);
+        const double span = fmin(fmin(g.xmax - g.xmin, g.ymax - g.ymin), g.zmax - g.zmin);
+        const double nudge = 1.0e-6 * span;
+        x -= nudge * rx; y -= nudge * ry; z -= nudge * rz;
+    } else {
+        // area-weighted box face, uniform point on it, Lambert inward direction
+        const DevGrid &g = a.g;
+        const double xr = g.xmax - g.xmin, yr = g.ymax - g.ymin, zr = g.zmax - g.zmin;
+        const double nudge = 1.0e-6 * fmin(fmin(xr, yr), zr);
+        const double fa = yr * zr, fb = zr * xr, fc = xr * yr;
+        const double cum[6] = {fa, 2 * fa, 2 * fa + fb, 2 * fa + 2 * fb, 2 * fa + 2 * fb + fc,
+                               2 * fa + 2 * fb + 2 * fc};
+        const double pick = u[4] * cum[5];
+        int f = 5;
+        for (int k = 0; k < 6; ++k) if (pick <= cum[k]) { f = k; break; }
+        const double mu = sqrt(umu), sm = sqrt(1.0 - mu * mu);
+        double sp, cp;
+        sincos(MC_TWOPI * uphi, &sp, &cp);
+        const double c1 = u[5], c2 = u[6];
+        if (f < 2) {
+            x = f == 0 ? g.xmax - nudge : g.xmin + nudge;
+            y = g.ymin + yr * c1; z = g.zmin + zr * c2;
+            kx = f == 0 ? -mu : mu; ky = sm * cp; kz = sm * sp;
+        } else if (f < 4) {
+            x = g.xmin + xr * c1; y = f == 2 ? g.ymax - nudge : g.ymin + nudge;
+            z = g.zmin + zr * c2;
+            kx = sm * cp; ky = f == 2 ? -mu : mu; kz = sm * sp;
+        } else {
+            x = g.xmin + xr * c1; y = g.ymin + yr * c2;
+            z = f == 4 ? g.zmax - nudge : g.zmin + nudge;
+            kx = sm * cp; ky = sm * sp; kz = f == 4 ? -mu : mu;
+        }
+    }
+    const double E = sample_energy(a.samp, a.nknots, T->samp_off[comp], T->samp_off[comp + 1], ue);
+    // band bin: ion_eedge(bin) <= E < ion_eedge(bin+1), clamped
+    int lo = 0, hi = T->nnu - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (T->eedge[mid] <= E) lo = mid; else hi = mid - 1;
+    }
+    PacketBuf &P = a.pk;
+    P.x[i] = x; P.y[i] = y; P.z[i] = z;
+    P.kx[i] = kx; P.ky[i] = ky; P.kz[i] = kz;
+    P.E[i] = E;
+    P.inu[i] = lo;
+    double sabs = 0.0, ssca = 0.0, gg = 0.0;
+    if (a.add_dust) dust_at(a, E, sabs, ssca, gg);
+    double *sg = P.sig + 6 * i;
+    sg[0] = sigma_hydrogenic(E, MC_ETH_HI, 1.0);
+    sg[1] = sigma_HeI(E);
+    sg[2] = sigma_hydrogenic(E, MC_ETH_HEII, 2.0);
+    sg[3] = sabs; sg[4] = ssca; sg[5] = gg;
+    P.leaf[i] = find_leaf(a.g, x, y, z);
+}
+
+// ---------------------------------------------------------------------------
+// counter-based generator for the post-launch draws (same definition as the
+// CPU oracle): three splitmix64 finalizer rounds over (seed, id, stream|k)
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z ^= z >> 30; z *= 0xBF58476D1CE4E5B9ull;
+    z ^= z >> 27; z *= 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return z;
+}
+
+__device__ __forceinline__ double rng_u(uint32_t seed, uint64_t id, uint32_t k)
+{
+    uint64_t h = mix64((uint64_t)seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull);
+    h = mix64(h ^ id);
+    h = mix64(h ^ ((1ull << 32) | (uint64_t)k));   // stream 1 (stellar)
+    return ((double)(h >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// ---------------------------------------------------------------------------
+// per-lane packet state
+struct Lane {
+    int phase;               // 0 idle, 1 edge walk, 2 tau flight, 3 drained
+    int inu, nscatt, leaf;
+    uint32_t nk;
+    int64_t id;
+    double x, y, z, kx, ky, kz;      // flight origin and direction
+    double wgt, E;
+    double sHI, sHeI, sHeII, dabs, dsca, dg;
+    double coeff[6];
+    double wl;
+    // walk state, FP64 form: DDA
+    int ix, iy, iz, sx, sy, sz;
+    double tm[3], td[3], tcur, expo, tau, tau_in;
+    // walk state, mixed form: cell-local position in [0,1]^3
+    float px, py, pz, fkx, fky, fkz, fsHI, fsHeI, fsHeII, fdust;
+    float fexpo, ftau, ftau_in, ftcur;
+};
+
+template <int P> struct Prec;
+template <> struct Prec<MC_FP64> { static constexpr bool mixed = false; };
+template <> struct Prec<MC_MIXED> { static constexpr bool mixed = true; };
+
+__device__ __forceinline__ double opacity_d(const KArgs &a, const Lane &L, int il)
+{
+#pragma clang fp contract(off)
+    const DevState &s = a.s;
+    double kap = s.nH[il] * (s.xHI[il] * L.sHI +
+                             a.He_abund * (s.xHeI[il] * L.sHeI + s.xHeII[il] * L.sHeII)) *
+                 a.d2cm;
+    if (a.add_dust) {
+        kap = kap + s.rhokap[il] * L.dabs;
+        if (a.dust_scatter) kap = kap + s.rhokap[il] * L.dsca;
+    }
+    return kap;
+}
+
+__device__ __forceinline__ float opacity_f(const KArgs &a, const Lane &L, int il)
+{
+    const float4 o = a.s.opac[il];
+    return o.x * L.fsHI + o.y * L.fsHeI + o.z * L.fsHeII + o.w * L.fdust;
+}
+
+struct Counters {
+    double e_emit, e_abs, e_esc;
+    unsigned long long sd, ss;
+};
+
+// score_ion_path: the packet's path quantity into its leaf
+__device__ __forceinline__ void score(const KArgs &a, const Lane &L, int il, double pl, Counters &cn)
+{
+    if (pl == 0.0) return;
+    const int nnu = a.tab->nnu;
+    unsafeAtomicAdd(a.s.jt + (size_t)il * nnu + L.inu, pl);
+    double *h = a.s.hhe + (size_t)il * 8;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) unsafeAtomicAdd(h + k, pl * L.coeff[k]);
+    if (a.add_dust) unsafeAtomicAdd(h + 6, pl * L.dabs);
+    if (L.nscatt > 0) cn.ss++; else cn.sd++;
+}
+
+// ---- car DDA, FP64 (reference arithmetic) ---------------------------------
+__device__ __forceinline__ void dda_axis_init(double pos, double k, double org, double d, int n, int &idx,
+                                      int &st, double &tm, double &td)
+{
+#pragma clang fp contract(off)
+    idx = min(max((int)((pos - org) / d), 0), n - 1);
+    if (k > 0.0) {
+        st = 1;
+        tm = (org + (double)(idx + 1) * d - pos) / k;
+        td = d / k;
+    } else if (k < 0.0) {
+        st = -1;
+        tm = (org + (double)idx * d - pos) / k;
+        td = -d / k;
+    } else {
+        st = 0;
+        tm = DBL_MAX;
+        td = DBL_MAX;
+    }
+}
+
+__device__ __forceinline__ void dda_init_d(const DevGrid &g, Lane &L)
+{
+    dda_axis_init(L.x, L.kx, g.xmin, g.dcell, g.nx, L.ix, L.sx, L.tm[0], L.td[0]);
+    dda_axis_init(L.y, L.ky, g.ymin, g.dcell, g.ny, L.iy, L.sy, L.tm[1], L.td[1]);
+    dda_axis_init(L.z, L.kz, g.zmin, g.dcell, g.nz, L.iz, L.sz, L.tm[2], L.td[2]);
+    L.tcur = 0.0;
+}
+
+// distance to the nearest face along axis ax (register selects, no
+// dynamic indexing of the per-lane arrays)
+__device__ __forceinline__ double tm_of(const Lane &L, int ax)
+{
+    return ax == 0 ? L.tm[0] : (ax == 1 ? L.tm[1] : L.tm[2]);
+}
+
+// advance along the axis of the nearest face; returns false when outside
+__device__ __forceinline__ bool dda_advance_d(const DevGrid &g, Lane &L, int ax)
+{
+    if (ax == 0) { L.tcur = L.tm[0]; L.tm[0] = L.tm[0] + L.td[0]; L.ix += L.sx; return L.ix >= 0 && L.ix < g.nx; }
+    if (ax == 1) { L.tcur = L.tm[1]; L.tm[1] = L.tm[1] + L.td[1]; L.iy += L.sy; return L.iy >= 0 && L.iy < g.ny; }
+    L.tcur = L.tm[2];
+    L.tm[2] = L.tm[2] + L.td[2];
+    L.iz += L.sz;
+    return L.iz >= 0 && L.iz < g.nz;
+}
+
+__device__ __forceinline__ int dda_axis_d(const Lane &L)
+{
+    int ax = 0;
+    if (L.tm[1] < L.tm[0]) ax = 1;
+    if (L.tm[2] < (ax == 0 ? L.tm[0] : L.tm[1])) ax = 2;
+    return ax;
+}
+
+// ---- car walk, mixed form: FP32 position local to the current cell --------
+__device__ __forceinline__ void dda_init_f(const DevGrid &g, Lane &L)
+{
+    const double gx = (L.x - g.xmin) / g.dcell, gy = (L.y - g.ymin) / g.dcell,
+                 gz = (L.z - g.zmin) / g.dcell;
+    L.ix = min(max((int)gx, 0), g.nx - 1);
+    L.iy = min(max((int)gy, 0), g.ny - 1);
+    L.iz = min(max((int)gz, 0), g.nz - 1);
+    L.px = fminf(fmaxf((float)(gx - L.ix), 0.0f), 1.0f);
+    L.py = fminf(fmaxf((float)(gy - L.iy), 0.0f), 1.0f);
+    L.pz = fminf(fmaxf((float)(gz - L.iz), 0.0f), 1.0f);
+    L.fkx = (float)L.kx; L.fky = (float)L.ky; L.fkz = (float)L.kz;
+    L.sx = L.fkx > 0.0f ? 1 : (L.fkx < 0.0f ? -1 : 0);
+    L.sy = L.fky > 0.0f ? 1 : (L.fky < 0.0f ? -1 : 0);
+    L.sz = L.fkz > 0.0f ? 1 : (L.fkz < 0.0f ? -1 : 0);
+    L.ftcur = 0.0f;
+}
+
+// distance (cell units) to the exit face of the current cell and its axis
+__device__ __forceinline__ float exit_f(const Lane &L, int &ax)
+{
+    const float inf = __builtin_huge_valf();
+    const float tx = L.sx > 0 ? (1.0f - L.px) / L.fkx : (L.sx < 0 ? -L.px / L.fkx : inf);
+    const float ty = L.sy > 0 ? (1.0f - L.py) / L.fky : (L.sy < 0 ? -L.py / L.fky : inf);
+    const float tz = L.sz > 0 ? (1.0f - L.pz) / L.fkz : (L.sz < 0 ? -L.pz / L.fkz : inf);
+    ax = 0;
+    float t = tx;
+    if (ty < t) { t = ty; ax = 1; }
+    if (tz < t) { t = tz; ax = 2; }
+    return fmaxf(t, 0.0f);
+}
+
+// move by t (cell units) without leaving the cell
+__device__ __forceinline__ void move_f(Lane &L, float t)
+{
+    L.px = fminf(fmaxf(L.px + t * L.fkx, 0.0f), 1.0f);
+    L.py = fminf(fmaxf(L.py + t * L.fky, 0.0f), 1.0f);
+    L.pz = fminf(fmaxf(L.pz + t * L.fkz, 0.0f), 1.0f);
+}
+
+// cross the face on axis ax; returns false when outside the box
+__device__ __forceinline__ bool cross_f(const DevGrid &g, Lane &L, int ax, float t)
+{
+    move_f(L, t);
+    if (ax == 0) { L.ix += L.sx; L.px = L.sx > 0 ? 0.0f : 1.0f; return L.ix >= 0 && L.ix < g.nx; }
+    if (ax == 1) { L.iy += L.sy; L.py = L.sy > 0 ? 0.0f : 1.0f; return L.iy >= 0 && L.iy < g.ny; }
+    L.iz += L.sz;
+    L.pz = L.sz > 0 ? 0.0f : 1.0f;
+    return L.iz >= 0 && L.iz < g.nz;
+}
+
+// ---------------------------------------------------------------------------
+template <int P>
+__device__ __forceinline__ void start_flight(const KArgs &a, Lane &L, double tau_in)
+{
+    L.wl = L.wgt * a.tab->Lpacket;
+    if constexpr (Prec<P>::mixed) {
+        dda_init_f(a.g, L);
+        L.ftau = 0.0f;
+        L.ftau_in = (float)tau_in;
+    } else {
+        dda_init_d(a.g, L);
+        L.tau = 0.0;
+        L.tau_in = tau_in;
+    }
+    L.phase = 2;
+}
+
+// scattering event at the end of a flight that stayed inside
+// (transport_ion_packet loop body, FP64 in both forms: events are rare)
+template <int P>
+__device__ __forceinline__ void scatter_event(const KArgs &a, Lane &L)
+{
+    const int il = L.leaf;
+    if (il < 0) { L.phase = 0; return; }
+    const double kap = opacity_d(a, L, il);
+    L.wgt = L.wgt * (a.s.rhokap[il] * L.dsca / fmax(kap, DBL_MIN));
+    if (L.wgt < kWmin) {
+        if (rng_u(a.seed, L.id, L.nk++) > kPsurv) { L.phase = 0; return; }
+        L.wgt = L.wgt / kPsurv;
+    }
+    const double gg = L.dg;
+    double cost;
+    if (fabs(gg) > 1.0e-3) {
+        const double q = (1.0 - gg * gg) / (1.0 + gg * (2.0 * rng_u(a.seed, L.id, L.nk++) - 1.0));
+        cost = (1.0 + gg * gg - q * q) / (2.0 * gg);
+    } else {
+        cost = 2.0 * rng_u(a.seed, L.id, L.nk++) - 1.0;
+    }
+    cost = fmin(fmax(cost, -1.0), 1.0);
+    const double sint = sqrt(1.0 - cost * cost);
+    const double phi = MC_TWOPI * rng_u(a.seed, L.id, L.nk++);
+    double kx = L.kx, ky = L.ky, kz = L.kz;
+    double nrm = sqrt(kx * kx + ky * ky + kz * kz);
+    if (nrm <= 0.0) { L.phase = 0; return; }
+    kx /= nrm; ky /= nrm; kz /= nrm;
+    double ux, uy, uz;
+    if (fabs(kz) < 0.99) { ux = -ky; uy = kx; uz = 0.0; }
+    else { ux = 0.0; uy = -kz; uz = ky; }
+    nrm = sqrt(ux * ux + uy * uy + uz * uz);
+    ux /= nrm; uy /= nrm; uz /= nrm;
+    const double vx = ky * uz - kz * uy, vy = kz * ux - kx * uz, vz = kx * uy - ky * ux;
+    const double cp = cos(phi), sp = sin(phi);
+    double nx = sint * (cp * ux + sp * vx) + cost * kx;
+    double ny = sint * (cp * uy + sp * vy) + cost * ky;
+    double nz = sint * (cp * uz + sp * vz) + cost * kz;
+    nrm = sqrt(nx * nx + ny * ny + nz * nz);
+    L.kx = nx / nrm; L.ky = ny / nrm; L.kz = nz / nrm;
+    L.nscatt++;
+    const double r = rng_u(a.seed, L.id, L.nk++);
+    start_flight<P>(a, L, -log(fmax(r, DBL_MIN)));
+}
+
+// end of the direct walk: energy ledger, then the forced first interaction
+template <int P>
+__device__ __forceinline__ void end_edge(const KArgs &a, Lane &L, double tau_edge, Counters &cn)
+{
+    const double lum = a.tab->Lpacket * L.wgt;
+    const double tr = exp(-fmax(tau_edge, 0.0));
+    cn.e_emit += lum;
+    cn.e_abs += lum * (1.0 - tr);
+    cn.e_esc += lum * tr;
+    if (!(a.add_dust && a.dust_scatter) || tau_edge <= 1.0e-12) { L.phase = 0; return; }
+    const double w1 = 1.0 - exp(-tau_edge);
+    L.wgt = L.wgt * w1;
+    const double tau = -log(1.0 - rng_u(a.seed, L.id, L.nk++) * w1);
+    start_flight<P>(a, L, tau);
+}
+
+// one segment of the current walk (edge walk or tau flight), car grid
+template <int P>
+__device__ __forceinline__ void car_step(const KArgs &a, Lane &L, Counters &cn)
+{
+    const DevGrid &g = a.g;
+    const int il = L.ix + g.nx * (L.iy + g.ny * L.iz);
+    if constexpr (!Prec<P>::mixed) {
+#pragma clang fp contract(off)
+        const int ax = dda_axis_d(L);
+        const double seg = tm_of(L, ax) - L.tcur;
+        const double kap = opacity_d(a, L, il);
+        if (L.phase == 1) {
+            double pl;
+            if (kap * seg > 0.0) {
+                const double eo = L.expo * exp(-kap * seg);
+                pl = L.wl * (L.expo - eo) / kap;
+                L.expo = eo;
+            } else {
+                pl = L.wl * L.expo * seg;
+            }
+            score(a, L, il, pl, cn);
+            L.tau = L.tau + kap * seg;
+            if (L.tau >= MC_TAU_HUGE || !dda_advance_d(g, L, ax)) end_edge<P>(a, L, L.tau, cn);
+        } else {
+            if (L.tau + seg * kap >= L.tau_in) {
+                const double ds = kap > 0.0 ? (L.tau_in - L.tau) / kap : seg;
+                if (L.nscatt > 0) score(a, L, il, L.wl * ds, cn);
+                L.tcur = L.tcur + ds;
+                L.x = L.x + L.tcur * L.kx; L.y = L.y + L.tcur * L.ky; L.z = L.z + L.tcur * L.kz;
+                L.leaf = il;
+                scatter_event<P>(a, L);
+                return;
+            }
+            if (L.nscatt > 0) score(a, L, il, L.wl * seg, cn);
+            L.tau = L.tau + seg * kap;
+            if (!dda_advance_d(g, L, ax)) L.phase = 0;
+        }
+    } else {
+        int ax;
+        const float t = exit_f(L, ax);
+        const float kc = opacity_f(a, L, il) * (float)g.dcell;   // per cell length
+        const float dt = kc * t;
+        const double wlc = L.wl * g.dcell;
+        if (L.phase == 1) {
+            double pl;
+            if (dt > 0.0f) {
+                const float f = L.fexpo * -expm1f(-dt);
+                pl = wlc * (double)(f / kc);
+                L.fexpo = L.fexpo * expf(-dt);
+            } else {
+                pl = wlc * (double)(L.fexpo * t);
+            }
+            score(a, L, il, pl, cn);
+            L.ftau = L.ftau + dt;
+            if (L.ftau >= (float)MC_TAU_HUGE || !cross_f(g, L, ax, t))
+                end_edge<P>(a, L, (double)L.ftau, cn);
+        } else {
+            if (L.ftau + dt >= L.ftau_in) {
+                const float ds = kc > 0.0f ? (L.ftau_in - L.ftau) / kc : t;
+                if (L.nscatt > 0) score(a, L, il, wlc * (double)ds, cn);
+                move_f(L, ds);
+                L.x = g.xmin + ((double)L.ix + (double)L.px) * g.dcell;
+                L.y = g.ymin + ((double)L.iy + (double)L.py) * g.dcell;
+                L.z = g.zmin + ((double)L.iz + (double)L.pz) * g.dcell;
+                L.leaf = il;
+                scatter_event<P>(a, L);
+                return;
+            }
+            if (L.nscatt > 0) score(a, L, il, wlc * (double)t, cn);
+            L.ftau = L.ftau + dt;
+            if (!cross_f(g, L, ax, t)) L.phase = 0;
+        }
+    }
+}
+
+template <int P>
+__device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r)
+{
+    const PacketBuf &pk = a.pk;
+    L.id = a.first_id + r;
+    L.x = pk.x[r]; L.y = pk.y[r]; L.z = pk.z[r];
+    L.kx = pk.kx[r]; L.ky = pk.ky[r]; L.kz = pk.kz[r];
+    L.E = pk.E[r];
+    L.inu = pk.inu[r];
+    L.leaf = pk.leaf[r];
+    const double *sg = pk.sig + 6 * r;
+    L.sHI = sg[0]; L.sHeI = sg[1]; L.sHeII = sg[2];
+    L.dabs = sg[3]; L.dsca = sg[4]; L.dg = sg[5];
+    const double he = L.E * MC_EV2ERG;
+    L.coeff[0] = L.sHI / he;
+    L.coeff[1] = L.sHeI / he;
+    L.coeff[2] = L.sHeII / he;
+    L.coeff[3] = L.sHI * fmax(1.0 - MC_ETH_HI / L.E, 0.0);
+    L.coeff[4] = L.sHeI * fmax(1.0 - MC_ETH_HEI / L.E, 0.0);
+    L.coeff[5] = L.sHeII * fmax(1.0 - MC_ETH_HEII / L.E, 0.0);
+    L.wgt = 1.0;
+    L.nscatt = 0;
+    L.nk = 0;
+    L.wl = a.tab->Lpacket;
+    L.phase = 1;
+    if constexpr (Prec<P>::mixed) {
+        L.fsHI = (float)L.sHI; L.fsHeI = (float)L.sHeI; L.fsHeII = (float)L.sHeII;
+        L.fdust = a.add_dust ? (float)(L.dabs + (a.dust_scatter ? L.dsca : 0.0)) : 0.0f;
+        dda_init_f(a.g, L);
+        L.fexpo = 1.0f;
+        L.ftau = 0.0f;
+    } else {
+        dda_init_d(a.g, L);
+        L.expo = 1.0;
+        L.tau = 0.0;
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_walk_car(KArgs a)
+{
+    Lane L;
+    L.phase = 0;
+    Counters cn = {0.0, 0.0, 0.0, 0ull, 0ull};
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        const unsigned long long need = __ballot(L.phase == 0);
+        if (need) {
+            const int leader = __ffsll((long long)need) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(a.s.counter, (unsigned long long)__popcll(need));
+            base = __shfl(base, leader);
+            if (L.phase == 0) {
+                const unsigned long long r = base + __popcll(need & ((1ull << lane) - 1ull));
+                if ((int64_t)r < a.npk) load_packet<P>(a, L, (int64_t)r);
+                else L.phase = 3;
+            }
+        }
+        const bool busy = L.phase == 1 || L.phase == 2;
+        if (!__ballot(busy)) break;
+        if (busy) car_step<P>(a, L, cn);
+    }
+    // wave reduction of the ledger and segment counts, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        cn.e_emit += __shfl_down(cn.e_emit, off);
+        cn.e_abs += __shfl_down(cn.e_abs, off);
+        cn.e_esc += __shfl_down(cn.e_esc, off);
+        cn.sd += __shfl_down(cn.sd, off);
+        cn.ss += __shfl_down(cn.ss, off);
+    }
+    if (lane == 0) {
+        atomicAdd(a.s.ledger + 0, cn.e_emit);
+        atomicAdd(a.s.ledger + 1, cn.e_abs);
+        atomicAdd(a.s.ledger + 2, cn.e_esc);
+        atomicAdd(a.s.segs + 0, cn.sd);
+        atomicAdd(a.s.segs + 1, cn.ss);
+    }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_zero(double *p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+__global__ void k_reset_counter(unsigned long long *p) { *p = 0ull; }
+
+__global__ void k_zero_counters(DevState s)
+{
+    if (threadIdx.x < 3) s.ledger[threadIdx.x] = 0.0;
+    if (threadIdx.x < 2) s.segs[threadIdx.x] = 0ull;
+    if (threadIdx.x == 0) *s.counter = 0ull;
+}
+
+// leaf half width (car: constant)
+__device__ __forceinline__ double leaf_half(const DevGrid &g, int il)
+{
+    return g.car ? 0.5 * g.dcell : g.ch[g.cell_of_leaf[il]];
+}
+
+// gas_rates_compute: rate sums / (leaf volume * d2cm^2)
+__global__ __launch_bounds__(kBlock) void k_rates(DevGrid g, DevState s, double d2cm)
+{
+    const int il = blockIdx.x * kBlock + threadIdx.x;
+    if (il >= g.nleaf) return;
+    const double w = 2.0 * leaf_half(g, il);
+    const double fac = 1.0 / ((w * w) * w * (d2cm * d2cm));
+    const double *h = s.hhe + (size_t)il * 8;
+    for (int k = 0; k < 6; ++k) s.rates[(size_t)k * g.nleaf + il] = h[k] * fac;
+}
+
+// ---------------------------------------------------------------------------
+// recombination and collisional-ionization coefficients (FP64)
+__device__ __forceinline__ double rr_badnell(double T, double A, double B, double T0, double T1, double Cc,
+                                    double T2)
+{
+    const double bp = B + Cc * exp(-T2 / T), tt = sqrt(T / T0);
+    return A / (tt * pow(1.0 + tt, 1.0 - bp) * pow(1.0 + sqrt(T / T1), 1.0 + bp));
+}
+__device__ __forceinline__ double rr_mao(double T, double a0, double b0, double c0, double a1, double b1,
+                                double a2, double b2)
+{
+    const double te = T * (MC_KBOLTZ / MC_EV2ERG);
+    return a0 * 1.0e-10 * pow(te, -b0 - c0 * log(te)) * (1.0 + a2 * pow(te, -b2)) /
+           (1.0 + a1 * pow(te, -b1));
+}
+__device__ __forceinline__ double voronov(double T, double dE, double P, double A, double X, double K)
+{
+    const double U = dE * MC_EV2ERG / (MC_KBOLTZ * T);
+    return A * (1.0 + P * sqrt(U)) * pow(U, K) * exp(-U) / (X + U);
+}
+
+struct RateSet { double aH, aHe2, aHe3, cH, cHe1, cHe2; };
+
+__device__ RateSet rates_at(double T, bool caseA)
+{
+    RateSet r;
+    const double aA_H = rr_badnell(T, 8.318e-11, 0.7472, 2.965, 7.001e5, 0.0, 0.0);
+    const double aA_He2 = rr_badnell(T, 5.235e-11, 0.6988, 7.301, 4.475e6, 0.0829, 1.682e5) +
+                          pow(T, -1.5) * (1.417e-3 * exp(-4.633e5 / T) + 2.235e-4 * exp(-5.532e5 / T) -
+                                          2.185e-5 * exp(-8.887e5 / T));
+    const double aA_He3 = rr_badnell(T, 1.818e-10, 0.7492, 1.017e1, 2.786e6, 0.0, 0.0);
+    if (caseA) {
+        r.aH = aA_H; r.aHe2 = aA_He2; r.aHe3 = aA_He3;
+    } else {
+        r.aH = aA_H - rr_mao(T, 1.87007e-2, 1.29077, -2.17465e-3, 1.29189e1, 8.05060e-1, 8.44399e-2,
+                             5.93940e-4);
+        r.aHe2 = aA_He2 - rr_mao(T, 3.26862e-3, 6.97361e-1, 3.14907e-4, 5.76341e1, 1.18534,
+                                 2.58417e1, 9.90134e-1);
+        r.aHe3 = aA_He3 - rr_mao(T, 2.61525e-1, 1.39527, -3.22079e-4, 6.40708e1, 8.97582e-1,
+                                 5.01551e-1, 1.21533e-3);
+    }
+    r.cH = voronov(T, 13.6, 0.0, 2.91e-8, 0.232, 0.39);
+    r.cHe1 = voronov(T, 24.6, 0.0, 1.75e-8, 0.180, 0.35);
+    r.cHe2 = voronov(T, 54.4, 1.0, 2.05e-9, 0.265, 0.25);
+    return r;
+}
+
+// solve_ion_cell without metals: damped fixed point on n_e
+__device__ double solve_cell(const RateSet &r, double gH, double gHe1, double gHe2, double nH,
+                             double yHe, double &xHI, double &xHeI, double &xHeII)
+{
+    double xHeIII = fmax(0.0, 1.0 - xHeI - xHeII);
+    double ne = fmax(nH * ((1.0 - xHI) + yHe * (xHeII + 2.0 * xHeIII)), 1.0e-12 * nH);
+    for (int it = 0; it < 200; ++it) {
+        const double ne_old = ne;
+        const double den = gH + (r.cH + r.aH) * ne;
+        xHI = den > 0.0 ? (r.aH * ne) / den : 1.0;
+        const double r1 = fmin((gHe1 + r.cHe1 * ne) / (r.aHe2 * ne), 1.0e150);
+        const double r2 = fmin((gHe2 + r.cHe2 * ne) / (r.aHe3 * ne), 1.0e150);
+        xHeI = 1.0 / (1.0 + r1 + r1 * r2);
+        xHeII = xHeI * r1;
+        xHeIII = xHeII * r2;
+        ne = nH * ((1.0 - xHI) + yHe * (xHeII + 2.0 * xHeIII));
+        ne = fmax(0.5 * (ne + ne_old), 1.0e-12 * nH);
+        if (fabs(ne - ne_old) <= 1.0e-10 * ne) break;
+    }
+    return ne;
+}
+
+__device__ __forceinline__ float4 opac_of(double nH, double xHI, double xHeI, double xHeII, double rk,
+                                 double yHe, double d2cm)
+{
+    return make_float4((float)(nH * xHI * d2cm), (float)(yHe * nH * xHeI * d2cm),
+                       (float)(yHe * nH * xHeII * d2cm), (float)rk);
+}
+
+template <int N>
+__device__ __forceinline__ void block_reduce(double (&v)[N], bool (&is_max)[N], double *out)
+{
+    __shared__ double sh[kBlock / 64][N];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int k = 0; k < N; ++k)
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_down(v[k], off);
+            v[k] = is_max[k] ? fmax(v[k], o) : v[k] + o;
+        }
+    if (lane == 0)
+        for (int k = 0; k < N; ++k) sh[wv][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < N; ++k) {
+            double acc = sh[0][k];
+            for (int w = 1; w < kBlock / 64; ++w) acc = is_max[k] ? fmax(acc, sh[w][k]) : acc + sh[w][k];
+            out[(size_t)blockIdx.x * N + k] = acc;
+        }
+}
+
+// gas_equilibrium_update at fixed T; partial metrics per block:
+// {max|dx|, sum V|dx|, sum V x_HII}
+__global__ __launch_bounds__(kBlock) void k_equilibrium(DevGrid g, DevState s, double T, bool caseA,
+                                                        double yHe, double w, double d2cm)
+{
+    const int il = blockIdx.x * kBlock + threadIdx.x;
+    double v[3] = {0.0, 0.0, 0.0};
+    bool ismax[3] = {true, false, false};
+    if (il < g.nleaf) {
+        const double nH = s.nH[il];
+        if (nH <= 0.0) {
+            s.ne[il] = 0.0;
+        } else {
+            const RateSet r = rates_at(T, caseA);
+            const size_t n = g.nleaf;
+            double xHI = s.xHI[il], xHeI = s.xHeI[il], xHeII = s.xHeII[il];
+            solve_cell(r, s.rates[il], s.rates[n + il], s.rates[2 * n + il], nH, yHe, xHI, xHeI,
+                       xHeII);
+            const double oHI = s.xHI[il];
+            const double nHI = (1.0 - w) * oHI + w * xHI;
+            const double nHeI = (1.0 - w) * s.xHeI[il] + w * xHeI;
+            const double nHeII = (1.0 - w) * s.xHeII[il] + w * xHeII;
+            const double xHeIII = fmax(0.0, 1.0 - nHeI - nHeII);
+            s.xHI[il] = nHI;
+            s.xHeI[il] = nHeI;
+            s.xHeII[il] = nHeII;
+            s.ne[il] = nH * ((1.0 - nHI) + yHe * (nHeII + 2.0 * xHeIII));
+            s.opac[il] = opac_of(nH, nHI, nHeI, nHeII, s.rhokap[il], yHe, d2cm);
+            const double wd = 2.0 * leaf_half(g, il), vol = (wd * wd) * wd;
+            const double d = fabs(nHI - oHI);
+            v[0] = d;
+            v[1] = vol * d;
+            v[2] = vol * (1.0 - nHI);
+        }
+    }
+    block_reduce<3>(v, ismax, s.partials);
+}
+
+// hydrogen_balance_residual_grid partials {max eps, sum V eps, sum V}
+__global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, DevState s, bool caseA)
+{
+    const int il = blockIdx.x * kBlock + threadIdx.x;
+    double v[3] = {0.0, 0.0, 0.0};
+    bool ismax[3] = {true, false, false};
+    if (il < g.nleaf && s.nH[il] > 0.0 && 1.0 - s.xHI[il] >= 1.0e-10) {
+        const RateSet r = rates_at(s.Te[il], caseA);
+        const double nHI = s.nH[il] * s.xHI[il], nHII = s.nH[il] * (1.0 - s.xHI[il]);
+        const double loss = nHI * (s.rates[il] + r.cH * s.ne[il]);
+        const double gain = (r.aH * s.ne[il]) * nHII;
+        const double eps = fabs(loss - gain) / fmax(loss, DBL_MIN);
+        const double wd = 2.0 * leaf_half(g, il), vol = (wd * wd) * wd;
+        v[0] = eps; v[1] = vol * eps; v[2] = vol;
+    }
+    block_reduce<3>(v, ismax, s.partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_opacity_fill(DevGrid g, DevState s, double yHe, double d2cm)
+{
+    const int il = blockIdx.x * kBlock + threadIdx.x;
+    if (il >= g.nleaf) return;
+    s.opac[il] = opac_of(s.nH[il], s.xHI[il], s.xHeI[il], s.xHeII[il], s.rhokap[il], yHe, d2cm);
+}
+
+// diagnostic: cells of one ray's direct walk (single thread)
+template <int P>
+__global__ void k_trace(KArgs a, double x, double y, double z, double kx, double ky, double kz,
+                        int32_t *cells, int maxn, int32_t *n)
+{
+    Lane L;
+    L.x = x; L.y = y; L.z = z; L.kx = kx; L.ky = ky; L.kz = kz;
+    int k = 0;
+    const DevGrid &g = a.g;
+    if constexpr (Prec<P>::mixed) {
+        dda_init_f(g, L);
+        for (; k < maxn;) {
+            cells[k++] = 1 + L.ix + g.nx * (L.iy + g.ny * L.iz);
+            int ax;
+            const float t = exit_f(L, ax);
+            if (!cross_f(g, L, ax, t)) break;
+        }
+    } else {
+        dda_init_d(g, L);
+        for (; k < maxn;) {
+            cells[k++] = 1 + L.ix + g.nx * (L.iy + g.ny * L.iz);
+            if (!dda_advance_d(g, L, dda_axis_d(L))) break;
+        }
+    }
+    *n = k;
+}
+
+KArgs make_args(mc_ctx *c)
+{
+    KArgs a;
+    a.tab = c->d_tab;
+    a.samp = c->d_samp;
+    a.nknots = (int32_t)c->samp_e.size();
+    a.dust = c->d_dust;
+    a.dn = c->dn;
+    a.dcref = c->dcref;
+    a.add_dust = c->p.ion_add_dust;
+    a.dust_scatter = c->p.ion_dust_scatter;
+    a.He_abund = c->p.He_abund;
+    a.d2cm = c->p.distance2cm;
+    a.seed = (uint32_t)c->p.qmc_seed;
+    a.g = c->grid;
+    a.s = c->st;
+    a.pk = c->pk;
+    a.first_id = 1;
+    a.npk = 0;
+    return a;
+}
+
+int persistent_blocks(const void *kern)
+{
+    int dev = 0, ncu = 0, per = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, kBlock, 0) != hipSuccess || per < 1)
+        per = 2;
+    return ncu * per;
+}
+
+int sum_partials(mc_ctx *c, int nblk, double out[3])
+{
+    std::vector<double> h((size_t)nblk * 3);
+    MC_HIP(hipMemcpyAsync(h.data(), c->st.partials, sizeof(double) * 3 * nblk,
+                          hipMemcpyDeviceToHost, c->stream));
+    MC_HIP(hipStreamSynchronize(c->stream));
+    out[0] = 0.0; out[1] = 0.0; out[2] = 0.0;
+    for (int b = 0; b < nblk; ++b) {      // fixed order: deterministic
+        out[0] = std::fmax(out[0], h[(size_t)b * 3]);
+        out[1] += h[(size_t)b * 3 + 1];
+        out[2] += h[(size_t)b * 3 + 2];
+    }
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+int mc_run_pass(mc_ctx *c)
+{
+    KArgs a = make_args(c);
+    const size_t nl = (size_t)c->nleaf;
+    hipStream_t st = c->stream;
+    MC_HIP(hipEventRecord(c->ev[0], st));
+    k_zero<<<1024, kBlock, 0, st>>>(c->st.hhe, nl * 8);
+    k_zero<<<1024, kBlock, 0, st>>>(c->st.jt, nl * c->nnu);
+    k_zero_counters<<<1, 64, 0, st>>>(c->st);
+    float t_launch = 0.0f, t_walk = 0.0f;
+    const int64_t N = c->p.nphotons;
+    const void *kw = c->p.precision == MC_MIXED ? (const void *)k_walk_car<MC_MIXED>
+                                                : (const void *)k_walk_car<MC_FP64>;
+    if (!c->car) {
+        mc_set_error("octree transport is not built into this library version");
+        return -3;
+    }
+    const int nblk = persistent_blocks(kw);
+    for (int64_t first = 1; first <= N; first += c->pk.cap) {
+        a.first_id = first;
+        a.npk = std::min<int64_t>(c->pk.cap, N - first + 1);
+        MC_HIP(hipEventRecord(c->ev[1], st));
+        k_launch<<<(unsigned)((a.npk + kBlock - 1) / kBlock), kBlock, 0, st>>>(a);
+        MC_HIP(hipGetLastError());
+        k_reset_counter<<<1, 1, 0, st>>>(c->st.counter);
+        MC_HIP(hipEventRecord(c->ev[2], st));
+        if (c->p.precision == MC_MIXED)
+            k_walk_car<MC_MIXED><<<nblk, kBlock, 0, st>>>(a);
+        else
+            k_walk_car<MC_FP64><<<nblk, kBlock, 0, st>>>(a);
+        MC_HIP(hipGetLastError());
+        MC_HIP(hipEventRecord(c->ev[3], st));
+        MC_HIP(hipEventSynchronize(c->ev[3]));
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, c->ev[1], c->ev[2]);
+        t_launch += ms;
+        hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
+        t_walk += ms;
+    }
+    double led[3];
+    unsigned long long sg[2];
+    MC_HIP(hipMemcpyAsync(led, c->st.ledger, sizeof(led), hipMemcpyDeviceToHost, st));
+    MC_HIP(hipMemcpyAsync(sg, c->st.segs, sizeof(sg), hipMemcpyDeviceToHost, st));
+    MC_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < 3; ++k) c->ledger[k] = led[k];
+    c->segs[0] = (int64_t)sg[0];
+    c->segs[1] = (int64_t)sg[1];
+    c->times[0] = t_launch;
+    c->times[1] = t_walk;
+    return 0;
+}
+
+int mc_run_rates(mc_ctx *c)
+{
+    hipStream_t st = c->stream;
+    MC_HIP(hipEventRecord(c->ev[4], st));
+    k_rates<<<(c->nleaf + kBlock - 1) / kBlock, kBlock, 0, st>>>(c->grid, c->st, c->p.distance2cm);
+    MC_HIP(hipGetLastError());
+    MC_HIP(hipEventRecord(c->ev[5], st));
+    MC_HIP(hipEventSynchronize(c->ev[5]));
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, c->ev[4], c->ev[5]);
+    c->times[2] = ms;
+    return 0;
+}
+
+int mc_run_equilibrium(mc_ctx *c, double *max_dx, double *dx_vol)
+{
+    const int nblk = (c->nleaf + kBlock - 1) / kBlock;
+    hipStream_t st = c->stream;
+    MC_HIP(hipEventRecord(c->ev[6], st));
+    k_equilibrium<<<nblk, kBlock, 0, st>>>(c->grid, c->st, c->p.te_fixed, c->p.case_A != 0,
+                                            c->p.He_abund, c->p.ion_relax, c->p.distance2cm);
+    MC_HIP(hipGetLastError());
+    MC_HIP(hipEventRecord(c->ev[7], st));
+    double m[3];
+    if (sum_partials(c, nblk, m)) return -100;
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, c->ev[6], c->ev[7]);
+    c->times[3] = ms;
+    *max_dx = m[0];
+    *dx_vol = m[1] / std::fmax(m[2], DBL_MIN);
+    return 0;
+}
+
+int mc_run_residual(mc_ctx *c, double *eps_max, double *eps_vol)
+{
+    const int nblk = (c->nleaf + kBlock - 1) / kBlock;
+    k_residual<<<nblk, kBlock, 0, c->stream>>>(c->grid, c->st, c->p.case_A != 0);
+    MC_HIP(hipGetLastError());
+    double m[3];
+    if (sum_partials(c, nblk, m)) return -100;
+    *eps_max = m[0];
+    *eps_vol = m[1] / std::fmax(m[2], DBL_MIN);
+    return 0;
+}
+
+int mc_run_opacity_fill(mc_ctx *c)
+{
+    k_opacity_fill<<<(c->nleaf + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
+        c->grid, c->st, c->p.He_abund, c->p.distance2cm);
+    MC_HIP(hipGetLastError());
+    MC_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mc_run_trace(mc_ctx *c, int precision, const double *pos, const double *dir, int32_t *cells,
+                 int maxn, int32_t *n)
+{
+    if (!c->car) {
+        mc_set_error("mc_trace_ray: octree walk not available");
+        return -3;
+    }
+    KArgs a = make_args(c);
+    int32_t *dcells = nullptr, *dn = nullptr;
+    MC_HIP(hipMalloc(&dcells, sizeof(int32_t) * (size_t)maxn));
+    MC_HIP(hipMalloc(&dn, sizeof(int32_t)));
+    if (precision == MC_MIXED)
+        k_trace<MC_MIXED><<<1, 1, 0, c->stream>>>(a, pos[0], pos[1], pos[2], dir[0], dir[1], dir[2],
+                                                  dcells, maxn, dn);
+    else
+        k_trace<MC_FP64><<<1, 1, 0, c->stream>>>(a, pos[0], pos[1], pos[2], dir[0], dir[1], dir[2],
+                                                 dcells, maxn, dn);
+    MC_HIP(hipGetLastError());
+    MC_HIP(hipMemcpyAsync(n, dn, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    MC_HIP(hipStreamSynchronize(c->stream));
+    MC_HIP(hipMemcpy(cells, dcells, sizeof(int32_t) * (size_t)(*n), hipMemcpyDeviceToHost));
+    hipFree(dcells);
+    hipFree(dn);
+    return 0;
+}

@@ -1,0 +1,333 @@
+// Host-side set-up of a transport context: band edges, the continuous
+// source samplers, the Sobol/Owen tables and the octree build.  Runs once per
+// context (or per grid), so it is plain FP64 C++ on the host.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+
+#include "mc_internal.h"
+
+// ---------------------------------------------------------------------------
+// Sobol direction numbers and Owen keys (reference: src/qmc_mod.f90:73-139).
+// Dimension 1 is van der Corput; dimensions 2..12 expand the published
+// Joe-Kuo new-joe-kuo-6 rows (degree, coefficient bits, initial m_k).
+void mc_qmc_tables(int seed, uint32_t V[32][MC_QMC_DIM], uint32_t key[MC_QMC_DIM][2])
+{
+    static const int deg[MC_QMC_DIM] = {0, 1, 2, 3, 3, 4, 4, 5, 5, 5, 5, 5};
+    static const int cof[MC_QMC_DIM] = {0, 0, 1, 1, 2, 1, 4, 2, 4, 7, 11, 13};
+    static const int m0[MC_QMC_DIM][5] = {
+        {0}, {1}, {1, 3}, {1, 3, 1}, {1, 1, 1}, {1, 1, 3, 3}, {1, 3, 5, 13},
+        {1, 1, 5, 5, 17}, {1, 1, 5, 5, 5}, {1, 1, 7, 11, 19}, {1, 1, 5, 1, 1},
+        {1, 1, 1, 3, 11}};
+    for (int b = 0; b < 32; ++b) V[b][0] = 0x80000000u >> b;
+    for (int d = 1; d < MC_QMC_DIM; ++d) {
+        const int s = deg[d];
+        for (int b = 0; b < s; ++b) V[b][d] = (uint32_t)m0[d][b] << (31 - b);
+        for (int b = s; b < 32; ++b) {
+            uint32_t v = V[b - s][d] ^ (V[b - s][d] >> s);
+            for (int j = 1; j < s; ++j)
+                if ((cof[d] >> (s - 1 - j)) & 1) v ^= V[b - j][d];
+            V[b][d] = v;
+        }
+    }
+    auto avalanche = [](uint32_t x) {
+        x ^= x >> 16; x *= 0x7FEB352Du; x ^= x >> 15; x *= 0x846CA68Bu; x ^= x >> 16;
+        return x;
+    };
+    for (int d = 0; d < MC_QMC_DIM; ++d) {
+        const uint32_t b = (uint32_t)seed + (uint32_t)(d + 1) * 0x9E3779B9u;
+        key[d][0] = avalanche(b);
+        key[d][1] = avalanche(b + 0x85EBCA6Bu);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Band edges.  With ion_align_edges the H/He thresholds inside the band split
+// it into sub-segments and the bins are shared out by log width, floor >= 1,
+// remainder by largest fractional part (reference: build_aligned_edges,
+// src/ion_band_mod.f90:1053-1214; no metal edges on this path).
+int mc_setup_band(mc_ctx *c)
+{
+    const mc_params &p = c->p;
+    const int n = p.nnu_ion;
+    if (n < 1 || n > MC_MAXBINS) {
+        mc_set_error("nnu_ion must be in [1, 64]");
+        return -1;
+    }
+    c->nnu = n;
+    c->eedge.assign(n + 1, 0.0);
+    const double lo = p.eion_min, hi = p.eion_max;
+    if (!p.ion_align_edges) {
+        for (int i = 0; i <= n; ++i)
+            c->eedge[i] = std::exp(std::log(lo) + (std::log(hi) - std::log(lo)) * (double)i / n);
+        return 0;
+    }
+    std::vector<double> cut;
+    for (double t : {MC_ETH_HI, MC_ETH_HEI, MC_ETH_HEII})
+        if (t > lo && t < hi) {
+            // thresholds within 1% in log merge (all three have equal priority)
+            if (!cut.empty() && std::fabs(std::log(t / cut.back())) < 0.01) continue;
+            cut.push_back(t);
+        }
+    std::vector<double> pts;
+    pts.push_back(lo);
+    pts.insert(pts.end(), cut.begin(), cut.end());
+    pts.push_back(hi);
+    const int nseg = (int)pts.size() - 1;
+    if (n < nseg) {
+        mc_set_error("nnu_ion too small for the aligned H/He thresholds");
+        return -1;
+    }
+    std::vector<double> w(nseg);
+    double wsum = 0.0;
+    for (int j = 0; j < nseg; ++j) {
+        w[j] = std::log(pts[j + 1]) - std::log(pts[j]);
+        wsum += w[j];
+    }
+    std::vector<int> nb(nseg);
+    int tot = 0;
+    for (int j = 0; j < nseg; ++j) {
+        nb[j] = std::max(1, (int)((double)n * w[j] / wsum));
+        tot += nb[j];
+    }
+    for (int r = n - tot; r > 0; --r) {
+        int best = 0;
+        double fr = -1.0;
+        for (int j = 0; j < nseg; ++j) {
+            const double f = (double)n * w[j] / wsum - (double)nb[j];
+            if (f > fr) { fr = f; best = j; }
+        }
+        nb[best]++;
+    }
+    for (int r = n - tot; r < 0;) {
+        // only reached when the floor of 1 overshoots; r re-evaluated below
+        int best = -1;
+        double ex = -1.0e30;
+        for (int j = 0; j < nseg; ++j) {
+            if (nb[j] <= 1) continue;
+            const double e = (double)nb[j] - (double)n * w[j] / wsum;
+            if (e > ex) { ex = e; best = j; }
+        }
+        if (best < 0) break;
+        nb[best]--;
+        ++r;
+    }
+    int k = 0;
+    c->eedge[0] = pts[0];
+    for (int j = 0; j < nseg; ++j) {
+        const double a = std::log(pts[j]), b = std::log(pts[j + 1]);
+        for (int ib = 1; ib <= nb[j]; ++ib) c->eedge[++k] = std::exp(a + (b - a) * (double)ib / nb[j]);
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Planck shape E^3/(e^x - 1) with the small- and large-x forms
+// (reference: planck_energy_shape, src/energy_sampler_mod.f90:277-290).
+static double planck_shape(double E, double T)
+{
+    const double x = E / (8.617333262145e-5 * T);
+    const double e3 = (E * E) * E;
+    if (x <= 0.0) return 0.0;
+    if (x < 1.0e-3) return e3 / (x * (1.0 + 0.5 * x + x * x / 6.0));
+    if (x < 700.0) return e3 / (std::exp(x) - 1.0);
+    return e3 * std::exp(-x);
+}
+
+// Adaptive piecewise-linear Planck density on [emin, emax] with mandatory
+// knots at the thresholds; an interval is split at its midpoint until the
+// trapezoid/Simpson difference is within rtol of the local and the average
+// band integral (reference: build_planck_sampler / refine_planck_interval,
+// src/energy_sampler_mod.f90:106-185).  Appends knots to e/f.
+static int planck_knots(double T, double emin, double emax, double rtol,
+                        std::vector<double> &e, std::vector<double> &f)
+{
+    std::vector<double> mand = {emin, emax};
+    for (double t : {MC_ETH_HI, MC_ETH_HEI, MC_ETH_HEII})
+        if (t > emin && t < emax) mand.push_back(t);
+    std::sort(mand.begin(), mand.end());
+    mand.erase(std::unique(mand.begin(), mand.end()), mand.end());
+    double scale = 0.0;
+    for (int i = 1; i <= 256; ++i) {
+        const double a = emin + (emax - emin) * (double)(i - 1) / 256.0;
+        const double b = emin + (emax - emin) * (double)i / 256.0;
+        scale += 0.5 * (planck_shape(a, T) + planck_shape(b, T)) * (b - a);
+    }
+    const double span = emax - emin;
+    e.push_back(mand[0]);
+    f.push_back(planck_shape(mand[0], T));
+    bool overflow = false;
+    std::function<void(double, double, double, double, int)> split =
+        [&](double e0, double e1, double f0, double f1, int depth) {
+            const double em = 0.5 * (e0 + e1), fm = planck_shape(em, T);
+            const double coarse = 0.5 * (f0 + f1) * (e1 - e0);
+            const double fine = 0.25 * (f0 + 2.0 * fm + f1) * (e1 - e0);
+            const double err = std::fabs(fine - coarse) / 3.0;
+            const double allow = rtol * (std::fabs(fine) + scale * (e1 - e0) / span);
+            if (err <= allow || depth >= 50) {
+                if (e.size() >= 200000) { overflow = true; return; }
+                e.push_back(e1);
+                f.push_back(f1);
+                return;
+            }
+            split(e0, em, f0, fm, depth + 1);
+            split(em, e1, fm, f1, depth + 1);
+        };
+    for (size_t i = 0; i + 1 < mand.size(); ++i)
+        split(mand[i], mand[i + 1], planck_shape(mand[i], T), planck_shape(mand[i + 1], T), 0);
+    if (overflow) {
+        mc_set_error("Planck sampler knot limit");
+        return -1;
+    }
+    return 0;
+}
+
+// One sampler per source component, its band luminosity, the component CDF
+// and the packet luminosity (reference: setup_continuous_source_sampler,
+// src/ion_band_mod.f90:275-353; Planck spectra, no FUV extension, so the
+// ionizing fraction is 1 and band_L is the component's own luminosity).
+int mc_setup_samplers(mc_ctx *c)
+{
+    const mc_params &p = c->p;
+    c->ncomp = p.nsource + (p.ext_on ? 1 : 0);
+    if (c->ncomp < 1 || c->ncomp > MC_MAXSRC) {
+        mc_set_error("need 1..16 source components");
+        return -1;
+    }
+    c->multi = c->ncomp >= 2;
+    c->samp_e.clear(); c->samp_f.clear(); c->samp_c.clear();
+    c->samp_off.assign(1, 0);
+    std::vector<double> bandL(c->ncomp);
+    for (int ic = 0; ic < c->ncomp; ++ic) {
+        const int kind = ic < p.nsource ? ic + 1 : 0;
+        c->comp_kind[ic] = kind;
+        const double T = kind ? p.tstar[kind - 1] : p.ext_tstar;
+        if (!(T > 0.0)) {
+            mc_set_error("source temperature must be positive");
+            return -1;
+        }
+        std::vector<double> e, f;
+        if (planck_knots(T, p.eion_min, p.eion_max, p.source_cdf_tol > 0 ? p.source_cdf_tol : 1e-8,
+                         e, f))
+            return -1;
+        std::vector<double> cdf(e.size(), 0.0);
+        for (size_t i = 0; i + 1 < e.size(); ++i)
+            cdf[i + 1] = cdf[i] + 0.5 * (f[i] + f[i + 1]) * (e[i + 1] - e[i]);
+        const double tot = cdf.back();
+        for (auto &v : cdf) v = v / tot;
+        cdf.back() = 1.0;
+        c->samp_e.insert(c->samp_e.end(), e.begin(), e.end());
+        c->samp_f.insert(c->samp_f.end(), f.begin(), f.end());
+        c->samp_c.insert(c->samp_c.end(), cdf.begin(), cdf.end());
+        c->samp_off.push_back((int)c->samp_e.size());
+        if (kind) {
+            bandL[ic] = p.src_lum[kind - 1];
+        } else {
+            double area;
+            if (p.ext_geometry == MC_EXT_SPH) {
+                const double r = p.rmax * p.distance2cm;
+                area = MC_FOURPI * (r * r);
+            } else {
+                const double xr = 2 * p.xmax * p.distance2cm, yr = 2 * p.ymax * p.distance2cm,
+                             zr = 2 * p.zmax * p.distance2cm;
+                area = 2.0 * (xr * yr + yr * zr + zr * xr);
+            }
+            bandL[ic] = MC_PI * p.ext_intensity * area;
+        }
+    }
+    c->Ltot = 0.0;
+    for (double L : bandL) c->Ltot += L;
+    double acc = 0.0;
+    for (int ic = 0; ic < c->ncomp; ++ic) {
+        acc += bandL[ic];
+        c->src_cdf[ic] = acc;
+    }
+    for (int ic = 0; ic < c->ncomp; ++ic) c->src_cdf[ic] /= acc;
+    c->qmc_ndim = (!c->multi && p.nsource == 1) ? 3 : 7;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Octree from a leaf list (reference: amr_build_tree and
+// amr_build_neighbors, src/octree_mod.f90:347-497).  Cells are numbered in
+// creation order (0-based); the neighbour of a face is the cell of the same
+// level (or the coarser leaf) across it, 0-based, -1 outside or when the
+// lookup lands on an ancestor.
+int mc_build_octree_host(mc_ctx *c, int nleaf, const double *xl, const double *yl,
+                         const double *zl, const int32_t *lev, double half)
+{
+    c->car = 0;
+    c->xmin = c->ymin = c->zmin = -half;
+    c->xmaxb = c->ymaxb = c->zmaxb = half;
+    auto &cx = c->hcx, &cy = c->hcy, &cz = c->hcz, &ch = c->hch;
+    auto &il = c->hileaf, &kid = c->hchildren, &lv = c->hlevel, &par = c->hparent;
+    cx.assign(1, 0.0); cy.assign(1, 0.0); cz.assign(1, 0.0); ch.assign(1, half);
+    il.assign(1, -1); kid.assign(8, -1); lv.assign(1, 0); par.assign(1, -1);
+    c->hcell_of_leaf.assign(nleaf, -1);
+    for (int i = 0; i < nleaf; ++i) {
+        int cell = 0;
+        for (int l = 0; l < lev[i]; ++l) {
+            const int o = (xl[i] >= cx[cell] ? 1 : 0) + (yl[i] >= cy[cell] ? 2 : 0) +
+                          (zl[i] >= cz[cell] ? 4 : 0);
+            if (kid[cell * 8 + o] < 0) {
+                const int n = (int)cx.size();
+                const double h = ch[cell] * 0.5;
+                cx.push_back(cx[cell] + (double)(2 * (o & 1) - 1) * h);
+                cy.push_back(cy[cell] + (double)(2 * ((o >> 1) & 1) - 1) * h);
+                cz.push_back(cz[cell] + (double)(2 * (o >> 2) - 1) * h);
+                ch.push_back(h);
+                il.push_back(-1);
+                lv.push_back(l + 1);
+                par.push_back(cell);
+                kid.insert(kid.end(), 8, -1);
+                kid[cell * 8 + o] = n;
+            }
+            cell = kid[cell * 8 + o];
+        }
+        if (il[cell] >= 0) {
+            mc_set_error("two leaves map to one octree cell");
+            return -1;
+        }
+        il[cell] = i;
+        c->hcell_of_leaf[i] = cell;
+    }
+    const int nc = (int)cx.size();
+    for (int i = 0; i < nleaf; ++i)
+        if (c->hcell_of_leaf[i] < 0) {
+            mc_set_error("leaf outside the tree");
+            return -1;
+        }
+    auto at_level = [&](double x, double y, double z, int level) {
+        if (x < -half || x > half || y < -half || y > half || z < -half || z > half) return -1;
+        int cell = 0;
+        for (;;) {
+            if (lv[cell] >= level || il[cell] >= 0) return cell;
+            const int o = (x >= cx[cell] ? 1 : 0) + (y >= cy[cell] ? 2 : 0) + (z >= cz[cell] ? 4 : 0);
+            const int k = kid[cell * 8 + o];
+            if (k < 0) return cell;
+            cell = k;
+        }
+    };
+    auto ancestor = [&](int a, int d) {
+        for (int k = par[d]; k >= 0; k = par[k])
+            if (k == a) return true;
+        return false;
+    };
+    c->hneighbor.assign((size_t)nc * 6, -1);
+    for (int i = 0; i < nc; ++i) {
+        const double x = cx[i], y = cy[i], z = cz[i], hp = 2.0 * ch[i];
+        int *nb = &c->hneighbor[(size_t)i * 6];
+        if (x + hp <= half) nb[0] = at_level(x + hp, y, z, lv[i]);
+        if (x - hp >= -half) nb[1] = at_level(x - hp, y, z, lv[i]);
+        if (y + hp <= half) nb[2] = at_level(x, y + hp, z, lv[i]);
+        if (y - hp >= -half) nb[3] = at_level(x, y - hp, z, lv[i]);
+        if (z + hp <= half) nb[4] = at_level(x, y, z + hp, lv[i]);
+        if (z - hp >= -half) nb[5] = at_level(x, y, z - hp, lv[i]);
+        for (int f = 0; f < 6; ++f)
+            if (nb[f] >= 0 && nb[f] != i && ancestor(nb[f], i)) nb[f] = -1;
+    }
+    c->ncells = nc;
+    c->nleaf = nleaf;
+    return 0;
+}

@@ -1,0 +1,178 @@
+"""Host driver over the C-ABI, mirroring the reference's module API.
+
+Method names follow the reference procedures they stand in for; run_model
+restates the iteration of src/main.f90:124-227 (transport -> rates -> cell
+update -> opacity feedback until converged, then one final consistency pass).
+"""
+import ctypes as C
+import time
+
+import numpy as np
+
+from .lib import FIELDS, MC_GRID_AMR, Params, check, library
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Model:
+    """One device-resident model: grid, leaf state, tallies, rates."""
+
+    def __init__(self, params=None, **kw):
+        L = library()
+        self.params = params if isinstance(params, Params) else Params.make(**(params or {}), **kw)
+        self._h = C.c_void_p()
+        check(L.mc_init(C.byref(self.params), C.byref(self._h)), "mc_init")
+        self._L = L
+        self._refresh_info()
+
+    def _refresh_info(self):
+        n, nnu, lt = C.c_int32(), C.c_int32(), C.c_double()
+        check(self._L.mc_info(self._h, C.byref(n), C.byref(nnu), C.byref(lt), None), "mc_info")
+        self.nleaf, self.nnu_band, self.ion_Ltot = n.value, nnu.value, lt.value
+
+    # ---- set-up --------------------------------------------------------------
+    def grid_octree(self, xleaf, yleaf, zleaf, level, half):
+        """amr_build_tree + amr_build_neighbors from a leaf list."""
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (xleaf, yleaf, zleaf))
+        lev = np.ascontiguousarray(level, dtype=np.int32)
+        check(self._L.mc_grid_octree(self._h, lev.size, _dp(x), _dp(y), _dp(z),
+                                     lev.ctypes.data_as(C.POINTER(C.c_int32)), float(half)),
+              "mc_grid_octree")
+        self._refresh_info()
+
+    def gas_state_setup(self, nH, xHI=None):
+        nH = np.ascontiguousarray(nH, dtype=np.float64)
+        if nH.size != self.nleaf:
+            raise ValueError(f"nH has {nH.size} values, grid has {self.nleaf} leaves")
+        x = None if xHI is None else np.ascontiguousarray(xHI, dtype=np.float64)
+        check(self._L.mc_gas_state_setup(self._h, _dp(nH), None if x is None else _dp(x)),
+              "mc_gas_state_setup")
+
+    def ion_dust_table(self, lam, albedo, gcos, cext):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (lam, albedo, gcos, cext)]
+        check(self._L.mc_ion_dust_table(self._h, arrs[0].size, *[_dp(a) for a in arrs]),
+              "mc_ion_dust_table")
+
+    def set_state(self, **fields):
+        for k, v in fields.items():
+            a = np.ascontiguousarray(v, dtype=np.float64)
+            check(self._L.mc_state_set(self._h, FIELDS[k], _dp(a)), "mc_state_set")
+
+    def state(self, *names):
+        names = names or ("nH", "xHI", "xHeI", "xHeII", "ne", "Te")
+        out = {}
+        for k in names:
+            a = np.empty(self.nleaf)
+            check(self._L.mc_state_get(self._h, FIELDS[k], _dp(a)), "mc_state_get")
+            out[k] = a
+        return out
+
+    def set_precision(self, precision):
+        check(self._L.mc_set_precision(self._h, int(precision)), "mc_set_precision")
+        self.params.precision = int(precision)
+
+    # ---- the hot path ----------------------------------------------------------
+    def ionizing_field_and_rates(self, with_peel=False):
+        check(self._L.mc_ionizing_field_and_rates(self._h, int(bool(with_peel))),
+              "mc_ionizing_field_and_rates")
+
+    def gas_equilibrium_update(self):
+        a, b = C.c_double(), C.c_double()
+        check(self._L.mc_gas_equilibrium_update(self._h, C.byref(a), C.byref(b)),
+              "mc_gas_equilibrium_update")
+        return a.value, b.value
+
+    def hydrogen_balance_residual_grid(self):
+        a, b = C.c_double(), C.c_double()
+        check(self._L.mc_hydrogen_balance_residual_grid(self._h, C.byref(a), C.byref(b)),
+              "mc_hydrogen_balance_residual_grid")
+        return a.value, b.value
+
+    def gas_opacity_fill(self):
+        check(self._L.mc_gas_opacity_fill(self._h), "mc_gas_opacity_fill")
+
+    # ---- results -----------------------------------------------------------------
+    def tallies(self):
+        n, nnu = self.nleaf, self.nnu_band
+        hhe, jt, dust = np.empty((n, 6)), np.empty((n, nnu)), np.empty(n)
+        segs = np.zeros(2, dtype=np.int64)
+        check(self._L.mc_tallies_get(self._h, _dp(hhe), _dp(jt), _dp(dust),
+                                     segs.ctypes.data_as(C.POINTER(C.c_int64))), "mc_tallies_get")
+        return dict(hhe=hhe, jt=jt, dust_abs=dust, seg_direct=int(segs[0]),
+                    seg_scattered=int(segs[1]))
+
+    def rates(self):
+        r = np.empty(6 * self.nleaf)
+        check(self._L.mc_rates_get(self._h, _dp(r)), "mc_rates_get")
+        r = r.reshape(6, self.nleaf)
+        return dict(gHI=r[0], gHeI=r[1], gHeII=r[2], hHI=r[3], hHeI=r[4], hHeII=r[5])
+
+    def energy_ledger(self):
+        a = np.empty(3)
+        check(self._L.mc_energy_ledger(self._h, _dp(a)), "mc_energy_ledger")
+        return a
+
+    def band_edges(self):
+        e = np.empty(self.nnu_band + 1)
+        check(self._L.mc_info(self._h, None, None, None, _dp(e)), "mc_info")
+        return e
+
+    def timing(self):
+        t = np.empty(4)
+        check(self._L.mc_timing(self._h, _dp(t)), "mc_timing")
+        return dict(launch_ms=t[0], walk_ms=t[1], rates_ms=t[2], solve_ms=t[3])
+
+    def trace_ray(self, pos, direction, precision=None, maxn=100000):
+        prec = self.params.precision if precision is None else precision
+        p = np.ascontiguousarray(pos, dtype=np.float64)
+        d = np.ascontiguousarray(direction, dtype=np.float64)
+        cells = np.zeros(maxn, dtype=np.int32)
+        n = C.c_int32()
+        check(self._L.mc_trace_ray(self._h, int(prec), _dp(p), _dp(d),
+                                   cells.ctypes.data_as(C.POINTER(C.c_int32)), maxn, C.byref(n)),
+              "mc_trace_ray")
+        return cells[:n.value].copy()
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.mc_finalize(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_model(model, niter=None, log=None, final_pass=True):
+    """The iteration of src/main.f90:124-227 on a set-up Model.
+
+    Returns a dict with the iteration count, convergence flag, the metrics of
+    the last update and wall-clock times."""
+    p = model.params
+    niter = max(p.gas_niter, 1) if niter is None else niter
+    converged = False
+    hist = []
+    t0 = time.perf_counter()
+    it = 0
+    for it in range(1, niter + 1):
+        model.ionizing_field_and_rates()
+        if p.gas_niter < 1:
+            break
+        if p.solve_te:
+            raise NotImplementedError("thermal update is not built into this version")
+        max_dx, dx_vol = model.gas_equilibrium_update()
+        converged = (dx_vol < p.gas_tol) if p.conv_vol else (max_dx < p.gas_tol)
+        hist.append((it, max_dx, dx_vol))
+        if log:
+            log(f"iteration {it:4d}: max |delta x_HII| = {max_dx:12.4e},  vol = {dx_vol:12.4e}")
+        if converged:
+            break
+    t_loop = time.perf_counter() - t0
+    if final_pass and p.gas_niter >= 1:
+        model.ionizing_field_and_rates()
+    return dict(iterations=it, converged=converged, history=hist, t_loop=t_loop,
+                t_total=time.perf_counter() - t0)

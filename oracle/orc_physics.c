@@ -12,8 +12,9 @@
  * when oracle/_ref is built, the reference photo_xsec and recomb_mod compiled
  * from their own source (bit for bit).
  *
- * Integer powers follow the reference compiler's expansion by repeated
- * squaring ((x*x)*(x*x) for x**4), real powers go through pow().
+ * Integer powers follow the reference compiler's left-to-right expansion
+ * (x**4 = ((x*x)*x)*x, checked bit for bit against the compiled reference),
+ * real powers go through pow().
  */
 #include <math.h>
 #include "mc_oracle.h"
@@ -27,8 +28,7 @@ double orc_sigma_hydrogenic(double E, double Eth, double Z)
     if (E > Eth) {
         double eps = sqrt(E / Eth - 1.0);
         double r = Eth / E;
-        double r2 = r * r;
-        return s0 * (r2 * r2) * exp(4.0 - 4.0 * atan(eps) / eps) /
+        return s0 * (((r * r) * r) * r) * exp(4.0 - 4.0 * atan(eps) / eps) /
                (1.0 - exp(-2.0 * ORC_PI / eps));
     }
     return s0;

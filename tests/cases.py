@@ -1,0 +1,62 @@
+"""Test helpers: build the same model on the oracle and on the HIP library."""
+import numpy as np
+
+from oracle import pyoracle as po
+
+_MAP = {  # library Params name -> oracle param name
+    "nphotons": "nphotons", "qmc_seed": "qmc_seed", "eion_min": "eion_min",
+    "eion_max": "eion_max", "source_cdf_tol": "source_cdf_tol", "nnu_ion": "nnu_ion",
+    "ion_align_edges": "align_edges", "nsource": "nsource", "tstar": "tstar",
+    "src_lum": "lum", "src_x": "sx", "src_y": "sy", "src_z": "sz", "ext_on": "ext_on",
+    "ext_geometry": "ext_geometry", "ext_tstar": "ext_tstar", "ext_intensity": "ext_intensity",
+    "rmax": "rmax", "grid_type": "grid_type", "car_walk_amr": "car_walk_amr",
+    "nx": "nx", "ny": "ny", "nz": "nz", "xmax": "xmax", "ymax": "ymax", "zmax": "zmax",
+    "distance2cm": "distance2cm", "He_abund": "He_abund", "te_fixed": "te_fixed",
+    "ion_relax": "ion_relax", "te_min": "te_min", "te_max": "te_max", "case_A": "caseA",
+    "ion_add_dust": "add_dust", "ion_dust_scatter": "dust_scatter", "lambda_ref": "lambda_ref",
+    "xHI_init": "xHI_init", "xHeI_init": "xHeI_init", "xHeII_init": "xHeII_init",
+}
+_ORC_DEFAULTS = dict(source_cdf_tol=1e-8, align_edges=1, lambda_ref=0.55, te_min=1e3,
+                     te_max=5e4, ext_geometry=2)
+
+
+def oracle_params(params):
+    out = dict(_ORC_DEFAULTS)
+    for k, v in params.items():
+        if k in _MAP:
+            out[_MAP[k]] = v
+    return out
+
+
+def oracle_model(params, nH, rhokap=None, dust_table=None, octree=None):
+    op = oracle_params(params)
+    m = po.Model({k: v for k, v in op.items()
+                  if k not in ("xHI_init", "xHeI_init", "xHeII_init")} | {}, octree=octree,
+                 dust_table=dust_table)
+    m.params.update(op)
+    m.state_init(nH, rhokap=rhokap)
+    return m
+
+
+def gpu_model(params, nH, rhokap=None, dust_table=None, octree=None):
+    import mochii_amd as mc
+    m = mc.Model(dict(params))
+    if octree is not None:
+        m.grid_octree(*octree)
+    if dust_table is not None:
+        m.ion_dust_table(*dust_table)
+    m.gas_state_setup(nH)
+    if rhokap is not None:
+        m.set_state(rhokap=rhokap)
+    return m
+
+
+def rel_close(a, b, rtol, floor_frac=1e-6):
+    """Elementwise |a-b| <= rtol*|b| on entries above floor_frac*max|b|;
+    returns (ok, worst relative error)."""
+    a, b = np.asarray(a), np.asarray(b)
+    big = np.abs(b) > floor_frac * np.abs(b).max() if b.size and np.abs(b).max() > 0 else np.zeros_like(b, bool)
+    if not big.any():
+        return np.allclose(a, b), 0.0
+    err = np.abs(a[big] - b[big]) / np.abs(b[big])
+    return bool(err.max() <= rtol), float(err.max())
