@@ -60,7 +60,7 @@ static int upload_tables(mc_ctx *c)
     std::copy(c->samp_e.begin(), c->samp_e.end(), s.begin());
     std::copy(c->samp_f.begin(), c->samp_f.end(), s.begin() + nk);
     std::copy(c->samp_c.begin(), c->samp_c.end(), s.begin() + 2 * nk);
-    if (c->d_samp) hipFree(c->d_samp);
+    if (c->d_samp) (void)hipFree(c->d_samp);
     if (dalloc(&c->d_samp, 3 * nk)) return -100;
     MC_HIP(hipMemcpy(c->d_samp, s.data(), sizeof(double) * 3 * nk, hipMemcpyHostToDevice));
     return 0;
@@ -72,7 +72,7 @@ static void free_leaf_arrays(mc_ctx *c)
     for (void *p : {(void *)s.nH, (void *)s.xHI, (void *)s.xHeI, (void *)s.xHeII, (void *)s.ne,
                     (void *)s.Te, (void *)s.rhokap, (void *)s.opac, (void *)s.hhe, (void *)s.jt,
                     (void *)s.rates, (void *)s.partials})
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     s.nH = s.xHI = s.xHeI = s.xHeII = s.ne = s.Te = s.rhokap = nullptr;
     s.opac = nullptr;
     s.hhe = s.jt = s.rates = s.partials = nullptr;
@@ -128,7 +128,7 @@ extern "C" int mc_init(const mc_params *p, mc_ctx **out)
         delete c;
         return -100;
     }
-    for (auto &e : c->ev) hipEventCreate(&e);
+    for (auto &e : c->ev) (void)hipEventCreate(&e);
     if (dalloc(&c->d_tab, 1) || upload_tables(c)) {
         mc_finalize(c);
         return -100;
@@ -170,6 +170,7 @@ extern "C" int mc_init(const mc_params *p, mc_ctx **out)
         g.xmin = c->xmin; g.ymin = c->ymin; g.zmin = c->zmin;
         g.xmax = c->xmaxb; g.ymax = c->ymaxb; g.zmax = c->zmaxb;
         g.dcell = c->dcell;
+        g.half = p->xmax;
         if (alloc_leaf_arrays(c)) {
             mc_finalize(c);
             return -100;
@@ -191,13 +192,12 @@ extern "C" int mc_finalize(mc_ctx *c)
                     (void *)c->st.segs, (void *)c->st.counter, (void *)c->pk.x, (void *)c->pk.y,
                     (void *)c->pk.z, (void *)c->pk.kx, (void *)c->pk.ky, (void *)c->pk.kz,
                     (void *)c->pk.E, (void *)c->pk.sig, (void *)c->pk.inu, (void *)c->pk.leaf,
-                    (void *)c->d_cx, (void *)c->d_cy, (void *)c->d_cz, (void *)c->d_ch,
-                    (void *)c->d_ileaf, (void *)c->d_children, (void *)c->d_neighbor,
-                    (void *)c->d_cell_of_leaf, (void *)c->d_leaf_geom})
-        if (p) hipFree(p);
+                    (void *)c->d_node_gd, (void *)c->d_node_gf, (void *)c->d_node_link,
+                    (void *)c->d_node_nbr, (void *)c->d_leaf_node})
+        if (p) (void)hipFree(p);
     for (auto &e : c->ev)
-        if (e) hipEventDestroy(e);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
 }
@@ -215,46 +215,79 @@ extern "C" int mc_grid_octree(mc_ctx *c, int nleaf, const double *xl, const doub
         return -1;
     }
     if (mc_build_octree_host(c, nleaf, xl, yl, zl, lev, half)) return -1;
-    const size_t nc = (size_t)c->ncells;
-    for (void *p : {(void *)c->d_cx, (void *)c->d_cy, (void *)c->d_cz, (void *)c->d_ch,
-                    (void *)c->d_ileaf, (void *)c->d_children, (void *)c->d_neighbor,
-                    (void *)c->d_cell_of_leaf, (void *)c->d_leaf_geom})
-        if (p) hipFree(p);
-    if (dalloc(&c->d_cx, nc) || dalloc(&c->d_cy, nc) || dalloc(&c->d_cz, nc) ||
-        dalloc(&c->d_ch, nc) || dalloc(&c->d_ileaf, nc) || dalloc(&c->d_children, nc * 8) ||
-        dalloc(&c->d_neighbor, nc * 6) || dalloc(&c->d_cell_of_leaf, (size_t)nleaf) ||
-        dalloc(&c->d_leaf_geom, (size_t)nleaf))
-        return -100;
-    MC_HIP(hipMemcpy(c->d_cx, c->hcx.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
-    MC_HIP(hipMemcpy(c->d_cy, c->hcy.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
-    MC_HIP(hipMemcpy(c->d_cz, c->hcz.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
-    MC_HIP(hipMemcpy(c->d_ch, c->hch.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
-    MC_HIP(hipMemcpy(c->d_ileaf, c->hileaf.data(), sizeof(int32_t) * nc, hipMemcpyHostToDevice));
-    MC_HIP(hipMemcpy(c->d_children, c->hchildren.data(), sizeof(int32_t) * nc * 8,
-                     hipMemcpyHostToDevice));
-    MC_HIP(hipMemcpy(c->d_neighbor, c->hneighbor.data(), sizeof(int32_t) * nc * 6,
-                     hipMemcpyHostToDevice));
-    MC_HIP(hipMemcpy(c->d_cell_of_leaf, c->hcell_of_leaf.data(), sizeof(int32_t) * nleaf,
-                     hipMemcpyHostToDevice));
-    std::vector<float4> lg(nleaf);
-    for (int i = 0; i < nleaf; ++i) {
-        const int k = c->hcell_of_leaf[i];
-        lg[i] = make_float4((float)c->hcx[k], (float)c->hcy[k], (float)c->hcz[k], (float)c->hch[k]);
+    // breadth-first renumbering; every internal node must have 8 children
+    const int nc = c->ncells;
+    std::vector<int> bfs;
+    bfs.reserve(nc);
+    bfs.push_back(0);
+    for (size_t q = 0; q < bfs.size(); ++q) {
+        const int k = bfs[q];
+        if (c->hileaf[k] >= 0) continue;
+        for (int o = 0; o < 8; ++o) {
+            const int ch = c->hchildren[(size_t)k * 8 + o];
+            if (ch < 0) {
+                mc_set_error("octree is not space-filling (an internal cell lacks a child)");
+                return -1;
+            }
+            bfs.push_back(ch);
+        }
     }
-    MC_HIP(hipMemcpy(c->d_leaf_geom, lg.data(), sizeof(float4) * nleaf, hipMemcpyHostToDevice));
+    if ((int)bfs.size() != nc) {
+        mc_set_error("octree has unreachable cells");
+        return -1;
+    }
+    std::vector<int> nid(nc);
+    for (int i = 0; i < nc; ++i) nid[bfs[i]] = i;
+    std::vector<double4> gd(nc);
+    std::vector<float4> gf(nc);
+    std::vector<int32_t> link(nc), nbr((size_t)nc * 6), lnode(nleaf);
+    for (int i = 0; i < nc; ++i) {
+        const int k = bfs[i];
+        gd[i] = make_double4(c->hcx[k], c->hcy[k], c->hcz[k], c->hch[k]);
+        gf[i] = make_float4((float)(c->hcx[k] / half), (float)(c->hcy[k] / half),
+                            (float)(c->hcz[k] / half), (float)(c->hch[k] / half));
+        link[i] = c->hileaf[k] >= 0 ? -(c->hileaf[k] + 1) : nid[c->hchildren[(size_t)k * 8]];
+        for (int f = 0; f < 6; ++f) {
+            const int nb = c->hneighbor[(size_t)k * 6 + f];
+            nbr[(size_t)i * 6 + f] = nb >= 0 ? nid[nb] : -1;
+        }
+    }
+    for (int l = 0; l < nleaf; ++l) lnode[l] = nid[c->hcell_of_leaf[l]];
+    // prefix of whole levels that fits the launch kernel's LDS budget
+    int nl = 0;
+    for (int lv = 0, cnt = 1; nl + cnt <= 2048; ++lv, cnt *= 8) {
+        int m = 0;
+        for (int i = 0; i < nc; ++i) m += c->hlevel[bfs[i]] == lv;
+        if (m == 0) break;
+        nl += m;
+        if (m < cnt) break;
+    }
+    c->n_lds_nodes = std::min(nl, nc);
+    for (void *p : {(void *)c->d_node_gd, (void *)c->d_node_gf, (void *)c->d_node_link,
+                    (void *)c->d_node_nbr, (void *)c->d_leaf_node})
+        if (p) (void)hipFree(p);
+    if (dalloc(&c->d_node_gd, nc) || dalloc(&c->d_node_gf, nc) || dalloc(&c->d_node_link, nc) ||
+        dalloc(&c->d_node_nbr, (size_t)nc * 6) || dalloc(&c->d_leaf_node, (size_t)nleaf))
+        return -100;
+    MC_HIP(hipMemcpy(c->d_node_gd, gd.data(), sizeof(double4) * nc, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(c->d_node_gf, gf.data(), sizeof(float4) * nc, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(c->d_node_link, link.data(), sizeof(int32_t) * nc, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(c->d_node_nbr, nbr.data(), sizeof(int32_t) * nc * 6, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(c->d_leaf_node, lnode.data(), sizeof(int32_t) * nleaf, hipMemcpyHostToDevice));
     DevGrid &g = c->grid;
     g.car = 0;
     g.car_walk_amr = 0;
     g.nleaf = nleaf;
+    g.nnode = nc;
     g.xmin = c->xmin; g.ymin = c->ymin; g.zmin = c->zmin;
     g.xmax = c->xmaxb; g.ymax = c->ymaxb; g.zmax = c->zmaxb;
     g.dcell = 0.0;
-    g.cx = c->d_cx; g.cy = c->d_cy; g.cz = c->d_cz; g.ch = c->d_ch;
-    g.ileaf = c->d_ileaf;
-    g.children = c->d_children;
-    g.neighbor = c->d_neighbor;
-    g.cell_of_leaf = c->d_cell_of_leaf;
-    g.leaf_geom = c->d_leaf_geom;
+    g.half = half;
+    g.node_gd = c->d_node_gd;
+    g.node_gf = c->d_node_gf;
+    g.node_link = c->d_node_link;
+    g.node_nbr = c->d_node_nbr;
+    g.leaf_node = c->d_leaf_node;
     return alloc_leaf_arrays(c);
 }
 
@@ -359,7 +392,7 @@ extern "C" int mc_ion_dust_table(mc_ctx *c, int n, const double *lam, const doub
     std::copy(c->dalb.begin(), c->dalb.end(), h.begin() + n);
     std::copy(c->dg.begin(), c->dg.end(), h.begin() + 2 * n);
     std::copy(c->dcext.begin(), c->dcext.end(), h.begin() + 3 * n);
-    if (c->d_dust) hipFree(c->d_dust);
+    if (c->d_dust) (void)hipFree(c->d_dust);
     if (dalloc(&c->d_dust, 4 * (size_t)n)) return -100;
     MC_HIP(hipMemcpy(c->d_dust, h.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice));
     c->dn = n;

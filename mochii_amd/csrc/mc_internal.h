@@ -37,19 +37,19 @@ struct DevTables {
     double   rmax;
 };
 
-// Grid description passed by value to kernels.
+// Grid description passed by value to kernels.  The octree node table is in
+// breadth-first order: the 8 children of a node are contiguous and the top
+// levels form a prefix of the table.
 struct DevGrid {
     int32_t car;                      // 1 = uniform Cartesian (DDA), 0 = octree
-    int32_t car_walk_amr;             // car grid walked with the octree face-exit walk
-    int32_t nx, ny, nz, nleaf;
-    double  xmin, ymin, zmin, xmax, ymax, zmax, dcell;
-    // octree (SoA, 0-based cell and leaf indices, -1 = none)
-    const double  *cx, *cy, *cz, *ch;       // ncells
-    const int32_t *ileaf;                   // ncells: leaf or -1
-    const int32_t *children;                // ncells*8 or -1
-    const int32_t *neighbor;                // ncells*6 or -1
-    const int32_t *cell_of_leaf;            // nleaf
-    const float4  *leaf_geom;               // nleaf: centre(3) + half width (FP32)
+    int32_t car_walk_amr;             // car grid walked with the face-exit walk
+    int32_t nx, ny, nz, nleaf, nnode;
+    double  xmin, ymin, zmin, xmax, ymax, zmax, dcell, half;
+    const double4 *node_gd;           // centre + half width (code units, FP64)
+    const float4  *node_gf;           // centre + half width / half (FP32)
+    const int32_t *node_link;         // >= 0: first child; < 0: -(leaf + 1)
+    const int32_t *node_nbr;          // 6 face neighbours per node, -1 outside
+    const int32_t *leaf_node;         // leaf -> node
 };
 
 // Packet launch records (SoA), written by the launch kernel.
@@ -103,9 +103,10 @@ struct mc_ctx {
     double *d_dust;          // 4 * dn (lam, alb, g, cext)
     int dn;
     DevGrid grid;
-    double *d_cx, *d_cy, *d_cz, *d_ch;
-    int32_t *d_ileaf, *d_children, *d_neighbor, *d_cell_of_leaf;
-    float4 *d_leaf_geom;
+    double4 *d_node_gd;
+    float4 *d_node_gf;
+    int32_t *d_node_link, *d_node_nbr, *d_leaf_node;
+    int n_lds_nodes;          // breadth-first prefix staged in LDS
     DevState st;
     PacketBuf pk;
     int nblocks_part;

@@ -145,24 +145,34 @@ __device__ __forceinline__ int car_index(const DevGrid &g, double x, double y, d
     return ix + g.nx * (iy + g.ny * iz);
 }
 
-// leaf containing (x,y,z), -1 outside (octree descent or raster index)
-__device__ int find_leaf(const DevGrid &g, double x, double y, double z)
+// leaf containing (x,y,z), -1 outside: raster index on a car grid, root-down
+// descent on the octree (amr_find_leaf), reading the first nl nodes from the
+// LDS copies when given
+__device__ int find_leaf(const DevGrid &g, double x, double y, double z,
+                         const double4 *sgd = nullptr, const int32_t *slink = nullptr, int nl = 0)
 {
     if (x < g.xmin || x > g.xmax || y < g.ymin || y > g.ymax || z < g.zmin || z > g.zmax)
         return -1;
     if (g.car) return car_index(g, x, y, z);
     int c = 0;
     for (;;) {
-        const int l = g.ileaf[c];
-        if (l >= 0) return l;
-        const int o = (x >= g.cx[c] ? 1 : 0) + (y >= g.cy[c] ? 2 : 0) + (z >= g.cz[c] ? 4 : 0);
-        c = g.children[c * 8 + o];
-        if (c < 0) return -1;
+        const int link = c < nl ? slink[c] : g.node_link[c];
+        if (link < 0) return -link - 1;
+        const double4 n = c < nl ? sgd[c] : g.node_gd[c];
+        c = link + (x >= n.x ? 1 : 0) + (y >= n.y ? 2 : 0) + (z >= n.z ? 4 : 0);
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_launch(KArgs a)
+__global__ __launch_bounds__(kBlock) void k_launch(KArgs a, int nlds)
 {
+    extern __shared__ unsigned char lds_raw[];
+    double4 *sgd = reinterpret_cast<double4 *>(lds_raw);
+    int32_t *slink = reinterpret_cast<int32_t *>(sgd + nlds);
+    for (int k = threadIdx.x; k < nlds; k += kBlock) {
+        sgd[k] = a.g.node_gd[k];
+        slink[k] = a.g.node_link[k];
+    }
+    __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.npk) return;
     const DevTables *T = a.tab;
@@ -265,7 +275,7 @@ __global__ __launch_bounds__(kBlock) void k_launch(KArgs a)
     sg[1] = sigma_HeI(E);
     sg[2] = sigma_hydrogenic(E, MC_ETH_HEII, 2.0);
     sg[3] = sabs; sg[4] = ssca; sg[5] = gg;
-    P.leaf[i] = find_leaf(a.g, x, y, z);
+    P.leaf[i] = find_leaf(a.g, x, y, z, sgd, slink, nlds);
 }
 
 // ---------------------------------------------------------------------------
@@ -302,9 +312,14 @@ struct Lane {
     // walk state, FP64 form: DDA
     int ix, iy, iz, sx, sy, sz;
     double tm[3], td[3], tcur, expo, tau, tau_in;
-    // walk state, mixed form: cell-local position in [0,1]^3
+    // walk state, mixed form: cell-local position in [0,1]^3 (car) or
+    // [-1,1]^3 in units of the half width (octree)
     float px, py, pz, fkx, fky, fkz, fsHI, fsHeI, fsHeII, fdust;
     float fexpo, ftau, ftau_in, ftcur;
+    // octree walk: current node, current position (FP64 form), node geometry
+    int node, leaf0;
+    double ox, oy, oz;
+    float4 gf;
 };
 
 template <int P> struct Prec;
@@ -453,6 +468,161 @@ __device__ __forceinline__ bool cross_f(const DevGrid &g, Lane &L, int ax, float
     L.iz += L.sz;
     L.pz = L.sz > 0 ? 0.0f : 1.0f;
     return L.iz >= 0 && L.iz < g.nz;
+}
+
+
+// ---- face-exit walk (octree, or car with the octree walk) ----------------
+__device__ __forceinline__ double4 node_geom_d(const DevGrid &g, int node)
+{
+    if (g.car) {
+        const int nxy = g.nx * g.ny, iz = node / nxy, rem = node - iz * nxy;
+        const int iy = rem / g.nx, ix = rem - iy * g.nx;
+        return make_double4(g.xmin + ((double)ix + 0.5) * g.dcell,
+                            g.ymin + ((double)iy + 0.5) * g.dcell,
+                            g.zmin + ((double)iz + 0.5) * g.dcell, 0.5 * g.dcell);
+    }
+    return g.node_gd[node];
+}
+
+__device__ __forceinline__ float4 node_geom_f(const DevGrid &g, int node)
+{
+    if (g.car) {
+        const double4 d = node_geom_d(g, node);
+        const double s = 1.0 / g.half;
+        return make_float4((float)(d.x * s), (float)(d.y * s), (float)(d.z * s), (float)(d.w * s));
+    }
+    return g.node_gf[node];
+}
+
+__device__ __forceinline__ int leaf_of(const DevGrid &g, int node)
+{
+    return g.car ? node : -g.node_link[node] - 1;
+}
+
+__device__ __forceinline__ int leaf_node_of(const DevGrid &g, int leaf)
+{
+    return g.car ? leaf : g.leaf_node[leaf];
+}
+
+// raster face neighbour (car_next), -1 outside
+__device__ __forceinline__ int car_next_node(const DevGrid &g, int node, int f)
+{
+    const int nxy = g.nx * g.ny, iz = node / nxy, rem = node - iz * nxy;
+    int iy = rem / g.nx, ix = rem - iy * g.nx, jz = iz;
+    ix += f == 0 ? 1 : (f == 1 ? -1 : 0);
+    iy += f == 2 ? 1 : (f == 3 ? -1 : 0);
+    jz += f == 4 ? 1 : (f == 5 ? -1 : 0);
+    if (ix < 0 || ix >= g.nx || iy < 0 || iy >= g.ny || jz < 0 || jz >= g.nz) return -1;
+    return ix + g.nx * (iy + g.ny * jz);
+}
+
+// amr_cell_exit: distance to the first face crossed, first minimum on ties in
+// the face order +x,-x,+y,-y,+z,-z (f = 0..5)
+__device__ __forceinline__ void cell_exit_d(const double4 G, double x, double y, double z,
+                                            double kx, double ky, double kz, double &t, int &f)
+{
+#pragma clang fp contract(off)
+    const double tx = kx > 0.0 ? (G.x + G.w - x) / kx : (kx < 0.0 ? (G.x - G.w - x) / kx : DBL_MAX);
+    const double ty = ky > 0.0 ? (G.y + G.w - y) / ky : (ky < 0.0 ? (G.y - G.w - y) / ky : DBL_MAX);
+    const double tz = kz > 0.0 ? (G.z + G.w - z) / kz : (kz < 0.0 ? (G.z - G.w - z) / kz : DBL_MAX);
+    // first minimum over the faces in order; a zero component leaves both of
+    // its faces at DBL_MAX, so with every component zero the +x face wins
+    t = tx;
+    f = kx < 0.0 ? 1 : 0;
+    if (ty < t) { t = ty; f = ky < 0.0 ? 3 : 2; }
+    if (tz < t) { t = tz; f = kz < 0.0 ? 5 : 4; }
+}
+
+// amr_next_leaf: neighbour across face f, then down to the leaf holding the
+// exit point; the crossed axis' octant bit is set by topology
+__device__ __forceinline__ int next_node_d(const DevGrid &g, int node, int f, double x, double y,
+                                           double z)
+{
+    if (g.car) return car_next_node(g, node, f);
+    int ng = g.node_nbr[node * 6 + f];
+    if (ng < 0) return -1;
+    for (;;) {
+        const int link = g.node_link[ng];
+        if (link < 0) return ng;
+        const double4 c = g.node_gd[ng];
+        const int bx = f < 2 ? (f == 1) : (x >= c.x);
+        const int by = (f >> 1) == 1 ? (f == 3) : (y >= c.y);
+        const int bz = f >= 4 ? (f == 5) : (z >= c.z);
+        ng = link + bx + 2 * by + 4 * bz;
+    }
+}
+
+// mixed form: exit of the local cube [-1,1]^3 (units of the half width)
+__device__ __forceinline__ float exit_oct_f(const Lane &L, int &ax)
+{
+    const float inf = __builtin_huge_valf();
+    const float tx = L.sx > 0 ? (1.0f - L.px) / L.fkx : (L.sx < 0 ? (-1.0f - L.px) / L.fkx : inf);
+    const float ty = L.sy > 0 ? (1.0f - L.py) / L.fky : (L.sy < 0 ? (-1.0f - L.py) / L.fky : inf);
+    const float tz = L.sz > 0 ? (1.0f - L.pz) / L.fkz : (L.sz < 0 ? (-1.0f - L.pz) / L.fkz : inf);
+    ax = 0;
+    float t = tx;
+    if (ty < t) { t = ty; ax = 1; }
+    if (tz < t) { t = tz; ax = 2; }
+    return fmaxf(t, 0.0f);
+}
+
+__device__ __forceinline__ void move_oct_f(Lane &L, float t)
+{
+    L.px = fminf(fmaxf(L.px + t * L.fkx, -1.0f), 1.0f);
+    L.py = fminf(fmaxf(L.py + t * L.fky, -1.0f), 1.0f);
+    L.pz = fminf(fmaxf(L.pz + t * L.fkz, -1.0f), 1.0f);
+}
+
+// cross the exit face on axis ax; returns false when the ray left the box
+__device__ __forceinline__ bool cross_oct_f(const DevGrid &g, Lane &L, int ax, float t)
+{
+    move_oct_f(L, t);
+    const int s = ax == 0 ? L.sx : (ax == 1 ? L.sy : L.sz);
+    const int f = 2 * ax + (s > 0 ? 0 : 1);
+    const float4 G = L.gf;
+    int ng = g.car ? car_next_node(g, L.node, f) : g.node_nbr[L.node * 6 + f];
+    if (ng < 0) return false;
+    if (!g.car) {
+        for (;;) {
+            const int link = g.node_link[ng];
+            if (link < 0) break;
+            const float4 C = g.node_gf[ng];
+            const int bx = ax == 0 ? (s < 0) : ((G.x - C.x) + L.px * G.w >= 0.0f);
+            const int by = ax == 1 ? (s < 0) : ((G.y - C.y) + L.py * G.w >= 0.0f);
+            const int bz = ax == 2 ? (s < 0) : ((G.z - C.z) + L.pz * G.w >= 0.0f);
+            ng = link + bx + 2 * by + 4 * bz;
+        }
+    }
+    const float4 N = node_geom_f(g, ng);
+    const float r = 1.0f / N.w;
+    L.px = fminf(fmaxf(((G.x - N.x) + L.px * G.w) * r, -1.0f), 1.0f);
+    L.py = fminf(fmaxf(((G.y - N.y) + L.py * G.w) * r, -1.0f), 1.0f);
+    L.pz = fminf(fmaxf(((G.z - N.z) + L.pz * G.w) * r, -1.0f), 1.0f);
+    if (ax == 0) L.px = s > 0 ? -1.0f : 1.0f;
+    else if (ax == 1) L.py = s > 0 ? -1.0f : 1.0f;
+    else L.pz = s > 0 ? -1.0f : 1.0f;
+    L.node = ng;
+    L.leaf = leaf_of(g, ng);
+    L.gf = N;
+    return true;
+}
+
+__device__ __forceinline__ void set_dir_f(Lane &L)
+{
+    L.fkx = (float)L.kx; L.fky = (float)L.ky; L.fkz = (float)L.kz;
+    L.sx = L.fkx > 0.0f ? 1 : (L.fkx < 0.0f ? -1 : 0);
+    L.sy = L.fky > 0.0f ? 1 : (L.fky < 0.0f ? -1 : 0);
+    L.sz = L.fkz > 0.0f ? 1 : (L.fkz < 0.0f ? -1 : 0);
+}
+
+// mixed form: local coordinates of (x,y,z) in the given node
+__device__ __forceinline__ void local_of(const DevGrid &g, Lane &L, double x, double y, double z)
+{
+    const double4 D = node_geom_d(g, L.node);
+    L.gf = node_geom_f(g, L.node);
+    L.px = fminf(fmaxf((float)((x - D.x) / D.w), -1.0f), 1.0f);
+    L.py = fminf(fmaxf((float)((y - D.y) / D.w), -1.0f), 1.0f);
+    L.pz = fminf(fmaxf((float)((z - D.z) / D.w), -1.0f), 1.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -707,7 +877,7 @@ __global__ void k_zero_counters(DevState s)
 // leaf half width (car: constant)
 __device__ __forceinline__ double leaf_half(const DevGrid &g, int il)
 {
-    return g.car ? 0.5 * g.dcell : g.ch[g.cell_of_leaf[il]];
+    return g.car ? 0.5 * g.dcell : g.node_gd[g.leaf_node[il]].w;
 }
 
 // gas_rates_compute: rate sums / (leaf volume * d2cm^2)
@@ -980,7 +1150,9 @@ int mc_run_pass(mc_ctx *c)
         a.first_id = first;
         a.npk = std::min<int64_t>(c->pk.cap, N - first + 1);
         MC_HIP(hipEventRecord(c->ev[1], st));
-        k_launch<<<(unsigned)((a.npk + kBlock - 1) / kBlock), kBlock, 0, st>>>(a);
+        const int nlds = c->car ? 0 : c->n_lds_nodes;
+        k_launch<<<(unsigned)((a.npk + kBlock - 1) / kBlock), kBlock,
+                   (size_t)nlds * (sizeof(double4) + sizeof(int32_t)), st>>>(a, nlds);
         MC_HIP(hipGetLastError());
         k_reset_counter<<<1, 1, 0, st>>>(c->st.counter);
         MC_HIP(hipEventRecord(c->ev[2], st));
