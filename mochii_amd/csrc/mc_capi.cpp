@@ -66,9 +66,60 @@ static int upload_tables(mc_ctx *c)
     return 0;
 }
 
+// Hot leaves: leaves whose centre lies within kHotRadius leaf widths of a
+// point source.  Every packet of that source crosses them, so their tally
+// updates are spread over kHotReplicas copies during the walk.
+static const double kHotRadius = 6.0;
+static const int kHotReplicas = 16;
+static const int kMaxHot = 1 << 16;
+
+static int setup_hot(mc_ctx *c)
+{
+    DevState &s = c->st;
+    std::vector<int32_t> slot(c->nleaf, -1), leaves;
+    for (int il = 0; il < c->nleaf && (int)leaves.size() < kMaxHot; ++il) {
+        double x, y, z, h;
+        if (c->car) {
+            const int nxy = c->p.nx * c->p.ny, iz = il / nxy, rem = il - iz * nxy;
+            const int iy = rem / c->p.nx, ix = rem - iy * c->p.nx;
+            x = c->xmin + (ix + 0.5) * c->dcell;
+            y = c->ymin + (iy + 0.5) * c->dcell;
+            z = c->zmin + (iz + 0.5) * c->dcell;
+            h = 0.5 * c->dcell;
+        } else {
+            const int k = c->hcell_of_leaf[il];
+            x = c->hcx[k]; y = c->hcy[k]; z = c->hcz[k]; h = c->hch[k];
+        }
+        for (int is = 0; is < c->p.nsource; ++is) {
+            const double dx = x - c->p.src_x[is], dy = y - c->p.src_y[is], dz = z - c->p.src_z[is];
+            const double r = kHotRadius * 2.0 * h;
+            if (dx * dx + dy * dy + dz * dz < r * r) {
+                slot[il] = (int32_t)leaves.size();
+                leaves.push_back(il);
+                break;
+            }
+        }
+    }
+    s.nhot = (int32_t)leaves.size();
+    s.nrep = kHotReplicas;
+    if (s.nhot == 0) return 0;
+    if (dalloc(&s.hot_slot, (size_t)c->nleaf) || dalloc(&s.hot_leaf, (size_t)s.nhot) ||
+        dalloc(&s.hot_hhe, (size_t)s.nrep * s.nhot * 8) ||
+        dalloc(&s.hot_jt, (size_t)s.nrep * s.nhot * c->nnu))
+        return -100;
+    MC_HIP(hipMemcpy(s.hot_slot, slot.data(), sizeof(int32_t) * c->nleaf, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(s.hot_leaf, leaves.data(), sizeof(int32_t) * s.nhot, hipMemcpyHostToDevice));
+    return 0;
+}
+
 static void free_leaf_arrays(mc_ctx *c)
 {
     DevState &s = c->st;
+    for (void *p : {(void *)s.hot_slot, (void *)s.hot_leaf, (void *)s.hot_hhe, (void *)s.hot_jt})
+        if (p) (void)hipFree(p);
+    s.hot_slot = s.hot_leaf = nullptr;
+    s.hot_hhe = s.hot_jt = nullptr;
+    s.nhot = 0;
     for (void *p : {(void *)s.nH, (void *)s.xHI, (void *)s.xHeI, (void *)s.xHeII, (void *)s.ne,
                     (void *)s.Te, (void *)s.rhokap, (void *)s.opac, (void *)s.hhe, (void *)s.jt,
                     (void *)s.rates, (void *)s.partials})
@@ -94,7 +145,7 @@ static int alloc_leaf_arrays(mc_ctx *c)
     MC_HIP(hipMemset(s.rates, 0, sizeof(double) * n * 6));
     c->grid.nleaf = c->nleaf;
     c->state_ready = false;
-    return 0;
+    return setup_hot(c);
 }
 
 extern "C" int mc_init(const mc_params *p, mc_ctx **out)

@@ -73,6 +73,14 @@ struct DevState {
     unsigned long long *segs;   // 2
     unsigned long long *counter;// work counter for persistent kernels
     double *partials;  // per-block metric partials
+    // leaves next to the point sources take many updates per pass; their
+    // tallies are spread over nrep replicas (folded after the walk) so that
+    // concurrent atomics hit different addresses
+    int32_t *hot_slot; // nleaf: slot in the hot tables or -1
+    int32_t *hot_leaf; // nhot: leaf of each slot
+    double *hot_hhe;   // nrep * nhot * 8
+    double *hot_jt;    // nrep * nhot * nnu
+    int32_t nhot, nrep;
 };
 
 struct mc_ctx {

@@ -351,16 +351,30 @@ struct Counters {
     unsigned long long sd, ss;
 };
 
-// score_ion_path: the packet's path quantity into its leaf
+// score_ion_path: the packet's path quantity into its leaf.  A coefficient
+// that is exactly zero (He I / He II terms below their thresholds) adds +0 to
+// the tally, so its atomic is skipped; hot leaves go to the lane's replica.
 __device__ __forceinline__ void score(const KArgs &a, const Lane &L, int il, double pl, Counters &cn)
 {
     if (pl == 0.0) return;
     const int nnu = a.tab->nnu;
-    unsafeAtomicAdd(a.s.jt + (size_t)il * nnu + L.inu, pl);
-    double *h = a.s.hhe + (size_t)il * 8;
+    const DevState &s = a.s;
+    const int hs = s.nhot > 0 ? s.hot_slot[il] : -1;
+    double *h, *j;
+    if (hs < 0) {
+        h = s.hhe + (size_t)il * 8;
+        j = s.jt + (size_t)il * nnu;
+    } else {
+        const int rep = (int)(((blockIdx.x * kBlock + threadIdx.x) * 0x9E3779B1u) >> 16) % s.nrep;
+        const size_t slot = (size_t)rep * s.nhot + hs;
+        h = s.hot_hhe + slot * 8;
+        j = s.hot_jt + slot * nnu;
+    }
+    unsafeAtomicAdd(j + L.inu, pl);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) unsafeAtomicAdd(h + k, pl * L.coeff[k]);
-    if (a.add_dust) unsafeAtomicAdd(h + 6, pl * L.dabs);
+    for (int k = 0; k < 6; ++k)
+        if (L.coeff[k] != 0.0) unsafeAtomicAdd(h + k, pl * L.coeff[k]);
+    if (a.add_dust && L.dabs != 0.0) unsafeAtomicAdd(h + 6, pl * L.dabs);
     if (L.nscatt > 0) cn.ss++; else cn.sd++;
 }
 
@@ -874,6 +888,26 @@ __global__ void k_zero_counters(DevState s)
     if (threadIdx.x == 0) *s.counter = 0ull;
 }
 
+// sum the replicas of each hot leaf (fixed order) into the main tallies; the
+// main tallies of hot leaves received no direct updates during the walk
+__global__ __launch_bounds__(kBlock) void k_fold_hot(DevState s, int nnu)
+{
+    const int stride = 8 + nnu;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= (int64_t)s.nhot * stride) return;
+    const int hs = (int)(i / stride), k = (int)(i % stride);
+    const int il = s.hot_leaf[hs];
+    double acc = 0.0;
+    if (k < 8) {
+        for (int r = 0; r < s.nrep; ++r) acc += s.hot_hhe[((size_t)r * s.nhot + hs) * 8 + k];
+        s.hhe[(size_t)il * 8 + k] = acc;
+    } else {
+        const int b = k - 8;
+        for (int r = 0; r < s.nrep; ++r) acc += s.hot_jt[((size_t)r * s.nhot + hs) * nnu + b];
+        s.jt[(size_t)il * nnu + b] = acc;
+    }
+}
+
 // leaf half width (car: constant)
 __device__ __forceinline__ double leaf_half(const DevGrid &g, int il)
 {
@@ -1137,6 +1171,10 @@ int mc_run_pass(mc_ctx *c)
     k_zero<<<1024, kBlock, 0, st>>>(c->st.hhe, nl * 8);
     k_zero<<<1024, kBlock, 0, st>>>(c->st.jt, nl * c->nnu);
     k_zero_counters<<<1, 64, 0, st>>>(c->st);
+    if (c->st.nhot > 0) {
+        k_zero<<<256, kBlock, 0, st>>>(c->st.hot_hhe, (size_t)c->st.nrep * c->st.nhot * 8);
+        k_zero<<<256, kBlock, 0, st>>>(c->st.hot_jt, (size_t)c->st.nrep * c->st.nhot * c->nnu);
+    }
     float t_launch = 0.0f, t_walk = 0.0f;
     const int64_t N = c->p.nphotons;
     const void *kw = c->p.precision == MC_MIXED ? (const void *)k_walk_car<MC_MIXED>
@@ -1168,6 +1206,11 @@ int mc_run_pass(mc_ctx *c)
         t_launch += ms;
         hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
         t_walk += ms;
+    }
+    if (c->st.nhot > 0) {
+        const int64_t nf = (int64_t)c->st.nhot * (8 + c->nnu);
+        k_fold_hot<<<(unsigned)((nf + kBlock - 1) / kBlock), kBlock, 0, st>>>(c->st, c->nnu);
+        MC_HIP(hipGetLastError());
     }
     double led[3];
     unsigned long long sg[2];
