@@ -640,16 +640,23 @@ __device__ __forceinline__ void local_of(const DevGrid &g, Lane &L, double x, do
 }
 
 // ---------------------------------------------------------------------------
-template <int P>
+// G selects the walk: kCarDDA (incremental DDA on a uniform grid) or kFaceExit
+// (face-exit walk over the octree node table, or over car cells when the
+// octree walk is requested on a car grid)
+constexpr int kCarDDA = 0, kFaceExit = 1;
+
+// start a tau-target flight from the current position and direction
+template <int P, int G>
 __device__ __forceinline__ void start_flight(const KArgs &a, Lane &L, double tau_in)
 {
     L.wl = L.wgt * a.tab->Lpacket;
     if constexpr (Prec<P>::mixed) {
-        dda_init_f(a.g, L);
+        if constexpr (G == kCarDDA) dda_init_f(a.g, L);
+        else set_dir_f(L);                 // position stays local to the node
         L.ftau = 0.0f;
         L.ftau_in = (float)tau_in;
     } else {
-        dda_init_d(a.g, L);
+        if constexpr (G == kCarDDA) dda_init_d(a.g, L);
         L.tau = 0.0;
         L.tau_in = tau_in;
     }
@@ -658,7 +665,7 @@ __device__ __forceinline__ void start_flight(const KArgs &a, Lane &L, double tau
 
 // scattering event at the end of a flight that stayed inside
 // (transport_ion_packet loop body, FP64 in both forms: events are rare)
-template <int P>
+template <int P, int G>
 __device__ __forceinline__ void scatter_event(const KArgs &a, Lane &L)
 {
     const int il = L.leaf;
@@ -698,11 +705,13 @@ __device__ __forceinline__ void scatter_event(const KArgs &a, Lane &L)
     L.kx = nx / nrm; L.ky = ny / nrm; L.kz = nz / nrm;
     L.nscatt++;
     const double r = rng_u(a.seed, L.id, L.nk++);
-    start_flight<P>(a, L, -log(fmax(r, DBL_MIN)));
+    start_flight<P, G>(a, L, -log(fmax(r, DBL_MIN)));
 }
 
 // end of the direct walk: energy ledger, then the forced first interaction
-template <int P>
+// (the direct walk works on a copy of the packet: the forced flight starts
+// again from the launch point, src/raytrace_amr.f90:259-269)
+template <int P, int G>
 __device__ __forceinline__ void end_edge(const KArgs &a, Lane &L, double tau_edge, Counters &cn)
 {
     const double lum = a.tab->Lpacket * L.wgt;
@@ -714,10 +723,16 @@ __device__ __forceinline__ void end_edge(const KArgs &a, Lane &L, double tau_edg
     const double w1 = 1.0 - exp(-tau_edge);
     L.wgt = L.wgt * w1;
     const double tau = -log(1.0 - rng_u(a.seed, L.id, L.nk++) * w1);
-    start_flight<P>(a, L, tau);
+    if constexpr (G == kFaceExit) {
+        L.leaf = L.leaf0;
+        L.node = leaf_node_of(a.g, L.leaf0);
+        if constexpr (Prec<P>::mixed) local_of(a.g, L, L.x, L.y, L.z);
+        else { L.ox = L.x; L.oy = L.y; L.oz = L.z; }
+    }
+    start_flight<P, G>(a, L, tau);
 }
 
-// one segment of the current walk (edge walk or tau flight), car grid
+// one segment of the current walk (edge walk or tau flight), car DDA
 template <int P>
 __device__ __forceinline__ void car_step(const KArgs &a, Lane &L, Counters &cn)
 {
@@ -739,7 +754,8 @@ __device__ __forceinline__ void car_step(const KArgs &a, Lane &L, Counters &cn)
             }
             score(a, L, il, pl, cn);
             L.tau = L.tau + kap * seg;
-            if (L.tau >= MC_TAU_HUGE || !dda_advance_d(g, L, ax)) end_edge<P>(a, L, L.tau, cn);
+            if (L.tau >= MC_TAU_HUGE || !dda_advance_d(g, L, ax))
+                end_edge<P, kCarDDA>(a, L, L.tau, cn);
         } else {
             if (L.tau + seg * kap >= L.tau_in) {
                 const double ds = kap > 0.0 ? (L.tau_in - L.tau) / kap : seg;
@@ -747,7 +763,7 @@ __device__ __forceinline__ void car_step(const KArgs &a, Lane &L, Counters &cn)
                 L.tcur = L.tcur + ds;
                 L.x = L.x + L.tcur * L.kx; L.y = L.y + L.tcur * L.ky; L.z = L.z + L.tcur * L.kz;
                 L.leaf = il;
-                scatter_event<P>(a, L);
+                scatter_event<P, kCarDDA>(a, L);
                 return;
             }
             if (L.nscatt > 0) score(a, L, il, L.wl * seg, cn);
@@ -772,7 +788,7 @@ __device__ __forceinline__ void car_step(const KArgs &a, Lane &L, Counters &cn)
             score(a, L, il, pl, cn);
             L.ftau = L.ftau + dt;
             if (L.ftau >= (float)MC_TAU_HUGE || !cross_f(g, L, ax, t))
-                end_edge<P>(a, L, (double)L.ftau, cn);
+                end_edge<P, kCarDDA>(a, L, (double)L.ftau, cn);
         } else {
             if (L.ftau + dt >= L.ftau_in) {
                 const float ds = kc > 0.0f ? (L.ftau_in - L.ftau) / kc : t;
@@ -782,7 +798,7 @@ __device__ __forceinline__ void car_step(const KArgs &a, Lane &L, Counters &cn)
                 L.y = g.ymin + ((double)L.iy + (double)L.py) * g.dcell;
                 L.z = g.zmin + ((double)L.iz + (double)L.pz) * g.dcell;
                 L.leaf = il;
-                scatter_event<P>(a, L);
+                scatter_event<P, kCarDDA>(a, L);
                 return;
             }
             if (L.nscatt > 0) score(a, L, il, wlc * (double)t, cn);
@@ -792,8 +808,90 @@ __device__ __forceinline__ void car_step(const KArgs &a, Lane &L, Counters &cn)
     }
 }
 
+// one segment of the face-exit walk (raytrace_ion_to_edge_amr /
+// raytrace_ion_to_tau_amr, src/raytrace_amr.f90:348-486)
 template <int P>
-__device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r)
+__device__ __forceinline__ void oct_step(const KArgs &a, Lane &L, Counters &cn)
+{
+    const DevGrid &g = a.g;
+    const int il = L.leaf;
+    if constexpr (!Prec<P>::mixed) {
+#pragma clang fp contract(off)
+        const double4 G = node_geom_d(g, L.node);
+        double t;
+        int f;
+        cell_exit_d(G, L.ox, L.oy, L.oz, L.kx, L.ky, L.kz, t, f);
+        const double kap = opacity_d(a, L, il);
+        if (L.phase == 1) {
+            double pl;
+            if (kap * t > 0.0) {
+                const double eo = L.expo * exp(-kap * t);
+                pl = L.wl * (L.expo - eo) / kap;
+                L.expo = eo;
+            } else {
+                pl = L.wl * L.expo * t;
+            }
+            score(a, L, il, pl, cn);
+            L.tau = L.tau + kap * t;
+            if (L.tau >= MC_TAU_HUGE) { end_edge<P, kFaceExit>(a, L, L.tau, cn); return; }
+            L.ox = L.ox + t * L.kx; L.oy = L.oy + t * L.ky; L.oz = L.oz + t * L.kz;
+            const int nn = next_node_d(g, L.node, f, L.ox, L.oy, L.oz);
+            if (nn < 0) { end_edge<P, kFaceExit>(a, L, L.tau, cn); return; }
+            L.node = nn;
+            L.leaf = leaf_of(g, nn);
+        } else {
+            if (L.tau + t * kap >= L.tau_in) {
+                const double ds = kap > 0.0 ? (L.tau_in - L.tau) / kap : t;
+                if (L.nscatt > 0) score(a, L, il, L.wl * ds, cn);
+                L.ox = L.ox + ds * L.kx; L.oy = L.oy + ds * L.ky; L.oz = L.oz + ds * L.kz;
+                scatter_event<P, kFaceExit>(a, L);
+                return;
+            }
+            if (L.nscatt > 0) score(a, L, il, L.wl * t, cn);
+            L.tau = L.tau + t * kap;
+            L.ox = L.ox + t * L.kx; L.oy = L.oy + t * L.ky; L.oz = L.oz + t * L.kz;
+            const int nn = next_node_d(g, L.node, f, L.ox, L.oy, L.oz);
+            if (nn < 0) { L.phase = 0; return; }
+            L.node = nn;
+            L.leaf = leaf_of(g, nn);
+        }
+    } else {
+        int ax;
+        const float t = exit_oct_f(L, ax);                // units of the half width
+        const float hw = L.gf.w * (float)g.half;          // half width, code units
+        const float kc = opacity_f(a, L, il) * hw;        // per local unit
+        const float dt = kc * t;
+        const double wlc = L.wl * (double)hw;
+        if (L.phase == 1) {
+            double pl;
+            if (dt > 0.0f) {
+                const float fr = L.fexpo * -expm1f(-dt);
+                pl = wlc * (double)(fr / kc);
+                L.fexpo = L.fexpo * expf(-dt);
+            } else {
+                pl = wlc * (double)(L.fexpo * t);
+            }
+            score(a, L, il, pl, cn);
+            L.ftau = L.ftau + dt;
+            if (L.ftau >= (float)MC_TAU_HUGE || !cross_oct_f(g, L, ax, t))
+                end_edge<P, kFaceExit>(a, L, (double)L.ftau, cn);
+        } else {
+            if (L.ftau + dt >= L.ftau_in) {
+                const float ds = kc > 0.0f ? (L.ftau_in - L.ftau) / kc : t;
+                if (L.nscatt > 0) score(a, L, il, wlc * (double)ds, cn);
+                move_oct_f(L, ds);
+                scatter_event<P, kFaceExit>(a, L);
+                return;
+            }
+            if (L.nscatt > 0) score(a, L, il, wlc * (double)t, cn);
+            L.ftau = L.ftau + dt;
+            if (!cross_oct_f(g, L, ax, t)) L.phase = 0;
+        }
+    }
+}
+
+template <int P, int G>
+__device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, Counters &cn)
 {
     const PacketBuf &pk = a.pk;
     L.id = a.first_id + r;
@@ -802,6 +900,7 @@ __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r)
     L.E = pk.E[r];
     L.inu = pk.inu[r];
     L.leaf = pk.leaf[r];
+    L.leaf0 = L.leaf;
     const double *sg = pk.sig + 6 * r;
     L.sHI = sg[0]; L.sHeI = sg[1]; L.sHeII = sg[2];
     L.dabs = sg[3]; L.dsca = sg[4]; L.dg = sg[5];
@@ -820,18 +919,32 @@ __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r)
     if constexpr (Prec<P>::mixed) {
         L.fsHI = (float)L.sHI; L.fsHeI = (float)L.sHeI; L.fsHeII = (float)L.sHeII;
         L.fdust = a.add_dust ? (float)(L.dabs + (a.dust_scatter ? L.dsca : 0.0)) : 0.0f;
-        dda_init_f(a.g, L);
         L.fexpo = 1.0f;
         L.ftau = 0.0f;
     } else {
-        dda_init_d(a.g, L);
         L.expo = 1.0;
         L.tau = 0.0;
     }
+    if constexpr (G == kCarDDA) {
+        if constexpr (Prec<P>::mixed) dda_init_f(a.g, L);
+        else dda_init_d(a.g, L);
+    } else {
+        if (L.leaf < 0) {                  // launched outside the box: nothing to walk
+            end_edge<P, G>(a, L, 0.0, cn);
+            return;
+        }
+        L.node = leaf_node_of(a.g, L.leaf);
+        if constexpr (Prec<P>::mixed) {
+            local_of(a.g, L, L.x, L.y, L.z);
+            set_dir_f(L);
+        } else {
+            L.ox = L.x; L.oy = L.y; L.oz = L.z;
+        }
+    }
 }
 
-template <int P>
-__global__ __launch_bounds__(kBlock) void k_walk_car(KArgs a)
+template <int P, int G>
+__global__ __launch_bounds__(kBlock) void k_walk(KArgs a)
 {
     Lane L;
     L.phase = 0;
@@ -846,13 +959,16 @@ __global__ __launch_bounds__(kBlock) void k_walk_car(KArgs a)
             base = __shfl(base, leader);
             if (L.phase == 0) {
                 const unsigned long long r = base + __popcll(need & ((1ull << lane) - 1ull));
-                if ((int64_t)r < a.npk) load_packet<P>(a, L, (int64_t)r);
+                if ((int64_t)r < a.npk) load_packet<P, G>(a, L, (int64_t)r, cn);
                 else L.phase = 3;
             }
         }
         const bool busy = L.phase == 1 || L.phase == 2;
         if (!__ballot(busy)) break;
-        if (busy) car_step<P>(a, L, cn);
+        if (busy) {
+            if constexpr (G == kCarDDA) car_step<P>(a, L, cn);
+            else oct_step<P>(a, L, cn);
+        }
     }
     // wave reduction of the ledger and segment counts, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
@@ -1086,7 +1202,7 @@ __global__ __launch_bounds__(kBlock) void k_opacity_fill(DevGrid g, DevState s, 
 }
 
 // diagnostic: cells of one ray's direct walk (single thread)
-template <int P>
+template <int P, int G>
 __global__ void k_trace(KArgs a, double x, double y, double z, double kx, double ky, double kz,
                         int32_t *cells, int maxn, int32_t *n)
 {
@@ -1094,19 +1210,49 @@ __global__ void k_trace(KArgs a, double x, double y, double z, double kx, double
     L.x = x; L.y = y; L.z = z; L.kx = kx; L.ky = ky; L.kz = kz;
     int k = 0;
     const DevGrid &g = a.g;
-    if constexpr (Prec<P>::mixed) {
-        dda_init_f(g, L);
-        for (; k < maxn;) {
-            cells[k++] = 1 + L.ix + g.nx * (L.iy + g.ny * L.iz);
-            int ax;
-            const float t = exit_f(L, ax);
-            if (!cross_f(g, L, ax, t)) break;
+    if constexpr (G == kCarDDA) {
+        if constexpr (Prec<P>::mixed) {
+            dda_init_f(g, L);
+            while (k < maxn) {
+                cells[k++] = 1 + L.ix + g.nx * (L.iy + g.ny * L.iz);
+                int ax;
+                const float t = exit_f(L, ax);
+                if (!cross_f(g, L, ax, t)) break;
+            }
+        } else {
+            dda_init_d(g, L);
+            while (k < maxn) {
+                cells[k++] = 1 + L.ix + g.nx * (L.iy + g.ny * L.iz);
+                if (!dda_advance_d(g, L, dda_axis_d(L))) break;
+            }
         }
     } else {
-        dda_init_d(g, L);
-        for (; k < maxn;) {
-            cells[k++] = 1 + L.ix + g.nx * (L.iy + g.ny * L.iz);
-            if (!dda_advance_d(g, L, dda_axis_d(L))) break;
+        L.leaf = find_leaf(g, x, y, z);
+        if (L.leaf >= 0) {
+            L.node = leaf_node_of(g, L.leaf);
+            if constexpr (Prec<P>::mixed) {
+                local_of(g, L, x, y, z);
+                set_dir_f(L);
+                while (k < maxn) {
+                    cells[k++] = 1 + L.leaf;
+                    int ax;
+                    const float t = exit_oct_f(L, ax);
+                    if (!cross_oct_f(g, L, ax, t)) break;
+                }
+            } else {
+#pragma clang fp contract(off)
+                while (k < maxn) {
+                    cells[k++] = 1 + L.leaf;
+                    double t;
+                    int f;
+                    cell_exit_d(node_geom_d(g, L.node), x, y, z, kx, ky, kz, t, f);
+                    x = x + t * kx; y = y + t * ky; z = z + t * kz;
+                    const int nn = next_node_d(g, L.node, f, x, y, z);
+                    if (nn < 0) break;
+                    L.node = nn;
+                    L.leaf = leaf_of(g, nn);
+                }
+            }
         }
     }
     *n = k;
@@ -1177,12 +1323,12 @@ int mc_run_pass(mc_ctx *c)
     }
     float t_launch = 0.0f, t_walk = 0.0f;
     const int64_t N = c->p.nphotons;
-    const void *kw = c->p.precision == MC_MIXED ? (const void *)k_walk_car<MC_MIXED>
-                                                : (const void *)k_walk_car<MC_FP64>;
-    if (!c->car) {
-        mc_set_error("octree transport is not built into this library version");
-        return -3;
-    }
+    const bool face = !c->car || c->p.car_walk_amr;
+    const bool mixed = c->p.precision == MC_MIXED;
+    const void *kw = face ? (mixed ? (const void *)k_walk<MC_MIXED, kFaceExit>
+                                   : (const void *)k_walk<MC_FP64, kFaceExit>)
+                          : (mixed ? (const void *)k_walk<MC_MIXED, kCarDDA>
+                                   : (const void *)k_walk<MC_FP64, kCarDDA>);
     const int nblk = persistent_blocks(kw);
     for (int64_t first = 1; first <= N; first += c->pk.cap) {
         a.first_id = first;
@@ -1194,10 +1340,13 @@ int mc_run_pass(mc_ctx *c)
         MC_HIP(hipGetLastError());
         k_reset_counter<<<1, 1, 0, st>>>(c->st.counter);
         MC_HIP(hipEventRecord(c->ev[2], st));
-        if (c->p.precision == MC_MIXED)
-            k_walk_car<MC_MIXED><<<nblk, kBlock, 0, st>>>(a);
-        else
-            k_walk_car<MC_FP64><<<nblk, kBlock, 0, st>>>(a);
+        if (face) {
+            if (mixed) k_walk<MC_MIXED, kFaceExit><<<nblk, kBlock, 0, st>>>(a);
+            else k_walk<MC_FP64, kFaceExit><<<nblk, kBlock, 0, st>>>(a);
+        } else {
+            if (mixed) k_walk<MC_MIXED, kCarDDA><<<nblk, kBlock, 0, st>>>(a);
+            else k_walk<MC_FP64, kCarDDA><<<nblk, kBlock, 0, st>>>(a);
+        }
         MC_HIP(hipGetLastError());
         MC_HIP(hipEventRecord(c->ev[3], st));
         MC_HIP(hipEventSynchronize(c->ev[3]));
@@ -1282,20 +1431,14 @@ int mc_run_opacity_fill(mc_ctx *c)
 int mc_run_trace(mc_ctx *c, int precision, const double *pos, const double *dir, int32_t *cells,
                  int maxn, int32_t *n)
 {
-    if (!c->car) {
-        mc_set_error("mc_trace_ray: octree walk not available");
-        return -3;
-    }
     KArgs a = make_args(c);
     int32_t *dcells = nullptr, *dn = nullptr;
     MC_HIP(hipMalloc(&dcells, sizeof(int32_t) * (size_t)maxn));
     MC_HIP(hipMalloc(&dn, sizeof(int32_t)));
-    if (precision == MC_MIXED)
-        k_trace<MC_MIXED><<<1, 1, 0, c->stream>>>(a, pos[0], pos[1], pos[2], dir[0], dir[1], dir[2],
-                                                  dcells, maxn, dn);
-    else
-        k_trace<MC_FP64><<<1, 1, 0, c->stream>>>(a, pos[0], pos[1], pos[2], dir[0], dir[1], dir[2],
-                                                 dcells, maxn, dn);
+    const bool face = !c->car || c->p.car_walk_amr;
+    auto kern = face ? (precision == MC_MIXED ? k_trace<MC_MIXED, kFaceExit> : k_trace<MC_FP64, kFaceExit>)
+                     : (precision == MC_MIXED ? k_trace<MC_MIXED, kCarDDA> : k_trace<MC_FP64, kCarDDA>);
+    kern<<<1, 1, 0, c->stream>>>(a, pos[0], pos[1], pos[2], dir[0], dir[1], dir[2], dcells, maxn, dn);
     MC_HIP(hipGetLastError());
     MC_HIP(hipMemcpyAsync(n, dn, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     MC_HIP(hipStreamSynchronize(c->stream));

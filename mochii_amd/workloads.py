@@ -72,3 +72,59 @@ def _mean_energy_ratio(T1, T2, emin=13.598, emax=100.0):
         return np.trapezoid(f, e) / np.trapezoid(f / e, e)
 
     return mean(T1) / mean(T2)
+
+
+def lognormal_field(n, mean=100.0, sigma_ln=0.5, seed=1, slope=-11.0 / 3.0):
+    """Seeded lognormal density on an n^3 periodic grid: a Gaussian random
+    field with power spectrum P(k) ~ k^slope, scaled to unit variance,
+    exponentiated with log-standard deviation sigma_ln and normalised to the
+    given mean.  Array index order [iz, iy, ix]."""
+    rng = np.random.default_rng(seed)
+    k = np.fft.fftfreq(n) * n
+    kz, ky, kx = np.meshgrid(k, k, np.fft.rfftfreq(n) * n, indexing="ij")
+    kk = np.sqrt(kx**2 + ky**2 + kz**2)
+    amp = np.zeros_like(kk)
+    amp[kk > 0] = kk[kk > 0] ** (slope / 2.0)
+    noise = rng.normal(size=kk.shape) + 1j * rng.normal(size=kk.shape)
+    g = np.fft.irfftn(noise * amp, s=(n, n, n), axes=(0, 1, 2))
+    g = (g - g.mean()) / g.std()
+    rho = np.exp(sigma_ln * g)
+    return rho * (mean / rho.mean())
+
+
+def octree_from_field(field, half, lmax=None, thresh_dex=0.3, lmin=1):
+    """Leaf list of an octree over the box [-half, half]^3 refined where the
+    log10 density range inside a cell exceeds thresh_dex (down to the field's
+    own resolution, level log2(n)), never coarser than lmin.  Returns
+    (x, y, z, level, nH) of the leaves, nH the mean over the covered cells."""
+    n = field.shape[0]
+    L = int(round(np.log2(n)))
+    lmax = L if lmax is None else min(lmax, L)
+    lg = np.log10(field)
+    mins, maxs, sums = {L: lg}, {L: lg}, {L: field.astype(np.float64)}
+    for l in range(L - 1, -1, -1):
+        m = 2 ** l
+        mins[l] = mins[l + 1].reshape(m, 2, m, 2, m, 2).min(axis=(1, 3, 5))
+        maxs[l] = maxs[l + 1].reshape(m, 2, m, 2, m, 2).max(axis=(1, 3, 5))
+        sums[l] = sums[l + 1].reshape(m, 2, m, 2, m, 2).sum(axis=(1, 3, 5))
+    xs, ys, zs, lv, nh = [], [], [], [], []
+    cand = np.zeros((1, 3), dtype=np.int64)          # (iz, iy, ix) at level 0
+    for l in range(0, lmax + 1):
+        iz, iy, ix = cand[:, 0], cand[:, 1], cand[:, 2]
+        rng_dex = maxs[l][iz, iy, ix] - mins[l][iz, iy, ix]
+        refine = (l < lmin) | ((rng_dex > thresh_dex) & (l < lmax))
+        leaf = ~refine
+        if leaf.any():
+            w = 2.0 * half / 2 ** l
+            xs.append(-half + (ix[leaf] + 0.5) * w)
+            ys.append(-half + (iy[leaf] + 0.5) * w)
+            zs.append(-half + (iz[leaf] + 0.5) * w)
+            lv.append(np.full(leaf.sum(), l, dtype=np.int32))
+            nh.append(sums[l][iz[leaf], iy[leaf], ix[leaf]] / (2 ** (L - l)) ** 3)
+        r = cand[refine]
+        if r.size == 0:
+            break
+        off = np.array([[a, b, c] for a in (0, 1) for b in (0, 1) for c in (0, 1)])
+        cand = (2 * r[:, None, :] + off[None, :, :]).reshape(-1, 3)
+    return (np.concatenate(xs), np.concatenate(ys), np.concatenate(zs), np.concatenate(lv),
+            np.concatenate(nh))
