@@ -137,6 +137,17 @@ int mc_ionizing_field_and_rates(mc_ctx *ctx, int with_peel);
 /* gas_equilibrium_update (src/ion_balance_mod.f90:352-434). */
 int mc_gas_equilibrium_update(mc_ctx *ctx, double *max_dx, double *dx_vol);
 
+/* cooling_setup (src/cooling_mod.f90:67-92): the H I line-cooling fit
+ * Lambda(T) = T^-1/2 sum_i A_i exp(-T_i/T) (nfit <= 8 terms, the rows of
+ * data/atomic/cooling_h_1.txt) and the free-free Gaunt table, which the
+ * library builds itself.  cooling_model 'low_density' (fit unsuppressed). */
+int mc_cooling_setup(mc_ctx *ctx, int nfit, const double *A, const double *Ti);
+
+/* gas_thermal_update (src/thermal_mod.f90:124-249): per-leaf T_e by
+ * bisection on log T with the ionisation state solved at each trial T. */
+int mc_gas_thermal_update(mc_ctx *ctx, double *max_dx, double *max_dte, double *dx_vol,
+                          double *dte_vol);
+
 /* hydrogen_balance_residual_grid (src/ion_balance_mod.f90:313-343). */
 int mc_hydrogen_balance_residual_grid(mc_ctx *ctx, double *eps_max, double *eps_vol);
 

@@ -137,7 +137,7 @@ static int alloc_leaf_arrays(mc_ctx *c)
     if (dalloc(&s.nH, n) || dalloc(&s.xHI, n) || dalloc(&s.xHeI, n) || dalloc(&s.xHeII, n) ||
         dalloc(&s.ne, n) || dalloc(&s.Te, n) || dalloc(&s.rhokap, n) || dalloc(&s.opac, n) ||
         dalloc(&s.hhe, n * 8) || dalloc(&s.jt, n * (size_t)c->nnu) || dalloc(&s.rates, n * 6) ||
-        dalloc(&s.partials, ((n + 255) / 256) * 3 + 3))
+        dalloc(&s.partials, ((n + 255) / 256) * 6 + 6))
         return -100;
     MC_HIP(hipMemset(s.rhokap, 0, sizeof(double) * n));
     MC_HIP(hipMemset(s.hhe, 0, sizeof(double) * n * 8));
@@ -492,6 +492,49 @@ extern "C" int mc_gas_equilibrium_update(mc_ctx *c, double *max_dx, double *dx_v
         return -1;
     }
     return mc_run_equilibrium(c, max_dx, dx_vol);
+}
+
+extern "C" int mc_cooling_setup(mc_ctx *c, int nfit, const double *A, const double *Ti)
+{
+    CHECK_CTX(c);
+    if (nfit < 0 || nfit > 8 || (nfit > 0 && (!A || !Ti))) {
+        mc_set_error("mc_cooling_setup: need 0..8 fit terms");
+        return -1;
+    }
+    double logT[41], gff[4][41];
+    mc_gff_table(logT, gff);
+    std::memcpy(c->cool.gff, gff, sizeof(gff));
+    c->cool.logT0 = logT[0];
+    c->cool.dlogT = logT[1] - logT[0];
+    c->cool.nfit = nfit;
+    for (int i = 0; i < nfit; ++i) {
+        c->cool.A[i] = A[i];
+        c->cool.Ti[i] = Ti[i];
+    }
+    c->cool_ready = true;
+    return 0;
+}
+
+extern "C" int mc_gas_thermal_update(mc_ctx *c, double *max_dx, double *max_dte, double *dx_vol,
+                                     double *dte_vol)
+{
+    CHECK_CTX(c);
+    if (!max_dx || !max_dte || !dx_vol || !dte_vol) {
+        mc_set_error("mc_gas_thermal_update: null output");
+        return -1;
+    }
+    if (!c->cool_ready) {
+        mc_set_error("mc_gas_thermal_update: call mc_cooling_setup first");
+        return -1;
+    }
+    double m[4];
+    const int rc = mc_run_thermal(c, m);
+    if (rc) return rc;
+    *max_dx = m[0];
+    *max_dte = m[1];
+    *dx_vol = m[2];
+    *dte_vol = m[3];
+    return 0;
 }
 
 extern "C" int mc_hydrogen_balance_residual_grid(mc_ctx *c, double *eps_max, double *eps_vol)

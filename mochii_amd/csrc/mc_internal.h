@@ -52,6 +52,15 @@ struct DevGrid {
     const int32_t *leaf_node;         // leaf -> node
 };
 
+// Cooling tables for the thermal balance (cooling_setup): gbar_ff on
+// log T = 2..6 for Z = 1..4 and the H I line-cooling fit terms.
+struct DevCool {
+    double gff[4][41];
+    double logT0, dlogT;
+    int32_t nfit;
+    double A[8], Ti[8];
+};
+
 // Packet launch records (SoA), written by the launch kernel.
 struct PacketBuf {
     double *x, *y, *z, *kx, *ky, *kz;
@@ -119,6 +128,8 @@ struct mc_ctx {
     PacketBuf pk;
     int nblocks_part;
     bool state_ready;
+    DevCool cool;
+    bool cool_ready;
     // last pass results
     double ledger[3];
     int64_t segs[2];
@@ -132,6 +143,7 @@ int mc_build_octree_host(mc_ctx *c, int nleaf, const double *xl, const double *y
                          const double *zl, const int32_t *lev, double half);
 void mc_qmc_tables(int seed, uint32_t V[32][MC_QMC_DIM], uint32_t key[MC_QMC_DIM][2]);
 void mc_set_error(const std::string &msg);
+void mc_gff_table(double logT[41], double gff[4][41]);
 
 // device work (mc_kernels.hip)
 int mc_run_pass(mc_ctx *c);
@@ -139,6 +151,7 @@ int mc_run_rates(mc_ctx *c);
 int mc_run_equilibrium(mc_ctx *c, double *max_dx, double *dx_vol);
 int mc_run_residual(mc_ctx *c, double *eps_max, double *eps_vol);
 int mc_run_opacity_fill(mc_ctx *c);
+int mc_run_thermal(mc_ctx *c, double out4[4]);
 int mc_run_trace(mc_ctx *c, int precision, const double *pos, const double *dir, int32_t *cells,
                  int maxn, int32_t *n);
 

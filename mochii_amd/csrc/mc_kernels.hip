@@ -1176,6 +1176,116 @@ __global__ __launch_bounds__(kBlock) void k_equilibrium(DevGrid g, DevState s, d
     block_reduce<3>(v, ismax, s.partials);
 }
 
+// ---- thermal balance -------------------------------------------------------
+__device__ __forceinline__ double gbar_ff(const DevCool &cl, double T, int Z)
+{
+    const int iz = min(max(Z, 1), 4) - 1;
+    const double lt = fmin(fmax(log10(T), cl.logT0), cl.logT0 + 40.0 * cl.dlogT);
+    const int i = min((int)((lt - cl.logT0) / cl.dlogT), 39);
+    const double t0 = cl.logT0 + i * cl.dlogT;
+    const double w = (lt - t0) / cl.dlogT;
+    return cl.gff[iz][i] * (1.0 - w) + cl.gff[iz][i + 1] * w;
+}
+
+// recombination cooling fits (Hui & Gnedin 1997) for H II and He III
+__device__ __forceinline__ double beta_hg(double T, double ttr, double z, bool caseA)
+{
+    const double lam = 2.0 * ttr / T;
+    return caseA ? z * 1.778e-29 * T * pow(lam, 1.965) / pow(1.0 + pow(lam / 0.541, 0.502), 2.697)
+                 : z * 3.435e-30 * T * pow(lam, 1.970) / pow(1.0 + pow(lam / 2.250, 0.376), 3.720);
+}
+
+// cooling_total: recombination, free-free, collisional ionisation and H I
+// line cooling [erg cm^-3 s^-1]
+__device__ double cooling_total(const DevCool &cl, const RateSet &r, double T, double nH,
+                                double ne, double xHI, double xHeI, double xHeII, double y,
+                                bool caseA)
+{
+    const double xHeIII = fmax(0.0, 1.0 - xHeI - xHeII);
+    const double nHI = nH * xHI, nHII = nH * (1.0 - xHI);
+    const double nHeI = nH * y * xHeI, nHeII = nH * y * xHeII, nHeIII = nH * y * xHeIII;
+    double cool = ne * (nHII * beta_hg(T, 157807.0, 1.0, caseA) + nHeII * (MC_KBOLTZ * T * r.aHe2) +
+                        nHeIII * beta_hg(T, 631515.0, 2.0, caseA));
+    cool += 1.42554e-27 * sqrt(T) * ne *
+            ((nHII + nHeII) * gbar_ff(cl, T, 1) + 4.0 * nHeIII * gbar_ff(cl, T, 2));
+    cool += ne * (nHI * r.cH * MC_ETH_HI + nHeI * r.cHe1 * MC_ETH_HEI + nHeII * r.cHe2 * MC_ETH_HEII) *
+            MC_EV2ERG;
+    double lam = 0.0;
+    for (int i = 0; i < cl.nfit; ++i) lam += cl.A[i] * exp(-cl.Ti[i] / T);
+    return cool + ne * nHI * (lam / sqrt(T));
+}
+
+struct CellIn { double nH, gH, gHe1, gHe2, hH, hHe1, hHe2; };
+
+// net heating - cooling at trial T, ionisation state solved at T (warm start
+// from the state passed in)
+__device__ double net_rate(const DevCool &cl, const CellIn &c, double T, bool caseA, double y,
+                           double &xHI, double &xHeI, double &xHeII, double &ne)
+{
+    const RateSet r = rates_at(T, caseA);
+    ne = solve_cell(r, c.gH, c.gHe1, c.gHe2, c.nH, y, xHI, xHeI, xHeII);
+    const double heat = c.nH * (xHI * c.hH + y * (xHeI * c.hHe1 + xHeII * c.hHe2));
+    return heat - cooling_total(cl, r, T, c.nH, ne, xHI, xHeI, xHeII, y, caseA);
+}
+
+// gas_thermal_update: bisection on log T in [te_min, te_max] to
+// thi/tlo - 1 < 1e-5, relaxation on x, T taken directly.  Partials
+// {max|dx|, max|dT|/T, sum V|dx|, sum V x_HII, sum V ne |dT|, sum V ne T}
+__global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCool cl, bool caseA,
+                                                    double y, double w, double tmin, double tmax,
+                                                    double d2cm)
+{
+    const int il = blockIdx.x * kBlock + threadIdx.x;
+    double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    bool ismax[6] = {true, true, false, false, false, false};
+    if (il < g.nleaf) {
+        const size_t n = g.nleaf;
+        const double nH = s.nH[il];
+        if (nH <= 0.0) {
+            s.ne[il] = 0.0;
+        } else {
+            CellIn c = {nH, s.rates[il], s.rates[n + il], s.rates[2 * n + il],
+                        s.rates[3 * n + il], s.rates[4 * n + il], s.rates[5 * n + il]};
+            double xHI = s.xHI[il], xHeI = s.xHeI[il], xHeII = s.xHeII[il], ne, te;
+            double tlo = tmin, thi = tmax;
+            const double nlo = net_rate(cl, c, tlo, caseA, y, xHI, xHeI, xHeII, ne);
+            const double nhi = net_rate(cl, c, thi, caseA, y, xHI, xHeI, xHeII, ne);
+            if (nlo <= 0.0) {
+                te = tlo;
+            } else if (nhi >= 0.0) {
+                te = thi;
+            } else {
+                for (int it = 0; it < 60; ++it) {
+                    const double tm = sqrt(tlo * thi);
+                    if (net_rate(cl, c, tm, caseA, y, xHI, xHeI, xHeII, ne) > 0.0) tlo = tm;
+                    else thi = tm;
+                    if (thi / tlo - 1.0 < 1.0e-5) break;
+                }
+                te = sqrt(tlo * thi);
+            }
+            net_rate(cl, c, te, caseA, y, xHI, xHeI, xHeII, ne);
+            const double oHI = s.xHI[il], oT = s.Te[il];
+            const double nHI = (1.0 - w) * oHI + w * xHI;
+            const double nHeI = (1.0 - w) * s.xHeI[il] + w * xHeI;
+            const double nHeII = (1.0 - w) * s.xHeII[il] + w * xHeII;
+            const double x3 = fmax(0.0, 1.0 - nHeI - nHeII);
+            const double nen = nH * ((1.0 - nHI) + y * (nHeII + 2.0 * x3));
+            s.xHI[il] = nHI; s.xHeI[il] = nHeI; s.xHeII[il] = nHeII;
+            s.ne[il] = nen; s.Te[il] = te;
+            s.opac[il] = opac_of(nH, nHI, nHeI, nHeII, s.rhokap[il], y, d2cm);
+            const double wd = 2.0 * leaf_half(g, il), vol = (wd * wd) * wd;
+            const double d = fabs(nHI - oHI);
+            v[0] = d;
+            v[1] = fabs(te - oT) / oT;
+            v[2] = vol * d;
+            v[3] = vol * (1.0 - nHI);
+            v[4] = vol * nen * fabs(te - oT);
+            v[5] = vol * nen * te;
+        }
+    }
+    block_reduce<6>(v, ismax, s.partials);
+}
+
 // hydrogen_balance_residual_grid partials {max eps, sum V eps, sum V}
 __global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, DevState s, bool caseA)
 {
@@ -1404,6 +1514,36 @@ int mc_run_equilibrium(mc_ctx *c, double *max_dx, double *dx_vol)
     c->times[3] = ms;
     *max_dx = m[0];
     *dx_vol = m[1] / std::fmax(m[2], DBL_MIN);
+    return 0;
+}
+
+int mc_run_thermal(mc_ctx *c, double out[4])
+{
+    const int nblk = (c->nleaf + kBlock - 1) / kBlock;
+    hipStream_t st = c->stream;
+    MC_HIP(hipEventRecord(c->ev[6], st));
+    k_thermal<<<nblk, kBlock, 0, st>>>(c->grid, c->st, c->cool, c->p.case_A != 0, c->p.He_abund,
+                                        c->p.ion_relax, c->p.te_min, c->p.te_max,
+                                        c->p.distance2cm);
+    MC_HIP(hipGetLastError());
+    MC_HIP(hipEventRecord(c->ev[7], st));
+    std::vector<double> h((size_t)nblk * 6);
+    MC_HIP(hipMemcpyAsync(h.data(), c->st.partials, sizeof(double) * 6 * nblk,
+                          hipMemcpyDeviceToHost, st));
+    MC_HIP(hipStreamSynchronize(st));
+    double m[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < nblk; ++b) {
+        m[0] = std::fmax(m[0], h[(size_t)b * 6]);
+        m[1] = std::fmax(m[1], h[(size_t)b * 6 + 1]);
+        for (int k = 2; k < 6; ++k) m[k] += h[(size_t)b * 6 + k];
+    }
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, c->ev[6], c->ev[7]);
+    c->times[3] = ms;
+    out[0] = m[0];
+    out[1] = m[1];
+    out[2] = m[2] / std::fmax(m[3], DBL_MIN);
+    out[3] = m[4] / std::fmax(m[5], DBL_MIN);
     return 0;
 }
 

@@ -331,3 +331,98 @@ int mc_build_octree_host(mc_ctx *c, int nleaf, const double *xl, const double *y
     c->nleaf = nleaf;
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Thermally averaged free-free Gaunt factor (reference: gaunt_ff_mean and
+// getGauntFF, src/cooling_mod.f90:182-220, src/gaunt.f90).  Hummer (1988,
+// ApJ 327, 477) two-dimensional Chebyshev fit in single precision; the
+// coefficients are single-precision values widened to double, as in the
+// reference's table.  Setup only.
+static const float kHummer[88] = {
+    8.986940175e+00f, -4.009515855e+00f, 8.808871266e-01f, 2.640245111e-02f, -4.580645915e-02f,
+    -3.568055702e-03f, 2.827798067e-03f, 3.365860195e-04f, -8.006936989e-01f, 9.466021705e-01f,
+    9.043402532e-02f, -9.608451450e-02f, -1.885629865e-02f, 1.050313890e-02f, 2.800889961e-03f,
+    -1.078209202e-03f, -3.781305103e-01f, 1.102726332e-01f, -1.543619180e-02f, 8.310561114e-03f,
+    2.179620525e-02f, 4.259726289e-03f, -4.181588794e-03f, -1.770208330e-03f, 1.877213132e-02f,
+    -1.004885705e-01f, -5.483366378e-02f, -4.520154409e-03f, 8.366530426e-03f, 3.700273930e-03f,
+    6.889320423e-04f, 9.460313195e-05f, 7.300158392e-02f, 3.576785497e-03f, -4.545307025e-03f,
+    -1.017965604e-02f, -9.530211924e-03f, -3.450186162e-03f, 1.040482914e-03f, 1.407073544e-03f,
+    -1.744671550e-03f, 2.864013856e-02f, 1.903394837e-02f, 7.091074494e-03f, -9.668371391e-04f,
+    -2.999107465e-03f, -1.820642230e-03f, -3.874082085e-04f, -1.707268366e-02f, -4.694254776e-03f,
+    1.311691517e-03f, 5.316703136e-03f, 5.178193095e-03f, 2.451228935e-03f, -2.277321615e-05f,
+    -8.182359057e-04f, 2.567331664e-04f, -9.155339970e-03f, -6.997479192e-03f, -3.571518641e-03f,
+    -2.096101038e-04f, 1.553822487e-03f, 1.509584686e-03f, 6.212627837e-04f, 4.098322531e-03f,
+    1.635218463e-03f, -5.918883504e-04f, -2.333091048e-03f, -2.484138313e-03f, -1.359996060e-03f,
+    -5.371426147e-05f, 5.553549563e-04f, 3.837562402e-05f, 2.938325230e-03f, 2.393747064e-03f,
+    1.328839809e-03f, 9.135013312e-05f, -7.137252303e-04f, -7.656848158e-04f, -3.504683798e-04f,
+    -8.491991820e-04f, -3.615327726e-04f, 3.148015257e-04f, 8.909207650e-04f, 9.869737522e-04f,
+    6.134671184e-04f, 1.068883394e-04f, -2.046080100e-04f};
+
+static void gaunt_ff(float z, float lt, const float *xlf, float *g, int n)
+{
+    const float xl = lt < 2.5f ? 5.1983649f - 2.5f : (lt > 8.0f ? 5.1983649f - 8.0f : 5.1983649f - lt);
+    const float txg = 0.66666667f * (2.0f * z + xl);
+    const float gamma2 = std::pow(10.0f, txg * 1.5f);
+    const float con = 0.72727273f * xl + 0.90909091f;
+    float c[9];
+    for (int j = 0; j < 8; ++j) {
+        // Clenshaw sum over the 11 gamma^2 coefficients of row j
+        float b[13];
+        b[11] = kHummer[10 * 8 + j];
+        b[10] = (float)((double)(txg * b[11]) + (double)kHummer[9 * 8 + j]);
+        for (int ir = 9; ir >= 1; --ir)
+            b[ir] = (float)((double)(txg * b[ir + 1] - b[ir + 2]) + (double)kHummer[(ir - 1) * 8 + j]);
+        c[j + 1] = 0.25f * (b[1] - b[3]);
+    }
+    for (int i = 0; i < n; ++i) {
+        const float txu = 0.72727273f * xlf[i] + con;
+        const float u = std::pow(10.0f, (txu - 0.90909091f) / 0.72727273f);
+        if (std::fabs(txu) <= 2.0f) {
+            float b[9];
+            b[8] = c[8];
+            b[7] = txu * b[8] + c[7];
+            for (int ir = 6; ir >= 1; --ir) b[ir] = txu * b[ir + 1] - b[ir + 2] + c[ir];
+            g[i] = b[1] - b[3];
+        } else if (std::log10(u) < -4.0f && gamma2 < 0.3f) {
+            g[i] = 0.551329f * std::log(2.24593f / u);
+        } else if (std::log10(u) < -4.0f) {
+            g[i] = 0.551329f * std::log(0.944931f / u / std::sqrt(gamma2));
+        } else if (txu > 2.0f) {
+            if (std::log10(gamma2) < -3.0f) {
+                g[i] = std::sqrt(0.9549297f / u);
+            } else if (std::log10(gamma2) < 0.0f) {
+                const float sl = (std::sqrt(12.0f * gamma2 / u) - std::sqrt(0.9549297f / u)) / 3.0f;
+                g[i] = std::sqrt(12.0f * gamma2 / u) + sl * std::log10(gamma2);
+            } else {
+                g[i] = std::fmin(1.05f, std::sqrt(12.0f * gamma2 / u));
+            }
+        }
+    }
+}
+
+// gbar_ff table on log T = 2..6 (41 points) for net charge Z = 1..4: the
+// 64-point midpoint rule in ln u over u in [1e-4, 20] with weight e^-u,
+// normalised by the integral of e^-u over the same range
+void mc_gff_table(double logT[41], double gff[4][41])
+{
+    const int nu = 64;
+    const double lo = std::log(1.0e-4), hi = std::log(20.0);
+    double u[64], du[64];
+    for (int i = 0; i < nu; ++i) {
+        u[i] = std::exp(lo + (hi - lo) * ((double)(i + 1) - 0.5) / nu);
+        du[i] = u[i] * (hi - lo) / nu;
+    }
+    for (int it = 0; it < 41; ++it) {
+        logT[it] = 2.0 + 4.0 * (double)it / 40.0;
+        const double T = std::pow(10.0, logT[it]);
+        const double r = 13.605693 * MC_EV2ERG / (MC_KBOLTZ * T);
+        float xlf[64], g[64];
+        for (int i = 0; i < nu; ++i) xlf[i] = (float)std::log10(u[i] / r);
+        for (int iz = 1; iz <= 4; ++iz) {
+            gaunt_ff((float)std::log10((double)iz), (float)std::log10(T), xlf, g, nu);
+            double s = 0.0;
+            for (int i = 0; i < nu; ++i) s += (double)g[i] * std::exp(-u[i]) * du[i];
+            gff[iz - 1][it] = s / (std::exp(-1.0e-4) - std::exp(-20.0));
+        }
+    }
+}

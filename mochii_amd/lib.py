@@ -95,6 +95,8 @@ def library():
         "mc_ionizing_field_and_rates": ([vp, C.c_int], C.c_int),
         "mc_gas_equilibrium_update": ([vp, dp, dp], C.c_int),
         "mc_hydrogen_balance_residual_grid": ([vp, dp, dp], C.c_int),
+        "mc_cooling_setup": ([vp, C.c_int, dp, dp], C.c_int),
+        "mc_gas_thermal_update": ([vp, dp, dp, dp, dp], C.c_int),
         "mc_tallies_get": ([vp, dp, dp, dp, i64p], C.c_int),
         "mc_rates_get": ([vp, dp], C.c_int),
         "mc_energy_ledger": ([vp, dp], C.c_int),

@@ -84,6 +84,18 @@ class Model:
               "mc_gas_equilibrium_update")
         return a.value, b.value
 
+    def cooling_setup(self, fit):
+        """fit: sequence of (A_i, T_i) terms of the H I line-cooling fit."""
+        A = np.ascontiguousarray([f[0] for f in fit], dtype=np.float64)
+        Ti = np.ascontiguousarray([f[1] for f in fit], dtype=np.float64)
+        check(self._L.mc_cooling_setup(self._h, A.size, _dp(A), _dp(Ti)), "mc_cooling_setup")
+
+    def gas_thermal_update(self):
+        a, b, c, d = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        check(self._L.mc_gas_thermal_update(self._h, C.byref(a), C.byref(b), C.byref(c),
+                                            C.byref(d)), "mc_gas_thermal_update")
+        return a.value, b.value, c.value, d.value
+
     def hydrogen_balance_residual_grid(self):
         a, b = C.c_double(), C.c_double()
         check(self._L.mc_hydrogen_balance_residual_grid(self._h, C.byref(a), C.byref(b)),
@@ -163,12 +175,18 @@ def run_model(model, niter=None, log=None, final_pass=True):
         if p.gas_niter < 1:
             break
         if p.solve_te:
-            raise NotImplementedError("thermal update is not built into this version")
-        max_dx, dx_vol = model.gas_equilibrium_update()
-        converged = (dx_vol < p.gas_tol) if p.conv_vol else (max_dx < p.gas_tol)
-        hist.append((it, max_dx, dx_vol))
+            max_dx, max_dte, dx_vol, dte_vol = model.gas_thermal_update()
+            if p.conv_vol:
+                converged = dx_vol < p.gas_tol and dte_vol < p.gas_tol_te
+            else:
+                converged = max_dx < p.gas_tol and max_dte < p.gas_tol_te
+            hist.append((it, max_dx, dx_vol, max_dte, dte_vol))
+        else:
+            max_dx, dx_vol = model.gas_equilibrium_update()
+            converged = (dx_vol < p.gas_tol) if p.conv_vol else (max_dx < p.gas_tol)
+            hist.append((it, max_dx, dx_vol))
         if log:
-            log(f"iteration {it:4d}: max |delta x_HII| = {max_dx:12.4e},  vol = {dx_vol:12.4e}")
+            log(f"iteration {it:4d}: " + ", ".join(f"{v:.4e}" for v in hist[-1][1:]))
         if converged:
             break
     t_loop = time.perf_counter() - t0

@@ -6,6 +6,13 @@ import numpy as np
 from .lib import MC_FP64, MC_GRID_CAR
 
 PC2CM = 3.0856776e18
+
+# H I collisional line-cooling fit Lambda(T) = T^-1/2 sum A_i exp(-T_i/T)
+# [erg cm^3 s^-1 K^1/2, K], the four rows of the reference's
+# data/atomic/cooling_h_1.txt (CHIANTI v11.0.2 low-density fit over
+# 1e3-1e5 K, max fit error 0.52%); atomic data, carried as data.
+HI_COOLING_FIT = ((4.42616622e-17, 1.184356e+05), (2.62917725e-17, 1.269656e+05),
+                  (1.03385873e-16, 1.528382e+05), (2.26605668e-16, 2.577561e+05))
 # Q(H) = 1e49 s^-1 for a 40000 K blackbody over 13.598-100 eV
 # (tests/g1_stromgren/g1_stromgren.in:17-22 of the reference)
 L_Q49_T40K = 3.177837e38
@@ -25,6 +32,26 @@ def stromgren(n=64, nphotons=1_000_000, precision=MC_FP64, nH=100.0, half=4.0,
                   xHI_init=1.0, xHeI_init=1.0, xHeII_init=0.0)
     nHarr = np.full(n ** 3, nH)
     return params, nHarr
+
+
+def hii_thermal(n=32, nphotons=200_000, precision=MC_FP64, seed=12345, niter=100,
+                tol=2e-3, tol_te=2e-3, half=8.0, octree=None, nH=None):
+    """H + He (He/H = 0.1) H II region with thermal balance: one 45000 K
+    blackbody at the box centre with Q(H) = 3e49 s^-1, case A, T_e solved in
+    [1e3, 5e4] K, volume convergence criterion.  Uniform car grid unless a
+    leaf list (octree) with per-leaf nH is given."""
+    lum = 3.0 * L_Q49_T40K * _mean_energy_ratio(4.5e4, 4.0e4)
+    params = dict(nphotons=nphotons, qmc_seed=seed, precision=precision, eion_min=13.598,
+                  eion_max=100.0, nnu_ion=16, ion_align_edges=1, nsource=1,
+                  tstar=[4.5e4], src_lum=[lum], src_x=[0.0], src_y=[0.0], src_z=[0.0],
+                  grid_type=MC_GRID_CAR if octree is None else 1, nx=n, ny=n, nz=n,
+                  xmax=half, ymax=half, zmax=half, distance2cm=PC2CM, He_abund=0.1,
+                  te_fixed=1.0e4, ion_relax=1.0, te_min=1.0e3, te_max=5.0e4, gas_tol=tol,
+                  gas_tol_te=tol_te, conv_vol=1, gas_niter=niter, case_A=1, solve_te=1,
+                  xHI_init=1.0, xHeI_init=1.0, xHeII_init=0.0)
+    if nH is None:
+        nH = np.full(n ** 3, 100.0)
+    return params, nH
 
 
 def synthetic_kext(nlam=200, lam_min=0.0912 / 8, lam_max=10.0, slope=-1.2, albedo=0.4,

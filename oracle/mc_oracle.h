@@ -80,6 +80,20 @@ double orc_ci_HI(double T);
 double orc_ci_HeI(double T);
 double orc_ci_HeII(double T);
 
+/* ---- cooling (src/cooling_mod.f90, src/gaunt.f90) ----------------------- */
+typedef struct {
+    int nfit;                 /* H I line-cooling fit terms */
+    double A[16], Ti[16];
+    double logT[41], gff[4 * 41];
+} orc_cooling;
+
+void   orc_gaunt_ff(float z, float log10Te, const float *xlf, float *g, int n);
+double orc_gaunt_ff_mean(double T, double Zc);
+void   orc_gff_table(double *logT41, double *gff41x4);
+double orc_gbar_ff(const double *logT, const double *gff, double T, int Z);
+double orc_cooling_total(const orc_cooling *cl, double T, double nH, double ne, double xHI,
+                         double xHeI, double xHeII, double yHe, int caseA);
+
 /* ---- counter-based RNG for the post-launch draws ------------------------ */
 double orc_rng_uniform(uint32_t seed, int stream, uint64_t id, uint32_t k);
 
@@ -140,6 +154,8 @@ typedef struct {
     double  e_emit, e_abs, e_esc;
     /* rates */
     double *gHI, *gHeI, *gHeII, *hHI, *hHeI, *hHeII;
+    /* cooling tables */
+    orc_cooling cool;
 } orc_ctx;
 
 orc_ctx *orc_create(const orc_params *p);
@@ -164,6 +180,11 @@ void     orc_solve_ion_cell(double gH, double gHe1, double gHe2, double nH, doub
                             int caseA, double yHe, double *xHI, double *xHeI,
                             double *xHeII, double *ne);
 void     orc_residual_grid(orc_ctx *c, double *eps_max, double *eps_vol);
+void     orc_cooling_setup(orc_ctx *c, int nfit, const double *A, const double *Ti);
+double   orc_net_rate(const orc_ctx *c, int il, double T, double *xHI, double *xHeI,
+                      double *xHeII, double *ne);
+void     orc_thermal_update(orc_ctx *c, double *max_dx, double *max_dte, double *dx_vol,
+                            double *dte_vol);
 
 /* diagnostics: leaves crossed by the direct walk of one ray (no tally) */
 int      orc_trace_ray(orc_ctx *c, double x, double y, double z, double kx,

@@ -107,3 +107,73 @@ void orc_residual_grid(orc_ctx *c, double *eps_max, double *eps_vol)
     *eps_max = emax;
     *eps_vol = sev / (sv > DBL_MIN ? sv : DBL_MIN);
 }
+
+/* cooling_setup (src/cooling_mod.f90:67-92): H I fit terms (from
+ * data/atomic/cooling_h_1.txt) and the gbar_ff table */
+void orc_cooling_setup(orc_ctx *c, int nfit, const double *A, const double *Ti)
+{
+    c->cool.nfit = nfit;
+    for (int i = 0; i < nfit; ++i) { c->cool.A[i] = A[i]; c->cool.Ti[i] = Ti[i]; }
+    orc_gff_table(c->cool.logT, c->cool.gff);
+}
+
+/* net_rate (src/thermal_mod.f90:38-115), no metals, no grain heating */
+double orc_net_rate(const orc_ctx *c, int il, double T, double *xHI, double *xHeI,
+                    double *xHeII, double *ne)
+{
+    const double nH = c->nH[il], y = c->p.He_abund;
+    orc_solve_ion_cell(c->gHI[il], c->gHeI[il], c->gHeII[il], nH, T, c->p.caseA, y, xHI, xHeI,
+                       xHeII, ne);
+    double heat = nH * (*xHI * c->hHI[il] + y * (*xHeI * c->hHeI[il] + *xHeII * c->hHeII[il]));
+    return heat - orc_cooling_total(&c->cool, T, nH, *ne, *xHI, *xHeI, *xHeII, y, c->p.caseA);
+}
+
+/* gas_thermal_update (src/thermal_mod.f90:124-249) on one process */
+void orc_thermal_update(orc_ctx *c, double *max_dx, double *max_dte, double *dx_vol,
+                        double *dte_vol)
+{
+    const double w = c->p.ion_relax, y = c->p.He_abund;
+    double mdx = 0.0, mdt = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+    for (int il = 0; il < c->nleaf; ++il) {
+        const double nH = c->nH[il];
+        if (nH <= 0.0) { c->ne[il] = 0.0; continue; }
+        double xHI = c->xHI[il], xHeI = c->xHeI[il], xHeII = c->xHeII[il], ne, te;
+        double tlo = c->p.te_min, thi = c->p.te_max;
+        double net_lo = orc_net_rate(c, il, tlo, &xHI, &xHeI, &xHeII, &ne);
+        double net_hi = orc_net_rate(c, il, thi, &xHI, &xHeI, &xHeII, &ne);
+        if (net_lo <= 0.0) {
+            te = tlo;
+        } else if (net_hi >= 0.0) {
+            te = thi;
+        } else {
+            for (int it = 1; it <= 60; ++it) {
+                double tmid = sqrt(tlo * thi);
+                double net_mid = orc_net_rate(c, il, tmid, &xHI, &xHeI, &xHeII, &ne);
+                if (net_mid > 0.0) tlo = tmid; else thi = tmid;
+                if (thi / tlo - 1.0 < 1.0e-5) break;
+            }
+            te = sqrt(tlo * thi);
+        }
+        orc_net_rate(c, il, te, &xHI, &xHeI, &xHeII, &ne);
+        double nHI = (1.0 - w) * c->xHI[il] + w * xHI;
+        double nHeI = (1.0 - w) * c->xHeI[il] + w * xHeI;
+        double nHeII = (1.0 - w) * c->xHeII[il] + w * xHeII;
+        double xHeIII = 1.0 - nHeI - nHeII;
+        if (xHeIII < 0.0) xHeIII = 0.0;
+        double ne_new = nH * ((1.0 - nHI) + y * (nHeII + 2.0 * xHeIII));
+        double d = fabs(nHI - c->xHI[il]);
+        if (d > mdx) mdx = d;
+        double dt = fabs(te - c->Te[il]) / c->Te[il];
+        if (dt > mdt) mdt = dt;
+        double v = leaf_vol(c, il);
+        s1 = s1 + v * d;
+        s2 = s2 + v * (1.0 - nHI);
+        s3 = s3 + v * ne_new * fabs(te - c->Te[il]);
+        s4 = s4 + v * ne_new * te;
+        c->xHI[il] = nHI; c->xHeI[il] = nHeI; c->xHeII[il] = nHeII;
+        c->ne[il] = ne_new; c->Te[il] = te;
+    }
+    *max_dx = mdx; *max_dte = mdt;
+    *dx_vol = s1 / (s2 > DBL_MIN ? s2 : DBL_MIN);
+    *dte_vol = s3 / (s4 > DBL_MIN ? s4 : DBL_MIN);
+}

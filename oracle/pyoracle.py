@@ -42,6 +42,45 @@ class OrcParams(C.Structure):
     ]
 
 
+class Cooling(C.Structure):
+    """Mirror of orc_cooling (H I line fit + free-free Gaunt table)."""
+    _fields_ = [("nfit", C.c_int), ("A", C.c_double * 16), ("Ti", C.c_double * 16),
+                ("logT", C.c_double * 41), ("gff", C.c_double * (4 * 41))]
+
+    @classmethod
+    def make(cls, fit=(), gaunt=True):
+        """fit: (A_i, T_i) pairs; gaunt=False leaves the Gaunt table at zero,
+        a number sets it to that constant."""
+        c = cls()
+        c.nfit = len(fit)
+        for i, (a, t) in enumerate(fit):
+            c.A[i], c.Ti[i] = a, t
+        lt, gf = np.zeros(41), np.zeros(4 * 41)
+        lib().orc_gff_table(ptr(lt), ptr(gf))
+        c.logT[:] = lt
+        if gaunt is True:
+            c.gff[:] = gf
+        elif gaunt is not False:
+            c.gff[:] = np.full(4 * 41, float(gaunt))
+        return c
+
+    def total(self, T, nH, ne, xHI, xHeI, xHeII, yHe, caseA):
+        return lib().orc_cooling_total(C.byref(self), T, nH, ne, xHI, xHeI, xHeII, yHe,
+                                       int(caseA))
+
+
+def gaunt_ff(z, log10Te, xlf):
+    """Hummer (1988) fit of the free-free Gaunt factor, single precision."""
+    x = np.ascontiguousarray(xlf, dtype=np.float32)
+    g = np.zeros(x.size, dtype=np.float32)
+    lib().orc_gaunt_ff(z, log10Te, ptr(x, C.c_float), ptr(g, C.c_float), x.size)
+    return g
+
+
+def gaunt_ff_mean(T, Z):
+    return lib().orc_gaunt_ff_mean(T, float(Z))
+
+
 class OrcCtx(C.Structure):
     """Leading fields of orc_ctx that Python reads directly."""
 
@@ -84,6 +123,19 @@ def _load():
         f = getattr(lib, name)
         f.restype = C.c_double
         f.argtypes = [C.c_double] * (2 if name == "orc_planck_shape" else 1)
+    lib.orc_cooling_setup.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    lib.orc_thermal_update.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+    lib.orc_net_rate.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
+    lib.orc_net_rate.restype = C.c_double
+    lib.orc_gaunt_ff.argtypes = [C.c_float, C.c_float, C.POINTER(C.c_float),
+                                 C.POINTER(C.c_float), C.c_int]
+    lib.orc_gaunt_ff_mean.argtypes = [C.c_double, C.c_double]
+    lib.orc_gaunt_ff_mean.restype = C.c_double
+    lib.orc_gff_table.argtypes = [_dp, _dp]
+    lib.orc_gbar_ff.argtypes = [_dp, _dp, C.c_double, C.c_int]
+    lib.orc_gbar_ff.restype = C.c_double
+    lib.orc_cooling_total.argtypes = [C.POINTER(Cooling)] + [C.c_double] * 7 + [C.c_int]
+    lib.orc_cooling_total.restype = C.c_double
     lib.orc_build_planck.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double,
                                      C.c_double, _dp, C.c_int]
     lib.orc_build_tabulated.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
@@ -244,6 +296,25 @@ class Model:
         a, b = C.c_double(), C.c_double()
         lib().orc_equilibrium_update(self.c, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def cooling_setup(self, A, Ti):
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        Ti = np.ascontiguousarray(Ti, dtype=np.float64)
+        lib().orc_cooling_setup(self.c, A.size, ptr(A), ptr(Ti))
+
+    def thermal_update(self):
+        a, b, c, d = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        lib().orc_thermal_update(self.c, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+        return a.value, b.value, c.value, d.value
+
+    def net_rate(self, il, T, x=None):
+        """Net heating - cooling of leaf il at T (ionisation solved at T,
+        warm start from x = (xHI, xHeI, xHeII) or the leaf's state)."""
+        st = self.state()
+        x0 = x if x is not None else (st["xHI"][il], st["xHeI"][il], st["xHeII"][il])
+        a, b, c, ne = C.c_double(x0[0]), C.c_double(x0[1]), C.c_double(x0[2]), C.c_double()
+        net = lib().orc_net_rate(self.c, il, T, C.byref(a), C.byref(b), C.byref(c), C.byref(ne))
+        return net, (a.value, b.value, c.value, ne.value)
 
     def residual_grid(self):
         a, b = C.c_double(), C.c_double()
