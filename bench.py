@@ -11,8 +11,14 @@ Prints ONE JSON line.  Extra fields:
   forms        per-form packets/s, walk-kernel time, time to converged model
   roofline     walk kernel: algorithmic bytes (segments x bytes/segment)
                over the kernel's event-timed duration, against 8 TB/s
-  cpu_baseline the FP64 CPU oracle timed on the host cores on a bounded
-               sample of the same pass
+  cpu_baseline the FP64 CPU oracle timed on the host cores: whole passes of
+               the same workload on the converged state, repeated for
+               about 10 s
+
+  python bench.py --trace-summary KERNEL_TRACE.csv [--steps K]
+    condenses a rocprofv3 --kernel-trace CSV of a bench run into per-kernel
+    statistics over all dispatches and over the last K dispatches of each
+    kernel (the timed steps of the form that launches it).
 """
 import argparse
 import json
@@ -24,10 +30,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per path segment, H/He on a Cartesian grid:
-# 8*(4 + d + n_tr) state reads + 16*(7 + d + 2 n_tr) tally read-modify-writes,
-# d = n_tr = 0
-BYTES_PER_SEGMENT = 8 * 4 + 16 * 7
+# algorithmic bytes per scored path segment, H/He on a Cartesian grid
+# (d = dust terms, n_tr = metal transitions, both 0 here):
+#   FP64 form:  8*(4 + d + n_tr) state reads + 16*(7 + d + 2 n_tr) tally RMWs
+#   mixed form: 4*(4 + d + n_tr) state reads (FP32 opacity record) + same RMWs
+BYTES_PER_SEGMENT = {"fp64": 8 * 4 + 16 * 7, "mixed": 4 * 4 + 16 * 7}
 
 
 def parse():
@@ -37,7 +44,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--packets", type=int, default=1_000_000)
-    ap.add_argument("--cpu-sample", type=int, default=200_000)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--trace-summary", default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-converge", action="store_true")
     return ap.parse_args()
@@ -55,8 +63,31 @@ def converge(make_model, log):
     return m, res, dt
 
 
+def trace_summary(path, last):
+    """Per-kernel duration statistics from a rocprofv3 kernel-trace CSV."""
+    import csv
+    import numpy as np
+    per = {}
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            d = int(row["End_Timestamp"]) - int(row["Start_Timestamp"])
+            per.setdefault(row["Kernel_Name"], []).append((int(row["Start_Timestamp"]), d))
+    total = sum(d for v in per.values() for _, d in v)
+    rows = sorted(per.items(), key=lambda kv: -sum(d for _, d in kv[1]))
+    print(f"{'kernel':70s} {'calls':>6s} {'avg_us':>10s} {'pct':>6s} {'last':>5s} {'avg_us_last':>12s}")
+    for name, v in rows:
+        v.sort()
+        d = np.array([x for _, x in v], dtype=np.float64)
+        tail = d[-last:]
+        print(f"{name[:70]:70s} {d.size:6d} {d.mean() / 1e3:10.1f} {100 * d.sum() / total:6.2f} "
+              f"{tail.size:5d} {tail.mean() / 1e3:12.1f}")
+
+
 def main():
     args = parse()
+    if args.trace_summary:
+        trace_summary(args.trace_summary, args.steps)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
               "(the pass would shard packets over GPUs and all-reduce the tallies)",
@@ -109,7 +140,7 @@ def main():
                            walk_ms=wms, segments_per_pass=nseg,
                            time_to_converged_s=t_conv, iterations=res["iterations"],
                            converged=res["converged"],
-                           walk_GBs=nseg * BYTES_PER_SEGMENT / (wms * 1e-3) / 1e9)
+                           walk_GBs=nseg * BYTES_PER_SEGMENT[name] / (wms * 1e-3) / 1e9)
         log(f"[{name}] {forms[name]}")
         state_for_cpu = m.state()
         m.close()
@@ -128,13 +159,20 @@ def main():
         orc = oracle_model(params, nH)
         orc.set_state(xHI=state_for_cpu["xHI"], xHeI=state_for_cpu["xHeI"],
                       xHeII=state_for_cpu["xHeII"])
-        ns = min(args.cpu_sample, N)
+        npass = 0
         t0 = time.perf_counter()
-        orc.transport_pass(1, ns + 1, ncpu)
-        dtc = time.perf_counter() - t0
-        cpu = {"value": ns / dtc, "unit": "packets/s", "cores": ncpu, "kind": "port",
-               "sample": f"FP64 C oracle, first {ns} of the {N} packet ids of one pass on the "
-                         f"converged state, {ncpu} threads"}
+        while True:
+            orc.transport_pass(1, N + 1, ncpu)
+            orc.rates_compute()
+            orc.equilibrium_update()
+            npass += 1
+            dtc = time.perf_counter() - t0
+            if dtc >= args.cpu_seconds or npass >= 50:
+                break
+        cpu = {"value": npass * N / dtc, "unit": "packets/s", "cores": ncpu, "kind": "port",
+               "sample": f"FP64 C oracle, {npass} whole steps (pass of {N} packets + rates + "
+                         f"cell update) on the GPU-converged state, {ncpu} threads, "
+                         f"{dtc:.1f} s"}
         log(f"[cpu] {cpu}")
 
     fm = forms["mixed"]
@@ -154,14 +192,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"Stromgren sphere, pure H, uniform car {n}^3, box 8 pc, "
                                f"nH=100, 40000 K Q(H)=1e49, case B, T=1e4 K, "
-                               f"{N:.0e} Sobol packets/iteration (configs[0])",
+                               f"{N:.0e} Sobol packets/iteration",
                    "grid": f"{n}^3", "packets_per_iteration": N},
         "forms": forms,
         "error_mixed_vs_fp64": err,
         "roofline": {"bound": "hbm", "achieved": fm["walk_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": fm["walk_GBs"] / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_walk_car<MIXED>",
-                     "bytes_per_segment": BYTES_PER_SEGMENT,
+                     "kernel": "k_walk<MC_MIXED, kCarDDA>",
+                     "bytes_per_segment": BYTES_PER_SEGMENT["mixed"],
                      "segments_per_launch": fm["segments_per_pass"]},
         "cpu_baseline": cpu,
     }
