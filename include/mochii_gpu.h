@@ -46,6 +46,11 @@ extern "C" {
 #define MC_EXT_SPH 1
 #define MC_EXT_REC 2
 
+/* diffuse-packet energy models (par%diffuse_energy_model) */
+#define MC_DIFFUSE_THRESHOLD   0
+#define MC_DIFFUSE_EXPONENTIAL 1
+#define MC_DIFFUSE_MILNE       2
+
 /* leaf-state fields for mc_state_get / mc_state_set
  * (src/gas_state_mod.f90:23-28, rhokap from src/octree_mod.f90:50) */
 #define MC_NH     0
@@ -88,6 +93,9 @@ typedef struct mc_params {
     int32_t ion_add_dust;      /* par%ion_add_dust                             */
     int32_t ion_dust_scatter;  /* par%ion_dust_scatter                         */
     double  lambda_ref;        /* par%lambda_ref [um]                          */
+    int32_t diffuse_field;     /* par%diffuse_field: explicit ground-recombination
+                                  packets (forces case A)                      */
+    int32_t diffuse_model;     /* par%diffuse_energy_model: MC_DIFFUSE_*       */
 } mc_params;
 
 typedef struct mc_ctx mc_ctx;
@@ -127,12 +135,20 @@ int mc_ion_dust_table(mc_ctx *ctx, int n, const double *lam, const double *albed
 int mc_gas_opacity_fill(mc_ctx *ctx);
 
 /* ionizing_field_and_rates (src/ionizing_field_mod.f90:72-170): one transport
- * pass of all par%nphotons stellar packets, reduction, gas_rates_compute and
- * the energy-closure check (returns an error if closure > 5e-13, as
+ * pass of all par%nphotons stellar packets and, with diffuse_field on, of the
+ * diffuse ground-recombination packets built from the current state
+ * (diffuse_build + gen_diffuse_photon_qmc, src/diffuse_mod.f90:60-296, on
+ * the second Sobol stream), reduction, gas_rates_compute and the
+ * energy-closure check (returns an error if closure > 5e-13, as
  * ion_score_report_energy_closure aborts, src/ion_score_mod.f90:364-387).
  * with_peel is accepted for signature parity and must be 0 (peel-off imaging
  * is not on this path). */
 int mc_ionizing_field_and_rates(mc_ctx *ctx, int with_peel);
+
+/* Diffuse field of the last pass: luminosity diffuse_lum [erg/s] and packet
+ * count diffuse_nphot (src/diffuse_mod.f90:38-39, printed by
+ * ionizing_field_and_rates at src/ionizing_field_mod.f90:141-143). */
+int mc_diffuse_info(mc_ctx *ctx, double *lum, int64_t *nphot);
 
 /* gas_equilibrium_update (src/ion_balance_mod.f90:352-434). */
 int mc_gas_equilibrium_update(mc_ctx *ctx, double *max_dx, double *dx_vol);

@@ -112,6 +112,31 @@ static int setup_hot(mc_ctx *c)
     return 0;
 }
 
+// Milne ground-continuum tables for the diffuse field (milne_setup, built on
+// the host after the band is fixed) -> device
+static int upload_milne(mc_ctx *c)
+{
+    const size_t NT = MC_MILNE_NT, NX = MC_MILNE_NX;
+    const size_t ncdf = 3 * NT * NX, nrow = 3 * NT;
+    std::vector<double> h(ncdf + 2 * nrow + NX);
+    std::vector<double> lgT(NT);
+    mc_milne_tables(c->p.eion_max, h.data(), h.data() + ncdf, h.data() + ncdf + nrow,
+                    h.data() + ncdf + 2 * nrow, lgT.data());
+    if (dalloc(&c->d_milne, h.size())) return -100;
+    MC_HIP(hipMemcpy(c->d_milne, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    DevMilne &M = c->milne;
+    M.cdf = c->d_milne;
+    M.xbar = c->d_milne + ncdf;
+    M.fin = c->d_milne + ncdf + nrow;
+    M.xg = c->d_milne + ncdf + 2 * nrow;
+    M.lgT0 = lgT[0];
+    M.lgT1 = lgT[1];
+    M.eth[0] = MC_ETH_HI; M.eth[1] = MC_ETH_HEI; M.eth[2] = MC_ETH_HEII;
+    M.eion_max = c->p.eion_max;
+    M.model = c->p.diffuse_model;
+    return 0;
+}
+
 static void free_leaf_arrays(mc_ctx *c)
 {
     DevState &s = c->st;
@@ -127,6 +152,9 @@ static void free_leaf_arrays(mc_ctx *c)
     s.nH = s.xHI = s.xHeI = s.xHeII = s.ne = s.Te = s.rhokap = nullptr;
     s.opac = nullptr;
     s.hhe = s.jt = s.rates = s.partials = nullptr;
+    for (void *p : {(void *)c->d_dif, (void *)c->d_dcdf, (void *)c->d_bsum})
+        if (p) (void)hipFree(p);
+    c->d_dif = c->d_dcdf = c->d_bsum = nullptr;
 }
 
 static int alloc_leaf_arrays(mc_ctx *c)
@@ -139,6 +167,11 @@ static int alloc_leaf_arrays(mc_ctx *c)
         dalloc(&s.hhe, n * 8) || dalloc(&s.jt, n * (size_t)c->nnu) || dalloc(&s.rates, n * 6) ||
         dalloc(&s.partials, ((n + 255) / 256) * 6 + 6))
         return -100;
+    if (c->p.diffuse_field) {
+        const size_t nb = (n + 1023) / 1024;        // scan tiles of 1024 leaves
+        if (dalloc(&c->d_dif, 4 * n) || dalloc(&c->d_dcdf, n) || dalloc(&c->d_bsum, nb + 1))
+            return -100;
+    }
     MC_HIP(hipMemset(s.rhokap, 0, sizeof(double) * n));
     MC_HIP(hipMemset(s.hhe, 0, sizeof(double) * n * 8));
     MC_HIP(hipMemset(s.jt, 0, sizeof(double) * n * c->nnu));
@@ -168,8 +201,15 @@ extern "C" int mc_init(const mc_params *p, mc_ctx **out)
         mc_set_error("no HIP device available");
         return -2;
     }
+    if (p->diffuse_field && (p->diffuse_model < 0 || p->diffuse_model > 2)) {
+        mc_set_error("diffuse_model must be 0 (threshold), 1 (exponential) or 2 (milne)");
+        return -1;
+    }
     mc_ctx *c = new mc_ctx();
     c->p = *p;
+    // the explicit diffuse field replaces on-the-spot absorption: case A
+    // (src/setup.f90:580-583)
+    if (c->p.diffuse_field) c->p.case_A = 1;
     if (mc_setup_band(c) || mc_setup_samplers(c)) {
         delete c;
         return -1;
@@ -185,6 +225,10 @@ extern "C" int mc_init(const mc_params *p, mc_ctx **out)
         return -100;
     }
     if (dalloc(&c->st.ledger, 3) || dalloc(&c->st.segs, 2) || dalloc(&c->st.counter, 1)) {
+        mc_finalize(c);
+        return -100;
+    }
+    if (c->p.diffuse_field && upload_milne(c)) {
         mc_finalize(c);
         return -100;
     }
@@ -244,7 +288,7 @@ extern "C" int mc_finalize(mc_ctx *c)
                     (void *)c->pk.z, (void *)c->pk.kx, (void *)c->pk.ky, (void *)c->pk.kz,
                     (void *)c->pk.E, (void *)c->pk.sig, (void *)c->pk.inu, (void *)c->pk.leaf,
                     (void *)c->d_node_gd, (void *)c->d_node_gf, (void *)c->d_node_link,
-                    (void *)c->d_node_nbr, (void *)c->d_leaf_node})
+                    (void *)c->d_node_nbr, (void *)c->d_leaf_node, (void *)c->d_milne})
         if (p) (void)hipFree(p);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -534,6 +578,14 @@ extern "C" int mc_gas_thermal_update(mc_ctx *c, double *max_dx, double *max_dte,
     *max_dte = m[1];
     *dx_vol = m[2];
     *dte_vol = m[3];
+    return 0;
+}
+
+extern "C" int mc_diffuse_info(mc_ctx *c, double *lum, int64_t *nphot)
+{
+    CHECK_CTX(c);
+    if (lum) *lum = c->diffuse_lum;
+    if (nphot) *nphot = c->diffuse_nphot;
     return 0;
 }
 

@@ -22,6 +22,47 @@
 #define MC_ETH_HEII 54.416
 #define MC_TAU_HUGE 745.2
 
+// Ground-state photoionisation cross sections [cm^2] at E [eV]
+// (src/photo_xsec.f90:46-126): hydrogenic (Osterbrock & Ferland eq. 2.4) for
+// H I and He II, the Verner et al. (1996) fit for He I.  Host and device use
+// the same code (the Milne tables are built on the host).
+__host__ __device__ inline double sigma_hydrogenic(double E, double Eth, double Z)
+{
+    const double s0 = 6.30e-18 / (Z * Z);
+    if (E < Eth) return 0.0;
+    if (E == Eth) return s0;
+    const double eps = sqrt(E / Eth - 1.0), r = Eth / E;
+    return s0 * (((r * r) * r) * r) * exp(4.0 - 4.0 * atan(eps) / eps) /
+           (1.0 - exp(-2.0 * MC_PI / eps));
+}
+
+__host__ __device__ inline double sigma_HeI(double E)
+{
+    if (E < MC_ETH_HEI) return 0.0;
+    const double x = E / 13.61 - 0.4434, z = sqrt(x * x + 2.136 * 2.136);
+    const double Q = 5.5 - 0.5 * 3.188, xm = x - 1.0;
+    const double F = (xm * xm + 2.039 * 2.039) * pow(z, -Q) * pow(1.0 + sqrt(z / 1.469), -3.188);
+    return 949.2 * F * 1.0e-18;
+}
+
+// Milne ground-continuum tables (src/milne_recomb_spectrum_mod.f90):
+// energy-luminosity CDF in x = (E - E_th)/kT on MC_MILNE_NX quadratically
+// clustered nodes for MC_MILNE_NT log-spaced temperatures, per channel
+// (0 H I, 1 He I, 2 He II), plus <x> under the photon PDF and the in-band
+// photon fraction.
+#define MC_MILNE_NT 192
+#define MC_MILNE_NX 512
+struct DevMilne {
+    const double *cdf;       // [3][NT][NX]
+    const double *xbar;      // [3][NT]
+    const double *fin;       // [3][NT]
+    const double *xg;        // [NX]
+    double lgT0, lgT1;       // first two log10 T nodes
+    double eth[3];
+    double eion_max;
+    int32_t model;           // 0 threshold, 1 exponential, 2 milne
+};
+
 // Small launch/band tables shared by every kernel of a context
 // (read through the scalar/constant cache).
 struct DevTables {
@@ -130,6 +171,14 @@ struct mc_ctx {
     bool state_ready;
     DevCool cool;
     bool cool_ready;
+    // diffuse field: Milne tables, per-leaf emission {ch0, ch1, ch2, total},
+    // leaf CDF, block sums of the scan, totals of the last build
+    double *d_milne;          // cdf | xbar | fin | xg
+    DevMilne milne;
+    double *d_dif, *d_dcdf, *d_bsum;
+    int nbsum;
+    double diffuse_lum;
+    int64_t diffuse_nphot;
     // last pass results
     double ledger[3];
     int64_t segs[2];
@@ -144,6 +193,8 @@ int mc_build_octree_host(mc_ctx *c, int nleaf, const double *xl, const double *y
 void mc_qmc_tables(int seed, uint32_t V[32][MC_QMC_DIM], uint32_t key[MC_QMC_DIM][2]);
 void mc_set_error(const std::string &msg);
 void mc_gff_table(double logT[41], double gff[4][41]);
+void mc_milne_tables(double eion_max, double *cdf, double *xbar, double *fin, double *xg,
+                     double *lgT);
 
 // device work (mc_kernels.hip)
 int mc_run_pass(mc_ctx *c);

@@ -40,6 +40,10 @@ struct KArgs {
     PacketBuf pk;
     int64_t first_id;        // 1-based id of record 0
     int64_t npk;             // records in this batch
+    int32_t stream;          // 1 stellar, 2 diffuse (Sobol keys and post-launch draws)
+    const double *dif;       // per-leaf diffuse emission {ch0, ch1, ch2, total}
+    const double *dcdf;      // diffuse leaf CDF
+    DevMilne milne;
 };
 
 // ---------------------------------------------------------------------------
@@ -88,25 +92,6 @@ __device__ double sample_energy(const double *samp, int nk, int lo0, int hi0, do
         off = 2.0 * local / (y0 + disc);
     }
     return e[lo] + fmin(fmax(off, 0.0), dx);
-}
-
-__device__ double sigma_hydrogenic(double E, double Eth, double Z)
-{
-    const double s0 = 6.30e-18 / (Z * Z);
-    if (E < Eth) return 0.0;
-    if (E == Eth) return s0;
-    const double eps = sqrt(E / Eth - 1.0), r = Eth / E;
-    return s0 * (((r * r) * r) * r) * exp(4.0 - 4.0 * atan(eps) / eps) /
-           (1.0 - exp(-2.0 * MC_PI / eps));
-}
-
-__device__ double sigma_HeI(double E)
-{
-    if (E < MC_ETH_HEI) return 0.0;
-    const double x = E / 13.61 - 0.4434, z = sqrt(x * x + 2.136 * 2.136);
-    const double Q = 5.5 - 0.5 * 3.188, xm = x - 1.0;
-    const double F = (xm * xm + 2.039 * 2.039) * pow(z, -Q) * pow(1.0 + sqrt(z / 1.469), -3.188);
-    return 949.2 * F * 1.0e-18;
 }
 
 // dust optics at E from the sorted (lambda, albedo, g, C_ext) table
@@ -161,6 +146,38 @@ __device__ int find_leaf(const DevGrid &g, double x, double y, double z,
         const double4 n = c < nl ? sgd[c] : g.node_gd[c];
         c = link + (x >= n.x ? 1 : 0) + (y >= n.y ? 2 : 0) + (z >= n.z ? 4 : 0);
     }
+}
+
+// band bin of E (ion_bin_of, src/ion_band_mod.f90:624-646):
+// ion_eedge(bin) <= E < ion_eedge(bin+1), clamped to the band
+__device__ __forceinline__ int bin_of(const DevTables *T, double E)
+{
+    int lo = 0, hi = T->nnu - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (T->eedge[mid] <= E) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// launch record i: position, direction, energy, bin, cross sections at E
+// (build_ion_packet_physics), start leaf
+__device__ __forceinline__ void store_packet(const KArgs &a, int64_t i, double x, double y, double z,
+                                             double kx, double ky, double kz, double E, int leaf)
+{
+    const PacketBuf &P = a.pk;
+    P.x[i] = x; P.y[i] = y; P.z[i] = z;
+    P.kx[i] = kx; P.ky[i] = ky; P.kz[i] = kz;
+    P.E[i] = E;
+    P.inu[i] = bin_of(a.tab, E);
+    double sabs = 0.0, ssca = 0.0, gg = 0.0;
+    if (a.add_dust) dust_at(a, E, sabs, ssca, gg);
+    double *sg = P.sig + 6 * i;
+    sg[0] = sigma_hydrogenic(E, MC_ETH_HI, 1.0);
+    sg[1] = sigma_HeI(E);
+    sg[2] = sigma_hydrogenic(E, MC_ETH_HEII, 2.0);
+    sg[3] = sabs; sg[4] = ssca; sg[5] = gg;
+    P.leaf[i] = leaf;
 }
 
 __global__ __launch_bounds__(kBlock) void k_launch(KArgs a, int nlds)
@@ -257,25 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_launch(KArgs a, int nlds)
         }
     }
     const double E = sample_energy(a.samp, a.nknots, T->samp_off[comp], T->samp_off[comp + 1], ue);
-    // band bin: ion_eedge(bin) <= E < ion_eedge(bin+1), clamped
-    int lo = 0, hi = T->nnu - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (T->eedge[mid] <= E) lo = mid; else hi = mid - 1;
-    }
-    PacketBuf &P = a.pk;
-    P.x[i] = x; P.y[i] = y; P.z[i] = z;
-    P.kx[i] = kx; P.ky[i] = ky; P.kz[i] = kz;
-    P.E[i] = E;
-    P.inu[i] = lo;
-    double sabs = 0.0, ssca = 0.0, gg = 0.0;
-    if (a.add_dust) dust_at(a, E, sabs, ssca, gg);
-    double *sg = P.sig + 6 * i;
-    sg[0] = sigma_hydrogenic(E, MC_ETH_HI, 1.0);
-    sg[1] = sigma_HeI(E);
-    sg[2] = sigma_hydrogenic(E, MC_ETH_HEII, 2.0);
-    sg[3] = sabs; sg[4] = ssca; sg[5] = gg;
-    P.leaf[i] = find_leaf(a.g, x, y, z, sgd, slink, nlds);
+    store_packet(a, i, x, y, z, kx, ky, kz, E, find_leaf(a.g, x, y, z, sgd, slink, nlds));
 }
 
 // ---------------------------------------------------------------------------
@@ -289,11 +288,11 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z)
     return z;
 }
 
-__device__ __forceinline__ double rng_u(uint32_t seed, uint64_t id, uint32_t k)
+__device__ __forceinline__ double rng_u(uint32_t seed, int stream, uint64_t id, uint32_t k)
 {
     uint64_t h = mix64((uint64_t)seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull);
     h = mix64(h ^ id);
-    h = mix64(h ^ ((1ull << 32) | (uint64_t)k));   // stream 1 (stellar)
+    h = mix64(h ^ (((uint64_t)stream << 32) | (uint64_t)k));
     return ((double)(h >> 11) + 0.5) * (1.0 / 9007199254740992.0);
 }
 
@@ -673,20 +672,20 @@ __device__ __forceinline__ void scatter_event(const KArgs &a, Lane &L)
     const double kap = opacity_d(a, L, il);
     L.wgt = L.wgt * (a.s.rhokap[il] * L.dsca / fmax(kap, DBL_MIN));
     if (L.wgt < kWmin) {
-        if (rng_u(a.seed, L.id, L.nk++) > kPsurv) { L.phase = 0; return; }
+        if (rng_u(a.seed, a.stream, L.id, L.nk++) > kPsurv) { L.phase = 0; return; }
         L.wgt = L.wgt / kPsurv;
     }
     const double gg = L.dg;
     double cost;
     if (fabs(gg) > 1.0e-3) {
-        const double q = (1.0 - gg * gg) / (1.0 + gg * (2.0 * rng_u(a.seed, L.id, L.nk++) - 1.0));
+        const double q = (1.0 - gg * gg) / (1.0 + gg * (2.0 * rng_u(a.seed, a.stream, L.id, L.nk++) - 1.0));
         cost = (1.0 + gg * gg - q * q) / (2.0 * gg);
     } else {
-        cost = 2.0 * rng_u(a.seed, L.id, L.nk++) - 1.0;
+        cost = 2.0 * rng_u(a.seed, a.stream, L.id, L.nk++) - 1.0;
     }
     cost = fmin(fmax(cost, -1.0), 1.0);
     const double sint = sqrt(1.0 - cost * cost);
-    const double phi = MC_TWOPI * rng_u(a.seed, L.id, L.nk++);
+    const double phi = MC_TWOPI * rng_u(a.seed, a.stream, L.id, L.nk++);
     double kx = L.kx, ky = L.ky, kz = L.kz;
     double nrm = sqrt(kx * kx + ky * ky + kz * kz);
     if (nrm <= 0.0) { L.phase = 0; return; }
@@ -704,7 +703,7 @@ __device__ __forceinline__ void scatter_event(const KArgs &a, Lane &L)
     nrm = sqrt(nx * nx + ny * ny + nz * nz);
     L.kx = nx / nrm; L.ky = ny / nrm; L.kz = nz / nrm;
     L.nscatt++;
-    const double r = rng_u(a.seed, L.id, L.nk++);
+    const double r = rng_u(a.seed, a.stream, L.id, L.nk++);
     start_flight<P, G>(a, L, -log(fmax(r, DBL_MIN)));
 }
 
@@ -722,7 +721,7 @@ __device__ __forceinline__ void end_edge(const KArgs &a, Lane &L, double tau_edg
     if (!(a.add_dust && a.dust_scatter) || tau_edge <= 1.0e-12) { L.phase = 0; return; }
     const double w1 = 1.0 - exp(-tau_edge);
     L.wgt = L.wgt * w1;
-    const double tau = -log(1.0 - rng_u(a.seed, L.id, L.nk++) * w1);
+    const double tau = -log(1.0 - rng_u(a.seed, a.stream, L.id, L.nk++) * w1);
     if constexpr (G == kFaceExit) {
         L.leaf = L.leaf0;
         L.node = leaf_node_of(a.g, L.leaf0);
@@ -1138,6 +1137,200 @@ __device__ __forceinline__ void block_reduce(double (&v)[N], bool (&is_max)[N], 
         }
 }
 
+// ---------------------------------------------------------------------------
+// explicit diffuse field (src/diffuse_mod.f90, src/milne_recomb_spectrum_mod.f90)
+
+// temperature bracket of the Milne tables (milne_tbracket): node it, weight w
+__device__ __forceinline__ void milne_tbracket(const DevMilne &M, double T, int &it, double &w)
+{
+    double s = (log10(fmax(T, DBL_MIN)) - M.lgT0) / (M.lgT1 - M.lgT0);
+    s = fmin(fmax(s, 0.0), (double)(MC_MILNE_NT - 1));
+    const int i = min((int)s + 1, MC_MILNE_NT - 1);
+    w = s - (double)(i - 1);
+    it = i - 1;
+}
+
+// in-band energy radiated per ground-state recombination [eV]
+// (milne_energy_per_recomb) or the simpler models of ground_mean_energy
+__device__ double ground_mean_energy(const DevMilne &M, int ich, double T)
+{
+    const double kT = MC_KBOLTZ * T / MC_EV2ERG;
+    if (M.model == 1) return M.eth[ich] + kT;
+    if (M.model != 2) return M.eth[ich];
+    int it;
+    double w;
+    milne_tbracket(M, T, it, w);
+    const double *xb = M.xbar + ich * MC_MILNE_NT, *fi = M.fin + ich * MC_MILNE_NT;
+    const double xbar = (1.0 - w) * xb[it] + w * xb[it + 1];
+    const double fkeep = (1.0 - w) * fi[it] + w * fi[it + 1];
+    return fkeep * (M.eth[ich] + xbar * kT);
+}
+
+// photon energy of one diffuse packet (ground_photon_energy with
+// milne_sample_energy): the energy CDF interpolated in log T, inverted once
+__device__ double ground_photon_energy(const DevMilne &M, int ich, double T, double u)
+{
+    const double kT = MC_KBOLTZ * T / MC_EV2ERG;
+    double E;
+    if (M.model == 2) {
+        int it;
+        double w;
+        milne_tbracket(M, T, it, w);
+        const double *c0 = M.cdf + ((size_t)ich * MC_MILNE_NT + it) * MC_MILNE_NX;
+        const double *c1 = c0 + MC_MILNE_NX;
+        const double uu = fmin(fmax(u, 0.0), 1.0);
+        int lo = 0, hi = MC_MILNE_NX - 1;
+        double ca = (1.0 - w) * c0[lo] + w * c1[lo];
+        double cb = (1.0 - w) * c0[hi] + w * c1[hi];
+        while (hi - lo > 1) {
+            const int mid = (lo + hi + 2) / 2 - 1;
+            const double cm = (1.0 - w) * c0[mid] + w * c1[mid];
+            if (uu > cm) { lo = mid; ca = cm; } else { hi = mid; cb = cm; }
+        }
+        const double x = cb > ca ? M.xg[lo] + ((M.xg[hi] - M.xg[lo]) * (uu - ca)) / (cb - ca)
+                                 : M.xg[lo];
+        E = fmin(M.eth[ich] + x * kT, M.eion_max * (1.0 - 1.0e-12));
+    } else if (M.model == 1) {
+        E = M.eth[ich] + kT * (-log(fmax(u, DBL_MIN)));
+    } else {
+        E = M.eth[ich];
+    }
+    return fmin(E, M.eion_max * 0.999);
+}
+
+// diffuse_build, channels 1-3: per-leaf energy luminosity of the ground
+// continua n_e n_ion alpha_1(T) E_rec(T) V [erg/s] and the leaf total
+__global__ __launch_bounds__(kBlock) void k_diffuse_build(DevGrid g, DevState s, DevMilne M,
+                                                          double y, double d2cm, double *dif)
+{
+#pragma clang fp contract(off)
+    const int il = blockIdx.x * kBlock + threadIdx.x;
+    if (il >= g.nleaf) return;
+    double *o = dif + (size_t)il * 4;
+    const double nH = s.nH[il];
+    if (nH <= 0.0) { o[0] = o[1] = o[2] = o[3] = 0.0; return; }
+    const double T = s.Te[il], ne = s.ne[il];
+    const double sd = (2.0 * leaf_half(g, il)) * d2cm, vol = (sd * sd) * sd;
+    const double x3 = fmax(0.0, (1.0 - s.xHeI[il]) - s.xHeII[il]);
+    const double nHII = nH * (1.0 - s.xHI[il]);
+    const double nHeII = (nH * y) * s.xHeII[il], nHeIII = (nH * y) * x3;
+    const RateSet rA = rates_at(T, true), rB = rates_at(T, false);
+    const double a1 = rA.aH - rB.aH, a2 = rA.aHe2 - rB.aHe2, a3 = rA.aHe3 - rB.aHe3;
+    o[0] = ((((ne * nHII) * a1) * ground_mean_energy(M, 0, T)) * MC_EV2ERG) * vol;
+    o[1] = ((((ne * nHeII) * a2) * ground_mean_energy(M, 1, T)) * MC_EV2ERG) * vol;
+    o[2] = ((((ne * nHeIII) * a3) * ground_mean_energy(M, 2, T)) * MC_EV2ERG) * vol;
+    o[3] = ((o[0] + o[1]) + o[2]) + 0.0;
+}
+
+// leaf CDF of the diffuse emission: inclusive scan of the leaf totals in
+// three passes (tile scan of kScanTile leaves per block -> scan of the tile
+// sums in one block -> offset and normalise)
+constexpr int kScanTile = 4 * kBlock;
+
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const double *dif, int n, double *cdf,
+                                                       double *bsum)
+{
+    __shared__ double sh[kBlock];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * 4;
+    double v[4], run = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        const int i = base + k;
+        run += i < n ? dif[(size_t)i * 4 + 3] : 0.0;
+        v[k] = run;
+    }
+    sh[threadIdx.x] = run;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {     // Hillis-Steele over thread sums
+        const double t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0.0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    const double before = threadIdx.x > 0 ? sh[threadIdx.x - 1] : 0.0;
+    for (int k = 0; k < 4; ++k) {
+        const int i = base + k;
+        if (i < n) cdf[i] = before + v[k];
+    }
+    if (threadIdx.x == kBlock - 1) bsum[blockIdx.x] = sh[kBlock - 1];
+}
+
+// exclusive scan of the nb tile sums in place; total into bsum[nb]
+__global__ __launch_bounds__(1024) void k_scan_sums(double *bsum, int nb)
+{
+    __shared__ double sh[1024];
+    const int per = (nb + 1023) / 1024, lo = threadIdx.x * per, hi = min(lo + per, nb);
+    double run = 0.0;
+    for (int i = lo; i < hi; ++i) run += bsum[i];
+    sh[threadIdx.x] = run;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const double t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0.0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    double acc = threadIdx.x > 0 ? sh[threadIdx.x - 1] : 0.0;
+    for (int i = lo; i < hi; ++i) {
+        const double v = bsum[i];
+        bsum[i] = acc;
+        acc += v;
+    }
+    if (threadIdx.x == 1023) bsum[nb] = sh[1023];
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply(double *cdf, int n, const double *bsum,
+                                                       int nb)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double tot = bsum[nb];
+    const double v = cdf[i] + bsum[i / kScanTile];
+    cdf[i] = tot > 0.0 ? v / tot : 0.0;
+}
+
+// gen_diffuse_photon_qmc (src/diffuse_mod.f90:221-296): 9 Sobol uniforms of
+// the diffuse stream; emitting leaf from the leaf CDF, uniform position in
+// it, isotropic direction, channel by luminosity, Milne energy at the leaf T
+__global__ __launch_bounds__(kBlock) void k_launch_diffuse(KArgs a)
+{
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.npk) return;
+    const DevTables *T = a.tab;
+    const uint32_t q = (uint32_t)(a.first_id + i - 1);
+    double u[9];
+    for (int d = 0; d < 9; ++d) u[d] = qmc_u(T, q, d, 1);
+    const DevGrid &g = a.g;
+    int lo = 0, hi = g.nleaf - 1;                  // first leaf with cdf >= u
+    while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (a.dcdf[mid] < u[0]) lo = mid + 1; else hi = mid;
+    }
+    const int il = lo;
+    double cx, cy, cz, half;
+    if (g.car) {
+        const int ix = il % g.nx, iy = (il / g.nx) % g.ny, iz = il / (g.nx * g.ny);
+        cx = g.xmin + ((double)ix + 0.5) * g.dcell;
+        cy = g.ymin + ((double)iy + 0.5) * g.dcell;
+        cz = g.zmin + ((double)iz + 0.5) * g.dcell;
+        half = 0.5 * g.dcell;
+    } else {
+        const double4 n = g.node_gd[g.leaf_node[il]];
+        cx = n.x; cy = n.y; cz = n.z; half = n.w;
+    }
+    const double x = cx + (2.0 * u[1] - 1.0) * half;
+    const double y = cy + (2.0 * u[2] - 1.0) * half;
+    const double z = cz + (2.0 * u[3] - 1.0) * half;
+    const double cost = 2.0 * u[4] - 1.0, sint = sqrt(1.0 - cost * cost);
+    double sp, cp;
+    sincos(MC_TWOPI * u[5], &sp, &cp);
+    const double *w = a.dif + (size_t)il * 4;
+    const double uu = u[6] * (((w[0] + w[1]) + w[2]) + 0.0);
+    const int ch = uu <= w[0] ? 0 : (uu <= w[0] + w[1] ? 1 : 2);
+    const double E = ground_photon_energy(a.milne, ch, a.s.Te[il], u[7]);
+    store_packet(a, i, x, y, z, sint * cp, sint * sp, cost, E, il);
+}
+
 // gas_equilibrium_update at fixed T; partial metrics per block:
 // {max|dx|, sum V|dx|, sum V x_HII}
 __global__ __launch_bounds__(kBlock) void k_equilibrium(DevGrid g, DevState s, double T, bool caseA,
@@ -1387,14 +1580,18 @@ KArgs make_args(mc_ctx *c)
     a.pk = c->pk;
     a.first_id = 1;
     a.npk = 0;
+    a.stream = 1;
+    a.dif = c->d_dif;
+    a.dcdf = c->d_dcdf;
+    a.milne = c->milne;
     return a;
 }
 
 int persistent_blocks(const void *kern)
 {
     int dev = 0, ncu = 0, per = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, kBlock, 0) != hipSuccess || per < 1)
         per = 2;
     return ncu * per;
@@ -1440,13 +1637,18 @@ int mc_run_pass(mc_ctx *c)
                           : (mixed ? (const void *)k_walk<MC_MIXED, kCarDDA>
                                    : (const void *)k_walk<MC_FP64, kCarDDA>);
     const int nblk = persistent_blocks(kw);
-    for (int64_t first = 1; first <= N; first += c->pk.cap) {
+    const int nlds = c->car ? 0 : c->n_lds_nodes;
+    // one batch of packet records: launch kernel, then the persistent walk
+    auto batch = [&](bool diffuse, int64_t first, int64_t n) -> int {
         a.first_id = first;
-        a.npk = std::min<int64_t>(c->pk.cap, N - first + 1);
+        a.npk = n;
+        a.stream = diffuse ? 2 : 1;
         MC_HIP(hipEventRecord(c->ev[1], st));
-        const int nlds = c->car ? 0 : c->n_lds_nodes;
-        k_launch<<<(unsigned)((a.npk + kBlock - 1) / kBlock), kBlock,
-                   (size_t)nlds * (sizeof(double4) + sizeof(int32_t)), st>>>(a, nlds);
+        if (diffuse)
+            k_launch_diffuse<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(a);
+        else
+            k_launch<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock,
+                       (size_t)nlds * (sizeof(double4) + sizeof(int32_t)), st>>>(a, nlds);
         MC_HIP(hipGetLastError());
         k_reset_counter<<<1, 1, 0, st>>>(c->st.counter);
         MC_HIP(hipEventRecord(c->ev[2], st));
@@ -1461,10 +1663,44 @@ int mc_run_pass(mc_ctx *c)
         MC_HIP(hipEventRecord(c->ev[3], st));
         MC_HIP(hipEventSynchronize(c->ev[3]));
         float ms = 0.0f;
-        hipEventElapsedTime(&ms, c->ev[1], c->ev[2]);
+        (void)hipEventElapsedTime(&ms, c->ev[1], c->ev[2]);
         t_launch += ms;
-        hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
+        (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
         t_walk += ms;
+        return 0;
+    };
+    for (int64_t first = 1; first <= N; first += c->pk.cap) {
+        const int rc = batch(false, first, std::min<int64_t>(c->pk.cap, N - first + 1));
+        if (rc) return rc;
+    }
+    c->diffuse_lum = 0.0;
+    c->diffuse_nphot = 0;
+    if (c->p.diffuse_field) {
+        // emission from the current state (diffuse_build), leaf CDF, count
+        const int nl1 = c->nleaf;
+        MC_HIP(hipEventRecord(c->ev[1], st));
+        k_diffuse_build<<<(nl1 + kBlock - 1) / kBlock, kBlock, 0, st>>>(
+            c->grid, c->st, c->milne, c->p.He_abund, c->p.distance2cm, c->d_dif);
+        MC_HIP(hipGetLastError());
+        const int nb = (nl1 + kScanTile - 1) / kScanTile;
+        k_scan_tiles<<<nb, kBlock, 0, st>>>(c->d_dif, nl1, c->d_dcdf, c->d_bsum);
+        k_scan_sums<<<1, 1024, 0, st>>>(c->d_bsum, nb);
+        k_scan_apply<<<(nl1 + kBlock - 1) / kBlock, kBlock, 0, st>>>(c->d_dcdf, nl1, c->d_bsum, nb);
+        MC_HIP(hipGetLastError());
+        double Ld = 0.0;
+        MC_HIP(hipMemcpyAsync(&Ld, c->d_bsum + nb, sizeof(double), hipMemcpyDeviceToHost, st));
+        MC_HIP(hipEventRecord(c->ev[2], st));
+        MC_HIP(hipStreamSynchronize(st));
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, c->ev[1], c->ev[2]);
+        t_launch += ms;
+        const double lpk = c->Ltot / (double)N;
+        c->diffuse_lum = Ld;
+        c->diffuse_nphot = Ld > 0.0 ? (int64_t)std::llround(Ld / lpk) : 0;
+        for (int64_t first = 1; first <= c->diffuse_nphot; first += c->pk.cap) {
+            const int rc = batch(true, first, std::min<int64_t>(c->pk.cap, c->diffuse_nphot - first + 1));
+            if (rc) return rc;
+        }
     }
     if (c->st.nhot > 0) {
         const int64_t nf = (int64_t)c->st.nhot * (8 + c->nnu);
@@ -1493,7 +1729,7 @@ int mc_run_rates(mc_ctx *c)
     MC_HIP(hipEventRecord(c->ev[5], st));
     MC_HIP(hipEventSynchronize(c->ev[5]));
     float ms = 0.0f;
-    hipEventElapsedTime(&ms, c->ev[4], c->ev[5]);
+    (void)hipEventElapsedTime(&ms, c->ev[4], c->ev[5]);
     c->times[2] = ms;
     return 0;
 }
@@ -1510,7 +1746,7 @@ int mc_run_equilibrium(mc_ctx *c, double *max_dx, double *dx_vol)
     double m[3];
     if (sum_partials(c, nblk, m)) return -100;
     float ms = 0.0f;
-    hipEventElapsedTime(&ms, c->ev[6], c->ev[7]);
+    (void)hipEventElapsedTime(&ms, c->ev[6], c->ev[7]);
     c->times[3] = ms;
     *max_dx = m[0];
     *dx_vol = m[1] / std::fmax(m[2], DBL_MIN);
@@ -1583,7 +1819,7 @@ int mc_run_trace(mc_ctx *c, int precision, const double *pos, const double *dir,
     MC_HIP(hipMemcpyAsync(n, dn, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     MC_HIP(hipStreamSynchronize(c->stream));
     MC_HIP(hipMemcpy(cells, dcells, sizeof(int32_t) * (size_t)(*n), hipMemcpyDeviceToHost));
-    hipFree(dcells);
-    hipFree(dn);
+    (void)hipFree(dcells);
+    (void)hipFree(dn);
     return 0;
 }

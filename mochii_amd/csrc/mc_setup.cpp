@@ -2,6 +2,7 @@
 // source samplers, the Sobol/Owen tables and the octree build.  Runs once per
 // context (or per grid), so it is plain FP64 C++ on the host.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -423,6 +424,80 @@ void mc_gff_table(double logT[41], double gff[4][41])
             double s = 0.0;
             for (int i = 0; i < nu; ++i) s += (double)g[i] * std::exp(-u[i]) * du[i];
             gff[iz - 1][it] = s / (std::exp(-1.0e-4) - std::exp(-20.0));
+        }
+    }
+}
+
+// ---- Milne ground-continuum tables ------------------------------------------
+// milne_setup (src/milne_recomb_spectrum_mod.f90:124-206): for each channel
+// and temperature node, cumulative Simpson of the photon emissivity
+// n(x) = E^2 sigma(E) e^-x and of E n(x) over x nodes 60 t^2 (t uniform on
+// [0,1]), truncated at the band top; the energy CDF is normalised to 1, and
+// <x> and the in-band photon fraction (against the integral to x = 60) are
+// kept for the energy radiated per recombination.
+#pragma clang fp contract(off)
+static double milne_shape(int ich, double eth, double x, double kT)
+{
+    const double E = eth + x * kT;
+    const double s = ich == 0 ? sigma_hydrogenic(E, MC_ETH_HI, 1.0)
+                              : (ich == 1 ? sigma_HeI(E) : sigma_hydrogenic(E, MC_ETH_HEII, 2.0));
+    return ((E * E) * s) * std::exp(-x);
+}
+
+void mc_milne_tables(double eion_max, double *cdf, double *xbar, double *fin, double *xg,
+                     double *lgT)
+{
+    const int NT = MC_MILNE_NT, NX = MC_MILNE_NX;
+    const double xmax = 60.0;
+    const double eth[3] = {MC_ETH_HI, MC_ETH_HEI, MC_ETH_HEII};
+    for (int i = 0; i < NX; ++i) {
+        const double t = (double)i / (double)(NX - 1);
+        xg[i] = (xmax * t) * t;
+    }
+    const double l0 = std::log10(1.0e2), l1 = std::log10(1.0e6);
+    for (int i = 0; i < NT; ++i) lgT[i] = l0 + ((l1 - l0) * (double)i) / (double)(NT - 1);
+    for (int ich = 0; ich < 3; ++ich) {
+        for (int it = 0; it < NT; ++it) {
+            double *row = cdf + ((size_t)ich * NT + it) * NX;
+            const double kT = MC_KBOLTZ * std::pow(10.0, lgT[it]) / MC_EV2ERG;
+            const double xhi = std::min(xmax, (eion_max - eth[ich]) / kT);
+            double &xb_out = xbar[ich * NT + it], &fin_out = fin[ich * NT + it];
+            if (xhi <= 0.0) {
+                for (int i = 0; i < NX; ++i) row[i] = 1.0;
+                xb_out = 0.0;
+                fin_out = 0.0;
+                continue;
+            }
+            double sn = 0.0, se = 0.0, sxn = 0.0;
+            row[0] = 0.0;
+            for (int i = 0; i + 1 < NX; ++i) {
+                const double xa = xg[i], xb = std::min(xg[i + 1], xhi);
+                if (xb > xa) {
+                    const double xm = 0.5 * (xa + xb);
+                    const double fa = milne_shape(ich, eth[ich], xa, kT);
+                    const double fm = milne_shape(ich, eth[ich], xm, kT);
+                    const double fb = milne_shape(ich, eth[ich], xb, kT);
+                    const double h6 = (xb - xa) / 6.0;
+                    sn = sn + h6 * ((fa + 4.0 * fm) + fb);
+                    se = se + h6 * ((fa * (eth[ich] + xa * kT) + (4.0 * fm) * (eth[ich] + xm * kT)) +
+                                    fb * (eth[ich] + xb * kT));
+                    sxn = sxn + h6 * ((fa * xa + (4.0 * fm) * xm) + fb * xb);
+                }
+                row[i + 1] = se;
+            }
+            double tail = sn;
+            for (double xa = xhi; xa < xmax;) {
+                const double xb = std::min(xa + 0.25, xmax), xm = 0.5 * (xa + xb);
+                tail = tail + ((xb - xa) / 6.0) *
+                                  ((milne_shape(ich, eth[ich], xa, kT) +
+                                    4.0 * milne_shape(ich, eth[ich], xm, kT)) +
+                                   milne_shape(ich, eth[ich], xb, kT));
+                xa = xb;
+            }
+            for (int i = 0; i < NX; ++i) row[i] = se > 0.0 ? row[i] / se : 1.0;
+            row[NX - 1] = 1.0;
+            xb_out = sxn / std::max(sn, DBL_MIN);
+            fin_out = sn / std::max(tail, DBL_MIN);
         }
     }
 }

@@ -7,6 +7,7 @@ MC_MAXSRC = 16
 MC_FP64, MC_MIXED = 0, 1
 MC_GRID_CAR, MC_GRID_AMR = 0, 1
 MC_EXT_SPH, MC_EXT_REC = 1, 2
+MC_DIFFUSE_THRESHOLD, MC_DIFFUSE_EXPONENTIAL, MC_DIFFUSE_MILNE = 0, 1, 2
 FIELDS = {"nH": 0, "xHI": 1, "xHeI": 2, "xHeII": 3, "ne": 4, "Te": 5, "rhokap": 6}
 
 
@@ -37,6 +38,7 @@ class Params(C.Structure):
         ("xHI_init", C.c_double), ("xHeI_init", C.c_double), ("xHeII_init", C.c_double),
         ("ion_add_dust", C.c_int32), ("ion_dust_scatter", C.c_int32),
         ("lambda_ref", C.c_double),
+        ("diffuse_field", C.c_int32), ("diffuse_model", C.c_int32),
     ]
 
     # reference defaults (src/define.f90) for the fields this path reads
@@ -48,7 +50,8 @@ class Params(C.Structure):
                     distance2cm=3.0856776e21, He_abund=0.1, te_fixed=1e4, ion_relax=1.0,
                     te_min=1e3, te_max=5e4, gas_tol=1e-3, gas_tol_te=1e-3, case_A=0,
                     solve_te=0, conv_vol=0, gas_niter=0, xHI_init=1.0, xHeI_init=1.0,
-                    xHeII_init=0.0, ion_add_dust=0, ion_dust_scatter=0, lambda_ref=0.55)
+                    xHeII_init=0.0, ion_add_dust=0, ion_dust_scatter=0, lambda_ref=0.55,
+                    diffuse_field=0, diffuse_model=MC_DIFFUSE_MILNE)
 
     @classmethod
     def make(cls, **kw):
@@ -95,6 +98,7 @@ def library():
         "mc_ionizing_field_and_rates": ([vp, C.c_int], C.c_int),
         "mc_gas_equilibrium_update": ([vp, dp, dp], C.c_int),
         "mc_hydrogen_balance_residual_grid": ([vp, dp, dp], C.c_int),
+        "mc_diffuse_info": ([vp, dp, i64p], C.c_int),
         "mc_cooling_setup": ([vp, C.c_int, dp, dp], C.c_int),
         "mc_gas_thermal_update": ([vp, dp, dp, dp, dp], C.c_int),
         "mc_tallies_get": ([vp, dp, dp, dp, i64p], C.c_int),

@@ -78,6 +78,12 @@ class Model:
         check(self._L.mc_ionizing_field_and_rates(self._h, int(bool(with_peel))),
               "mc_ionizing_field_and_rates")
 
+    def diffuse_info(self):
+        """(diffuse_lum [erg/s], diffuse_nphot) of the last pass."""
+        L, n = C.c_double(), C.c_int64()
+        check(self._L.mc_diffuse_info(self._h, C.byref(L), C.byref(n)), "mc_diffuse_info")
+        return L.value, n.value
+
     def gas_equilibrium_update(self):
         a, b = C.c_double(), C.c_double()
         check(self._L.mc_gas_equilibrium_update(self._h, C.byref(a), C.byref(b)),

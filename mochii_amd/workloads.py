@@ -35,11 +35,12 @@ def stromgren(n=64, nphotons=1_000_000, precision=MC_FP64, nH=100.0, half=4.0,
 
 
 def hii_thermal(n=32, nphotons=200_000, precision=MC_FP64, seed=12345, niter=100,
-                tol=2e-3, tol_te=2e-3, half=8.0, octree=None, nH=None):
+                tol=2e-3, tol_te=2e-3, half=8.0, octree=None, nH=None, diffuse=False):
     """H + He (He/H = 0.1) H II region with thermal balance: one 45000 K
     blackbody at the box centre with Q(H) = 3e49 s^-1, case A, T_e solved in
-    [1e3, 5e4] K, volume convergence criterion.  Uniform car grid unless a
-    leaf list (octree) with per-leaf nH is given."""
+    [1e3, 5e4] K, volume convergence criterion; with diffuse=True explicit
+    diffuse ground-recombination packets (Milne spectra).  Uniform car grid
+    unless a leaf list (octree) with per-leaf nH is given."""
     lum = 3.0 * L_Q49_T40K * _mean_energy_ratio(4.5e4, 4.0e4)
     params = dict(nphotons=nphotons, qmc_seed=seed, precision=precision, eion_min=13.598,
                   eion_max=100.0, nnu_ion=16, ion_align_edges=1, nsource=1,
@@ -48,7 +49,8 @@ def hii_thermal(n=32, nphotons=200_000, precision=MC_FP64, seed=12345, niter=100
                   xmax=half, ymax=half, zmax=half, distance2cm=PC2CM, He_abund=0.1,
                   te_fixed=1.0e4, ion_relax=1.0, te_min=1.0e3, te_max=5.0e4, gas_tol=tol,
                   gas_tol_te=tol_te, conv_vol=1, gas_niter=niter, case_A=1, solve_te=1,
-                  xHI_init=1.0, xHeI_init=1.0, xHeII_init=0.0)
+                  xHI_init=1.0, xHeI_init=1.0, xHeII_init=0.0, diffuse_field=int(diffuse),
+                  diffuse_model=2)
     if nH is None:
         nH = np.full(n ** 3, 100.0)
     return params, nH

@@ -124,7 +124,25 @@ typedef struct {
     /* dust */
     int     add_dust, dust_scatter;
     double  lambda_ref;
+    /* explicit diffuse ground-recombination field (par%diffuse_field,
+     * par%diffuse_energy_model: 0 threshold, 1 exponential, 2 milne) */
+    int     diffuse_field, diffuse_model;
 } orc_params;
+
+/* ---- Milne ground-continuum tables (src/milne_recomb_spectrum_mod.f90) -- */
+#define ORC_MILNE_NT 192
+#define ORC_MILNE_NX 512
+typedef struct {
+    double lgT[ORC_MILNE_NT], xg[ORC_MILNE_NX];
+    double cdf[3][ORC_MILNE_NT][ORC_MILNE_NX];   /* energy-luminosity CDF in x */
+    double xbar[3][ORC_MILNE_NT];                /* <x> under the photon PDF   */
+    double fin[3][ORC_MILNE_NT];                 /* in-band photon fraction    */
+    double eth[3], eion_max;
+} orc_milne;
+
+void   orc_milne_setup(orc_milne *m, double eion_max);
+double orc_milne_energy_per_recomb(const orc_milne *m, int ich, double T);
+double orc_milne_sample_energy(const orc_milne *m, int ich, double T, double u);
 
 typedef struct {
     orc_params p;
@@ -156,6 +174,11 @@ typedef struct {
     double *gHI, *gHeI, *gHeII, *hHI, *hHeI, *hHeII;
     /* cooling tables */
     orc_cooling cool;
+    /* diffuse field: Milne tables, per-leaf channel luminosities dif_ch
+     * (leaf-major, 3 per leaf), leaf CDF, totals of the last build */
+    orc_milne *milne;
+    double *dif_ch, *dif_cdf, diffuse_lum;
+    int64_t diffuse_nphot;
 } orc_ctx;
 
 orc_ctx *orc_create(const orc_params *p);
@@ -191,6 +214,12 @@ int      orc_trace_ray(orc_ctx *c, double x, double y, double z, double kx,
                        double ky, double kz, int *cells, double *segs, int maxn);
 /* diagnostics: launch state of one stellar packet */
 int      orc_launch(orc_ctx *c, int64_t id, double *out9);
+
+/* diffuse field (src/diffuse_mod.f90): tables, per-pass emission build and
+ * one diffuse launch (diagnostic; 1-based id on stream 2) */
+int      orc_diffuse_setup(orc_ctx *c);
+void     orc_diffuse_build(orc_ctx *c);
+int      orc_launch_diffuse(orc_ctx *c, int64_t id, double *out9);
 
 #ifdef __cplusplus
 }

@@ -574,6 +574,65 @@ static void launch(const orc_ctx *c, int64_t id, pkt_t *ph)
     ph->nk = 0;
 }
 
+/* gen_diffuse_photon_qmc (src/diffuse_mod.f90:221-296) with
+ * ground_photon_energy (:341-355); channels 1-3, 9 uniforms on stream 2 */
+static void launch_diffuse(const orc_ctx *c, int64_t id, pkt_t *ph)
+{
+    double u[ORC_QMC_MAXDIM];
+    orc_qmc_uniforms(&c->qmc, (uint64_t)(id - 1), 9, 2, u);
+    int lo = 1, hi = c->nleaf;
+    while (lo < hi) {
+        int mid = (lo + hi) / 2;
+        if (c->dif_cdf[mid - 1] < u[0]) lo = mid + 1; else hi = mid;
+    }
+    const int il = lo;
+    double cx, cy, cz, half;
+    cell_geom(c, leaf_cell(c, il), &cx, &cy, &cz, &half);
+    ph->x = cx + (2.0 * u[1] - 1.0) * half;
+    ph->y = cy + (2.0 * u[2] - 1.0) * half;
+    ph->z = cz + (2.0 * u[3] - 1.0) * half;
+    double cost = 2.0 * u[4] - 1.0, sint = sqrt(1.0 - cost * cost), phi = ORC_TWOPI * u[5];
+    ph->kx = sint * cos(phi);
+    ph->ky = sint * sin(phi);
+    ph->kz = cost;
+    const double *w = c->dif_ch + (size_t)(il - 1) * 3;
+    double uu = u[6] * (((w[0] + w[1]) + w[2]) + 0.0);
+    int ch;
+    double eph;
+    if (uu <= w[0]) { ch = 0; eph = ORC_ETH_HI; }
+    else if (uu <= w[0] + w[1]) { ch = 1; eph = ORC_ETH_HEI; }
+    else { ch = 2; eph = ORC_ETH_HEII; }
+    const double T = c->Te[il - 1];
+    if (c->p.diffuse_model == 2) {
+        eph = orc_milne_sample_energy(c->milne, ch, T, u[7]);
+    } else if (c->p.diffuse_model == 1) {
+        double kT = ORC_KBOLTZ * T / ORC_EV2ERG;
+        eph = eph + kT * (-log(u[7] > DBL_MIN ? u[7] : DBL_MIN));
+    }
+    if (eph > c->p.eion_max * 0.999) eph = c->p.eion_max * 0.999;
+    ph->inu = orc_bin_of(c, eph);
+    ph->wgt = 1.0;
+    ph->Lpacket = c->Ltot / (double)c->p.nphotons;
+    ph->E = eph;
+    ph->nscatt = 0;
+    ph->inside = 1;
+    ph->icell = il;
+    ph->id = id;
+    ph->istream = 2;
+    ph->nk = 0;
+}
+
+int orc_launch_diffuse(orc_ctx *c, int64_t id, double *out)
+{
+    pkt_t ph;
+    memset(&ph, 0, sizeof(ph));
+    launch_diffuse(c, id, &ph);
+    out[0] = ph.x; out[1] = ph.y; out[2] = ph.z;
+    out[3] = ph.kx; out[4] = ph.ky; out[5] = ph.kz;
+    out[6] = ph.E; out[7] = (double)ph.inu; out[8] = (double)ph.icell;
+    return 0;
+}
+
 int orc_launch(orc_ctx *c, int64_t id, double *out)
 {
     pkt_t ph;
@@ -589,7 +648,8 @@ int orc_launch(orc_ctx *c, int64_t id, double *out)
 typedef struct {
     const orc_ctx *c;
     tally_t t;
-    int64_t first, last;
+    int64_t first, last;      /* stellar ids [first, last) */
+    int64_t dfirst, dlast;    /* diffuse ids [dfirst, dlast) */
 } work_t;
 
 static void *worker(void *arg)
@@ -600,15 +660,27 @@ static void *worker(void *arg)
         launch(w->c, id, &ph);
         transport(w->c, &w->t, &ph);
     }
+    for (int64_t id = w->dfirst; id < w->dlast; ++id) {
+        pkt_t ph;
+        launch_diffuse(w->c, id, &ph);
+        transport(w->c, &w->t, &ph);
+    }
     return NULL;
 }
 
-/* ionizing_field_and_rates, stellar part (src/ionizing_field_mod.f90:90-136,
- * 163-164): zero tallies, transport ids [first,last), reduce */
+/* ionizing_field_and_rates (src/ionizing_field_mod.f90:90-164): zero
+ * tallies, transport stellar ids [first,last) and, with the diffuse field on,
+ * all diffuse packets of the emission built from the current state, reduce */
 int orc_transport_pass(orc_ctx *c, int64_t first, int64_t last, int nthreads)
 {
     int nl = c->nleaf, nnu = c->nnu_band;
     if (nthreads < 1) nthreads = 1;
+    int64_t nd = 0;
+    if (c->p.diffuse_field) {
+        if (!c->milne && orc_diffuse_setup(c)) return -1;
+        orc_diffuse_build(c);
+        nd = c->diffuse_nphot;
+    }
     work_t *w = (work_t *)calloc(nthreads, sizeof(work_t));
     pthread_t *th = (pthread_t *)calloc(nthreads, sizeof(pthread_t));
     int64_t n = last - first;
@@ -616,6 +688,8 @@ int orc_transport_pass(orc_ctx *c, int64_t first, int64_t last, int nthreads)
         w[i].c = c;
         w[i].first = first + n * i / nthreads;
         w[i].last = first + n * (i + 1) / nthreads;
+        w[i].dfirst = 1 + nd * i / nthreads;
+        w[i].dlast = 1 + nd * (i + 1) / nthreads;
         if (i == 0) {
             w[i].t.hhe = c->hhe; w[i].t.jt = c->jt; w[i].t.dust = c->dust_abs;
             memset(c->hhe, 0, sizeof(double) * 6 * (size_t)nl);

@@ -39,6 +39,7 @@ class OrcParams(C.Structure):
         ("caseA", C.c_int),
         ("add_dust", C.c_int), ("dust_scatter", C.c_int),
         ("lambda_ref", C.c_double),
+        ("diffuse_field", C.c_int), ("diffuse_model", C.c_int),
     ]
 
 
@@ -136,6 +137,16 @@ def _load():
     lib.orc_gbar_ff.restype = C.c_double
     lib.orc_cooling_total.argtypes = [C.POINTER(Cooling)] + [C.c_double] * 7 + [C.c_int]
     lib.orc_cooling_total.restype = C.c_double
+    lib.orc_milne_setup.argtypes = [C.c_void_p, C.c_double]
+    lib.orc_milne_energy_per_recomb.argtypes = [C.c_void_p, C.c_int, C.c_double]
+    lib.orc_milne_energy_per_recomb.restype = C.c_double
+    lib.orc_milne_sample_energy.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
+    lib.orc_milne_sample_energy.restype = C.c_double
+    lib.orc_diffuse_setup.argtypes = [C.c_void_p]
+    lib.orc_diffuse_setup.restype = C.c_int
+    lib.orc_diffuse_build.argtypes = [C.c_void_p]
+    lib.orc_launch_diffuse.argtypes = [C.c_void_p, C.c_int64, _dp]
+    lib.orc_transport_pass.restype = C.c_int
     lib.orc_build_planck.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double,
                                      C.c_double, _dp, C.c_int]
     lib.orc_build_tabulated.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
@@ -366,6 +377,25 @@ class Model:
         lib().orc_launch(self.c, pid, ptr(out))
         return out
 
+    def diffuse_build(self):
+        """Emission table from the current state; returns (L_diffuse, n_packets)."""
+        L = lib()
+        if not self._head().milne and L.orc_diffuse_setup(self.c):
+            raise RuntimeError("orc_diffuse_setup failed")
+        L.orc_diffuse_build(self.c)
+        h = self._head()
+        return h.diffuse_lum, h.diffuse_nphot
+
+    def diffuse_tables(self):
+        h = self._head()
+        return (np.ctypeslib.as_array(h.dif_ch, (self.nleaf * 3,)).reshape(self.nleaf, 3).copy(),
+                np.ctypeslib.as_array(h.dif_cdf, (self.nleaf,)).copy())
+
+    def launch_diffuse(self, pid):
+        out = np.zeros(9)
+        lib().orc_launch_diffuse(self.c, pid, ptr(out))
+        return out
+
     def __del__(self):
         try:
             if self.c:
@@ -376,7 +406,7 @@ class Model:
 
 
 class _CtxHead(C.Structure):
-    """Mirror of orc_ctx (mc_oracle.h) up to the rate arrays."""
+    """Mirror of orc_ctx (mc_oracle.h)."""
     _fields_ = [
         ("p", OrcParams),
         ("qmc", C.c_uint32 * (32 * 12 + 12 * 2)),
@@ -401,6 +431,9 @@ class _CtxHead(C.Structure):
         ("seg_direct", C.c_int64), ("seg_scattered", C.c_int64),
         ("e_emit", C.c_double), ("e_abs", C.c_double), ("e_esc", C.c_double),
         ("gHI", _dp), ("gHeI", _dp), ("gHeII", _dp), ("hHI", _dp), ("hHeI", _dp), ("hHeII", _dp),
+        ("cool", Cooling),
+        ("milne", C.c_void_p), ("dif_ch", _dp), ("dif_cdf", _dp),
+        ("diffuse_lum", C.c_double), ("diffuse_nphot", C.c_int64),
     ]
 
 
@@ -439,4 +472,25 @@ def ref_lib():
     L.ref_rate.restype = C.c_double
     L.ref_gaunt_ff.argtypes = [C.c_float, C.c_float, C.POINTER(C.c_float),
                                C.POINTER(C.c_float), C.c_int]
+    L.ref_milne_setup.argtypes = [C.c_double] * 3
+    L.ref_milne_sample.argtypes = [C.c_int, C.c_double, C.c_double]
+    L.ref_milne_sample.restype = C.c_double
+    L.ref_milne_eper.argtypes = [C.c_int, C.c_double]
+    L.ref_milne_eper.restype = C.c_double
     return L
+
+
+class Milne:
+    """Milne ground-continuum tables of the oracle (channels 0 H I, 1 He I,
+    2 He II)."""
+    SIZE = 8 * (192 + 512 + 3 * 192 * 512 + 2 * 3 * 192 + 3 + 1)
+
+    def __init__(self, eion_max=100.0):
+        self.buf = C.create_string_buffer(self.SIZE)
+        lib().orc_milne_setup(self.buf, eion_max)
+
+    def energy_per_recomb(self, ich, T):
+        return lib().orc_milne_energy_per_recomb(self.buf, ich, T)
+
+    def sample(self, ich, T, u):
+        return lib().orc_milne_sample_energy(self.buf, ich, T, u)

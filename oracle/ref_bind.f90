@@ -1,5 +1,6 @@
 ! C-callable entry points into the reference's own Fortran modules
-! (qmc_mod, energy_sampler_mod, photo_xsec, recomb_mod, gaunt), compiled from
+! (qmc_mod, energy_sampler_mod, photo_xsec, recomb_mod, gaunt,
+! milne_recomb_spectrum_mod), compiled from
 ! their sources under the reference checkout by oracle/Makefile.  Test
 ! infrastructure only: the tests use it to check the C oracle against the
 ! reference code itself.
@@ -11,6 +12,8 @@ module ref_bind
        build_tabulated_sampler, sample_energy_cdf, energy_cdf_value
   use photo_xsec, only : sigma_HI, sigma_HeI, sigma_HeII
   use recomb_mod
+  use milne_recomb_spectrum_mod, only : milne_setup, milne_sample_energy, &
+       milne_energy_per_recomb
   implicit none
   type(energy_sampler_type), save :: smp
 
@@ -117,5 +120,26 @@ contains
     integer :: iflag
     call getGauntFF(z, log10te, xlf, g, iflag)
   end subroutine ref_gaunt_ff
+
+  subroutine ref_milne_setup(eion_max, te_min, te_max) bind(C, name='ref_milne_setup')
+    real(c_double), value :: eion_max, te_min, te_max
+    par%eion_max = eion_max
+    par%te_min = te_min
+    par%te_max = te_max
+    par%diffuse_energy_model = 'milne'
+    call milne_setup()
+  end subroutine ref_milne_setup
+
+  real(c_double) function ref_milne_sample(ich, T, u) bind(C, name='ref_milne_sample')
+    integer(c_int), value :: ich
+    real(c_double), value :: T, u
+    ref_milne_sample = milne_sample_energy(int(ich), T, u)
+  end function ref_milne_sample
+
+  real(c_double) function ref_milne_eper(ich, T) bind(C, name='ref_milne_eper')
+    integer(c_int), value :: ich
+    real(c_double), value :: T
+    ref_milne_eper = milne_energy_per_recomb(int(ich), T)
+  end function ref_milne_eper
 
 end module ref_bind

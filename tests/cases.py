@@ -15,6 +15,7 @@ _MAP = {  # library Params name -> oracle param name
     "ion_relax": "ion_relax", "te_min": "te_min", "te_max": "te_max", "case_A": "caseA",
     "ion_add_dust": "add_dust", "ion_dust_scatter": "dust_scatter", "lambda_ref": "lambda_ref",
     "xHI_init": "xHI_init", "xHeI_init": "xHeI_init", "xHeII_init": "xHeII_init",
+    "diffuse_field": "diffuse_field", "diffuse_model": "diffuse_model",
 }
 _ORC_DEFAULTS = dict(source_cdf_tol=1e-8, align_edges=1, lambda_ref=0.55, te_min=1e3,
                      te_max=5e4, ext_geometry=2)
@@ -25,6 +26,9 @@ def oracle_params(params):
     for k, v in params.items():
         if k in _MAP:
             out[_MAP[k]] = v
+    if out.get("diffuse_field"):
+        out["caseA"] = 1                   # the diffuse field forces case A
+        out.setdefault("diffuse_model", 2)
     return out
 
 
