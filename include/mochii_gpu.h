@@ -51,6 +51,14 @@ extern "C" {
 #define MC_DIFFUSE_EXPONENTIAL 1
 #define MC_DIFFUSE_MILNE       2
 
+/* warm start of the thermal solve (mc_set_warm_start) */
+#define MC_WARM_NONE   0   /* the reference's bisection from [te_min, te_max]  */
+#define MC_WARM_NEWTON 1   /* safeguarded Newton from the leaf's current T_e    */
+#define MC_WARM_MLP    2   /* safeguarded Newton from the network's T_e and x   */
+
+/* learned warm start: network 8 -> 64 -> 64 -> 64 -> 4 (GELU) */
+#define MC_MLP_NPARAM 9180
+
 /* leaf-state fields for mc_state_get / mc_state_set
  * (src/gas_state_mod.f90:23-28, rhokap from src/octree_mod.f90:50) */
 #define MC_NH     0
@@ -163,6 +171,42 @@ int mc_cooling_setup(mc_ctx *ctx, int nfit, const double *A, const double *Ti);
  * bisection on log T with the ionisation state solved at each trial T. */
 int mc_gas_thermal_update(mc_ctx *ctx, double *max_dx, double *max_dte, double *dx_vol,
                           double *dte_vol);
+
+/* Warm start of the per-leaf solve in mc_gas_thermal_update (MC_WARM_*).
+ * No counterpart in the reference (whose solve is the MC_WARM_NONE
+ * bisection): every warm mode keeps the reference's edge tests and accepts
+ * only a sign-change bracket of the same width thi/tlo - 1 < 1e-5, falling
+ * back to bisection from the bracket it has when Newton stalls. */
+int mc_set_warm_start(mc_ctx *ctx, int mode);
+
+/* Work of the last thermal update: {net-rate evaluations spent on the
+ * bracketed leaves (those not pinned at te_min / te_max, whose two edge
+ * evaluations settle them), bracketed leaves that fell back to bisection,
+ * bracketed leaves, all leaves}.  No counterpart in the reference. */
+int mc_thermal_stats(mc_ctx *ctx, int64_t *out4);
+
+/* ---- learned warm start (no counterpart in the reference) -----------------
+ * Parameter blob (float, MC_MLP_NPARAM): W1[8][64] b1[64] W2[64][64] b2[64]
+ * W3[64][64] b3[64] W4[64][4] b4[4] in_mu[8] in_sd[8] out_mu[4] out_sd[4];
+ * W[k][n] connects input k to unit n.  Features per leaf (float[8]): log10 of
+ * n_H, Gamma_HI, Gamma_HeI, Gamma_HeII, H_HI, H_HeI, H_HeII, n_e (rates as in
+ * mc_rates_get, floored at 1e-30 / 1e-40); targets (float[4]): log10 x_HI,
+ * x_HeI, x_HeII, T_e.  Standardisation (x - mu) / sd is part of the blob. */
+int mc_mlp_set(mc_ctx *ctx, const float *params, int nparam);
+int mc_mlp_get(mc_ctx *ctx, float *params);
+/* Features of every leaf from the current state and rates (nleaf x 8). */
+int mc_mlp_features(mc_ctx *ctx, float *X);
+/* Adam training on the device (FP32 forward/backward kernels): steps
+ * minibatches of `batch` rows drawn without replacement from (X, Y) (n rows),
+ * learning rate lr; loss_hist (may be NULL) receives the minibatch MSE of
+ * the standardised outputs per step. */
+int mc_mlp_train(mc_ctx *ctx, const float *X, const float *Y, int64_t n, int steps, int batch,
+                 float lr, uint32_t seed, float *loss_hist);
+/* One FP32 forward/backward over n rows: loss and the gradient in the
+ * parameter layout (standardisation entries 0). */
+int mc_mlp_grad(mc_ctx *ctx, const float *X, const float *Y, int n, float *loss, float *grad);
+/* BF16 matrix-core inference over n feature rows -> n x 4 targets. */
+int mc_mlp_infer(mc_ctx *ctx, const float *X, int n, float *out);
 
 /* hydrogen_balance_residual_grid (src/ion_balance_mod.f90:313-343). */
 int mc_hydrogen_balance_residual_grid(mc_ctx *ctx, double *eps_max, double *eps_vol);

@@ -155,6 +155,9 @@ static void free_leaf_arrays(mc_ctx *c)
     for (void *p : {(void *)c->d_dif, (void *)c->d_dcdf, (void *)c->d_bsum})
         if (p) (void)hipFree(p);
     c->d_dif = c->d_dcdf = c->d_bsum = nullptr;
+    for (void *p : {(void *)c->d_feat, (void *)c->d_pred})
+        if (p) (void)hipFree(p);
+    c->d_feat = c->d_pred = nullptr;
 }
 
 static int alloc_leaf_arrays(mc_ctx *c)
@@ -165,7 +168,7 @@ static int alloc_leaf_arrays(mc_ctx *c)
     if (dalloc(&s.nH, n) || dalloc(&s.xHI, n) || dalloc(&s.xHeI, n) || dalloc(&s.xHeII, n) ||
         dalloc(&s.ne, n) || dalloc(&s.Te, n) || dalloc(&s.rhokap, n) || dalloc(&s.opac, n) ||
         dalloc(&s.hhe, n * 8) || dalloc(&s.jt, n * (size_t)c->nnu) || dalloc(&s.rates, n * 6) ||
-        dalloc(&s.partials, ((n + 255) / 256) * 6 + 6))
+        dalloc(&s.partials, ((n + 255) / 256) * 10 + 10))
         return -100;
     if (c->p.diffuse_field) {
         const size_t nb = (n + 1023) / 1024;        // scan tiles of 1024 leaves
@@ -288,7 +291,8 @@ extern "C" int mc_finalize(mc_ctx *c)
                     (void *)c->pk.z, (void *)c->pk.kx, (void *)c->pk.ky, (void *)c->pk.kz,
                     (void *)c->pk.E, (void *)c->pk.sig, (void *)c->pk.inu, (void *)c->pk.leaf,
                     (void *)c->d_node_gd, (void *)c->d_node_gf, (void *)c->d_node_link,
-                    (void *)c->d_node_nbr, (void *)c->d_leaf_node, (void *)c->d_milne})
+                    (void *)c->d_node_nbr, (void *)c->d_leaf_node, (void *)c->d_milne,
+                    (void *)c->mlp.P, (void *)c->d_feat, (void *)c->d_pred})
         if (p) (void)hipFree(p);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -578,6 +582,28 @@ extern "C" int mc_gas_thermal_update(mc_ctx *c, double *max_dx, double *max_dte,
     *max_dte = m[1];
     *dx_vol = m[2];
     *dte_vol = m[3];
+    return 0;
+}
+
+extern "C" int mc_set_warm_start(mc_ctx *c, int mode)
+{
+    CHECK_CTX(c);
+    if (mode != MC_WARM_NONE && mode != MC_WARM_NEWTON && mode != MC_WARM_MLP) {
+        mc_set_error("mc_set_warm_start: mode must be MC_WARM_NONE, MC_WARM_NEWTON or MC_WARM_MLP");
+        return -1;
+    }
+    c->warm_mode = mode;
+    return 0;
+}
+
+extern "C" int mc_thermal_stats(mc_ctx *c, int64_t *out4)
+{
+    CHECK_CTX(c);
+    if (!out4) {
+        mc_set_error("mc_thermal_stats: null output");
+        return -1;
+    }
+    for (int k = 0; k < 4; ++k) out4[k] = c->thermal_stats[k];
     return 0;
 }
 

@@ -63,6 +63,16 @@ struct DevMilne {
     int32_t model;           // 0 threshold, 1 exponential, 2 milne
 };
 
+// Learned warm start: network sizes (the FP32 parameter blob length
+// MC_MLP_NPARAM is part of the C-ABI)
+#define MC_MLP_IN 8
+#define MC_MLP_HID 64
+#define MC_MLP_OUT 4
+struct MlpState {
+    float *P = nullptr;        // device parameter blob
+    bool ready = false;
+};
+
 // Small launch/band tables shared by every kernel of a context
 // (read through the scalar/constant cache).
 struct DevTables {
@@ -179,6 +189,11 @@ struct mc_ctx {
     int nbsum;
     double diffuse_lum;
     int64_t diffuse_nphot;
+    // learned warm start of the thermal solve
+    MlpState mlp;
+    int warm_mode;            // MC_WARM_*
+    float *d_feat, *d_pred;   // nleaf x 8 features, nleaf x 4 predictions
+    int64_t thermal_stats[4]; // evaluations (bracketed leaves), fallbacks, bracketed, leaves
     // last pass results
     double ledger[3];
     int64_t segs[2];
@@ -203,6 +218,8 @@ int mc_run_equilibrium(mc_ctx *c, double *max_dx, double *dx_vol);
 int mc_run_residual(mc_ctx *c, double *eps_max, double *eps_vol);
 int mc_run_opacity_fill(mc_ctx *c);
 int mc_run_thermal(mc_ctx *c, double out4[4]);
+int mc_run_mlp_infer(mc_ctx *c, const float *dX, int n, float *dOut);
+int mc_run_mlp_features(mc_ctx *c, float *dX);
 int mc_run_trace(mc_ctx *c, int precision, const double *pos, const double *dir, int32_t *cells,
                  int maxn, int32_t *n);
 

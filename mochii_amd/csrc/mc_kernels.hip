@@ -1421,16 +1421,46 @@ __device__ double net_rate(const DevCool &cl, const CellIn &c, double T, bool ca
     return heat - cooling_total(cl, r, T, c.nH, ne, xHI, xHeI, xHeII, y, caseA);
 }
 
-// gas_thermal_update: bisection on log T in [te_min, te_max] to
-// thi/tlo - 1 < 1e-5, relaxation on x, T taken directly.  Partials
-// {max|dx|, max|dT|/T, sum V|dx|, sum V x_HII, sum V ne |dT|, sum V ne T}
+// the same balance with ln(heating / cooling) beside it: the Newton variable
+// of the warm-started solve.  Near a root where one side varies
+// exponentially with 1/T (collisional line cooling) the log ratio is close
+// to linear in ln T while the difference is not.
+__device__ double net_rate_lr(const DevCool &cl, const CellIn &c, double T, bool caseA, double y,
+                              double &xHI, double &xHeI, double &xHeII, double &ne, double &lr)
+{
+    const RateSet r = rates_at(T, caseA);
+    ne = solve_cell(r, c.gH, c.gHe1, c.gHe2, c.nH, y, xHI, xHeI, xHeII);
+    const double heat = c.nH * (xHI * c.hH + y * (xHeI * c.hHe1 + xHeII * c.hHe2));
+    const double cool = cooling_total(cl, r, T, c.nH, ne, xHI, xHeI, xHeII, y, caseA);
+    lr = log(fmax(heat, DBL_MIN)) - log(fmax(cool, DBL_MIN));
+    return heat - cool;
+}
+
+// gas_thermal_update: T_e in [te_min, te_max] with net(T) = 0 to a verified
+// sign-change bracket thi/tlo - 1 < 1e-5, relaxation on x, T taken directly.
+//   W = MC_WARM_NONE   the reference's bisection on log T from the full range;
+//   W = MC_WARM_NEWTON safeguarded Newton / secant on log T started at the
+//                      leaf's current T_e;
+//   W = MC_WARM_MLP    the same started from the network's T_e and x guess.
+// Both warm forms keep the reference's edge tests (net(te_min) <= 0 pins
+// te_min, net(te_max) >= 0 pins te_max), step only inside the current
+// bracket, and accept only a bracket of the bisection's width: every leaf is
+// judged by the same criterion as the plain solve; a leaf whose Newton steps
+// do not close the bracket within kNewtonMax evaluations continues with
+// bisection from the bracket it has (counted as a fallback).
+// Partials {max|dx|, max|dT|/T, sum V|dx|, sum V x_HII, sum V ne |dT|,
+// sum V ne T, net-rate evaluations of bracketed leaves, fallbacks, bracketed
+// leaves (those not pinned at te_min / te_max)}
+constexpr int kNewtonMax = 24;
+
+template <int W>
 __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCool cl, bool caseA,
                                                     double y, double w, double tmin, double tmax,
-                                                    double d2cm)
+                                                    double d2cm, const float *pred)
 {
     const int il = blockIdx.x * kBlock + threadIdx.x;
-    double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    bool ismax[6] = {true, true, false, false, false, false};
+    double v[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    bool ismax[9] = {true, true, false, false, false, false, false, false, false};
     if (il < g.nleaf) {
         const size_t n = g.nleaf;
         const double nH = s.nH[il];
@@ -1441,22 +1471,79 @@ __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCo
                         s.rates[3 * n + il], s.rates[4 * n + il], s.rates[5 * n + il]};
             double xHI = s.xHI[il], xHeI = s.xHeI[il], xHeII = s.xHeII[il], ne, te;
             double tlo = tmin, thi = tmax;
+            int evals = 2, fallback = 0, bracketed = 0;
             const double nlo = net_rate(cl, c, tlo, caseA, y, xHI, xHeI, xHeII, ne);
             const double nhi = net_rate(cl, c, thi, caseA, y, xHI, xHeI, xHeII, ne);
             if (nlo <= 0.0) {
                 te = tlo;
             } else if (nhi >= 0.0) {
                 te = thi;
-            } else {
+            } else if constexpr (W == MC_WARM_NONE) {
+                bracketed = 1;
                 for (int it = 0; it < 60; ++it) {
                     const double tm = sqrt(tlo * thi);
+                    ++evals;
                     if (net_rate(cl, c, tm, caseA, y, xHI, xHeI, xHeII, ne) > 0.0) tlo = tm;
                     else thi = tm;
                     if (thi / tlo - 1.0 < 1.0e-5) break;
                 }
                 te = sqrt(tlo * thi);
+            } else {
+                bracketed = 1;
+                double t0 = s.Te[il];
+                if constexpr (W == MC_WARM_MLP) {
+                    const float *pr = pred + (size_t)il * 4;
+                    t0 = exp10((double)pr[3]);
+                    xHI = fmin(fmax(exp10((double)pr[0]), 1.0e-12), 1.0);
+                    xHeI = fmin(fmax(exp10((double)pr[1]), 1.0e-12), 1.0);
+                    xHeII = fmin(fmax(exp10((double)pr[2]), 1.0e-12), 1.0 - xHeI);
+                }
+                // Newton / secant on g(ln T) = ln(heating / cooling); the bracket
+                // side of every evaluation is decided by the sign of the
+                // difference, exactly as in the bisection
+                const double ltol = log1p(1.0e-5);
+                double u = log(fmin(fmax(t0, tlo * 1.0000001), thi * 0.9999999));
+                double f;
+                if (net_rate_lr(cl, c, exp(u), caseA, y, xHI, xHeI, xHeII, ne, f) > 0.0) tlo = exp(u);
+                else thi = exp(u);
+                ++evals;
+                // slope from a probe 1e-3 in ln T towards the root
+                double up = u + (f > 0.0 ? 1.0e-3 : -1.0e-3);
+                up = fmin(fmax(up, log(tlo)), log(thi));
+                double fp;
+                if (net_rate_lr(cl, c, exp(up), caseA, y, xHI, xHeI, xHeII, ne, fp) > 0.0)
+                    tlo = fmax(tlo, exp(up));
+                else
+                    thi = fmin(thi, exp(up));
+                ++evals;
+                int k = 0;
+                while (thi / tlo - 1.0 >= 1.0e-5 && k < kNewtonMax) {
+                    const double ul = log(tlo), uh = log(thi);
+                    const double slope = up != u ? (f - fp) / (u - up) : 0.0;
+                    double un = slope < 0.0 ? u - f / slope : 0.5 * (ul + uh);
+                    if (!(un > ul && un < uh)) un = 0.5 * (ul + uh);
+                    // a converged step lands within the tolerance: place it half a
+                    // tolerance past the estimate so the bracket closes around it
+                    if (fabs(un - u) < 0.5 * ltol) un = f > 0.0 ? un + 0.5 * ltol : un - 0.5 * ltol;
+                    if (!(un > ul && un < uh)) un = 0.5 * (ul + uh);
+                    up = u; fp = f; u = un;
+                    if (net_rate_lr(cl, c, exp(u), caseA, y, xHI, xHeI, xHeII, ne, f) > 0.0) tlo = exp(u);
+                    else thi = exp(u);
+                    ++evals; ++k;
+                }
+                if (thi / tlo - 1.0 >= 1.0e-5) {
+                    fallback = 1;
+                    for (int it = 0; it < 60 && thi / tlo - 1.0 >= 1.0e-5; ++it) {
+                        const double tm = sqrt(tlo * thi);
+                        ++evals;
+                        if (net_rate(cl, c, tm, caseA, y, xHI, xHeI, xHeII, ne) > 0.0) tlo = tm;
+                        else thi = tm;
+                    }
+                }
+                te = sqrt(tlo * thi);
             }
             net_rate(cl, c, te, caseA, y, xHI, xHeI, xHeII, ne);
+            ++evals;
             const double oHI = s.xHI[il], oT = s.Te[il];
             const double nHI = (1.0 - w) * oHI + w * xHI;
             const double nHeI = (1.0 - w) * s.xHeI[il] + w * xHeI;
@@ -1474,9 +1561,12 @@ __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCo
             v[3] = vol * (1.0 - nHI);
             v[4] = vol * nen * fabs(te - oT);
             v[5] = vol * nen * te;
+            v[6] = bracketed ? (double)evals : 0.0;
+            v[7] = (double)fallback;
+            v[8] = (double)bracketed;
         }
     }
-    block_reduce<6>(v, ismax, s.partials);
+    block_reduce<9>(v, ismax, s.partials);
 }
 
 // hydrogen_balance_residual_grid partials {max eps, sum V eps, sum V}
@@ -1758,24 +1848,46 @@ int mc_run_thermal(mc_ctx *c, double out[4])
     const int nblk = (c->nleaf + kBlock - 1) / kBlock;
     hipStream_t st = c->stream;
     MC_HIP(hipEventRecord(c->ev[6], st));
-    k_thermal<<<nblk, kBlock, 0, st>>>(c->grid, c->st, c->cool, c->p.case_A != 0, c->p.He_abund,
-                                        c->p.ion_relax, c->p.te_min, c->p.te_max,
-                                        c->p.distance2cm);
+    const int mode = c->warm_mode;
+    if (mode == MC_WARM_MLP) {
+        if (!c->mlp.ready) {
+            mc_set_error("gas_thermal_update: warm start MC_WARM_MLP needs a network (mc_mlp_set)");
+            return -1;
+        }
+        if (!c->d_feat) {
+            MC_HIP(hipMalloc(&c->d_feat, sizeof(float) * (size_t)c->nleaf * MC_MLP_IN));
+            MC_HIP(hipMalloc(&c->d_pred, sizeof(float) * (size_t)c->nleaf * MC_MLP_OUT));
+        }
+        int rc = mc_run_mlp_features(c, c->d_feat);
+        if (!rc) rc = mc_run_mlp_infer(c, c->d_feat, c->nleaf, c->d_pred);
+        if (rc) return rc;
+    }
+    const bool caseA = c->p.case_A != 0;
+#define MC_THERMAL_ARGS c->grid, c->st, c->cool, caseA, c->p.He_abund, c->p.ion_relax, \
+                        c->p.te_min, c->p.te_max, c->p.distance2cm, c->d_pred
+    if (mode == MC_WARM_MLP) k_thermal<MC_WARM_MLP><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
+    else if (mode == MC_WARM_NEWTON) k_thermal<MC_WARM_NEWTON><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
+    else k_thermal<MC_WARM_NONE><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
+#undef MC_THERMAL_ARGS
     MC_HIP(hipGetLastError());
     MC_HIP(hipEventRecord(c->ev[7], st));
-    std::vector<double> h((size_t)nblk * 6);
-    MC_HIP(hipMemcpyAsync(h.data(), c->st.partials, sizeof(double) * 6 * nblk,
+    std::vector<double> h((size_t)nblk * 9);
+    MC_HIP(hipMemcpyAsync(h.data(), c->st.partials, sizeof(double) * 9 * nblk,
                           hipMemcpyDeviceToHost, st));
     MC_HIP(hipStreamSynchronize(st));
-    double m[6] = {0, 0, 0, 0, 0, 0};
+    double m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int b = 0; b < nblk; ++b) {
-        m[0] = std::fmax(m[0], h[(size_t)b * 6]);
-        m[1] = std::fmax(m[1], h[(size_t)b * 6 + 1]);
-        for (int k = 2; k < 6; ++k) m[k] += h[(size_t)b * 6 + k];
+        m[0] = std::fmax(m[0], h[(size_t)b * 9]);
+        m[1] = std::fmax(m[1], h[(size_t)b * 9 + 1]);
+        for (int k = 2; k < 9; ++k) m[k] += h[(size_t)b * 9 + k];
     }
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, c->ev[6], c->ev[7]);
     c->times[3] = ms;
+    c->thermal_stats[0] = (int64_t)m[6];
+    c->thermal_stats[1] = (int64_t)m[7];
+    c->thermal_stats[2] = (int64_t)m[8];
+    c->thermal_stats[3] = c->nleaf;
     out[0] = m[0];
     out[1] = m[1];
     out[2] = m[2] / std::fmax(m[3], DBL_MIN);

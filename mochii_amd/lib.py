@@ -8,6 +8,8 @@ MC_FP64, MC_MIXED = 0, 1
 MC_GRID_CAR, MC_GRID_AMR = 0, 1
 MC_EXT_SPH, MC_EXT_REC = 1, 2
 MC_DIFFUSE_THRESHOLD, MC_DIFFUSE_EXPONENTIAL, MC_DIFFUSE_MILNE = 0, 1, 2
+MC_WARM_NONE, MC_WARM_NEWTON, MC_WARM_MLP = 0, 1, 2
+MC_MLP_NPARAM = 9180
 FIELDS = {"nH": 0, "xHI": 1, "xHeI": 2, "xHeII": 3, "ne": 4, "Te": 5, "rhokap": 6}
 
 
@@ -85,6 +87,7 @@ def library():
         raise MochiiError(f"HIP library not built: {path} (run make -C mochii_amd)")
     L = C.CDLL(path)
     vp, dp, ip, i64p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+    fp = C.POINTER(C.c_float)
     sig = {
         "mc_last_error": ([], C.c_char_p),
         "mc_init": ([C.POINTER(Params), C.POINTER(vp)], C.c_int),
@@ -99,6 +102,15 @@ def library():
         "mc_gas_equilibrium_update": ([vp, dp, dp], C.c_int),
         "mc_hydrogen_balance_residual_grid": ([vp, dp, dp], C.c_int),
         "mc_diffuse_info": ([vp, dp, i64p], C.c_int),
+        "mc_set_warm_start": ([vp, C.c_int], C.c_int),
+        "mc_thermal_stats": ([vp, i64p], C.c_int),
+        "mc_mlp_set": ([vp, fp, C.c_int], C.c_int),
+        "mc_mlp_get": ([vp, fp], C.c_int),
+        "mc_mlp_features": ([vp, fp], C.c_int),
+        "mc_mlp_train": ([vp, fp, fp, C.c_int64, C.c_int, C.c_int, C.c_float, C.c_uint32, fp],
+                         C.c_int),
+        "mc_mlp_grad": ([vp, fp, fp, C.c_int, fp, fp], C.c_int),
+        "mc_mlp_infer": ([vp, fp, C.c_int, fp], C.c_int),
         "mc_cooling_setup": ([vp, C.c_int, dp, dp], C.c_int),
         "mc_gas_thermal_update": ([vp, dp, dp, dp, dp], C.c_int),
         "mc_tallies_get": ([vp, dp, dp, dp, i64p], C.c_int),

@@ -16,6 +16,17 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and a.shape != shape:
+        a = a.reshape(shape)
+    return a
+
+
 class Model:
     """One device-resident model: grid, leaf state, tallies, rates."""
 
@@ -101,6 +112,54 @@ class Model:
         check(self._L.mc_gas_thermal_update(self._h, C.byref(a), C.byref(b), C.byref(c),
                                             C.byref(d)), "mc_gas_thermal_update")
         return a.value, b.value, c.value, d.value
+
+    # ---- learned warm start (no counterpart in the reference) -----------------
+    def set_warm_start(self, mode):
+        check(self._L.mc_set_warm_start(self._h, int(mode)), "mc_set_warm_start")
+
+    def thermal_stats(self):
+        a = np.zeros(4, dtype=np.int64)
+        check(self._L.mc_thermal_stats(self._h, a.ctypes.data_as(C.POINTER(C.c_int64))),
+              "mc_thermal_stats")
+        return dict(evaluations=int(a[0]), fallbacks=int(a[1]), bracketed=int(a[2]),
+                    leaves=int(a[3]))
+
+    def mlp_set(self, params):
+        p = _f32(params)
+        check(self._L.mc_mlp_set(self._h, _fp(p), p.size), "mc_mlp_set")
+
+    def mlp_get(self):
+        from .lib import MC_MLP_NPARAM
+        p = np.zeros(MC_MLP_NPARAM, dtype=np.float32)
+        check(self._L.mc_mlp_get(self._h, _fp(p)), "mc_mlp_get")
+        return p
+
+    def mlp_features(self):
+        X = np.zeros((self.nleaf, 8), dtype=np.float32)
+        check(self._L.mc_mlp_features(self._h, _fp(X)), "mc_mlp_features")
+        return X
+
+    def mlp_train(self, X, Y, steps, batch=1024, lr=1e-3, seed=1):
+        X, Y = _f32(X), _f32(Y)
+        hist = np.zeros(max(steps, 1), dtype=np.float32)
+        check(self._L.mc_mlp_train(self._h, _fp(X), _fp(Y), X.shape[0], int(steps), int(batch),
+                                   float(lr), int(seed), _fp(hist)), "mc_mlp_train")
+        return hist[:steps]
+
+    def mlp_grad(self, X, Y):
+        from .lib import MC_MLP_NPARAM
+        X, Y = _f32(X), _f32(Y)
+        g = np.zeros(MC_MLP_NPARAM, dtype=np.float32)
+        loss = C.c_float()
+        check(self._L.mc_mlp_grad(self._h, _fp(X), _fp(Y), X.shape[0], C.byref(loss), _fp(g)),
+              "mc_mlp_grad")
+        return loss.value, g
+
+    def mlp_infer(self, X):
+        X = _f32(X)
+        out = np.zeros((X.shape[0], 4), dtype=np.float32)
+        check(self._L.mc_mlp_infer(self._h, _fp(X), X.shape[0], _fp(out)), "mc_mlp_infer")
+        return out
 
     def hydrogen_balance_residual_grid(self):
         a, b = C.c_double(), C.c_double()
