@@ -1,19 +1,36 @@
 #!/usr/bin/env python3
-"""Benchmark: packets/s of one nonlinear iteration (transport pass + cell
-update) and time to a converged model, FP32-mixed vs plain FP64, on one
-MI355X.
+"""Benchmark: packets/s and time to a converged model on one MI355X, the
+surrogate + FP32 form against the plain all-FP64 form.
+
+Default workload: H + He H II region with thermal balance on an adaptive
+octree (n_H lognormal, mean 100 cm^-3, sigma_ln 0.5, from a seeded k^-11/3
+Gaussian random field on 128^3; at most 7 levels, refined where the density
+varies by more than 0.3 dex; one 45000 K star with Q(H) = 3e49 s^-1; explicit
+diffuse recombination packets, case A; 1e7 stellar packets per iteration).
+`--workload stromgren` runs the pure-H Stromgren sphere on a 64^3 car grid
+(fixed T, 1e6 packets per iteration) instead.
 
 A step is one iteration of src/main.f90:128-185 of the reference: a full
-transport pass of all packets (launch, walk, tallies, rates) followed by the
-per-leaf cell update.  Inputs are resident in HBM before the timed region.
+transport pass (stellar + diffuse packets: launch, walk, tallies, rates) and
+the per-leaf cell update (thermal balance, or fixed-T balance for the
+Stromgren workload).  Inputs are resident in HBM before the timed region.
+
+Forms: "fp64"      all-FP64 transport, the reference's bisection solve;
+       "mixed_mlp" FP32 cell-local geometry and opacity (FP64 tallies and
+                   solves) and the learned warm start of the thermal solve
+                   (network trained on the GPU in an untimed set-up phase on
+                   cell states of smaller models of the same physics).
 
 Prints ONE JSON line.  Extra fields:
-  forms        per-form packets/s, walk-kernel time, time to converged model
-  roofline     walk kernel: algorithmic bytes (segments x bytes/segment)
-               over the kernel's event-timed duration, against 8 TB/s
-  cpu_baseline the FP64 CPU oracle timed on the host cores: whole passes of
-               the same workload on the converged state, repeated for
-               about 10 s
+  forms        per form: packets/s, walk-kernel and solve times, time to the
+               converged model, iterations, solve work
+  error        per-cell error of the mixed_mlp converged model against the
+               fp64 one: T_e relative; x_HII relative where x_HII > 1e-3,
+               absolute elsewhere
+  roofline     the walk kernel of the value's form: algorithmic bytes per
+               scored segment x segments over its event-timed duration
+  cpu_baseline the FP64 CPU oracle on the host cores on a bounded sample of
+               the same pass (~10 s)
 
   python bench.py --trace-summary KERNEL_TRACE.csv [--steps K]
     condenses a rocprofv3 --kernel-trace CSV of a bench run into per-kernel
@@ -30,37 +47,32 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per scored path segment, H/He on a Cartesian grid
-# (d = dust terms, n_tr = metal transitions, both 0 here):
-#   FP64 form:  8*(4 + d + n_tr) state reads + 16*(7 + d + 2 n_tr) tally RMWs
-#   mixed form: 4*(4 + d + n_tr) state reads (FP32 opacity record) + same RMWs
-BYTES_PER_SEGMENT = {"fp64": 8 * 4 + 16 * 7, "mixed": 4 * 4 + 16 * 7}
+# Algorithmic bytes per scored path segment (d = dust, n_tr = metal
+# transitions, both 0 here): state reads + 7 FP64 tally read-modify-writes
+#   car FP64:  8 (4 + d + n_tr) + 16 (7 + d + 2 n_tr)           = 144
+#   car mixed: 4 (4 + d + n_tr) (FP32 opacity record) + 112      = 128
+#   octree:    + leaf geometry per segment: FP64 centre + half + neighbour
+#              and child links (44 B), FP32 centre + half + links (24 B)
+BYTES_PER_SEGMENT = {("car", "fp64"): 144, ("car", "mixed_mlp"): 128,
+                     ("octree", "fp64"): 188, ("octree", "mixed_mlp"): 152}
+WALK_KERNEL = {("car", "fp64"): "k_walk<MC_FP64, kCarDDA>",
+               ("car", "mixed_mlp"): "k_walk<MC_MIXED, kCarDDA>",
+               ("octree", "fp64"): "k_walk<MC_FP64, kFaceExit>",
+               ("octree", "mixed_mlp"): "k_walk<MC_MIXED, kFaceExit>"}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--grid", type=int, default=64)
-    ap.add_argument("--packets", type=int, default=1_000_000)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=("hii_octree", "stromgren"), default="hii_octree")
+    ap.add_argument("--packets", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--trace-summary", default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-converge", action="store_true")
     return ap.parse_args()
-
-
-def converge(make_model, log):
-    """Time to converged model: the full iteration to conv_crit='vol' plus the
-    final consistency pass (src/main.f90:124-227)."""
-    from mochii_amd import run_model
-    m = make_model()
-    t0 = time.perf_counter()
-    res = run_model(m)
-    dt = time.perf_counter() - t0
-    log(f"  converged={res['converged']} after {res['iterations']} iterations in {dt:.3f} s")
-    return m, res, dt
 
 
 def trace_summary(path, last):
@@ -83,6 +95,191 @@ def trace_summary(path, last):
               f"{tail.size:5d} {tail.mean() / 1e3:12.1f}")
 
 
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_workload(name, packets):
+    from mochii_amd.workloads import hii_octree, stromgren
+    if name == "hii_octree":
+        N = packets or 10_000_000
+        params, nH, octree = hii_octree(nphotons=N)
+        desc = (f"H+He H II region with thermal balance, octree of {nH.size} leaves (<=7 levels, "
+                f"0.3 dex) over a lognormal n_H field (mean 100, sigma_ln 0.5, k^-11/3, 128^3), "
+                f"box 16 pc, 45000 K Q(H)=3e49, He/H=0.1, no trace metals, explicit diffuse "
+                f"recombination packets (case A), {N:.0e} stellar packets/iteration")
+        return params, nH, octree, "octree", desc
+    N = packets or 1_000_000
+    params, nH = stromgren(n=64, nphotons=N)
+    desc = (f"Stromgren sphere, pure H, uniform car 64^3, box 8 pc, nH=100, 40000 K Q(H)=1e49, "
+            f"case B, T=1e4 K, {N:.0e} Sobol packets/iteration")
+    return params, nH, None, "car", desc
+
+
+def make_model(params, nH, octree, precision, warm=None, mlp=None):
+    import mochii_amd as mc
+    from mochii_amd.workloads import HI_COOLING_FIT
+    m = mc.Model(dict(params, precision=precision))
+    if octree is not None:
+        m.grid_octree(*octree)
+    m.gas_state_setup(nH)
+    if params.get("solve_te"):
+        m.cooling_setup(HI_COOLING_FIT)
+        if mlp is not None:
+            m.mlp_set(mlp)
+        if warm is not None:
+            m.set_warm_start(warm)
+    return m
+
+
+def train_surrogate():
+    """Train the warm-start network on cell states of smaller models of the
+    same physics (a 32^3 car H II region and a 64^3-field octree)."""
+    import numpy as np
+    import mochii_amd as mc
+    from mochii_amd import surrogate
+    from mochii_amd.workloads import HI_COOLING_FIT, hii_octree, hii_thermal
+    t0 = time.perf_counter()
+    Xs, Ys = [], []
+    p, nH = hii_thermal(n=32, nphotons=300_000, diffuse=True, seed=99)
+    m = mc.Model(dict(p))
+    m.gas_state_setup(nH)
+    m.cooling_setup(HI_COOLING_FIT)
+    X, Y = surrogate.collect(m, 40)
+    Xs.append(X)
+    Ys.append(Y)
+    m.close()
+    p, nH, octree = hii_octree(n_field=64, nphotons=1_000_000, field_seed=7, seed=98, lmax=6)
+    m = make_model(p, nH, octree, mc.MC_MIXED)
+    X, Y = surrogate.collect(m, 40)
+    Xs.append(X)
+    Ys.append(Y)
+    m.close()
+    t_collect = time.perf_counter() - t0
+    X, Y = surrogate.balance(np.concatenate(Xs), np.concatenate(Ys), max_rows=400_000)
+    tm = mc.Model(dict(p))
+    P, hist = surrogate.train(tm, X, Y, schedule=((3000, 2e-3), (2000, 4e-4)), batch=2048,
+                              log=log)
+    tm.close()
+    t_total = time.perf_counter() - t0
+    return P, dict(rows=int(X.shape[0]), collect_s=t_collect, train_s=t_total - t_collect,
+                   final_loss=float(hist[-200:].mean()))
+
+
+def run_form(name, params, nH, octree, args, mlp=None):
+    import numpy as np
+    import mochii_amd as mc
+    from mochii_amd import run_model
+    prec = mc.MC_FP64 if name == "fp64" else mc.MC_MIXED
+    warm = mc.lib.MC_WARM_NONE if name == "fp64" else mc.lib.MC_WARM_MLP
+    thermal = bool(params.get("solve_te"))
+    m = make_model(params, nH, octree, prec, warm if thermal else None, mlp if thermal else None)
+
+    def update():
+        return m.gas_thermal_update() if thermal else m.gas_equilibrium_update()
+
+    res, t_conv = {"iterations": 0, "converged": False}, None
+    if not args.no_converge:
+        log(f"[{name}] converging the model")
+        t0 = time.perf_counter()
+        res = run_model(m, log=lambda msg: log('  ' + msg))
+        t_conv = time.perf_counter() - t0
+        log(f"  converged={res['converged']} after {res['iterations']} iterations in {t_conv:.2f} s")
+    else:
+        for _ in range(10):
+            m.ionizing_field_and_rates()
+            update()
+    state = m.state()
+    for _ in range(args.warmup):
+        m.ionizing_field_and_rates()
+        update()
+    walk_ms, solve_ms, npk, nseg, evals, brk = [], [], [], [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        m.ionizing_field_and_rates()
+        tm = m.timing()
+        walk_ms.append(tm["walk_ms"])
+        npk.append(params["nphotons"] + m.diffuse_info()[1])
+        tal = m.tallies()
+        nseg.append(tal["seg_direct"] + tal["seg_scattered"])
+        update()
+        solve_ms.append(m.timing()["solve_ms"])
+        if thermal:
+            st = m.thermal_stats()
+            evals.append(st["evaluations"])
+            brk.append(st["bracketed"])
+    dt = time.perf_counter() - t0
+    out = dict(packets_per_s=float(np.sum(npk) / dt), ms_per_step=1e3 * dt / args.steps,
+               stellar_packets=int(params["nphotons"]),
+               diffuse_packets=int(np.mean(npk)) - int(params["nphotons"]),
+               walk_ms=float(np.mean(walk_ms)), solve_ms=float(np.mean(solve_ms)),
+               segments_per_pass=int(np.mean(nseg)), time_to_converged_s=t_conv,
+               iterations=res["iterations"], converged=res["converged"])
+    if thermal:
+        out["net_rate_evals_per_bracketed_leaf"] = float(np.sum(evals) / max(np.sum(brk), 1))
+    log(f"[{name}] {out}")
+    m.close()
+    return out, state
+
+
+def cell_errors(a, b):
+    """Per-cell error of state a against state b."""
+    import numpy as np
+    xa, xb = 1 - a["xHI"], 1 - b["xHI"]
+    ion = xb > 1e-3
+    rel_x = np.abs(xa[ion] / xb[ion] - 1)
+    abs_x = np.abs(xa[~ion] - xb[~ion])
+    out = dict(max_rel_xHII_ionised=float(rel_x.max()) if rel_x.size else 0.0,
+               p99_rel_xHII_ionised=float(np.percentile(rel_x, 99)) if rel_x.size else 0.0,
+               max_abs_xHII_neutral=float(abs_x.max()) if abs_x.size else 0.0,
+               frac_cells_xHII_within_1pct=float((np.sum(rel_x <= 0.01) + np.sum(abs_x <= 0.01))
+                                                 / xa.size))
+    rel_T = np.abs(a["Te"] / b["Te"] - 1)
+    out.update(max_rel_Te=float(rel_T.max()), p99_rel_Te=float(np.percentile(rel_T, 99)),
+               frac_cells_Te_within_1pct=float(np.mean(rel_T <= 0.01)))
+    return out
+
+
+def one_step(name, params, nH, octree, state, mlp):
+    """One iteration (pass + cell update) of a form from a given state."""
+    import mochii_amd as mc
+    prec = mc.MC_FP64 if name == "fp64" else mc.MC_MIXED
+    warm = mc.lib.MC_WARM_NONE if name == "fp64" else mc.lib.MC_WARM_MLP
+    thermal = bool(params.get("solve_te"))
+    m = make_model(params, nH, octree, prec, warm if thermal else None, mlp if thermal else None)
+    m.set_state(**{k: state[k] for k in ("xHI", "xHeI", "xHeII", "ne", "Te")})
+    m.gas_opacity_fill()
+    m.ionizing_field_and_rates()
+    m.gas_thermal_update() if thermal else m.gas_equilibrium_update()
+    out = m.state()
+    m.close()
+    return out
+
+
+def cpu_baseline(params, nH, octree, state, seconds):
+    """The FP64 oracle's transport pass on the host cores over a bounded
+    sample of the stellar packets of one pass on the converged state."""
+    from tests.cases import oracle_model
+    ncpu = min(os.cpu_count() or 1, 16)
+    orc = oracle_model(dict(params, diffuse_field=0), nH, octree=octree)
+    orc.set_state(**{k: state[k] for k in ("xHI", "xHeI", "xHeII", "ne", "Te")})
+    N = params["nphotons"]
+    n = min(20_000, N)
+    t0 = time.perf_counter()
+    orc.transport_pass(1, n + 1, ncpu)
+    dt = time.perf_counter() - t0
+    n2 = int(min(N, max(n, n * seconds / max(dt, 1e-3))))
+    if n2 > n:
+        t0 = time.perf_counter()
+        orc.transport_pass(1, n2 + 1, ncpu)
+        dt = time.perf_counter() - t0
+        n = n2
+    return {"value": n / dt, "unit": "packets/s", "cores": ncpu, "kind": "port",
+            "sample": f"FP64 C oracle transport pass over the first {n} of the {N} stellar "
+                      f"packet ids on the GPU-converged state (diffuse packets off), {ncpu} "
+                      f"threads, {dt:.1f} s"}
+
+
 def main():
     args = parse()
     if args.trace_summary:
@@ -93,89 +290,40 @@ def main():
               "(the pass would shard packets over GPUs and all-reduce the tallies)",
               file=sys.stderr)
         sys.exit(2)
-    import numpy as np
 
-    import mochii_amd as mc
-    from mochii_amd.workloads import stromgren
+    params, nH, octree, grid, desc = build_workload(args.workload, args.packets)
+    thermal = bool(params.get("solve_te"))
+    mlp, train_info = None, None
+    if thermal:
+        log("[surrogate] collecting cell states and training the warm-start network")
+        mlp, train_info = train_surrogate()
+        log(f"  {train_info}")
 
-    def log(msg):
-        print(msg, file=sys.stderr, flush=True)
-
-    n, N = args.grid, args.packets
-    params, nH = stromgren(n=n, nphotons=N)
-
-    def make_model(prec=mc.MC_MIXED):
-        m = mc.Model(dict(params, precision=prec))
-        m.gas_state_setup(nH)
-        return m
-
-    forms = {}
-    conv_state = {}
-    for name, prec in (("fp64", mc.MC_FP64), ("mixed", mc.MC_MIXED)):
-        log(f"[{name}] converging the model")
-        if not args.no_converge:
-            m, res, t_conv = converge(lambda: make_model(prec), log)
-            conv_state[name] = m.state("xHI")["xHI"]
-        else:
-            m, res, t_conv = make_model(prec), {"iterations": 0, "converged": False}, None
-            for _ in range(10):
-                m.ionizing_field_and_rates()
-                m.gas_equilibrium_update()
-        # timed steps on the (converged) model: transport pass + cell update
-        for _ in range(args.warmup):
-            m.ionizing_field_and_rates()
-            m.gas_equilibrium_update()
-        walk_ms = []
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            m.ionizing_field_and_rates()
-            t = m.timing()
-            walk_ms.append(t["walk_ms"])
-            m.gas_equilibrium_update()
-        dt = time.perf_counter() - t0
-        tal = m.tallies()
-        nseg = tal["seg_direct"] + tal["seg_scattered"]
-        wms = float(np.mean(walk_ms))
-        forms[name] = dict(packets_per_s=N * args.steps / dt, ms_per_step=1e3 * dt / args.steps,
-                           walk_ms=wms, segments_per_pass=nseg,
-                           time_to_converged_s=t_conv, iterations=res["iterations"],
-                           converged=res["converged"],
-                           walk_GBs=nseg * BYTES_PER_SEGMENT[name] / (wms * 1e-3) / 1e9)
-        log(f"[{name}] {forms[name]}")
-        state_for_cpu = m.state()
-        m.close()
-
+    forms, states = {}, {}
+    for name in ("fp64", "mixed_mlp"):
+        forms[name], states[name] = run_form(name, params, nH, octree, args, mlp)
+    # form error: one iteration of each form from the same (fp64-converged)
+    # state, so that only the form differs; the converged models themselves
+    # also differ by the iteration's cell-level Monte Carlo noise (front
+    # leaves that flip between neutral and ionised), reported beside it
     err = None
-    if "fp64" in conv_state and "mixed" in conv_state:
-        a, b = 1 - conv_state["mixed"], 1 - conv_state["fp64"]
-        ion = b > 1e-3
-        err = dict(max_rel_xHII_ionised=float(np.max(np.abs(a[ion] / b[ion] - 1))),
-                   max_abs_xHII_neutral=float(np.max(np.abs(a[~ion] - b[~ion]))) if (~ion).any() else 0.0)
+    if not args.no_converge:
+        s64 = one_step("fp64", params, nH, octree, states["fp64"], mlp)
+        smx = one_step("mixed_mlp", params, nH, octree, states["fp64"], mlp)
+        err = {"one_step_from_common_state": cell_errors(smx, s64),
+               "converged_models": cell_errors(states["mixed_mlp"], states["fp64"])}
+    fm, f64 = forms["mixed_mlp"], forms["fp64"]
+    speed = None
+    if f64["time_to_converged_s"] and fm["time_to_converged_s"]:
+        speed = f64["time_to_converged_s"] / fm["time_to_converged_s"]
 
-    cpu = None
-    if not args.no_cpu:
-        from tests.cases import oracle_model
-        ncpu = min(os.cpu_count() or 1, 16)
-        orc = oracle_model(params, nH)
-        orc.set_state(xHI=state_for_cpu["xHI"], xHeI=state_for_cpu["xHeI"],
-                      xHeII=state_for_cpu["xHeII"])
-        npass = 0
-        t0 = time.perf_counter()
-        while True:
-            orc.transport_pass(1, N + 1, ncpu)
-            orc.rates_compute()
-            orc.equilibrium_update()
-            npass += 1
-            dtc = time.perf_counter() - t0
-            if dtc >= args.cpu_seconds or npass >= 50:
-                break
-        cpu = {"value": npass * N / dtc, "unit": "packets/s", "cores": ncpu, "kind": "port",
-               "sample": f"FP64 C oracle, {npass} whole steps (pass of {N} packets + rates + "
-                         f"cell update) on the GPU-converged state, {ncpu} threads, "
-                         f"{dtc:.1f} s"}
+    cpu = None if args.no_cpu else cpu_baseline(params, nH, octree, states["mixed_mlp"],
+                                                args.cpu_seconds)
+    if cpu:
         log(f"[cpu] {cpu}")
 
-    fm = forms["mixed"]
+    bps = BYTES_PER_SEGMENT[(grid, "mixed_mlp")]
+    achieved = fm["segments_per_pass"] * bps / (fm["walk_ms"] * 1e-3) / 1e9
     line = {
         "metric": "packets/s + time-to-converged model, surrogate+FP32 vs plain FP64, "
                   "<=1% T_e/x_HII",
@@ -190,17 +338,17 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"Stromgren sphere, pure H, uniform car {n}^3, box 8 pc, "
-                               f"nH=100, 40000 K Q(H)=1e49, case B, T=1e4 K, "
-                               f"{N:.0e} Sobol packets/iteration",
-                   "grid": f"{n}^3", "packets_per_iteration": N},
+        "config": {"workload": desc, "leaves": int(nH.size),
+                   "packets_per_iteration": int(params["nphotons"])},
         "forms": forms,
-        "error_mixed_vs_fp64": err,
-        "roofline": {"bound": "hbm", "achieved": fm["walk_GBs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": fm["walk_GBs"] / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_walk<MC_MIXED, kCarDDA>",
-                     "bytes_per_segment": BYTES_PER_SEGMENT["mixed"],
-                     "segments_per_launch": fm["segments_per_pass"]},
+        "time_to_converged_speedup_vs_fp64": speed,
+        "surrogate_training": train_info,
+        "error_mixed_mlp_vs_fp64": err,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": WALK_KERNEL[(grid, "mixed_mlp")], "bytes_per_segment": bps,
+                     "segments_per_pass": fm["segments_per_pass"],
+                     "walk_ms_per_pass": fm["walk_ms"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -157,3 +157,23 @@ def octree_from_field(field, half, lmax=None, thresh_dex=0.3, lmin=1):
         cand = (2 * r[:, None, :] + off[None, :, :]).reshape(-1, 3)
     return (np.concatenate(xs), np.concatenate(ys), np.concatenate(zs), np.concatenate(lv),
             np.concatenate(nh))
+
+
+def hii_octree(n_field=128, nphotons=10_000_000, precision=MC_FP64, seed=12345, field_seed=1,
+               half=8.0, lmax=7, thresh_dex=0.3, lmin=1, niter=100, tol=2e-3, tol_te=2e-3,
+               diffuse=True):
+    """H + He H II region with thermal balance on an adaptive octree: n_H
+    lognormal (mean 100 cm^-3, sigma_ln 0.5) from a seeded Gaussian random
+    field with P(k) ~ k^-11/3 on n_field^3, leaves refined to at most lmax
+    levels where the density varies by more than thresh_dex across a cell,
+    He/H = 0.1, one 45000 K blackbody with Q(H) = 3e49 s^-1 at the centre,
+    explicit diffuse recombination packets (case A), nphotons stellar
+    packets per iteration.  Returns (params, nH per leaf, octree leaf list
+    (x, y, z, level, half))."""
+    field = lognormal_field(n_field, mean=100.0, sigma_ln=0.5, seed=field_seed)
+    xl, yl, zl, lev, nHl = octree_from_field(field, half=half, lmax=lmax, thresh_dex=thresh_dex,
+                                             lmin=lmin)
+    params, _ = hii_thermal(n=n_field, nphotons=nphotons, precision=precision, seed=seed,
+                            niter=niter, tol=tol, tol_te=tol_te, half=half, octree=True,
+                            nH=nHl, diffuse=diffuse)
+    return params, nHl, (xl, yl, zl, lev, half)
