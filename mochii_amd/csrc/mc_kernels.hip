@@ -731,6 +731,30 @@ __device__ __forceinline__ void end_edge(const KArgs &a, Lane &L, double tau_edg
     start_flight<P, G>(a, L, tau);
 }
 
+// Path luminosity of one edge-walk segment in the mixed form, wl
+// (e^-tau_in - e^-tau_out) / kappa with the segment terms in FP32.  The
+// running attenuation is an FP32 product while tau < kTauF32 (e^-80 is well
+// inside the normal FP32 range); past it e^-tau_in is taken in FP64 from the
+// FP32 optical depth, so that deep, weakly lit leaves keep the FP64 form's
+// dynamic range down to the walk's end at tau = 745.2.
+constexpr float kTauF32 = 80.0f;
+
+__device__ __forceinline__ double seg_lum_f(Lane &L, double wlc, float kc, float t, float dt)
+{
+    if (L.ftau < kTauF32) {
+        double pl;
+        if (dt > 0.0f) {
+            pl = wlc * (double)(L.fexpo * -expm1f(-dt) / kc);
+            L.fexpo = L.fexpo * expf(-dt);
+        } else {
+            pl = wlc * (double)(L.fexpo * t);
+        }
+        return pl;
+    }
+    const double ein = exp(-(double)L.ftau);
+    return dt > 0.0f ? wlc * ein * (double)(-expm1f(-dt) / kc) : wlc * ein * (double)t;
+}
+
 // one segment of the current walk (edge walk or tau flight), car DDA
 template <int P>
 __device__ __forceinline__ void car_step(const KArgs &a, Lane &L, Counters &cn)
@@ -776,15 +800,7 @@ __device__ __forceinline__ void car_step(const KArgs &a, Lane &L, Counters &cn)
         const float dt = kc * t;
         const double wlc = L.wl * g.dcell;
         if (L.phase == 1) {
-            double pl;
-            if (dt > 0.0f) {
-                const float f = L.fexpo * -expm1f(-dt);
-                pl = wlc * (double)(f / kc);
-                L.fexpo = L.fexpo * expf(-dt);
-            } else {
-                pl = wlc * (double)(L.fexpo * t);
-            }
-            score(a, L, il, pl, cn);
+            score(a, L, il, seg_lum_f(L, wlc, kc, t, dt), cn);
             L.ftau = L.ftau + dt;
             if (L.ftau >= (float)MC_TAU_HUGE || !cross_f(g, L, ax, t))
                 end_edge<P, kCarDDA>(a, L, (double)L.ftau, cn);
@@ -862,15 +878,7 @@ __device__ __forceinline__ void oct_step(const KArgs &a, Lane &L, Counters &cn)
         const float dt = kc * t;
         const double wlc = L.wl * (double)hw;
         if (L.phase == 1) {
-            double pl;
-            if (dt > 0.0f) {
-                const float fr = L.fexpo * -expm1f(-dt);
-                pl = wlc * (double)(fr / kc);
-                L.fexpo = L.fexpo * expf(-dt);
-            } else {
-                pl = wlc * (double)(L.fexpo * t);
-            }
-            score(a, L, il, pl, cn);
+            score(a, L, il, seg_lum_f(L, wlc, kc, t, dt), cn);
             L.ftau = L.ftau + dt;
             if (L.ftau >= (float)MC_TAU_HUGE || !cross_oct_f(g, L, ax, t))
                 end_edge<P, kFaceExit>(a, L, (double)L.ftau, cn);
