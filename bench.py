@@ -264,16 +264,14 @@ def cpu_baseline(params, nH, octree, state, seconds):
     orc = oracle_model(dict(params, diffuse_field=0), nH, octree=octree)
     orc.set_state(**{k: state[k] for k in ("xHI", "xHeI", "xHeII", "ne", "Te")})
     N = params["nphotons"]
-    n = min(20_000, N)
-    t0 = time.perf_counter()
-    orc.transport_pass(1, n + 1, ncpu)
-    dt = time.perf_counter() - t0
-    n2 = int(min(N, max(n, n * seconds / max(dt, 1e-3))))
-    if n2 > n:
+    n, dt = min(20_000, N), 0.0
+    while True:                          # grow the sample until it takes ~seconds
         t0 = time.perf_counter()
-        orc.transport_pass(1, n2 + 1, ncpu)
+        orc.transport_pass(1, n + 1, ncpu)
         dt = time.perf_counter() - t0
-        n = n2
+        if dt >= 0.7 * seconds or n >= N:
+            break
+        n = int(min(N, max(2 * n, n * seconds / max(dt, 1e-3))))
     return {"value": n / dt, "unit": "packets/s", "cores": ncpu, "kind": "port",
             "sample": f"FP64 C oracle transport pass over the first {n} of the {N} stellar "
                       f"packet ids on the GPU-converged state (diffuse packets off), {ncpu} "
