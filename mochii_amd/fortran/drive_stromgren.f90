@@ -1,0 +1,94 @@
+! The iteration of src/main.f90:124-227 (reference) driven through the
+! ISO_C_BINDING module: a pure-H Stromgren sphere on a uniform car grid at
+! fixed T, transport pass -> fixed-T cell update until the volume criterion
+! holds, then the final consistency pass.
+!   drive_stromgren layout        prints the byte size and field offsets of
+!                                 mc_params (no GPU needed)
+!   drive_stromgren [n] [npk]     runs the model and prints the iteration log,
+!                                 the energy ledger and Gamma_HI of leaf 1
+program drive_stromgren
+  use, intrinsic :: iso_c_binding
+  use mochii_gpu
+  implicit none
+  type(mc_params), target :: p
+  type(c_ptr) :: ctx
+  real(c_double), allocatable :: nH(:), rates(:)
+  real(c_double) :: max_dx, dx_vol, led(3)
+  integer(c_int32_t) :: nleaf, nnu
+  real(c_double) :: Ltot
+  character(len=32) :: arg
+  integer :: it, n, rc
+  integer(c_int64_t) :: npk
+
+  call get_command_argument(1, arg)
+  if (trim(arg) == 'layout') then
+     call print_layout()
+     stop
+  end if
+  n = 16;  npk = 20000_c_int64_t
+  if (len_trim(arg) > 0) read(arg, *) n
+  call get_command_argument(2, arg)
+  if (len_trim(arg) > 0) read(arg, *) npk
+
+  p = mc_params_default()
+  p%nphotons = npk
+  p%tstar(1) = 4.0d4;  p%src_lum(1) = 3.177837d38
+  p%nx = n;  p%ny = n;  p%nz = n
+  p%xmax = 4d0;  p%ymax = 4d0;  p%zmax = 4d0;  p%distance2cm = 3.0856776d18
+  p%He_abund = 0d0;  p%gas_tol = 2d-3;  p%conv_vol = 1;  p%gas_niter = 100
+
+  call check(mc_init(p, ctx), 'mc_init')
+  call check(mc_info(ctx, nleaf, nnu, Ltot, c_null_ptr), 'mc_info')
+  allocate(nH(nleaf), rates(6 * nleaf))
+  nH = 100d0
+  call check(mc_gas_state_setup(ctx, nH, c_null_ptr), 'mc_gas_state_setup')
+  do it = 1, p%gas_niter
+     call check(mc_ionizing_field_and_rates(ctx, 0), 'mc_ionizing_field_and_rates')
+     call check(mc_gas_equilibrium_update(ctx, max_dx, dx_vol), 'mc_gas_equilibrium_update')
+     write(*, '(a,i4,2es14.6)') 'iteration', it, max_dx, dx_vol
+     if (dx_vol < p%gas_tol) exit
+  end do
+  call check(mc_ionizing_field_and_rates(ctx, 0), 'final pass')
+  call check(mc_energy_ledger(ctx, led), 'mc_energy_ledger')
+  call check(mc_rates_get(ctx, rates), 'mc_rates_get')
+  write(*, '(a,3es24.16)') 'ledger', led
+  write(*, '(a,es24.16)') 'gamma_HI(1)', rates(1)
+  call check(mc_finalize(ctx), 'mc_finalize')
+
+contains
+
+  subroutine check(rc_, what)
+    integer(c_int), intent(in) :: rc_
+    character(len=*), intent(in) :: what
+    if (rc_ /= 0) then
+       write(*, '(3a)') trim(what), ' failed: ', mc_error_message()
+       error stop 1
+    end if
+  end subroutine check
+
+  subroutine print_layout()
+    integer(c_intptr_t) :: b
+    b = transfer(c_loc(p), b)
+    write(*, '(a,i0)') 'sizeof ', c_sizeof(p)
+    call off('nphotons', c_loc(p%nphotons));      call off('precision', c_loc(p%precision))
+    call off('eion_min', c_loc(p%eion_min));      call off('nnu_ion', c_loc(p%nnu_ion))
+    call off('tstar', c_loc(p%tstar));            call off('src_z', c_loc(p%src_z))
+    call off('ext_geometry', c_loc(p%ext_geometry)); call off('ext_tstar', c_loc(p%ext_tstar))
+    call off('grid_type', c_loc(p%grid_type));    call off('xmax', c_loc(p%xmax))
+    call off('distance2cm', c_loc(p%distance2cm)); call off('He_abund', c_loc(p%He_abund))
+    call off('te_fixed', c_loc(p%te_fixed));      call off('gas_tol_te', c_loc(p%gas_tol_te))
+    call off('case_A', c_loc(p%case_A));          call off('gas_niter', c_loc(p%gas_niter))
+    call off('xHI_init', c_loc(p%xHI_init));      call off('ion_add_dust', c_loc(p%ion_add_dust))
+    call off('lambda_ref', c_loc(p%lambda_ref));  call off('diffuse_field', c_loc(p%diffuse_field))
+    call off('diffuse_model', c_loc(p%diffuse_model))
+  contains
+    subroutine off(name, q)
+      character(len=*), intent(in) :: name
+      type(c_ptr), intent(in) :: q
+      integer(c_intptr_t) :: a
+      a = transfer(q, a)
+      write(*, '(a,1x,i0)') name, a - b
+    end subroutine off
+  end subroutine print_layout
+
+end program drive_stromgren

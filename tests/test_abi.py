@@ -53,3 +53,17 @@ def test_errors_are_reported_not_aborted():
     assert rc < 0 and lib.mc_last_error()
     assert lib.mc_ionizing_field_and_rates(None, 0) < 0
     assert b"null context" in lib.mc_last_error()
+
+
+def test_fortran_binding_layout_matches_header():
+    """The bind(C) mc_params of the Fortran module has the C struct's size and
+    field offsets (the driver prints them; no GPU needed)."""
+    import subprocess
+    exe = os.path.join(ROOT, "mochii_amd", "fortran", "_build", "drive_stromgren")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built (amdflang absent)")
+    out = subprocess.check_output([exe, "layout"], text=True).split("\n")
+    got = dict(line.split() for line in out if line.strip() and not line.startswith("Fortran"))
+    assert int(got.pop("sizeof")) == C.sizeof(Params)
+    for name, off in got.items():
+        assert int(off) == getattr(Params, name).offset, name

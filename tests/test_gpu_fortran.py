@@ -1,0 +1,37 @@
+"""A Fortran main program drives a model through the ISO_C_BINDING module
+(mochii_amd/fortran): the same iteration through Python gives the same
+iteration log, energy ledger and rates."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import mochii_amd as mc
+from tests.cases import gpu_model
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_fortran_driver_matches_python_driver():
+    exe = os.path.join(ROOT, "mochii_amd", "fortran", "_build", "drive_stromgren")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built (amdflang absent at build time)")
+    out = subprocess.run([exe, "16", "20000"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = out.stdout.split("\n")
+    f_iters = [[float(v) for v in l.split()[2:]] for l in lines if l.startswith("iteration")]
+    f_led = [float(v) for v in next(l for l in lines if l.startswith("ledger")).split()[1:]]
+    f_g1 = float(next(l for l in lines if l.startswith("gamma_HI(1)")).split()[1])
+    params = dict(nphotons=20000, tstar=[4.0e4], src_lum=[3.177837e38], src_x=[0.0],
+                  src_y=[0.0], src_z=[0.0], nx=16, ny=16, nz=16, xmax=4.0, ymax=4.0, zmax=4.0,
+                  distance2cm=3.0856776e18, He_abund=0.0, gas_tol=2e-3, conv_vol=1,
+                  gas_niter=100)
+    g = gpu_model(params, np.full(16 ** 3, 100.0))
+    res = mc.run_model(g)
+    p_iters = [list(h[1:]) for h in res["history"]]
+    assert len(p_iters) == len(f_iters)
+    np.testing.assert_allclose(np.array(f_iters), np.array(p_iters), rtol=1e-5, atol=1e-12)
+    np.testing.assert_allclose(f_led, g.energy_ledger(), rtol=1e-12)
+    assert abs(f_g1 / g.rates()["gHI"][0] - 1.0) < 1e-10
