@@ -56,6 +56,10 @@ extern "C" {
 #define MC_WARM_NEWTON 1   /* safeguarded Newton from the leaf's current T_e    */
 #define MC_WARM_MLP    2   /* safeguarded Newton from the network's T_e and x   */
 
+/* walk order of the stellar packets of a pass (mc_set_packet_order) */
+#define MC_ORDER_LAUNCH    0   /* launch (packet id) order                      */
+#define MC_ORDER_DIRECTION 1   /* sorted by source and direction (default)      */
+
 /* learned warm start: network 8 -> 64 -> 64 -> 64 -> 4 (GELU) */
 #define MC_MLP_NPARAM 9180
 
@@ -228,6 +232,14 @@ int mc_energy_ledger(mc_ctx *ctx, double *out3);
 /* Sizes: nleaf, nnu_band; band total ion_Ltot; band edges ion_eedge(nnu+1)
  * (eedge may be NULL). */
 int mc_info(mc_ctx *ctx, int32_t *nleaf, int32_t *nnu_band, double *Ltot, double *eedge);
+
+/* Walk order of the stellar packets (MC_ORDER_*).  The tallies are the same
+ * sums in either order (to FP64 summation order): with MC_ORDER_DIRECTION
+ * the launch records are sorted by source and direction so that a wavefront
+ * walks 64 packets through the same leaves and sums their rate tallies
+ * before the atomics.  No counterpart in the reference (a CPU rank walks one
+ * packet at a time). */
+int mc_set_packet_order(mc_ctx *ctx, int order);
 
 /* Switch the transport precision form (MC_FP64 | MC_MIXED) between passes. */
 int mc_set_precision(mc_ctx *ctx, int precision);

@@ -241,7 +241,15 @@ extern "C" int mc_init(const mc_params *p, mc_ctx **out)
     if (dalloc(&c->pk.x, cap) || dalloc(&c->pk.y, cap) || dalloc(&c->pk.z, cap) ||
         dalloc(&c->pk.kx, cap) || dalloc(&c->pk.ky, cap) || dalloc(&c->pk.kz, cap) ||
         dalloc(&c->pk.E, cap) || dalloc(&c->pk.sig, 6 * cap) || dalloc(&c->pk.inu, cap) ||
-        dalloc(&c->pk.leaf, cap)) {
+        dalloc(&c->pk.leaf, cap) || dalloc(&c->pk.key, cap) || dalloc(&c->d_key_sorted, cap) ||
+        dalloc(&c->d_perm, cap)) {
+        mc_finalize(c);
+        return -100;
+    }
+    c->packet_order = MC_ORDER_DIRECTION;
+    c->sort_tmp_bytes = mc_sort_temp_bytes(cap);
+    if (hipMalloc(&c->d_sort_tmp, std::max<size_t>(c->sort_tmp_bytes, 16)) != hipSuccess) {
+        mc_set_error("mc_init: sort workspace allocation failed");
         mc_finalize(c);
         return -100;
     }
@@ -292,7 +300,8 @@ extern "C" int mc_finalize(mc_ctx *c)
                     (void *)c->pk.E, (void *)c->pk.sig, (void *)c->pk.inu, (void *)c->pk.leaf,
                     (void *)c->d_node_gd, (void *)c->d_node_gf, (void *)c->d_node_link,
                     (void *)c->d_node_nbr, (void *)c->d_leaf_node, (void *)c->d_milne,
-                    (void *)c->mlp.P, (void *)c->d_feat, (void *)c->d_pred})
+                    (void *)c->mlp.P, (void *)c->d_feat, (void *)c->d_pred, (void *)c->pk.key,
+                    (void *)c->d_key_sorted, (void *)c->d_perm, c->d_sort_tmp})
         if (p) (void)hipFree(p);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -582,6 +591,17 @@ extern "C" int mc_gas_thermal_update(mc_ctx *c, double *max_dx, double *max_dte,
     *max_dte = m[1];
     *dx_vol = m[2];
     *dte_vol = m[3];
+    return 0;
+}
+
+extern "C" int mc_set_packet_order(mc_ctx *c, int order)
+{
+    CHECK_CTX(c);
+    if (order != MC_ORDER_LAUNCH && order != MC_ORDER_DIRECTION) {
+        mc_set_error("mc_set_packet_order: order must be MC_ORDER_LAUNCH or MC_ORDER_DIRECTION");
+        return -1;
+    }
+    c->packet_order = order;
     return 0;
 }
 

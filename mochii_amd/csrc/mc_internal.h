@@ -119,6 +119,7 @@ struct PacketBuf {
     double *sig;       // 6 per packet: sHI, sHeI, sHeII, dabs, dsca, g
     int32_t *inu;      // 0-based band bin
     int32_t *leaf;     // launch leaf (0-based, -1 outside)
+    uint32_t *key;     // walk-order key (direction-sorted walks), or nullptr
     int64_t cap;
 };
 
@@ -194,6 +195,12 @@ struct mc_ctx {
     int warm_mode;            // MC_WARM_*
     float *d_feat, *d_pred;   // nleaf x 8 features, nleaf x 4 predictions
     int64_t thermal_stats[4]; // evaluations (bracketed leaves), fallbacks, bracketed, leaves
+    // direction-sorted walk order of stellar batches
+    int packet_order;         // MC_ORDER_*
+    uint32_t *d_key_sorted;
+    int32_t *d_perm;
+    void *d_sort_tmp;
+    size_t sort_tmp_bytes;
     // last pass results
     double ledger[3];
     int64_t segs[2];
@@ -220,6 +227,8 @@ int mc_run_opacity_fill(mc_ctx *c);
 int mc_run_thermal(mc_ctx *c, double out4[4]);
 int mc_run_mlp_infer(mc_ctx *c, const float *dX, int n, float *dOut);
 int mc_run_mlp_features(mc_ctx *c, float *dX);
+size_t mc_sort_temp_bytes(int64_t n);
+int mc_sort_records(mc_ctx *c, int64_t n);
 int mc_run_trace(mc_ctx *c, int precision, const double *pos, const double *dir, int32_t *cells,
                  int maxn, int32_t *n);
 

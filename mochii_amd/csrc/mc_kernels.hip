@@ -41,6 +41,9 @@ struct KArgs {
     int64_t first_id;        // 1-based id of record 0
     int64_t npk;             // records in this batch
     int32_t stream;          // 1 stellar, 2 diffuse (Sobol keys and post-launch draws)
+    const int32_t *perm;     // walk order of the launch records (nullptr: launch order)
+    int32_t coherent;        // records sorted by direction: whole-wave refill and
+                             // wave-aggregated tallies
     const double *dif;       // per-leaf diffuse emission {ch0, ch1, ch2, total}
     const double *dcdf;      // diffuse leaf CDF
     DevMilne milne;
@@ -180,6 +183,34 @@ __device__ __forceinline__ void store_packet(const KArgs &a, int64_t i, double x
     P.leaf[i] = leaf;
 }
 
+// walk-order key of a launch record: emitting component in the top bits, then
+// the direction on an octahedral map (2^11 x 2^11 cells, Morton order), so
+// that sorted records give waves of packets leaving one source in nearly the
+// same direction
+__device__ __forceinline__ uint32_t spread11(uint32_t v)
+{
+    v &= 0x7FFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t direction_key(int comp, double kx, double ky, double kz)
+{
+    const double n = fabs(kx) + fabs(ky) + fabs(kz);
+    double u = kx / n, v = ky / n;
+    if (kz < 0.0) {
+        const double uu = (1.0 - fabs(v)) * (u >= 0.0 ? 1.0 : -1.0);
+        const double vv = (1.0 - fabs(u)) * (v >= 0.0 ? 1.0 : -1.0);
+        u = uu; v = vv;
+    }
+    const uint32_t iu = (uint32_t)fmin(fmax((u + 1.0) * 1024.0, 0.0), 2047.0);
+    const uint32_t iv = (uint32_t)fmin(fmax((v + 1.0) * 1024.0, 0.0), 2047.0);
+    return ((uint32_t)comp << 22) | (spread11(iu) | (spread11(iv) << 1));
+}
+
 __global__ __launch_bounds__(kBlock) void k_launch(KArgs a, int nlds)
 {
     extern __shared__ unsigned char lds_raw[];
@@ -275,6 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_launch(KArgs a, int nlds)
     }
     const double E = sample_energy(a.samp, a.nknots, T->samp_off[comp], T->samp_off[comp + 1], ue);
     store_packet(a, i, x, y, z, kx, ky, kz, E, find_leaf(a.g, x, y, z, sgd, slink, nlds));
+    if (a.pk.key) a.pk.key[i] = direction_key(comp, kx, ky, kz);
 }
 
 // ---------------------------------------------------------------------------
@@ -315,6 +347,9 @@ struct Lane {
     // [-1,1]^3 in units of the half width (octree)
     float px, py, pz, fkx, fky, fkz, fsHI, fsHeI, fsHeII, fdust;
     float fexpo, ftau, ftau_in, ftcur;
+    // tally of this step, scored after the step by the whole wave
+    int s_il;
+    double s_pl;
     // octree walk: current node, current position (FP64 form), node geometry
     int node, leaf0;
     double ox, oy, oz;
@@ -353,13 +388,26 @@ struct Counters {
 // score_ion_path: the packet's path quantity into its leaf.  A coefficient
 // that is exactly zero (He I / He II terms below their thresholds) adds +0 to
 // the tally, so its atomic is skipped; hot leaves go to the lane's replica.
-__device__ __forceinline__ void score(const KArgs &a, const Lane &L, int il, double pl, Counters &cn)
+// score_ion_path for one segment: the record is kept in the lane and the
+// atomics are issued by the whole wave after the step (flush_tally), so that
+// lanes of a wave that scored the same leaf can be summed first.  Segments
+// with zero path luminosity are neither scored nor counted, as in the
+// reference (src/ion_score_mod.f90:148).
+__device__ __forceinline__ void score(const KArgs &, Lane &L, int il, double pl, Counters &cn)
 {
     if (pl == 0.0) return;
-    const int nnu = a.tab->nnu;
+    L.s_il = il;
+    L.s_pl = pl;
+    if (L.nscatt > 0) cn.ss++; else cn.sd++;
+}
+
+// tally destination of a leaf: its own rows, or one of the replicas of a hot
+// leaf chosen by the issuing lane
+__device__ __forceinline__ void tally_rows(const KArgs &a, int il, double *&h, double *&j)
+{
     const DevState &s = a.s;
+    const int nnu = a.tab->nnu;
     const int hs = s.nhot > 0 ? s.hot_slot[il] : -1;
-    double *h, *j;
     if (hs < 0) {
         h = s.hhe + (size_t)il * 8;
         j = s.jt + (size_t)il * nnu;
@@ -369,12 +417,74 @@ __device__ __forceinline__ void score(const KArgs &a, const Lane &L, int il, dou
         h = s.hot_hhe + slot * 8;
         j = s.hot_jt + slot * nnu;
     }
+}
+
+// one lane's segment: 1 band-J atomic + one per nonzero rate coefficient
+__device__ __forceinline__ void tally_one(const KArgs &a, const Lane &L, int il, double pl)
+{
+    double *h, *j;
+    tally_rows(a, il, h, j);
     unsafeAtomicAdd(j + L.inu, pl);
 #pragma unroll
     for (int k = 0; k < 6; ++k)
         if (L.coeff[k] != 0.0) unsafeAtomicAdd(h + k, pl * L.coeff[k]);
     if (a.add_dust && L.dabs != 0.0) unsafeAtomicAdd(h + 6, pl * L.dabs);
-    if (L.nscatt > 0) cn.ss++; else cn.sd++;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+constexpr int kAggRounds = 4;   // leaf groups tried per step
+constexpr int kAggMin = 3;      // smallest group worth a wave reduction
+
+// Issue the tallies of the step's segments, called by all 64 lanes.  With
+// direction-coherent packets a leaf group (lanes that scored the same leaf)
+// is summed across the wave and its rate sums go out as one atomic per
+// coefficient from the group's first lane; band-J (keyed by the packet's own
+// energy bin) and lanes outside a large group use per-lane atomics.
+__device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane)
+{
+    bool has = L.s_il >= 0;
+    if (a.coherent) {
+        unsigned long long pend = __ballot(has);
+        for (int r = 0; r < kAggRounds && pend; ++r) {
+            const int leader = __ffsll((long long)pend) - 1;
+            const int lil = __shfl(L.s_il, leader);
+            const bool in = has && L.s_il == lil;
+            const unsigned long long grp = __ballot(in);
+            pend &= ~grp;
+            if (__popcll(grp) < kAggMin) continue;
+            double *h, *j;
+            tally_rows(a, lil, h, j);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const double v = in ? L.s_pl * L.coeff[k] : 0.0;
+                if (__ballot(v != 0.0)) {
+                    const double sum = wave_sum(v);
+                    if (lane == leader) unsafeAtomicAdd(h + k, sum);
+                }
+            }
+            if (a.add_dust) {
+                const double v = in ? L.s_pl * L.dabs : 0.0;
+                if (__ballot(v != 0.0)) {
+                    const double sum = wave_sum(v);
+                    if (lane == leader) unsafeAtomicAdd(h + 6, sum);
+                }
+            }
+            if (in) {
+                double *hh, *jj;
+                tally_rows(a, lil, hh, jj);
+                unsafeAtomicAdd(jj + L.inu, L.s_pl);
+                has = false;
+            }
+        }
+    }
+    if (has) tally_one(a, L, L.s_il, L.s_pl);
+    L.s_il = -1;
 }
 
 // ---- car DDA, FP64 (reference arithmetic) ---------------------------------
@@ -406,11 +516,13 @@ __device__ __forceinline__ void dda_init_d(const DevGrid &g, Lane &L)
     L.tcur = 0.0;
 }
 
-// distance to the nearest face along axis ax (register selects, no
-// dynamic indexing of the per-lane arrays)
-__device__ __forceinline__ double tm_of(const Lane &L, int ax)
+// distance to the nearest face, tm[ax] for the axis dda_axis_d chose (the
+// minimum).  Written as a minimum rather than a select on ax: a select of
+// per-lane fields becomes a select of their addresses in the optimiser,
+// which keeps the whole lane state in scratch memory.
+__device__ __forceinline__ double tm_of(const Lane &L, int)
 {
-    return ax == 0 ? L.tm[0] : (ax == 1 ? L.tm[1] : L.tm[2]);
+    return fmin(L.tm[0], fmin(L.tm[1], L.tm[2]));
 }
 
 // advance along the axis of the nearest face; returns false when outside
@@ -428,7 +540,7 @@ __device__ __forceinline__ int dda_axis_d(const Lane &L)
 {
     int ax = 0;
     if (L.tm[1] < L.tm[0]) ax = 1;
-    if (L.tm[2] < (ax == 0 ? L.tm[0] : L.tm[1])) ax = 2;
+    if (L.tm[2] < fmin(L.tm[0], L.tm[1])) ax = 2;
     return ax;
 }
 
@@ -590,7 +702,8 @@ __device__ __forceinline__ void move_oct_f(Lane &L, float t)
 __device__ __forceinline__ bool cross_oct_f(const DevGrid &g, Lane &L, int ax, float t)
 {
     move_oct_f(L, t);
-    const int s = ax == 0 ? L.sx : (ax == 1 ? L.sy : L.sz);
+    // sign of the exit axis by arithmetic, not by a select of lane fields
+    const int s = (ax == 0) * L.sx + (ax == 1) * L.sy + (ax == 2) * L.sz;
     const int f = 2 * ax + (s > 0 ? 0 : 1);
     const float4 G = L.gf;
     int ng = g.car ? car_next_node(g, L.node, f) : g.node_nbr[L.node * 6 + f];
@@ -901,6 +1014,7 @@ template <int P, int G>
 __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, Counters &cn)
 {
     const PacketBuf &pk = a.pk;
+    if (a.perm) r = a.perm[r];
     L.id = a.first_id + r;
     L.x = pk.x[r]; L.y = pk.y[r]; L.z = pk.z[r];
     L.kx = pk.kx[r]; L.ky = pk.ky[r]; L.kz = pk.kz[r];
@@ -955,11 +1069,15 @@ __global__ __launch_bounds__(kBlock) void k_walk(KArgs a)
 {
     Lane L;
     L.phase = 0;
+    L.s_il = -1;
     Counters cn = {0.0, 0.0, 0.0, 0ull, 0ull};
     const int lane = threadIdx.x & 63;
     for (;;) {
+        // direction-sorted records: the wave takes 64 consecutive records at
+        // once, when every lane is idle, so its packets stay together
         const unsigned long long need = __ballot(L.phase == 0);
-        if (need) {
+        const bool refill = a.coherent ? need == ~0ull : need != 0ull;
+        if (refill) {
             const int leader = __ffsll((long long)need) - 1;
             unsigned long long base = 0;
             if (lane == leader) base = atomicAdd(a.s.counter, (unsigned long long)__popcll(need));
@@ -976,6 +1094,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(KArgs a)
             if constexpr (G == kCarDDA) car_step<P>(a, L, cn);
             else oct_step<P>(a, L, cn);
         }
+        flush_tally(a, L, lane);
     }
     // wave reduction of the ledger and segment counts, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
@@ -1679,6 +1798,8 @@ KArgs make_args(mc_ctx *c)
     a.first_id = 1;
     a.npk = 0;
     a.stream = 1;
+    a.perm = nullptr;
+    a.coherent = 0;
     a.dif = c->d_dif;
     a.dcdf = c->d_dcdf;
     a.milne = c->milne;
@@ -1737,10 +1858,17 @@ int mc_run_pass(mc_ctx *c)
     const int nblk = persistent_blocks(kw);
     const int nlds = c->car ? 0 : c->n_lds_nodes;
     // one batch of packet records: launch kernel, then the persistent walk
+    // stellar batches walk in direction order (records sorted by source and
+    // direction); diffuse packets, spread over the volume, in launch order
+    const bool sorted = c->packet_order == MC_ORDER_DIRECTION;
     auto batch = [&](bool diffuse, int64_t first, int64_t n) -> int {
         a.first_id = first;
         a.npk = n;
         a.stream = diffuse ? 2 : 1;
+        const bool coh = sorted && !diffuse;
+        a.pk.key = coh ? c->pk.key : nullptr;
+        a.perm = coh ? c->d_perm : nullptr;
+        a.coherent = coh ? 1 : 0;
         MC_HIP(hipEventRecord(c->ev[1], st));
         if (diffuse)
             k_launch_diffuse<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(a);
@@ -1748,6 +1876,10 @@ int mc_run_pass(mc_ctx *c)
             k_launch<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock,
                        (size_t)nlds * (sizeof(double4) + sizeof(int32_t)), st>>>(a, nlds);
         MC_HIP(hipGetLastError());
+        if (coh) {
+            const int rc = mc_sort_records(c, n);
+            if (rc) return rc;
+        }
         k_reset_counter<<<1, 1, 0, st>>>(c->st.counter);
         MC_HIP(hipEventRecord(c->ev[2], st));
         if (face) {

@@ -5,7 +5,8 @@
 !   drive_stromgren layout        prints the byte size and field offsets of
 !                                 mc_params (no GPU needed)
 !   drive_stromgren [n] [npk]     runs the model and prints the iteration log,
-!                                 the energy ledger and Gamma_HI of leaf 1
+!                                 the energy ledger and the sum of Gamma_HI
+!                                 over the leaves
 program drive_stromgren
   use, intrinsic :: iso_c_binding
   use mochii_gpu
@@ -52,7 +53,7 @@ program drive_stromgren
   call check(mc_energy_ledger(ctx, led), 'mc_energy_ledger')
   call check(mc_rates_get(ctx, rates), 'mc_rates_get')
   write(*, '(a,3es24.16)') 'ledger', led
-  write(*, '(a,es24.16)') 'gamma_HI(1)', rates(1)
+  write(*, '(a,es24.16)') 'sum_gamma_HI', sum(rates(1:nleaf))
   call check(mc_finalize(ctx), 'mc_finalize')
 
 contains

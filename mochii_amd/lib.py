@@ -9,6 +9,7 @@ MC_GRID_CAR, MC_GRID_AMR = 0, 1
 MC_EXT_SPH, MC_EXT_REC = 1, 2
 MC_DIFFUSE_THRESHOLD, MC_DIFFUSE_EXPONENTIAL, MC_DIFFUSE_MILNE = 0, 1, 2
 MC_WARM_NONE, MC_WARM_NEWTON, MC_WARM_MLP = 0, 1, 2
+MC_ORDER_LAUNCH, MC_ORDER_DIRECTION = 0, 1
 MC_MLP_NPARAM = 9180
 FIELDS = {"nH": 0, "xHI": 1, "xHeI": 2, "xHeII": 3, "ne": 4, "Te": 5, "rhokap": 6}
 
@@ -103,6 +104,7 @@ def library():
         "mc_hydrogen_balance_residual_grid": ([vp, dp, dp], C.c_int),
         "mc_diffuse_info": ([vp, dp, i64p], C.c_int),
         "mc_set_warm_start": ([vp, C.c_int], C.c_int),
+        "mc_set_packet_order": ([vp, C.c_int], C.c_int),
         "mc_thermal_stats": ([vp, i64p], C.c_int),
         "mc_mlp_set": ([vp, fp, C.c_int], C.c_int),
         "mc_mlp_get": ([vp, fp], C.c_int),

@@ -80,6 +80,10 @@ class Model:
             out[k] = a
         return out
 
+    def set_packet_order(self, order):
+        """MC_ORDER_DIRECTION (default) or MC_ORDER_LAUNCH walk order."""
+        check(self._L.mc_set_packet_order(self._h, int(order)), "mc_set_packet_order")
+
     def set_precision(self, precision):
         check(self._L.mc_set_precision(self._h, int(precision)), "mc_set_precision")
         self.params.precision = int(precision)

@@ -136,3 +136,25 @@ def test_dust_scattering_pass_matches_oracle():
     ok, err = rel_close(tg["dust_abs"], to["dust_abs"], 1e-3, floor_frac=1e-3)
     assert ok, err
     np.testing.assert_allclose(gm.energy_ledger(), to["ledger"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("prec", [MC_FP64, MC_MIXED])
+def test_walk_order_does_not_change_tallies(prec):
+    """Direction-sorted walk with wave-summed tallies against launch order:
+    the same sums up to FP64 summation order."""
+    from mochii_amd.lib import MC_ORDER_LAUNCH
+    n = 16
+    params, nH = stromgren(n=n, nphotons=50000, precision=prec)
+    a, b = gpu_model(params, nH), gpu_model(params, nH)
+    xHI = _partial_state(n)
+    a.set_state(xHI=xHI)
+    b.set_state(xHI=xHI)
+    b.set_packet_order(MC_ORDER_LAUNCH)
+    a.ionizing_field_and_rates()
+    b.ionizing_field_and_rates()
+    ta, tb = a.tallies(), b.tallies()
+    for k in ("hhe", "jt"):
+        ok, err = rel_close(ta[k], tb[k], 1e-12)
+        assert ok, (k, err)
+    assert ta["seg_direct"] == tb["seg_direct"]
+    np.testing.assert_allclose(a.energy_ledger(), b.energy_ledger(), rtol=1e-13)
