@@ -444,11 +444,14 @@ constexpr int kAggMin = 3;      // smallest group worth a wave reduction
 // Issue the tallies of the step's segments, called by all 64 lanes.  With
 // direction-coherent packets a leaf group (lanes that scored the same leaf)
 // is summed across the wave and its rate sums go out as one atomic per
-// coefficient from the group's first lane; band-J (keyed by the packet's own
-// energy bin) and lanes outside a large group use per-lane atomics.
-__device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane)
+// coefficient from the group's first lane; its band-J contributions (keyed
+// by each packet's energy bin) are binned in the wave's LDS row first and go
+// out as one atomic per nonzero bin.  Lanes outside a large group use
+// per-lane atomics.  jbin: this wave's nnu LDS bins, zero between calls.
+__device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, double *jbin)
 {
     bool has = L.s_il >= 0;
+    const int nnu = a.tab->nnu;
     if (a.coherent) {
         unsigned long long pend = __ballot(has);
         for (int r = 0; r < kAggRounds && pend; ++r) {
@@ -475,12 +478,23 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane)
                     if (lane == leader) unsafeAtomicAdd(h + 6, sum);
                 }
             }
-            if (in) {
-                double *hh, *jj;
-                tally_rows(a, lil, hh, jj);
-                unsafeAtomicAdd(jj + L.inu, L.s_pl);
-                has = false;
+            // band J of the group: per-wave bins in LDS, then one atomic
+            // per nonzero bin
+            if (in) atomicAdd(&jbin[L.inu], L.s_pl);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (int b = lane; b < nnu; b += 64) {
+                const double v = jbin[b];
+                if (v != 0.0) {
+                    unsafeAtomicAdd(j + b, v);
+                    jbin[b] = 0.0;
+                }
             }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (in) has = false;
         }
     }
     if (has) tally_one(a, L, L.s_il, L.s_pl);
@@ -1065,8 +1079,14 @@ __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, 
 }
 
 template <int P, int G>
-__global__ __launch_bounds__(kBlock) void k_walk(KArgs a)
+__global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
 {
+    __shared__ double jbins[kBlock / 64][MC_MAXBINS];
+    double *jbin = jbins[threadIdx.x >> 6];
+    for (int b = threadIdx.x & 63; b < MC_MAXBINS; b += 64) jbin[b] = 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     Lane L;
     L.phase = 0;
     L.s_il = -1;
@@ -1094,7 +1114,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(KArgs a)
             if constexpr (G == kCarDDA) car_step<P>(a, L, cn);
             else oct_step<P>(a, L, cn);
         }
-        flush_tally(a, L, lane);
+        flush_tally(a, L, lane, jbin);
     }
     // wave reduction of the ledger and segment counts, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) {

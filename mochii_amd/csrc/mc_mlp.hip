@@ -99,17 +99,35 @@ __global__ void k_loss(const float *Z, const float *Y, float *dZ, int B, float *
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// dW[k][n] = sum_r A[r][k] dZ[r][n]; db[n] = sum_r dZ[r][n] (row k == K)
-__global__ void k_wgrad(const float *A, const float *dZ, float *dW, float *db, int B, int K, int N)
+// dW[k][n] = sum_r A[r][k] dZ[r][n]; db[n] = sum_r dZ[r][n] (row k == K):
+// partial sums over kWgradParts row ranges (blockIdx.y), then a fixed-order
+// sum of the partials
+constexpr int kWgradParts = 32;
+
+__global__ void k_wgrad_part(const float *A, const float *dZ, float *part, int B, int K, int N)
 {
     const int t = blockIdx.x * kB + threadIdx.x;
-    if (t >= (K + 1) * N) return;
+    const int M = (K + 1) * N;
+    if (t >= M) return;
     const int k = t / N, n = t - k * N;
+    const int r0 = (int)((long long)B * blockIdx.y / kWgradParts);
+    const int r1 = (int)((long long)B * (blockIdx.y + 1) / kWgradParts);
     float s = 0.0f;
     if (k < K)
-        for (int r = 0; r < B; ++r) s += A[r * K + k] * dZ[r * N + n];
+        for (int r = r0; r < r1; ++r) s += A[r * K + k] * dZ[r * N + n];
     else
-        for (int r = 0; r < B; ++r) s += dZ[r * N + n];
+        for (int r = r0; r < r1; ++r) s += dZ[r * N + n];
+    part[(size_t)blockIdx.y * M + t] = s;
+}
+
+__global__ void k_wgrad_sum(const float *part, float *dW, float *db, int K, int N)
+{
+    const int t = blockIdx.x * kB + threadIdx.x;
+    const int M = (K + 1) * N;
+    if (t >= M) return;
+    float s = 0.0f;
+    for (int p = 0; p < kWgradParts; ++p) s += part[(size_t)p * M + t];
+    const int k = t / N, n = t - k * N;
     if (k < K) dW[k * N + n] = s; else db[n] = s;
 }
 
@@ -300,7 +318,7 @@ int ensure_mlp(mc_ctx *c)
 // device scratch of one training / gradient call (freed by the caller)
 struct TrainBufs {
     float *Xn = nullptr, *Yn = nullptr, *Z[4] = {}, *H[4] = {}, *dZ[4] = {}, *G = nullptr,
-          *part = nullptr;
+          *part = nullptr, *wpart = nullptr;
     int B = 0;
     int alloc(int b)
     {
@@ -315,11 +333,13 @@ struct TrainBufs {
         }
         MC_HIP(hipMalloc(&G, sizeof(float) * MC_MLP_NPARAM));
         MC_HIP(hipMalloc(&part, sizeof(float) * ((b * NO + kB - 1) / kB)));
+        MC_HIP(hipMalloc(&wpart, sizeof(float) * kWgradParts * (NH + 1) * NH));
         return 0;
     }
     ~TrainBufs()
     {
-        for (void *p : {(void *)Xn, (void *)Yn, (void *)G, (void *)part}) if (p) (void)hipFree(p);
+        for (void *p : {(void *)Xn, (void *)Yn, (void *)G, (void *)part, (void *)wpart})
+            if (p) (void)hipFree(p);
         for (int l = 0; l < 4; ++l)
             for (void *p : {(void *)Z[l], (void *)H[l], (void *)dZ[l]}) if (p) (void)hipFree(p);
     }
@@ -344,8 +364,9 @@ int fwd_bwd(mc_ctx *c, TrainBufs &T, hipStream_t st, float *loss)
     MC_HIP(hipMemsetAsync(T.G, 0, sizeof(float) * MC_MLP_NPARAM, st));
     for (int l = 3; l >= 0; --l) {
         const float *Ain = l == 0 ? T.Xn : T.H[l - 1];
-        k_wgrad<<<Grid1((long long)(K[l] + 1) * N[l]).g, kB, 0, st>>>(Ain, T.dZ[l], T.G + offW[l],
-                                                                      T.G + offB[l], B, K[l], N[l]);
+        const unsigned gw = Grid1((long long)(K[l] + 1) * N[l]).g;
+        k_wgrad_part<<<dim3(gw, kWgradParts), kB, 0, st>>>(Ain, T.dZ[l], T.wpart, B, K[l], N[l]);
+        k_wgrad_sum<<<gw, kB, 0, st>>>(T.wpart, T.G + offW[l], T.G + offB[l], K[l], N[l]);
         if (l > 0)
             k_dgrad<<<Grid1((long long)B * K[l]).g, kB, 0, st>>>(T.dZ[l], P + offW[l], T.Z[l - 1],
                                                                  T.dZ[l - 1], B, K[l], N[l]);
