@@ -60,6 +60,26 @@ extern "C" {
 #define MC_ORDER_LAUNCH    0   /* launch (packet id) order                      */
 #define MC_ORDER_DIRECTION 1   /* sorted by source and direction (default)      */
 
+/* trace-metal registry (src/species_mod.f90:54-99; MAX_EL, MAX_ST there,
+ * MAX_METAL_TRANSITIONS in src/define.f90:80) */
+#define MC_MAX_EL 11
+#define MC_MAX_ST 6
+#define MC_MAX_METAL_TR 64
+#define MC_MAX_COOLFIT 8
+
+/* par%charge_exchange (src/define.f90:790-801) */
+#define MC_CX_TWO_WAY    0   /* hydrogen's side of each reaction in its balance */
+#define MC_CX_METAL_ONLY 1   /* charge exchange in the metal cascade only       */
+
+/* par%ci_model (src/define.f90:773-776) */
+#define MC_CI_VORONOV     0
+#define MC_CI_DERE_HYBRID 1   /* Voronov x the Dere (2007) ratio per stage */
+
+/* par%cooling_model (src/define.f90:781-789) */
+#define MC_COOL_LOW_DENSITY 0   /* line-cooling fits unsuppressed             */
+#define MC_COOL_LOCAL_NE    1   /* fits x the n-level (T, n_e) suppression:
+                                   needs the n-level tables, not built here  */
+
 /* learned warm start: network 8 -> 64 -> 64 -> 64 -> 4 (GELU) */
 #define MC_MLP_NPARAM 9180
 
@@ -108,7 +128,50 @@ typedef struct mc_params {
     int32_t diffuse_field;     /* par%diffuse_field: explicit ground-recombination
                                   packets (forces case A)                      */
     int32_t diffuse_model;     /* par%diffuse_energy_model: MC_DIFFUSE_*       */
+    /* active metal photoionisation thresholds [eV] gathered by the driver
+     * from the registry (src/main.f90:58-72), the ion_setup(metal_eth,
+     * nmetal) arguments: band-edge alignment and sampler knots */
+    int32_t nmetal;
+    double  metal_eth[MC_MAX_METAL_TR];
+    /* metal switches, used once mc_species_setup has loaded a registry */
+    int32_t ion_metal_abs;     /* par%ion_metal_abs                            */
+    int32_t metal_ne;          /* par%metal_ne                                 */
+    int32_t metal_heat;        /* par%metal_heat                               */
+    int32_t metal_freefree;    /* par%metal_freefree                           */
+    int32_t charge_exchange;   /* par%charge_exchange: MC_CX_*                 */
+    int32_t ci_model;          /* par%ci_model: MC_CI_*                        */
+    int32_t cooling_model;     /* par%cooling_model: MC_COOL_*                 */
 } mc_params;
+
+/* One element of the trace-metal registry, the fields read_element fills
+ * from data/atomic/element_<el>.txt (src/species_mod.f90:54-99, 358-422)
+ * plus the stage cooling fits of cooling_<el>_<i>.txt (cooling_load,
+ * src/cooling_mod.f90:98-123).  Transition it (0-based here) ionises stage
+ * it to it+1; arrays over transitions use the first nstage-1 entries, the
+ * cooling arrays the first nstage.  photo_form 1: VFKY96 (Eth, E0, s0, ya,
+ * P, yw, y0, y1); 2: VY95 outer shell (Eth, E0, s0, ya, P, yw, l). */
+typedef struct mc_element {
+    char    name[4];           /* "c", "n", "o", "ne", ... (lower case)        */
+    int32_t Z, nstage;
+    double  abund;             /* n(X)/n(H); the element is active when > 0    */
+    double  eth[MC_MAX_ST];
+    int32_t photo_form[MC_MAX_ST];
+    double  photo[MC_MAX_ST][8];
+    double  ci[MC_MAX_ST][5];  /* Voronov dE, P, A, X, K                       */
+    double  rr[MC_MAX_ST][6];  /* Badnell A, B, T0, T1, C, T2                  */
+    int32_t ndr[MC_MAX_ST];
+    double  dr_c[MC_MAX_ST][9], dr_e[MC_MAX_ST][9];
+    int32_t has_dr2[MC_MAX_ST];
+    double  dr2[MC_MAX_ST][4];
+    int32_t has_rr2[MC_MAX_ST];
+    double  rr2[MC_MAX_ST][2];
+    int32_t cxi_form[MC_MAX_ST];
+    double  cxi[MC_MAX_ST][6];
+    int32_t cxr_form[MC_MAX_ST];
+    double  cxr[MC_MAX_ST][6];
+    int32_t ncool[MC_MAX_ST];  /* 0 = no cooling fit for the stage             */
+    double  cool_A[MC_MAX_ST][MC_MAX_COOLFIT], cool_T[MC_MAX_ST][MC_MAX_COOLFIT];
+} mc_element;
 
 typedef struct mc_ctx mc_ctx;
 
@@ -211,6 +274,32 @@ int mc_mlp_train(mc_ctx *ctx, const float *X, const float *Y, int64_t n, int ste
 int mc_mlp_grad(mc_ctx *ctx, const float *X, const float *Y, int n, float *loss, float *grad);
 /* BF16 matrix-core inference over n feature rows -> n x 4 targets. */
 int mc_mlp_infer(mc_ctx *ctx, const float *X, int n, float *out);
+
+/* ---- trace metals ----------------------------------------------------------
+ * species_setup (src/species_mod.f90:143-228) + species_rate_path_setup
+ * (:308-319): load the registry (n_elements entries with abund > 0, in the
+ * reference's element order c n o ne s ar mg fe si cl ca).  Turns the metal
+ * path on (par%use_metals): metal opacity in the walk (ion_metal_abs), the
+ * raw(2,ntr,nleaf) path tallies, species_gamma_compute after each pass, the
+ * metal electrons / charge exchange in the cell solve and the metal cooling
+ * and heating terms of the thermal balance.  Refreshes the stage cache when
+ * the gas state is already set (species_stage_cache_refresh via
+ * gas_opacity_fill, :247-261).  n_elements = 0 turns the metal path off. */
+int mc_species_setup(mc_ctx *ctx, int n_elements, const mc_element *elements);
+
+/* Stage fractions stage_frac(ie)%g(nstage, nleaf) of registry element ie
+ * (0-based), column-major, as of the last stage-cache refresh
+ * (src/species_mod.f90:247-261). */
+int mc_species_fractions_get(mc_ctx *ctx, int ie, double *frac);
+
+/* Metal photoionisation and photoheating rates of the last pass:
+ * egam(ie)%g(nstage-1, nleaf) [s^-1] and eheat(ie)%g [erg/s]
+ * (species_gamma_compute, src/species_mod.f90:493-512); either may be NULL. */
+int mc_species_rates_get(mc_ctx *ctx, int ie, double *gam, double *heat);
+
+/* Metal path tallies of the last pass, raw(2, nstage-1, nleaf) of element ie
+ * (metal_rate_path(ie)%raw, src/species_mod.f90:102-108, 329-343). */
+int mc_species_tallies_get(mc_ctx *ctx, int ie, double *raw);
 
 /* hydrogen_balance_residual_grid (src/ion_balance_mod.f90:313-343). */
 int mc_hydrogen_balance_residual_grid(mc_ctx *ctx, double *eps_max, double *eps_vol);

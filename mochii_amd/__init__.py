@@ -13,7 +13,8 @@ from .lib import (MC_FP64, MC_MIXED, MC_GRID_AMR, MC_GRID_CAR, MC_EXT_SPH, MC_EX
                   MC_DIFFUSE_THRESHOLD, MC_DIFFUSE_EXPONENTIAL, MC_DIFFUSE_MILNE,
                   MochiiError, Params, library, library_path)
 from .model import Model, run_model
+from . import atomic
 
 __all__ = ["MC_FP64", "MC_MIXED", "MC_GRID_AMR", "MC_GRID_CAR", "MC_EXT_SPH", "MC_EXT_REC",
            "MC_DIFFUSE_THRESHOLD", "MC_DIFFUSE_EXPONENTIAL", "MC_DIFFUSE_MILNE",
-           "MochiiError", "Params", "Model", "run_model", "library", "library_path"]
+           "MochiiError", "Params", "Model", "run_model", "library", "library_path", "atomic"]

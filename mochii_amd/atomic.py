@@ -1,0 +1,219 @@
+"""Trace-metal registry on the host: element data, the C struct, thresholds.
+
+Mirrors species_setup / read_element (src/species_mod.f90:143-228, 358-422)
+and cooling_load (src/cooling_mod.f90:98-123).  The element and cooling files
+use the reference's own text formats (data/atomic/element_<el>.txt: keyword
+rows ELEMENT, TRANSITION, ETH, PHOTO/PHOTO2, CI, RR/RR2, DR/DR2, CXI, CXR;
+cooling_<el>_<i>.txt: the term count then A_i T_i rows), so a driver can read
+a data directory in that format directly with ``read_registry``.  A snapshot
+of the eleven registry elements is kept as data in
+``mochii_amd/data/atomic_registry.json`` (written by
+``mochii_amd/data/make_atomic_json.py``) and is what ``registry`` loads.
+"""
+import ctypes as C
+import json
+import os
+
+MC_MAX_EL, MC_MAX_ST, MC_MAX_METAL_TR, MC_MAX_COOLFIT = 11, 6, 64, 8
+MC_CX_TWO_WAY, MC_CX_METAL_ONLY = 0, 1
+MC_CI_VORONOV, MC_CI_DERE_HYBRID = 0, 1
+MC_COOL_LOW_DENSITY, MC_COOL_LOCAL_NE = 0, 1
+
+# species_setup's element order and the par%abund_* defaults
+# (src/species_mod.f90:153-157, src/define.f90:578-580, 847-857)
+ORDER = ("c", "n", "o", "ne", "s", "ar", "mg", "fe", "si", "cl", "ca")
+DEFAULT_ABUND = {"c": 2.2e-4, "n": 4.0e-5, "o": 3.3e-4, "ne": 5.0e-5, "s": 9.0e-6,
+                 "ar": 0.0, "mg": 0.0, "fe": 0.0, "si": 0.0, "cl": 0.0, "ca": 0.0}
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SNAPSHOT = os.path.join(HERE, "data", "atomic_registry.json")
+
+_ST = MC_MAX_ST
+
+
+def _d(*dims):
+    t = C.c_double
+    for n in reversed(dims):
+        t = t * n
+    return t
+
+
+class Element(C.Structure):
+    """Mirror of mc_element (include/mochii_gpu.h)."""
+    _fields_ = [
+        ("name", C.c_char * 4), ("Z", C.c_int32), ("nstage", C.c_int32),
+        ("abund", C.c_double),
+        ("eth", _d(_ST)), ("photo_form", C.c_int32 * _ST), ("photo", _d(_ST, 8)),
+        ("ci", _d(_ST, 5)), ("rr", _d(_ST, 6)),
+        ("ndr", C.c_int32 * _ST), ("dr_c", _d(_ST, 9)), ("dr_e", _d(_ST, 9)),
+        ("has_dr2", C.c_int32 * _ST), ("dr2", _d(_ST, 4)),
+        ("has_rr2", C.c_int32 * _ST), ("rr2", _d(_ST, 2)),
+        ("cxi_form", C.c_int32 * _ST), ("cxi", _d(_ST, 6)),
+        ("cxr_form", C.c_int32 * _ST), ("cxr", _d(_ST, 6)),
+        ("ncool", C.c_int32 * _ST),
+        ("cool_A", _d(_ST, MC_MAX_COOLFIT)), ("cool_T", _d(_ST, MC_MAX_COOLFIT)),
+    ]
+
+
+def _floats(tok):
+    return [float(t.replace("D", "E").replace("d", "e")) for t in tok]
+
+
+def read_element(path):
+    """Parse one element file (read_element, src/species_mod.f90:358-422) into
+    a dict: name, Z, nstage, transitions (list of per-transition dicts)."""
+    el = {"name": "", "Z": 0, "nstage": 0, "transitions": []}
+    cur = None
+    toks = []
+    with open(path) as f:
+        for line in f:
+            s = line.strip()
+            if not s or s.startswith("#"):
+                continue
+            toks.append(s.split())
+    i = 0
+    while i < len(toks):
+        t = toks[i]
+        key = t[0]
+        if key == "ELEMENT":
+            el["name"], el["Z"], el["nstage"] = t[1].lower(), int(t[2]), int(t[3])
+        elif key == "TRANSITION":
+            it = int(t[1])
+            while len(el["transitions"]) < it:
+                el["transitions"].append(dict(eth=0.0, photo_form=1, photo=[0.0] * 8,
+                                              ci=[0.0] * 5, rr=[0.0] * 6, ndr=0, dr_c=[],
+                                              dr_e=[], dr2=None, rr2=None, cxi_form=0,
+                                              cxi=[0.0] * 6, cxr_form=0, cxr=[0.0] * 6))
+            cur = el["transitions"][it - 1]
+        elif key == "ETH":
+            cur["eth"] = _floats(t[1:2])[0]
+        elif key == "PHOTO":
+            cur["photo"], cur["photo_form"] = _floats(t[1:9]), 1
+        elif key == "PHOTO2":
+            cur["photo"], cur["photo_form"] = _floats(t[1:8]) + [0.0], 2
+        elif key == "CI":
+            cur["ci"] = _floats(t[1:6])
+        elif key == "RR":
+            cur["rr"] = _floats(t[1:7])
+        elif key == "DR":
+            n = int(t[1])
+            vals = _floats(t[2:])
+            # list-directed read continues over following lines when needed
+            while len(vals) < 2 * n:
+                i += 1
+                vals += _floats(toks[i])
+            cur["ndr"], cur["dr_c"], cur["dr_e"] = n, vals[:n], vals[n:2 * n]
+        elif key == "DR2":
+            cur["dr2"] = _floats(t[1:5])
+        elif key == "RR2":
+            cur["rr2"] = _floats(t[1:3])
+        elif key in ("CXI", "CXR"):
+            form = int(float(t[1]))
+            vals = _floats(t[2:8]) if form > 0 else [0.0] * 6
+            vals += [0.0] * (6 - len(vals))
+            if key == "CXI":
+                cur["cxi_form"], cur["cxi"] = form, vals
+            else:
+                cur["cxr_form"], cur["cxr"] = form, vals
+        i += 1
+    return el
+
+
+def read_cooling_fit(path):
+    """cooling_load (src/cooling_mod.f90:98-123): [(A_i, T_i), ...]."""
+    with open(path) as f:
+        rows = [ln.split() for ln in f if ln.strip() and not ln.lstrip().startswith("#")]
+    n = int(rows[0][0])
+    return [tuple(_floats(r[:2])) for r in rows[1:1 + n]]
+
+
+def read_registry(atomic_dir):
+    """Every element file of a data directory with its stage cooling fits:
+    {name: element dict with 'cooling': {stage (1-based): fit}}."""
+    out = {}
+    for name in ORDER:
+        path = os.path.join(atomic_dir, f"element_{name}.txt")
+        if not os.path.exists(path):
+            continue
+        el = read_element(path)
+        el["cooling"] = {}
+        for i in range(1, el["nstage"] + 1):
+            cp = os.path.join(atomic_dir, f"cooling_{name}_{i}.txt")
+            if os.path.exists(cp):
+                el["cooling"][i] = read_cooling_fit(cp)
+        out[name] = el
+    return out
+
+
+def load_snapshot(path=SNAPSHOT):
+    with open(path) as f:
+        data = json.load(f)
+    els = data["elements"]
+    for el in els.values():
+        el["cooling"] = {int(k): [tuple(r) for r in v] for k, v in el["cooling"].items()}
+    return els
+
+
+def registry(abund=None, data=None):
+    """Active registry in species_setup order: the elements with abundance
+    > 0 (par%abund_* defaults overridden by ``abund``), each a dict with its
+    'abund' set."""
+    ab = dict(DEFAULT_ABUND)
+    ab.update(abund or {})
+    data = data if data is not None else load_snapshot()
+    out = []
+    for name in ORDER:
+        if ab.get(name, 0.0) <= 0.0:
+            continue
+        el = dict(data[name])
+        el["abund"] = float(ab[name])
+        out.append(el)
+    if len(out) > MC_MAX_EL:
+        raise ValueError("too many elements")
+    return out
+
+
+def metal_thresholds(elements):
+    """metal_eth(1:nmetal) as src/main.f90:66-82 gathers them."""
+    return [tr["eth"] for el in elements if el["abund"] > 0.0 for tr in el["transitions"]]
+
+
+def to_struct(elements):
+    """ctypes array of Element for mc_species_setup / the oracle."""
+    arr = (Element * max(len(elements), 1))()
+    for k, el in enumerate(elements):
+        e = arr[k]
+        e.name = el["name"].encode()[:3]
+        e.Z, e.nstage, e.abund = el["Z"], el["nstage"], el.get("abund", 0.0)
+        if el["nstage"] > MC_MAX_ST:
+            raise ValueError(f"{el['name']}: {el['nstage']} stages > {MC_MAX_ST}")
+        for it, tr in enumerate(el["transitions"][:el["nstage"] - 1]):
+            e.eth[it] = tr["eth"]
+            e.photo_form[it] = tr["photo_form"]
+            for j, v in enumerate(tr["photo"][:8]):
+                e.photo[it][j] = v
+            for j, v in enumerate(tr["ci"]):
+                e.ci[it][j] = v
+            for j, v in enumerate(tr["rr"]):
+                e.rr[it][j] = v
+            e.ndr[it] = tr["ndr"]
+            for j in range(tr["ndr"]):
+                e.dr_c[it][j], e.dr_e[it][j] = tr["dr_c"][j], tr["dr_e"][j]
+            if tr["dr2"] is not None:
+                e.has_dr2[it] = 1
+                for j, v in enumerate(tr["dr2"]):
+                    e.dr2[it][j] = v
+            if tr["rr2"] is not None:
+                e.has_rr2[it] = 1
+                for j, v in enumerate(tr["rr2"]):
+                    e.rr2[it][j] = v
+            e.cxi_form[it], e.cxr_form[it] = tr["cxi_form"], tr["cxr_form"]
+            for j in range(6):
+                e.cxi[it][j], e.cxr[it][j] = tr["cxi"][j], tr["cxr"][j]
+        for st, fit in el.get("cooling", {}).items():
+            if len(fit) > MC_MAX_COOLFIT:
+                raise ValueError(f"{el['name']} {st}: {len(fit)} cooling terms")
+            e.ncool[st - 1] = len(fit)
+            for j, (a, t) in enumerate(fit):
+                e.cool_A[st - 1][j], e.cool_T[st - 1][j] = a, t
+    return arr

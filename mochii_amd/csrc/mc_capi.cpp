@@ -137,8 +137,37 @@ static int upload_milne(mc_ctx *c)
     return 0;
 }
 
+static void free_metal_arrays(mc_ctx *c)
+{
+    for (void *p : {(void *)c->d_mraw, (void *)c->d_mgam, (void *)c->d_mheat, (void *)c->d_mfrac,
+                    (void *)c->d_mop, (void *)c->d_mopf})
+        if (p) (void)hipFree(p);
+    c->d_mraw = c->d_mgam = c->d_mheat = c->d_mfrac = c->d_mop = nullptr;
+    c->d_mopf = nullptr;
+}
+
+// per-leaf metal arrays (species_setup / species_resize,
+// src/species_mod.f90:143-245), zeroed
+static int alloc_metal_arrays(mc_ctx *c)
+{
+    free_metal_arrays(c);
+    if (c->ntr <= 0 || c->nleaf <= 0) return 0;
+    const size_t n = (size_t)c->nleaf, nt = (size_t)c->ntr, ns = (size_t)c->nst;
+    if (dalloc(&c->d_mraw, n * nt * 2) || dalloc(&c->d_mgam, n * nt) || dalloc(&c->d_mheat, n * nt) ||
+        dalloc(&c->d_mfrac, n * ns) || dalloc(&c->d_mop, n * nt) || dalloc(&c->d_mopf, n * nt))
+        return -100;
+    MC_HIP(hipMemset(c->d_mraw, 0, sizeof(double) * n * nt * 2));
+    MC_HIP(hipMemset(c->d_mgam, 0, sizeof(double) * n * nt));
+    MC_HIP(hipMemset(c->d_mheat, 0, sizeof(double) * n * nt));
+    MC_HIP(hipMemset(c->d_mfrac, 0, sizeof(double) * n * ns));
+    MC_HIP(hipMemset(c->d_mop, 0, sizeof(double) * n * nt));
+    MC_HIP(hipMemset(c->d_mopf, 0, sizeof(float) * n * nt));
+    return 0;
+}
+
 static void free_leaf_arrays(mc_ctx *c)
 {
+    free_metal_arrays(c);
     DevState &s = c->st;
     for (void *p : {(void *)s.hot_slot, (void *)s.hot_leaf, (void *)s.hot_hhe, (void *)s.hot_jt})
         if (p) (void)hipFree(p);
@@ -181,6 +210,7 @@ static int alloc_leaf_arrays(mc_ctx *c)
     MC_HIP(hipMemset(s.rates, 0, sizeof(double) * n * 6));
     c->grid.nleaf = c->nleaf;
     c->state_ready = false;
+    if (alloc_metal_arrays(c)) return -100;
     return setup_hot(c);
 }
 
@@ -206,6 +236,19 @@ extern "C" int mc_init(const mc_params *p, mc_ctx **out)
     }
     if (p->diffuse_field && (p->diffuse_model < 0 || p->diffuse_model > 2)) {
         mc_set_error("diffuse_model must be 0 (threshold), 1 (exponential) or 2 (milne)");
+        return -1;
+    }
+    if (p->cooling_model != MC_COOL_LOW_DENSITY) {
+        mc_set_error("cooling_model: only MC_COOL_LOW_DENSITY is built (the local_ne "
+                     "suppression needs the n-level atom tables)");
+        return -1;
+    }
+    if (p->ci_model != MC_CI_VORONOV) {
+        mc_set_error("ci_model: only MC_CI_VORONOV is built");
+        return -1;
+    }
+    if (p->charge_exchange != MC_CX_TWO_WAY && p->charge_exchange != MC_CX_METAL_ONLY) {
+        mc_set_error("charge_exchange must be MC_CX_TWO_WAY or MC_CX_METAL_ONLY");
         return -1;
     }
     mc_ctx *c = new mc_ctx();
@@ -548,7 +591,9 @@ extern "C" int mc_gas_equilibrium_update(mc_ctx *c, double *max_dx, double *dx_v
         mc_set_error("mc_gas_equilibrium_update: null output");
         return -1;
     }
-    return mc_run_equilibrium(c, max_dx, dx_vol);
+    const int rc = mc_run_equilibrium(c, max_dx, dx_vol);
+    if (rc) return rc;
+    return mc_run_stage_refresh(c);       // gas_opacity_fill after the update
 }
 
 extern "C" int mc_cooling_setup(mc_ctx *c, int nfit, const double *A, const double *Ti)
@@ -585,7 +630,8 @@ extern "C" int mc_gas_thermal_update(mc_ctx *c, double *max_dx, double *max_dte,
         return -1;
     }
     double m[4];
-    const int rc = mc_run_thermal(c, m);
+    int rc = mc_run_thermal(c, m);
+    if (!rc) rc = mc_run_stage_refresh(c); // gas_opacity_fill after the update
     if (rc) return rc;
     *max_dx = m[0];
     *max_dte = m[1];
@@ -722,4 +768,169 @@ extern "C" int mc_trace_ray(mc_ctx *c, int precision, const double *pos, const d
         return -1;
     }
     return mc_run_trace(c, precision, pos, dir, cells, maxn, n);
+}
+
+// ---- trace metals -----------------------------------------------------------
+extern "C" int mc_species_setup(mc_ctx *c, int n, const mc_element *els)
+{
+    CHECK_CTX(c);
+    if (n < 0 || n > MC_MAX_EL || (n > 0 && !els)) {
+        mc_set_error("mc_species_setup: need 0..11 elements");
+        return -1;
+    }
+    int ntr = 0, nst = 0;
+    for (int ie = 0; ie < n; ++ie) {
+        const mc_element &e = els[ie];
+        if (e.nstage < 2 || e.nstage > MC_MAX_ST || !(e.abund > 0.0)) {
+            mc_set_error("mc_species_setup: element needs 2..6 stages and abund > 0");
+            return -1;
+        }
+        for (int it = 0; it < e.nstage - 1; ++it) {
+            if ((e.photo_form[it] != 1 && e.photo_form[it] != 2) || e.ndr[it] < 0 ||
+                e.ndr[it] > 9 || e.cxi_form[it] < 0 || e.cxi_form[it] > 7 || e.cxr_form[it] < 0 ||
+                e.cxr_form[it] > 7) {
+                mc_set_error("mc_species_setup: bad transition record");
+                return -1;
+            }
+        }
+        for (int i = 0; i < e.nstage; ++i)
+            if (e.ncool[i] < 0 || e.ncool[i] > MC_MAX_COOLFIT) {
+                mc_set_error("mc_species_setup: cooling fits have at most 8 terms");
+                return -1;
+            }
+        ntr += e.nstage - 1;
+        nst += e.nstage;
+    }
+    if (ntr > MC_GPU_MAXTR) {
+        mc_set_error("mc_species_setup: " + std::to_string(ntr) +
+                     " metal transitions; the device cell solve holds at most 16 "
+                     "(C N O Ne S Ar have 15)");
+        return -1;
+    }
+    DevMetalTab &m = c->mtab;
+    std::memset(&m, 0, sizeof(m));
+    m.nel = n; m.ntr = ntr; m.nst = nst;
+    int t = 0, st = 0;
+    for (int ie = 0; ie < n; ++ie) {
+        const mc_element &e = els[ie];
+        const int t0 = t;
+        const bool isC = std::strcmp(e.name, "c") == 0, isO = std::strcmp(e.name, "o") == 0;
+        for (int i = 0; i < e.nstage; ++i) {
+            m.s_ncool[st + i] = e.ncool[i];
+            m.s_charge[st + i] = i;
+            for (int k = 0; k < e.ncool[i]; ++k) {
+                m.s_A[st + i][k] = e.cool_A[i][k];
+                m.s_T[st + i][k] = e.cool_T[i][k];
+            }
+        }
+        for (int it = 0; it < e.nstage - 1; ++it, ++t) {
+            m.t_first[t] = t0;
+            m.t_last[t] = it == e.nstage - 2;
+            m.t_it[t] = it;
+            m.t_st[t] = st + it;
+            m.t_cII[t] = isC && it == 0;
+            m.t_oI[t] = isO && it == 0;
+            m.t_ab[t] = e.abund;
+            m.eth[t] = e.eth[it];
+            m.photo_form[t] = e.photo_form[it];
+            for (int k = 0; k < 8; ++k) m.photo[t][k] = e.photo[it][k];
+            for (int k = 0; k < 5; ++k) m.ci[t][k] = e.ci[it][k];
+            m.ci_ratio[t] = 1.0;                       // ci_model 'voronov'
+            m.has_rr2[t] = e.has_rr2[it];
+            for (int k = 0; k < 6; ++k) m.rr[t][k] = e.rr[it][k];
+            for (int k = 0; k < 2; ++k) m.rr2[t][k] = e.rr2[it][k];
+            m.ndr[t] = e.ndr[it];
+            for (int k = 0; k < 9; ++k) { m.dr_c[t][k] = e.dr_c[it][k]; m.dr_e[t][k] = e.dr_e[it][k]; }
+            m.has_dr2[t] = e.has_dr2[it];
+            for (int k = 0; k < 4; ++k) m.dr2[t][k] = e.dr2[it][k];
+            m.cxi_form[t] = e.cxi_form[it];
+            m.cxr_form[t] = e.cxr_form[it];
+            for (int k = 0; k < 6; ++k) { m.cxi[t][k] = e.cxi[it][k]; m.cxr[t][k] = e.cxr[it][k]; }
+        }
+        st += e.nstage;
+    }
+    // threshold order: stable insertion sort (src/species_mod.f90:205-226)
+    for (int i = 0; i < ntr; ++i) { m.flat_t[i] = i; m.flat_eth[i] = m.eth[i]; }
+    for (int i = 1; i < ntr; ++i) {
+        const int ti = m.flat_t[i];
+        const double e = m.flat_eth[i];
+        int j = i - 1;
+        while (j >= 0 && m.flat_eth[j] > e) {
+            m.flat_t[j + 1] = m.flat_t[j];
+            m.flat_eth[j + 1] = m.flat_eth[j];
+            --j;
+        }
+        m.flat_t[j + 1] = ti;
+        m.flat_eth[j + 1] = e;
+    }
+    for (int k = 0; k < ntr; ++k) m.t_flat[m.flat_t[k]] = k;
+    c->elements.assign(els, els + n);
+    c->nel = n; c->ntr = ntr; c->nst = nst;
+    if (alloc_metal_arrays(c)) return -100;
+    if (c->state_ready) return mc_run_stage_refresh(c);
+    return 0;
+}
+
+// element ie's offsets: first transition and first stage (element order)
+static int element_offsets(mc_ctx *c, int ie, int &t0, int &s0)
+{
+    if (ie < 0 || ie >= c->nel) {
+        mc_set_error("element index out of range");
+        return -1;
+    }
+    t0 = 0; s0 = 0;
+    for (int k = 0; k < ie; ++k) {
+        t0 += c->elements[k].nstage - 1;
+        s0 += c->elements[k].nstage;
+    }
+    return 0;
+}
+
+extern "C" int mc_species_fractions_get(mc_ctx *c, int ie, double *frac)
+{
+    CHECK_CTX(c);
+    int t0, s0;
+    if (element_offsets(c, ie, t0, s0) || !frac) return -1;
+    const int ns = c->elements[ie].nstage, nl = c->nleaf;
+    std::vector<double> h((size_t)nl * c->nst);
+    MC_HIP(hipMemcpy(h.data(), c->d_mfrac, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int il = 0; il < nl; ++il)
+        for (int i = 0; i < ns; ++i) frac[(size_t)il * ns + i] = h[(size_t)il * c->nst + s0 + i];
+    return 0;
+}
+
+extern "C" int mc_species_rates_get(mc_ctx *c, int ie, double *gam, double *heat)
+{
+    CHECK_CTX(c);
+    int t0, s0;
+    if (element_offsets(c, ie, t0, s0)) return -1;
+    const int nt = c->elements[ie].nstage - 1, nl = c->nleaf;
+    std::vector<double> h((size_t)nl * c->ntr);
+    for (int q = 0; q < 2; ++q) {
+        double *dst = q ? heat : gam;
+        if (!dst) continue;
+        MC_HIP(hipMemcpy(h.data(), q ? c->d_mheat : c->d_mgam, sizeof(double) * h.size(),
+                         hipMemcpyDeviceToHost));
+        for (int il = 0; il < nl; ++il)
+            for (int it = 0; it < nt; ++it)
+                dst[(size_t)il * nt + it] = h[(size_t)il * c->ntr + t0 + it];
+    }
+    return 0;
+}
+
+extern "C" int mc_species_tallies_get(mc_ctx *c, int ie, double *raw)
+{
+    CHECK_CTX(c);
+    int t0, s0;
+    if (element_offsets(c, ie, t0, s0) || !raw) return -1;
+    const int nt = c->elements[ie].nstage - 1, nl = c->nleaf, ntr = c->ntr;
+    std::vector<double> h((size_t)nl * ntr * 2);
+    MC_HIP(hipMemcpy(h.data(), c->d_mraw, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int il = 0; il < nl; ++il)
+        for (int it = 0; it < nt; ++it) {
+            const int k = c->mtab.t_flat[t0 + it];
+            for (int q = 0; q < 2; ++q)
+                raw[((size_t)il * nt + it) * 2 + q] = h[((size_t)il * ntr + k) * 2 + q];
+        }
+    return 0;
 }

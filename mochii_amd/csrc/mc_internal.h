@@ -73,6 +73,41 @@ struct MlpState {
     bool ready = false;
 };
 
+// Trace-metal registry on the device (species_mod module state,
+// src/species_mod.f90:54-128), read with wave-uniform indices.  Transitions t
+// are in element order (element by element, stage by stage), stages s
+// likewise; flat_t lists the transitions by threshold (the packet cache
+// order of species_packet_cross_sections).  The cell-solve kernels keep the
+// (leaf, T) rate cache in registers with loops unrolled to MC_GPU_MAXTR.
+#define MC_GPU_MAXTR 16
+#define MC_GPU_MAXST (MC_GPU_MAXTR + MC_MAX_EL)
+struct DevMetalTab {
+    int32_t nel, ntr, nst;
+    int32_t t_first[MC_GPU_MAXTR];   // first transition of t's element
+    int32_t t_last[MC_GPU_MAXTR];    // t is its element's last transition
+    int32_t t_it[MC_GPU_MAXTR];      // transition index within the element
+    int32_t t_st[MC_GPU_MAXTR];      // stage s of t's lower (ionised-from) stage
+    int32_t t_flat[MC_GPU_MAXTR];    // position of t in threshold order
+    int32_t t_cII[MC_GPU_MAXTR];     // t = C I -> C II (its upper stage is C II)
+    int32_t t_oI[MC_GPU_MAXTR];      // t = O I -> O II (its lower stage is O I)
+    double  t_ab[MC_GPU_MAXTR];      // abundance of t's element
+    double  eth[MC_GPU_MAXTR];
+    int32_t photo_form[MC_GPU_MAXTR];
+    double  photo[MC_GPU_MAXTR][8];
+    double  ci[MC_GPU_MAXTR][5];
+    double  ci_ratio[MC_GPU_MAXTR];
+    int32_t has_rr2[MC_GPU_MAXTR], ndr[MC_GPU_MAXTR], has_dr2[MC_GPU_MAXTR];
+    double  rr[MC_GPU_MAXTR][6], rr2[MC_GPU_MAXTR][2];
+    double  dr_c[MC_GPU_MAXTR][9], dr_e[MC_GPU_MAXTR][9], dr2[MC_GPU_MAXTR][4];
+    int32_t cxi_form[MC_GPU_MAXTR], cxr_form[MC_GPU_MAXTR];
+    double  cxi[MC_GPU_MAXTR][6], cxr[MC_GPU_MAXTR][6];
+    int32_t flat_t[MC_GPU_MAXTR];
+    double  flat_eth[MC_GPU_MAXTR];
+    int32_t s_ncool[MC_GPU_MAXST];
+    int32_t s_charge[MC_GPU_MAXST];  // stage j of its element has charge j - 1
+    double  s_A[MC_GPU_MAXST][MC_MAX_COOLFIT], s_T[MC_GPU_MAXST][MC_MAX_COOLFIT];
+};
+
 // Small launch/band tables shared by every kernel of a context
 // (read through the scalar/constant cache).
 struct DevTables {
@@ -201,6 +236,17 @@ struct mc_ctx {
     int32_t *d_perm;
     void *d_sort_tmp;
     size_t sort_tmp_bytes;
+    // trace metals: registry (host copy and device table), switches, path
+    // tallies mraw[(il*ntr + k)*2 + {0,1}] (threshold order k), rates
+    // mgam / mheat[il*ntr + t] and stage fractions mfrac[il*nst + s] (element
+    // order), opacity weights abund x fraction of each transition's lower
+    // stage mop[il*ntr + k] (FP64) and n_H d2cm x that mopf (FP32, mixed form)
+    int nel, ntr, nst;
+    std::vector<mc_element> elements;
+    DevMetalTab mtab;
+    DevMetalTab *d_mtab;
+    double *d_mraw, *d_mgam, *d_mheat, *d_mfrac, *d_mop;
+    float *d_mopf;
     // last pass results
     double ledger[3];
     int64_t segs[2];
@@ -224,6 +270,7 @@ int mc_run_rates(mc_ctx *c);
 int mc_run_equilibrium(mc_ctx *c, double *max_dx, double *dx_vol);
 int mc_run_residual(mc_ctx *c, double *eps_max, double *eps_vol);
 int mc_run_opacity_fill(mc_ctx *c);
+int mc_run_stage_refresh(mc_ctx *c);
 int mc_run_thermal(mc_ctx *c, double out4[4]);
 int mc_run_mlp_infer(mc_ctx *c, const float *dX, int n, float *dOut);
 int mc_run_mlp_features(mc_ctx *c, float *dX);

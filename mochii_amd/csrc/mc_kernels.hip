@@ -47,7 +47,31 @@ struct KArgs {
     const double *dif;       // per-leaf diffuse emission {ch0, ch1, ch2, total}
     const double *dcdf;      // diffuse leaf CDF
     DevMilne milne;
+    // trace metals (the registry itself is c_mt)
+    int32_t ntr;             // metal transitions (0: metal path off)
+    int32_t metal_abs;       // metal opacity in the walk (par%ion_metal_abs)
+    double *mraw;            // path tallies [(il*ntr + k)*2 + {0,1}]
+    const double *mop;       // opacity weights abund x stage fraction [il*ntr + k]
+    const float *mopf;       // the same x n_H d2cm in FP32 (mixed form)
 };
+
+// The trace-metal registry of the calling context, copied in before every
+// call that uses it (calls are synchronous and come from one host thread).
+// Read with wave-uniform indices through the scalar cache.
+__constant__ DevMetalTab c_mt;
+
+// H/He recombination and collisional-ionisation coefficients at one T
+struct RateSet { double aH, aHe2, aHe3, cH, cHe1, cHe2; };
+
+#include "mc_metal_dev.h"
+
+// Metal cross sections of a lane's packet, threshold order, in the walk's
+// dynamic LDS: [k][kBlock] doubles, k < the registry's transition count.
+__device__ __forceinline__ double &msig(int k)
+{
+    extern __shared__ double mc_msig[];
+    return mc_msig[k * kBlock + threadIdx.x];
+}
 
 // ---------------------------------------------------------------------------
 // launch helpers
@@ -347,6 +371,7 @@ struct Lane {
     // [-1,1]^3 in units of the half width (octree)
     float px, py, pz, fkx, fky, fkz, fsHI, fsHeI, fsHeII, fdust;
     float fexpo, ftau, ftau_in, ftcur;
+    int nmet;                // metal transitions below the packet's energy
     // tally of this step, scored after the step by the whole wave
     int s_il;
     double s_pl;
@@ -360,13 +385,23 @@ template <int P> struct Prec;
 template <> struct Prec<MC_FP64> { static constexpr bool mixed = false; };
 template <> struct Prec<MC_MIXED> { static constexpr bool mixed = true; };
 
+// ion_packet_opacity with species_cached_opacity (src/ion_packet_mod.f90:55-86,
+// src/species_mod.f90:291-301)
+template <int M>
 __device__ __forceinline__ double opacity_d(const KArgs &a, const Lane &L, int il)
 {
 #pragma clang fp contract(off)
     const DevState &s = a.s;
-    double kap = s.nH[il] * (s.xHI[il] * L.sHI +
-                             a.He_abund * (s.xHeI[il] * L.sHeI + s.xHeII[il] * L.sHeII)) *
-                 a.d2cm;
+    double gas = s.xHI[il] * L.sHI + a.He_abund * (s.xHeI[il] * L.sHeI + s.xHeII[il] * L.sHeII);
+    if constexpr (M != 0) {
+        if (a.metal_abs) {
+            double metal = 0.0;
+            const double *w = a.mop + (size_t)il * a.ntr;
+            for (int k = 0; k < L.nmet; ++k) metal = metal + w[k] * msig(k);
+            gas = gas + metal;
+        }
+    }
+    double kap = s.nH[il] * gas * a.d2cm;
     if (a.add_dust) {
         kap = kap + s.rhokap[il] * L.dabs;
         if (a.dust_scatter) kap = kap + s.rhokap[il] * L.dsca;
@@ -374,10 +409,18 @@ __device__ __forceinline__ double opacity_d(const KArgs &a, const Lane &L, int i
     return kap;
 }
 
+template <int M>
 __device__ __forceinline__ float opacity_f(const KArgs &a, const Lane &L, int il)
 {
     const float4 o = a.s.opac[il];
-    return o.x * L.fsHI + o.y * L.fsHeI + o.z * L.fsHeII + o.w * L.fdust;
+    float k = o.x * L.fsHI + o.y * L.fsHeI + o.z * L.fsHeII + o.w * L.fdust;
+    if constexpr (M != 0) {
+        if (a.metal_abs) {
+            const float *w = a.mopf + (size_t)il * a.ntr;
+            for (int q = 0; q < L.nmet; ++q) k = k + w[q] * (float)msig(q);
+        }
+    }
+    return k;
 }
 
 struct Counters {
@@ -419,7 +462,18 @@ __device__ __forceinline__ void tally_rows(const KArgs &a, int il, double *&h, d
     }
 }
 
+// species_rate_path_score coefficients of transition k (threshold order)
+// for a packet of energy E (src/species_mod.f90:329-343)
+__device__ __forceinline__ void metal_coeff(int k, double E, double &c1, double &c2)
+{
+    const double sg = msig(k);
+    c1 = sg / (E * MC_EV2ERG);
+    c2 = sg * fmax(1.0 - c_mt.flat_eth[k] / E, 0.0);
+}
+
 // one lane's segment: 1 band-J atomic + one per nonzero rate coefficient
+// (+ 2 per metal transition below the packet's energy)
+template <int M>
 __device__ __forceinline__ void tally_one(const KArgs &a, const Lane &L, int il, double pl)
 {
     double *h, *j;
@@ -429,6 +483,15 @@ __device__ __forceinline__ void tally_one(const KArgs &a, const Lane &L, int il,
     for (int k = 0; k < 6; ++k)
         if (L.coeff[k] != 0.0) unsafeAtomicAdd(h + k, pl * L.coeff[k]);
     if (a.add_dust && L.dabs != 0.0) unsafeAtomicAdd(h + 6, pl * L.dabs);
+    if constexpr (M != 0) {
+        double *r = a.mraw + (size_t)il * a.ntr * 2;
+        for (int k = 0; k < L.nmet; ++k) {
+            double c1, c2;
+            metal_coeff(k, L.E, c1, c2);
+            unsafeAtomicAdd(r + 2 * k, pl * c1);
+            if (c2 != 0.0) unsafeAtomicAdd(r + 2 * k + 1, pl * c2);
+        }
+    }
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -448,6 +511,7 @@ constexpr int kAggMin = 3;      // smallest group worth a wave reduction
 // by each packet's energy bin) are binned in the wave's LDS row first and go
 // out as one atomic per nonzero bin.  Lanes outside a large group use
 // per-lane atomics.  jbin: this wave's nnu LDS bins, zero between calls.
+template <int M>
 __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, double *jbin)
 {
     bool has = L.s_il >= 0;
@@ -478,6 +542,21 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
                     if (lane == leader) unsafeAtomicAdd(h + 6, sum);
                 }
             }
+            if constexpr (M != 0) {
+                double *r = a.mraw + (size_t)lil * a.ntr * 2;
+                for (int k = 0; k < a.ntr; ++k) {
+                    const bool on = in && k < L.nmet;
+                    if (!__ballot(on)) break;
+                    double c1 = 0.0, c2 = 0.0;
+                    if (on) metal_coeff(k, L.E, c1, c2);
+                    const double s1 = wave_sum(on ? L.s_pl * c1 : 0.0);
+                    const double s2 = wave_sum(on ? L.s_pl * c2 : 0.0);
+                    if (lane == leader) {
+                        unsafeAtomicAdd(r + 2 * k, s1);
+                        if (s2 != 0.0) unsafeAtomicAdd(r + 2 * k + 1, s2);
+                    }
+                }
+            }
             // band J of the group: per-wave bins in LDS, then one atomic
             // per nonzero bin
             if (in) atomicAdd(&jbin[L.inu], L.s_pl);
@@ -497,7 +576,7 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
             if (in) has = false;
         }
     }
-    if (has) tally_one(a, L, L.s_il, L.s_pl);
+    if (has) tally_one<M>(a, L, L.s_il, L.s_pl);
     L.s_il = -1;
 }
 
@@ -791,12 +870,12 @@ __device__ __forceinline__ void start_flight(const KArgs &a, Lane &L, double tau
 
 // scattering event at the end of a flight that stayed inside
 // (transport_ion_packet loop body, FP64 in both forms: events are rare)
-template <int P, int G>
+template <int P, int G, int M>
 __device__ __forceinline__ void scatter_event(const KArgs &a, Lane &L)
 {
     const int il = L.leaf;
     if (il < 0) { L.phase = 0; return; }
-    const double kap = opacity_d(a, L, il);
+    const double kap = opacity_d<M>(a, L, il);
     L.wgt = L.wgt * (a.s.rhokap[il] * L.dsca / fmax(kap, DBL_MIN));
     if (L.wgt < kWmin) {
         if (rng_u(a.seed, a.stream, L.id, L.nk++) > kPsurv) { L.phase = 0; return; }
@@ -883,7 +962,7 @@ __device__ __forceinline__ double seg_lum_f(Lane &L, double wlc, float kc, float
 }
 
 // one segment of the current walk (edge walk or tau flight), car DDA
-template <int P>
+template <int P, int M>
 __device__ __forceinline__ void car_step(const KArgs &a, Lane &L, Counters &cn)
 {
     const DevGrid &g = a.g;
@@ -892,7 +971,7 @@ __device__ __forceinline__ void car_step(const KArgs &a, Lane &L, Counters &cn)
 #pragma clang fp contract(off)
         const int ax = dda_axis_d(L);
         const double seg = tm_of(L, ax) - L.tcur;
-        const double kap = opacity_d(a, L, il);
+        const double kap = opacity_d<M>(a, L, il);
         if (L.phase == 1) {
             double pl;
             if (kap * seg > 0.0) {
@@ -913,7 +992,7 @@ __device__ __forceinline__ void car_step(const KArgs &a, Lane &L, Counters &cn)
                 L.tcur = L.tcur + ds;
                 L.x = L.x + L.tcur * L.kx; L.y = L.y + L.tcur * L.ky; L.z = L.z + L.tcur * L.kz;
                 L.leaf = il;
-                scatter_event<P, kCarDDA>(a, L);
+                scatter_event<P, kCarDDA, M>(a, L);
                 return;
             }
             if (L.nscatt > 0) score(a, L, il, L.wl * seg, cn);
@@ -923,7 +1002,7 @@ __device__ __forceinline__ void car_step(const KArgs &a, Lane &L, Counters &cn)
     } else {
         int ax;
         const float t = exit_f(L, ax);
-        const float kc = opacity_f(a, L, il) * (float)g.dcell;   // per cell length
+        const float kc = opacity_f<M>(a, L, il) * (float)g.dcell;   // per cell length
         const float dt = kc * t;
         const double wlc = L.wl * g.dcell;
         if (L.phase == 1) {
@@ -940,7 +1019,7 @@ __device__ __forceinline__ void car_step(const KArgs &a, Lane &L, Counters &cn)
                 L.y = g.ymin + ((double)L.iy + (double)L.py) * g.dcell;
                 L.z = g.zmin + ((double)L.iz + (double)L.pz) * g.dcell;
                 L.leaf = il;
-                scatter_event<P, kCarDDA>(a, L);
+                scatter_event<P, kCarDDA, M>(a, L);
                 return;
             }
             if (L.nscatt > 0) score(a, L, il, wlc * (double)t, cn);
@@ -952,7 +1031,7 @@ __device__ __forceinline__ void car_step(const KArgs &a, Lane &L, Counters &cn)
 
 // one segment of the face-exit walk (raytrace_ion_to_edge_amr /
 // raytrace_ion_to_tau_amr, src/raytrace_amr.f90:348-486)
-template <int P>
+template <int P, int M>
 __device__ __forceinline__ void oct_step(const KArgs &a, Lane &L, Counters &cn)
 {
     const DevGrid &g = a.g;
@@ -963,7 +1042,7 @@ __device__ __forceinline__ void oct_step(const KArgs &a, Lane &L, Counters &cn)
         double t;
         int f;
         cell_exit_d(G, L.ox, L.oy, L.oz, L.kx, L.ky, L.kz, t, f);
-        const double kap = opacity_d(a, L, il);
+        const double kap = opacity_d<M>(a, L, il);
         if (L.phase == 1) {
             double pl;
             if (kap * t > 0.0) {
@@ -986,7 +1065,7 @@ __device__ __forceinline__ void oct_step(const KArgs &a, Lane &L, Counters &cn)
                 const double ds = kap > 0.0 ? (L.tau_in - L.tau) / kap : t;
                 if (L.nscatt > 0) score(a, L, il, L.wl * ds, cn);
                 L.ox = L.ox + ds * L.kx; L.oy = L.oy + ds * L.ky; L.oz = L.oz + ds * L.kz;
-                scatter_event<P, kFaceExit>(a, L);
+                scatter_event<P, kFaceExit, M>(a, L);
                 return;
             }
             if (L.nscatt > 0) score(a, L, il, L.wl * t, cn);
@@ -1001,7 +1080,7 @@ __device__ __forceinline__ void oct_step(const KArgs &a, Lane &L, Counters &cn)
         int ax;
         const float t = exit_oct_f(L, ax);                // units of the half width
         const float hw = L.gf.w * (float)g.half;          // half width, code units
-        const float kc = opacity_f(a, L, il) * hw;        // per local unit
+        const float kc = opacity_f<M>(a, L, il) * hw;        // per local unit
         const float dt = kc * t;
         const double wlc = L.wl * (double)hw;
         if (L.phase == 1) {
@@ -1014,7 +1093,7 @@ __device__ __forceinline__ void oct_step(const KArgs &a, Lane &L, Counters &cn)
                 const float ds = kc > 0.0f ? (L.ftau_in - L.ftau) / kc : t;
                 if (L.nscatt > 0) score(a, L, il, wlc * (double)ds, cn);
                 move_oct_f(L, ds);
-                scatter_event<P, kFaceExit>(a, L);
+                scatter_event<P, kFaceExit, M>(a, L);
                 return;
             }
             if (L.nscatt > 0) score(a, L, il, wlc * (double)t, cn);
@@ -1024,7 +1103,7 @@ __device__ __forceinline__ void oct_step(const KArgs &a, Lane &L, Counters &cn)
     }
 }
 
-template <int P, int G>
+template <int P, int G, int M>
 __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, Counters &cn)
 {
     const PacketBuf &pk = a.pk;
@@ -1046,6 +1125,17 @@ __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, 
     L.coeff[3] = L.sHI * fmax(1.0 - MC_ETH_HI / L.E, 0.0);
     L.coeff[4] = L.sHeI * fmax(1.0 - MC_ETH_HEI / L.E, 0.0);
     L.coeff[5] = L.sHeII * fmax(1.0 - MC_ETH_HEII / L.E, 0.0);
+    if constexpr (M != 0) {
+        // species_packet_cross_sections (src/species_mod.f90:263-275)
+        int n = 0;
+        const int ntr = a.ntr;
+        for (int k = 0; k < ntr; ++k) {
+            if (L.E < c_mt.flat_eth[k]) break;
+            msig(k) = m_sigma(&c_mt, c_mt.flat_t[k], L.E);
+            n = k + 1;
+        }
+        L.nmet = n;
+    }
     L.wgt = 1.0;
     L.nscatt = 0;
     L.nk = 0;
@@ -1078,7 +1168,7 @@ __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, 
     }
 }
 
-template <int P, int G>
+template <int P, int G, int M>
 __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
 {
     __shared__ double jbins[kBlock / 64][MC_MAXBINS];
@@ -1090,6 +1180,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
     Lane L;
     L.phase = 0;
     L.s_il = -1;
+    L.nmet = 0;
     Counters cn = {0.0, 0.0, 0.0, 0ull, 0ull};
     const int lane = threadIdx.x & 63;
     for (;;) {
@@ -1104,17 +1195,17 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
             base = __shfl(base, leader);
             if (L.phase == 0) {
                 const unsigned long long r = base + __popcll(need & ((1ull << lane) - 1ull));
-                if ((int64_t)r < a.npk) load_packet<P, G>(a, L, (int64_t)r, cn);
+                if ((int64_t)r < a.npk) load_packet<P, G, M>(a, L, (int64_t)r, cn);
                 else L.phase = 3;
             }
         }
         const bool busy = L.phase == 1 || L.phase == 2;
         if (!__ballot(busy)) break;
         if (busy) {
-            if constexpr (G == kCarDDA) car_step<P>(a, L, cn);
-            else oct_step<P>(a, L, cn);
+            if constexpr (G == kCarDDA) car_step<P, M>(a, L, cn);
+            else oct_step<P, M>(a, L, cn);
         }
-        flush_tally(a, L, lane, jbin);
+        flush_tally<M>(a, L, lane, jbin);
     }
     // wave reduction of the ledger and segment counts, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
@@ -1207,8 +1298,6 @@ __device__ __forceinline__ double voronov(double T, double dE, double P, double 
     const double U = dE * MC_EV2ERG / (MC_KBOLTZ * T);
     return A * (1.0 + P * sqrt(U)) * pow(U, K) * exp(-U) / (X + U);
 }
-
-struct RateSet { double aH, aHe2, aHe3, cH, cHe1, cHe2; };
 
 __device__ RateSet rates_at(double T, bool caseA)
 {
@@ -1480,8 +1569,10 @@ __global__ __launch_bounds__(kBlock) void k_launch_diffuse(KArgs a)
 
 // gas_equilibrium_update at fixed T; partial metrics per block:
 // {max|dx|, sum V|dx|, sum V x_HII}
+template <int MT>
 __global__ __launch_bounds__(kBlock) void k_equilibrium(DevGrid g, DevState s, double T, bool caseA,
-                                                        double yHe, double w, double d2cm)
+                                                        double yHe, double w, double d2cm,
+                                                        MetalSw ms)
 {
     const int il = blockIdx.x * kBlock + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
@@ -1493,9 +1584,16 @@ __global__ __launch_bounds__(kBlock) void k_equilibrium(DevGrid g, DevState s, d
         } else {
             const RateSet r = rates_at(T, caseA);
             const size_t n = g.nleaf;
-            double xHI = s.xHI[il], xHeI = s.xHeI[il], xHeII = s.xHeII[il];
-            solve_cell(r, s.rates[il], s.rates[n + il], s.rates[2 * n + il], nH, yHe, xHI, xHeI,
-                       xHeII);
+            double xHI = s.xHI[il], xHeI = s.xHeI[il], xHeII = s.xHeII[il], ne;
+            MCache<(MT > 0 ? MT : 1)> cc;
+            if constexpr (MT == 0) {
+                ne = solve_cell(r, s.rates[il], s.rates[n + il], s.rates[2 * n + il], nH, yHe, xHI,
+                                xHeI, xHeII);
+            } else {
+                m_prepare<MT>((&c_mt), ms.gam + (size_t)il * ms.ntr, T, cc);
+                ne = solve_cell_m<MT>(ms, cc, r, s.rates[il], s.rates[n + il], s.rates[2 * n + il],
+                                      nH, yHe, xHI, xHeI, xHeII);
+            }
             const double oHI = s.xHI[il];
             const double nHI = (1.0 - w) * oHI + w * xHI;
             const double nHeI = (1.0 - w) * s.xHeI[il] + w * xHeI;
@@ -1504,7 +1602,10 @@ __global__ __launch_bounds__(kBlock) void k_equilibrium(DevGrid g, DevState s, d
             s.xHI[il] = nHI;
             s.xHeI[il] = nHeI;
             s.xHeII[il] = nHeII;
-            s.ne[il] = nH * ((1.0 - nHI) + yHe * (nHeII + 2.0 * xHeIII));
+            double nen = nH * ((1.0 - nHI) + yHe * (nHeII + 2.0 * xHeIII));
+            if constexpr (MT != 0)
+                if (ms.metal_ne) nen = charge_neutrality_m<MT>((&c_mt), cc, nH, nen, nHI, ne);
+            s.ne[il] = nen;
             s.opac[il] = opac_of(nH, nHI, nHeI, nHeII, s.rhokap[il], yHe, d2cm);
             const double wd = 2.0 * leaf_half(g, il), vol = (wd * wd) * wd;
             const double d = fabs(nHI - oHI);
@@ -1583,6 +1684,45 @@ __device__ double net_rate_lr(const DevCool &cl, const CellIn &c, double T, bool
     return heat - cool;
 }
 
+// net_rate with the metal terms (src/thermal_mod.f90:38-71); lr (may be
+// null) receives ln(heating / cooling) with the metal heating and cooling on
+// their sides.  The rate cache lives only through the solve and the stage
+// fractions.
+struct NetOut { double net, lr, xHI, xHeI, xHeII, ne; };
+
+template <int MT>
+__device__ __noinline__ NetOut net_rate_mo(const DevCool &cl, const MetalSw &ms, int il,
+                                           const CellIn &c, double T, bool caseA, double y,
+                                           double xHI, double xHeI, double xHeII)
+{
+    double ne;
+    const RateSet r = rates_at(T, caseA);
+    double lo[MT], up[MT];
+    {
+        MCache<MT> cc;
+        m_prepare<MT>((&c_mt), ms.gam + (size_t)il * ms.ntr, T, cc);
+        ne = solve_cell_m<MT>(ms, cc, r, c.gH, c.gHe1, c.gHe2, c.nH, y, xHI, xHeI, xHeII);
+        m_fracs<MT>((&c_mt), cc, ne, c.nH * xHI, c.nH * (1.0 - xHI), lo, up);
+    }
+    const double heat = c.nH * (xHI * c.hH + y * (xHeI * c.hHe1 + xHeII * c.hHe2));
+    const double cool = cooling_total(cl, r, T, c.nH, ne, xHI, xHeI, xHeII, y, caseA);
+    const double gff[4] = {gbar_ff(cl, T, 1), gbar_ff(cl, T, 2), gbar_ff(cl, T, 3),
+                           gbar_ff(cl, T, 4)};
+    double mc, mch, mff, mh;
+    m_thermal_terms<MT>(ms, lo, up, ms.heat + (size_t)il * ms.ntr, T, c.nH, ne, c.nH * xHI, gff,
+                        mc, mch, mff, mh);
+    double net = heat - cool;
+    net = net - mc;
+    net = net - mch;
+    if (ms.metal_ff) net = net - mff;
+    if (ms.metal_heat) net = net + mh;
+    NetOut o;
+    o.net = net;
+    o.lr = log(fmax(heat + mh, DBL_MIN)) - log(fmax(cool + mc + mch + mff, DBL_MIN));
+    o.xHI = xHI; o.xHeI = xHeI; o.xHeII = xHeII; o.ne = ne;
+    return o;
+}
+
 // gas_thermal_update: T_e in [te_min, te_max] with net(T) = 0 to a verified
 // sign-change bracket thi/tlo - 1 < 1e-5, relaxation on x, T taken directly.
 //   W = MC_WARM_NONE   the reference's bisection on log T from the full range;
@@ -1600,10 +1740,10 @@ __device__ double net_rate_lr(const DevCool &cl, const CellIn &c, double T, bool
 // leaves (those not pinned at te_min / te_max)}
 constexpr int kNewtonMax = 24;
 
-template <int W>
+template <int W, int MT>
 __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCool cl, bool caseA,
                                                     double y, double w, double tmin, double tmax,
-                                                    double d2cm, const float *pred)
+                                                    double d2cm, const float *pred, MetalSw ms)
 {
     const int il = blockIdx.x * kBlock + threadIdx.x;
     double v[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1617,6 +1757,27 @@ __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCo
             CellIn c = {nH, s.rates[il], s.rates[n + il], s.rates[2 * n + il],
                         s.rates[3 * n + il], s.rates[4 * n + il], s.rates[5 * n + il]};
             double xHI = s.xHI[il], xHeI = s.xHeI[il], xHeII = s.xHeII[il], ne, te;
+            auto net_rate = [&](const DevCool &cl_, const CellIn &c_, double T, bool cA, double y_,
+                                double &a1, double &a2, double &a3, double &ne_) -> double {
+                if constexpr (MT == 0) {
+                    return ::net_rate(cl_, c_, T, cA, y_, a1, a2, a3, ne_);
+                } else {
+                    const NetOut o = net_rate_mo<MT>(cl_, ms, il, c_, T, cA, y_, a1, a2, a3);
+                    a1 = o.xHI; a2 = o.xHeI; a3 = o.xHeII; ne_ = o.ne;
+                    return o.net;
+                }
+            };
+            auto net_rate_lr = [&](const DevCool &cl_, const CellIn &c_, double T, bool cA,
+                                   double y_, double &a1, double &a2, double &a3, double &ne_,
+                                   double &lr) -> double {
+                if constexpr (MT == 0) {
+                    return ::net_rate_lr(cl_, c_, T, cA, y_, a1, a2, a3, ne_, lr);
+                } else {
+                    const NetOut o = net_rate_mo<MT>(cl_, ms, il, c_, T, cA, y_, a1, a2, a3);
+                    a1 = o.xHI; a2 = o.xHeI; a3 = o.xHeII; ne_ = o.ne; lr = o.lr;
+                    return o.net;
+                }
+            };
             double tlo = tmin, thi = tmax;
             int evals = 2, fallback = 0, bracketed = 0;
             const double nlo = net_rate(cl, c, tlo, caseA, y, xHI, xHeI, xHeII, ne);
@@ -1696,7 +1857,14 @@ __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCo
             const double nHeI = (1.0 - w) * s.xHeI[il] + w * xHeI;
             const double nHeII = (1.0 - w) * s.xHeII[il] + w * xHeII;
             const double x3 = fmax(0.0, 1.0 - nHeI - nHeII);
-            const double nen = nH * ((1.0 - nHI) + y * (nHeII + 2.0 * x3));
+            double nen = nH * ((1.0 - nHI) + y * (nHeII + 2.0 * x3));
+            if constexpr (MT != 0) {
+                if (ms.metal_ne) {
+                    MCache<MT> cc;
+                    m_prepare<MT>((&c_mt), ms.gam + (size_t)il * ms.ntr, te, cc);
+                    nen = charge_neutrality_m<MT>((&c_mt), cc, nH, nen, nHI, ne);
+                }
+            }
             s.xHI[il] = nHI; s.xHeI[il] = nHeI; s.xHeII[il] = nHeII;
             s.ne[il] = nen; s.Te[il] = te;
             s.opac[il] = opac_of(nH, nHI, nHeI, nHeII, s.rhokap[il], y, d2cm);
@@ -1717,7 +1885,8 @@ __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCo
 }
 
 // hydrogen_balance_residual_grid partials {max eps, sum V eps, sum V}
-__global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, DevState s, bool caseA)
+template <int MT>
+__global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, DevState s, bool caseA, MetalSw ms)
 {
     const int il = blockIdx.x * kBlock + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
@@ -1725,8 +1894,17 @@ __global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, DevState s, bool
     if (il < g.nleaf && s.nH[il] > 0.0 && 1.0 - s.xHI[il] >= 1.0e-10) {
         const RateSet r = rates_at(s.Te[il], caseA);
         const double nHI = s.nH[il] * s.xHI[il], nHII = s.nH[il] * (1.0 - s.xHI[il]);
-        const double loss = nHI * (s.rates[il] + r.cH * s.ne[il]);
-        const double gain = (r.aH * s.ne[il]) * nHII;
+        double rin = 0.0, rout = 0.0;
+        if constexpr (MT != 0) {
+            if (ms.cx_two_way) {
+                MCache<MT> cc;
+                double nem;
+                m_prepare<MT>((&c_mt), ms.gam + (size_t)il * ms.ntr, s.Te[il], cc);
+                m_sums<MT>((&c_mt), cc, s.nH[il], s.ne[il], nHI, nHII, nem, rin, rout);
+            }
+        }
+        const double loss = nHI * (s.rates[il] + r.cH * s.ne[il] + rout);
+        const double gain = (r.aH * s.ne[il] + rin) * nHII;
         const double eps = fabs(loss - gain) / fmax(loss, DBL_MIN);
         const double wd = 2.0 * leaf_half(g, il), vol = (wd * wd) * wd;
         v[0] = eps; v[1] = vol * eps; v[2] = vol;
@@ -1739,6 +1917,56 @@ __global__ __launch_bounds__(kBlock) void k_opacity_fill(DevGrid g, DevState s, 
     const int il = blockIdx.x * kBlock + threadIdx.x;
     if (il >= g.nleaf) return;
     s.opac[il] = opac_of(s.nH[il], s.xHI[il], s.xHeI[il], s.xHeII[il], s.rhokap[il], yHe, d2cm);
+}
+
+// species_gamma_compute (src/species_mod.f90:493-512): path tallies
+// (threshold order) -> metal Gammas and photoheating (element order)
+__global__ __launch_bounds__(kBlock) void k_metal_rates(DevGrid g, const double *mraw, double *mgam,
+                                                        double *mheat, int ntr, double d2cm)
+{
+    const int il = blockIdx.x * kBlock + threadIdx.x;
+    if (il >= g.nleaf) return;
+    const double w = 2.0 * leaf_half(g, il);
+    const double fac = 1.0 / ((w * w) * w * (d2cm * d2cm));
+    for (int k = 0; k < ntr; ++k) {
+        const int t = c_mt.flat_t[k];
+        const double *r = mraw + ((size_t)il * ntr + k) * 2;
+        mgam[(size_t)il * ntr + t] = r[0] * fac;
+        mheat[(size_t)il * ntr + t] = r[1] * fac;
+    }
+}
+
+// species_stage_cache_refresh (src/species_mod.f90:247-261) at each leaf's
+// (T_e, n_e, x_HI), and the walk's opacity weights: abund x the fraction of
+// each transition's lower stage (threshold order), FP64 and, times n_H d2cm,
+// FP32.  Leaves without gas get zero weights.
+template <int MT>
+__global__ __launch_bounds__(kBlock) void k_stage_refresh(DevGrid g, DevState s, MetalSw ms,
+                                                          double *mfrac, double *mop, float *mopf,
+                                                          int nst, double d2cm)
+{
+    const int il = blockIdx.x * kBlock + threadIdx.x;
+    if (il >= g.nleaf) return;
+    const double nH = s.nH[il];
+    MCache<MT> cc;
+    m_prepare<MT>((&c_mt), ms.gam + (size_t)il * ms.ntr, s.Te[il], cc);
+    double *fr = mfrac + (size_t)il * nst;
+    double *wp = mop + (size_t)il * ms.ntr;
+    float *wf = mopf + (size_t)il * ms.ntr;
+    const DevMetalTab *mt = &c_mt;
+    double lo[MT], up[MT];
+    m_fracs<MT>(mt, cc, s.ne[il], nH * s.xHI[il], nH * (1.0 - s.xHI[il]), lo, up);
+    const int ntr = mt->ntr;
+#pragma unroll
+    for (int u = 0; u < MT; ++u) {
+        if (u < ntr) {
+            if (mt->t_first[u] == u) fr[mt->t_st[u]] = lo[u];
+            fr[mt->t_st[u] + 1] = up[u];
+            const double wv = nH > 0.0 ? mt->t_ab[u] * lo[u] : 0.0;
+            wp[mt->t_flat[u]] = wv;
+            wf[mt->t_flat[u]] = (float)(nH * d2cm * wv);
+        }
+    }
 }
 
 // diagnostic: cells of one ray's direct walk (single thread)
@@ -1823,17 +2051,45 @@ KArgs make_args(mc_ctx *c)
     a.dif = c->d_dif;
     a.dcdf = c->d_dcdf;
     a.milne = c->milne;
+    a.ntr = c->ntr;
+    a.metal_abs = c->p.ion_metal_abs != 0;
+    a.mraw = c->d_mraw;
+    a.mop = c->d_mop;
+    a.mopf = c->d_mopf;
     return a;
 }
 
-int persistent_blocks(const void *kern)
+int persistent_blocks(const void *kern, size_t dyn_lds = 0)
 {
     int dev = 0, ncu = 0, per = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, kBlock, 0) != hipSuccess || per < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, kBlock, dyn_lds) != hipSuccess ||
+        per < 1)
         per = 2;
     return ncu * per;
+}
+
+MetalSw make_sw(mc_ctx *c)
+{
+    MetalSw m;
+    m.gam = c->d_mgam;
+    m.heat = c->d_mheat;
+    m.ntr = c->ntr;
+    m.metal_ne = c->p.metal_ne != 0;
+    m.cx_two_way = c->p.charge_exchange == MC_CX_TWO_WAY;
+    m.metal_heat = c->p.metal_heat != 0;
+    m.metal_ff = c->p.metal_freefree != 0;
+    return m;
+}
+
+// the context's registry into c_mt, ordered before the kernels that read it
+int metal_sync(mc_ctx *c)
+{
+    if (c->ntr > 0)
+        MC_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_mt), &c->mtab, sizeof(DevMetalTab), 0,
+                                      hipMemcpyHostToDevice, c->stream));
+    return 0;
 }
 
 int sum_partials(mc_ctx *c, int nblk, double out[3])
@@ -1859,7 +2115,10 @@ int mc_run_pass(mc_ctx *c)
     KArgs a = make_args(c);
     const size_t nl = (size_t)c->nleaf;
     hipStream_t st = c->stream;
+    if (metal_sync(c)) return -100;
     MC_HIP(hipEventRecord(c->ev[0], st));
+    const bool metals = c->ntr > 0;
+    if (metals) k_zero<<<1024, kBlock, 0, st>>>(c->d_mraw, nl * c->ntr * 2);
     k_zero<<<1024, kBlock, 0, st>>>(c->st.hhe, nl * 8);
     k_zero<<<1024, kBlock, 0, st>>>(c->st.jt, nl * c->nnu);
     k_zero_counters<<<1, 64, 0, st>>>(c->st);
@@ -1871,11 +2130,20 @@ int mc_run_pass(mc_ctx *c)
     const int64_t N = c->p.nphotons;
     const bool face = !c->car || c->p.car_walk_amr;
     const bool mixed = c->p.precision == MC_MIXED;
-    const void *kw = face ? (mixed ? (const void *)k_walk<MC_MIXED, kFaceExit>
-                                   : (const void *)k_walk<MC_FP64, kFaceExit>)
-                          : (mixed ? (const void *)k_walk<MC_MIXED, kCarDDA>
-                                   : (const void *)k_walk<MC_FP64, kCarDDA>);
-    const int nblk = persistent_blocks(kw);
+    // walk kernel: precision form x walk x metal path
+    const void *kw;
+    if (metals)
+        kw = face ? (mixed ? (const void *)k_walk<MC_MIXED, kFaceExit, 1>
+                           : (const void *)k_walk<MC_FP64, kFaceExit, 1>)
+                  : (mixed ? (const void *)k_walk<MC_MIXED, kCarDDA, 1>
+                           : (const void *)k_walk<MC_FP64, kCarDDA, 1>);
+    else
+        kw = face ? (mixed ? (const void *)k_walk<MC_MIXED, kFaceExit, 0>
+                           : (const void *)k_walk<MC_FP64, kFaceExit, 0>)
+                  : (mixed ? (const void *)k_walk<MC_MIXED, kCarDDA, 0>
+                           : (const void *)k_walk<MC_FP64, kCarDDA, 0>);
+    const size_t dlds = metals ? (size_t)c->ntr * kBlock * sizeof(double) : 0;
+    const int nblk = persistent_blocks(kw, dlds);
     const int nlds = c->car ? 0 : c->n_lds_nodes;
     // one batch of packet records: launch kernel, then the persistent walk
     // stellar batches walk in direction order (records sorted by source and
@@ -1902,13 +2170,8 @@ int mc_run_pass(mc_ctx *c)
         }
         k_reset_counter<<<1, 1, 0, st>>>(c->st.counter);
         MC_HIP(hipEventRecord(c->ev[2], st));
-        if (face) {
-            if (mixed) k_walk<MC_MIXED, kFaceExit><<<nblk, kBlock, 0, st>>>(a);
-            else k_walk<MC_FP64, kFaceExit><<<nblk, kBlock, 0, st>>>(a);
-        } else {
-            if (mixed) k_walk<MC_MIXED, kCarDDA><<<nblk, kBlock, 0, st>>>(a);
-            else k_walk<MC_FP64, kCarDDA><<<nblk, kBlock, 0, st>>>(a);
-        }
+        void *kargs[] = {&a};
+        MC_HIP(hipLaunchKernel(kw, dim3(nblk), dim3(kBlock), kargs, dlds, st));
         MC_HIP(hipGetLastError());
         MC_HIP(hipEventRecord(c->ev[3], st));
         MC_HIP(hipEventSynchronize(c->ev[3]));
@@ -1976,6 +2239,12 @@ int mc_run_rates(mc_ctx *c)
     MC_HIP(hipEventRecord(c->ev[4], st));
     k_rates<<<(c->nleaf + kBlock - 1) / kBlock, kBlock, 0, st>>>(c->grid, c->st, c->p.distance2cm);
     MC_HIP(hipGetLastError());
+    if (c->ntr > 0) {
+        if (metal_sync(c)) return -100;
+        k_metal_rates<<<(c->nleaf + kBlock - 1) / kBlock, kBlock, 0, st>>>(
+            c->grid, c->d_mraw, c->d_mgam, c->d_mheat, c->ntr, c->p.distance2cm);
+        MC_HIP(hipGetLastError());
+    }
     MC_HIP(hipEventRecord(c->ev[5], st));
     MC_HIP(hipEventSynchronize(c->ev[5]));
     float ms = 0.0f;
@@ -1988,9 +2257,16 @@ int mc_run_equilibrium(mc_ctx *c, double *max_dx, double *dx_vol)
 {
     const int nblk = (c->nleaf + kBlock - 1) / kBlock;
     hipStream_t st = c->stream;
+    if (metal_sync(c)) return -100;
     MC_HIP(hipEventRecord(c->ev[6], st));
-    k_equilibrium<<<nblk, kBlock, 0, st>>>(c->grid, c->st, c->p.te_fixed, c->p.case_A != 0,
-                                            c->p.He_abund, c->p.ion_relax, c->p.distance2cm);
+    if (c->ntr > 0)
+        k_equilibrium<MC_GPU_MAXTR><<<nblk, kBlock, 0, st>>>(
+            c->grid, c->st, c->p.te_fixed, c->p.case_A != 0, c->p.He_abund, c->p.ion_relax,
+            c->p.distance2cm, make_sw(c));
+    else
+        k_equilibrium<0><<<nblk, kBlock, 0, st>>>(c->grid, c->st, c->p.te_fixed, c->p.case_A != 0,
+                                                  c->p.He_abund, c->p.ion_relax, c->p.distance2cm,
+                                                  make_sw(c));
     MC_HIP(hipGetLastError());
     MC_HIP(hipEventRecord(c->ev[7], st));
     double m[3];
@@ -2023,11 +2299,22 @@ int mc_run_thermal(mc_ctx *c, double out[4])
         if (rc) return rc;
     }
     const bool caseA = c->p.case_A != 0;
+    if (metal_sync(c)) return -100;
 #define MC_THERMAL_ARGS c->grid, c->st, c->cool, caseA, c->p.He_abund, c->p.ion_relax, \
-                        c->p.te_min, c->p.te_max, c->p.distance2cm, c->d_pred
-    if (mode == MC_WARM_MLP) k_thermal<MC_WARM_MLP><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
-    else if (mode == MC_WARM_NEWTON) k_thermal<MC_WARM_NEWTON><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
-    else k_thermal<MC_WARM_NONE><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
+                        c->p.te_min, c->p.te_max, c->p.distance2cm, c->d_pred, make_sw(c)
+    if (c->ntr > 0) {
+        if (mode == MC_WARM_MLP)
+            k_thermal<MC_WARM_MLP, MC_GPU_MAXTR><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
+        else if (mode == MC_WARM_NEWTON)
+            k_thermal<MC_WARM_NEWTON, MC_GPU_MAXTR><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
+        else
+            k_thermal<MC_WARM_NONE, MC_GPU_MAXTR><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
+    } else {
+        if (mode == MC_WARM_MLP) k_thermal<MC_WARM_MLP, 0><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
+        else if (mode == MC_WARM_NEWTON)
+            k_thermal<MC_WARM_NEWTON, 0><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
+        else k_thermal<MC_WARM_NONE, 0><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
+    }
 #undef MC_THERMAL_ARGS
     MC_HIP(hipGetLastError());
     MC_HIP(hipEventRecord(c->ev[7], st));
@@ -2058,7 +2345,12 @@ int mc_run_thermal(mc_ctx *c, double out[4])
 int mc_run_residual(mc_ctx *c, double *eps_max, double *eps_vol)
 {
     const int nblk = (c->nleaf + kBlock - 1) / kBlock;
-    k_residual<<<nblk, kBlock, 0, c->stream>>>(c->grid, c->st, c->p.case_A != 0);
+    if (metal_sync(c)) return -100;
+    if (c->ntr > 0)
+        k_residual<MC_GPU_MAXTR><<<nblk, kBlock, 0, c->stream>>>(c->grid, c->st, c->p.case_A != 0,
+                                                                 make_sw(c));
+    else
+        k_residual<0><<<nblk, kBlock, 0, c->stream>>>(c->grid, c->st, c->p.case_A != 0, make_sw(c));
     MC_HIP(hipGetLastError());
     double m[3];
     if (sum_partials(c, nblk, m)) return -100;
@@ -2071,6 +2363,18 @@ int mc_run_opacity_fill(mc_ctx *c)
 {
     k_opacity_fill<<<(c->nleaf + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
         c->grid, c->st, c->p.He_abund, c->p.distance2cm);
+    MC_HIP(hipGetLastError());
+    if (c->ntr > 0) return mc_run_stage_refresh(c);
+    MC_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mc_run_stage_refresh(mc_ctx *c)
+{
+    if (c->ntr <= 0) return 0;
+    if (metal_sync(c)) return -100;
+    k_stage_refresh<MC_GPU_MAXTR><<<(c->nleaf + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
+        c->grid, c->st, make_sw(c), c->d_mfrac, c->d_mop, c->d_mopf, c->nst, c->p.distance2cm);
     MC_HIP(hipGetLastError());
     MC_HIP(hipStreamSynchronize(c->stream));
     return 0;

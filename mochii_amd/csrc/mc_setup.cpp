@@ -44,16 +44,23 @@ void mc_qmc_tables(int seed, uint32_t V[32][MC_QMC_DIM], uint32_t key[MC_QMC_DIM
 }
 
 // ---------------------------------------------------------------------------
-// Band edges.  With ion_align_edges the H/He thresholds inside the band split
-// it into sub-segments and the bins are shared out by log width, floor >= 1,
-// remainder by largest fractional part (reference: build_aligned_edges,
-// src/ion_band_mod.f90:1053-1214; no metal edges on this path).
+// Band edges.  With ion_align_edges the H/He thresholds (priority 1) and the
+// active metal thresholds (priority 0) strictly inside the band split it into
+// sub-segments: near-coincident thresholds (|ln ratio| < 0.01) merge keeping
+// the H/He member; while the thresholds outnumber the bins the most redundant
+// metal one (smallest log gap to a neighbour) is dropped; the bins are shared
+// out by log width, floor >= 1, remainder by largest fractional part
+// (reference: build_aligned_edges, src/ion_band_mod.f90:1053-1214).
 int mc_setup_band(mc_ctx *c)
 {
     const mc_params &p = c->p;
     const int n = p.nnu_ion;
     if (n < 1 || n > MC_MAXBINS) {
         mc_set_error("nnu_ion must be in [1, 64]");
+        return -1;
+    }
+    if (p.nmetal < 0 || p.nmetal > MC_MAX_METAL_TR) {
+        mc_set_error("nmetal must be in [0, 64]");
         return -1;
     }
     c->nnu = n;
@@ -64,13 +71,40 @@ int mc_setup_band(mc_ctx *c)
             c->eedge[i] = std::exp(std::log(lo) + (std::log(hi) - std::log(lo)) * (double)i / n);
         return 0;
     }
-    std::vector<double> cut;
+    std::vector<std::pair<double, int>> cand;
     for (double t : {MC_ETH_HI, MC_ETH_HEI, MC_ETH_HEII})
-        if (t > lo && t < hi) {
-            // thresholds within 1% in log merge (all three have equal priority)
-            if (!cut.empty() && std::fabs(std::log(t / cut.back())) < 0.01) continue;
-            cut.push_back(t);
+        if (t > lo && t < hi) cand.push_back({t, 1});
+    for (int i = 0; i < p.nmetal; ++i)
+        if (p.metal_eth[i] > 0.0 && p.metal_eth[i] > lo && p.metal_eth[i] < hi)
+            cand.push_back({p.metal_eth[i], 0});
+    std::stable_sort(cand.begin(), cand.end(),
+                     [](const std::pair<double, int> &x, const std::pair<double, int> &y) {
+                         return x.first < y.first;
+                     });
+    std::vector<double> cut;
+    std::vector<int> pri;
+    for (const auto &cd : cand) {
+        if (!cut.empty() && std::fabs(std::log(cd.first / cut.back())) < 0.01) {
+            if (cd.second > pri.back()) { cut.back() = cd.first; pri.back() = cd.second; }
+            continue;
         }
+        cut.push_back(cd.first);
+        pri.push_back(cd.second);
+    }
+    while ((int)cut.size() + 1 > n) {
+        int best = -1;
+        double gap = 1.0e30;
+        const int m = (int)cut.size();
+        for (int i = 0; i < m; ++i) {
+            if (pri[i] != 0) continue;
+            double t = i == 0 ? std::log(cut[i] / lo) : std::log(cut[i] / cut[i - 1]);
+            t = std::min(t, i == m - 1 ? std::log(hi / cut[i]) : std::log(cut[i + 1] / cut[i]));
+            if (t < gap) { gap = t; best = i; }
+        }
+        if (best < 0) break;
+        cut.erase(cut.begin() + best);
+        pri.erase(pri.begin() + best);
+    }
     std::vector<double> pts;
     pts.push_back(lo);
     pts.insert(pts.end(), cut.begin(), cut.end());
@@ -141,12 +175,15 @@ static double planck_shape(double E, double T)
 // trapezoid/Simpson difference is within rtol of the local and the average
 // band integral (reference: build_planck_sampler / refine_planck_interval,
 // src/energy_sampler_mod.f90:106-185).  Appends knots to e/f.
-static int planck_knots(double T, double emin, double emax, double rtol,
-                        std::vector<double> &e, std::vector<double> &f)
+static int planck_knots(double T, double emin, double emax, double rtol, int nmet,
+                        const double *meth, std::vector<double> &e, std::vector<double> &f)
 {
     std::vector<double> mand = {emin, emax};
     for (double t : {MC_ETH_HI, MC_ETH_HEI, MC_ETH_HEII})
         if (t > emin && t < emax) mand.push_back(t);
+    // photoionization_threshold_knots (src/ion_band_mod.f90:408-420)
+    for (int i = 0; i < nmet; ++i)
+        if (meth[i] > emin && meth[i] < emax) mand.push_back(meth[i]);
     std::sort(mand.begin(), mand.end());
     mand.erase(std::unique(mand.begin(), mand.end()), mand.end());
     double scale = 0.0;
@@ -210,7 +247,7 @@ int mc_setup_samplers(mc_ctx *c)
         }
         std::vector<double> e, f;
         if (planck_knots(T, p.eion_min, p.eion_max, p.source_cdf_tol > 0 ? p.source_cdf_tol : 1e-8,
-                         e, f))
+                         p.nmetal, p.metal_eth, e, f))
             return -1;
         std::vector<double> cdf(e.size(), 0.0);
         for (size_t i = 0; i + 1 < e.size(); ++i)

@@ -1,0 +1,44 @@
+"""Write mochii_amd/data/atomic_registry.json from a MoCHII data/atomic
+directory (element_<el>.txt and cooling_<el>_<i>.txt in the reference's text
+formats).  The JSON is a data snapshot of the published coefficients those
+files hold (VFKY96 / VY95 photoionisation, Voronov collisional ionisation,
+Badnell / CHIANTI recombination, Huang et al. 2023 and Kingdon & Ferland 1996
+charge exchange, CHIANTI line-cooling fits), with each file's provenance
+header kept beside it.
+
+    python mochii_amd/data/make_atomic_json.py /path/to/MoCHII/data/atomic
+"""
+import json
+import os
+import sys
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", ".."))
+from mochii_amd.atomic import ORDER, SNAPSHOT, read_registry  # noqa: E402
+
+
+def provenance(path):
+    with open(path) as f:
+        return [ln.rstrip("\n") for ln in f if ln.startswith("#")]
+
+
+def main(atomic_dir, out=SNAPSHOT):
+    reg = read_registry(atomic_dir)
+    prov = {}
+    for name in ORDER:
+        if name not in reg:
+            continue
+        prov[f"element_{name}.txt"] = provenance(os.path.join(atomic_dir, f"element_{name}.txt"))
+        for st in reg[name]["cooling"]:
+            fn = f"cooling_{name}_{st}.txt"
+            prov[fn] = provenance(os.path.join(atomic_dir, fn))[:4]
+    doc = {"source": "MoCHII data/atomic (element and cooling-fit files)",
+           "elements": {k: dict(v, cooling={str(s): f for s, f in v["cooling"].items()})
+                        for k, v in reg.items()},
+           "provenance": prov}
+    with open(out, "w") as f:
+        json.dump(doc, f, indent=1)
+    print(f"wrote {out}: {len(reg)} elements")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1])

@@ -12,6 +12,7 @@ program drive_stromgren
   use mochii_gpu
   implicit none
   type(mc_params), target :: p
+  type(mc_element), target :: el
   type(c_ptr) :: ctx
   real(c_double), allocatable :: nH(:), rates(:)
   real(c_double) :: max_dx, dx_vol, led(3)
@@ -24,6 +25,7 @@ program drive_stromgren
   call get_command_argument(1, arg)
   if (trim(arg) == 'layout') then
      call print_layout()
+     call element_layout()
      stop
   end if
   n = 16;  npk = 20000_c_int64_t
@@ -82,6 +84,9 @@ contains
     call off('xHI_init', c_loc(p%xHI_init));      call off('ion_add_dust', c_loc(p%ion_add_dust))
     call off('lambda_ref', c_loc(p%lambda_ref));  call off('diffuse_field', c_loc(p%diffuse_field))
     call off('diffuse_model', c_loc(p%diffuse_model))
+    call off('nmetal', c_loc(p%nmetal));          call off('metal_eth', c_loc(p%metal_eth))
+    call off('ion_metal_abs', c_loc(p%ion_metal_abs))
+    call off('cooling_model', c_loc(p%cooling_model))
   contains
     subroutine off(name, q)
       character(len=*), intent(in) :: name
@@ -91,5 +96,20 @@ contains
       write(*, '(a,1x,i0)') name, a - b
     end subroutine off
   end subroutine print_layout
+
+  ! the element record: size and field offsets, prefixed "element."
+  subroutine element_layout()
+    integer(c_intptr_t) :: eb
+    eb = transfer(c_loc(el), eb)
+    write(*, '(a,i0)') 'element.sizeof ', c_sizeof(el)
+    write(*, '(a,1x,i0)') 'element.abund', transfer(c_loc(el%abund), eb) - eb
+    write(*, '(a,1x,i0)') 'element.photo', transfer(c_loc(el%photo), eb) - eb
+    write(*, '(a,1x,i0)') 'element.ndr', transfer(c_loc(el%ndr), eb) - eb
+    write(*, '(a,1x,i0)') 'element.dr2', transfer(c_loc(el%dr2), eb) - eb
+    write(*, '(a,1x,i0)') 'element.rr2', transfer(c_loc(el%rr2), eb) - eb
+    write(*, '(a,1x,i0)') 'element.cxr', transfer(c_loc(el%cxr), eb) - eb
+    write(*, '(a,1x,i0)') 'element.ncool', transfer(c_loc(el%ncool), eb) - eb
+    write(*, '(a,1x,i0)') 'element.cool_T', transfer(c_loc(el%cool_T), eb) - eb
+  end subroutine element_layout
 
 end program drive_stromgren

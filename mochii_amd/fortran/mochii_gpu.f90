@@ -18,6 +18,11 @@ module mochii_gpu
   integer(c_int), parameter, public :: MC_WARM_NONE = 0, MC_WARM_NEWTON = 1, MC_WARM_MLP = 2
   integer(c_int), parameter, public :: MC_DIFFUSE_THRESHOLD = 0, MC_DIFFUSE_EXPONENTIAL = 1, &
                                        MC_DIFFUSE_MILNE = 2
+  integer, parameter, public :: MC_MAX_EL = 11, MC_MAX_ST = 6, MC_MAX_METAL_TR = 64, &
+                                MC_MAX_COOLFIT = 8
+  integer(c_int), parameter, public :: MC_CX_TWO_WAY = 0, MC_CX_METAL_ONLY = 1
+  integer(c_int), parameter, public :: MC_CI_VORONOV = 0, MC_CI_DERE_HYBRID = 1
+  integer(c_int), parameter, public :: MC_COOL_LOW_DENSITY = 0, MC_COOL_LOCAL_NE = 1
 
   type, bind(C), public :: mc_params
      integer(c_int64_t) :: nphotons
@@ -38,7 +43,36 @@ module mochii_gpu
      integer(c_int32_t) :: ion_add_dust, ion_dust_scatter
      real(c_double)     :: lambda_ref
      integer(c_int32_t) :: diffuse_field, diffuse_model
+     integer(c_int32_t) :: nmetal
+     real(c_double)     :: metal_eth(MC_MAX_METAL_TR)
+     integer(c_int32_t) :: ion_metal_abs, metal_ne, metal_heat, metal_freefree
+     integer(c_int32_t) :: charge_exchange, ci_model, cooling_model
   end type mc_params
+
+  ! one registry element (read_element + cooling_load); C arrays [stage][k]
+  ! are Fortran arrays (k, stage)
+  type, bind(C), public :: mc_element
+     character(kind=c_char) :: name(4)
+     integer(c_int32_t) :: Z, nstage
+     real(c_double)     :: abund
+     real(c_double)     :: eth(MC_MAX_ST)
+     integer(c_int32_t) :: photo_form(MC_MAX_ST)
+     real(c_double)     :: photo(8, MC_MAX_ST)
+     real(c_double)     :: ci(5, MC_MAX_ST)
+     real(c_double)     :: rr(6, MC_MAX_ST)
+     integer(c_int32_t) :: ndr(MC_MAX_ST)
+     real(c_double)     :: dr_c(9, MC_MAX_ST), dr_e(9, MC_MAX_ST)
+     integer(c_int32_t) :: has_dr2(MC_MAX_ST)
+     real(c_double)     :: dr2(4, MC_MAX_ST)
+     integer(c_int32_t) :: has_rr2(MC_MAX_ST)
+     real(c_double)     :: rr2(2, MC_MAX_ST)
+     integer(c_int32_t) :: cxi_form(MC_MAX_ST)
+     real(c_double)     :: cxi(6, MC_MAX_ST)
+     integer(c_int32_t) :: cxr_form(MC_MAX_ST)
+     real(c_double)     :: cxr(6, MC_MAX_ST)
+     integer(c_int32_t) :: ncool(MC_MAX_ST)
+     real(c_double)     :: cool_A(MC_MAX_COOLFIT, MC_MAX_ST), cool_T(MC_MAX_COOLFIT, MC_MAX_ST)
+  end type mc_element
 
   public :: mc_params_default, mc_error_message
 
@@ -203,6 +237,41 @@ module mochii_gpu
        integer(c_int), value :: precision
        integer(c_int) :: rc
      end function mc_set_precision
+
+     function mc_species_setup(ctx, n_elements, elements) bind(C, name='mc_species_setup') &
+          result(rc)
+       import :: c_int, c_ptr, mc_element
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n_elements
+       type(mc_element), intent(in) :: elements(*)
+       integer(c_int) :: rc
+     end function mc_species_setup
+
+     function mc_species_fractions_get(ctx, ie, frac) bind(C, name='mc_species_fractions_get') &
+          result(rc)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ie
+       real(c_double), intent(out) :: frac(*)
+       integer(c_int) :: rc
+     end function mc_species_fractions_get
+
+     function mc_species_rates_get(ctx, ie, gam, heat) bind(C, name='mc_species_rates_get') &
+          result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, gam, heat
+       integer(c_int), value :: ie
+       integer(c_int) :: rc
+     end function mc_species_rates_get
+
+     function mc_species_tallies_get(ctx, ie, raw) bind(C, name='mc_species_tallies_get') &
+          result(rc)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ie
+       real(c_double), intent(out) :: raw(*)
+       integer(c_int) :: rc
+     end function mc_species_tallies_get
   end interface
 
   public :: mc_last_error, mc_init, mc_finalize, mc_grid_octree, mc_gas_state_setup
@@ -211,6 +280,8 @@ module mochii_gpu
   public :: mc_cooling_setup, mc_gas_thermal_update, mc_set_warm_start
   public :: mc_hydrogen_balance_residual_grid, mc_tallies_get, mc_rates_get
   public :: mc_energy_ledger, mc_info, mc_set_precision
+  public :: mc_species_setup, mc_species_fractions_get, mc_species_rates_get
+  public :: mc_species_tallies_get
 
 contains
 
@@ -232,6 +303,10 @@ contains
     p%xHI_init = 1d0;  p%xHeI_init = 1d0;  p%xHeII_init = 0d0
     p%ion_add_dust = 0;  p%ion_dust_scatter = 0;  p%lambda_ref = 0.55d0
     p%diffuse_field = 0;  p%diffuse_model = MC_DIFFUSE_MILNE
+    p%nmetal = 0;  p%metal_eth = 0d0
+    p%ion_metal_abs = 1;  p%metal_ne = 1;  p%metal_heat = 1;  p%metal_freefree = 1
+    p%charge_exchange = MC_CX_TWO_WAY;  p%ci_model = MC_CI_VORONOV
+    p%cooling_model = MC_COOL_LOW_DENSITY
   end function mc_params_default
 
   ! mc_last_error() as a Fortran string (for fatal_error).

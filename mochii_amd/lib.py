@@ -11,6 +11,10 @@ MC_DIFFUSE_THRESHOLD, MC_DIFFUSE_EXPONENTIAL, MC_DIFFUSE_MILNE = 0, 1, 2
 MC_WARM_NONE, MC_WARM_NEWTON, MC_WARM_MLP = 0, 1, 2
 MC_ORDER_LAUNCH, MC_ORDER_DIRECTION = 0, 1
 MC_MLP_NPARAM = 9180
+MC_MAX_METAL_TR = 64
+MC_CX_TWO_WAY, MC_CX_METAL_ONLY = 0, 1
+MC_CI_VORONOV, MC_CI_DERE_HYBRID = 0, 1
+MC_COOL_LOW_DENSITY, MC_COOL_LOCAL_NE = 0, 1
 FIELDS = {"nH": 0, "xHI": 1, "xHeI": 2, "xHeII": 3, "ne": 4, "Te": 5, "rhokap": 6}
 
 
@@ -42,6 +46,10 @@ class Params(C.Structure):
         ("ion_add_dust", C.c_int32), ("ion_dust_scatter", C.c_int32),
         ("lambda_ref", C.c_double),
         ("diffuse_field", C.c_int32), ("diffuse_model", C.c_int32),
+        ("nmetal", C.c_int32), ("metal_eth", C.c_double * MC_MAX_METAL_TR),
+        ("ion_metal_abs", C.c_int32), ("metal_ne", C.c_int32), ("metal_heat", C.c_int32),
+        ("metal_freefree", C.c_int32), ("charge_exchange", C.c_int32), ("ci_model", C.c_int32),
+        ("cooling_model", C.c_int32),
     ]
 
     # reference defaults (src/define.f90) for the fields this path reads
@@ -54,7 +62,9 @@ class Params(C.Structure):
                     te_min=1e3, te_max=5e4, gas_tol=1e-3, gas_tol_te=1e-3, case_A=0,
                     solve_te=0, conv_vol=0, gas_niter=0, xHI_init=1.0, xHeI_init=1.0,
                     xHeII_init=0.0, ion_add_dust=0, ion_dust_scatter=0, lambda_ref=0.55,
-                    diffuse_field=0, diffuse_model=MC_DIFFUSE_MILNE)
+                    diffuse_field=0, diffuse_model=MC_DIFFUSE_MILNE, nmetal=0, ion_metal_abs=1,
+                    metal_ne=1, metal_heat=1, metal_freefree=1, charge_exchange=MC_CX_TWO_WAY,
+                    ci_model=MC_CI_VORONOV, cooling_model=MC_COOL_LOW_DENSITY)
 
     @classmethod
     def make(cls, **kw):
@@ -62,7 +72,12 @@ class Params(C.Structure):
         vals = dict(cls.DEFAULTS)
         vals.update(kw)
         for k, v in vals.items():
-            if k in ("tstar", "src_lum", "src_x", "src_y", "src_z"):
+            if k == "metal_eth":
+                arr = getattr(p, k)
+                for i, x in enumerate(v):
+                    arr[i] = x
+                p.nmetal = len(v)
+            elif k in ("tstar", "src_lum", "src_x", "src_y", "src_z"):
                 arr = getattr(p, k)
                 for i, x in enumerate(v):
                     arr[i] = x
@@ -122,6 +137,10 @@ def library():
         "mc_set_precision": ([vp, C.c_int], C.c_int),
         "mc_timing": ([vp, dp], C.c_int),
         "mc_trace_ray": ([vp, C.c_int, dp, dp, ip, C.c_int, ip], C.c_int),
+        "mc_species_setup": ([vp, C.c_int, vp], C.c_int),
+        "mc_species_fractions_get": ([vp, C.c_int, dp], C.c_int),
+        "mc_species_rates_get": ([vp, C.c_int, dp, dp], C.c_int),
+        "mc_species_tallies_get": ([vp, C.c_int, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -61,6 +61,37 @@ class Model:
         check(self._L.mc_gas_state_setup(self._h, _dp(nH), None if x is None else _dp(x)),
               "mc_gas_state_setup")
 
+    def species_setup(self, elements):
+        """species_setup: load the trace-metal registry (list of element dicts
+        from mochii_amd.atomic.registry, with 'abund' set)."""
+        from .atomic import to_struct
+        self.elements = list(elements)
+        self._els = to_struct(self.elements)
+        check(self._L.mc_species_setup(self._h, len(self.elements), self._els),
+              "mc_species_setup")
+
+    def species_fractions(self, ie):
+        """stage_frac of registry element ie: (nleaf, nstage)."""
+        ns = self.elements[ie]["nstage"]
+        out = np.zeros((self.nleaf, ns))
+        check(self._L.mc_species_fractions_get(self._h, ie, _dp(out)), "mc_species_fractions_get")
+        return out
+
+    def species_rates(self, ie):
+        """(egam, eheat) of registry element ie: (nleaf, nstage-1) each."""
+        nt = self.elements[ie]["nstage"] - 1
+        g = np.zeros((self.nleaf, nt))
+        h = np.zeros((self.nleaf, nt))
+        check(self._L.mc_species_rates_get(self._h, ie, _dp(g), _dp(h)), "mc_species_rates_get")
+        return g, h
+
+    def species_tallies(self, ie):
+        """raw(2, nstage-1) of registry element ie per leaf: (nleaf, nstage-1, 2)."""
+        nt = self.elements[ie]["nstage"] - 1
+        out = np.zeros((self.nleaf, nt, 2))
+        check(self._L.mc_species_tallies_get(self._h, ie, _dp(out)), "mc_species_tallies_get")
+        return out
+
     def ion_dust_table(self, lam, albedo, gcos, cext):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (lam, albedo, gcos, cext)]
         check(self._L.mc_ion_dust_table(self._h, arrs[0].size, *[_dp(a) for a in arrs]),

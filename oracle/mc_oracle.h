@@ -127,7 +127,58 @@ typedef struct {
     /* explicit diffuse ground-recombination field (par%diffuse_field,
      * par%diffuse_energy_model: 0 threshold, 1 exponential, 2 milne) */
     int     diffuse_field, diffuse_model;
+    /* trace metals: ion_setup's metal thresholds (src/main.f90:66-82) and the
+     * par switches (src/define.f90:593-652, 776-801); the registry itself is
+     * loaded by orc_species_setup */
+    int     nmetal;
+    double  metal_eth[64];
+    int     ion_metal_abs, metal_ne, metal_heat, metal_freefree;
+    int     cx_two_way;              /* par%charge_exchange == 'two_way' */
+    int     ci_dere;                 /* par%ci_model == 'dere_hybrid'    */
 } orc_params;
+
+/* ---- trace-metal registry (src/species_mod.f90:54-128) -------------------
+ * One element as read_element fills it, laid out exactly like mc_element of
+ * the C-ABI header so one host-side record serves both. */
+#define ORC_MAX_EL 11
+#define ORC_MAX_ST 6
+#define ORC_MAX_TR 64
+typedef struct {
+    char    name[4];
+    int32_t Z, nstage;
+    double  abund;
+    double  eth[ORC_MAX_ST];
+    int32_t photo_form[ORC_MAX_ST];
+    double  photo[ORC_MAX_ST][8];
+    double  ci[ORC_MAX_ST][5];
+    double  rr[ORC_MAX_ST][6];
+    int32_t ndr[ORC_MAX_ST];
+    double  dr_c[ORC_MAX_ST][9], dr_e[ORC_MAX_ST][9];
+    int32_t has_dr2[ORC_MAX_ST];
+    double  dr2[ORC_MAX_ST][4];
+    int32_t has_rr2[ORC_MAX_ST];
+    double  rr2[ORC_MAX_ST][2];
+    int32_t cxi_form[ORC_MAX_ST];
+    double  cxi[ORC_MAX_ST][6];
+    int32_t cxr_form[ORC_MAX_ST];
+    double  cxr[ORC_MAX_ST][6];
+    int32_t ncool[ORC_MAX_ST];
+    double  cool_A[ORC_MAX_ST][8], cool_T[ORC_MAX_ST][8];
+} orc_element;
+
+/* (leaf, T) rate cache of species_ne_prepare (src/species_mod.f90:114-128,
+ * 517-536), element-order transitions */
+typedef struct {
+    double gam[ORC_MAX_TR], ci[ORC_MAX_TR], rec[ORC_MAX_TR], cxi[ORC_MAX_TR], cxr[ORC_MAX_TR];
+} orc_mcache;
+
+double orc_sigma_vy95(double E, double Eth, double E0, double s0, double ya, double P,
+                      double yw, double l);
+double orc_ci_dere_ratio(int ci_dere, int Z, int stage);
+double orc_voronov_ci(const double c[5], double T);
+double orc_alpha_rec(const orc_element *el, int it, double T);
+double orc_cx_rate(int form, const double c[6], double T);
+double orc_species_sigma(const orc_element *el, int it, double E);
 
 /* ---- Milne ground-continuum tables (src/milne_recomb_spectrum_mod.f90) -- */
 #define ORC_MILNE_NT 192
@@ -179,6 +230,18 @@ typedef struct {
     orc_milne *milne;
     double *dif_ch, *dif_cdf, diffuse_lum;
     int64_t diffuse_nphot;
+    /* trace metals (species_mod module state): registry, element-order
+     * transition/stage offsets, the threshold-sorted flat registry
+     * (flat_ie/flat_it/flat_eth, src/species_mod.f90:109-112, 196-227),
+     * path tallies mraw[(il*ntr + k)*2 + {0,1}] in flat order k, rates
+     * mgam/mheat[il*ntr + t] and stage fractions mfrac[il*nst + s] in
+     * element order */
+    int     use_metals, nel, ntr, nst;
+    orc_element el[ORC_MAX_EL];
+    int     tr_off[ORC_MAX_EL], st_off[ORC_MAX_EL];
+    int     flat_ie[ORC_MAX_TR], flat_it[ORC_MAX_TR];
+    double  flat_eth[ORC_MAX_TR];
+    double *mraw, *mgam, *mheat, *mfrac;
 } orc_ctx;
 
 orc_ctx *orc_create(const orc_params *p);
@@ -208,6 +271,33 @@ double   orc_net_rate(const orc_ctx *c, int il, double T, double *xHI, double *x
                       double *xHeII, double *ne);
 void     orc_thermal_update(orc_ctx *c, double *max_dx, double *max_dte, double *dx_vol,
                             double *dte_vol);
+
+/* trace metals (src/species_mod.f90) */
+int      orc_species_setup(orc_ctx *c, int nel, const orc_element *els);
+void     orc_species_packet_cross_sections(const orc_ctx *c, double E, double *sigma, int *ntr);
+void     orc_species_fractions(const orc_ctx *c, int ie, int il, double T, double ne,
+                               double nHI, double nHII, double *frac);
+void     orc_species_ne_prepare(const orc_ctx *c, int il, double T, orc_mcache *cc);
+void     orc_species_metal_sums(const orc_ctx *c, const orc_mcache *cc, double nH, double ne,
+                                double nHI, double nHII, double *nem, double *r_cx_in,
+                                double *r_cx_out);
+double   orc_metal_cooling(const orc_ctx *c, int il, double T, double nH, double ne,
+                           double nHI, double nHII);
+double   orc_metal_cooling_H(const orc_ctx *c, int il, double T, double nH, double ne,
+                             double nHI, double nHII);
+double   orc_metal_freefree(const orc_ctx *c, int il, double T, double nH, double ne,
+                            double nHI, double nHII);
+double   orc_metal_heating(const orc_ctx *c, int il, double T, double nH, double ne,
+                           double nHI, double nHII);
+void     orc_species_gamma_compute(orc_ctx *c);
+void     orc_species_stage_cache_refresh(orc_ctx *c);
+void     orc_gas_opacity_fill(orc_ctx *c);
+double   orc_charge_neutrality_ne(const orc_ctx *c, int il, double T, double nH,
+                                  double ne_HHe, double xHI, double ne_seed);
+/* solve_ion_cell with the leaf's metal terms (il >= 0 and metals loaded) */
+void     orc_solve_ion_cell_leaf(const orc_ctx *c, int il, double gH, double gHe1, double gHe2,
+                                 double nH, double T, int caseA, double yHe, double *xHI,
+                                 double *xHeI, double *xHeII, double *ne);
 
 /* diagnostics: leaves crossed by the direct walk of one ray (no tally) */
 int      orc_trace_ray(orc_ctx *c, double x, double y, double z, double kx,

@@ -43,20 +43,28 @@ void orc_destroy(orc_ctx *c)
     free(c->hhe); free(c->jt); free(c->dust_abs);
     free(c->gHI); free(c->gHeI); free(c->gHeII); free(c->hHI); free(c->hHeI); free(c->hHeII);
     free(c->milne); free(c->dif_ch); free(c->dif_cdf);
+    free(c->mraw); free(c->mgam); free(c->mheat); free(c->mfrac);
     free(c);
 }
 
-/* build_aligned_edges (src/ion_band_mod.f90:1053-1214) with the H/He
- * thresholds only (no metal edges on this path) */
-static int aligned_edges(double *eedge, int nnu, double lo, double hi)
+/* build_aligned_edges (src/ion_band_mod.f90:1053-1214): H/He thresholds
+ * (priority 1) and the active metal thresholds (priority 0) strictly inside
+ * (lo, hi), merged within |ln ratio| < 0.01, metal ones dropped (most
+ * redundant first) while they outnumber the bins */
+static int aligned_edges(double *eedge, int nnu, double lo, double hi, int nmet,
+                         const double *meth)
 {
     const double MERGE_TOL = 0.01;
-    double cand[8], thr[8], points[10], w[9];
-    int cpri[8], tpri[8], nb[9];
+    double cand[80], thr[80], points[82], w[81];
+    int cpri[80], tpri[80], nb[81];
     int ncand = 0;
     const double th[3] = {ORC_ETH_HI, ORC_ETH_HEI, ORC_ETH_HEII};
     for (int i = 0; i < 3; ++i)
         if (th[i] > lo && th[i] < hi) { cand[ncand] = th[i]; cpri[ncand] = 1; ++ncand; }
+    for (int i = 0; i < nmet && i < 64; ++i)
+        if (meth[i] > 0.0 && meth[i] > lo && meth[i] < hi) {
+            cand[ncand] = meth[i]; cpri[ncand] = 0; ++ncand;
+        }
     for (int i = 1; i < ncand; ++i) {       /* insertion sort */
         double t = cand[i]; int tp = cpri[i], j = i - 1;
         while (j >= 0 && cand[j] > t) { cand[j + 1] = cand[j]; cpri[j + 1] = cpri[j]; --j; }
@@ -70,6 +78,19 @@ static int aligned_edges(double *eedge, int nnu, double lo, double hi)
         } else {
             thr[nthr] = cand[i]; tpri[nthr] = cpri[i]; ++nthr;
         }
+    }
+    while (nthr + 1 > nnu) {                /* 2c: drop the most redundant metal edge */
+        int best = -1; double frac = 1.0e30;
+        for (int i = 0; i < nthr; ++i) {
+            if (tpri[i] != 0) continue;
+            double t = i == 0 ? log(thr[i] / lo) : log(thr[i] / thr[i - 1]);
+            if (i == nthr - 1) t = fmin(t, log(hi / thr[i]));
+            else t = fmin(t, log(thr[i + 1] / thr[i]));
+            if (t < frac) { frac = t; best = i; }
+        }
+        if (best < 0) break;
+        for (int i = best; i < nthr - 1; ++i) { thr[i] = thr[i + 1]; tpri[i] = tpri[i + 1]; }
+        --nthr;
     }
     points[0] = lo;
     for (int i = 0; i < nthr; ++i) points[i + 1] = thr[i];
@@ -123,7 +144,8 @@ int orc_setup_band(orc_ctx *c)
     c->nnu_band = nnu;
     c->eedge = (double *)malloc(sizeof(double) * (nnu + 1));
     if (p->align_edges) {
-        if (aligned_edges(c->eedge, nnu, p->eion_min, p->eion_max)) return -1;
+        if (aligned_edges(c->eedge, nnu, p->eion_min, p->eion_max, p->nmetal, p->metal_eth))
+            return -1;
     } else {
         double lo = log(p->eion_min), hi = log(p->eion_max);
         for (int i = 0; i <= nnu; ++i)
@@ -132,13 +154,16 @@ int orc_setup_band(orc_ctx *c)
     c->ncomp = p->nsource + (p->ext_on ? 1 : 0);
     c->multi_src = c->ncomp >= 2;
     double band_L[ORC_MAXSRC + 1];
-    const double knots[3] = {ORC_ETH_HI, ORC_ETH_HEI, ORC_ETH_HEII};
+    /* photoionization_threshold_knots (src/ion_band_mod.f90:408-420) */
+    double knots[3 + 64] = {ORC_ETH_HI, ORC_ETH_HEI, ORC_ETH_HEII};
+    int nknots = 3;
+    for (int i = 0; i < p->nmetal && i < 64; ++i) knots[nknots++] = p->metal_eth[i];
     for (int ic = 0; ic < c->ncomp; ++ic) {
         int ik = ic < p->nsource ? ic + 1 : 0;          /* points first, then external */
         c->comp_kind[ic] = ik;
         double T = ik >= 1 ? p->tstar[ik - 1] : p->ext_tstar;
         int rc = orc_build_planck(&c->samp[ic], T, p->eion_min, p->eion_max,
-                                  p->source_cdf_tol, knots, 3);
+                                  p->source_cdf_tol, knots, nknots);
         if (rc) return rc;
         double f_ion = 1.0 - orc_energy_cdf_value(&c->samp[ic], p->eion_min);
         if (ik >= 1) {

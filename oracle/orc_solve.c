@@ -1,10 +1,9 @@
 /*
  * Oracle: per-cell H/He ionization balance at fixed T (TEST INFRASTRUCTURE).
  *
- * Restates src/ion_balance_mod.f90 without the metal registry
- * (use_metals off, so the charge-exchange and metal-electron terms are
- * identically zero):
- *   solve_ion_cell                 :53-150
+ * Restates src/ion_balance_mod.f90; the metal-electron and charge-exchange
+ * branches of solve_ion_cell and charge_neutrality_ne are in orc_metals.c:
+ *   solve_ion_cell (no registry)   :53-150
  *   hydrogen_balance_residual(_grid):280-343
  *   gas_equilibrium_update          :352-434
  *
@@ -64,8 +63,8 @@ void orc_equilibrium_update(orc_ctx *c, double *max_dx, double *dx_vol)
         double nH = c->nH[i];
         if (nH <= 0.0) { c->ne[i] = 0.0; continue; }
         double xHI = c->xHI[i], xHeI = c->xHeI[i], xHeII = c->xHeII[i], ne;
-        orc_solve_ion_cell(c->gHI[i], c->gHeI[i], c->gHeII[i], nH, T, c->p.caseA, yHe, &xHI,
-                           &xHeI, &xHeII, &ne);
+        orc_solve_ion_cell_leaf(c, i, c->gHI[i], c->gHeI[i], c->gHeII[i], nH, T, c->p.caseA, yHe,
+                                &xHI, &xHeI, &xHeII, &ne);
         double nHI = (1.0 - w) * c->xHI[i] + w * xHI;
         double nHeI = (1.0 - w) * c->xHeI[i] + w * xHeI;
         double nHeII = (1.0 - w) * c->xHeII[i] + w * xHeII;
@@ -80,6 +79,8 @@ void orc_equilibrium_update(orc_ctx *c, double *max_dx, double *dx_vol)
         c->xHeI[i] = nHeI;
         c->xHeII[i] = nHeII;
         c->ne[i] = nH * ((1.0 - nHI) + yHe * (nHeII + 2.0 * xHeIII));
+        if (c->p.metal_ne && c->use_metals && c->nel > 0)
+            c->ne[i] = orc_charge_neutrality_ne(c, i, T, nH, c->ne[i], nHI, ne);
     }
     *max_dx = mdx;
     *dx_vol = sdx / (sx > DBL_MIN ? sx : DBL_MIN);
@@ -93,11 +94,18 @@ void orc_residual_grid(orc_ctx *c, double *eps_max, double *eps_vol)
         if (c->nH[i] <= 0.0) continue;
         if (1.0 - c->xHI[i] < 1.0e-10) continue;
         double T = c->Te[i];
+        double rin = 0.0, rout = 0.0, nem;
+        if (c->use_metals && c->nel > 0 && c->p.cx_two_way) {
+            orc_mcache cc;
+            orc_species_ne_prepare(c, i, T, &cc);
+            orc_species_metal_sums(c, &cc, c->nH[i], c->ne[i], c->nH[i] * c->xHI[i],
+                                   c->nH[i] * (1.0 - c->xHI[i]), &nem, &rin, &rout);
+        }
         double aH = c->p.caseA ? orc_alphaA_HII(T) : orc_alphaB_HII(T);
         double cH = orc_ci_HI(T);
         double nHI = c->nH[i] * c->xHI[i], nHII = c->nH[i] * (1.0 - c->xHI[i]);
-        double loss = nHI * (c->gHI[i] + cH * c->ne[i]);
-        double gain = (aH * c->ne[i]) * nHII;
+        double loss = nHI * (c->gHI[i] + cH * c->ne[i] + rout);
+        double gain = (aH * c->ne[i] + rin) * nHII;
         double eps = fabs(loss - gain) / (loss > DBL_MIN ? loss : DBL_MIN);
         if (eps > emax) emax = eps;
         double v = leaf_vol(c, i);
@@ -117,15 +125,25 @@ void orc_cooling_setup(orc_ctx *c, int nfit, const double *A, const double *Ti)
     orc_gff_table(c->cool.logT, c->cool.gff);
 }
 
-/* net_rate (src/thermal_mod.f90:38-115), no metals, no grain heating */
+/* net_rate (src/thermal_mod.f90:38-115) with the metal terms, no grain
+ * heating */
 double orc_net_rate(const orc_ctx *c, int il, double T, double *xHI, double *xHeI,
                     double *xHeII, double *ne)
 {
     const double nH = c->nH[il], y = c->p.He_abund;
-    orc_solve_ion_cell(c->gHI[il], c->gHeI[il], c->gHeII[il], nH, T, c->p.caseA, y, xHI, xHeI,
-                       xHeII, ne);
+    orc_solve_ion_cell_leaf(c, il, c->gHI[il], c->gHeI[il], c->gHeII[il], nH, T, c->p.caseA, y,
+                            xHI, xHeI, xHeII, ne);
     double heat = nH * (*xHI * c->hHI[il] + y * (*xHeI * c->hHeI[il] + *xHeII * c->hHeII[il]));
-    return heat - orc_cooling_total(&c->cool, T, nH, *ne, *xHI, *xHeI, *xHeII, y, c->p.caseA);
+    double net = heat - orc_cooling_total(&c->cool, T, nH, *ne, *xHI, *xHeI, *xHeII, y,
+                                          c->p.caseA);
+    if (c->use_metals) {
+        double nHI = nH * *xHI, nHII = nH * (1.0 - *xHI);
+        net = net - orc_metal_cooling(c, il, T, nH, *ne, nHI, nHII);
+        net = net - orc_metal_cooling_H(c, il, T, nH, *ne, nHI, nHII);
+        if (c->p.metal_freefree) net = net - orc_metal_freefree(c, il, T, nH, *ne, nHI, nHII);
+        if (c->p.metal_heat) net = net + orc_metal_heating(c, il, T, nH, *ne, nHI, nHII);
+    }
+    return net;
 }
 
 /* gas_thermal_update (src/thermal_mod.f90:124-249) on one process */
@@ -161,6 +179,8 @@ void orc_thermal_update(orc_ctx *c, double *max_dx, double *max_dte, double *dx_
         double xHeIII = 1.0 - nHeI - nHeII;
         if (xHeIII < 0.0) xHeIII = 0.0;
         double ne_new = nH * ((1.0 - nHI) + y * (nHeII + 2.0 * xHeIII));
+        if (c->p.metal_ne && c->use_metals && c->nel > 0)
+            ne_new = orc_charge_neutrality_ne(c, il, te, nH, ne_new, nHI, ne);
         double d = fabs(nHI - c->xHI[il]);
         if (d > mdx) mdx = d;
         double dt = fabs(te - c->Te[il]) / c->Te[il];

@@ -5,6 +5,8 @@
  *   src/ionizing_field_mod.f90:72-170      pass driver (stellar packets)
  *   src/ion_band_mod.f90:1880-1935,2232-2385 QMC launch and emitters
  *   src/ion_packet_mod.f90:21-87            packet cross sections, opacity
+ *   src/species_mod.f90:263-343             metal cross sections, cached
+ *                                           opacity and path tallies
  *   src/raytrace_amr.f90:249-676            transport_ion_packet, edge and
  *                                           tau walks (octree and car DDA)
  *   src/ion_score_mod.f90:137-252           path scoring and energy ledger
@@ -42,10 +44,12 @@ typedef struct {
     int istream;
     uint32_t nk;                       /* draws consumed from the generator */
     double sHI, sHeI, sHeII, dabs, dsca, dg;
+    int nmet;                          /* metal transitions below E (flat order) */
+    double smet[ORC_MAX_TR];
 } pkt_t;
 
 typedef struct {
-    double *hhe, *jt, *dust;
+    double *hhe, *jt, *dust, *mraw;
     int64_t seg_direct, seg_scattered;
     long double e_emit, e_abs, e_esc;
 } tally_t;
@@ -175,6 +179,8 @@ static void build_physics(const orc_ctx *c, pkt_t *ph)
     ph->sHI = orc_sigma_HI(ph->E);
     ph->sHeI = orc_sigma_HeI(ph->E);
     ph->sHeII = orc_sigma_HeII(ph->E);
+    ph->nmet = 0;
+    if (c->use_metals) orc_species_packet_cross_sections(c, ph->E, ph->smet, &ph->nmet);
     ph->dabs = ph->dsca = ph->dg = 0.0;
     if (c->p.add_dust) orc_dust_at(c, ph->E, &ph->dabs, &ph->dsca, &ph->dg);
 }
@@ -182,8 +188,18 @@ static void build_physics(const orc_ctx *c, pkt_t *ph)
 static double opacity(const orc_ctx *c, const pkt_t *ph, int il)
 {
     int i = il - 1;
+    /* species_cached_opacity (src/species_mod.f90:291-301) */
+    double metal = 0.0;
+    if (c->use_metals && c->p.ion_metal_abs)
+        for (int k = 0; k < ph->nmet; ++k) {
+            int ie = c->flat_ie[k];
+            metal = metal + c->el[ie].abund *
+                                c->mfrac[(size_t)i * c->nst + c->st_off[ie] + c->flat_it[k]] *
+                                ph->smet[k];
+        }
     double kap = c->nH[i] * (c->xHI[i] * ph->sHI +
-                             c->p.He_abund * (c->xHeI[i] * ph->sHeI + c->xHeII[i] * ph->sHeII)) *
+                             c->p.He_abund * (c->xHeI[i] * ph->sHeI + c->xHeII[i] * ph->sHeII) +
+                             metal) *
                  c->p.distance2cm;
     if (c->p.add_dust) {
         kap = kap + c->rhokap[i] * ph->dabs;
@@ -210,6 +226,15 @@ static void score(const orc_ctx *c, tally_t *t, const pkt_t *ph, int il, double 
     f = 1.0 - ORC_ETH_HEII / E; coeff[5] = ph->sHeII * (f > 0.0 ? f : 0.0);
     double *h = t->hhe + (size_t)i * 6;
     for (int k = 0; k < 6; ++k) h[k] = h[k] + pl * coeff[k];
+    /* species_rate_path_score (src/species_mod.f90:329-343) */
+    if (c->use_metals) {
+        double *r = t->mraw + (size_t)i * c->ntr * 2;
+        for (int k = 0; k < ph->nmet; ++k) {
+            double f = 1.0 - c->flat_eth[k] / E;
+            r[2 * k] = r[2 * k] + pl * ph->smet[k] / (E * ORC_EV2ERG);
+            r[2 * k + 1] = r[2 * k + 1] + pl * ph->smet[k] * (f > 0.0 ? f : 0.0);
+        }
+    }
 }
 
 /* ion_score_energy_packet (src/ion_score_mod.f90:233-252) */
@@ -692,13 +717,17 @@ int orc_transport_pass(orc_ctx *c, int64_t first, int64_t last, int nthreads)
         w[i].dlast = 1 + nd * (i + 1) / nthreads;
         if (i == 0) {
             w[i].t.hhe = c->hhe; w[i].t.jt = c->jt; w[i].t.dust = c->dust_abs;
+            w[i].t.mraw = c->mraw;
             memset(c->hhe, 0, sizeof(double) * 6 * (size_t)nl);
             memset(c->jt, 0, sizeof(double) * (size_t)nnu * nl);
             memset(c->dust_abs, 0, sizeof(double) * nl);
+            if (c->use_metals) memset(c->mraw, 0, sizeof(double) * (size_t)nl * c->ntr * 2);
         } else {
             w[i].t.hhe = (double *)calloc((size_t)nl * 6, sizeof(double));
             w[i].t.jt = (double *)calloc((size_t)nl * nnu, sizeof(double));
             w[i].t.dust = (double *)calloc(nl, sizeof(double));
+            w[i].t.mraw = c->use_metals ? (double *)calloc((size_t)nl * c->ntr * 2, sizeof(double))
+                                        : NULL;
         }
     }
     for (int i = 1; i < nthreads; ++i) pthread_create(&th[i], NULL, worker, &w[i]);
@@ -710,9 +739,11 @@ int orc_transport_pass(orc_ctx *c, int64_t first, int64_t last, int nthreads)
         for (size_t k = 0; k < (size_t)nl * 6; ++k) c->hhe[k] += w[i].t.hhe[k];
         for (size_t k = 0; k < (size_t)nl * nnu; ++k) c->jt[k] += w[i].t.jt[k];
         for (int k = 0; k < nl; ++k) c->dust_abs[k] += w[i].t.dust[k];
+        if (c->use_metals)
+            for (size_t k = 0; k < (size_t)nl * c->ntr * 2; ++k) c->mraw[k] += w[i].t.mraw[k];
         ee += w[i].t.e_emit; ea += w[i].t.e_abs; es += w[i].t.e_esc;
         sd += w[i].t.seg_direct; ss += w[i].t.seg_scattered;
-        free(w[i].t.hhe); free(w[i].t.jt); free(w[i].t.dust);
+        free(w[i].t.hhe); free(w[i].t.jt); free(w[i].t.dust); free(w[i].t.mraw);
     }
     c->e_emit = (double)ee; c->e_abs = (double)ea; c->e_esc = (double)es;
     c->seg_direct = sd; c->seg_scattered = ss;
@@ -732,6 +763,7 @@ void orc_rates_compute(orc_ctx *c)
         c->gHI[il - 1] = h[0] * fac; c->gHeI[il - 1] = h[1] * fac; c->gHeII[il - 1] = h[2] * fac;
         c->hHI[il - 1] = h[3] * fac; c->hHeI[il - 1] = h[4] * fac; c->hHeII[il - 1] = h[5] * fac;
     }
+    orc_species_gamma_compute(c);
 }
 
 /* cells of one ray's direct walk to the box edge (geometry only) */

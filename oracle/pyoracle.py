@@ -40,7 +40,14 @@ class OrcParams(C.Structure):
         ("add_dust", C.c_int), ("dust_scatter", C.c_int),
         ("lambda_ref", C.c_double),
         ("diffuse_field", C.c_int), ("diffuse_model", C.c_int),
+        ("nmetal", C.c_int), ("metal_eth", C.c_double * 64),
+        ("ion_metal_abs", C.c_int), ("metal_ne", C.c_int), ("metal_heat", C.c_int),
+        ("metal_freefree", C.c_int), ("cx_two_way", C.c_int), ("ci_dere", C.c_int),
     ]
+
+    # metal switches at the reference defaults (src/define.f90:593-652, 801)
+    METAL_DEFAULTS = dict(ion_metal_abs=1, metal_ne=1, metal_heat=1, metal_freefree=1,
+                          cx_two_way=1, ci_dere=0)
 
 
 class Cooling(C.Structure):
@@ -147,6 +154,36 @@ def _load():
     lib.orc_diffuse_build.argtypes = [C.c_void_p]
     lib.orc_launch_diffuse.argtypes = [C.c_void_p, C.c_int64, _dp]
     lib.orc_transport_pass.restype = C.c_int
+    vp = C.c_void_p
+    lib.orc_species_setup.argtypes = [vp, C.c_int, vp]
+    lib.orc_species_setup.restype = C.c_int
+    lib.orc_gas_opacity_fill.argtypes = [vp]
+    lib.orc_species_ne_prepare.argtypes = [vp, C.c_int, C.c_double, vp]
+    lib.orc_species_metal_sums.argtypes = [vp, vp] + [C.c_double] * 4 + [_dp, _dp, _dp]
+    lib.orc_solve_ion_cell_leaf.argtypes = ([vp, C.c_int] + [C.c_double] * 5 +
+                                            [C.c_int, C.c_double, _dp, _dp, _dp, _dp])
+    for name in ("orc_metal_cooling", "orc_metal_cooling_H", "orc_metal_freefree",
+                 "orc_metal_heating"):
+        getattr(lib, name).argtypes = [vp, C.c_int] + [C.c_double] * 5
+        getattr(lib, name).restype = C.c_double
+    lib.orc_species_fractions.argtypes = [vp, C.c_int, C.c_int] + [C.c_double] * 4 + [_dp]
+    lib.orc_charge_neutrality_ne.argtypes = [vp, C.c_int] + [C.c_double] * 5
+    lib.orc_charge_neutrality_ne.restype = C.c_double
+    lib.orc_element_size.restype = C.c_int
+    lib.orc_sigma_vy95.argtypes = [C.c_double] * 8
+    lib.orc_sigma_vy95.restype = C.c_double
+    lib.orc_sigma_vfky96.argtypes = [C.c_double] * 9
+    lib.orc_sigma_vfky96.restype = C.c_double
+    lib.orc_ci_dere_ratio.argtypes = [C.c_int, C.c_int, C.c_int]
+    lib.orc_ci_dere_ratio.restype = C.c_double
+    lib.orc_voronov_ci.argtypes = [_dp, C.c_double]
+    lib.orc_voronov_ci.restype = C.c_double
+    lib.orc_alpha_rec.argtypes = [vp, C.c_int, C.c_double]
+    lib.orc_alpha_rec.restype = C.c_double
+    lib.orc_cx_rate.argtypes = [C.c_int, _dp, C.c_double]
+    lib.orc_cx_rate.restype = C.c_double
+    lib.orc_species_sigma.argtypes = [vp, C.c_int, C.c_double]
+    lib.orc_species_sigma.restype = C.c_double
     lib.orc_build_planck.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double,
                                      C.c_double, _dp, C.c_int]
     lib.orc_build_tabulated.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
@@ -247,10 +284,16 @@ class Sampler:
 class Model:
     """One oracle context: params, band, grid, leaf state, tallies."""
 
-    def __init__(self, params: dict, octree=None, nH=None, dust_table=None):
+    def __init__(self, params: dict, octree=None, nH=None, dust_table=None, elements=None):
         p = OrcParams()
+        params = dict(OrcParams.METAL_DEFAULTS, **params)
+        if elements:
+            from mochii_amd.atomic import metal_thresholds
+            params.setdefault("metal_eth", metal_thresholds(elements))
+        if "metal_eth" in params:
+            params["nmetal"] = len(params["metal_eth"])
         for k, v in params.items():
-            if k in ("tstar", "lum", "sx", "sy", "sz"):
+            if k in ("tstar", "lum", "sx", "sy", "sz", "metal_eth"):
                 arr = getattr(p, k)
                 for i, x in enumerate(v):
                     arr[i] = x
@@ -275,8 +318,104 @@ class Model:
         if dust_table is not None:
             lam, alb, g, cext = [np.ascontiguousarray(a, dtype=np.float64) for a in dust_table]
             L.orc_set_dust_table(self.c, lam.size, ptr(lam), ptr(alb), ptr(g), ptr(cext))
+        self.elements = list(elements or [])
+        if self.elements:
+            self.species_setup(self.elements)
         if nH is not None:
             self.state_init(nH)
+
+    # ---- trace metals -------------------------------------------------------
+    def species_setup(self, elements):
+        from mochii_amd.atomic import to_struct
+        self.elements = list(elements)
+        self._els = to_struct(self.elements)
+        rc = lib().orc_species_setup(self.c, len(self.elements), self._els)
+        if rc:
+            raise RuntimeError(f"orc_species_setup rc={rc}")
+
+    def opacity_fill(self):
+        lib().orc_gas_opacity_fill(self.c)
+
+    def _flat(self):
+        h = self._head()
+        return h.ntr, h.nst, list(h.tr_off)[:h.nel], list(h.st_off)[:h.nel]
+
+    def species_fractions(self, ie):
+        """stage_frac of element ie: (nleaf, nstage)."""
+        ntr, nst, toff, soff = self._flat()
+        ns = self.elements[ie]["nstage"]
+        a = np.ctypeslib.as_array(self._head().mfrac, (self.nleaf * nst,)).reshape(self.nleaf, nst)
+        return a[:, soff[ie]:soff[ie] + ns].copy()
+
+    def species_rates(self, ie):
+        """(egam, eheat) of element ie: (nleaf, nstage-1) each."""
+        ntr, nst, toff, soff = self._flat()
+        nt = self.elements[ie]["nstage"] - 1
+        h = self._head()
+        g = np.ctypeslib.as_array(h.mgam, (self.nleaf * ntr,)).reshape(self.nleaf, ntr)
+        q = np.ctypeslib.as_array(h.mheat, (self.nleaf * ntr,)).reshape(self.nleaf, ntr)
+        return g[:, toff[ie]:toff[ie] + nt].copy(), q[:, toff[ie]:toff[ie] + nt].copy()
+
+    def species_tallies(self, ie):
+        """raw(2, nstage-1) of element ie per leaf: (nleaf, nstage-1, 2)."""
+        h = self._head()
+        ntr = h.ntr
+        raw = np.ctypeslib.as_array(h.mraw, (self.nleaf * ntr * 2,)).reshape(self.nleaf, ntr, 2)
+        nt = self.elements[ie]["nstage"] - 1
+        out = np.zeros((self.nleaf, nt, 2))
+        for k in range(ntr):
+            if h.flat_ie[k] == ie:
+                out[:, h.flat_it[k], :] = raw[:, k, :]
+        return out
+
+    def set_species_rates(self, ie, gam, heat=None):
+        ntr, nst, toff, soff = self._flat()
+        h = self._head()
+        nt = self.elements[ie]["nstage"] - 1
+        g = np.ctypeslib.as_array(h.mgam, (self.nleaf * ntr,)).reshape(self.nleaf, ntr)
+        g[:, toff[ie]:toff[ie] + nt] = gam
+        if heat is not None:
+            q = np.ctypeslib.as_array(h.mheat, (self.nleaf * ntr,)).reshape(self.nleaf, ntr)
+            q[:, toff[ie]:toff[ie] + nt] = heat
+
+    def metal_sums(self, il, T, ne, xHI):
+        """species_ne_prepare + species_metal_sums: (nem, r_cx_in, r_cx_out)."""
+        cc = C.create_string_buffer(8 * 5 * 64)
+        L = lib()
+        L.orc_species_ne_prepare(self.c, il, T, cc)
+        nH = self.arr("nH")[il]
+        a, b, d = C.c_double(), C.c_double(), C.c_double()
+        L.orc_species_metal_sums(self.c, cc, nH, ne, nH * xHI, nH * (1.0 - xHI), C.byref(a),
+                                 C.byref(b), C.byref(d))
+        return a.value, b.value, d.value
+
+    def solve_cell(self, il, T, x=None, gam=None):
+        """solve_ion_cell for leaf il with its metal terms:
+        (xHI, xHeI, xHeII, ne)."""
+        st = self.state()
+        x0 = x if x is not None else (st["xHI"][il], st["xHeI"][il], st["xHeII"][il])
+        r = self.rates()
+        g = gam if gam is not None else (r["gHI"][il], r["gHeI"][il], r["gHeII"][il])
+        a, b, c, ne = C.c_double(x0[0]), C.c_double(x0[1]), C.c_double(x0[2]), C.c_double()
+        lib().orc_solve_ion_cell_leaf(self.c, il, g[0], g[1], g[2], st["nH"][il], T,
+                                      int(self.p.caseA), self.p.He_abund, C.byref(a),
+                                      C.byref(b), C.byref(c), C.byref(ne))
+        return a.value, b.value, c.value, ne.value
+
+    def metal_terms(self, il, T, ne, xHI):
+        """(metal_cooling, metal_cooling_H, metal_freefree, metal_heating) of
+        leaf il at T, n_e, x_HI."""
+        L = lib()
+        nH = self.arr("nH")[il]
+        args = (self.c, il, T, nH, ne, nH * xHI, nH * (1.0 - xHI))
+        return (L.orc_metal_cooling(*args), L.orc_metal_cooling_H(*args),
+                L.orc_metal_freefree(*args), L.orc_metal_heating(*args))
+
+    def species_fractions_at(self, ie, il, T, ne, xHI):
+        out = np.zeros(6)
+        nH = self.arr("nH")[il]
+        lib().orc_species_fractions(self.c, ie, il, T, ne, nH * xHI, nH * (1.0 - xHI), ptr(out))
+        return out[:self.elements[ie]["nstage"]].copy()
 
     # raw pointer access to the context arrays
     def _head(self):
@@ -294,6 +433,8 @@ class Model:
                              P.get("xHeII_init", 0.0))
         if rhokap is not None:
             self.arr("rhokap")[:] = rhokap
+        if self.elements:
+            self.opacity_fill()          # gas_opacity_setup -> gas_opacity_fill
 
     def transport_pass(self, first=1, last=None, nthreads=1):
         if last is None:
@@ -434,6 +575,11 @@ class _CtxHead(C.Structure):
         ("cool", Cooling),
         ("milne", C.c_void_p), ("dif_ch", _dp), ("dif_cdf", _dp),
         ("diffuse_lum", C.c_double), ("diffuse_nphot", C.c_int64),
+        ("use_metals", C.c_int), ("nel", C.c_int), ("ntr", C.c_int), ("nst", C.c_int),
+        ("el", C.c_byte * (11 * 3648)),
+        ("tr_off", C.c_int * 11), ("st_off", C.c_int * 11),
+        ("flat_ie", C.c_int * 64), ("flat_it", C.c_int * 64), ("flat_eth", C.c_double * 64),
+        ("mraw", _dp), ("mgam", _dp), ("mheat", _dp), ("mfrac", _dp),
     ]
 
 
@@ -477,6 +623,12 @@ def ref_lib():
     L.ref_milne_sample.restype = C.c_double
     L.ref_milne_eper.argtypes = [C.c_int, C.c_double]
     L.ref_milne_eper.restype = C.c_double
+    L.ref_vfky96.argtypes = [C.c_double, _dp]
+    L.ref_vfky96.restype = C.c_double
+    L.ref_vy95.argtypes = [C.c_double, _dp]
+    L.ref_vy95.restype = C.c_double
+    L.ref_ci_dere_ratio.argtypes = [C.c_int, C.c_int]
+    L.ref_ci_dere_ratio.restype = C.c_double
     return L
 
 

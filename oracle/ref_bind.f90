@@ -10,7 +10,7 @@ module ref_bind
   use qmc_mod, only : qmc_setup, qmc_uniforms_stream, qmc_uniforms_raw
   use energy_sampler_mod, only : energy_sampler_type, build_planck_sampler, &
        build_tabulated_sampler, sample_energy_cdf, energy_cdf_value
-  use photo_xsec, only : sigma_HI, sigma_HeI, sigma_HeII
+  use photo_xsec, only : sigma_HI, sigma_HeI, sigma_HeII, sigma_vfky96, sigma_vy95
   use recomb_mod
   use milne_recomb_spectrum_mod, only : milne_setup, milne_sample_energy, &
        milne_energy_per_recomb
@@ -111,6 +111,25 @@ contains
     case default; ref_rate = ci_HeII(T)
     end select
   end function ref_rate
+
+  real(c_double) function ref_vfky96(e, p) bind(C, name='ref_vfky96')
+    real(c_double), value :: e
+    real(c_double), intent(in) :: p(8)
+    ref_vfky96 = sigma_vfky96(e, p(1), p(2), p(3), p(4), p(5), p(6), p(7), p(8))
+  end function ref_vfky96
+
+  real(c_double) function ref_vy95(e, p) bind(C, name='ref_vy95')
+    real(c_double), value :: e
+    real(c_double), intent(in) :: p(7)
+    ref_vy95 = sigma_vy95(e, p(1), p(2), p(3), p(4), p(5), p(6), p(7))
+  end function ref_vy95
+
+  real(c_double) function ref_ci_dere_ratio(Z, stage) bind(C, name='ref_ci_dere_ratio')
+    integer(c_int), value :: Z, stage
+    par%ci_model = 'dere_hybrid'
+    ref_ci_dere_ratio = ci_dere_ratio(Z, stage)
+    par%ci_model = 'voronov'
+  end function ref_ci_dere_ratio
 
   subroutine ref_gaunt_ff(z, log10te, xlf, g, n) bind(C, name='ref_gaunt_ff')
     real(c_float), value :: z, log10te

@@ -64,6 +64,35 @@ def test_fortran_binding_layout_matches_header():
         pytest.skip("Fortran driver not built (amdflang absent)")
     out = subprocess.check_output([exe, "layout"], text=True).split("\n")
     got = dict(line.split() for line in out if line.strip() and not line.startswith("Fortran"))
+    el = {k.split(".", 1)[1]: v for k, v in got.items() if k.startswith("element.")}
+    got = {k: v for k, v in got.items() if not k.startswith("element.")}
     assert int(got.pop("sizeof")) == C.sizeof(Params)
     for name, off in got.items():
         assert int(off) == getattr(Params, name).offset, name
+    from mochii_amd.atomic import Element
+    assert int(el.pop("sizeof")) == C.sizeof(Element)
+    for name, off in el.items():
+        assert int(off) == getattr(Element, name).offset, name
+
+
+def test_element_layout_matches_header_and_oracle():
+    """sizeof/offsets of mc_element (C compiler) equal the ctypes mirror and
+    the oracle's orc_element, so one host record serves both."""
+    import subprocess
+    import tempfile
+    from mochii_amd.atomic import Element
+    fields = ["abund", "photo", "ndr", "dr2", "rr2", "cxr", "ncool", "cool_T"]
+    src = ('#include "mochii_gpu.h"\n#include <stdio.h>\n#include <stddef.h>\n'
+           'int main(){printf("%zu", sizeof(mc_element));' +
+           "".join(f'printf(" %zu", offsetof(mc_element, {f}));' for f in fields) +
+           'printf("\\n"); return 0;}\n')
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "t.c"), "w") as f:
+            f.write(src)
+        exe = os.path.join(d, "t")
+        subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), "-o", exe,
+                               os.path.join(d, "t.c")])
+        got = [int(x) for x in subprocess.check_output([exe]).split()]
+    assert got == [C.sizeof(Element)] + [getattr(Element, f).offset for f in fields]
+    from oracle import pyoracle as po
+    assert po.lib().orc_element_size() == C.sizeof(Element)
