@@ -19,6 +19,11 @@
 
 #include "mc_internal.h"
 
+// The trace-metal registry of the calling context, copied in before every
+// call that uses it (calls are synchronous and come from one host thread).
+// Read with wave-uniform indices through the scalar cache.
+__constant__ DevMetalTab c_mt;
+
 namespace {
 
 constexpr int kBlock = 256;
@@ -54,11 +59,6 @@ struct KArgs {
     const double *mop;       // opacity weights abund x stage fraction [il*ntr + k]
     const float *mopf;       // the same x n_H d2cm in FP32 (mixed form)
 };
-
-// The trace-metal registry of the calling context, copied in before every
-// call that uses it (calls are synchronous and come from one host thread).
-// Read with wave-uniform indices through the scalar cache.
-__constant__ DevMetalTab c_mt;
 
 // H/He recombination and collisional-ionisation coefficients at one T
 struct RateSet { double aH, aHe2, aHe3, cH, cHe1, cHe2; };

@@ -34,8 +34,23 @@ def stromgren(n=64, nphotons=1_000_000, precision=MC_FP64, nH=100.0, half=4.0,
     return params, nHarr
 
 
+# Trace metals of the H II region workloads: C N O Ne S at the reference's
+# defaults (src/define.f90:847-851) and Ar, which the reference leaves at 0,
+# at the solar Ar/H = 2.5e-6 (Asplund et al. 2009), giving the 15 transitions
+# of C N O Ne S Ar.
+METAL_ABUND = {"ar": 2.5e-6}
+
+
+def elements_of(params):
+    """The trace-metal registry a workload's params were built for ([] when
+    the metal path is off)."""
+    from .atomic import registry
+    return registry(METAL_ABUND) if params.get("metal_eth") else []
+
+
 def hii_thermal(n=32, nphotons=200_000, precision=MC_FP64, seed=12345, niter=100,
-                tol=2e-3, tol_te=2e-3, half=8.0, octree=None, nH=None, diffuse=False):
+                tol=2e-3, tol_te=2e-3, half=8.0, octree=None, nH=None, diffuse=False,
+                metals=False):
     """H + He (He/H = 0.1) H II region with thermal balance: one 45000 K
     blackbody at the box centre with Q(H) = 3e49 s^-1, case A, T_e solved in
     [1e3, 5e4] K, volume convergence criterion; with diffuse=True explicit
@@ -51,6 +66,9 @@ def hii_thermal(n=32, nphotons=200_000, precision=MC_FP64, seed=12345, niter=100
                   gas_tol_te=tol_te, conv_vol=1, gas_niter=niter, case_A=1, solve_te=1,
                   xHI_init=1.0, xHeI_init=1.0, xHeII_init=0.0, diffuse_field=int(diffuse),
                   diffuse_model=2)
+    if metals:
+        from .atomic import metal_thresholds, registry
+        params["metal_eth"] = metal_thresholds(registry(METAL_ABUND))
     if nH is None:
         nH = np.full(n ** 3, 100.0)
     return params, nH
@@ -161,19 +179,19 @@ def octree_from_field(field, half, lmax=None, thresh_dex=0.3, lmin=1):
 
 def hii_octree(n_field=128, nphotons=10_000_000, precision=MC_FP64, seed=12345, field_seed=1,
                half=8.0, lmax=7, thresh_dex=0.3, lmin=1, niter=100, tol=2e-3, tol_te=2e-3,
-               diffuse=True):
+               diffuse=True, metals=True):
     """H + He H II region with thermal balance on an adaptive octree: n_H
     lognormal (mean 100 cm^-3, sigma_ln 0.5) from a seeded Gaussian random
     field with P(k) ~ k^-11/3 on n_field^3, leaves refined to at most lmax
     levels where the density varies by more than thresh_dex across a cell,
-    He/H = 0.1, one 45000 K blackbody with Q(H) = 3e49 s^-1 at the centre,
-    explicit diffuse recombination packets (case A), nphotons stellar
-    packets per iteration.  Returns (params, nH per leaf, octree leaf list
+    He/H = 0.1, trace metals C N O Ne S Ar, one 45000 K blackbody with
+    Q(H) = 3e49 s^-1 at the centre, explicit diffuse recombination packets
+    (case A), nphotons stellar packets per iteration.  Returns (params, nH per leaf, octree leaf list
     (x, y, z, level, half))."""
     field = lognormal_field(n_field, mean=100.0, sigma_ln=0.5, seed=field_seed)
     xl, yl, zl, lev, nHl = octree_from_field(field, half=half, lmax=lmax, thresh_dex=thresh_dex,
                                              lmin=lmin)
     params, _ = hii_thermal(n=n_field, nphotons=nphotons, precision=precision, seed=seed,
                             niter=niter, tol=tol, tol_te=tol_te, half=half, octree=True,
-                            nH=nHl, diffuse=diffuse)
+                            nH=nHl, diffuse=diffuse, metals=metals)
     return params, nHl, (xl, yl, zl, lev, half)

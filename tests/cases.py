@@ -16,6 +16,8 @@ _MAP = {  # library Params name -> oracle param name
     "ion_add_dust": "add_dust", "ion_dust_scatter": "dust_scatter", "lambda_ref": "lambda_ref",
     "xHI_init": "xHI_init", "xHeI_init": "xHeI_init", "xHeII_init": "xHeII_init",
     "diffuse_field": "diffuse_field", "diffuse_model": "diffuse_model",
+    "metal_eth": "metal_eth", "ion_metal_abs": "ion_metal_abs", "metal_ne": "metal_ne",
+    "metal_heat": "metal_heat", "metal_freefree": "metal_freefree",
 }
 _ORC_DEFAULTS = dict(source_cdf_tol=1e-8, align_edges=1, lambda_ref=0.55, te_min=1e3,
                      te_max=5e4, ext_geometry=2)
@@ -29,26 +31,34 @@ def oracle_params(params):
     if out.get("diffuse_field"):
         out["caseA"] = 1                   # the diffuse field forces case A
         out.setdefault("diffuse_model", 2)
+    if "charge_exchange" in params:
+        out["cx_two_way"] = int(params["charge_exchange"] == 0)
     return out
 
 
-def oracle_model(params, nH, rhokap=None, dust_table=None, octree=None):
+def oracle_model(params, nH, rhokap=None, dust_table=None, octree=None, elements=None):
+    from mochii_amd.workloads import elements_of
     op = oracle_params(params)
+    els = elements if elements is not None else elements_of(params)
     m = po.Model({k: v for k, v in op.items()
                   if k not in ("xHI_init", "xHeI_init", "xHeII_init")} | {}, octree=octree,
-                 dust_table=dust_table)
+                 dust_table=dust_table, elements=els)
     m.params.update(op)
     m.state_init(nH, rhokap=rhokap)
     return m
 
 
-def gpu_model(params, nH, rhokap=None, dust_table=None, octree=None):
+def gpu_model(params, nH, rhokap=None, dust_table=None, octree=None, elements=None):
     import mochii_amd as mc
+    from mochii_amd.workloads import elements_of
     m = mc.Model(dict(params))
     if octree is not None:
         m.grid_octree(*octree)
     if dust_table is not None:
         m.ion_dust_table(*dust_table)
+    els = elements if elements is not None else elements_of(params)
+    if els:
+        m.species_setup(els)
     m.gas_state_setup(nH)
     if rhokap is not None:
         m.set_state(rhokap=rhokap)
