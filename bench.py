@@ -5,8 +5,9 @@ surrogate + FP32 form against the plain all-FP64 form.
 Default workload: H + He H II region with thermal balance on an adaptive
 octree (n_H lognormal, mean 100 cm^-3, sigma_ln 0.5, from a seeded k^-11/3
 Gaussian random field on 128^3; at most 7 levels, refined where the density
-varies by more than 0.3 dex; one 45000 K star with Q(H) = 3e49 s^-1; explicit
-diffuse recombination packets, case A; 1e7 stellar packets per iteration).
+varies by more than 0.3 dex; He/H = 0.1 and the trace metals C N O Ne S Ar;
+one 45000 K star with Q(H) = 3e49 s^-1; explicit diffuse recombination
+packets, case A; 1e7 stellar packets per iteration).
 `--workload stromgren` runs the pure-H Stromgren sphere on a 64^3 car grid
 (fixed T, 1e6 packets per iteration) instead.
 
@@ -47,18 +48,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per scored path segment (d = dust, n_tr = metal
-# transitions, both 0 here): state reads + 7 FP64 tally read-modify-writes
-#   car FP64:  8 (4 + d + n_tr) + 16 (7 + d + 2 n_tr)           = 144
-#   car mixed: 4 (4 + d + n_tr) (FP32 opacity record) + 112      = 128
+# Algorithmic bytes per scored path segment (d = dust, 0 here; n_tr = metal
+# transitions below the packet's energy, counted per segment by the walk):
+# state reads + 7 FP64 tally read-modify-writes + 2 per metal transition
+#   car FP64:  8 (4 + d + n_tr) + 16 (7 + d + 2 n_tr)           = 144 + 40 n_tr
+#   car mixed: 4 (4 + d + n_tr) (FP32 opacity record) + 112      = 128 + 36 n_tr
 #   octree:    + leaf geometry per segment: FP64 centre + half + neighbour
 #              and child links (44 B), FP32 centre + half + links (24 B)
 BYTES_PER_SEGMENT = {("car", "fp64"): 144, ("car", "mixed_mlp"): 128,
                      ("octree", "fp64"): 188, ("octree", "mixed_mlp"): 152}
-WALK_KERNEL = {("car", "fp64"): "k_walk<MC_FP64, kCarDDA>",
-               ("car", "mixed_mlp"): "k_walk<MC_MIXED, kCarDDA>",
-               ("octree", "fp64"): "k_walk<MC_FP64, kFaceExit>",
-               ("octree", "mixed_mlp"): "k_walk<MC_MIXED, kFaceExit>"}
+BYTES_PER_METAL_TR = {"fp64": 40, "mixed_mlp": 36}
+WALK_KERNEL = {("car", "fp64"): "k_walk<MC_FP64, kCarDDA, M>",
+               ("car", "mixed_mlp"): "k_walk<MC_MIXED, kCarDDA, M>",
+               ("octree", "fp64"): "k_walk<MC_FP64, kFaceExit, M>",
+               ("octree", "mixed_mlp"): "k_walk<MC_MIXED, kFaceExit, M>"}
 
 
 def parse():
@@ -106,8 +109,9 @@ def build_workload(name, packets):
         params, nH, octree = hii_octree(nphotons=N)
         desc = (f"H+He H II region with thermal balance, octree of {nH.size} leaves (<=7 levels, "
                 f"0.3 dex) over a lognormal n_H field (mean 100, sigma_ln 0.5, k^-11/3, 128^3), "
-                f"box 16 pc, 45000 K Q(H)=3e49, He/H=0.1, no trace metals, explicit diffuse "
-                f"recombination packets (case A), {N:.0e} stellar packets/iteration")
+                f"box 16 pc, 45000 K Q(H)=3e49, He/H=0.1, trace metals C N O Ne S Ar "
+                f"(15 transitions), explicit diffuse recombination packets (case A), "
+                f"{N:.0e} stellar packets/iteration")
         return params, nH, octree, "octree", desc
     N = packets or 1_000_000
     params, nH = stromgren(n=64, nphotons=N)
@@ -119,9 +123,13 @@ def build_workload(name, packets):
 def make_model(params, nH, octree, precision, warm=None, mlp=None):
     import mochii_amd as mc
     from mochii_amd.workloads import HI_COOLING_FIT
+    from mochii_amd.workloads import elements_of
     m = mc.Model(dict(params, precision=precision))
     if octree is not None:
         m.grid_octree(*octree)
+    els = elements_of(params)
+    if els:
+        m.species_setup(els)
     m.gas_state_setup(nH)
     if params.get("solve_te"):
         m.cooling_setup(HI_COOLING_FIT)
@@ -141,10 +149,8 @@ def train_surrogate():
     from mochii_amd.workloads import HI_COOLING_FIT, hii_octree, hii_thermal
     t0 = time.perf_counter()
     Xs, Ys = [], []
-    p, nH = hii_thermal(n=32, nphotons=300_000, diffuse=True, seed=99)
-    m = mc.Model(dict(p))
-    m.gas_state_setup(nH)
-    m.cooling_setup(HI_COOLING_FIT)
+    p, nH = hii_thermal(n=32, nphotons=300_000, diffuse=True, seed=99, metals=True)
+    m = make_model(p, nH, None, mc.MC_FP64)
     X, Y = surrogate.collect(m, 40)
     Xs.append(X)
     Ys.append(Y)
@@ -193,7 +199,7 @@ def run_form(name, params, nH, octree, args, mlp=None):
     for _ in range(args.warmup):
         m.ionizing_field_and_rates()
         update()
-    walk_ms, solve_ms, npk, nseg, evals, brk = [], [], [], [], [], []
+    walk_ms, solve_ms, npk, nseg, nmet, evals, brk = [], [], [], [], [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         m.ionizing_field_and_rates()
@@ -202,6 +208,7 @@ def run_form(name, params, nH, octree, args, mlp=None):
         npk.append(params["nphotons"] + m.diffuse_info()[1])
         tal = m.tallies()
         nseg.append(tal["seg_direct"] + tal["seg_scattered"])
+        nmet.append(m.metal_segment_transitions())
         update()
         solve_ms.append(m.timing()["solve_ms"])
         if thermal:
@@ -213,7 +220,9 @@ def run_form(name, params, nH, octree, args, mlp=None):
                stellar_packets=int(params["nphotons"]),
                diffuse_packets=int(np.mean(npk)) - int(params["nphotons"]),
                walk_ms=float(np.mean(walk_ms)), solve_ms=float(np.mean(solve_ms)),
-               segments_per_pass=int(np.mean(nseg)), time_to_converged_s=t_conv,
+               segments_per_pass=int(np.mean(nseg)),
+               metal_transitions_per_segment=float(np.sum(nmet) / max(np.sum(nseg), 1)),
+               time_to_converged_s=t_conv,
                iterations=res["iterations"], converged=res["converged"])
     if thermal:
         out["net_rate_evals_per_bracketed_leaf"] = float(np.sum(evals) / max(np.sum(brk), 1))
@@ -263,6 +272,7 @@ def cpu_baseline(params, nH, octree, state, seconds):
     ncpu = min(os.cpu_count() or 1, 16)
     orc = oracle_model(dict(params, diffuse_field=0), nH, octree=octree)
     orc.set_state(**{k: state[k] for k in ("xHI", "xHeI", "xHeII", "ne", "Te")})
+    orc.opacity_fill()                   # stage cache of the metal opacity
     N = params["nphotons"]
     n, dt = min(20_000, N), 0.0
     while True:                          # grow the sample until it takes ~seconds
@@ -320,7 +330,10 @@ def main():
     if cpu:
         log(f"[cpu] {cpu}")
 
-    bps = BYTES_PER_SEGMENT[(grid, "mixed_mlp")]
+    # algorithmic bytes per segment: the H/He record plus the metal weights
+    # and raw tallies of the transitions below each packet's energy
+    bps = (BYTES_PER_SEGMENT[(grid, "mixed_mlp")] +
+           BYTES_PER_METAL_TR["mixed_mlp"] * fm["metal_transitions_per_segment"])
     achieved = fm["segments_per_pass"] * bps / (fm["walk_ms"] * 1e-3) / 1e9
     line = {
         "metric": "packets/s + time-to-converged model, surrogate+FP32 vs plain FP64, "
@@ -344,7 +357,10 @@ def main():
         "error_mixed_mlp_vs_fp64": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": WALK_KERNEL[(grid, "mixed_mlp")], "bytes_per_segment": bps,
+                     "kernel": WALK_KERNEL[(grid, "mixed_mlp")].replace(
+                         "M>", "1>" if params.get("metal_eth") else "0>"),
+                     "bytes_per_segment": bps,
+                     "metal_transitions_per_segment": fm["metal_transitions_per_segment"],
                      "segments_per_pass": fm["segments_per_pass"],
                      "walk_ms_per_pass": fm["walk_ms"]},
         "cpu_baseline": cpu,

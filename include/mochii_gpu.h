@@ -301,6 +301,12 @@ int mc_species_rates_get(mc_ctx *ctx, int ie, double *gam, double *heat);
  * (metal_rate_path(ie)%raw, src/species_mod.f90:102-108, 329-343). */
 int mc_species_tallies_get(mc_ctx *ctx, int ie, double *raw);
 
+/* Metal transitions below the packet energy summed over the scored segments
+ * of the last pass: each scored segment updates 2 raw entries per such
+ * transition (the metal part of the walk's algorithmic tally traffic).  No
+ * counterpart in the reference. */
+int mc_metal_segment_transitions(mc_ctx *ctx, int64_t *sum_ntr);
+
 /* hydrogen_balance_residual_grid (src/ion_balance_mod.f90:313-343). */
 int mc_hydrogen_balance_residual_grid(mc_ctx *ctx, double *eps_max, double *eps_vol);
 

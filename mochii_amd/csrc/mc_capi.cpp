@@ -270,7 +270,7 @@ extern "C" int mc_init(const mc_params *p, mc_ctx **out)
         mc_finalize(c);
         return -100;
     }
-    if (dalloc(&c->st.ledger, 3) || dalloc(&c->st.segs, 2) || dalloc(&c->st.counter, 1)) {
+    if (dalloc(&c->st.ledger, 3) || dalloc(&c->st.segs, 3) || dalloc(&c->st.counter, 1)) {
         mc_finalize(c);
         return -100;
     }
@@ -932,5 +932,16 @@ extern "C" int mc_species_tallies_get(mc_ctx *c, int ie, double *raw)
             for (int q = 0; q < 2; ++q)
                 raw[((size_t)il * nt + it) * 2 + q] = h[((size_t)il * ntr + k) * 2 + q];
         }
+    return 0;
+}
+
+extern "C" int mc_metal_segment_transitions(mc_ctx *c, int64_t *sum_ntr)
+{
+    CHECK_CTX(c);
+    if (!sum_ntr) {
+        mc_set_error("mc_metal_segment_transitions: null output");
+        return -1;
+    }
+    *sum_ntr = c->segs[2];
     return 0;
 }

@@ -166,7 +166,7 @@ struct DevState {
     double *jt;        // nleaf * nnu
     double *rates;     // 6 * nleaf
     double *ledger;    // 3
-    unsigned long long *segs;   // 2
+    unsigned long long *segs;   // 3: direct, scattered, sum of metal transitions
     unsigned long long *counter;// work counter for persistent kernels
     double *partials;  // per-block metric partials
     // leaves next to the point sources take many updates per pass; their
@@ -249,7 +249,7 @@ struct mc_ctx {
     float *d_mopf;
     // last pass results
     double ledger[3];
-    int64_t segs[2];
+    int64_t segs[3];          // direct, scattered, metal transitions over scored segments
     double times[4];
 };
 

@@ -426,6 +426,7 @@ __device__ __forceinline__ float opacity_f(const KArgs &a, const Lane &L, int il
 struct Counters {
     double e_emit, e_abs, e_esc;
     unsigned long long sd, ss;
+    unsigned long long smet;     // metal transitions summed over scored segments
 };
 
 // score_ion_path: the packet's path quantity into its leaf.  A coefficient
@@ -442,6 +443,7 @@ __device__ __forceinline__ void score(const KArgs &, Lane &L, int il, double pl,
     L.s_il = il;
     L.s_pl = pl;
     if (L.nscatt > 0) cn.ss++; else cn.sd++;
+    cn.smet += (unsigned long long)L.nmet;
 }
 
 // tally destination of a leaf: its own rows, or one of the replicas of a hot
@@ -1181,7 +1183,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
     L.phase = 0;
     L.s_il = -1;
     L.nmet = 0;
-    Counters cn = {0.0, 0.0, 0.0, 0ull, 0ull};
+    Counters cn = {0.0, 0.0, 0.0, 0ull, 0ull, 0ull};
     const int lane = threadIdx.x & 63;
     for (;;) {
         // direction-sorted records: the wave takes 64 consecutive records at
@@ -1214,6 +1216,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
         cn.e_esc += __shfl_down(cn.e_esc, off);
         cn.sd += __shfl_down(cn.sd, off);
         cn.ss += __shfl_down(cn.ss, off);
+        cn.smet += __shfl_down(cn.smet, off);
     }
     if (lane == 0) {
         atomicAdd(a.s.ledger + 0, cn.e_emit);
@@ -1221,6 +1224,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
         atomicAdd(a.s.ledger + 2, cn.e_esc);
         atomicAdd(a.s.segs + 0, cn.sd);
         atomicAdd(a.s.segs + 1, cn.ss);
+        if (cn.smet) atomicAdd(a.s.segs + 2, cn.smet);
     }
 }
 
@@ -1237,7 +1241,7 @@ __global__ void k_reset_counter(unsigned long long *p) { *p = 0ull; }
 __global__ void k_zero_counters(DevState s)
 {
     if (threadIdx.x < 3) s.ledger[threadIdx.x] = 0.0;
-    if (threadIdx.x < 2) s.segs[threadIdx.x] = 0ull;
+    if (threadIdx.x < 3) s.segs[threadIdx.x] = 0ull;
     if (threadIdx.x == 0) *s.counter = 0ull;
 }
 
@@ -2221,13 +2225,14 @@ int mc_run_pass(mc_ctx *c)
         MC_HIP(hipGetLastError());
     }
     double led[3];
-    unsigned long long sg[2];
+    unsigned long long sg[3];
     MC_HIP(hipMemcpyAsync(led, c->st.ledger, sizeof(led), hipMemcpyDeviceToHost, st));
     MC_HIP(hipMemcpyAsync(sg, c->st.segs, sizeof(sg), hipMemcpyDeviceToHost, st));
     MC_HIP(hipStreamSynchronize(st));
     for (int k = 0; k < 3; ++k) c->ledger[k] = led[k];
     c->segs[0] = (int64_t)sg[0];
     c->segs[1] = (int64_t)sg[1];
+    c->segs[2] = (int64_t)sg[2];
     c->times[0] = t_launch;
     c->times[1] = t_walk;
     return 0;

@@ -141,6 +141,7 @@ def library():
         "mc_species_fractions_get": ([vp, C.c_int, dp], C.c_int),
         "mc_species_rates_get": ([vp, C.c_int, dp, dp], C.c_int),
         "mc_species_tallies_get": ([vp, C.c_int, dp], C.c_int),
+        "mc_metal_segment_transitions": ([vp, i64p], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -70,6 +70,13 @@ class Model:
         check(self._L.mc_species_setup(self._h, len(self.elements), self._els),
               "mc_species_setup")
 
+    def metal_segment_transitions(self):
+        """Metal transitions summed over the last pass's scored segments."""
+        v = C.c_int64()
+        check(self._L.mc_metal_segment_transitions(self._h, C.byref(v)),
+              "mc_metal_segment_transitions")
+        return v.value
+
     def species_fractions(self, ie):
         """stage_frac of registry element ie: (nleaf, nstage)."""
         ns = self.elements[ie]["nstage"]
