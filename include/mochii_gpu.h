@@ -243,7 +243,12 @@ int mc_gas_thermal_update(mc_ctx *ctx, double *max_dx, double *max_dte, double *
  * No counterpart in the reference (whose solve is the MC_WARM_NONE
  * bisection): every warm mode keeps the reference's edge tests and accepts
  * only a sign-change bracket of the same width thi/tlo - 1 < 1e-5, falling
- * back to bisection from the bracket it has when Newton stalls. */
+ * back to bisection from the bracket it has when Newton stalls.  With the
+ * metal registry and two-way charge exchange, a warm mode also ends a trial
+ * solve's n_e iteration after 24 consecutive passes in which the leaf's
+ * hydrogen is unresolved (x_HII < 1e-10, leaves without ionising flux): there
+ * the reference's iteration wanders in a noise band of n_e until its
+ * 200-pass cap. */
 int mc_set_warm_start(mc_ctx *ctx, int mode);
 
 /* Work of the last thermal update: {net-rate evaluations spent on the

@@ -864,6 +864,19 @@ extern "C" int mc_species_setup(mc_ctx *c, int n, const mc_element *els)
         m.flat_eth[j + 1] = e;
     }
     for (int k = 0; k < ntr; ++k) m.t_flat[m.flat_t[k]] = k;
+    for (int ie = 0; ie < n; ++ie) m.el_ab[ie] = els[ie].abund;
+    // compiled-in element layouts of the cell-solve kernels
+    static const char *kCNONeSAr[] = {"c", "n", "o", "ne", "s", "ar"};
+    auto layout_is = [&](const char *const *names, int ne_, const int *nst) {
+        if (n != ne_) return false;
+        for (int ie = 0; ie < n; ++ie)
+            if (std::strcmp(els[ie].name, names[ie]) != 0 || els[ie].nstage != nst[ie]) return false;
+        return true;
+    };
+    static const int kStages[] = {4, 3, 3, 3, 4, 4};
+    c->metal_layout = layout_is(kCNONeSAr, 6, kStages)   ? MC_LAYOUT_CNONeSAr
+                      : layout_is(kCNONeSAr, 5, kStages) ? MC_LAYOUT_CNONeS
+                                                         : MC_LAYOUT_GENERIC;
     c->elements.assign(els, els + n);
     c->nel = n; c->ntr = ntr; c->nst = nst;
     if (alloc_metal_arrays(c)) return -100;

@@ -80,6 +80,11 @@ struct MlpState {
 // order of species_packet_cross_sections).  The cell-solve kernels keep the
 // (leaf, T) rate cache in registers with loops unrolled to MC_GPU_MAXTR.
 #define MC_GPU_MAXTR 16
+// Registry layouts with compiled-in element structure (transitions per
+// element in registry order); any other registry takes the generic path
+#define MC_LAYOUT_GENERIC 0
+#define MC_LAYOUT_CNONeSAr 1    // C N O Ne S Ar: 3 2 2 2 3 3 transitions
+#define MC_LAYOUT_CNONeS 2      // C N O Ne S:    3 2 2 2 3
 #define MC_GPU_MAXST (MC_GPU_MAXTR + MC_MAX_EL)
 struct DevMetalTab {
     int32_t nel, ntr, nst;
@@ -103,6 +108,7 @@ struct DevMetalTab {
     double  cxi[MC_GPU_MAXTR][6], cxr[MC_GPU_MAXTR][6];
     int32_t flat_t[MC_GPU_MAXTR];
     double  flat_eth[MC_GPU_MAXTR];
+    double  el_ab[MC_MAX_EL];        // abundance of each element
     int32_t s_ncool[MC_GPU_MAXST];
     int32_t s_charge[MC_GPU_MAXST];  // stage j of its element has charge j - 1
     double  s_A[MC_GPU_MAXST][MC_MAX_COOLFIT], s_T[MC_GPU_MAXST][MC_MAX_COOLFIT];
@@ -245,6 +251,7 @@ struct mc_ctx {
     std::vector<mc_element> elements;
     DevMetalTab mtab;
     DevMetalTab *d_mtab;
+    int metal_layout;         // MC_LAYOUT_* of the registry (cell-solve kernels)
     double *d_mraw, *d_mgam, *d_mheat, *d_mfrac, *d_mop;
     float *d_mopf;
     // last pass results

@@ -1573,7 +1573,7 @@ __global__ __launch_bounds__(kBlock) void k_launch_diffuse(KArgs a)
 
 // gas_equilibrium_update at fixed T; partial metrics per block:
 // {max|dx|, sum V|dx|, sum V x_HII}
-template <int MT>
+template <int MT, int LAY>
 __global__ __launch_bounds__(kBlock) void k_equilibrium(DevGrid g, DevState s, double T, bool caseA,
                                                         double yHe, double w, double d2cm,
                                                         MetalSw ms)
@@ -1595,7 +1595,7 @@ __global__ __launch_bounds__(kBlock) void k_equilibrium(DevGrid g, DevState s, d
                                 xHeI, xHeII);
             } else {
                 m_prepare<MT>((&c_mt), ms.gam + (size_t)il * ms.ntr, T, cc);
-                ne = solve_cell_m<MT>(ms, cc, r, s.rates[il], s.rates[n + il], s.rates[2 * n + il],
+                ne = solve_cell_m<MT, LAY>(ms, cc, r, s.rates[il], s.rates[n + il], s.rates[2 * n + il],
                                       nH, yHe, xHI, xHeI, xHeII);
             }
             const double oHI = s.xHI[il];
@@ -1608,7 +1608,7 @@ __global__ __launch_bounds__(kBlock) void k_equilibrium(DevGrid g, DevState s, d
             s.xHeII[il] = nHeII;
             double nen = nH * ((1.0 - nHI) + yHe * (nHeII + 2.0 * xHeIII));
             if constexpr (MT != 0)
-                if (ms.metal_ne) nen = charge_neutrality_m<MT>((&c_mt), cc, nH, nen, nHI, ne);
+                if (ms.metal_ne) nen = charge_neutrality_m<MT, LAY>(cc, nH, nen, nHI, ne);
             s.ne[il] = nen;
             s.opac[il] = opac_of(nH, nHI, nHeI, nHeII, s.rhokap[il], yHe, d2cm);
             const double wd = 2.0 * leaf_half(g, il), vol = (wd * wd) * wd;
@@ -1694,9 +1694,9 @@ __device__ double net_rate_lr(const DevCool &cl, const CellIn &c, double T, bool
 // fractions.
 struct NetOut { double net, lr, xHI, xHeI, xHeII, ne; };
 
-template <int MT>
-__device__ __noinline__ NetOut net_rate_mo(const DevCool &cl, const MetalSw &ms, int il,
-                                           const CellIn &c, double T, bool caseA, double y,
+template <int MT, int LAY>
+__device__ __noinline__ NetOut net_rate_mo(const DevCool &cl, const MetalSw ms, int il,
+                                           const CellIn c, double T, bool caseA, double y,
                                            double xHI, double xHeI, double xHeII)
 {
     double ne;
@@ -1705,7 +1705,7 @@ __device__ __noinline__ NetOut net_rate_mo(const DevCool &cl, const MetalSw &ms,
     {
         MCache<MT> cc;
         m_prepare<MT>((&c_mt), ms.gam + (size_t)il * ms.ntr, T, cc);
-        ne = solve_cell_m<MT>(ms, cc, r, c.gH, c.gHe1, c.gHe2, c.nH, y, xHI, xHeI, xHeII);
+        ne = solve_cell_m<MT, LAY>(ms, cc, r, c.gH, c.gHe1, c.gHe2, c.nH, y, xHI, xHeI, xHeII);
         m_fracs<MT>((&c_mt), cc, ne, c.nH * xHI, c.nH * (1.0 - xHI), lo, up);
     }
     const double heat = c.nH * (xHI * c.hH + y * (xHeI * c.hHe1 + xHeII * c.hHe2));
@@ -1744,7 +1744,7 @@ __device__ __noinline__ NetOut net_rate_mo(const DevCool &cl, const MetalSw &ms,
 // leaves (those not pinned at te_min / te_max)}
 constexpr int kNewtonMax = 24;
 
-template <int W, int MT>
+template <int W, int MT, int LAY>
 __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCool cl, bool caseA,
                                                     double y, double w, double tmin, double tmax,
                                                     double d2cm, const float *pred, MetalSw ms)
@@ -1766,7 +1766,7 @@ __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCo
                 if constexpr (MT == 0) {
                     return ::net_rate(cl_, c_, T, cA, y_, a1, a2, a3, ne_);
                 } else {
-                    const NetOut o = net_rate_mo<MT>(cl_, ms, il, c_, T, cA, y_, a1, a2, a3);
+                    const NetOut o = net_rate_mo<MT, LAY>(cl_, ms, il, c_, T, cA, y_, a1, a2, a3);
                     a1 = o.xHI; a2 = o.xHeI; a3 = o.xHeII; ne_ = o.ne;
                     return o.net;
                 }
@@ -1777,7 +1777,7 @@ __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCo
                 if constexpr (MT == 0) {
                     return ::net_rate_lr(cl_, c_, T, cA, y_, a1, a2, a3, ne_, lr);
                 } else {
-                    const NetOut o = net_rate_mo<MT>(cl_, ms, il, c_, T, cA, y_, a1, a2, a3);
+                    const NetOut o = net_rate_mo<MT, LAY>(cl_, ms, il, c_, T, cA, y_, a1, a2, a3);
                     a1 = o.xHI; a2 = o.xHeI; a3 = o.xHeII; ne_ = o.ne; lr = o.lr;
                     return o.net;
                 }
@@ -1866,7 +1866,7 @@ __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCo
                 if (ms.metal_ne) {
                     MCache<MT> cc;
                     m_prepare<MT>((&c_mt), ms.gam + (size_t)il * ms.ntr, te, cc);
-                    nen = charge_neutrality_m<MT>((&c_mt), cc, nH, nen, nHI, ne);
+                    nen = charge_neutrality_m<MT, LAY>(cc, nH, nen, nHI, ne);
                 }
             }
             s.xHI[il] = nHI; s.xHeI[il] = nHeI; s.xHeII[il] = nHeII;
@@ -1889,7 +1889,7 @@ __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCo
 }
 
 // hydrogen_balance_residual_grid partials {max eps, sum V eps, sum V}
-template <int MT>
+template <int MT, int LAY>
 __global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, DevState s, bool caseA, MetalSw ms)
 {
     const int il = blockIdx.x * kBlock + threadIdx.x;
@@ -1904,7 +1904,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, DevState s, bool
                 MCache<MT> cc;
                 double nem;
                 m_prepare<MT>((&c_mt), ms.gam + (size_t)il * ms.ntr, s.Te[il], cc);
-                m_sums<MT>((&c_mt), cc, s.nH[il], s.ne[il], nHI, nHII, nem, rin, rout);
+                m_sums_l<MT, LAY>(cc, s.nH[il], s.ne[il], nHI, nHII, nem, rin, rout);
             }
         }
         const double loss = nHI * (s.rates[il] + r.cH * s.ne[il] + rout);
@@ -2074,7 +2074,9 @@ int persistent_blocks(const void *kern, size_t dyn_lds = 0)
     return ncu * per;
 }
 
-MetalSw make_sw(mc_ctx *c)
+// metal switches of a kernel; thermal = the thermal update (where the
+// warm-started solve applies its noise-band exit)
+MetalSw make_sw(mc_ctx *c, bool thermal = false)
 {
     MetalSw m;
     m.gam = c->d_mgam;
@@ -2084,6 +2086,7 @@ MetalSw make_sw(mc_ctx *c)
     m.cx_two_way = c->p.charge_exchange == MC_CX_TWO_WAY;
     m.metal_heat = c->p.metal_heat != 0;
     m.metal_ff = c->p.metal_freefree != 0;
+    m.noise_cap = thermal && c->warm_mode != MC_WARM_NONE ? 24 : 0;
     return m;
 }
 
@@ -2264,14 +2267,19 @@ int mc_run_equilibrium(mc_ctx *c, double *max_dx, double *dx_vol)
     hipStream_t st = c->stream;
     if (metal_sync(c)) return -100;
     MC_HIP(hipEventRecord(c->ev[6], st));
-    if (c->ntr > 0)
-        k_equilibrium<MC_GPU_MAXTR><<<nblk, kBlock, 0, st>>>(
-            c->grid, c->st, c->p.te_fixed, c->p.case_A != 0, c->p.He_abund, c->p.ion_relax,
-            c->p.distance2cm, make_sw(c));
-    else
-        k_equilibrium<0><<<nblk, kBlock, 0, st>>>(c->grid, c->st, c->p.te_fixed, c->p.case_A != 0,
-                                                  c->p.He_abund, c->p.ion_relax, c->p.distance2cm,
-                                                  make_sw(c));
+#define MC_EQ_ARGS c->grid, c->st, c->p.te_fixed, c->p.case_A != 0, c->p.He_abund, \
+                   c->p.ion_relax, c->p.distance2cm, make_sw(c)
+    if (c->ntr > 0) {
+        if (c->metal_layout == MC_LAYOUT_CNONeSAr)
+            k_equilibrium<MC_GPU_MAXTR, MC_LAYOUT_CNONeSAr><<<nblk, kBlock, 0, st>>>(MC_EQ_ARGS);
+        else if (c->metal_layout == MC_LAYOUT_CNONeS)
+            k_equilibrium<MC_GPU_MAXTR, MC_LAYOUT_CNONeS><<<nblk, kBlock, 0, st>>>(MC_EQ_ARGS);
+        else
+            k_equilibrium<MC_GPU_MAXTR, MC_LAYOUT_GENERIC><<<nblk, kBlock, 0, st>>>(MC_EQ_ARGS);
+    } else {
+        k_equilibrium<0, MC_LAYOUT_GENERIC><<<nblk, kBlock, 0, st>>>(MC_EQ_ARGS);
+    }
+#undef MC_EQ_ARGS
     MC_HIP(hipGetLastError());
     MC_HIP(hipEventRecord(c->ev[7], st));
     double m[3];
@@ -2306,20 +2314,24 @@ int mc_run_thermal(mc_ctx *c, double out[4])
     const bool caseA = c->p.case_A != 0;
     if (metal_sync(c)) return -100;
 #define MC_THERMAL_ARGS c->grid, c->st, c->cool, caseA, c->p.He_abund, c->p.ion_relax, \
-                        c->p.te_min, c->p.te_max, c->p.distance2cm, c->d_pred, make_sw(c)
+                        c->p.te_min, c->p.te_max, c->p.distance2cm, c->d_pred, make_sw(c, true)
+#define MC_THERMAL_W(MT_, LAY_)                                                               \
+    do {                                                                                    \
+        if (mode == MC_WARM_MLP)                                                            \
+            k_thermal<MC_WARM_MLP, MT_, LAY_><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);    \
+        else if (mode == MC_WARM_NEWTON)                                                    \
+            k_thermal<MC_WARM_NEWTON, MT_, LAY_><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS); \
+        else                                                                                \
+            k_thermal<MC_WARM_NONE, MT_, LAY_><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);   \
+    } while (0)
     if (c->ntr > 0) {
-        if (mode == MC_WARM_MLP)
-            k_thermal<MC_WARM_MLP, MC_GPU_MAXTR><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
-        else if (mode == MC_WARM_NEWTON)
-            k_thermal<MC_WARM_NEWTON, MC_GPU_MAXTR><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
-        else
-            k_thermal<MC_WARM_NONE, MC_GPU_MAXTR><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
+        if (c->metal_layout == MC_LAYOUT_CNONeSAr) MC_THERMAL_W(MC_GPU_MAXTR, MC_LAYOUT_CNONeSAr);
+        else if (c->metal_layout == MC_LAYOUT_CNONeS) MC_THERMAL_W(MC_GPU_MAXTR, MC_LAYOUT_CNONeS);
+        else MC_THERMAL_W(MC_GPU_MAXTR, MC_LAYOUT_GENERIC);
     } else {
-        if (mode == MC_WARM_MLP) k_thermal<MC_WARM_MLP, 0><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
-        else if (mode == MC_WARM_NEWTON)
-            k_thermal<MC_WARM_NEWTON, 0><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
-        else k_thermal<MC_WARM_NONE, 0><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);
+        MC_THERMAL_W(0, MC_LAYOUT_GENERIC);
     }
+#undef MC_THERMAL_W
 #undef MC_THERMAL_ARGS
     MC_HIP(hipGetLastError());
     MC_HIP(hipEventRecord(c->ev[7], st));
@@ -2351,11 +2363,20 @@ int mc_run_residual(mc_ctx *c, double *eps_max, double *eps_vol)
 {
     const int nblk = (c->nleaf + kBlock - 1) / kBlock;
     if (metal_sync(c)) return -100;
-    if (c->ntr > 0)
-        k_residual<MC_GPU_MAXTR><<<nblk, kBlock, 0, c->stream>>>(c->grid, c->st, c->p.case_A != 0,
-                                                                 make_sw(c));
-    else
-        k_residual<0><<<nblk, kBlock, 0, c->stream>>>(c->grid, c->st, c->p.case_A != 0, make_sw(c));
+    if (c->ntr > 0) {
+        if (c->metal_layout == MC_LAYOUT_CNONeSAr)
+            k_residual<MC_GPU_MAXTR, MC_LAYOUT_CNONeSAr><<<nblk, kBlock, 0, c->stream>>>(
+                c->grid, c->st, c->p.case_A != 0, make_sw(c));
+        else if (c->metal_layout == MC_LAYOUT_CNONeS)
+            k_residual<MC_GPU_MAXTR, MC_LAYOUT_CNONeS><<<nblk, kBlock, 0, c->stream>>>(
+                c->grid, c->st, c->p.case_A != 0, make_sw(c));
+        else
+            k_residual<MC_GPU_MAXTR, MC_LAYOUT_GENERIC><<<nblk, kBlock, 0, c->stream>>>(
+                c->grid, c->st, c->p.case_A != 0, make_sw(c));
+    } else {
+        k_residual<0, MC_LAYOUT_GENERIC><<<nblk, kBlock, 0, c->stream>>>(c->grid, c->st,
+                                                                         c->p.case_A != 0, make_sw(c));
+    }
     MC_HIP(hipGetLastError());
     double m[3];
     if (sum_partials(c, nblk, m)) return -100;

@@ -7,6 +7,7 @@ import pytest
 
 from mochii_amd import MC_FP64, MC_MIXED
 from mochii_amd.lib import MC_CX_METAL_ONLY, MC_WARM_NEWTON, MC_WARM_NONE
+from mochii_amd.atomic import metal_thresholds, registry
 from mochii_amd.workloads import HI_COOLING_FIT, elements_of, hii_thermal
 from tests.cases import gpu_model, oracle_model, rel_close
 
@@ -78,14 +79,23 @@ def test_metal_pass_matches_oracle(prec):
     assert n_tested > 1000
 
 
-@pytest.mark.parametrize("cx,metal_ne", [(0, 1), (MC_CX_METAL_ONLY, 1), (0, 0)])
-def test_metal_equilibrium_update_matches_oracle(cx, metal_ne):
+# registries: C N O Ne S Ar and C N O Ne S take the cell-solve kernels with
+# the element layout compiled in, C O S the generic registry path
+REGISTRIES = {"CNONeSAr": {"ar": 2.5e-6}, "CNONeS": {}, "COS": {"n": 0.0, "ne": 0.0}}
+
+
+@pytest.mark.parametrize("cx,metal_ne,reg", [(0, 1, "CNONeSAr"), (MC_CX_METAL_ONLY, 1, "CNONeSAr"),
+                                             (0, 0, "CNONeSAr"), (0, 1, "CNONeS"),
+                                             (0, 1, "COS")])
+def test_metal_equilibrium_update_matches_oracle(cx, metal_ne, reg):
     """Fixed-T update: solve_ion_cell with the metal electrons and
     (two_way) charge exchange, relaxation, charge neutrality of the written
     state, then the stage cache and the hydrogen residual."""
+    els = registry(REGISTRIES[reg])
     params, nH = hii_thermal(n=8, nphotons=20000, metals=True)
-    params.update(solve_te=0, charge_exchange=cx, metal_ne=metal_ne)
-    orc, gm = oracle_model(params, nH), gpu_model(params, nH)
+    params.update(solve_te=0, charge_exchange=cx, metal_ne=metal_ne,
+                  metal_eth=metal_thresholds(els))
+    orc, gm = oracle_model(params, nH, elements=els), gpu_model(params, nH, elements=els)
     x = _core(8)
     orc.set_state(xHI=x)
     orc.opacity_fill()
@@ -99,7 +109,7 @@ def test_metal_equilibrium_update_matches_oracle(cx, metal_ne):
         ok, err = rel_close(sg[k], so[k], 1e-8, floor_frac=1e-10)
         assert ok, (k, err)
     np.testing.assert_allclose(mg, mo, rtol=1e-8, atol=1e-12)
-    for ie, el in enumerate(elements_of(params)):
+    for ie, el in enumerate(els):
         ok, err = rel_close(gm.species_fractions(ie), orc.species_fractions(ie), 1e-7,
                             floor_frac=1e-10)
         assert ok, (el["name"], err)
@@ -154,8 +164,26 @@ def test_metal_warm_start_matches_bisection():
     b.set_warm_start(MC_WARM_NEWTON)
     a.gas_thermal_update()
     b.gas_thermal_update()
-    ta, tb = a.state()["Te"], b.state()["Te"]
+    sa, sb = a.state(), b.state()
+    ta, tb = sa["Te"], sb["Te"]
     rel = np.abs(tb / ta - 1.0)
     print("warm vs bisection: max", rel.max(), "within 2.1e-5:", np.mean(rel < 2.1e-5),
           "stats", a.thermal_stats(), b.thermal_stats())
     assert rel.max() < 5e-3 and np.mean(rel < 2.1e-5) > 0.99
+    for m in (a, b):              # the work counters are consistent
+        st = m.thermal_stats()
+        assert 0 < st["bracketed"] <= st["leaves"] == m.nleaf, st
+        assert 4 * st["bracketed"] <= st["evaluations"] <= 64 * st["bracketed"], st
+        assert 0 <= st["fallbacks"] <= st["bracketed"], st
+    # leaves whose hydrogen is resolved: the same ionisation state; leaves
+    # without it (where the warm form stops the n_e iteration once it is
+    # inside its noise band) stay inside that band
+    res = (1.0 - sa["xHI"] >= 1e-8) & (rel < 2.1e-5)
+    assert res.sum() > 100
+    ok, err = rel_close(1.0 - sb["xHI"][res], 1.0 - sa["xHI"][res], 1e-3, floor_frac=1e-6)
+    assert ok, err
+    ok, err = rel_close(sb["ne"][res], sa["ne"][res], 1e-3, floor_frac=1e-6)
+    assert ok, err
+    dark = 1.0 - sa["xHI"] < 1e-10
+    assert np.all(1.0 - sb["xHI"][dark] < 1e-9)
+    assert np.all(sb["ne"][dark] < 1e-6 * sb["nH"][dark])
