@@ -372,6 +372,7 @@ struct Lane {
     float px, py, pz, fkx, fky, fkz, fsHI, fsHeI, fsHeII, fdust;
     float fexpo, ftau, ftau_in, ftcur;
     int nmet;                // metal transitions below the packet's energy
+    double rhe, rE;          // 1 / (E ev2erg) and 1 / E of the packet (metal tallies)
     // tally of this step, scored after the step by the whole wave
     int s_il;
     double s_pl;
@@ -465,12 +466,13 @@ __device__ __forceinline__ void tally_rows(const KArgs &a, int il, double *&h, d
 }
 
 // species_rate_path_score coefficients of transition k (threshold order)
-// for a packet of energy E (src/species_mod.f90:329-343)
-__device__ __forceinline__ void metal_coeff(int k, double E, double &c1, double &c2)
+// for the lane's packet (src/species_mod.f90:329-343), with the packet's
+// reciprocals of E taken once at load
+__device__ __forceinline__ void metal_coeff(int k, const Lane &L, double &c1, double &c2)
 {
     const double sg = msig(k);
-    c1 = sg / (E * MC_EV2ERG);
-    c2 = sg * fmax(1.0 - c_mt.flat_eth[k] / E, 0.0);
+    c1 = sg * L.rhe;
+    c2 = sg * fmax(1.0 - c_mt.flat_eth[k] * L.rE, 0.0);
 }
 
 // one lane's segment: 1 band-J atomic + one per nonzero rate coefficient
@@ -489,7 +491,7 @@ __device__ __forceinline__ void tally_one(const KArgs &a, const Lane &L, int il,
         double *r = a.mraw + (size_t)il * a.ntr * 2;
         for (int k = 0; k < L.nmet; ++k) {
             double c1, c2;
-            metal_coeff(k, L.E, c1, c2);
+            metal_coeff(k, L, c1, c2);
             unsafeAtomicAdd(r + 2 * k, pl * c1);
             if (c2 != 0.0) unsafeAtomicAdd(r + 2 * k + 1, pl * c2);
         }
@@ -550,7 +552,7 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
                     const bool on = in && k < L.nmet;
                     if (!__ballot(on)) break;
                     double c1 = 0.0, c2 = 0.0;
-                    if (on) metal_coeff(k, L.E, c1, c2);
+                    if (on) metal_coeff(k, L, c1, c2);
                     const double s1 = wave_sum(on ? L.s_pl * c1 : 0.0);
                     const double s2 = wave_sum(on ? L.s_pl * c2 : 0.0);
                     if (lane == leader) {
@@ -1137,6 +1139,8 @@ __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, 
             n = k + 1;
         }
         L.nmet = n;
+        L.rhe = 1.0 / he;
+        L.rE = 1.0 / L.E;
     }
     L.wgt = 1.0;
     L.nscatt = 0;
