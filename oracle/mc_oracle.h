@@ -195,6 +195,27 @@ void   orc_milne_setup(orc_milne *m, double eion_max);
 double orc_milne_energy_per_recomb(const orc_milne *m, int ich, double T);
 double orc_milne_sample_energy(const orc_milne *m, int ich, double T, double u);
 
+/* ---- dust thermal-infrared band (src/dust_temp_mod.f90,
+ * src/dust_emis_transport_mod.f90, src/dust_ir_band_mod.f90) ------------- */
+#define ORC_DT_NT      61        /* femit temperature knots             */
+#define ORC_NLAM_BB    300       /* equilibrium emission grid           */
+#define ORC_LAM_BB_MIN 0.1       /* um */
+#define ORC_LAM_BB_MAX 3.0e3     /* um */
+#define ORC_IR_NREPORT 10
+typedef struct {
+    double  tgrid[ORC_DT_NT], femit[ORC_DT_NT];
+    int     temp_ready, setup_ready;
+    int     nl, nzero;           /* emission grid, leading nodes zeroed */
+    double *lam, *sabs, *edge;   /* nl, nl, nl+1 [um] */
+    double  lam_hardest, frac_discarded;
+    double *tdust;               /* nleaf */
+    double *irabs, *heat_ir;     /* nleaf: path tally, heating */
+    double *lesc;                /* nl: escaped luminosity per bin */
+    int64_t *khi;                /* nleaf: last packet index of each leaf */
+    double  emitted, extinct, escaped, Ltot, Lpacket, xi;
+    int64_t seg_direct, seg_scattered;
+} orc_dustir;
+
 typedef struct {
     orc_params p;
     orc_qmc    qmc;
@@ -242,6 +263,8 @@ typedef struct {
     int     flat_ie[ORC_MAX_TR], flat_it[ORC_MAX_TR];
     double  flat_eth[ORC_MAX_TR];
     double *mraw, *mgam, *mheat, *mfrac;
+    /* dust thermal-infrared band, NULL until its setup */
+    orc_dustir *ir;
 } orc_ctx;
 
 orc_ctx *orc_create(const orc_params *p);
@@ -298,6 +321,26 @@ double   orc_charge_neutrality_ne(const orc_ctx *c, int il, double T, double nH,
 void     orc_solve_ion_cell_leaf(const orc_ctx *c, int il, double gH, double gHe1, double gHe2,
                                  double nH, double T, int caseA, double yHe, double *xHI,
                                  double *xHeI, double *xHeII, double *ne);
+
+/* dust thermal-infrared band (orc_dustir.c; the pass in orc_transport.c) */
+int      orc_dust_temp_setup(orc_ctx *c);
+double   orc_dust_femit_of(const orc_ctx *c, double T);
+void     orc_dust_temp_compute(orc_ctx *c, const double *heat);
+void     orc_dust_heat_compute(const orc_ctx *c, double *heat);
+int      orc_dust_emis_transport_setup(orc_ctx *c);
+double   orc_ir_leaf_spectrum(const orc_ctx *c, int il, double *pdf, double *cdf,
+                              double *fdisc);
+double   orc_sample_linear_bin(double lam0, double lam1, double p0, double p1, double u);
+double   orc_ir_counts(orc_ctx *c, int pass, int64_t N, const double *heat, double *xi);
+void     orc_ir_launch_state(const orc_ctx *c, int il, const double *pdf, const double *cdf,
+                             uint64_t key, double *out7);
+int      orc_ir_transport(orc_ctx *c, int pass, int64_t N, int nthreads);
+int      orc_transport_dust_emission(orc_ctx *c, double *heat, int64_t npacket, int niter,
+                                     double tol, int nthreads, double *report);
+int      orc_dust_ir_arrays(const orc_ctx *c, double *tdust, double *heat_ir, double *irabs,
+                            double *lesc, double *lam, double *sabs, double *edge,
+                            double *femit);
+void     orc_dustir_free(orc_dustir *ir);
 
 /* diagnostics: leaves crossed by the direct walk of one ray (no tally) */
 int      orc_trace_ray(orc_ctx *c, double x, double y, double z, double kx,

@@ -39,6 +39,7 @@
 typedef struct {
     double x, y, z, kx, ky, kz;
     double wgt, Lpacket, E;
+    double lam;                        /* wavelength [um] (infrared band) */
     int inu, icell, nscatt, inside;
     int64_t id;
     int istream;
@@ -52,6 +53,11 @@ typedef struct {
     double *hhe, *jt, *dust, *mraw;
     int64_t seg_direct, seg_scattered;
     long double e_emit, e_abs, e_esc;
+    /* infrared band (transported_band == BAND_DUST_IR): grain-absorption path
+     * tally, escaped luminosity per emission-grid bin and its edges */
+    int ir, nl;
+    double *irabs, *lesc;
+    const double *edge;
 } tally_t;
 
 static double rnd(const orc_ctx *c, pkt_t *ph)
@@ -213,6 +219,14 @@ static void score(const orc_ctx *c, tally_t *t, const pkt_t *ph, int il, double 
 {
     if (pl == 0.0) return;
     int i = il - 1;
+    if (t->ir) {
+        /* the infrared branch (src/ion_score_mod.f90:149-158): grain
+         * absorption into the band's own estimator (dust_ir_absorb_path,
+         * src/dust_ir_band_mod.f90:102-107), nothing else */
+        t->irabs[i] += pl * ph->dabs;
+        if (ph->nscatt > 0) t->seg_scattered++; else t->seg_direct++;
+        return;
+    }
     t->jt[(size_t)i * c->nnu_band + ph->inu - 1] += pl;
     if (c->p.add_dust) t->dust[i] += pl * ph->dabs;
     if (ph->nscatt > 0) t->seg_scattered++; else t->seg_direct++;
@@ -245,6 +259,21 @@ static void ledger(tally_t *t, const pkt_t *ph, double tau_edge)
     t->e_emit += (long double)L;
     t->e_abs += (long double)(L * (1.0 - tr));
     t->e_esc += (long double)(L * tr);
+    if (!t->ir) return;
+    /* dust_ir_escape_packet (src/dust_ir_band_mod.f90:123-150): the escaped
+     * luminosity into the bin of the packet's wavelength, the first bin k
+     * with lambda < edge(k+1); outside the outer edges nothing is binned */
+    double lesc = L * tr;
+    int n = t->nl;
+    if (n <= 0 || lesc <= 0.0) return;
+    if (ph->lam <= t->edge[0] || ph->lam >= t->edge[n]) return;
+    int lo = 1, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) / 2;
+        if (ph->lam < t->edge[mid]) hi = mid;
+        else lo = mid + 1;
+    }
+    t->lesc[lo - 1] += lesc;
 }
 
 /* ---- Cartesian DDA (src/raytrace_amr.f90:495-676) ----------------------- */
@@ -691,6 +720,151 @@ static void *worker(void *arg)
         transport(w->c, &w->t, &ph);
     }
     return NULL;
+}
+
+/* ---- dust thermal-infrared band ------------------------------------------ */
+
+/* launch_and_transport's launch (src/dust_emis_transport_mod.f90:489-544):
+ * uniform position in leaf il (0-based), isotropic direction, wavelength bin
+ * from the leaf's cumulative by binary search, then continuously inside the
+ * bin; draws 0-5 of the packet's key on stream 3.
+ * out = x, y, z, kx, ky, kz, lambda [um] */
+void orc_ir_launch_state(const orc_ctx *c, int il, const double *pdf, const double *cdf,
+                         uint64_t key, double *out)
+{
+    const orc_dustir *ir = c->ir;
+    const uint32_t seed = (uint32_t)c->p.qmc_seed;
+    double cx, cy, cz, half;
+    cell_geom(c, leaf_cell(c, il + 1), &cx, &cy, &cz, &half);
+    out[0] = cx + (2.0 * orc_rng_uniform(seed, 3, key, 0) - 1.0) * half;
+    out[1] = cy + (2.0 * orc_rng_uniform(seed, 3, key, 1) - 1.0) * half;
+    out[2] = cz + (2.0 * orc_rng_uniform(seed, 3, key, 2) - 1.0) * half;
+    double cost = 2.0 * orc_rng_uniform(seed, 3, key, 3) - 1.0;
+    double sint = sqrt(1.0 - cost * cost);
+    double phi = ORC_TWOPI * orc_rng_uniform(seed, 3, key, 4);
+    out[3] = sint * cos(phi);
+    out[4] = sint * sin(phi);
+    out[5] = cost;
+    int n = ir->nl;
+    double u = orc_rng_uniform(seed, 3, key, 5) * cdf[n - 1];
+    int lo = 1, hi = n - 1;                  /* 1-based bins */
+    while (lo < hi) {
+        int mid = (lo + hi) / 2;
+        if (u <= cdf[mid]) hi = mid;         /* cdf(mid+1), 1-based */
+        else lo = mid + 1;
+    }
+    int k = lo - 1;                          /* 0-based lower node */
+    double ulo = cdf[k], ubin = 0.0;
+    if (cdf[k + 1] > ulo) ubin = (u - ulo) / (cdf[k + 1] - ulo);
+    out[6] = orc_sample_linear_bin(ir->lam[k], ir->lam[k + 1], pdf[k], pdf[k + 1], ubin);
+}
+
+typedef struct {
+    orc_ctx *c;
+    tally_t t;
+    int pass;
+    int64_t k0, k1;                          /* packet indices, inclusive */
+    double fdisc;
+} irwork_t;
+
+/* the leaves' packets of one pass (src/dust_emis_transport_mod.f90:267-284):
+ * leaf of packet k = the first leaf with khi >= k; its spectrum is rebuilt
+ * when the leaf changes; leaves whose spectrum is empty drop their packets */
+static void *ir_worker(void *arg)
+{
+    irwork_t *w = (irwork_t *)arg;
+    const orc_ctx *c = w->c;
+    const orc_dustir *ir = c->ir;
+    if (w->k1 < w->k0) return NULL;
+    double *pdf = (double *)malloc(sizeof(double) * ir->nl);
+    double *cdf = (double *)malloc(sizeof(double) * ir->nl);
+    int lo = 0, hi = c->nleaf - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) / 2;
+        if (ir->khi[mid] >= w->k0) hi = mid; else lo = mid + 1;
+    }
+    int il = lo, cur = -1;
+    double esum = 0.0;
+    for (int64_t k = w->k0; k <= w->k1; ++k) {
+        while (ir->khi[il] < k) ++il;
+        if (il != cur) {
+            double fd;
+            esum = orc_ir_leaf_spectrum(c, il, pdf, cdf, &fd);
+            if (esum > 0.0 && fd > w->fdisc) w->fdisc = fd;
+            cur = il;
+        }
+        if (!(esum > 0.0)) continue;
+        uint64_t key = ((uint64_t)w->pass << 40) | (uint64_t)k;
+        double o[7];
+        orc_ir_launch_state(c, il, pdf, cdf, key, o);
+        pkt_t ph;
+        memset(&ph, 0, sizeof(ph));
+        ph.x = o[0]; ph.y = o[1]; ph.z = o[2];
+        ph.kx = o[3]; ph.ky = o[4]; ph.kz = o[5];
+        ph.lam = o[6];
+        ph.E = 1.23984 / o[6];
+        ph.inu = 1;
+        ph.icell = il + 1;
+        ph.nscatt = 0;
+        ph.wgt = 1.0;
+        ph.inside = 1;
+        ph.Lpacket = ir->Lpacket;
+        ph.id = (int64_t)key;
+        ph.istream = 3;
+        ph.nk = 6;
+        transport(c, &w->t, &ph);
+    }
+    free(pdf); free(cdf);
+    return NULL;
+}
+
+/* one infrared pass: packets 1..N of the counts in ir->khi, launched and
+ * carried by the shared transport with the band's estimators
+ * (dust_ir_band_reset / _reduce, src/dust_ir_band_mod.f90:87-95, 153-174) */
+int orc_ir_transport(orc_ctx *c, int pass, int64_t N, int nthreads)
+{
+    orc_dustir *ir = c->ir;
+    int nl = c->nleaf, nb = ir->nl;
+    if (nthreads < 1) nthreads = 1;
+    irwork_t *w = (irwork_t *)calloc(nthreads, sizeof(irwork_t));
+    pthread_t *th = (pthread_t *)calloc(nthreads, sizeof(pthread_t));
+    memset(ir->irabs, 0, sizeof(double) * nl);
+    memset(ir->lesc, 0, sizeof(double) * nb);
+    for (int i = 0; i < nthreads; ++i) {
+        w[i].c = c;
+        w[i].pass = pass;
+        w[i].k0 = 1 + N * i / nthreads;
+        w[i].k1 = N * (i + 1) / nthreads;
+        w[i].t.ir = 1;
+        w[i].t.nl = nb;
+        w[i].t.edge = ir->edge;
+        if (i == 0) {
+            w[i].t.irabs = ir->irabs;
+            w[i].t.lesc = ir->lesc;
+        } else {
+            w[i].t.irabs = (double *)calloc((size_t)nl, sizeof(double));
+            w[i].t.lesc = (double *)calloc((size_t)nb, sizeof(double));
+        }
+    }
+    for (int i = 1; i < nthreads; ++i) pthread_create(&th[i], NULL, ir_worker, &w[i]);
+    ir_worker(&w[0]);
+    for (int i = 1; i < nthreads; ++i) pthread_join(th[i], NULL);
+    long double ee = w[0].t.e_emit, ea = w[0].t.e_abs, es = w[0].t.e_esc;
+    int64_t sd = w[0].t.seg_direct, ss = w[0].t.seg_scattered;
+    double fd = w[0].fdisc;
+    for (int i = 1; i < nthreads; ++i) {
+        for (int k = 0; k < nl; ++k) ir->irabs[k] += w[i].t.irabs[k];
+        for (int k = 0; k < nb; ++k) ir->lesc[k] += w[i].t.lesc[k];
+        ee += w[i].t.e_emit; ea += w[i].t.e_abs; es += w[i].t.e_esc;
+        sd += w[i].t.seg_direct; ss += w[i].t.seg_scattered;
+        if (w[i].fdisc > fd) fd = w[i].fdisc;
+        free(w[i].t.irabs); free(w[i].t.lesc);
+    }
+    ir->emitted = (double)ee; ir->extinct = (double)ea; ir->escaped = (double)es;
+    ir->seg_direct = sd; ir->seg_scattered = ss;
+    if (fd > ir->frac_discarded) ir->frac_discarded = fd;
+    free(w); free(th);
+    return 0;
 }
 
 /* ionizing_field_and_rates (src/ionizing_field_mod.f90:90-164): zero

@@ -192,6 +192,28 @@ def _load():
     lib.orc_energy_cdf_value.argtypes = [C.c_void_p, C.c_double]
     lib.orc_energy_cdf_value.restype = C.c_double
     lib.orc_sampler_free.argtypes = [C.c_void_p]
+    # dust thermal-infrared band
+    for name in ("orc_dust_temp_setup", "orc_dust_emis_transport_setup"):
+        getattr(lib, name).argtypes = [vp]
+        getattr(lib, name).restype = C.c_int
+    lib.orc_dust_femit_of.argtypes = [vp, C.c_double]
+    lib.orc_dust_femit_of.restype = C.c_double
+    lib.orc_dust_temp_compute.argtypes = [vp, _dp]
+    lib.orc_dust_heat_compute.argtypes = [vp, _dp]
+    lib.orc_ir_leaf_spectrum.argtypes = [vp, C.c_int, _dp, _dp, _dp]
+    lib.orc_ir_leaf_spectrum.restype = C.c_double
+    lib.orc_sample_linear_bin.argtypes = [C.c_double] * 5
+    lib.orc_sample_linear_bin.restype = C.c_double
+    lib.orc_ir_counts.argtypes = [vp, C.c_int, C.c_int64, _dp, _dp]
+    lib.orc_ir_counts.restype = C.c_double
+    lib.orc_ir_launch_state.argtypes = [vp, C.c_int, _dp, _dp, C.c_uint64, _dp]
+    lib.orc_ir_transport.argtypes = [vp, C.c_int, C.c_int64, C.c_int]
+    lib.orc_ir_transport.restype = C.c_int
+    lib.orc_transport_dust_emission.argtypes = [vp, _dp, C.c_int64, C.c_int, C.c_double,
+                                                C.c_int, _dp]
+    lib.orc_transport_dust_emission.restype = C.c_int
+    lib.orc_dust_ir_arrays.argtypes = [vp] + [_dp] * 8
+    lib.orc_dust_ir_arrays.restype = C.c_int
     return lib
 
 
@@ -537,6 +559,80 @@ class Model:
         lib().orc_launch_diffuse(self.c, pid, ptr(out))
         return out
 
+    # ---- dust thermal-infrared band -----------------------------------------
+    def dust_temp_setup(self):
+        rc = lib().orc_dust_temp_setup(self.c)
+        if rc:
+            raise RuntimeError(f"orc_dust_temp_setup rc={rc}")
+
+    def femit_of(self, T):
+        return lib().orc_dust_femit_of(self.c, T)
+
+    def dust_heat(self):
+        """Heat_dust of the last ionizing pass (ion_score_apply_dust_heat)."""
+        out = np.zeros(self.nleaf)
+        lib().orc_dust_heat_compute(self.c, ptr(out))
+        return out
+
+    def dust_temp_compute(self, heat):
+        heat = np.ascontiguousarray(heat, dtype=np.float64)
+        lib().orc_dust_temp_compute(self.c, ptr(heat))
+        return self.ir_arrays()["tdust"]
+
+    def dust_emis_transport_setup(self):
+        rc = lib().orc_dust_emis_transport_setup(self.c)
+        if rc:
+            raise RuntimeError(f"orc_dust_emis_transport_setup rc={rc}")
+
+    def _ir_head(self):
+        return C.cast(self._head().ir, C.POINTER(_IrHead)).contents
+
+    def ir_arrays(self):
+        n = self.nleaf
+        ih = self._ir_head()
+        nl = ih.nl
+        out = dict(tdust=np.zeros(n), heat_ir=np.zeros(n), irabs=np.zeros(n),
+                   lesc=np.zeros(nl), lam=np.zeros(nl), sabs=np.zeros(nl),
+                   edge=np.zeros(nl + 1), femit=np.zeros(2 * 61))
+        lib().orc_dust_ir_arrays(self.c, *[ptr(out[k]) for k in
+                                           ("tdust", "heat_ir", "irabs", "lesc", "lam", "sabs",
+                                            "edge", "femit")])
+        out["tgrid"], out["femit"] = out["femit"][:61].copy(), out["femit"][61:].copy()
+        out.update(nzero=ih.nzero, lam_hardest=ih.lam_hardest,
+                   frac_discarded=ih.frac_discarded, emitted=ih.emitted, extinct=ih.extinct,
+                   escaped=ih.escaped, Ltot=ih.Ltot, Lpacket=ih.Lpacket, xi=ih.xi,
+                   seg_direct=ih.seg_direct, seg_scattered=ih.seg_scattered,
+                   khi=np.ctypeslib.as_array(ih.khi, (n,)).copy() if ih.khi else None)
+        return out
+
+    def ir_leaf_spectrum(self, il):
+        nl = self._ir_head().nl
+        pdf, cdf, fd = np.zeros(nl), np.zeros(nl), C.c_double()
+        esum = lib().orc_ir_leaf_spectrum(self.c, il, ptr(pdf), ptr(cdf), C.byref(fd))
+        return pdf, cdf, esum, fd.value
+
+    def ir_counts(self, npass, N, heat):
+        heat = np.ascontiguousarray(heat, dtype=np.float64)
+        xi = C.c_double()
+        L = lib().orc_ir_counts(self.c, npass, N, ptr(heat), C.byref(xi))
+        return L, xi.value, self.ir_arrays()["khi"]
+
+    def ir_launch(self, il, pdf, cdf, key):
+        out = np.zeros(7)
+        lib().orc_ir_launch_state(self.c, il, ptr(np.ascontiguousarray(pdf)),
+                                  ptr(np.ascontiguousarray(cdf)), key, ptr(out))
+        return out
+
+    def transport_dust_emission(self, heat, npacket=1000000, niter=2, tol=1e-3, nthreads=4):
+        """transport_dust_emission: returns (heat_dust out, report dict)."""
+        heat = np.array(heat, dtype=np.float64)
+        rep = np.zeros(10)
+        rc = lib().orc_transport_dust_emission(self.c, ptr(heat), npacket, niter, tol, nthreads,
+                                               ptr(rep))
+        if rc:
+            raise RuntimeError(f"orc_transport_dust_emission rc={rc}")
+        return heat, ir_report(rep)
+
     def __del__(self):
         try:
             if self.c:
@@ -580,7 +676,36 @@ class _CtxHead(C.Structure):
         ("tr_off", C.c_int * 11), ("st_off", C.c_int * 11),
         ("flat_ie", C.c_int * 64), ("flat_it", C.c_int * 64), ("flat_eth", C.c_double * 64),
         ("mraw", _dp), ("mgam", _dp), ("mheat", _dp), ("mfrac", _dp),
+        ("ir", C.c_void_p),
     ]
+
+
+class _IrHead(C.Structure):
+    """Mirror of orc_dustir (mc_oracle.h)."""
+    _fields_ = [
+        ("tgrid", C.c_double * 61), ("femit", C.c_double * 61),
+        ("temp_ready", C.c_int), ("setup_ready", C.c_int),
+        ("nl", C.c_int), ("nzero", C.c_int),
+        ("lam", _dp), ("sabs", _dp), ("edge", _dp),
+        ("lam_hardest", C.c_double), ("frac_discarded", C.c_double),
+        ("tdust", _dp), ("irabs", _dp), ("heat_ir", _dp), ("lesc", _dp),
+        ("khi", C.POINTER(C.c_int64)),
+        ("emitted", C.c_double), ("extinct", C.c_double), ("escaped", C.c_double),
+        ("Ltot", C.c_double), ("Lpacket", C.c_double), ("xi", C.c_double),
+        ("seg_direct", C.c_int64), ("seg_scattered", C.c_int64),
+    ]
+
+
+IR_REPORT_KEYS = ("passes", "converged", "L_emit", "emitted", "extinguished", "escaped",
+                  "closure", "L_abs_path", "drel", "frac_discarded")
+
+
+def ir_report(rep):
+    return dict(zip(IR_REPORT_KEYS, [float(v) for v in rep]))
+
+
+def sample_linear_bin(lam0, lam1, p0, p1, u):
+    return lib().orc_sample_linear_bin(lam0, lam1, p0, p1, u)
 
 
 def solve_ion_cell(gH, gHe1, gHe2, nH, T, caseA, yHe, xHI, xHeI, xHeII):
