@@ -80,6 +80,24 @@ extern "C" {
 #define MC_COOL_LOCAL_NE    1   /* fits x the n-level (T, n_e) suppression:
                                    needs the n-level tables, not built here  */
 
+/* dust thermal-infrared band: per-leaf fields of mc_dust_get / mc_dust_set */
+#define MC_DUST_HEAT    0   /* Heat_dust [erg/s/cm^3] (gas_rates_mod heat_dust)     */
+#define MC_DUST_TEMP    1   /* T_dust [K] (dust_temp_mod t_dust)                    */
+#define MC_DUST_HEAT_IR 2   /* reabsorbed-infrared heating of the last pass          */
+#define MC_DUST_IR_ABS  3   /* its path tally ir_abs_path (dust_ir_band_mod)        */
+/* report of mc_transport_dust_emission */
+#define MC_DUSTRT_NREPORT 10
+#define MC_DUSTRT_PASSES     0   /* passes run                                      */
+#define MC_DUSTRT_CONVERGED  1   /* 1 when the relative change fell below dust_tol  */
+#define MC_DUSTRT_L_EMIT     2   /* emitted luminosity of the last pass [erg/s]     */
+#define MC_DUSTRT_EMITTED    3   /* IR ledger of the last pass: emitted,           */
+#define MC_DUSTRT_EXTINCT    4   /*   extinguished,                                */
+#define MC_DUSTRT_ESCAPED    5   /*   escaped                                      */
+#define MC_DUSTRT_CLOSURE    6   /* |extinct + escaped - emitted| / emitted         */
+#define MC_DUSTRT_L_ABS_PATH 7   /* grain-absorbed power of the path estimator      */
+#define MC_DUSTRT_DREL       8   /* relative change of the total emitted luminosity */
+#define MC_DUSTRT_FDISC      9   /* largest emission fraction the band cut removed  */
+
 /* learned warm start: network 8 -> 64 -> 64 -> 64 -> 4 (GELU) */
 #define MC_MLP_NPARAM 9180
 
@@ -141,6 +159,10 @@ typedef struct mc_params {
     int32_t charge_exchange;   /* par%charge_exchange: MC_CX_*                 */
     int32_t ci_model;          /* par%ci_model: MC_CI_*                        */
     int32_t cooling_model;     /* par%cooling_model: MC_COOL_*                 */
+    /* dust thermal-infrared band (src/define.f90:422-433) */
+    int32_t dust_niter;        /* par%dust_niter: self-heating passes           */
+    double  dust_no_photons;   /* par%dust_no_photons: packets per pass         */
+    double  dust_tol;          /* par%dust_tol: relative change that stops it   */
 } mc_params;
 
 /* One element of the trace-metal registry, the fields read_element fills
@@ -347,6 +369,61 @@ int mc_set_precision(mc_ctx *ctx, int precision);
 /* Device times of the last pass/update [ms]: {launch, walk, rates, solve}
  * measured with events on the library's stream. */
 int mc_timing(mc_ctx *ctx, double *out4);
+
+/* ---- dust thermal-infrared band --------------------------------------------
+ * Heat_dust is device-resident: every ionizing pass with ion_add_dust on
+ * refreshes it from the dust-absorption path tally (ion_score_apply_dust_heat,
+ * src/ion_score_mod.f90:319-338); mc_dust_set replaces it. */
+
+/* dust_temp_setup (src/dust_temp_mod.f90:39-118): the equilibrium emission
+ * function femit(T) of the extinction table loaded by mc_ion_dust_table. */
+int mc_dust_temp_setup(mc_ctx *ctx);
+
+/* dust_temp_compute (src/dust_temp_mod.f90:145-177): T_dust of every leaf
+ * from the current Heat_dust. */
+int mc_dust_temp_compute(mc_ctx *ctx);
+
+/* dust_emis_transport_setup (src/dust_emis_transport_mod.f90:100-180) for the
+ * equilibrium modified blackbody (par%dust_sed off): the 0.1-3000 um
+ * emission grid, its cut below the band floor and every active gas
+ * threshold, and the escaped-spectrum bins (dust_ir_band_setup,
+ * src/dust_ir_band_mod.f90:63-84).  Needs ion_add_dust and the femit table. */
+int mc_dust_emis_transport_setup(mc_ctx *ctx);
+
+/* transport_dust_emission (src/dust_emis_transport_mod.f90:208-366): up to
+ * dust_niter passes of dust_no_photons packets launched from every leaf in
+ * proportion to Heat_dust V (stratified counts, continuous wavelengths),
+ * carried by the shared walk with grain absorption and scattering and the
+ * infrared estimators (src/ion_score_mod.f90:149-158, 236-244); after each
+ * pass Heat_dust = EUV part + reabsorbed infrared and T_dust follows.
+ * report: MC_DUSTRT_NREPORT values (may be NULL).  Returns an error when the
+ * infrared energy closure exceeds 5e-13 (the reference aborts). */
+int mc_transport_dust_emission(mc_ctx *ctx, double *report);
+
+/* Per-leaf dust fields (MC_DUST_*) of the band; mc_dust_set accepts
+ * MC_DUST_HEAT only. */
+int mc_dust_get(mc_ctx *ctx, int field, double *dst);
+int mc_dust_set(mc_ctx *ctx, int field, const double *src);
+
+/* dust_ir_energy_budget (src/dust_ir_band_mod.f90:195-199): the infrared
+ * ledger of the last pass {emitted, extinguished, escaped} [erg/s]. */
+int mc_dust_ir_energy_budget(mc_ctx *ctx, double *out3);
+
+/* dust_ir_escaped_spectrum (src/dust_ir_band_mod.f90:204-214): bin centres
+ * lam [um], L_lambda [erg/s/um] and bin widths of the escaped infrared, nl
+ * values each (any pointer may be NULL). */
+int mc_dust_ir_escaped_spectrum(mc_ctx *ctx, double *lam, double *Llam, double *dlam);
+
+/* The band's set-up: emission grid size *nl, its nodes lam [um] and s_abs
+ * (nl each), the escaped-spectrum bin edges (nl + 1), the number of leading
+ * nodes the band cut zeroes, the shortest emitted wavelength, and the femit
+ * table {tgrid(61), femit(61)} (pointers may be NULL).  Diagnostic. */
+int mc_dust_ir_grid(mc_ctx *ctx, int32_t *nl, double *lam, double *sabs, double *edge,
+                    int32_t *nzero, double *lam_hardest, double *femit122);
+
+/* Device times of the last mc_transport_dust_emission [ms]: {launch, walk,
+ * other}, and its segment counts {direct, scattered}. */
+int mc_dust_ir_timing(mc_ctx *ctx, double *out3, int64_t *segments2);
 
 /* Diagnostic: leaves crossed by the direct walk of one ray to the box edge
  * (geometry only), in the given precision form.  Writes up to maxn leaf

@@ -168,6 +168,7 @@ static int alloc_metal_arrays(mc_ctx *c)
 static void free_leaf_arrays(mc_ctx *c)
 {
     free_metal_arrays(c);
+    mc_ir_free(c);
     DevState &s = c->st;
     for (void *p : {(void *)s.hot_slot, (void *)s.hot_leaf, (void *)s.hot_hhe, (void *)s.hot_jt})
         if (p) (void)hipFree(p);
@@ -176,11 +177,11 @@ static void free_leaf_arrays(mc_ctx *c)
     s.nhot = 0;
     for (void *p : {(void *)s.nH, (void *)s.xHI, (void *)s.xHeI, (void *)s.xHeII, (void *)s.ne,
                     (void *)s.Te, (void *)s.rhokap, (void *)s.opac, (void *)s.hhe, (void *)s.jt,
-                    (void *)s.rates, (void *)s.partials})
+                    (void *)s.rates, (void *)s.partials, (void *)s.heat_dust})
         if (p) (void)hipFree(p);
     s.nH = s.xHI = s.xHeI = s.xHeII = s.ne = s.Te = s.rhokap = nullptr;
     s.opac = nullptr;
-    s.hhe = s.jt = s.rates = s.partials = nullptr;
+    s.hhe = s.jt = s.rates = s.partials = s.heat_dust = nullptr;
     for (void *p : {(void *)c->d_dif, (void *)c->d_dcdf, (void *)c->d_bsum})
         if (p) (void)hipFree(p);
     c->d_dif = c->d_dcdf = c->d_bsum = nullptr;
@@ -204,6 +205,10 @@ static int alloc_leaf_arrays(mc_ctx *c)
         if (dalloc(&c->d_dif, 4 * n) || dalloc(&c->d_dcdf, n) || dalloc(&c->d_bsum, nb + 1))
             return -100;
     }
+    if (c->p.ion_add_dust) {
+        if (dalloc(&s.heat_dust, n)) return -100;
+        MC_HIP(hipMemset(s.heat_dust, 0, sizeof(double) * n));
+    }
     MC_HIP(hipMemset(s.rhokap, 0, sizeof(double) * n));
     MC_HIP(hipMemset(s.hhe, 0, sizeof(double) * n * 8));
     MC_HIP(hipMemset(s.jt, 0, sizeof(double) * n * c->nnu));
@@ -212,6 +217,37 @@ static int alloc_leaf_arrays(mc_ctx *c)
     c->state_ready = false;
     if (alloc_metal_arrays(c)) return -100;
     return setup_hot(c);
+}
+
+// launch-record buffers and the direction-sort workspace for batches of
+// min(n, 4M) packets; grown (never shrunk) when a band needs larger batches
+int mc_packet_capacity(mc_ctx *c, int64_t n)
+{
+    const int64_t cap = std::min<int64_t>(std::max<int64_t>(n, 1), (int64_t)1 << 22);
+    if (c->pk.x && c->pk.cap >= cap) return 0;
+    PacketBuf &pk = c->pk;
+    for (void *q : {(void *)pk.x, (void *)pk.y, (void *)pk.z, (void *)pk.kx, (void *)pk.ky,
+                    (void *)pk.kz, (void *)pk.E, (void *)pk.sig, (void *)pk.inu, (void *)pk.leaf,
+                    (void *)pk.key, (void *)c->d_key_sorted, (void *)c->d_perm, c->d_sort_tmp})
+        if (q) (void)hipFree(q);
+    pk.x = pk.y = pk.z = pk.kx = pk.ky = pk.kz = pk.E = pk.sig = nullptr;
+    pk.inu = pk.leaf = nullptr;
+    pk.key = c->d_key_sorted = nullptr;
+    c->d_perm = nullptr;
+    c->d_sort_tmp = nullptr;
+    pk.cap = cap;
+    if (dalloc(&pk.x, cap) || dalloc(&pk.y, cap) || dalloc(&pk.z, cap) || dalloc(&pk.kx, cap) ||
+        dalloc(&pk.ky, cap) || dalloc(&pk.kz, cap) || dalloc(&pk.E, cap) ||
+        dalloc(&pk.sig, 6 * cap) || dalloc(&pk.inu, cap) || dalloc(&pk.leaf, cap) ||
+        dalloc(&pk.key, cap) || dalloc(&c->d_key_sorted, cap) || dalloc(&c->d_perm, cap))
+        return -100;
+    c->sort_tmp_bytes = mc_sort_temp_bytes(cap);
+    if (hipMalloc(&c->d_sort_tmp, std::max<size_t>(c->sort_tmp_bytes, 16)) != hipSuccess) {
+        c->d_sort_tmp = nullptr;
+        mc_set_error("sort workspace allocation failed");
+        return -100;
+    }
+    return 0;
 }
 
 extern "C" int mc_init(const mc_params *p, mc_ctx **out)
@@ -279,20 +315,8 @@ extern "C" int mc_init(const mc_params *p, mc_ctx **out)
         return -100;
     }
     // launch-record buffers: batches of up to 4M packets
-    const int64_t cap = std::min<int64_t>(p->nphotons, (int64_t)1 << 22);
-    c->pk.cap = cap;
-    if (dalloc(&c->pk.x, cap) || dalloc(&c->pk.y, cap) || dalloc(&c->pk.z, cap) ||
-        dalloc(&c->pk.kx, cap) || dalloc(&c->pk.ky, cap) || dalloc(&c->pk.kz, cap) ||
-        dalloc(&c->pk.E, cap) || dalloc(&c->pk.sig, 6 * cap) || dalloc(&c->pk.inu, cap) ||
-        dalloc(&c->pk.leaf, cap) || dalloc(&c->pk.key, cap) || dalloc(&c->d_key_sorted, cap) ||
-        dalloc(&c->d_perm, cap)) {
-        mc_finalize(c);
-        return -100;
-    }
     c->packet_order = MC_ORDER_DIRECTION;
-    c->sort_tmp_bytes = mc_sort_temp_bytes(cap);
-    if (hipMalloc(&c->d_sort_tmp, std::max<size_t>(c->sort_tmp_bytes, 16)) != hipSuccess) {
-        mc_set_error("mc_init: sort workspace allocation failed");
+    if (mc_packet_capacity(c, p->nphotons)) {
         mc_finalize(c);
         return -100;
     }

@@ -114,6 +114,36 @@ struct DevMetalTab {
     double  s_A[MC_GPU_MAXST][MC_MAX_COOLFIT], s_T[MC_GPU_MAXST][MC_MAX_COOLFIT];
 };
 
+// Metal photoionisation cross sections [cm^2] (src/photo_xsec.f90:46-89):
+// VFKY96 and VY95 forms; host (band cut of the infrared band) and device
+// (packet cross sections) use the same code.
+__host__ __device__ inline double m_vfky96(double E, const double *p)
+{
+    if (E < p[0]) return 0.0;
+    const double x = E / p[1] - p[6];
+    const double z = sqrt(x * x + p[7] * p[7]);
+    const double Q = 5.5 - 0.5 * p[4];
+    const double xm = x - 1.0;
+    const double Fy = (xm * xm + p[5] * p[5]) * pow(z, -Q) * pow(1.0 + sqrt(z / p[3]), -p[4]);
+    return p[2] * Fy * 1.0e-18;
+}
+
+__host__ __device__ inline double m_vy95(double E, const double *p)
+{
+    if (E < p[0]) return 0.0;
+    const double y = E / p[1];
+    const double Q = 5.5 + p[6] - 0.5 * p[4];
+    const double ym = y - 1.0;
+    const double Fy = (ym * ym + p[5] * p[5]) * pow(y, -Q) * pow(1.0 + sqrt(y / p[3]), -p[4]);
+    return p[2] * Fy * 1.0e-18;
+}
+
+// species_sigma (src/species_mod.f90:763-776)
+__host__ __device__ inline double m_sigma(const DevMetalTab *mt, int t, double E)
+{
+    return mt->photo_form[t] == 2 ? m_vy95(E, mt->photo[t]) : m_vfky96(E, mt->photo[t]);
+}
+
 // Small launch/band tables shared by every kernel of a context
 // (read through the scalar/constant cache).
 struct DevTables {
@@ -183,6 +213,38 @@ struct DevState {
     double *hot_hhe;   // nrep * nhot * 8
     double *hot_jt;    // nrep * nhot * nnu
     int32_t nhot, nrep;
+    double *heat_dust; // nleaf: Heat_dust (ion_add_dust on), else nullptr
+};
+
+// Dust thermal-infrared band (dust_temp_mod, dust_emis_transport_mod,
+// dust_ir_band_mod): the femit table, the equilibrium emission grid with its
+// band cut, and the per-leaf arrays of the self-heating iteration.
+#define MC_DT_NT 61
+#define MC_NLAM_BB 300
+struct IrBand {
+    bool temp_ready = false, setup_ready = false;
+    double tgrid[MC_DT_NT], femit[MC_DT_NT];
+    int nl = 0, nzero = 0;
+    std::vector<double> lam, sabs, edge;      // nl, nl, nl + 1 [um]
+    double lam_hardest = 0.0;
+    double *d_dt = nullptr;                   // tgrid | femit
+    double *d_lam = nullptr, *d_sabs = nullptr, *d_edge = nullptr;
+    double *d_tdust = nullptr, *d_heat_euv = nullptr, *d_heat_ir = nullptr;
+    double *d_irabs = nullptr;                // grain-absorption path tally
+    double *d_lum = nullptr, *d_lcum = nullptr, *d_bsum = nullptr;   // leaf emission scan
+    int64_t *d_khi = nullptr;                 // last packet index of each leaf
+    int64_t *d_kmax = nullptr;                // tile maxima of its prefix maximum
+    double *d_lesc = nullptr;                 // escaped luminosity per bin
+    double *d_led = nullptr;                  // emitted, extinguished, escaped
+    double *d_wave = nullptr;                 // per-wave ledger slots of the walk
+    size_t n_wave = 0;
+    unsigned long long *d_segs = nullptr;     // direct, scattered, dropped, fdisc bits
+    // last call
+    double led[3] = {0, 0, 0};
+    int64_t segs[2] = {0, 0};
+    double frac_discarded = 0.0;
+    double report[MC_DUSTRT_NREPORT];
+    double times[3] = {0, 0, 0};              // launch, walk, rest [ms] of the last call
 };
 
 struct mc_ctx {
@@ -254,6 +316,8 @@ struct mc_ctx {
     int metal_layout;         // MC_LAYOUT_* of the registry (cell-solve kernels)
     double *d_mraw, *d_mgam, *d_mheat, *d_mfrac, *d_mop;
     float *d_mopf;
+    // dust thermal-infrared band
+    IrBand ir;
     // last pass results
     double ledger[3];
     int64_t segs[3];          // direct, scattered, metal transitions over scored segments
@@ -285,6 +349,15 @@ size_t mc_sort_temp_bytes(int64_t n);
 int mc_sort_records(mc_ctx *c, int64_t n);
 int mc_run_trace(mc_ctx *c, int precision, const double *pos, const double *dir, int32_t *cells,
                  int maxn, int32_t *n);
+int mc_run_dust_temp(mc_ctx *c, const double *d_heat);
+int mc_run_dust_emission(mc_ctx *c, double report[MC_DUSTRT_NREPORT]);
+
+// host set-up of the infrared band (mc_dustir.cpp)
+int mc_ir_femit_table(mc_ctx *c);
+int mc_ir_emission_grid(mc_ctx *c);
+void mc_ir_free(mc_ctx *c);
+// launch-record buffers of at least min(n, 4M) packets (mc_capi.cpp)
+int mc_packet_capacity(mc_ctx *c, int64_t n);
 
 #define MC_HIP(call)                                                          \
     do {                                                                      \

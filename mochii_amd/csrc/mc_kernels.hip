@@ -14,8 +14,10 @@
 //   k_rates       leaf-volume normalisation of the tallies
 //   k_equilibrium per-leaf H/He balance at fixed T, relaxation, metrics,
 //                 and the FP32 opacity inputs for the next pass
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstring>
 
 #include "mc_internal.h"
 
@@ -58,6 +60,13 @@ struct KArgs {
     double *mraw;            // path tallies [(il*ntr + k)*2 + {0,1}]
     const double *mop;       // opacity weights abund x stage fraction [il*ntr + k]
     const float *mopf;       // the same x n_H d2cm in FP32 (mixed form)
+    double Lpk;              // packet luminosity of the band walked
+    // dust infrared band (walk variant M = 2): grain-absorption path tally,
+    // escaped luminosity per emission bin and the bins' edges [um]
+    double *irabs, *ir_lesc;
+    const double *ir_edge;
+    int32_t ir_nl;
+    double *ir_wave;         // per-wave ledger slots of the infrared walk (5 each)
 };
 
 // H/He recombination and collisional-ionisation coefficients at one T
@@ -336,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_launch(KArgs a, int nlds)
 // ---------------------------------------------------------------------------
 // counter-based generator for the post-launch draws (same definition as the
 // CPU oracle): three splitmix64 finalizer rounds over (seed, id, stream|k)
-__device__ __forceinline__ uint64_t mix64(uint64_t z)
+__host__ __device__ inline uint64_t mix64(uint64_t z)
 {
     z ^= z >> 30; z *= 0xBF58476D1CE4E5B9ull;
     z ^= z >> 27; z *= 0x94D049BB133111EBull;
@@ -344,7 +353,7 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z)
     return z;
 }
 
-__device__ __forceinline__ double rng_u(uint32_t seed, int stream, uint64_t id, uint32_t k)
+__host__ __device__ inline double rng_u(uint32_t seed, int stream, uint64_t id, uint32_t k)
 {
     uint64_t h = mix64((uint64_t)seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull);
     h = mix64(h ^ id);
@@ -373,6 +382,7 @@ struct Lane {
     float fexpo, ftau, ftau_in, ftcur;
     int nmet;                // metal transitions below the packet's energy
     double rhe, rE;          // 1 / (E ev2erg) and 1 / E of the packet (metal tallies)
+    double lam;              // wavelength [um] (infrared band)
     // tally of this step, scored after the step by the whole wave
     int s_il;
     double s_pl;
@@ -393,8 +403,15 @@ __device__ __forceinline__ double opacity_d(const KArgs &a, const Lane &L, int i
 {
 #pragma clang fp contract(off)
     const DevState &s = a.s;
+    if constexpr (M == 2) {
+        // infrared band: every gas cross section vanishes below the band cut,
+        // so n_H (0) d2cm + rhokap sigma is rhokap sigma exactly
+        double kap = s.rhokap[il] * L.dabs;
+        if (a.dust_scatter) kap = kap + s.rhokap[il] * L.dsca;
+        return kap;
+    }
     double gas = s.xHI[il] * L.sHI + a.He_abund * (s.xHeI[il] * L.sHeI + s.xHeII[il] * L.sHeII);
-    if constexpr (M != 0) {
+    if constexpr (M == 1) {
         if (a.metal_abs) {
             double metal = 0.0;
             const double *w = a.mop + (size_t)il * a.ntr;
@@ -413,9 +430,10 @@ __device__ __forceinline__ double opacity_d(const KArgs &a, const Lane &L, int i
 template <int M>
 __device__ __forceinline__ float opacity_f(const KArgs &a, const Lane &L, int il)
 {
+    if constexpr (M == 2) return (float)a.s.rhokap[il] * L.fdust;   // = opac.w x fdust
     const float4 o = a.s.opac[il];
     float k = o.x * L.fsHI + o.y * L.fsHeI + o.z * L.fsHeII + o.w * L.fdust;
-    if constexpr (M != 0) {
+    if constexpr (M == 1) {
         if (a.metal_abs) {
             const float *w = a.mopf + (size_t)il * a.ntr;
             for (int q = 0; q < L.nmet; ++q) k = k + w[q] * (float)msig(q);
@@ -428,7 +446,20 @@ struct Counters {
     double e_emit, e_abs, e_esc;
     unsigned long long sd, ss;
     unsigned long long smet;     // metal transitions summed over scored segments
+    // infrared band: packets ledgered (every one at weight 1) and the
+    // rounding errors of the extinguished / escaped sums
+    unsigned long long n_emit;
+    double c_abs, c_esc;
 };
+
+// s + x with its rounding error carried in c (Knuth's TwoSum)
+__device__ __forceinline__ void two_sum_acc(double &s, double &c, double x)
+{
+    const double t = s + x;
+    const double bp = t - s;
+    c += (s - (t - bp)) + (x - bp);
+    s = t;
+}
 
 // score_ion_path: the packet's path quantity into its leaf.  A coefficient
 // that is exactly zero (He I / He II terms below their thresholds) adds +0 to
@@ -487,7 +518,7 @@ __device__ __forceinline__ void tally_one(const KArgs &a, const Lane &L, int il,
     for (int k = 0; k < 6; ++k)
         if (L.coeff[k] != 0.0) unsafeAtomicAdd(h + k, pl * L.coeff[k]);
     if (a.add_dust && L.dabs != 0.0) unsafeAtomicAdd(h + 6, pl * L.dabs);
-    if constexpr (M != 0) {
+    if constexpr (M == 1) {
         double *r = a.mraw + (size_t)il * a.ntr * 2;
         for (int k = 0; k < L.nmet; ++k) {
             double c1, c2;
@@ -518,6 +549,13 @@ constexpr int kAggMin = 3;      // smallest group worth a wave reduction
 template <int M>
 __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, double *jbin)
 {
+    if constexpr (M == 2) {
+        // infrared band (src/ion_score_mod.f90:149-158): grain absorption
+        // into the band's own path tally (dust_ir_absorb_path), nothing else
+        if (L.s_il >= 0) unsafeAtomicAdd(a.irabs + L.s_il, L.s_pl * L.dabs);
+        L.s_il = -1;
+        return;
+    }
     bool has = L.s_il >= 0;
     const int nnu = a.tab->nnu;
     if (a.coherent) {
@@ -546,7 +584,7 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
                     if (lane == leader) unsafeAtomicAdd(h + 6, sum);
                 }
             }
-            if constexpr (M != 0) {
+            if constexpr (M == 1) {
                 double *r = a.mraw + (size_t)lil * a.ntr * 2;
                 for (int k = 0; k < a.ntr; ++k) {
                     const bool on = in && k < L.nmet;
@@ -858,7 +896,7 @@ constexpr int kCarDDA = 0, kFaceExit = 1;
 template <int P, int G>
 __device__ __forceinline__ void start_flight(const KArgs &a, Lane &L, double tau_in)
 {
-    L.wl = L.wgt * a.tab->Lpacket;
+    L.wl = L.wgt * a.Lpk;
     if constexpr (Prec<P>::mixed) {
         if constexpr (G == kCarDDA) dda_init_f(a.g, L);
         else set_dir_f(L);                 // position stays local to the node
@@ -920,14 +958,39 @@ __device__ __forceinline__ void scatter_event(const KArgs &a, Lane &L)
 // end of the direct walk: energy ledger, then the forced first interaction
 // (the direct walk works on a copy of the packet: the forced flight starts
 // again from the launch point, src/raytrace_amr.f90:259-269)
-template <int P, int G>
+template <int P, int G, int M>
 __device__ __forceinline__ void end_edge(const KArgs &a, Lane &L, double tau_edge, Counters &cn)
 {
-    const double lum = a.tab->Lpacket * L.wgt;
+    const double lum = a.Lpk * L.wgt;
     const double tr = exp(-fmax(tau_edge, 0.0));
-    cn.e_emit += lum;
-    cn.e_abs += lum * (1.0 - tr);
-    cn.e_esc += lum * tr;
+    if constexpr (M == 2) {
+        // the reference sums this ledger in extended precision
+        // (ENERGY_ACC_WP, src/dust_ir_band_mod.f90:40-50): compensated sums
+        // here, emitted = packets x L_packet (weight 1 at the ledger)
+        cn.n_emit++;
+        two_sum_acc(cn.e_abs, cn.c_abs, lum * (1.0 - tr));
+        two_sum_acc(cn.e_esc, cn.c_esc, lum * tr);
+    } else {
+        cn.e_emit += lum;
+        cn.e_abs += lum * (1.0 - tr);
+        cn.e_esc += lum * tr;
+    }
+    if constexpr (M == 2) {
+        // dust_ir_escape_packet (src/dust_ir_band_mod.f90:123-150): the
+        // escaped luminosity into the bin of the packet's wavelength, the
+        // first bin k with lambda < edge(k+1); nothing outside the edges
+        const double lesc = lum * tr;
+        const int n = a.ir_nl;
+        if (lesc > 0.0 && L.lam > a.ir_edge[0] && L.lam < a.ir_edge[n]) {
+            int lo = 1, hi = n;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (L.lam < a.ir_edge[mid]) hi = mid;
+                else lo = mid + 1;
+            }
+            unsafeAtomicAdd(a.ir_lesc + (lo - 1), lesc);
+        }
+    }
     if (!(a.add_dust && a.dust_scatter) || tau_edge <= 1.0e-12) { L.phase = 0; return; }
     const double w1 = 1.0 - exp(-tau_edge);
     L.wgt = L.wgt * w1;
@@ -988,7 +1051,7 @@ __device__ __forceinline__ void car_step(const KArgs &a, Lane &L, Counters &cn)
             score(a, L, il, pl, cn);
             L.tau = L.tau + kap * seg;
             if (L.tau >= MC_TAU_HUGE || !dda_advance_d(g, L, ax))
-                end_edge<P, kCarDDA>(a, L, L.tau, cn);
+                end_edge<P, kCarDDA, M>(a, L, L.tau, cn);
         } else {
             if (L.tau + seg * kap >= L.tau_in) {
                 const double ds = kap > 0.0 ? (L.tau_in - L.tau) / kap : seg;
@@ -1013,7 +1076,7 @@ __device__ __forceinline__ void car_step(const KArgs &a, Lane &L, Counters &cn)
             score(a, L, il, seg_lum_f(L, wlc, kc, t, dt), cn);
             L.ftau = L.ftau + dt;
             if (L.ftau >= (float)MC_TAU_HUGE || !cross_f(g, L, ax, t))
-                end_edge<P, kCarDDA>(a, L, (double)L.ftau, cn);
+                end_edge<P, kCarDDA, M>(a, L, (double)L.ftau, cn);
         } else {
             if (L.ftau + dt >= L.ftau_in) {
                 const float ds = kc > 0.0f ? (L.ftau_in - L.ftau) / kc : t;
@@ -1058,10 +1121,10 @@ __device__ __forceinline__ void oct_step(const KArgs &a, Lane &L, Counters &cn)
             }
             score(a, L, il, pl, cn);
             L.tau = L.tau + kap * t;
-            if (L.tau >= MC_TAU_HUGE) { end_edge<P, kFaceExit>(a, L, L.tau, cn); return; }
+            if (L.tau >= MC_TAU_HUGE) { end_edge<P, kFaceExit, M>(a, L, L.tau, cn); return; }
             L.ox = L.ox + t * L.kx; L.oy = L.oy + t * L.ky; L.oz = L.oz + t * L.kz;
             const int nn = next_node_d(g, L.node, f, L.ox, L.oy, L.oz);
-            if (nn < 0) { end_edge<P, kFaceExit>(a, L, L.tau, cn); return; }
+            if (nn < 0) { end_edge<P, kFaceExit, M>(a, L, L.tau, cn); return; }
             L.node = nn;
             L.leaf = leaf_of(g, nn);
         } else {
@@ -1091,7 +1154,7 @@ __device__ __forceinline__ void oct_step(const KArgs &a, Lane &L, Counters &cn)
             score(a, L, il, seg_lum_f(L, wlc, kc, t, dt), cn);
             L.ftau = L.ftau + dt;
             if (L.ftau >= (float)MC_TAU_HUGE || !cross_oct_f(g, L, ax, t))
-                end_edge<P, kFaceExit>(a, L, (double)L.ftau, cn);
+                end_edge<P, kFaceExit, M>(a, L, (double)L.ftau, cn);
         } else {
             if (L.ftau + dt >= L.ftau_in) {
                 const float ds = kc > 0.0f ? (L.ftau_in - L.ftau) / kc : t;
@@ -1115,9 +1178,18 @@ __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, 
     L.id = a.first_id + r;
     L.x = pk.x[r]; L.y = pk.y[r]; L.z = pk.z[r];
     L.kx = pk.kx[r]; L.ky = pk.ky[r]; L.kz = pk.kz[r];
-    L.E = pk.E[r];
     L.inu = pk.inu[r];
     L.leaf = pk.leaf[r];
+    if constexpr (M == 2) {
+        // infrared record: the wavelength [um] in E's slot
+        // (photon%energy_eV = 1.23984 / lambda); leaf -2 marks a packet of a
+        // leaf whose spectrum came back empty, dropped without a ledger entry
+        L.lam = pk.E[r];
+        L.E = 1.23984 / L.lam;
+        if (L.leaf == -2) { L.phase = 0; return; }
+    } else {
+        L.E = pk.E[r];
+    }
     L.leaf0 = L.leaf;
     const double *sg = pk.sig + 6 * r;
     L.sHI = sg[0]; L.sHeI = sg[1]; L.sHeII = sg[2];
@@ -1129,7 +1201,7 @@ __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, 
     L.coeff[3] = L.sHI * fmax(1.0 - MC_ETH_HI / L.E, 0.0);
     L.coeff[4] = L.sHeI * fmax(1.0 - MC_ETH_HEI / L.E, 0.0);
     L.coeff[5] = L.sHeII * fmax(1.0 - MC_ETH_HEII / L.E, 0.0);
-    if constexpr (M != 0) {
+    if constexpr (M == 1) {
         // species_packet_cross_sections (src/species_mod.f90:263-275)
         int n = 0;
         const int ntr = a.ntr;
@@ -1144,8 +1216,8 @@ __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, 
     }
     L.wgt = 1.0;
     L.nscatt = 0;
-    L.nk = 0;
-    L.wl = a.tab->Lpacket;
+    L.nk = M == 2 ? 6 : 0;       // an infrared launch used draws 0-5 of its key
+    L.wl = a.Lpk;
     L.phase = 1;
     if constexpr (Prec<P>::mixed) {
         L.fsHI = (float)L.sHI; L.fsHeI = (float)L.sHeI; L.fsHeII = (float)L.sHeII;
@@ -1161,7 +1233,7 @@ __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, 
         else dda_init_d(a.g, L);
     } else {
         if (L.leaf < 0) {                  // launched outside the box: nothing to walk
-            end_edge<P, G>(a, L, 0.0, cn);
+            end_edge<P, G, M>(a, L, 0.0, cn);
             return;
         }
         L.node = leaf_node_of(a.g, L.leaf);
@@ -1187,7 +1259,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
     L.phase = 0;
     L.s_il = -1;
     L.nmet = 0;
-    Counters cn = {0.0, 0.0, 0.0, 0ull, 0ull, 0ull};
+    Counters cn = {0.0, 0.0, 0.0, 0ull, 0ull, 0ull, 0ull, 0.0, 0.0};
     const int lane = threadIdx.x & 63;
     for (;;) {
         // direction-sorted records: the wave takes 64 consecutive records at
@@ -1206,12 +1278,34 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
             }
         }
         const bool busy = L.phase == 1 || L.phase == 2;
-        if (!__ballot(busy)) break;
+        // launch-order walks refill idle lanes until every lane is drained
+        // (a record may leave its lane idle at once: dropped, or outside)
+        if (!__ballot(busy) && (a.coherent || !__ballot(L.phase == 0))) break;
         if (busy) {
             if constexpr (G == kCarDDA) car_step<P, M>(a, L, cn);
             else oct_step<P, M>(a, L, cn);
         }
         flush_tally<M>(a, L, lane, jbin);
+    }
+    if constexpr (M == 2) {
+        // compensated wave sums of the infrared ledger into this wave's slot
+        // {packets, extinguished, its error, escaped, its error}; the host
+        // adds the slots in extended precision
+        for (int off = 32; off > 0; off >>= 1) {
+            const double oa = __shfl_down(cn.e_abs, off), oca = __shfl_down(cn.c_abs, off);
+            const double oe = __shfl_down(cn.e_esc, off), oce = __shfl_down(cn.c_esc, off);
+            cn.c_abs += oca;
+            two_sum_acc(cn.e_abs, cn.c_abs, oa);
+            cn.c_esc += oce;
+            two_sum_acc(cn.e_esc, cn.c_esc, oe);
+            cn.n_emit += __shfl_down(cn.n_emit, off);
+        }
+        if (lane == 0) {
+            double *w = a.ir_wave + ((size_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * 5;
+            w[0] = (double)cn.n_emit;
+            w[1] = cn.e_abs; w[2] = cn.c_abs;
+            w[3] = cn.e_esc; w[4] = cn.c_esc;
+        }
     }
     // wave reduction of the ledger and segment counts, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
@@ -1222,10 +1316,12 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
         cn.ss += __shfl_down(cn.ss, off);
         cn.smet += __shfl_down(cn.smet, off);
     }
-    if (lane == 0) {
+    if (lane == 0 && M != 2) {
         atomicAdd(a.s.ledger + 0, cn.e_emit);
         atomicAdd(a.s.ledger + 1, cn.e_abs);
         atomicAdd(a.s.ledger + 2, cn.e_esc);
+    }
+    if (lane == 0) {
         atomicAdd(a.s.segs + 0, cn.sd);
         atomicAdd(a.s.segs + 1, cn.ss);
         if (cn.smet) atomicAdd(a.s.segs + 2, cn.smet);
@@ -1284,6 +1380,8 @@ __global__ __launch_bounds__(kBlock) void k_rates(DevGrid g, DevState s, double 
     const double fac = 1.0 / ((w * w) * w * (d2cm * d2cm));
     const double *h = s.hhe + (size_t)il * 8;
     for (int k = 0; k < 6; ++k) s.rates[(size_t)k * g.nleaf + il] = h[k] * fac;
+    // ion_score_apply_dust_heat (src/ion_score_mod.f90:319-338)
+    if (s.heat_dust) s.heat_dust[il] = h[6] * fac * s.rhokap[il] / d2cm;
 }
 
 // ---------------------------------------------------------------------------
@@ -1471,15 +1569,15 @@ __global__ __launch_bounds__(kBlock) void k_diffuse_build(DevGrid g, DevState s,
 // sums in one block -> offset and normalise)
 constexpr int kScanTile = 4 * kBlock;
 
-__global__ __launch_bounds__(kBlock) void k_scan_tiles(const double *dif, int n, double *cdf,
-                                                       double *bsum)
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const double *val, int stride, int n,
+                                                       double *cdf, double *bsum)
 {
     __shared__ double sh[kBlock];
     const int base = blockIdx.x * kScanTile + threadIdx.x * 4;
     double v[4], run = 0.0;
     for (int k = 0; k < 4; ++k) {
         const int i = base + k;
-        run += i < n ? dif[(size_t)i * 4 + 3] : 0.0;
+        run += i < n ? val[(size_t)i * stride] : 0.0;
         v[k] = run;
     }
     sh[threadIdx.x] = run;
@@ -1978,6 +2076,314 @@ __global__ __launch_bounds__(kBlock) void k_stage_refresh(DevGrid g, DevState s,
 }
 
 // diagnostic: cells of one ray's direct walk (single thread)
+
+// ---------------------------------------------------------------------------
+// dust thermal-infrared band (src/dust_temp_mod.f90,
+// src/dust_emis_transport_mod.f90, src/dust_ir_band_mod.f90)
+
+// femit_of (src/dust_temp_mod.f90:132-140): log-log interpolation of the
+// equilibrium emission function on its 61 temperature knots (dt = tgrid |
+// femit)
+__device__ double femit_of(const double *dt, double T)
+{
+#pragma clang fp contract(off)
+    const double *tg = dt, *fe = dt + MC_DT_NT;
+    const double l1 = log10(tg[0]), ln = log10(tg[MC_DT_NT - 1]);
+    const double lt = fmin(fmax(log10(T), l1), ln);
+    const int i = min((int)((lt - l1) / (3.0 / (double)(MC_DT_NT - 1))) + 1, MC_DT_NT - 1);
+    const double li = log10(tg[i - 1]), lj = log10(tg[i]);
+    const double w = (lt - li) / (lj - li);
+    return exp(log(fe[i - 1]) * (1.0 - w) + log(fe[i]) * w);
+}
+
+// dust_temp_compute (src/dust_temp_mod.f90:145-177): per leaf, bisection in
+// log T for femit(T_d) kd = heat_dust, kd = rhokap / distance2cm
+__global__ __launch_bounds__(kBlock) void k_dust_temp(int n, const double *rhokap, const double *heat,
+                                                      double *tdust, const double *dt, double d2cm)
+{
+#pragma clang fp contract(off)
+    const int il = blockIdx.x * kBlock + threadIdx.x;
+    if (il >= n) return;
+    const double kd = rhokap[il] / d2cm, h = heat[il];
+    double td = 0.0;
+    if (kd > 0.0 && h > 0.0) {
+        const double target = h / kd;
+        double tlo = dt[0], thi = dt[MC_DT_NT - 1];
+        if (femit_of(dt, thi) <= target) {
+            td = thi;
+        } else {
+            for (int it = 1; it <= 60; ++it) {
+                const double tm = sqrt(tlo * thi);
+                if (femit_of(dt, tm) < target) tlo = tm;
+                else thi = tm;
+                if (thi / tlo - 1.0 < 1.0e-5) break;
+            }
+            td = sqrt(tlo * thi);
+        }
+    }
+    tdust[il] = td;
+}
+
+// emission luminosity of each leaf, max(heat_dust, 0) V
+// (src/dust_emis_transport_mod.f90:246-250)
+__global__ __launch_bounds__(kBlock) void k_ir_lum(DevGrid g, const double *heat, double d2cm,
+                                                   double *lum)
+{
+#pragma clang fp contract(off)
+    const int il = blockIdx.x * kBlock + threadIdx.x;
+    if (il >= g.nleaf) return;
+    const double w = 2.0 * leaf_half(g, il) * d2cm;
+    lum[il] = fmax(heat[il], 0.0) * ((w * w) * w);
+}
+
+// stratified counts (src/dust_emis_transport_mod.f90:267-276): the last
+// packet index of each leaf, k_hi = ceiling(N C(il)/L_tot - xi) with C the
+// inclusive prefix of the leaf luminosities (tile scan + tile offsets);
+// L_tot is the last leaf's prefix computed the same way, so the last leaf's
+// ratio is 1 exactly and k_hi = N there.  The parallel prefix is not
+// monotone to the last ulp at tile and thread seams, so k_hi is clamped to
+// [0, N] here and made non-decreasing by an exact prefix maximum after:
+// the leaves' ranges [k_hi(il-1)+1, k_hi(il)] then partition 1..N
+__global__ __launch_bounds__(kBlock) void k_ir_counts(int n, const double *lcum, const double *bsum,
+                                                      int64_t N, double xi, int64_t *khi)
+{
+#pragma clang fp contract(off)
+    const int il = blockIdx.x * kBlock + threadIdx.x;
+    if (il >= n) return;
+    const double Ltot = lcum[n - 1] + bsum[(n - 1) / kScanTile];
+    const double v = lcum[il] + bsum[il / kScanTile];
+    const double k = ceil((double)N * (v / Ltot) - xi);
+    khi[il] = k <= 0.0 ? 0 : (k >= (double)N ? N : (int64_t)k);
+}
+
+// inclusive prefix maximum of non-negative integers in place (exact in any
+// order): tiles of kScanTile, then the tiles' exclusive maxima, then apply
+__global__ __launch_bounds__(kBlock) void k_imax_tiles(int64_t *a, int n, int64_t *tmax)
+{
+    __shared__ int64_t sh[kBlock];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * 4;
+    int64_t v[4], run = 0;
+    for (int k = 0; k < 4; ++k) {
+        const int i = base + k;
+        if (i < n && a[i] > run) run = a[i];
+        v[k] = run;
+    }
+    sh[threadIdx.x] = run;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        const int64_t t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        if (t > sh[threadIdx.x]) sh[threadIdx.x] = t;
+        __syncthreads();
+    }
+    const int64_t before = threadIdx.x > 0 ? sh[threadIdx.x - 1] : 0;
+    for (int k = 0; k < 4; ++k) {
+        const int i = base + k;
+        if (i < n) a[i] = v[k] > before ? v[k] : before;
+    }
+    if (threadIdx.x == kBlock - 1) tmax[blockIdx.x] = sh[kBlock - 1];
+}
+
+__global__ __launch_bounds__(1024) void k_imax_sums(int64_t *tmax, int nb)
+{
+    __shared__ int64_t sh[1024];
+    const int per = (nb + 1023) / 1024, lo = threadIdx.x * per, hi = min(lo + per, nb);
+    int64_t run = 0;
+    for (int i = lo; i < hi; ++i) run = tmax[i] > run ? tmax[i] : run;
+    sh[threadIdx.x] = run;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int64_t t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        if (t > sh[threadIdx.x]) sh[threadIdx.x] = t;
+        __syncthreads();
+    }
+    int64_t acc = threadIdx.x > 0 ? sh[threadIdx.x - 1] : 0;
+    for (int i = lo; i < hi; ++i) {
+        const int64_t v = tmax[i];
+        tmax[i] = acc;
+        acc = v > acc ? v : acc;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_imax_apply(int64_t *a, int n, const int64_t *tmax)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int64_t t = tmax[i / kScanTile];
+    if (t > a[i]) a[i] = t;
+}
+
+// B_lambda(T) up to a constant factor, lambda in um
+// (planck_lambda, src/dust_emis_transport_mod.f90:418-429)
+__device__ __forceinline__ double planck_lambda(double lam_um, double T)
+{
+#pragma clang fp contract(off)
+    const double lc = lam_um * 1.0e-4;
+    const double x = 6.62607015e-27 * 2.99792458e10 / (lc * MC_KBOLTZ * T);
+    if (x > 700.0) return 0.0;
+    const double l2 = lc * lc;
+    return 1.0 / ((l2 * l2 * lc) * (exp(x) - 1.0));
+}
+
+// sample_linear_spectrum_bin (src/dust_emis_transport_mod.f90:467-483)
+__device__ __forceinline__ double sample_linear_bin(double lam0, double lam1, double p0, double p1,
+                                                    double u)
+{
+#pragma clang fp contract(off)
+    const double uu = fmin(fmax(u, 0.0), 1.0);
+    if (p0 + p1 <= 0.0) return lam0 + uu * (lam1 - lam0);
+    const double d = p1 - p0, twoc = uu * (p0 + p1);
+    const double disc = p0 * p0 + d * twoc;
+    double t = twoc / (p0 + sqrt(fmax(disc, 0.0)));
+    t = fmin(fmax(t, 0.0), 1.0);
+    return lam0 + t * (lam1 - lam0);
+}
+
+struct IrLaunch {
+    const double *lam, *sabs, *tdust;
+    const int64_t *khi;
+    int32_t nl, nzero, pass;
+    int64_t b0, b1;                 // packet indices of this batch (1-based, inclusive)
+    unsigned long long *fdisc;      // bits of the largest discarded fraction
+};
+
+// One wave per emitting leaf: its spectrum (leaf_emission_spectrum with the
+// band cut, src/dust_emis_transport_mod.f90:372-413) in the wave's LDS, the
+// trapezoid cumulative summed in the reference's order by one lane, then the
+// leaf's packets of this batch, one per lane (launch_and_transport's launch,
+// :489-544): uniform position in the leaf, isotropic direction, wavelength
+// bin by binary search, continuous wavelength inside it, the grain optics at
+// the photon energy.  Draws 0-5 of key (pass << 40 | k) on stream 3.
+__global__ __launch_bounds__(kBlock) void k_ir_launch(KArgs a, IrLaunch q)
+{
+#pragma clang fp contract(off)
+    __shared__ double s_pdf[kBlock / 64][MC_NLAM_BB];
+    __shared__ double s_cdf[kBlock / 64][MC_NLAM_BB];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int il = blockIdx.x * (kBlock / 64) + wv;
+    if (il >= a.g.nleaf) return;
+    const int64_t klo = il > 0 ? q.khi[il - 1] + 1 : 1, khi = q.khi[il];
+    const int64_t lo = klo > q.b0 ? klo : q.b0, hi = khi < q.b1 ? khi : q.b1;
+    if (lo > hi) return;                                  // wave-uniform
+    const PacketBuf &pk = a.pk;
+    const int n = q.nl;
+    double *pdf = s_pdf[wv], *cdf = s_cdf[wv];
+    const double T = q.tdust[il];
+    bool empty = !(T > 0.0);
+    if (!empty) {
+        for (int k = lane; k < n; k += 64) pdf[k] = q.sabs[k] * planck_lambda(q.lam[k], T);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double full = 0.0;
+        if (q.nzero > 0) {                                // emission before the cut
+            for (int k = lane; k < n - 1; k += 64)
+                full += 0.5 * (pdf[k] + pdf[k + 1]) * (q.lam[k + 1] - q.lam[k]);
+            full = wave_sum(full);
+            for (int k = lane; k < q.nzero; k += 64) pdf[k] = 0.0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        // trapezoid terms in parallel, the running sum in order by lane 0
+        for (int k = lane; k < n - 1; k += 64)
+            cdf[k + 1] = 0.5 * (pdf[k] + pdf[k + 1]) * (q.lam[k + 1] - q.lam[k]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane == 0) {
+            double c = 0.0;
+            cdf[0] = 0.0;
+            for (int k = 1; k < n; ++k) {
+                c = c + cdf[k];
+                cdf[k] = c;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const double esum = cdf[n - 1];
+        empty = !(esum > 0.0);
+        if (!empty && q.nzero > 0 && lane == 0) {
+            const double f = (full - esum) / full;
+            if (f > 0.0) atomicMax(q.fdisc, (unsigned long long)__double_as_longlong(f));
+        }
+    }
+    double cx, cy, cz, half;
+    if (a.g.car) {
+        const int ix = il % a.g.nx, iy = (il / a.g.nx) % a.g.ny, iz = il / (a.g.nx * a.g.ny);
+        cx = a.g.xmin + ((double)ix + 0.5) * a.g.dcell;
+        cy = a.g.ymin + ((double)iy + 0.5) * a.g.dcell;
+        cz = a.g.zmin + ((double)iz + 0.5) * a.g.dcell;
+        half = 0.5 * a.g.dcell;
+    } else {
+        const double4 G = a.g.node_gd[a.g.leaf_node[il]];
+        cx = G.x; cy = G.y; cz = G.z; half = G.w;
+    }
+    for (int64_t k = lo + lane; k <= hi; k += 64) {
+        const int64_t r = k - q.b0;
+        if (empty) {
+            pk.leaf[r] = -2;
+            continue;
+        }
+        const uint64_t key = ((uint64_t)q.pass << 40) | (uint64_t)k;
+        pk.x[r] = cx + (2.0 * rng_u(a.seed, 3, key, 0) - 1.0) * half;
+        pk.y[r] = cy + (2.0 * rng_u(a.seed, 3, key, 1) - 1.0) * half;
+        pk.z[r] = cz + (2.0 * rng_u(a.seed, 3, key, 2) - 1.0) * half;
+        const double cost = 2.0 * rng_u(a.seed, 3, key, 3) - 1.0;
+        const double sint = sqrt(1.0 - cost * cost);
+        const double phi = MC_TWOPI * rng_u(a.seed, 3, key, 4);
+        pk.kx[r] = sint * cos(phi);
+        pk.ky[r] = sint * sin(phi);
+        pk.kz[r] = cost;
+        const double u = rng_u(a.seed, 3, key, 5) * cdf[n - 1];
+        int blo = 1, bhi = n - 1;                         // 1-based bins
+        while (blo < bhi) {
+            const int mid = (blo + bhi) >> 1;
+            if (u <= cdf[mid]) bhi = mid;
+            else blo = mid + 1;
+        }
+        const int j = blo - 1;
+        const double ulo = cdf[j];
+        const double ubin = cdf[j + 1] > ulo ? (u - ulo) / (cdf[j + 1] - ulo) : 0.0;
+        const double lam = sample_linear_bin(q.lam[j], q.lam[j + 1], pdf[j], pdf[j + 1], ubin);
+        double sa, ss, gg;
+        dust_at(a, 1.23984 / lam, sa, ss, gg);
+        pk.E[r] = lam;
+        double *sg = pk.sig + 6 * r;
+        sg[0] = 0.0; sg[1] = 0.0; sg[2] = 0.0;
+        sg[3] = sa; sg[4] = ss; sg[5] = gg;
+        pk.inu[r] = 0;
+        pk.leaf[r] = il;
+    }
+}
+
+// dust_ir_heating (src/dust_ir_band_mod.f90:182-192) and the new total
+// Heat_dust = EUV part + reabsorbed infrared; partials {0, sum heat_ir V,
+// sum heat_dust V} per block
+__global__ __launch_bounds__(kBlock) void k_ir_heat(DevGrid g, const double *irabs, const double *rhokap,
+                                                    const double *heat_euv, double *heat_ir,
+                                                    double *heat, double d2cm, double *partials)
+{
+#pragma clang fp contract(off)
+    const int il = blockIdx.x * kBlock + threadIdx.x;
+    double v[3] = {0.0, 0.0, 0.0};
+    bool ismax[3] = {true, false, false};
+    if (il < g.nleaf) {
+        const double s2 = 2.0 * leaf_half(g, il);
+        const double fac = 1.0 / ((s2 * s2) * s2 * (d2cm * d2cm));
+        const double hir = irabs[il] * fac * rhokap[il] / d2cm;
+        const double ht = heat_euv[il] + hir;
+        heat_ir[il] = hir;
+        heat[il] = ht;
+        const double w = s2 * d2cm, vol = (w * w) * w;
+        v[1] = hir * vol;
+        v[2] = ht * vol;
+    }
+    block_reduce<3>(v, ismax, partials);
+}
+
 template <int P, int G>
 __global__ void k_trace(KArgs a, double x, double y, double z, double kx, double ky, double kz,
                         int32_t *cells, int maxn, int32_t *n)
@@ -2064,6 +2470,12 @@ KArgs make_args(mc_ctx *c)
     a.mraw = c->d_mraw;
     a.mop = c->d_mop;
     a.mopf = c->d_mopf;
+    a.Lpk = c->Ltot / (double)c->p.nphotons;
+    a.irabs = nullptr;
+    a.ir_lesc = nullptr;
+    a.ir_edge = nullptr;
+    a.ir_nl = 0;
+    a.ir_wave = nullptr;
     return a;
 }
 
@@ -2207,7 +2619,7 @@ int mc_run_pass(mc_ctx *c)
             c->grid, c->st, c->milne, c->p.He_abund, c->p.distance2cm, c->d_dif);
         MC_HIP(hipGetLastError());
         const int nb = (nl1 + kScanTile - 1) / kScanTile;
-        k_scan_tiles<<<nb, kBlock, 0, st>>>(c->d_dif, nl1, c->d_dcdf, c->d_bsum);
+        k_scan_tiles<<<nb, kBlock, 0, st>>>(c->d_dif + 3, 4, nl1, c->d_dcdf, c->d_bsum);
         k_scan_sums<<<1, 1024, 0, st>>>(c->d_bsum, nb);
         k_scan_apply<<<(nl1 + kBlock - 1) / kBlock, kBlock, 0, st>>>(c->d_dcdf, nl1, c->d_bsum, nb);
         MC_HIP(hipGetLastError());
@@ -2428,4 +2840,190 @@ int mc_run_trace(mc_ctx *c, int precision, const double *pos, const double *dir,
     (void)hipFree(dcells);
     (void)hipFree(dn);
     return 0;
+}
+
+// dust_temp_compute over the grid from a device Heat_dust array
+int mc_run_dust_temp(mc_ctx *c, const double *d_heat)
+{
+    const IrBand &b = c->ir;
+    k_dust_temp<<<(c->nleaf + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
+        c->nleaf, c->st.rhokap, d_heat, b.d_tdust, b.d_dt, c->p.distance2cm);
+    MC_HIP(hipGetLastError());
+    MC_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// transport_dust_emission (src/dust_emis_transport_mod.f90:208-366) on the
+// device-resident Heat_dust.  Per pass: leaf luminosities and their scan,
+// stratified counts, then batches of launch records (one wave per emitting
+// leaf) walked by the infrared walk variant; then heat_ir, the new
+// Heat_dust, T_dust and the ledger / luminosity sums that decide the
+// iteration.
+int mc_run_dust_emission(mc_ctx *c, double rep[MC_DUSTRT_NREPORT])
+{
+    IrBand &b = c->ir;
+    hipStream_t st = c->stream;
+    const int nl = c->nleaf;
+    for (int k = 0; k < MC_DUSTRT_NREPORT; ++k) rep[k] = 0.0;
+    const int64_t N = (int64_t)std::fmax(c->p.dust_no_photons, 1.0);
+    const int niter = std::max(c->p.dust_niter, 1);
+    MC_HIP(hipMemcpyAsync(b.d_heat_euv, c->st.heat_dust, sizeof(double) * nl,
+                          hipMemcpyDeviceToDevice, st));
+    KArgs a = make_args(c);
+    a.stream = 3;
+    a.perm = nullptr;
+    a.coherent = 0;
+    a.pk.key = nullptr;
+    a.ntr = 0;
+    a.irabs = b.d_irabs;
+    a.ir_lesc = b.d_lesc;
+    a.ir_edge = b.d_edge;
+    a.ir_nl = b.nl;
+    a.s.ledger = b.d_led;
+    a.s.segs = b.d_segs;
+    const bool face = !c->car || c->p.car_walk_amr;
+    const bool mixed = c->p.precision == MC_MIXED;
+    const void *kw = face ? (mixed ? (const void *)k_walk<MC_MIXED, kFaceExit, 2>
+                                   : (const void *)k_walk<MC_FP64, kFaceExit, 2>)
+                          : (mixed ? (const void *)k_walk<MC_MIXED, kCarDDA, 2>
+                                   : (const void *)k_walk<MC_FP64, kCarDDA, 2>);
+    const int nblk_walk = persistent_blocks(kw, 0);
+    const size_t nslot = (size_t)nblk_walk * (kBlock / 64) * 5;
+    if (b.n_wave < nslot) {
+        if (b.d_wave) (void)hipFree(b.d_wave);
+        b.d_wave = nullptr;
+        MC_HIP(hipMalloc(&b.d_wave, sizeof(double) * nslot));
+        b.n_wave = nslot;
+    }
+    a.ir_wave = b.d_wave;
+    std::vector<double> hw(nslot);
+    const int nblk = (nl + kBlock - 1) / kBlock;
+    const int ntile = (nl + kScanTile - 1) / kScanTile;
+    const int nwblk = (nl + kBlock / 64 - 1) / (kBlock / 64);
+    IrLaunch q;
+    q.lam = b.d_lam;
+    q.sabs = b.d_sabs;
+    q.tdust = b.d_tdust;
+    q.khi = b.d_khi;
+    q.nl = b.nl;
+    q.nzero = b.nzero;
+    q.fdisc = b.d_segs + 3;
+    float t_launch = 0.0f, t_walk = 0.0f, t_rest = 0.0f;
+    double L_prev = 0.0;
+    int done = 0, converged = 0, rc = 0;
+    b.segs[0] = b.segs[1] = 0;
+    for (int iter = 1; iter <= niter; ++iter) {
+        done = iter;
+        MC_HIP(hipEventRecord(c->ev[4], st));
+        k_ir_lum<<<nblk, kBlock, 0, st>>>(c->grid, c->st.heat_dust, c->p.distance2cm, b.d_lum);
+        k_scan_tiles<<<ntile, kBlock, 0, st>>>(b.d_lum, 1, nl, b.d_lcum, b.d_bsum);
+        k_scan_sums<<<1, 1024, 0, st>>>(b.d_bsum, ntile);
+        MC_HIP(hipGetLastError());
+        double tail[2];
+        MC_HIP(hipMemcpyAsync(&tail[0], b.d_lcum + (nl - 1), sizeof(double), hipMemcpyDeviceToHost, st));
+        MC_HIP(hipMemcpyAsync(&tail[1], b.d_bsum + (nl - 1) / kScanTile, sizeof(double),
+                              hipMemcpyDeviceToHost, st));
+        MC_HIP(hipStreamSynchronize(st));
+        const double L_tot = tail[0] + tail[1];
+        if (iter == 1) L_prev = L_tot;
+        if (!(L_tot > 0.0)) break;
+        // one shared stratification offset per pass
+        double xi = rng_u((uint32_t)c->p.qmc_seed, 3, (uint64_t)iter << 40, 0);
+        xi = std::fmin(std::fmax(xi, 1.0e-12), 1.0 - 1.0e-12);
+        a.Lpk = L_tot / (double)N;
+        k_ir_counts<<<nblk, kBlock, 0, st>>>(nl, b.d_lcum, b.d_bsum, N, xi, b.d_khi);
+        k_imax_tiles<<<ntile, kBlock, 0, st>>>(b.d_khi, nl, b.d_kmax);
+        k_imax_sums<<<1, 1024, 0, st>>>(b.d_kmax, ntile);
+        k_imax_apply<<<nblk, kBlock, 0, st>>>(b.d_khi, nl, b.d_kmax);
+        k_zero<<<64, kBlock, 0, st>>>(b.d_irabs, (size_t)nl);
+        k_zero<<<1, kBlock, 0, st>>>(b.d_lesc, (size_t)b.nl);
+        k_zero_counters<<<1, 64, 0, st>>>(a.s);
+        MC_HIP(hipGetLastError());
+        MC_HIP(hipEventRecord(c->ev[5], st));
+        MC_HIP(hipEventSynchronize(c->ev[5]));
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, c->ev[4], c->ev[5]);
+        t_rest += ms;
+        q.pass = iter;
+        long double n_emit = 0.0L, e_abs = 0.0L, e_esc = 0.0L;
+        for (int64_t b0 = 1; b0 <= N; b0 += c->pk.cap) {
+            const int64_t b1 = std::min<int64_t>(N, b0 + c->pk.cap - 1);
+            q.b0 = b0;
+            q.b1 = b1;
+            a.first_id = ((int64_t)iter << 40) + b0;
+            a.npk = b1 - b0 + 1;
+            MC_HIP(hipEventRecord(c->ev[1], st));
+            k_ir_launch<<<nwblk, kBlock, 0, st>>>(a, q);
+            MC_HIP(hipGetLastError());
+            k_reset_counter<<<1, 1, 0, st>>>(c->st.counter);
+            MC_HIP(hipEventRecord(c->ev[2], st));
+            void *kargs[] = {&a};
+            MC_HIP(hipLaunchKernel(kw, dim3(nblk_walk), dim3(kBlock), kargs, 0, st));
+            MC_HIP(hipGetLastError());
+            MC_HIP(hipEventRecord(c->ev[3], st));
+            MC_HIP(hipEventSynchronize(c->ev[3]));
+            (void)hipEventElapsedTime(&ms, c->ev[1], c->ev[2]);
+            t_launch += ms;
+            (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
+            t_walk += ms;
+            MC_HIP(hipMemcpy(hw.data(), b.d_wave, sizeof(double) * nslot, hipMemcpyDeviceToHost));
+            for (size_t w = 0; w < nslot; w += 5) {
+                n_emit += (long double)hw[w];
+                e_abs += (long double)hw[w + 1] + (long double)hw[w + 2];
+                e_esc += (long double)hw[w + 3] + (long double)hw[w + 4];
+            }
+        }
+        MC_HIP(hipEventRecord(c->ev[4], st));
+        k_ir_heat<<<nblk, kBlock, 0, st>>>(c->grid, b.d_irabs, c->st.rhokap, b.d_heat_euv,
+                                           b.d_heat_ir, c->st.heat_dust, c->p.distance2cm,
+                                           c->st.partials);
+        k_dust_temp<<<nblk, kBlock, 0, st>>>(nl, c->st.rhokap, c->st.heat_dust, b.d_tdust, b.d_dt,
+                                             c->p.distance2cm);
+        MC_HIP(hipGetLastError());
+        const double led[3] = {(double)(n_emit * (long double)a.Lpk), (double)e_abs, (double)e_esc};
+        unsigned long long sg[4];
+        MC_HIP(hipMemcpyAsync(sg, b.d_segs, sizeof(sg), hipMemcpyDeviceToHost, st));
+        double m[3];
+        if (sum_partials(c, nblk, m)) return -100;
+        MC_HIP(hipEventRecord(c->ev[5], st));
+        MC_HIP(hipEventSynchronize(c->ev[5]));
+        (void)hipEventElapsedTime(&ms, c->ev[4], c->ev[5]);
+        t_rest += ms;
+        for (int k = 0; k < 3; ++k) b.led[k] = led[k];
+        b.segs[0] = (int64_t)sg[0];
+        b.segs[1] = (int64_t)sg[1];
+        double fd;
+        std::memcpy(&fd, &sg[3], sizeof(fd));
+        b.frac_discarded = fd;
+        const double emitted = led[0], extinct = led[1], escaped = led[2];
+        const double closure = std::fabs(extinct + escaped - emitted) / std::fmax(emitted, DBL_MIN);
+        const double L_abs = m[1], L_new = m[2];
+        const double drel = std::fabs(L_new - L_prev) / std::fmax(L_new, DBL_MIN);
+        rep[MC_DUSTRT_L_EMIT] = L_tot;
+        rep[MC_DUSTRT_EMITTED] = emitted;
+        rep[MC_DUSTRT_EXTINCT] = extinct;
+        rep[MC_DUSTRT_ESCAPED] = escaped;
+        rep[MC_DUSTRT_CLOSURE] = closure;
+        rep[MC_DUSTRT_L_ABS_PATH] = L_abs;
+        rep[MC_DUSTRT_DREL] = drel;
+        if (closure > 5.0e-13) {
+            mc_set_error("mc_transport_dust_emission: the transported infrared does not conserve "
+                         "energy (closure " + std::to_string(closure) + ")");
+            rc = -3;
+            break;
+        }
+        L_prev = L_new;
+        if (drel < c->p.dust_tol) {
+            converged = 1;
+            break;
+        }
+    }
+    rep[MC_DUSTRT_PASSES] = done;
+    rep[MC_DUSTRT_CONVERGED] = converged;
+    rep[MC_DUSTRT_FDISC] = b.nzero > 0 ? b.frac_discarded : 0.0;
+    std::memcpy(b.report, rep, sizeof(double) * MC_DUSTRT_NREPORT);
+    b.times[0] = t_launch;
+    b.times[1] = t_walk;
+    b.times[2] = t_rest;
+    return rc;
 }

@@ -8,33 +8,7 @@
 // and the per-transition registry fields are scalar loads.
 #pragma once
 
-// photoionisation cross sections [cm^2] (src/photo_xsec.f90:46-89)
-__device__ __forceinline__ double m_vfky96(double E, const double *p)
-{
-    if (E < p[0]) return 0.0;
-    const double x = E / p[1] - p[6];
-    const double z = sqrt(x * x + p[7] * p[7]);
-    const double Q = 5.5 - 0.5 * p[4];
-    const double xm = x - 1.0;
-    const double Fy = (xm * xm + p[5] * p[5]) * pow(z, -Q) * pow(1.0 + sqrt(z / p[3]), -p[4]);
-    return p[2] * Fy * 1.0e-18;
-}
-
-__device__ __forceinline__ double m_vy95(double E, const double *p)
-{
-    if (E < p[0]) return 0.0;
-    const double y = E / p[1];
-    const double Q = 5.5 + p[6] - 0.5 * p[4];
-    const double ym = y - 1.0;
-    const double Fy = (ym * ym + p[5] * p[5]) * pow(y, -Q) * pow(1.0 + sqrt(y / p[3]), -p[4]);
-    return p[2] * Fy * 1.0e-18;
-}
-
-// species_sigma (src/species_mod.f90:763-776)
-__device__ __forceinline__ double m_sigma(const DevMetalTab *mt, int t, double E)
-{
-    return mt->photo_form[t] == 2 ? m_vy95(E, mt->photo[t]) : m_vfky96(E, mt->photo[t]);
-}
+// photoionisation cross sections m_vfky96 / m_vy95 / m_sigma: mc_internal.h
 
 // voronov_ci x ci_dere_ratio (src/species_mod.f90:815-821, 524-525)
 __device__ __forceinline__ double m_ci(const DevMetalTab *mt, int t, double T)

@@ -87,6 +87,7 @@ contains
     call off('nmetal', c_loc(p%nmetal));          call off('metal_eth', c_loc(p%metal_eth))
     call off('ion_metal_abs', c_loc(p%ion_metal_abs))
     call off('cooling_model', c_loc(p%cooling_model))
+    call off('dust_niter', c_loc(p%dust_niter));  call off('dust_tol', c_loc(p%dust_tol))
   contains
     subroutine off(name, q)
       character(len=*), intent(in) :: name

@@ -23,6 +23,10 @@ module mochii_gpu
   integer(c_int), parameter, public :: MC_CX_TWO_WAY = 0, MC_CX_METAL_ONLY = 1
   integer(c_int), parameter, public :: MC_CI_VORONOV = 0, MC_CI_DERE_HYBRID = 1
   integer(c_int), parameter, public :: MC_COOL_LOW_DENSITY = 0, MC_COOL_LOCAL_NE = 1
+  ! dust thermal-infrared band: per-leaf fields and the report length
+  integer(c_int), parameter, public :: MC_DUST_HEAT = 0, MC_DUST_TEMP = 1
+  integer(c_int), parameter, public :: MC_DUST_HEAT_IR = 2, MC_DUST_IR_ABS = 3
+  integer(c_int), parameter, public :: MC_DUSTRT_NREPORT = 10
 
   type, bind(C), public :: mc_params
      integer(c_int64_t) :: nphotons
@@ -47,6 +51,8 @@ module mochii_gpu
      real(c_double)     :: metal_eth(MC_MAX_METAL_TR)
      integer(c_int32_t) :: ion_metal_abs, metal_ne, metal_heat, metal_freefree
      integer(c_int32_t) :: charge_exchange, ci_model, cooling_model
+     integer(c_int32_t) :: dust_niter
+     real(c_double)     :: dust_no_photons, dust_tol
   end type mc_params
 
   ! one registry element (read_element + cooling_load); C arrays [stage][k]
@@ -272,6 +278,60 @@ module mochii_gpu
        real(c_double), intent(out) :: raw(*)
        integer(c_int) :: rc
      end function mc_species_tallies_get
+     ! dust_temp_setup / dust_temp_compute / dust_emis_transport_setup /
+     ! transport_dust_emission (src/dust_temp_mod.f90,
+     ! src/dust_emis_transport_mod.f90) on the device-resident heat_dust
+     function mc_dust_temp_setup(ctx) bind(C, name='mc_dust_temp_setup') result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int) :: rc
+     end function mc_dust_temp_setup
+     function mc_dust_temp_compute(ctx) bind(C, name='mc_dust_temp_compute') result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int) :: rc
+     end function mc_dust_temp_compute
+     function mc_dust_emis_transport_setup(ctx) bind(C, name='mc_dust_emis_transport_setup') &
+          result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int) :: rc
+     end function mc_dust_emis_transport_setup
+     function mc_transport_dust_emission(ctx, report) bind(C, name='mc_transport_dust_emission') &
+          result(rc)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: report(*)
+       integer(c_int) :: rc
+     end function mc_transport_dust_emission
+     function mc_dust_get(ctx, field, dst) bind(C, name='mc_dust_get') result(rc)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: field
+       real(c_double), intent(out) :: dst(*)
+       integer(c_int) :: rc
+     end function mc_dust_get
+     function mc_dust_set(ctx, field, src) bind(C, name='mc_dust_set') result(rc)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: field
+       real(c_double), intent(in) :: src(*)
+       integer(c_int) :: rc
+     end function mc_dust_set
+     function mc_dust_ir_energy_budget(ctx, out3) bind(C, name='mc_dust_ir_energy_budget') &
+          result(rc)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: out3(3)
+       integer(c_int) :: rc
+     end function mc_dust_ir_energy_budget
+     function mc_dust_ir_escaped_spectrum(ctx, lam, Llam, dlam) &
+          bind(C, name='mc_dust_ir_escaped_spectrum') result(rc)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: lam(*), Llam(*), dlam(*)
+       integer(c_int) :: rc
+     end function mc_dust_ir_escaped_spectrum
   end interface
 
   public :: mc_last_error, mc_init, mc_finalize, mc_grid_octree, mc_gas_state_setup
@@ -282,6 +342,9 @@ module mochii_gpu
   public :: mc_energy_ledger, mc_info, mc_set_precision
   public :: mc_species_setup, mc_species_fractions_get, mc_species_rates_get
   public :: mc_species_tallies_get
+  public :: mc_dust_temp_setup, mc_dust_temp_compute, mc_dust_emis_transport_setup
+  public :: mc_transport_dust_emission, mc_dust_get, mc_dust_set
+  public :: mc_dust_ir_energy_budget, mc_dust_ir_escaped_spectrum
 
 contains
 
@@ -307,6 +370,7 @@ contains
     p%ion_metal_abs = 1;  p%metal_ne = 1;  p%metal_heat = 1;  p%metal_freefree = 1
     p%charge_exchange = MC_CX_TWO_WAY;  p%ci_model = MC_CI_VORONOV
     p%cooling_model = MC_COOL_LOW_DENSITY
+    p%dust_niter = 2;  p%dust_no_photons = 1d6;  p%dust_tol = 1d-3
   end function mc_params_default
 
   ! mc_last_error() as a Fortran string (for fatal_error).

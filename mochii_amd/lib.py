@@ -16,6 +16,10 @@ MC_CX_TWO_WAY, MC_CX_METAL_ONLY = 0, 1
 MC_CI_VORONOV, MC_CI_DERE_HYBRID = 0, 1
 MC_COOL_LOW_DENSITY, MC_COOL_LOCAL_NE = 0, 1
 FIELDS = {"nH": 0, "xHI": 1, "xHeI": 2, "xHeII": 3, "ne": 4, "Te": 5, "rhokap": 6}
+DUST_FIELDS = {"heat_dust": 0, "t_dust": 1, "heat_ir": 2, "ir_abs_path": 3}
+MC_DUSTRT_NREPORT = 10
+DUSTRT_KEYS = ("passes", "converged", "L_emit", "emitted", "extinguished", "escaped",
+               "closure", "L_abs_path", "drel", "frac_discarded")
 
 
 class MochiiError(RuntimeError):
@@ -50,6 +54,7 @@ class Params(C.Structure):
         ("ion_metal_abs", C.c_int32), ("metal_ne", C.c_int32), ("metal_heat", C.c_int32),
         ("metal_freefree", C.c_int32), ("charge_exchange", C.c_int32), ("ci_model", C.c_int32),
         ("cooling_model", C.c_int32),
+        ("dust_niter", C.c_int32), ("dust_no_photons", C.c_double), ("dust_tol", C.c_double),
     ]
 
     # reference defaults (src/define.f90) for the fields this path reads
@@ -64,7 +69,8 @@ class Params(C.Structure):
                     xHeII_init=0.0, ion_add_dust=0, ion_dust_scatter=0, lambda_ref=0.55,
                     diffuse_field=0, diffuse_model=MC_DIFFUSE_MILNE, nmetal=0, ion_metal_abs=1,
                     metal_ne=1, metal_heat=1, metal_freefree=1, charge_exchange=MC_CX_TWO_WAY,
-                    ci_model=MC_CI_VORONOV, cooling_model=MC_COOL_LOW_DENSITY)
+                    ci_model=MC_CI_VORONOV, cooling_model=MC_COOL_LOW_DENSITY,
+                    dust_niter=2, dust_no_photons=1.0e6, dust_tol=1.0e-3)
 
     @classmethod
     def make(cls, **kw):
@@ -142,6 +148,16 @@ def library():
         "mc_species_rates_get": ([vp, C.c_int, dp, dp], C.c_int),
         "mc_species_tallies_get": ([vp, C.c_int, dp], C.c_int),
         "mc_metal_segment_transitions": ([vp, i64p], C.c_int),
+        "mc_dust_temp_setup": ([vp], C.c_int),
+        "mc_dust_temp_compute": ([vp], C.c_int),
+        "mc_dust_emis_transport_setup": ([vp], C.c_int),
+        "mc_transport_dust_emission": ([vp, dp], C.c_int),
+        "mc_dust_get": ([vp, C.c_int, dp], C.c_int),
+        "mc_dust_set": ([vp, C.c_int, dp], C.c_int),
+        "mc_dust_ir_energy_budget": ([vp, dp], C.c_int),
+        "mc_dust_ir_escaped_spectrum": ([vp, dp, dp, dp], C.c_int),
+        "mc_dust_ir_grid": ([vp, ip, dp, dp, dp, ip, dp, dp], C.c_int),
+        "mc_dust_ir_timing": ([vp, dp, i64p], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

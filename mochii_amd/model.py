@@ -9,7 +9,7 @@ import time
 
 import numpy as np
 
-from .lib import FIELDS, MC_GRID_AMR, Params, check, library
+from .lib import DUST_FIELDS, DUSTRT_KEYS, FIELDS, MC_DUSTRT_NREPORT, MC_GRID_AMR, Params, check, library
 
 
 def _dp(a):
@@ -98,6 +98,66 @@ class Model:
         out = np.zeros((self.nleaf, nt, 2))
         check(self._L.mc_species_tallies_get(self._h, ie, _dp(out)), "mc_species_tallies_get")
         return out
+
+    # ---- dust thermal-infrared band -----------------------------------------
+    def dust_temp_setup(self):
+        """dust_temp_setup: femit(T) of the loaded extinction table."""
+        check(self._L.mc_dust_temp_setup(self._h), "mc_dust_temp_setup")
+
+    def dust_temp_compute(self):
+        """dust_temp_compute: T_dust of every leaf from the current Heat_dust."""
+        check(self._L.mc_dust_temp_compute(self._h), "mc_dust_temp_compute")
+        return self.dust("t_dust")
+
+    def dust_emis_transport_setup(self):
+        check(self._L.mc_dust_emis_transport_setup(self._h), "mc_dust_emis_transport_setup")
+
+    def transport_dust_emission(self):
+        """transport_dust_emission: the infrared self-heating iteration on the
+        device-resident Heat_dust; returns its report as a dict."""
+        rep = np.zeros(MC_DUSTRT_NREPORT)
+        check(self._L.mc_transport_dust_emission(self._h, _dp(rep)), "mc_transport_dust_emission")
+        return dict(zip(DUSTRT_KEYS, [float(v) for v in rep]))
+
+    def dust(self, name):
+        """Per-leaf dust field: heat_dust, t_dust, heat_ir or ir_abs_path."""
+        out = np.empty(self.nleaf)
+        check(self._L.mc_dust_get(self._h, DUST_FIELDS[name], _dp(out)), "mc_dust_get")
+        return out
+
+    def set_heat_dust(self, heat):
+        a = np.ascontiguousarray(heat, dtype=np.float64)
+        check(self._L.mc_dust_set(self._h, DUST_FIELDS["heat_dust"], _dp(a)), "mc_dust_set")
+
+    def dust_ir_energy_budget(self):
+        out = np.zeros(3)
+        check(self._L.mc_dust_ir_energy_budget(self._h, _dp(out)), "mc_dust_ir_energy_budget")
+        return out
+
+    def dust_ir_escaped_spectrum(self):
+        nl = self.dust_ir_grid()["nl"]
+        lam, Llam, dlam = np.zeros(nl), np.zeros(nl), np.zeros(nl)
+        check(self._L.mc_dust_ir_escaped_spectrum(self._h, _dp(lam), _dp(Llam), _dp(dlam)),
+              "mc_dust_ir_escaped_spectrum")
+        return lam, Llam, dlam
+
+    def dust_ir_grid(self):
+        nl, nz, lh = C.c_int32(), C.c_int32(), C.c_double()
+        check(self._L.mc_dust_ir_grid(self._h, C.byref(nl), None, None, None, None, None, None),
+              "mc_dust_ir_grid")
+        n = nl.value
+        lam, sabs, edge, fe = np.zeros(n), np.zeros(n), np.zeros(n + 1), np.zeros(122)
+        check(self._L.mc_dust_ir_grid(self._h, C.byref(nl), _dp(lam), _dp(sabs), _dp(edge),
+                                      C.byref(nz), C.byref(lh), _dp(fe)), "mc_dust_ir_grid")
+        return dict(nl=n, lam=lam, sabs=sabs, edge=edge, nzero=nz.value, lam_hardest=lh.value,
+                    tgrid=fe[:61].copy(), femit=fe[61:].copy())
+
+    def dust_ir_timing(self):
+        t, sg = np.zeros(3), np.zeros(2, dtype=np.int64)
+        check(self._L.mc_dust_ir_timing(self._h, _dp(t), sg.ctypes.data_as(C.POINTER(C.c_int64))),
+              "mc_dust_ir_timing")
+        return dict(launch_ms=t[0], walk_ms=t[1], other_ms=t[2], seg_direct=int(sg[0]),
+                    seg_scattered=int(sg[1]))
 
     def ion_dust_table(self, lam, albedo, gcos, cext):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (lam, albedo, gcos, cext)]
