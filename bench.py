@@ -32,6 +32,12 @@ Prints ONE JSON line.  Extra fields:
                scored segment x segments over its event-timed duration
   cpu_baseline the FP64 CPU oracle on the host cores on a bounded sample of
                the same pass (~10 s)
+  ir_band      the dust thermal-infrared band on the converged model with
+               grains added (dust-to-gas 0.01, power-law extinction to the
+               far infrared, scattering on): Heat_dust of one ionizing pass,
+               then transport_dust_emission, 2 passes of 1e7 packets, per
+               form: packets/s, walk time, segments, reabsorbed fraction,
+               closure, and the infrared walk's algorithmic bandwidth
 
   python bench.py --trace-summary KERNEL_TRACE.csv [--steps K]
     condenses a rocprofv3 --kernel-trace CSV of a bench run into per-kernel
@@ -58,6 +64,11 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_SEGMENT = {("car", "fp64"): 144, ("car", "mixed_mlp"): 128,
                      ("octree", "fp64"): 188, ("octree", "mixed_mlp"): 152}
 BYTES_PER_METAL_TR = {"fp64": 40, "mixed_mlp": 36}
+# the infrared walk per scored segment: leaf geometry (octree FP64 44 B /
+# FP32 24 B; car 0) + rhokap 8 B + one FP64 path-tally read-modify-write 16 B
+IR_BYTES_PER_SEGMENT = {("car", "fp64"): 24, ("car", "mixed"): 24,
+                        ("octree", "fp64"): 68, ("octree", "mixed"): 48}
+PC2CM = 3.0856776e18
 WALK_KERNEL = {("car", "fp64"): "k_walk<MC_FP64, kCarDDA, M>",
                ("car", "mixed_mlp"): "k_walk<MC_MIXED, kCarDDA, M>",
                ("octree", "fp64"): "k_walk<MC_FP64, kFaceExit, M>",
@@ -75,6 +86,8 @@ def parse():
     ap.add_argument("--trace-summary", default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-converge", action="store_true")
+    ap.add_argument("--ir-packets", type=int, default=10_000_000)
+    ap.add_argument("--no-ir", action="store_true")
     return ap.parse_args()
 
 
@@ -265,6 +278,57 @@ def one_step(name, params, nH, octree, state, mlp):
     return out
 
 
+def ir_band(params, nH, octree, grid, state, packets):
+    """The dust thermal-infrared band (transport_dust_emission) on the
+    converged model with grains added, in both precision forms."""
+    import numpy as np
+    import mochii_amd as mc
+    from mochii_amd.workloads import elements_of, synthetic_kext
+    tab = synthetic_kext(lam_max=1.0e4)
+    out = {"workload": f"converged model + grains: dust-to-gas 0.01 (rhokap = n_H 4.868e-22 "
+                       f"cm^2 d2cm), C_ext ~ lambda^-1.2 to 1e4 um, albedo 0.4, g 0.6, "
+                       f"scattering on; Heat_dust of one ionizing pass; 2 passes of "
+                       f"{packets:.0e} packets",
+           "packets_per_pass": int(packets)}
+    for name, prec in (("fp64", mc.MC_FP64), ("mixed", mc.MC_MIXED)):
+        p = dict(params, precision=prec, ion_add_dust=1, ion_dust_scatter=1, lambda_ref=0.55,
+                 dust_no_photons=float(packets), dust_niter=2, dust_tol=1e-12)
+        m = mc.Model(p)
+        if octree is not None:
+            m.grid_octree(*octree)
+        m.ion_dust_table(*tab)
+        els = elements_of(params)
+        if els:
+            m.species_setup(els)
+        m.gas_state_setup(nH)
+        m.set_state(rhokap=nH * 4.868e-22 * PC2CM,
+                    **{k: state[k] for k in ("xHI", "xHeI", "xHeII", "ne", "Te")})
+        m.ionizing_field_and_rates()
+        m.dust_temp_setup()
+        m.dust_temp_compute()
+        m.dust_emis_transport_setup()
+        m.transport_dust_emission()                  # warm-up (same work)
+        t0 = time.perf_counter()
+        rep = m.transport_dust_emission()
+        dt = time.perf_counter() - t0
+        tm = m.dust_ir_timing()
+        nseg = tm["seg_direct"] + tm["seg_scattered"]     # last pass
+        walk_pass = tm["walk_ms"] / rep["passes"]
+        bps = IR_BYTES_PER_SEGMENT[(grid, name)]
+        out[name] = dict(packets_per_s=rep["passes"] * packets / dt, s_per_call=dt,
+                         walk_ms_per_pass=walk_pass,
+                         launch_ms_per_pass=tm["launch_ms"] / rep["passes"],
+                         other_ms=tm["other_ms"], segments_last_pass=int(nseg),
+                         scattered_segments=int(tm["seg_scattered"]),
+                         reabsorbed_fraction=rep["L_abs_path"] / rep["L_emit"],
+                         closure=rep["closure"], drel=rep["drel"],
+                         t_dust_max=float(np.max(m.dust("t_dust"))),
+                         walk_GBps=nseg * bps / (walk_pass * 1e-3) / 1e9, bytes_per_segment=bps)
+        log(f"[ir {name}] {out[name]}")
+        m.close()
+    return out
+
+
 def cpu_baseline(params, nH, octree, state, seconds):
     """The FP64 oracle's transport pass on the host cores over a bounded
     sample of the stellar packets of one pass on the converged state."""
@@ -325,6 +389,10 @@ def main():
     if f64["time_to_converged_s"] and fm["time_to_converged_s"]:
         speed = f64["time_to_converged_s"] / fm["time_to_converged_s"]
 
+    ir = None
+    if not args.no_ir:
+        log("[ir] dust thermal-infrared band on the converged model")
+        ir = ir_band(params, nH, octree, grid, states["fp64"], args.ir_packets)
     cpu = None if args.no_cpu else cpu_baseline(params, nH, octree, states["mixed_mlp"],
                                                 args.cpu_seconds)
     if cpu:
@@ -364,6 +432,7 @@ def main():
                      "segments_per_pass": fm["segments_per_pass"],
                      "walk_ms_per_pass": fm["walk_ms"]},
         "cpu_baseline": cpu,
+        "ir_band": ir,
     }
     print(json.dumps(line))
 

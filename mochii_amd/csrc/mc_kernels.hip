@@ -1175,6 +1175,13 @@ __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, 
 {
     const PacketBuf &pk = a.pk;
     if (a.perm) r = a.perm[r];
+    if constexpr (M == 2) {
+        // infrared records are in leaf order and the brightest leaves emit
+        // most packets: walk them in a scattered order (r -> r p mod n with
+        // the prime p = 2^32 - 5 > n), so that lanes in flight start in
+        // different leaves instead of contending on one leaf's tally
+        r = (int64_t)(((uint64_t)r * 4294967291ull) % (uint64_t)a.npk);
+    }
     L.id = a.first_id + r;
     L.x = pk.x[r]; L.y = pk.y[r]; L.z = pk.z[r];
     L.kx = pk.kx[r]; L.ky = pk.ky[r]; L.kz = pk.kz[r];
@@ -1246,6 +1253,8 @@ __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, 
     }
 }
 
+constexpr int kWalkChunk = 256;   // records a launch-order wave takes at once
+
 template <int P, int G, int M>
 __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
 {
@@ -1261,20 +1270,45 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
     L.nmet = 0;
     Counters cn = {0.0, 0.0, 0.0, 0ull, 0ull, 0ull, 0ull, 0.0, 0.0};
     const int lane = threadIdx.x & 63;
+    // launch-order walks: the wave's own queue of records [qn, qe), taken
+    // from the batch counter kWalkChunk at a time, so that refilling a lane
+    // does not cost an atomic on the one counter every wave shares
+    unsigned long long qn = 0, qe = 0;
+    bool dry = false;
     for (;;) {
-        // direction-sorted records: the wave takes 64 consecutive records at
-        // once, when every lane is idle, so its packets stay together
         const unsigned long long need = __ballot(L.phase == 0);
-        const bool refill = a.coherent ? need == ~0ull : need != 0ull;
-        if (refill) {
-            const int leader = __ffsll((long long)need) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(a.s.counter, (unsigned long long)__popcll(need));
-            base = __shfl(base, leader);
-            if (L.phase == 0) {
-                const unsigned long long r = base + __popcll(need & ((1ull << lane) - 1ull));
+        if (a.coherent) {
+            // direction-sorted records: the wave takes 64 consecutive records
+            // at once, when every lane is idle, so its packets stay together
+            if (need == ~0ull) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(a.s.counter, 64ull);
+                base = __shfl(base, 0);
+                const unsigned long long r = base + (unsigned long long)lane;
                 if ((int64_t)r < a.npk) load_packet<P, G, M>(a, L, (int64_t)r, cn);
                 else L.phase = 3;
+            }
+        } else if (need != 0ull) {
+            if (!dry && qn >= qe) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(a.s.counter, (unsigned long long)kWalkChunk);
+                base = __shfl(base, 0);
+                qn = base;
+                qe = base + kWalkChunk;
+                dry = (int64_t)base >= a.npk;
+            }
+            if (dry) {
+                if (L.phase == 0) L.phase = 3;
+            } else {
+                const unsigned long long avail = qe - qn;
+                const unsigned long long rank = __popcll(need & ((1ull << lane) - 1ull));
+                if (L.phase == 0 && rank < avail) {
+                    const unsigned long long r = qn + rank;
+                    if ((int64_t)r < a.npk) load_packet<P, G, M>(a, L, (int64_t)r, cn);
+                    else L.phase = 3;
+                }
+                const unsigned long long took = __popcll(need);
+                qn += took < avail ? took : avail;
             }
         }
         const bool busy = L.phase == 1 || L.phase == 2;
@@ -2248,114 +2282,161 @@ struct IrLaunch {
     unsigned long long *fdisc;      // bits of the largest discarded fraction
 };
 
-// One wave per emitting leaf: its spectrum (leaf_emission_spectrum with the
-// band cut, src/dust_emis_transport_mod.f90:372-413) in the wave's LDS, the
-// trapezoid cumulative summed in the reference's order by one lane, then the
-// leaf's packets of this batch, one per lane (launch_and_transport's launch,
-// :489-544): uniform position in the leaf, isotropic direction, wavelength
-// bin by binary search, continuous wavelength inside it, the grain optics at
-// the photon energy.  Draws 0-5 of key (pass << 40 | k) on stream 3.
+// A leaf's emission spectrum in the wave's LDS (leaf_emission_spectrum with
+// the band cut, src/dust_emis_transport_mod.f90:372-413, and
+// emission_cumulative, :447-457): pdf(k) = s_abs B_lambda(T_d), the trapezoid
+// terms in parallel, their running sum in the reference's order as a
+// wave-uniform chain over lane reads.  Returns esum = cdf(nl) (0: empty).
+__device__ double ir_leaf_spectrum(const IrLaunch &q, double T, double *pdf, double *cdf, int lane)
+{
+#pragma clang fp contract(off)
+    const int n = q.nl;
+    if (!(T > 0.0)) return 0.0;
+    for (int k = lane; k < n; k += 64) pdf[k] = q.sabs[k] * planck_lambda(q.lam[k], T);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double full = 0.0;
+    if (q.nzero > 0) {                                   // emission before the cut
+        for (int k = lane; k < n - 1; k += 64)
+            full += 0.5 * (pdf[k] + pdf[k + 1]) * (q.lam[k + 1] - q.lam[k]);
+        full = wave_sum(full);
+        for (int k = lane; k < q.nzero; k += 64) pdf[k] = 0.0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // term j of lane l is t_{64 j + l}; c_{k+1} = c_k + t_k in order of k
+    constexpr int kJ = (MC_NLAM_BB + 63) / 64;
+    double t[kJ], cv[kJ];
+#pragma unroll
+    for (int j = 0; j < kJ; ++j) {
+        const int k = 64 * j + lane;
+        t[j] = k < n - 1 ? 0.5 * (pdf[k] + pdf[k + 1]) * (q.lam[k + 1] - q.lam[k]) : 0.0;
+        cv[j] = 0.0;
+    }
+    double c = 0.0;
+#pragma unroll
+    for (int j = 0; j < kJ; ++j) {
+        const int lmax = min(64, n - 1 - 64 * j);
+        const int lo32 = __double2loint(t[j]), hi32 = __double2hiint(t[j]);
+        for (int l = 0; l < lmax; ++l) {
+            const double tk = __hiloint2double(__builtin_amdgcn_readlane(hi32, l),
+                                               __builtin_amdgcn_readlane(lo32, l));
+            c = c + tk;
+            if (lane == l) cv[j] = c;                    // cdf(64 j + l + 1), 0-based k + 1
+        }
+    }
+    cdf[0] = 0.0;
+#pragma unroll
+    for (int j = 0; j < kJ; ++j) {
+        const int k = 64 * j + lane;
+        if (k < n - 1) cdf[k + 1] = cv[j];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const double esum = c;
+    if (esum > 0.0 && q.nzero > 0 && lane == 0) {
+        const double f = (full - esum) / full;
+        if (f > 0.0) atomicMax(q.fdisc, (unsigned long long)__double_as_longlong(f));
+    }
+    return esum;
+}
+
+constexpr int kIrChunk = 256;    // packets per wave of the infrared launch
+
+// first leaf in [lo, n) whose last packet index is >= k (wave-uniform)
+__device__ __forceinline__ int ir_leaf_of(const int64_t *khi, int lo, int n, int64_t k)
+{
+    int hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (khi[mid] >= k) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// One wave per range of kIrChunk consecutive packets of the batch, so a leaf
+// that emits many packets is launched by many waves: for each leaf in the
+// range its spectrum (above), then its packets of the range one per lane
+// (launch_and_transport's launch, src/dust_emis_transport_mod.f90:489-544):
+// uniform position in the leaf, isotropic direction, wavelength bin by binary
+// search, continuous wavelength inside it, the grain optics at the photon
+// energy.  Draws 0-5 of key (pass << 40 | k) on stream 3.  Packets of a leaf
+// whose spectrum is empty get leaf -2 (dropped by the walk).
 __global__ __launch_bounds__(kBlock) void k_ir_launch(KArgs a, IrLaunch q)
 {
 #pragma clang fp contract(off)
     __shared__ double s_pdf[kBlock / 64][MC_NLAM_BB];
     __shared__ double s_cdf[kBlock / 64][MC_NLAM_BB];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int il = blockIdx.x * (kBlock / 64) + wv;
-    if (il >= a.g.nleaf) return;
-    const int64_t klo = il > 0 ? q.khi[il - 1] + 1 : 1, khi = q.khi[il];
-    const int64_t lo = klo > q.b0 ? klo : q.b0, hi = khi < q.b1 ? khi : q.b1;
-    if (lo > hi) return;                                  // wave-uniform
+    const int64_t k0 = q.b0 + ((int64_t)blockIdx.x * (kBlock / 64) + wv) * kIrChunk;
+    if (k0 > q.b1) return;
+    const int64_t k1 = k0 + kIrChunk - 1 < q.b1 ? k0 + kIrChunk - 1 : q.b1;
     const PacketBuf &pk = a.pk;
-    const int n = q.nl;
+    const int n = q.nl, nleaf = a.g.nleaf;
     double *pdf = s_pdf[wv], *cdf = s_cdf[wv];
-    const double T = q.tdust[il];
-    bool empty = !(T > 0.0);
-    if (!empty) {
-        for (int k = lane; k < n; k += 64) pdf[k] = q.sabs[k] * planck_lambda(q.lam[k], T);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        double full = 0.0;
-        if (q.nzero > 0) {                                // emission before the cut
-            for (int k = lane; k < n - 1; k += 64)
-                full += 0.5 * (pdf[k] + pdf[k + 1]) * (q.lam[k + 1] - q.lam[k]);
-            full = wave_sum(full);
-            for (int k = lane; k < q.nzero; k += 64) pdf[k] = 0.0;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int il = ir_leaf_of(q.khi, 0, nleaf, k0);
+    for (int64_t ks = k0; ks <= k1;) {
+        const int64_t kh = q.khi[il] < k1 ? q.khi[il] : k1;
+        const double esum = ir_leaf_spectrum(q, q.tdust[il], pdf, cdf, lane);
+        const bool empty = !(esum > 0.0);
+        double cx, cy, cz, half;
+        if (a.g.car) {
+            const int ix = il % a.g.nx, iy = (il / a.g.nx) % a.g.ny, iz = il / (a.g.nx * a.g.ny);
+            cx = a.g.xmin + ((double)ix + 0.5) * a.g.dcell;
+            cy = a.g.ymin + ((double)iy + 0.5) * a.g.dcell;
+            cz = a.g.zmin + ((double)iz + 0.5) * a.g.dcell;
+            half = 0.5 * a.g.dcell;
+        } else {
+            const double4 G = a.g.node_gd[a.g.leaf_node[il]];
+            cx = G.x; cy = G.y; cz = G.z; half = G.w;
         }
-        // trapezoid terms in parallel, the running sum in order by lane 0
-        for (int k = lane; k < n - 1; k += 64)
-            cdf[k + 1] = 0.5 * (pdf[k] + pdf[k + 1]) * (q.lam[k + 1] - q.lam[k]);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (lane == 0) {
-            double c = 0.0;
-            cdf[0] = 0.0;
-            for (int k = 1; k < n; ++k) {
-                c = c + cdf[k];
-                cdf[k] = c;
+        for (int64_t k = ks + lane; k <= kh; k += 64) {
+            const int64_t r = k - q.b0;
+            if (empty) {
+                pk.leaf[r] = -2;
+                continue;
             }
+            const uint64_t key = ((uint64_t)q.pass << 40) | (uint64_t)k;
+            pk.x[r] = cx + (2.0 * rng_u(a.seed, 3, key, 0) - 1.0) * half;
+            pk.y[r] = cy + (2.0 * rng_u(a.seed, 3, key, 1) - 1.0) * half;
+            pk.z[r] = cz + (2.0 * rng_u(a.seed, 3, key, 2) - 1.0) * half;
+            const double cost = 2.0 * rng_u(a.seed, 3, key, 3) - 1.0;
+            const double sint = sqrt(1.0 - cost * cost);
+            const double phi = MC_TWOPI * rng_u(a.seed, 3, key, 4);
+            pk.kx[r] = sint * cos(phi);
+            pk.ky[r] = sint * sin(phi);
+            pk.kz[r] = cost;
+            const double u = rng_u(a.seed, 3, key, 5) * cdf[n - 1];
+            int blo = 1, bhi = n - 1;                         // 1-based bins
+            while (blo < bhi) {
+                const int mid = (blo + bhi) >> 1;
+                if (u <= cdf[mid]) bhi = mid;
+                else blo = mid + 1;
+            }
+            const int j = blo - 1;
+            const double ulo = cdf[j];
+            const double ubin = cdf[j + 1] > ulo ? (u - ulo) / (cdf[j + 1] - ulo) : 0.0;
+            const double lam = sample_linear_bin(q.lam[j], q.lam[j + 1], pdf[j], pdf[j + 1], ubin);
+            double sa, ss, gg;
+            dust_at(a, 1.23984 / lam, sa, ss, gg);
+            pk.E[r] = lam;
+            double *sg = pk.sig + 6 * r;
+            sg[0] = 0.0; sg[1] = 0.0; sg[2] = 0.0;
+            sg[3] = sa; sg[4] = ss; sg[5] = gg;
+            pk.inu[r] = 0;
+            pk.leaf[r] = il;
         }
+        ks = kh + 1;
+        if (ks > k1) break;
+        // the leaf's spectrum arrays are rewritten by the next leaf
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const double esum = cdf[n - 1];
-        empty = !(esum > 0.0);
-        if (!empty && q.nzero > 0 && lane == 0) {
-            const double f = (full - esum) / full;
-            if (f > 0.0) atomicMax(q.fdisc, (unsigned long long)__double_as_longlong(f));
-        }
-    }
-    double cx, cy, cz, half;
-    if (a.g.car) {
-        const int ix = il % a.g.nx, iy = (il / a.g.nx) % a.g.ny, iz = il / (a.g.nx * a.g.ny);
-        cx = a.g.xmin + ((double)ix + 0.5) * a.g.dcell;
-        cy = a.g.ymin + ((double)iy + 0.5) * a.g.dcell;
-        cz = a.g.zmin + ((double)iz + 0.5) * a.g.dcell;
-        half = 0.5 * a.g.dcell;
-    } else {
-        const double4 G = a.g.node_gd[a.g.leaf_node[il]];
-        cx = G.x; cy = G.y; cz = G.z; half = G.w;
-    }
-    for (int64_t k = lo + lane; k <= hi; k += 64) {
-        const int64_t r = k - q.b0;
-        if (empty) {
-            pk.leaf[r] = -2;
-            continue;
-        }
-        const uint64_t key = ((uint64_t)q.pass << 40) | (uint64_t)k;
-        pk.x[r] = cx + (2.0 * rng_u(a.seed, 3, key, 0) - 1.0) * half;
-        pk.y[r] = cy + (2.0 * rng_u(a.seed, 3, key, 1) - 1.0) * half;
-        pk.z[r] = cz + (2.0 * rng_u(a.seed, 3, key, 2) - 1.0) * half;
-        const double cost = 2.0 * rng_u(a.seed, 3, key, 3) - 1.0;
-        const double sint = sqrt(1.0 - cost * cost);
-        const double phi = MC_TWOPI * rng_u(a.seed, 3, key, 4);
-        pk.kx[r] = sint * cos(phi);
-        pk.ky[r] = sint * sin(phi);
-        pk.kz[r] = cost;
-        const double u = rng_u(a.seed, 3, key, 5) * cdf[n - 1];
-        int blo = 1, bhi = n - 1;                         // 1-based bins
-        while (blo < bhi) {
-            const int mid = (blo + bhi) >> 1;
-            if (u <= cdf[mid]) bhi = mid;
-            else blo = mid + 1;
-        }
-        const int j = blo - 1;
-        const double ulo = cdf[j];
-        const double ubin = cdf[j + 1] > ulo ? (u - ulo) / (cdf[j + 1] - ulo) : 0.0;
-        const double lam = sample_linear_bin(q.lam[j], q.lam[j + 1], pdf[j], pdf[j + 1], ubin);
-        double sa, ss, gg;
-        dust_at(a, 1.23984 / lam, sa, ss, gg);
-        pk.E[r] = lam;
-        double *sg = pk.sig + 6 * r;
-        sg[0] = 0.0; sg[1] = 0.0; sg[2] = 0.0;
-        sg[3] = sa; sg[4] = ss; sg[5] = gg;
-        pk.inu[r] = 0;
-        pk.leaf[r] = il;
+        il = ir_leaf_of(q.khi, il + 1, nleaf, ks);
     }
 }
 
@@ -2899,7 +2980,6 @@ int mc_run_dust_emission(mc_ctx *c, double rep[MC_DUSTRT_NREPORT])
     std::vector<double> hw(nslot);
     const int nblk = (nl + kBlock - 1) / kBlock;
     const int ntile = (nl + kScanTile - 1) / kScanTile;
-    const int nwblk = (nl + kBlock / 64 - 1) / (kBlock / 64);
     IrLaunch q;
     q.lam = b.d_lam;
     q.sabs = b.d_sabs;
@@ -2953,7 +3033,8 @@ int mc_run_dust_emission(mc_ctx *c, double rep[MC_DUSTRT_NREPORT])
             a.first_id = ((int64_t)iter << 40) + b0;
             a.npk = b1 - b0 + 1;
             MC_HIP(hipEventRecord(c->ev[1], st));
-            k_ir_launch<<<nwblk, kBlock, 0, st>>>(a, q);
+            const int64_t nwave = (b1 - b0 + kIrChunk) / kIrChunk;
+            k_ir_launch<<<(unsigned)((nwave + kBlock / 64 - 1) / (kBlock / 64)), kBlock, 0, st>>>(a, q);
             MC_HIP(hipGetLastError());
             k_reset_counter<<<1, 1, 0, st>>>(c->st.counter);
             MC_HIP(hipEventRecord(c->ev[2], st));
