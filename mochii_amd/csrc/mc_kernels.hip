@@ -1255,6 +1255,14 @@ __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, 
 
 constexpr int kWalkChunk = 256;   // records a launch-order wave takes at once
 
+// lane 0's value as a wave-uniform (scalar) value
+__device__ __forceinline__ unsigned long long wave_first(unsigned long long v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 template <int P, int G, int M>
 __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
 {
@@ -1277,22 +1285,22 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
     bool dry = false;
     for (;;) {
         const unsigned long long need = __ballot(L.phase == 0);
+        unsigned long long r = 0;
+        bool take = false;
         if (a.coherent) {
             // direction-sorted records: the wave takes 64 consecutive records
             // at once, when every lane is idle, so its packets stay together
             if (need == ~0ull) {
                 unsigned long long base = 0;
                 if (lane == 0) base = atomicAdd(a.s.counter, 64ull);
-                base = __shfl(base, 0);
-                const unsigned long long r = base + (unsigned long long)lane;
-                if ((int64_t)r < a.npk) load_packet<P, G, M>(a, L, (int64_t)r, cn);
-                else L.phase = 3;
+                r = wave_first(base) + (unsigned long long)lane;
+                take = true;
             }
         } else if (need != 0ull) {
             if (!dry && qn >= qe) {
                 unsigned long long base = 0;
                 if (lane == 0) base = atomicAdd(a.s.counter, (unsigned long long)kWalkChunk);
-                base = __shfl(base, 0);
+                base = wave_first(base);
                 qn = base;
                 qe = base + kWalkChunk;
                 dry = (int64_t)base >= a.npk;
@@ -1303,13 +1311,16 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
                 const unsigned long long avail = qe - qn;
                 const unsigned long long rank = __popcll(need & ((1ull << lane) - 1ull));
                 if (L.phase == 0 && rank < avail) {
-                    const unsigned long long r = qn + rank;
-                    if ((int64_t)r < a.npk) load_packet<P, G, M>(a, L, (int64_t)r, cn);
-                    else L.phase = 3;
+                    r = qn + rank;
+                    take = true;
                 }
                 const unsigned long long took = __popcll(need);
                 qn += took < avail ? took : avail;
             }
+        }
+        if (take) {
+            if ((int64_t)r < a.npk) load_packet<P, G, M>(a, L, (int64_t)r, cn);
+            else L.phase = 3;
         }
         const bool busy = L.phase == 1 || L.phase == 2;
         // launch-order walks refill idle lanes until every lane is drained
