@@ -43,6 +43,12 @@ Prints ONE JSON line.  Extra fields:
     condenses a rocprofv3 --kernel-trace CSV of a bench run into per-kernel
     statistics over all dispatches and over the last K dispatches of each
     kernel (the timed steps of the form that launches it).
+
+  python bench.py --copy-calibration ... runs a plain device copy of 1 GiB
+    (3 times) before the benchmark, so that a rocprofv3 --pmc run of
+    FETCH_SIZE / WRITE_SIZE can be scaled by what the counter reads for a
+    copy of known size; --pmc-summary COUNTER_CSV [--kernel NAME] then
+    prints the per-dispatch counter means of the copy and of the kernel.
 """
 import argparse
 import json
@@ -87,6 +93,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--ir-packets", type=int, default=10_000_000)
+    ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--pmc-summary", default=None)
+    ap.add_argument("--kernel", default="k_walk<1, 1, 1>")
     ap.add_argument("--no-ir", action="store_true")
     return ap.parse_args()
 
@@ -113,6 +123,40 @@ def trace_summary(path, last):
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+COPY_BYTES = 1 << 30
+
+
+def copy_calibration():
+    """A plain device-to-device copy of COPY_BYTES, three times: the
+    reference reading for the HBM byte counters."""
+    import torch
+    src = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def pmc_summary(path, kernel):
+    """Mean counter value per dispatch (rocprofv3 --pmc counter_collection
+    CSV) of the calibration copy and of the named kernel, and the kernel's
+    bytes per dispatch scaled by the copy's reading."""
+    import csv
+    import numpy as np
+    per = {}
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            key = (row["Kernel_Name"], row["Counter_Name"])
+            per.setdefault(key, {}).setdefault(row.get("Dispatch_Id", len(per)), 0.0)
+            per[key][row.get("Dispatch_Id")] += float(row["Counter_Value"])
+    for (name, ctr), d in sorted(per.items()):
+        if "copy" in name.lower() or kernel in name:
+            v = np.array(list(d.values()))
+            print(f"{name[:72]:72s} {ctr:12s} dispatches {v.size:5d} mean {v.mean():14.1f} "
+                  f"last {v[-1]:14.1f}")
 
 
 def build_workload(name, packets):
@@ -190,7 +234,8 @@ def run_form(name, params, nH, octree, args, mlp=None):
     import mochii_amd as mc
     from mochii_amd import run_model
     prec = mc.MC_FP64 if name == "fp64" else mc.MC_MIXED
-    warm = mc.lib.MC_WARM_NONE if name == "fp64" else mc.lib.MC_WARM_MLP
+    warm = (mc.lib.MC_WARM_NONE if name == "fp64" else
+            mc.lib.MC_WARM_MLP if mlp is not None else mc.lib.MC_WARM_NEWTON)
     thermal = bool(params.get("solve_te"))
     m = make_model(params, nH, octree, prec, warm if thermal else None, mlp if thermal else None)
 
@@ -266,7 +311,8 @@ def one_step(name, params, nH, octree, state, mlp):
     """One iteration (pass + cell update) of a form from a given state."""
     import mochii_amd as mc
     prec = mc.MC_FP64 if name == "fp64" else mc.MC_MIXED
-    warm = mc.lib.MC_WARM_NONE if name == "fp64" else mc.lib.MC_WARM_MLP
+    warm = (mc.lib.MC_WARM_NONE if name == "fp64" else
+            mc.lib.MC_WARM_MLP if mlp is not None else mc.lib.MC_WARM_NEWTON)
     thermal = bool(params.get("solve_te"))
     m = make_model(params, nH, octree, prec, warm if thermal else None, mlp if thermal else None)
     m.set_state(**{k: state[k] for k in ("xHI", "xHeI", "xHeII", "ne", "Te")})
@@ -357,6 +403,11 @@ def main():
     if args.trace_summary:
         trace_summary(args.trace_summary, args.steps)
         return
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary, args.kernel)
+        return
+    if args.copy_calibration:
+        copy_calibration()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
               "(the pass would shard packets over GPUs and all-reduce the tallies)",
@@ -366,7 +417,7 @@ def main():
     params, nH, octree, grid, desc = build_workload(args.workload, args.packets)
     thermal = bool(params.get("solve_te"))
     mlp, train_info = None, None
-    if thermal:
+    if thermal and not args.no_surrogate:
         log("[surrogate] collecting cell states and training the warm-start network")
         mlp, train_info = train_surrogate()
         log(f"  {train_info}")

@@ -74,12 +74,29 @@ struct RateSet { double aH, aHe2, aHe3, cH, cHe1, cHe2; };
 
 #include "mc_metal_dev.h"
 
-// Metal cross sections of a lane's packet, threshold order, in the walk's
-// dynamic LDS: [k][kBlock] doubles, k < the registry's transition count.
-__device__ __forceinline__ double &msig(int k)
+// The walk's dynamic LDS, [row][kBlock] doubles per block: rows 0-5 the
+// H/He rate-tally coefficients of each lane's packet (coef), rows 6-8 the
+// lane's energy-ledger sums (led), then its metal cross sections in
+// threshold order (msig), one row per registry transition.
+// Kept out of the lane's registers: the walk's live state is register-bound.
+constexpr int kCoefRows = 9;     // 6 coefficients + 3 ledger sums
+__device__ __forceinline__ double &coef(int k)
 {
     extern __shared__ double mc_msig[];
     return mc_msig[k * kBlock + threadIdx.x];
+}
+
+// the lane's energy-ledger sums {emitted, absorbed, escaped} (rows 6-8)
+__device__ __forceinline__ double &led(int k)
+{
+    extern __shared__ double mc_msig[];
+    return mc_msig[(6 + k) * kBlock + threadIdx.x];
+}
+
+__device__ __forceinline__ double &msig(int k)
+{
+    extern __shared__ double mc_msig[];
+    return mc_msig[(kCoefRows + k) * kBlock + threadIdx.x];
 }
 
 // ---------------------------------------------------------------------------
@@ -371,7 +388,6 @@ struct Lane {
     double x, y, z, kx, ky, kz;      // flight origin and direction
     double wgt, E;
     double sHI, sHeI, sHeII, dabs, dsca, dg;
-    double coeff[6];
     double wl;
     // walk state, FP64 form: DDA
     int ix, iy, iz, sx, sy, sz;
@@ -443,9 +459,9 @@ __device__ __forceinline__ float opacity_f(const KArgs &a, const Lane &L, int il
 }
 
 struct Counters {
-    double e_emit, e_abs, e_esc;
-    unsigned long long sd, ss;
-    unsigned long long smet;     // metal transitions summed over scored segments
+    double e_emit, e_abs, e_esc; // infrared band only; the ionizing ledger is in LDS (led)
+    uint32_t sd, ss;             // per lane and batch: far below 2^32
+    uint32_t smet;               // metal transitions summed over scored segments
     // infrared band: packets ledgered (every one at weight 1) and the
     // rounding errors of the extinguished / escaped sums
     unsigned long long n_emit;
@@ -475,7 +491,7 @@ __device__ __forceinline__ void score(const KArgs &, Lane &L, int il, double pl,
     L.s_il = il;
     L.s_pl = pl;
     if (L.nscatt > 0) cn.ss++; else cn.sd++;
-    cn.smet += (unsigned long long)L.nmet;
+    cn.smet += (uint32_t)L.nmet;
 }
 
 // tally destination of a leaf: its own rows, or one of the replicas of a hot
@@ -515,8 +531,10 @@ __device__ __forceinline__ void tally_one(const KArgs &a, const Lane &L, int il,
     tally_rows(a, il, h, j);
     unsafeAtomicAdd(j + L.inu, pl);
 #pragma unroll
-    for (int k = 0; k < 6; ++k)
-        if (L.coeff[k] != 0.0) unsafeAtomicAdd(h + k, pl * L.coeff[k]);
+    for (int k = 0; k < 6; ++k) {
+        const double ck = coef(k);
+        if (ck != 0.0) unsafeAtomicAdd(h + k, pl * ck);
+    }
     if (a.add_dust && L.dabs != 0.0) unsafeAtomicAdd(h + 6, pl * L.dabs);
     if constexpr (M == 1) {
         double *r = a.mraw + (size_t)il * a.ntr * 2;
@@ -571,7 +589,7 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
             tally_rows(a, lil, h, j);
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
-                const double v = in ? L.s_pl * L.coeff[k] : 0.0;
+                const double v = in ? L.s_pl * coef(k) : 0.0;
                 if (__ballot(v != 0.0)) {
                     const double sum = wave_sum(v);
                     if (lane == leader) unsafeAtomicAdd(h + k, sum);
@@ -971,9 +989,9 @@ __device__ __forceinline__ void end_edge(const KArgs &a, Lane &L, double tau_edg
         two_sum_acc(cn.e_abs, cn.c_abs, lum * (1.0 - tr));
         two_sum_acc(cn.e_esc, cn.c_esc, lum * tr);
     } else {
-        cn.e_emit += lum;
-        cn.e_abs += lum * (1.0 - tr);
-        cn.e_esc += lum * tr;
+        led(0) += lum;
+        led(1) += lum * (1.0 - tr);
+        led(2) += lum * tr;
     }
     if constexpr (M == 2) {
         // dust_ir_escape_packet (src/dust_ir_band_mod.f90:123-150): the
@@ -1202,12 +1220,14 @@ __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, 
     L.sHI = sg[0]; L.sHeI = sg[1]; L.sHeII = sg[2];
     L.dabs = sg[3]; L.dsca = sg[4]; L.dg = sg[5];
     const double he = L.E * MC_EV2ERG;
-    L.coeff[0] = L.sHI / he;
-    L.coeff[1] = L.sHeI / he;
-    L.coeff[2] = L.sHeII / he;
-    L.coeff[3] = L.sHI * fmax(1.0 - MC_ETH_HI / L.E, 0.0);
-    L.coeff[4] = L.sHeI * fmax(1.0 - MC_ETH_HEI / L.E, 0.0);
-    L.coeff[5] = L.sHeII * fmax(1.0 - MC_ETH_HEII / L.E, 0.0);
+    if constexpr (M != 2) {
+        coef(0) = L.sHI / he;
+        coef(1) = L.sHeI / he;
+        coef(2) = L.sHeII / he;
+        coef(3) = L.sHI * fmax(1.0 - MC_ETH_HI / L.E, 0.0);
+        coef(4) = L.sHeI * fmax(1.0 - MC_ETH_HEI / L.E, 0.0);
+        coef(5) = L.sHeII * fmax(1.0 - MC_ETH_HEII / L.E, 0.0);
+    }
     if constexpr (M == 1) {
         // species_packet_cross_sections (src/species_mod.f90:263-275)
         int n = 0;
@@ -1276,7 +1296,12 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
     L.phase = 0;
     L.s_il = -1;
     L.nmet = 0;
-    Counters cn = {0.0, 0.0, 0.0, 0ull, 0ull, 0ull, 0ull, 0.0, 0.0};
+    Counters cn = {0.0, 0.0, 0.0, 0u, 0u, 0u, 0ull, 0.0, 0.0};
+    if constexpr (M != 2) {
+        led(0) = 0.0;
+        led(1) = 0.0;
+        led(2) = 0.0;
+    }
     const int lane = threadIdx.x & 63;
     // launch-order walks: the wave's own queue of records [qn, qe), taken
     // from the batch counter kWalkChunk at a time, so that refilling a lane
@@ -1352,6 +1377,11 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
             w[3] = cn.e_esc; w[4] = cn.c_esc;
         }
     }
+    if constexpr (M != 2) {
+        cn.e_emit = led(0);
+        cn.e_abs = led(1);
+        cn.e_esc = led(2);
+    }
     // wave reduction of the ledger and segment counts, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
         cn.e_emit += __shfl_down(cn.e_emit, off);
@@ -1367,9 +1397,9 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
         atomicAdd(a.s.ledger + 2, cn.e_esc);
     }
     if (lane == 0) {
-        atomicAdd(a.s.segs + 0, cn.sd);
-        atomicAdd(a.s.segs + 1, cn.ss);
-        if (cn.smet) atomicAdd(a.s.segs + 2, cn.smet);
+        atomicAdd(a.s.segs + 0, (unsigned long long)cn.sd);
+        atomicAdd(a.s.segs + 1, (unsigned long long)cn.ss);
+        if (cn.smet) atomicAdd(a.s.segs + 2, (unsigned long long)cn.smet);
     }
 }
 
@@ -2657,7 +2687,7 @@ int mc_run_pass(mc_ctx *c)
                            : (const void *)k_walk<MC_FP64, kFaceExit, 0>)
                   : (mixed ? (const void *)k_walk<MC_MIXED, kCarDDA, 0>
                            : (const void *)k_walk<MC_FP64, kCarDDA, 0>);
-    const size_t dlds = metals ? (size_t)c->ntr * kBlock * sizeof(double) : 0;
+    const size_t dlds = (size_t)(kCoefRows + (metals ? c->ntr : 0)) * kBlock * sizeof(double);
     const int nblk = persistent_blocks(kw, dlds);
     const int nlds = c->car ? 0 : c->n_lds_nodes;
     // one batch of packet records: launch kernel, then the persistent walk
