@@ -97,6 +97,7 @@ def parse():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--pmc-summary", default=None)
     ap.add_argument("--kernel", default="k_walk<1, 1, 1>")
+    ap.add_argument("--per-pass", type=int, default=5)
     ap.add_argument("--no-ir", action="store_true")
     return ap.parse_args()
 
@@ -140,23 +141,39 @@ def copy_calibration():
     del src, dst
 
 
-def pmc_summary(path, kernel):
-    """Mean counter value per dispatch (rocprofv3 --pmc counter_collection
-    CSV) of the calibration copy and of the named kernel, and the kernel's
-    bytes per dispatch scaled by the copy's reading."""
+def pmc_summary(path, kernel, per_pass=5):
+    """HBM bytes of the named kernel from a rocprofv3 --pmc run of
+    FETCH_SIZE or WRITE_SIZE (KiB per dispatch): the counter of the three
+    calibration copies of COPY_BYTES gives the scale, the last per_pass
+    dispatches of the kernel are one transport pass."""
     import csv
     import numpy as np
-    per = {}
-    with open(path) as f:
-        for row in csv.DictReader(f):
-            key = (row["Kernel_Name"], row["Counter_Name"])
-            per.setdefault(key, {}).setdefault(row.get("Dispatch_Id", len(per)), 0.0)
-            per[key][row.get("Dispatch_Id")] += float(row["Counter_Value"])
-    for (name, ctr), d in sorted(per.items()):
-        if "copy" in name.lower() or kernel in name:
-            v = np.array(list(d.values()))
-            print(f"{name[:72]:72s} {ctr:12s} dispatches {v.size:5d} mean {v.mean():14.1f} "
-                  f"last {v[-1]:14.1f}")
+    rows = list(csv.DictReader(open(path)))
+    ctr = rows[0]["Counter_Name"]
+    cp = sorted((float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]),
+                reverse=True)[:3]
+    expect = COPY_BYTES / 1024.0                      # read once, written once
+    scale = expect / float(np.mean(cp))
+    v = [float(r["Counter_Value"]) for r in rows if kernel in r["Kernel_Name"]]
+    last = v[-per_pass:]
+    total = sum(last) * 1024.0 * scale
+    print(f"counter {ctr}: calibration copy of {COPY_BYTES} B reads {np.round(cp, 1)} KiB "
+          f"(expected {expect:.0f}), scale {scale:.4f}")
+    print(f"{kernel}: {len(v)} dispatches; last {per_pass} (one pass) KiB {np.round(last)}")
+    print(f"{ctr.lower()}_bytes_per_pass: {total:.6e}")
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_walk_hii_octree_metals_1e7_traffic.md")
+
+
+def measured_traffic(workload):
+    """HBM bytes per pass of the walk kernel, from the committed summary of
+    the two counter runs (FETCH_SIZE, WRITE_SIZE) of this workload."""
+    import re
+    if workload != "hii_octree" or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    m = re.search(r"^traffic_bytes_per_pass:\s*([0-9.eE+-]+)", open(TRAFFIC_PROFILE).read(), re.M)
+    return float(m.group(1)) if m else None
 
 
 def build_workload(name, packets):
@@ -404,7 +421,7 @@ def main():
         trace_summary(args.trace_summary, args.steps)
         return
     if args.pmc_summary:
-        pmc_summary(args.pmc_summary, args.kernel)
+        pmc_summary(args.pmc_summary, args.kernel, args.per_pass)
         return
     if args.copy_calibration:
         copy_calibration()
@@ -475,7 +492,10 @@ def main():
         "surrogate_training": train_info,
         "error_mixed_mlp_vs_fp64": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
+                     "traffic_basis": "HBM bytes per pass (all 5 walk launches), rocprofv3 "
+                                      "FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy "
+                                      "(profiles/k_walk_hii_octree_metals_1e7_traffic.md)",
                      "kernel": WALK_KERNEL[(grid, "mixed_mlp")].replace(
                          "M>", "1>" if params.get("metal_eth") else "0>"),
                      "bytes_per_segment": bps,
