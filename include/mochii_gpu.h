@@ -267,7 +267,7 @@ int mc_gas_thermal_update(mc_ctx *ctx, double *max_dx, double *max_dte, double *
  * only a sign-change bracket of the same width thi/tlo - 1 < 1e-5, falling
  * back to bisection from the bracket it has when Newton stalls.  With the
  * metal registry and two-way charge exchange, a warm mode also ends a trial
- * solve's n_e iteration after 24 consecutive passes in which the leaf's
+ * solve's n_e iteration after 2 consecutive passes in which the leaf's
  * hydrogen is unresolved (x_HII < 1e-10, leaves without ionising flux): there
  * the reference's iteration wanders in a noise band of n_e until its
  * 200-pass cap. */

@@ -2624,7 +2624,7 @@ MetalSw make_sw(mc_ctx *c, bool thermal = false)
     m.cx_two_way = c->p.charge_exchange == MC_CX_TWO_WAY;
     m.metal_heat = c->p.metal_heat != 0;
     m.metal_ff = c->p.metal_freefree != 0;
-    m.noise_cap = thermal && c->warm_mode != MC_WARM_NONE ? 24 : 0;
+    m.noise_cap = thermal && c->warm_mode != MC_WARM_NONE ? 2 : 0;
     return m;
 }
 
