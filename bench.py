@@ -39,10 +39,11 @@ Prints ONE JSON line.  Extra fields:
                form: packets/s, walk time, segments, reabsorbed fraction,
                closure, and the infrared walk's algorithmic bandwidth
 
-  python bench.py --trace-summary KERNEL_TRACE.csv [--steps K]
+  python bench.py --trace-summary KERNEL_TRACE.csv [--steps K] [--skip-last S]
     condenses a rocprofv3 --kernel-trace CSV of a bench run into per-kernel
     statistics over all dispatches and over the last K dispatches of each
-    kernel (the timed steps of the form that launches it).
+    kernel before its final S (for the walk of a default run, K = 5 x steps
+    and S = 10 select the timed steps of the form that launches it).
 
   python bench.py --copy-calibration ... runs a plain device copy of 1 GiB
     (3 times) before the benchmark, so that a rocprofv3 --pmc run of
@@ -98,12 +99,17 @@ def parse():
     ap.add_argument("--pmc-summary", default=None)
     ap.add_argument("--kernel", default="k_walk<1, 1, 1>")
     ap.add_argument("--per-pass", type=int, default=5)
+    ap.add_argument("--skip-last", type=int, default=0)
     ap.add_argument("--no-ir", action="store_true")
     return ap.parse_args()
 
 
-def trace_summary(path, last):
-    """Per-kernel duration statistics from a rocprofv3 kernel-trace CSV."""
+def trace_summary(path, last, skip=0):
+    """Per-kernel duration statistics from a rocprofv3 kernel-trace CSV; the
+    'last' columns are the last K dispatches of each kernel before its final
+    `skip` ones (a default bench run ends with one pass of each form from a
+    common state and one ionizing pass of each form for the infrared leg,
+    5 walk dispatches each, after the timed steps)."""
     import csv
     import numpy as np
     per = {}
@@ -117,7 +123,7 @@ def trace_summary(path, last):
     for name, v in rows:
         v.sort()
         d = np.array([x for _, x in v], dtype=np.float64)
-        tail = d[-last:]
+        tail = d[:d.size - skip][-last:] if d.size > skip else d[-last:]
         print(f"{name[:70]:70s} {d.size:6d} {d.mean() / 1e3:10.1f} {100 * d.sum() / total:6.2f} "
               f"{tail.size:5d} {tail.mean() / 1e3:12.1f}")
 
@@ -418,7 +424,7 @@ def cpu_baseline(params, nH, octree, state, seconds):
 def main():
     args = parse()
     if args.trace_summary:
-        trace_summary(args.trace_summary, args.steps)
+        trace_summary(args.trace_summary, args.steps, args.skip_last)
         return
     if args.pmc_summary:
         pmc_summary(args.pmc_summary, args.kernel, args.per_pass)
