@@ -1960,13 +1960,23 @@ __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCo
                 }
             };
             double tlo = tmin, thi = tmax;
-            int evals = 2, fallback = 0, bracketed = 0;
+            int evals = 1, fallback = 0, bracketed = 0;
+            // the warm-started forms keep the state of an edge evaluation that
+            // decides T_e (the leaf is pinned at te_min or te_max) instead of
+            // evaluating that edge again, and skip te_max once te_min decides
+            bool at_te = false;
             const double nlo = net_rate(cl, c, tlo, caseA, y, xHI, xHeI, xHeII, ne);
-            const double nhi = net_rate(cl, c, thi, caseA, y, xHI, xHeI, xHeII, ne);
+            double nhi = -1.0;
+            if (W == MC_WARM_NONE || nlo > 0.0) {
+                nhi = net_rate(cl, c, thi, caseA, y, xHI, xHeI, xHeII, ne);
+                ++evals;
+            }
             if (nlo <= 0.0) {
                 te = tlo;
+                at_te = W != MC_WARM_NONE;
             } else if (nhi >= 0.0) {
                 te = thi;
+                at_te = W != MC_WARM_NONE;
             } else if constexpr (W == MC_WARM_NONE) {
                 bracketed = 1;
                 for (int it = 0; it < 60; ++it) {
@@ -2031,8 +2041,10 @@ __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCo
                 }
                 te = sqrt(tlo * thi);
             }
-            net_rate(cl, c, te, caseA, y, xHI, xHeI, xHeII, ne);
-            ++evals;
+            if (!at_te) {
+                net_rate(cl, c, te, caseA, y, xHI, xHeI, xHeII, ne);
+                ++evals;
+            }
             const double oHI = s.xHI[il], oT = s.Te[il];
             const double nHI = (1.0 - w) * oHI + w * xHI;
             const double nHeI = (1.0 - w) * s.xHeI[il] + w * xHeI;
