@@ -7,6 +7,15 @@
 !   drive_stromgren [n] [npk]     runs the model and prints the iteration log,
 !                                 the energy ledger and the sum of Gamma_HI
 !                                 over the leaves
+!   drive_stromgren dust [n] [npk] [nir]
+!                                 the same sphere with grains (power-law
+!                                 extinction, rhokap = n_H C_ext(V) d2cm):
+!                                 one ionizing pass, then the block of
+!                                 src/main.f90:242-260 (dust_temp_setup /
+!                                 _compute, dust_emis_transport_setup,
+!                                 transport_dust_emission with nir packets
+!                                 per pass); prints the report and the sums of
+!                                 Heat_dust and T_dust
 program drive_stromgren
   use, intrinsic :: iso_c_binding
   use mochii_gpu
@@ -19,13 +28,20 @@ program drive_stromgren
   integer(c_int32_t) :: nleaf, nnu
   real(c_double) :: Ltot
   character(len=32) :: arg
-  integer :: it, n, rc
+  integer :: it, n, rc, iarg
   integer(c_int64_t) :: npk
+  logical :: dust
+  real(c_double) :: nir
 
   call get_command_argument(1, arg)
   if (trim(arg) == 'layout') then
      call print_layout()
      call element_layout()
+     stop
+  end if
+  dust = trim(arg) == 'dust'
+  if (dust) then
+     call run_dust()
      stop
   end if
   n = 16;  npk = 20000_c_int64_t
@@ -59,6 +75,52 @@ program drive_stromgren
   call check(mc_finalize(ctx), 'mc_finalize')
 
 contains
+
+  ! The dusty sphere and the dust infrared band (src/main.f90:242-260).
+  subroutine run_dust()
+    integer, parameter :: nt = 600
+    real(c_double) :: lam(nt), alb(nt), gc(nt), cext(nt), report(MC_DUSTRT_NREPORT)
+    real(c_double), allocatable :: rk(:), heat(:), td(:)
+    integer :: k
+    n = 16;  npk = 20000_c_int64_t;  nir = 1d5
+    call get_command_argument(2, arg)
+    if (len_trim(arg) > 0) read(arg, *) n
+    call get_command_argument(3, arg)
+    if (len_trim(arg) > 0) read(arg, *) npk
+    call get_command_argument(4, arg)
+    if (len_trim(arg) > 0) read(arg, *) nir
+    p = mc_params_default()
+    p%nphotons = npk
+    p%tstar(1) = 4.0d4;  p%src_lum(1) = 3.177837d38
+    p%nx = n;  p%ny = n;  p%nz = n
+    p%xmax = 4d0;  p%ymax = 4d0;  p%zmax = 4d0;  p%distance2cm = 3.0856776d18
+    p%He_abund = 0d0;  p%ion_add_dust = 1;  p%ion_dust_scatter = 0;  p%lambda_ref = 0.55d0
+    p%dust_no_photons = nir;  p%dust_niter = 2;  p%dust_tol = 1d-12
+    do k = 1, nt                       ! C_ext ~ lambda^-1.6 from 0.01 to 1e4 um
+       lam(k) = 1d-2 * (1d6)**(real(k - 1, c_double) / real(nt - 1, c_double))
+       alb(k) = 0.35d0;  gc(k) = 0.55d0
+       cext(k) = 4.868d-22 * (lam(k) / 0.55d0)**(-1.6d0)
+    end do
+    call check(mc_init(p, ctx), 'mc_init')
+    call check(mc_info(ctx, nleaf, nnu, Ltot, c_null_ptr), 'mc_info')
+    call check(mc_ion_dust_table(ctx, nt, lam, alb, gc, cext), 'mc_ion_dust_table')
+    allocate(nH(nleaf), rk(nleaf), heat(nleaf), td(nleaf))
+    nH = 100d0
+    rk = nH * 4.868d-22 * p%distance2cm * 3d0
+    call check(mc_gas_state_setup(ctx, nH, c_null_ptr), 'mc_gas_state_setup')
+    call check(mc_state_set(ctx, MC_RHOKAP, rk), 'mc_state_set')
+    call check(mc_ionizing_field_and_rates(ctx, 0), 'mc_ionizing_field_and_rates')
+    call check(mc_dust_temp_setup(ctx), 'dust_temp_setup')
+    call check(mc_dust_temp_compute(ctx), 'dust_temp_compute')
+    call check(mc_dust_emis_transport_setup(ctx), 'dust_emis_transport_setup')
+    call check(mc_transport_dust_emission(ctx, report), 'transport_dust_emission')
+    call check(mc_dust_get(ctx, MC_DUST_HEAT, heat), 'heat_dust')
+    call check(mc_dust_get(ctx, MC_DUST_TEMP, td), 't_dust')
+    write(*, '(a,10es24.16)') 'report', report
+    write(*, '(a,es24.16)') 'sum_heat_dust', sum(heat)
+    write(*, '(a,es24.16)') 'sum_t_dust', sum(td)
+    call check(mc_finalize(ctx), 'mc_finalize')
+  end subroutine run_dust
 
   subroutine check(rc_, what)
     integer(c_int), intent(in) :: rc_

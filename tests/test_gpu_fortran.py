@@ -36,3 +36,37 @@ def test_fortran_driver_matches_python_driver():
     np.testing.assert_allclose(f_led, g.energy_ledger(), rtol=1e-12)
     p_sum = g.rates()["gHI"].sum()
     assert p_sum > 0 and abs(f_sum / p_sum - 1.0) < 1e-10
+
+
+def test_fortran_dust_band_matches_python_driver():
+    """The dust infrared band through the Fortran binding (the block of
+    src/main.f90:242-260) and through Python: the same report, Heat_dust and
+    T_dust."""
+    exe = os.path.join(ROOT, "mochii_amd", "fortran", "_build", "drive_stromgren")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built (amdflang absent at build time)")
+    out = subprocess.run([exe, "dust", "16", "20000", "100000"], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = out.stdout.split("\n")
+    f_rep = [float(v) for v in next(l for l in lines if l.startswith("report")).split()[1:]]
+    f_heat = float(next(l for l in lines if l.startswith("sum_heat_dust")).split()[1])
+    f_td = float(next(l for l in lines if l.startswith("sum_t_dust")).split()[1])
+    nt = 600
+    lam = 1e-2 * np.power(1e6, np.arange(nt) / (nt - 1.0))
+    tab = (lam, np.full(nt, 0.35), np.full(nt, 0.55), 4.868e-22 * (lam / 0.55) ** -1.6)
+    params = dict(nphotons=20000, tstar=[4.0e4], src_lum=[3.177837e38], src_x=[0.0],
+                  src_y=[0.0], src_z=[0.0], nx=16, ny=16, nz=16, xmax=4.0, ymax=4.0, zmax=4.0,
+                  distance2cm=3.0856776e18, He_abund=0.0, ion_add_dust=1, ion_dust_scatter=0,
+                  lambda_ref=0.55, dust_no_photons=1e5, dust_niter=2, dust_tol=1e-12)
+    nH = np.full(16 ** 3, 100.0)
+    g = gpu_model(params, nH, rhokap=nH * 4.868e-22 * 3.0856776e18 * 3.0, dust_table=tab)
+    g.ionizing_field_and_rates()
+    g.dust_temp_setup()
+    g.dust_temp_compute()
+    g.dust_emis_transport_setup()
+    rep = g.transport_dust_emission()
+    assert f_rep[0] == rep["passes"] == 2 and f_rep[6] <= 5e-13
+    np.testing.assert_allclose(f_rep, list(rep.values()), rtol=1e-9, atol=1e-30)
+    assert abs(f_heat / g.dust("heat_dust").sum() - 1.0) < 1e-10
+    assert abs(f_td / g.dust("t_dust").sum() - 1.0) < 1e-10
