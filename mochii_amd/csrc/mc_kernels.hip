@@ -2489,7 +2489,11 @@ __global__ __launch_bounds__(kBlock) void k_ir_launch(KArgs a, IrLaunch q)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        il = ir_leaf_of(q.khi, il + 1, nleaf, ks);
+        // the next emitting leaf is usually among the next few: probe them
+        // before a binary search over the rest
+        int nx = il + 1;
+        while (nx < il + 9 && nx < nleaf - 1 && q.khi[nx] < ks) ++nx;
+        il = q.khi[nx] >= ks ? nx : ir_leaf_of(q.khi, nx, nleaf, ks);
     }
 }
 
