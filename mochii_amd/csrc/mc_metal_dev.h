@@ -260,16 +260,36 @@ __device__ __forceinline__ void m_sums_l(const MCache<MT> &cc, double nH, double
 // hydrogen_neutral_fraction (src/ion_balance_mod.f90:177-208)
 template <int MT, int LAY>
 __device__ double m_hydrogen_neutral_fraction(const MCache<MT> &cc, double nH, double ne,
-                                              double A, double B)
+                                              double A, double B, double xw = -1.0)
 {
     double nem, rin, rout;
-    double xa = 0.0;
-    m_sums_l<MT, LAY>(cc, nH, ne, 0.0, nH, nem, rin, rout);
-    double Ra = -(A + rin);
-    double xb = 1.0;
-    m_sums_l<MT, LAY>(cc, nH, ne, nH, 0.0, nem, rin, rout);
-    double Rb = B + rout;
-    const double scale = A + B + fabs(Ra) + fabs(Rb);
+    double xa = 0.0, Ra, xb = 1.0, Rb, scale = 0.0;
+    bool warm = false;
+    if (xw > 0.0 && xw < 1.0) {
+        // warm-started form: a bracket of +-10% of the distance to the
+        // nearer end around the previous pass's root, when the residual
+        // changes sign across it (else the reference's [0, 1])
+        const double d = 0.1 * fmin(xw, 1.0 - xw);
+        const double wa = xw - d, wb = xw + d;
+        m_sums_l<MT, LAY>(cc, nH, ne, nH * wa, nH * (1.0 - wa), nem, rin, rout);
+        const double ra = wa * (B + rin + rout) - (A + rin), rin_a = rin;
+        m_sums_l<MT, LAY>(cc, nH, ne, nH * wb, nH * (1.0 - wb), nem, rin, rout);
+        const double rb = wb * (B + rin + rout) - (A + rin);
+        if (ra < 0.0 && rb > 0.0) {
+            xa = wa; Ra = ra; xb = wb; Rb = rb;
+            // the tolerance keeps the size of the end-point residuals
+            // |R(0)| + |R(1)| of the full bracket, from the sums at wa, wb
+            scale = 2.0 * (A + B) + rin_a + rout;
+            warm = true;
+        }
+    }
+    if (!warm) {
+        m_sums_l<MT, LAY>(cc, nH, ne, 0.0, nH, nem, rin, rout);
+        Ra = -(A + rin);
+        m_sums_l<MT, LAY>(cc, nH, ne, nH, 0.0, nem, rin, rout);
+        Rb = B + rout;
+        scale = A + B + fabs(Ra) + fabs(Rb);
+    }
     double x = xb;
     for (int k = 0; k < 80; ++k) {
         if (Rb == Ra) break;
@@ -316,7 +336,8 @@ __device__ double solve_cell_m(const MetalSw &ms, const MCache<MT> &cc, const Ra
         }
         if (ms.cx_two_way) {
             xHI = m_hydrogen_neutral_fraction<MT, LAY>(cc, nH, ne, r.aH * ne,
-                                                       gH + (r.cH + r.aH) * ne);
+                                                       gH + (r.cH + r.aH) * ne,
+                                                       ms.noise_cap > 0 ? xHI : -1.0);
         } else {
             const double den = gH + (r.cH + r.aH) * ne + rin + rout;
             xHI = den > 0.0 ? (r.aH * ne + rin) / den : 1.0;
