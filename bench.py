@@ -287,8 +287,7 @@ def run_form(name, params, nH, octree, args, mlp=None):
         tm = m.timing()
         walk_ms.append(tm["walk_ms"])
         npk.append(params["nphotons"] + m.diffuse_info()[1])
-        tal = m.tallies()
-        nseg.append(tal["seg_direct"] + tal["seg_scattered"])
+        nseg.append(sum(m.segment_counts()))
         nmet.append(m.metal_segment_transitions())
         update()
         solve_ms.append(m.timing()["solve_ms"])

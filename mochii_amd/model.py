@@ -282,6 +282,14 @@ class Model:
         return dict(hhe=hhe, jt=jt, dust_abs=dust, seg_direct=int(segs[0]),
                     seg_scattered=int(segs[1]))
 
+    def segment_counts(self):
+        """{direct, scattered} scored segments of the last pass, without
+        copying the tallies."""
+        segs = np.zeros(2, dtype=np.int64)
+        check(self._L.mc_tallies_get(self._h, None, None, None,
+                                     segs.ctypes.data_as(C.POINTER(C.c_int64))), "mc_tallies_get")
+        return int(segs[0]), int(segs[1])
+
     def rates(self):
         r = np.empty(6 * self.nleaf)
         check(self._L.mc_rates_get(self._h, _dp(r)), "mc_rates_get")
