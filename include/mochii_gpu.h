@@ -363,6 +363,22 @@ int mc_info(mc_ctx *ctx, int32_t *nleaf, int32_t *nnu_band, double *Ltot, double
  * packet at a time). */
 int mc_set_packet_order(mc_ctx *ctx, int order);
 
+/* Energy-moment tallies of the MC_MIXED metal walk (on by default): a scored
+ * segment adds its path weight times the Chebyshev polynomials T_0..T_5 of
+ * its packet's position in an ln-E sub-bin of its band bin to six per-leaf
+ * moments instead of one atomic per nonzero rate coefficient; after the walk
+ * each coefficient sum is rebuilt from the moments with the coefficient's
+ * interpolant on the sub-bin, which matches it to a relative 1e-7 (the
+ * sub-bins are chosen for that on the host).  Off, or when a band bin cannot
+ * be resolved, the walk keeps the per-coefficient sums; the MC_FP64 form
+ * always does.  No counterpart in the reference. */
+int mc_set_tally_moments(mc_ctx *ctx, int on);
+
+/* Sub-bins of the energy-moment tallies over the band (0: not used with this
+ * band and registry) and the largest relative interpolation error of any
+ * coefficient on its check points.  No counterpart in the reference. */
+int mc_tally_moment_info(mc_ctx *ctx, int32_t *nsub, double *fit_err);
+
 /* Switch the transport precision form (MC_FP64 | MC_MIXED) between passes. */
 int mc_set_precision(mc_ctx *ctx, int precision);
 

@@ -169,6 +169,7 @@ static void free_leaf_arrays(mc_ctx *c)
 {
     free_metal_arrays(c);
     mc_ir_free(c);
+    mc_mom_free(c);
     DevState &s = c->st;
     for (void *p : {(void *)s.hot_slot, (void *)s.hot_leaf, (void *)s.hot_hhe, (void *)s.hot_jt})
         if (p) (void)hipFree(p);
@@ -316,6 +317,7 @@ extern "C" int mc_init(const mc_params *p, mc_ctx **out)
     }
     // launch-record buffers: batches of up to 4M packets
     c->packet_order = MC_ORDER_DIRECTION;
+    c->mom_on = 1;
     if (mc_packet_capacity(c, p->nphotons)) {
         mc_finalize(c);
         return -100;
@@ -675,6 +677,25 @@ extern "C" int mc_set_packet_order(mc_ctx *c, int order)
     return 0;
 }
 
+extern "C" int mc_set_tally_moments(mc_ctx *c, int on)
+{
+    CHECK_CTX(c);
+    c->mom_on = on != 0;
+    return 0;
+}
+
+extern "C" int mc_tally_moment_info(mc_ctx *c, int32_t *nsub, double *fit_err)
+{
+    CHECK_CTX(c);
+    if (c->ntr > 0 && c->mom.built_ntr != c->ntr) {
+        mc_mom_free(c);
+        if (mc_mom_build(c)) return -1;
+    }
+    if (nsub) *nsub = c->ntr > 0 ? c->mom.nsub : 0;
+    if (fit_err) *fit_err = c->ntr > 0 ? c->mom.fit_err : 0.0;
+    return 0;
+}
+
 extern "C" int mc_set_warm_start(mc_ctx *c, int mode)
 {
     CHECK_CTX(c);
@@ -903,6 +924,7 @@ extern "C" int mc_species_setup(mc_ctx *c, int n, const mc_element *els)
                                                          : MC_LAYOUT_GENERIC;
     c->elements.assign(els, els + n);
     c->nel = n; c->ntr = ntr; c->nst = nst;
+    mc_mom_free(c);            // the moment tables follow the registry
     if (alloc_metal_arrays(c)) return -100;
     if (c->state_ready) return mc_run_stage_refresh(c);
     return 0;

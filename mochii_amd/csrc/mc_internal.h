@@ -221,6 +221,22 @@ struct DevState {
 // band cut, and the per-leaf arrays of the self-heating iteration.
 #define MC_DT_NT 61
 #define MC_NLAM_BB 300
+// Energy-moment path tallies of the mixed-form metal walk (mc_moments.cpp):
+// band bin b holds pieces bseg[2b] .. bseg[2b] + bseg[2b+1] - 1 (cut at the
+// thresholds inside it); piece g = seg[5g..5g+4] = {lower energy, its ln,
+// n / ln width, n, first sub-bin} is cut into n sub-bins equal in ln E, a
+// packet's coordinate in the piece being f = (ln E - seg[5g+1]) seg[5g+2]
+// in [0, n).  coef[(s nq + q) MC_MOM_NM + m] are the Chebyshev coefficients
+// of coefficient q on sub-bin s, band[s] its band bin.
+#define MC_MOM_NM 6          // moments per sub-bin (Chebyshev degree 5)
+#define MC_MOM_MAXSUB 192    // sub-bins over the band at most
+struct MomTab {
+    int nsub = 0, nq = 0, built_ntr = -1;
+    double fit_err = 0.0;
+    std::vector<double> seg, bseg, coef;
+    std::vector<int32_t> band;
+};
+
 struct IrBand {
     bool temp_ready = false, setup_ready = false;
     double tgrid[MC_DT_NT], femit[MC_DT_NT];
@@ -318,6 +334,13 @@ struct mc_ctx {
     float *d_mopf;
     // dust thermal-infrared band
     IrBand ir;
+    // energy-moment tallies (mixed-form metal walk): switch, tables, device
+    // moments [leaf][sub-bin][MC_MOM_NM] and tables
+    int mom_on;
+    MomTab mom;
+    double *d_mom, *d_mom_coef, *d_mom_tab, *d_mom_bseg;
+    int32_t *d_mom_band;
+    size_t mom_cap;
     // last pass results
     double ledger[3];
     int64_t segs[3];          // direct, scattered, metal transitions over scored segments
@@ -356,6 +379,9 @@ int mc_run_dust_emission(mc_ctx *c, double report[MC_DUSTRT_NREPORT]);
 int mc_ir_femit_table(mc_ctx *c);
 int mc_ir_emission_grid(mc_ctx *c);
 void mc_ir_free(mc_ctx *c);
+int mc_mom_build(mc_ctx *c);
+int mc_mom_ready(mc_ctx *c);
+void mc_mom_free(mc_ctx *c);
 // launch-record buffers of at least min(n, 4M) packets (mc_capi.cpp)
 int mc_packet_capacity(mc_ctx *c, int64_t n);
 

@@ -67,6 +67,13 @@ struct KArgs {
     const double *ir_edge;
     int32_t ir_nl;
     double *ir_wave;         // per-wave ledger slots of the infrared walk (5 each)
+    // energy-moment tallies of the mixed-form metal walk (nullptr: per-
+    // coefficient sums): moments [(il*mom_ns + s)*MC_MOM_NM + m], the pieces
+    // of the band bins {lower energy, its ln, n / ln width, n, first
+    // sub-bin} and each band bin's {first piece, pieces}
+    double *mom;
+    const double *mom_tab, *mom_bseg;
+    int32_t mom_ns;
 };
 
 // H/He recombination and collisional-ionisation coefficients at one T
@@ -76,10 +83,11 @@ struct RateSet { double aH, aHe2, aHe3, cH, cHe1, cHe2; };
 
 // The walk's dynamic LDS, [row][kBlock] doubles per block: rows 0-5 the
 // H/He rate-tally coefficients of each lane's packet (coef), rows 6-8 the
-// lane's energy-ledger sums (led), then its metal cross sections in
-// threshold order (msig), one row per registry transition.
+// lane's energy-ledger sums (led), row 9 its packet's energy-moment
+// sub-bin coordinate (mom_t), then its metal cross sections in threshold
+// order (msig), one row per registry transition.
 // Kept out of the lane's registers: the walk's live state is register-bound.
-constexpr int kCoefRows = 9;     // 6 coefficients + 3 ledger sums
+constexpr int kCoefRows = 10;    // 6 coefficients + 3 ledger sums + mom_t
 __device__ __forceinline__ double &coef(int k)
 {
     extern __shared__ double mc_msig[];
@@ -91,6 +99,12 @@ __device__ __forceinline__ double &led(int k)
 {
     extern __shared__ double mc_msig[];
     return mc_msig[(6 + k) * kBlock + threadIdx.x];
+}
+
+__device__ __forceinline__ double &mom_t()
+{
+    extern __shared__ double mc_msig[];
+    return mc_msig[9 * kBlock + threadIdx.x];
 }
 
 __device__ __forceinline__ double &msig(int k)
@@ -397,6 +411,7 @@ struct Lane {
     float px, py, pz, fkx, fky, fkz, fsHI, fsHeI, fsHeII, fdust;
     float fexpo, ftau, ftau_in, ftcur;
     int nmet;                // metal transitions below the packet's energy
+    int ms;                  // energy-moment sub-bin of the packet (-1: none)
     double rhe, rE;          // 1 / (E ev2erg) and 1 / E of the packet (metal tallies)
     double lam;              // wavelength [um] (infrared band)
     // tally of this step, scored after the step by the whole wave
@@ -523,10 +538,31 @@ __device__ __forceinline__ void metal_coeff(int k, const Lane &L, double &c1, do
 }
 
 // one lane's segment: 1 band-J atomic + one per nonzero rate coefficient
-// (+ 2 per metal transition below the packet's energy)
-template <int M>
+// (+ 2 per metal transition below the packet's energy); with energy-moment
+// tallies (MOM, a leaf without replicas) the six moments of the packet's
+// sub-bin instead, pl T_m(t) by the Chebyshev recurrence
+template <int M, bool MOM>
 __device__ __forceinline__ void tally_one(const KArgs &a, const Lane &L, int il, double pl)
 {
+    if constexpr (MOM) {
+        if (L.ms >= 0 && (a.s.nhot == 0 || a.s.hot_slot[il] < 0)) {
+            double *m = a.mom + ((size_t)il * a.mom_ns + L.ms) * MC_MOM_NM;
+            const double t2 = 2.0 * mom_t();
+            const double p1 = 0.5 * t2 * pl;
+            unsafeAtomicAdd(m + 0, pl);
+            unsafeAtomicAdd(m + 1, p1);
+            const double p2 = t2 * p1 - pl;
+            unsafeAtomicAdd(m + 2, p2);
+            const double p3 = t2 * p2 - p1;
+            unsafeAtomicAdd(m + 3, p3);
+            const double p4 = t2 * p3 - p2;
+            unsafeAtomicAdd(m + 4, p4);
+            unsafeAtomicAdd(m + 5, t2 * p4 - p3);
+            if (a.add_dust && L.dabs != 0.0)
+                unsafeAtomicAdd(a.s.hhe + (size_t)il * 8 + 6, pl * L.dabs);
+            return;
+        }
+    }
     double *h, *j;
     tally_rows(a, il, h, j);
     unsafeAtomicAdd(j + L.inu, pl);
@@ -564,7 +600,7 @@ constexpr int kAggMin = 3;      // smallest group worth a wave reduction
 // by each packet's energy bin) are binned in the wave's LDS row first and go
 // out as one atomic per nonzero bin.  Lanes outside a large group use
 // per-lane atomics.  jbin: this wave's nnu LDS bins, zero between calls.
-template <int M>
+template <int P, int M>
 __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, double *jbin)
 {
     if constexpr (M == 2) {
@@ -636,7 +672,7 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
             if (in) has = false;
         }
     }
-    if (has) tally_one<M>(a, L, L.s_il, L.s_pl);
+    if (has) tally_one<M, Prec<P>::mixed && M == 1>(a, L, L.s_il, L.s_pl);
     L.s_il = -1;
 }
 
@@ -1240,6 +1276,28 @@ __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, 
         L.nmet = n;
         L.rhe = 1.0 / he;
         L.rE = 1.0 / L.E;
+        // the packet's energy-moment sub-bin and coordinate t in [-1, 1];
+        // a packet outside its band bin keeps the per-coefficient sums
+        L.ms = -1;
+        if constexpr (Prec<P>::mixed) {
+            if (a.mom) {
+                // the piece of the band bin that holds E (pieces start at
+                // the thresholds: E >= threshold selects the piece where
+                // that transition is on, as in the walk's nmet)
+                const double *bs = a.mom_bseg + 2 * L.inu;
+                int g = (int)bs[0];
+                const int ge = g + (int)bs[1];
+                while (g + 1 < ge && L.E >= a.mom_tab[5 * (g + 1)]) ++g;
+                const double *mt = a.mom_tab + 5 * g;
+                const double f = (log(L.E) - mt[1]) * mt[2];
+                const double nsub = mt[3];
+                if (L.E >= mt[0] && f <= nsub * (1.0 + 1.0e-9)) {
+                    const int j = min(max((int)floor(f), 0), (int)nsub - 1);
+                    mom_t() = fmin(fmax(2.0 * (f - (double)j) - 1.0, -1.0), 1.0);
+                    L.ms = (int)mt[4] + j;
+                }
+            }
+        }
     }
     L.wgt = 1.0;
     L.nscatt = 0;
@@ -1355,7 +1413,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
             if constexpr (G == kCarDDA) car_step<P, M>(a, L, cn);
             else oct_step<P, M>(a, L, cn);
         }
-        flush_tally<M>(a, L, lane, jbin);
+        flush_tally<P, M>(a, L, lane, jbin);
     }
     if constexpr (M == 2) {
         // compensated wave sums of the infrared ledger into this wave's slot
@@ -1438,6 +1496,50 @@ __global__ __launch_bounds__(kBlock) void k_fold_hot(DevState s, int nnu)
         for (int r = 0; r < s.nrep; ++r) acc += s.hot_jt[((size_t)r * s.nhot + hs) * nnu + b];
         s.jt[(size_t)il * nnu + b] = acc;
     }
+}
+
+// Coefficient sums of every leaf from its energy moments, added to the
+// per-coefficient sums of the walk (wave groups and replicated leaves):
+// tally_q = sum_s sum_m coef[(s nq + q) MC_MOM_NM + m] M[s][m]; band J is
+// the sum of M_0 over the band bin's sub-bins.  One thread per leaf.
+__global__ __launch_bounds__(kBlock) void k_mom_fold(DevState s, const double *__restrict__ mom,
+                                                     const double *__restrict__ coef,
+                                                     const int32_t *__restrict__ band, int ns,
+                                                     int ntr, int nnu, double *mraw, int nleaf)
+{
+    const int il = blockIdx.x * kBlock + threadIdx.x;
+    if (il >= nleaf) return;
+    constexpr int kQ = 6 + 2 * MC_GPU_MAXTR;
+    const int nq = 6 + 2 * ntr;
+    double acc[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) acc[q] = 0.0;
+    const double *M = mom + (size_t)il * ns * MC_MOM_NM;
+    double *jt = s.jt + (size_t)il * nnu;
+    for (int sb = 0; sb < ns; ++sb) {
+        double v[MC_MOM_NM];
+#pragma unroll
+        for (int m = 0; m < MC_MOM_NM; ++m) v[m] = M[(size_t)sb * MC_MOM_NM + m];
+        if (v[0] == 0.0) continue;           // no segment of this sub-bin
+        jt[band[sb]] += v[0];
+        const double *cf = coef + (size_t)sb * nq * MC_MOM_NM;
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            if (q < nq) {
+                double t = 0.0;
+#pragma unroll
+                for (int m = 0; m < MC_MOM_NM; ++m) t += cf[q * MC_MOM_NM + m] * v[m];
+                acc[q] += t;
+            }
+        }
+    }
+    double *h = s.hhe + (size_t)il * 8;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) h[k] += acc[k];
+    double *r = mraw + (size_t)il * ntr * 2;
+#pragma unroll
+    for (int q = 6; q < kQ; ++q)
+        if (q < nq) r[q - 6] += acc[q];
 }
 
 // leaf half width (car: constant)
@@ -2614,6 +2716,10 @@ KArgs make_args(mc_ctx *c)
     a.ir_edge = nullptr;
     a.ir_nl = 0;
     a.ir_wave = nullptr;
+    a.mom = nullptr;
+    a.mom_tab = nullptr;
+    a.mom_bseg = nullptr;
+    a.mom_ns = 0;
     return a;
 }
 
@@ -2679,6 +2785,19 @@ int mc_run_pass(mc_ctx *c)
     if (metal_sync(c)) return -100;
     MC_HIP(hipEventRecord(c->ev[0], st));
     const bool metals = c->ntr > 0;
+    // energy-moment tallies in the mixed-form metal walk
+    int use_mom = 0;
+    if (metals && c->p.precision == MC_MIXED) {
+        use_mom = mc_mom_ready(c);
+        if (use_mom < 0) return -100;
+    }
+    if (use_mom) {
+        a.mom = c->d_mom;
+        a.mom_tab = c->d_mom_tab;
+        a.mom_bseg = c->d_mom_bseg;
+        a.mom_ns = c->mom.nsub;
+        k_zero<<<2048, kBlock, 0, st>>>(c->d_mom, nl * c->mom.nsub * MC_MOM_NM);
+    }
     if (metals) k_zero<<<1024, kBlock, 0, st>>>(c->d_mraw, nl * c->ntr * 2);
     k_zero<<<1024, kBlock, 0, st>>>(c->st.hhe, nl * 8);
     k_zero<<<1024, kBlock, 0, st>>>(c->st.jt, nl * c->nnu);
@@ -2779,6 +2898,12 @@ int mc_run_pass(mc_ctx *c)
     if (c->st.nhot > 0) {
         const int64_t nf = (int64_t)c->st.nhot * (8 + c->nnu);
         k_fold_hot<<<(unsigned)((nf + kBlock - 1) / kBlock), kBlock, 0, st>>>(c->st, c->nnu);
+        MC_HIP(hipGetLastError());
+    }
+    if (use_mom) {
+        k_mom_fold<<<(c->nleaf + kBlock - 1) / kBlock, kBlock, 0, st>>>(
+            c->st, c->d_mom, c->d_mom_coef, c->d_mom_band, c->mom.nsub, c->ntr, c->nnu,
+            c->d_mraw, c->nleaf);
         MC_HIP(hipGetLastError());
     }
     double led[3];

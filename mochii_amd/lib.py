@@ -126,6 +126,8 @@ def library():
         "mc_diffuse_info": ([vp, dp, i64p], C.c_int),
         "mc_set_warm_start": ([vp, C.c_int], C.c_int),
         "mc_set_packet_order": ([vp, C.c_int], C.c_int),
+        "mc_set_tally_moments": ([vp, C.c_int], C.c_int),
+        "mc_tally_moment_info": ([vp, C.POINTER(C.c_int32), dp], C.c_int),
         "mc_thermal_stats": ([vp, i64p], C.c_int),
         "mc_mlp_set": ([vp, fp, C.c_int], C.c_int),
         "mc_mlp_get": ([vp, fp], C.c_int),

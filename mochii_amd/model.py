@@ -216,6 +216,18 @@ class Model:
         return a.value, b.value, c.value, d.value
 
     # ---- learned warm start (no counterpart in the reference) -----------------
+    def set_tally_moments(self, on):
+        """Energy-moment tallies of the MC_MIXED metal walk (default on)."""
+        check(self._L.mc_set_tally_moments(self._h, int(bool(on))), "mc_set_tally_moments")
+
+    def tally_moment_info(self):
+        """(sub-bins over the band, largest relative interpolation error);
+        (0, 0.0) when the walk keeps the per-coefficient sums."""
+        n, e = C.c_int32(), C.c_double()
+        check(self._L.mc_tally_moment_info(self._h, C.byref(n), C.byref(e)),
+              "mc_tally_moment_info")
+        return n.value, e.value
+
     def set_warm_start(self, mode):
         check(self._L.mc_set_warm_start(self._h, int(mode)), "mc_set_warm_start")
 
