@@ -247,10 +247,15 @@ __device__ __forceinline__ void store_packet(const KArgs &a, int64_t i, double x
     P.leaf[i] = leaf;
 }
 
-// walk-order key of a launch record: emitting component in the top bits, then
-// the direction on an octahedral map (2^11 x 2^11 cells, Morton order), so
-// that sorted records give waves of packets leaving one source in nearly the
-// same direction
+// walk-order key of a launch record: emitting component in the top bits,
+// then the direction on an octahedral map (2^kKeyDirBits cells a side,
+// Morton order), then the packet's ln E across the band in the remaining
+// low bits, so that sorted records give waves of packets leaving one source
+// in nearly the same direction with nearby energies: a wave group's rate
+// sums then span few metal transitions and band bins
+constexpr int kKeyDirBits = 7;
+constexpr int kKeyEBits = 22 - 2 * kKeyDirBits;
+
 __device__ __forceinline__ uint32_t spread11(uint32_t v)
 {
     v &= 0x7FFu;
@@ -261,7 +266,8 @@ __device__ __forceinline__ uint32_t spread11(uint32_t v)
     return v;
 }
 
-__device__ __forceinline__ uint32_t direction_key(int comp, double kx, double ky, double kz)
+__device__ __forceinline__ uint32_t direction_key(const DevTables *T, int comp, double kx, double ky,
+                                                  double kz, double E)
 {
     const double n = fabs(kx) + fabs(ky) + fabs(kz);
     double u = kx / n, v = ky / n;
@@ -270,9 +276,13 @@ __device__ __forceinline__ uint32_t direction_key(int comp, double kx, double ky
         const double vv = (1.0 - fabs(u)) * (v >= 0.0 ? 1.0 : -1.0);
         u = uu; v = vv;
     }
-    const uint32_t iu = (uint32_t)fmin(fmax((u + 1.0) * 1024.0, 0.0), 2047.0);
-    const uint32_t iv = (uint32_t)fmin(fmax((v + 1.0) * 1024.0, 0.0), 2047.0);
-    return ((uint32_t)comp << 22) | (spread11(iu) | (spread11(iv) << 1));
+    const double cells = (double)(1 << kKeyDirBits);
+    const uint32_t iu = (uint32_t)fmin(fmax((u + 1.0) * 0.5 * cells, 0.0), cells - 1.0);
+    const uint32_t iv = (uint32_t)fmin(fmax((v + 1.0) * 0.5 * cells, 0.0), cells - 1.0);
+    const double elev = (double)((1 << kKeyEBits) - 1);
+    const double e0 = T->eedge[0], e1 = T->eedge[T->nnu];
+    const uint32_t ie = (uint32_t)fmin(fmax(elev * log(E / e0) / log(e1 / e0), 0.0), elev);
+    return ((uint32_t)comp << 22) | ((spread11(iu) | (spread11(iv) << 1)) << kKeyEBits) | ie;
 }
 
 __global__ __launch_bounds__(kBlock) void k_launch(KArgs a, int nlds)
@@ -370,7 +380,7 @@ __global__ __launch_bounds__(kBlock) void k_launch(KArgs a, int nlds)
     }
     const double E = sample_energy(a.samp, a.nknots, T->samp_off[comp], T->samp_off[comp + 1], ue);
     store_packet(a, i, x, y, z, kx, ky, kz, E, find_leaf(a.g, x, y, z, sgd, slink, nlds));
-    if (a.pk.key) a.pk.key[i] = direction_key(comp, kx, ky, kz);
+    if (a.pk.key) a.pk.key[i] = direction_key(T, comp, kx, ky, kz, E);
 }
 
 // ---------------------------------------------------------------------------
