@@ -306,6 +306,9 @@ def run_form(name, params, nH, octree, args, mlp=None):
                iterations=res["iterations"], converged=res["converged"])
     if thermal:
         out["net_rate_evals_per_bracketed_leaf"] = float(np.sum(evals) / max(np.sum(brk), 1))
+    if prec == mc.MC_MIXED and params.get("metal_eth") is not None:
+        nsub, ferr = m.tally_moment_info()
+        out["moment_sub_bins"], out["moment_fit_err"] = int(nsub), float(ferr)
     log(f"[{name}] {out}")
     m.close()
     return out, state

@@ -228,8 +228,8 @@ struct DevState {
 // packet's coordinate in the piece being f = (ln E - seg[5g+1]) seg[5g+2]
 // in [0, n).  coef[(s nq + q) MC_MOM_NM + m] are the Chebyshev coefficients
 // of coefficient q on sub-bin s, band[s] its band bin.
-#define MC_MOM_NM 6          // moments per sub-bin (Chebyshev degree 5)
-#define MC_MOM_MAXSUB 192    // sub-bins over the band at most
+#define MC_MOM_NM 4          // moments per sub-bin (Chebyshev degree 3)
+#define MC_MOM_MAXSUB 512    // sub-bins over the band at most
 struct MomTab {
     int nsub = 0, nq = 0, built_ntr = -1;
     double fit_err = 0.0;

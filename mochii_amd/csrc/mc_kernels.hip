@@ -549,8 +549,8 @@ __device__ __forceinline__ void metal_coeff(int k, const Lane &L, double &c1, do
 
 // one lane's segment: 1 band-J atomic + one per nonzero rate coefficient
 // (+ 2 per metal transition below the packet's energy); with energy-moment
-// tallies (MOM, a leaf without replicas) the six moments of the packet's
-// sub-bin instead, pl T_m(t) by the Chebyshev recurrence
+// tallies (MOM, a leaf without replicas) the MC_MOM_NM moments of the
+// packet's sub-bin instead, pl T_m(t) by the Chebyshev recurrence
 template <int M, bool MOM>
 __device__ __forceinline__ void tally_one(const KArgs &a, const Lane &L, int il, double pl)
 {
@@ -558,16 +558,16 @@ __device__ __forceinline__ void tally_one(const KArgs &a, const Lane &L, int il,
         if (L.ms >= 0 && (a.s.nhot == 0 || a.s.hot_slot[il] < 0)) {
             double *m = a.mom + ((size_t)il * a.mom_ns + L.ms) * MC_MOM_NM;
             const double t2 = 2.0 * mom_t();
-            const double p1 = 0.5 * t2 * pl;
-            unsafeAtomicAdd(m + 0, pl);
-            unsafeAtomicAdd(m + 1, p1);
-            const double p2 = t2 * p1 - pl;
-            unsafeAtomicAdd(m + 2, p2);
-            const double p3 = t2 * p2 - p1;
-            unsafeAtomicAdd(m + 3, p3);
-            const double p4 = t2 * p3 - p2;
-            unsafeAtomicAdd(m + 4, p4);
-            unsafeAtomicAdd(m + 5, t2 * p4 - p3);
+            double pm2 = pl, pm1 = 0.5 * t2 * pl;
+            unsafeAtomicAdd(m + 0, pm2);
+            unsafeAtomicAdd(m + 1, pm1);
+#pragma unroll
+            for (int q = 2; q < MC_MOM_NM; ++q) {
+                const double pq = t2 * pm1 - pm2;
+                unsafeAtomicAdd(m + q, pq);
+                pm2 = pm1;
+                pm1 = pq;
+            }
             if (a.add_dust && L.dabs != 0.0)
                 unsafeAtomicAdd(a.s.hhe + (size_t)il * 8 + 6, pl * L.dabs);
             return;
@@ -622,6 +622,7 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
     }
     bool has = L.s_il >= 0;
     const int nnu = a.tab->nnu;
+    constexpr bool kMom = Prec<P>::mixed && M == 1;
     if (a.coherent) {
         unsigned long long pend = __ballot(has);
         for (int r = 0; r < kAggRounds && pend; ++r) {
@@ -631,6 +632,42 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
             const unsigned long long grp = __ballot(in);
             pend &= ~grp;
             if (__popcll(grp) < kAggMin) continue;
+            if constexpr (kMom) {
+                // energy-moment group: when every lane of the leaf group is
+                // in the leader's moment sub-bin (and the leaf has no
+                // replicas) the group sums the Chebyshev moments of its
+                // segments, one atomic per moment, instead of the ~20
+                // per-coefficient sums below
+                const int lms = __shfl(L.ms, leader);
+                if (a.mom && lms >= 0 && __ballot(in && L.ms == lms) == grp &&
+                    (a.s.nhot == 0 || a.s.hot_slot[lil] < 0)) {
+                    double *m = a.mom + ((size_t)lil * a.mom_ns + lms) * MC_MOM_NM;
+                    const double pl = in ? L.s_pl : 0.0;
+                    const double t2 = in ? 2.0 * mom_t() : 0.0;
+                    double pm2 = pl, pm1 = 0.5 * t2 * pl;
+                    double sum = wave_sum(pl);
+                    if (lane == leader) unsafeAtomicAdd(m + 0, sum);
+                    sum = wave_sum(pm1);
+                    if (lane == leader) unsafeAtomicAdd(m + 1, sum);
+#pragma unroll
+                    for (int q = 2; q < MC_MOM_NM; ++q) {
+                        const double pq = t2 * pm1 - pm2;
+                        sum = wave_sum(pq);
+                        if (lane == leader) unsafeAtomicAdd(m + q, sum);
+                        pm2 = pm1;
+                        pm1 = pq;
+                    }
+                    if (a.add_dust) {
+                        const double v = in ? L.s_pl * L.dabs : 0.0;
+                        if (__ballot(v != 0.0)) {
+                            sum = wave_sum(v);
+                            if (lane == leader) unsafeAtomicAdd(a.s.hhe + (size_t)lil * 8 + 6, sum);
+                        }
+                    }
+                    if (in) has = false;
+                    continue;
+                }
+            }
             double *h, *j;
             tally_rows(a, lil, h, j);
 #pragma unroll

@@ -9,14 +9,17 @@
 // (src/jtally_mod.f90).  Inside one band bin no threshold is crossed (the
 // band edges are the thresholds), so every coefficient is a smooth function
 // of ln E there.  Each bin is cut into sub-bins, equal in ln E, on which
-// every coefficient equals its degree-5 Chebyshev interpolant in the
+// every coefficient equals its degree-3 Chebyshev interpolant in the
 // sub-bin coordinate t in [-1, 1] to a relative 1e-7 (checked on 48 interior
-// points per sub-bin); a segment then adds pl T_m(t), m = 0..5, to its
-// leaf's six moments of its sub-bin, and after the walk
+// points per sub-bin); a segment then adds pl T_m(t), m = 0..3, to its
+// leaf's four moments of its sub-bin, and after the walk
 //   tally_q(leaf) = sum_s sum_m a[s][q][m] M[leaf][s][m]
-// gives every coefficient sum.  Six atomics per segment instead of one per
-// nonzero coefficient (~19 at 15 metal transitions).  The FP64 form keeps
-// the reference's per-coefficient sums.
+// gives every coefficient sum.  Four atomics per segment instead of one per
+// nonzero coefficient (~19 at 15 metal transitions).  Degree 3 needs ~2x the
+// sub-bins of degree 5 for the same 1e-7 (218 against ~100 on the C N O Ne S
+// Ar band), which costs only moment memory; degree 2 does not reach 1e-7
+// within 512 sub-bins per band bin.  The FP64 form keeps the reference's
+// per-coefficient sums.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -26,7 +29,7 @@
 namespace {
 
 constexpr int kMomDeg = MC_MOM_NM - 1;   // Chebyshev degree
-constexpr int kMomMaxSub = 48;           // sub-bins per band bin at most
+constexpr int kMomMaxSub = 128;          // sub-bins per band bin at most
 constexpr int kMomCheck = 48;            // interior check points per sub-bin
 constexpr double kMomTol = 1.0e-7;       // relative to the sub-bin's max |c|
 

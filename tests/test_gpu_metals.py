@@ -190,7 +190,7 @@ def test_metal_warm_start_matches_bisection():
 
 
 def test_energy_moment_tallies_match_per_coefficient_sums():
-    """MC_MIXED metal walk: the energy-moment tallies (six Chebyshev moments
+    """MC_MIXED metal walk: the energy-moment tallies (four Chebyshev moments
     of the packet's ln-E sub-bin per segment, every coefficient sum rebuilt
     from its interpolant on the sub-bin) against the per-coefficient sums of
     the same walk.  The walk is the same (same segments); H/He rate sums,
