@@ -71,7 +71,8 @@ struct KArgs {
     // coefficient sums): moments [(il*mom_ns + s)*MC_MOM_NM + m], the pieces
     // of the band bins {lower energy, its ln, n / ln width, n, first
     // sub-bin} and each band bin's {first piece, pieces}
-    double *mom;
+    // replicas of the hot leaves' moment rows, [(rep*nhot + slot)*mom_ns + s]
+    double *mom, *mom_hot;
     const double *mom_tab, *mom_bseg;
     int32_t mom_ns;
 };
@@ -521,6 +522,11 @@ __device__ __forceinline__ void score(const KArgs &, Lane &L, int il, double pl,
 
 // tally destination of a leaf: its own rows, or one of the replicas of a hot
 // leaf chosen by the issuing lane
+__device__ __forceinline__ int hot_rep(const DevState &s)
+{
+    return (int)(((blockIdx.x * kBlock + threadIdx.x) * 0x9E3779B1u) >> 16) % s.nrep;
+}
+
 __device__ __forceinline__ void tally_rows(const KArgs &a, int il, double *&h, double *&j)
 {
     const DevState &s = a.s;
@@ -530,8 +536,7 @@ __device__ __forceinline__ void tally_rows(const KArgs &a, int il, double *&h, d
         h = s.hhe + (size_t)il * 8;
         j = s.jt + (size_t)il * nnu;
     } else {
-        const int rep = (int)(((blockIdx.x * kBlock + threadIdx.x) * 0x9E3779B1u) >> 16) % s.nrep;
-        const size_t slot = (size_t)rep * s.nhot + hs;
+        const size_t slot = (size_t)hot_rep(s) * s.nhot + hs;
         h = s.hot_hhe + slot * 8;
         j = s.hot_jt + slot * nnu;
     }
@@ -547,16 +552,26 @@ __device__ __forceinline__ void metal_coeff(int k, const Lane &L, double &c1, do
     c2 = sg * fmax(1.0 - c_mt.flat_eth[k] * L.rE, 0.0);
 }
 
+// energy moments of sub-bin ms of a leaf: its own row, or one of the
+// replicas of a hot leaf chosen by the issuing lane
+__device__ __forceinline__ double *mom_row(const KArgs &a, int il, int ms)
+{
+    const DevState &s = a.s;
+    const int hs = s.nhot > 0 ? s.hot_slot[il] : -1;
+    if (hs < 0) return a.mom + ((size_t)il * a.mom_ns + ms) * MC_MOM_NM;
+    return a.mom_hot + (((size_t)hot_rep(s) * s.nhot + hs) * a.mom_ns + ms) * MC_MOM_NM;
+}
+
 // one lane's segment: 1 band-J atomic + one per nonzero rate coefficient
 // (+ 2 per metal transition below the packet's energy); with energy-moment
-// tallies (MOM, a leaf without replicas) the MC_MOM_NM moments of the
-// packet's sub-bin instead, pl T_m(t) by the Chebyshev recurrence
+// tallies (MOM) the MC_MOM_NM moments of the packet's sub-bin instead,
+// pl T_m(t) by the Chebyshev recurrence
 template <int M, bool MOM>
 __device__ __forceinline__ void tally_one(const KArgs &a, const Lane &L, int il, double pl)
 {
     if constexpr (MOM) {
-        if (L.ms >= 0 && (a.s.nhot == 0 || a.s.hot_slot[il] < 0)) {
-            double *m = a.mom + ((size_t)il * a.mom_ns + L.ms) * MC_MOM_NM;
+        if (L.ms >= 0) {
+            double *m = mom_row(a, il, L.ms);
             const double t2 = 2.0 * mom_t();
             double pm2 = pl, pm1 = 0.5 * t2 * pl;
             unsafeAtomicAdd(m + 0, pm2);
@@ -634,14 +649,13 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
             if (__popcll(grp) < kAggMin) continue;
             if constexpr (kMom) {
                 // energy-moment group: when every lane of the leaf group is
-                // in the leader's moment sub-bin (and the leaf has no
-                // replicas) the group sums the Chebyshev moments of its
-                // segments, one atomic per moment, instead of the ~20
-                // per-coefficient sums below
+                // in the leader's moment sub-bin the group sums the
+                // Chebyshev moments of its segments, one atomic per moment
+                // (into the leader's replica of a hot leaf), instead of the
+                // ~20 per-coefficient sums below
                 const int lms = __shfl(L.ms, leader);
-                if (a.mom && lms >= 0 && __ballot(in && L.ms == lms) == grp &&
-                    (a.s.nhot == 0 || a.s.hot_slot[lil] < 0)) {
-                    double *m = a.mom + ((size_t)lil * a.mom_ns + lms) * MC_MOM_NM;
+                if (a.mom && lms >= 0 && __ballot(in && L.ms == lms) == grp) {
+                    double *m = mom_row(a, lil, lms);
                     const double pl = in ? L.s_pl : 0.0;
                     const double t2 = in ? 2.0 * mom_t() : 0.0;
                     double pm2 = pl, pm1 = 0.5 * t2 * pl;
@@ -1548,27 +1562,44 @@ __global__ __launch_bounds__(kBlock) void k_fold_hot(DevState s, int nnu)
 // Coefficient sums of every leaf from its energy moments, added to the
 // per-coefficient sums of the walk (wave groups and replicated leaves):
 // tally_q = sum_s sum_m coef[(s nq + q) MC_MOM_NM + m] M[s][m]; band J is
-// the sum of M_0 over the band bin's sub-bins.  One thread per leaf.
+// the sum of M_0 over the band bin's sub-bins.  One wave per leaf, lanes
+// over its sub-bins (a hot leaf adds its replicas' rows first); band J
+// binned in the wave's LDS row, the coefficient sums reduced across the wave.
 __global__ __launch_bounds__(kBlock) void k_mom_fold(DevState s, const double *__restrict__ mom,
+                                                     const double *__restrict__ mom_hot,
                                                      const double *__restrict__ coef,
                                                      const int32_t *__restrict__ band, int ns,
                                                      int ntr, int nnu, double *mraw, int nleaf)
 {
-    const int il = blockIdx.x * kBlock + threadIdx.x;
-    if (il >= nleaf) return;
+    __shared__ double jrow[kBlock / 64][MC_MAXBINS];
+    const int lane = threadIdx.x & 63;
+    double *jl = jrow[threadIdx.x >> 6];
+    for (int b = lane; b < MC_MAXBINS; b += 64) jl[b] = 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int il = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (il >= nleaf) return;                 // uniform over the wave
+    const int hs = s.nhot > 0 ? s.hot_slot[il] : -1;
     constexpr int kQ = 6 + 2 * MC_GPU_MAXTR;
     const int nq = 6 + 2 * ntr;
     double acc[kQ];
 #pragma unroll
     for (int q = 0; q < kQ; ++q) acc[q] = 0.0;
     const double *M = mom + (size_t)il * ns * MC_MOM_NM;
-    double *jt = s.jt + (size_t)il * nnu;
-    for (int sb = 0; sb < ns; ++sb) {
+    for (int sb = lane; sb < ns; sb += 64) {
         double v[MC_MOM_NM];
 #pragma unroll
         for (int m = 0; m < MC_MOM_NM; ++m) v[m] = M[(size_t)sb * MC_MOM_NM + m];
+        if (hs >= 0) {
+            for (int r = 0; r < s.nrep; ++r) {
+                const double *R = mom_hot + (((size_t)r * s.nhot + hs) * ns + sb) * MC_MOM_NM;
+#pragma unroll
+                for (int m = 0; m < MC_MOM_NM; ++m) v[m] += R[m];
+            }
+        }
         if (v[0] == 0.0) continue;           // no segment of this sub-bin
-        jt[band[sb]] += v[0];
+        atomicAdd(&jl[band[sb]], v[0]);
         const double *cf = coef + (size_t)sb * nq * MC_MOM_NM;
 #pragma unroll
         for (int q = 0; q < kQ; ++q) {
@@ -1580,13 +1611,24 @@ __global__ __launch_bounds__(kBlock) void k_mom_fold(DevState s, const double *_
             }
         }
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double *jt = s.jt + (size_t)il * nnu;
+    for (int b = lane; b < nnu; b += 64)
+        if (jl[b] != 0.0) jt[b] += jl[b];
     double *h = s.hhe + (size_t)il * 8;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) h[k] += acc[k];
     double *r = mraw + (size_t)il * ntr * 2;
 #pragma unroll
-    for (int q = 6; q < kQ; ++q)
-        if (q < nq) r[q - 6] += acc[q];
+    for (int q = 0; q < kQ; ++q) {
+        if (q < nq) {
+            const double sum = wave_sum(acc[q]);
+            if (lane == 0) {
+                if (q < 6) h[q] += sum;
+                else r[q - 6] += sum;
+            }
+        }
+    }
 }
 
 // leaf half width (car: constant)
@@ -2764,6 +2806,7 @@ KArgs make_args(mc_ctx *c)
     a.ir_nl = 0;
     a.ir_wave = nullptr;
     a.mom = nullptr;
+    a.mom_hot = nullptr;
     a.mom_tab = nullptr;
     a.mom_bseg = nullptr;
     a.mom_ns = 0;
@@ -2840,10 +2883,12 @@ int mc_run_pass(mc_ctx *c)
     }
     if (use_mom) {
         a.mom = c->d_mom;
+        a.mom_hot = c->d_mom + (size_t)nl * c->mom.nsub * MC_MOM_NM;
         a.mom_tab = c->d_mom_tab;
         a.mom_bseg = c->d_mom_bseg;
         a.mom_ns = c->mom.nsub;
-        k_zero<<<2048, kBlock, 0, st>>>(c->d_mom, nl * c->mom.nsub * MC_MOM_NM);
+        k_zero<<<2048, kBlock, 0, st>>>(
+            c->d_mom, ((size_t)nl + (size_t)c->st.nrep * c->st.nhot) * c->mom.nsub * MC_MOM_NM);
     }
     if (metals) k_zero<<<1024, kBlock, 0, st>>>(c->d_mraw, nl * c->ntr * 2);
     k_zero<<<1024, kBlock, 0, st>>>(c->st.hhe, nl * 8);
@@ -2948,8 +2993,8 @@ int mc_run_pass(mc_ctx *c)
         MC_HIP(hipGetLastError());
     }
     if (use_mom) {
-        k_mom_fold<<<(c->nleaf + kBlock - 1) / kBlock, kBlock, 0, st>>>(
-            c->st, c->d_mom, c->d_mom_coef, c->d_mom_band, c->mom.nsub, c->ntr, c->nnu,
+        k_mom_fold<<<(c->nleaf + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, st>>>(
+            c->st, c->d_mom, c->d_mom + (size_t)c->nleaf * c->mom.nsub * MC_MOM_NM, c->d_mom_coef, c->d_mom_band, c->mom.nsub, c->ntr, c->nnu,
             c->d_mraw, c->nleaf);
         MC_HIP(hipGetLastError());
     }

@@ -213,7 +213,9 @@ int mc_mom_ready(mc_ctx *c)
         }
     }
     if (c->mom.nsub == 0) return 0;
-    const size_t need = sizeof(double) * (size_t)c->nleaf * c->mom.nsub * MC_MOM_NM;
+    // the leaves' rows, then the replica rows of the hot leaves
+    const size_t rows = (size_t)c->nleaf + (size_t)c->st.nrep * c->st.nhot;
+    const size_t need = sizeof(double) * rows * c->mom.nsub * MC_MOM_NM;
     if (c->mom_cap < need) {
         if (c->d_mom) (void)hipFree(c->d_mom);
         c->d_mom = nullptr;
