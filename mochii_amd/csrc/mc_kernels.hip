@@ -68,13 +68,15 @@ struct KArgs {
     int32_t ir_nl;
     double *ir_wave;         // per-wave ledger slots of the infrared walk (5 each)
     // energy-moment tallies of the mixed-form metal walk (nullptr: per-
-    // coefficient sums): moments [(il*mom_ns + s)*MC_MOM_NM + m], the pieces
+    // coefficient sums): moments [(s*mom_rows + row)*MC_MOM_NM + m], sub-bin
+    // major, row = leaf or nleaf + rep*nhot + slot for a hot leaf's
+    // replica; the pieces
     // of the band bins {lower energy, its ln, n / ln width, n, first
     // sub-bin} and each band bin's {first piece, pieces}
-    // replicas of the hot leaves' moment rows, [(rep*nhot + slot)*mom_ns + s]
-    double *mom, *mom_hot;
+    double *mom;
     const double *mom_tab, *mom_bseg;
     int32_t mom_ns;
+    int64_t mom_rows;
 };
 
 // H/He recombination and collisional-ionisation coefficients at one T
@@ -558,8 +560,10 @@ __device__ __forceinline__ double *mom_row(const KArgs &a, int il, int ms)
 {
     const DevState &s = a.s;
     const int hs = s.nhot > 0 ? s.hot_slot[il] : -1;
-    if (hs < 0) return a.mom + ((size_t)il * a.mom_ns + ms) * MC_MOM_NM;
-    return a.mom_hot + (((size_t)hot_rep(s) * s.nhot + hs) * a.mom_ns + ms) * MC_MOM_NM;
+    // replica rows follow the nleaf = mom_rows - nrep nhot leaf rows
+    const size_t row = hs < 0 ? (size_t)il
+                              : (size_t)a.mom_rows - (size_t)(s.nrep - hot_rep(s)) * s.nhot + hs;
+    return a.mom + ((size_t)ms * a.mom_rows + row) * MC_MOM_NM;
 }
 
 // one lane's segment: 1 band-J atomic + one per nonzero rate coefficient
@@ -1562,44 +1566,38 @@ __global__ __launch_bounds__(kBlock) void k_fold_hot(DevState s, int nnu)
 // Coefficient sums of every leaf from its energy moments, added to the
 // per-coefficient sums of the walk (wave groups and replicated leaves):
 // tally_q = sum_s sum_m coef[(s nq + q) MC_MOM_NM + m] M[s][m]; band J is
-// the sum of M_0 over the band bin's sub-bins.  One wave per leaf, lanes
-// over its sub-bins (a hot leaf adds its replicas' rows first); band J
-// binned in the wave's LDS row, the coefficient sums reduced across the wave.
+// the sum of M_0 over the band bin's sub-bins.  One thread per leaf over the
+// sub-bins in step across the wave: the sub-bin-major moments are read
+// coalesced and the interpolation coefficients are wave-uniform loads; a hot
+// leaf adds its replicas' rows.
 __global__ __launch_bounds__(kBlock) void k_mom_fold(DevState s, const double *__restrict__ mom,
-                                                     const double *__restrict__ mom_hot,
-                                                     const double *__restrict__ coef,
+                                                     int64_t rows, const double *__restrict__ coef,
                                                      const int32_t *__restrict__ band, int ns,
                                                      int ntr, int nnu, double *mraw, int nleaf)
 {
-    __shared__ double jrow[kBlock / 64][MC_MAXBINS];
-    const int lane = threadIdx.x & 63;
-    double *jl = jrow[threadIdx.x >> 6];
-    for (int b = lane; b < MC_MAXBINS; b += 64) jl[b] = 0.0;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const int il = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    if (il >= nleaf) return;                 // uniform over the wave
+    const int il = blockIdx.x * kBlock + threadIdx.x;
+    if (il >= nleaf) return;
     const int hs = s.nhot > 0 ? s.hot_slot[il] : -1;
     constexpr int kQ = 6 + 2 * MC_GPU_MAXTR;
     const int nq = 6 + 2 * ntr;
     double acc[kQ];
 #pragma unroll
     for (int q = 0; q < kQ; ++q) acc[q] = 0.0;
-    const double *M = mom + (size_t)il * ns * MC_MOM_NM;
-    for (int sb = lane; sb < ns; sb += 64) {
+    double *jt = s.jt + (size_t)il * nnu;
+    for (int sb = 0; sb < ns; ++sb) {
+        const double *M = mom + ((size_t)sb * rows + il) * MC_MOM_NM;
         double v[MC_MOM_NM];
 #pragma unroll
-        for (int m = 0; m < MC_MOM_NM; ++m) v[m] = M[(size_t)sb * MC_MOM_NM + m];
+        for (int m = 0; m < MC_MOM_NM; ++m) v[m] = M[m];
         if (hs >= 0) {
             for (int r = 0; r < s.nrep; ++r) {
-                const double *R = mom_hot + (((size_t)r * s.nhot + hs) * ns + sb) * MC_MOM_NM;
+                const double *R = mom + ((size_t)sb * rows + nleaf + (size_t)r * s.nhot + hs) * MC_MOM_NM;
 #pragma unroll
                 for (int m = 0; m < MC_MOM_NM; ++m) v[m] += R[m];
             }
         }
         if (v[0] == 0.0) continue;           // no segment of this sub-bin
-        atomicAdd(&jl[band[sb]], v[0]);
+        jt[band[sb]] += v[0];
         const double *cf = coef + (size_t)sb * nq * MC_MOM_NM;
 #pragma unroll
         for (int q = 0; q < kQ; ++q) {
@@ -1611,24 +1609,13 @@ __global__ __launch_bounds__(kBlock) void k_mom_fold(DevState s, const double *_
             }
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    double *jt = s.jt + (size_t)il * nnu;
-    for (int b = lane; b < nnu; b += 64)
-        if (jl[b] != 0.0) jt[b] += jl[b];
     double *h = s.hhe + (size_t)il * 8;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) h[k] += acc[k];
     double *r = mraw + (size_t)il * ntr * 2;
 #pragma unroll
-    for (int q = 0; q < kQ; ++q) {
-        if (q < nq) {
-            const double sum = wave_sum(acc[q]);
-            if (lane == 0) {
-                if (q < 6) h[q] += sum;
-                else r[q - 6] += sum;
-            }
-        }
-    }
+    for (int q = 6; q < kQ; ++q)
+        if (q < nq) r[q - 6] += acc[q];
 }
 
 // leaf half width (car: constant)
@@ -2806,7 +2793,7 @@ KArgs make_args(mc_ctx *c)
     a.ir_nl = 0;
     a.ir_wave = nullptr;
     a.mom = nullptr;
-    a.mom_hot = nullptr;
+    a.mom_rows = 0;
     a.mom_tab = nullptr;
     a.mom_bseg = nullptr;
     a.mom_ns = 0;
@@ -2883,7 +2870,7 @@ int mc_run_pass(mc_ctx *c)
     }
     if (use_mom) {
         a.mom = c->d_mom;
-        a.mom_hot = c->d_mom + (size_t)nl * c->mom.nsub * MC_MOM_NM;
+        a.mom_rows = (int64_t)nl + (int64_t)c->st.nrep * c->st.nhot;
         a.mom_tab = c->d_mom_tab;
         a.mom_bseg = c->d_mom_bseg;
         a.mom_ns = c->mom.nsub;
@@ -2993,8 +2980,8 @@ int mc_run_pass(mc_ctx *c)
         MC_HIP(hipGetLastError());
     }
     if (use_mom) {
-        k_mom_fold<<<(c->nleaf + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, st>>>(
-            c->st, c->d_mom, c->d_mom + (size_t)c->nleaf * c->mom.nsub * MC_MOM_NM, c->d_mom_coef, c->d_mom_band, c->mom.nsub, c->ntr, c->nnu,
+        k_mom_fold<<<(c->nleaf + kBlock - 1) / kBlock, kBlock, 0, st>>>(
+            c->st, c->d_mom, (int64_t)c->nleaf + (int64_t)c->st.nrep * c->st.nhot, c->d_mom_coef, c->d_mom_band, c->mom.nsub, c->ntr, c->nnu,
             c->d_mraw, c->nleaf);
         MC_HIP(hipGetLastError());
     }
