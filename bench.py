@@ -71,6 +71,10 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_SEGMENT = {("car", "fp64"): 144, ("car", "mixed_mlp"): 128,
                      ("octree", "fp64"): 188, ("octree", "mixed_mlp"): 152}
 BYTES_PER_METAL_TR = {"fp64": 40, "mixed_mlp": 36}
+# the mixed-form metal walk's own tally bytes: MOMENTS FP64 energy moments of
+# the packet's sub-bin (read-modify-write 16 B each) replace the 7 H/He + 2
+# per metal transition coefficient sums; its reads are those of the record
+MOMENTS = 4
 # the infrared walk per scored segment: leaf geometry (octree FP64 44 B /
 # FP32 24 B; car 0) + rhokap 8 B + one FP64 path-tally read-modify-write 16 B
 IR_BYTES_PER_SEGMENT = {("car", "fp64"): 24, ("car", "mixed"): 24,
@@ -479,6 +483,10 @@ def main():
     bps = (BYTES_PER_SEGMENT[(grid, "mixed_mlp")] +
            BYTES_PER_METAL_TR["mixed_mlp"] * fm["metal_transitions_per_segment"])
     achieved = fm["segments_per_pass"] * bps / (fm["walk_ms"] * 1e-3) / 1e9
+    form_bps = None
+    if fm.get("moment_sub_bins"):
+        form_bps = (BYTES_PER_SEGMENT[(grid, "mixed_mlp")] - 16 * 7 + 16 * MOMENTS +
+                    4 * fm["metal_transitions_per_segment"])
     line = {
         "metric": "packets/s + time-to-converged model, surrogate+FP32 vs plain FP64, "
                   "<=1% T_e/x_HII",
@@ -507,6 +515,10 @@ def main():
                      "kernel": WALK_KERNEL[(grid, "mixed_mlp")].replace(
                          "M>", "1>" if params.get("metal_eth") else "0>"),
                      "bytes_per_segment": bps,
+                     "form_bytes_per_segment": form_bps,
+                     "form_achieved": (None if form_bps is None else
+                                       fm["segments_per_pass"] * form_bps /
+                                       (fm["walk_ms"] * 1e-3) / 1e9),
                      "metal_transitions_per_segment": fm["metal_transitions_per_segment"],
                      "segments_per_pass": fm["segments_per_pass"],
                      "walk_ms_per_pass": fm["walk_ms"]},

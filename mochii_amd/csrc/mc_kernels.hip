@@ -1567,13 +1567,37 @@ __global__ __launch_bounds__(kBlock) void k_fold_hot(DevState s, int nnu)
     }
 }
 
+// Replica rows of the hot leaves' energy moments added into the leaves' own
+// rows before k_mom_fold: one thread per (sub-bin, hot slot), coalesced
+// over the slots.
+__global__ __launch_bounds__(kBlock) void k_mom_fold_hot(DevState s, double *mom, int64_t rows,
+                                                         int ns, int nleaf)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= (int64_t)ns * s.nhot) return;
+    const int64_t sb = i / s.nhot;
+    const int hs = (int)(i - sb * s.nhot);
+    double v[MC_MOM_NM];
+#pragma unroll
+    for (int m = 0; m < MC_MOM_NM; ++m) v[m] = 0.0;
+    for (int r = 0; r < s.nrep; ++r) {
+        const double *R = mom + ((size_t)sb * rows + nleaf + (size_t)r * s.nhot + hs) * MC_MOM_NM;
+#pragma unroll
+        for (int m = 0; m < MC_MOM_NM; ++m) v[m] += R[m];
+    }
+    if (v[0] == 0.0) return;
+    double *M = mom + ((size_t)sb * rows + s.hot_leaf[hs]) * MC_MOM_NM;
+#pragma unroll
+    for (int m = 0; m < MC_MOM_NM; ++m) M[m] += v[m];
+}
+
 // Coefficient sums of every leaf from its energy moments, added to the
 // per-coefficient sums of the walk (wave groups and replicated leaves):
 // tally_q = sum_s sum_m coef[(s nq + q) MC_MOM_NM + m] M[s][m]; band J is
 // the sum of M_0 over the band bin's sub-bins.  One thread per leaf over the
 // sub-bins in step across the wave: the sub-bin-major moments are read
-// coalesced and the interpolation coefficients are wave-uniform loads; a hot
-// leaf adds its replicas' rows.
+// coalesced and the interpolation coefficients are wave-uniform loads (the
+// hot leaves' replica rows are already in their own rows, k_mom_fold_hot).
 __global__ __launch_bounds__(kBlock) void k_mom_fold(DevState s, const double *__restrict__ mom,
                                                      int64_t rows, const double *__restrict__ coef,
                                                      const int32_t *__restrict__ band, int ns,
@@ -1581,7 +1605,6 @@ __global__ __launch_bounds__(kBlock) void k_mom_fold(DevState s, const double *_
 {
     const int il = blockIdx.x * kBlock + threadIdx.x;
     if (il >= nleaf) return;
-    const int hs = s.nhot > 0 ? s.hot_slot[il] : -1;
     constexpr int kQ = 6 + 2 * MC_GPU_MAXTR;
     const int nq = 6 + 2 * ntr;
     double acc[kQ];
@@ -1593,13 +1616,6 @@ __global__ __launch_bounds__(kBlock) void k_mom_fold(DevState s, const double *_
         double v[MC_MOM_NM];
 #pragma unroll
         for (int m = 0; m < MC_MOM_NM; ++m) v[m] = M[m];
-        if (hs >= 0) {
-            for (int r = 0; r < s.nrep; ++r) {
-                const double *R = mom + ((size_t)sb * rows + nleaf + (size_t)r * s.nhot + hs) * MC_MOM_NM;
-#pragma unroll
-                for (int m = 0; m < MC_MOM_NM; ++m) v[m] += R[m];
-            }
-        }
         if (v[0] == 0.0) continue;           // no segment of this sub-bin
         jt[band[sb]] += v[0];
         const double *cf = coef + (size_t)sb * nq * MC_MOM_NM;
@@ -2984,8 +3000,15 @@ int mc_run_pass(mc_ctx *c)
         MC_HIP(hipGetLastError());
     }
     if (use_mom) {
+        const int64_t mrows = (int64_t)c->nleaf + (int64_t)c->st.nrep * c->st.nhot;
+        if (c->st.nhot > 0) {
+            const int64_t nh = (int64_t)c->mom.nsub * c->st.nhot;
+            k_mom_fold_hot<<<(unsigned)((nh + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+                c->st, c->d_mom, mrows, c->mom.nsub, c->nleaf);
+            MC_HIP(hipGetLastError());
+        }
         k_mom_fold<<<(c->nleaf + kBlock - 1) / kBlock, kBlock, 0, st>>>(
-            c->st, c->d_mom, (int64_t)c->nleaf + (int64_t)c->st.nrep * c->st.nhot, c->d_mom_coef, c->d_mom_band, c->mom.nsub, c->ntr, c->nnu,
+            c->st, c->d_mom, mrows, c->d_mom_coef, c->d_mom_band, c->mom.nsub, c->ntr, c->nnu,
             c->d_mraw, c->nleaf);
         MC_HIP(hipGetLastError());
     }
