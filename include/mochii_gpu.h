@@ -273,6 +273,16 @@ int mc_gas_thermal_update(mc_ctx *ctx, double *max_dx, double *max_dte, double *
  * 200-pass cap. */
 int mc_set_warm_start(mc_ctx *ctx, int mode);
 
+/* Bisection levels a warm start runs first (default MC_WARM_LEVELS_DEFAULT,
+ * 0..24): the reference's bisection from [te_min, te_max] for that many
+ * levels, then the warm solve inside the level's cell.  A leaf whose net
+ * rate crosses zero more than once then ends on the bisection's crossing,
+ * or on another one inside that cell: within a factor
+ * (te_max/te_min)^(2^-levels) of it in T_e (0.77% for 9 levels over
+ * [1e3, 5e4] K).  No counterpart in the reference. */
+#define MC_WARM_LEVELS_DEFAULT 9
+int mc_set_warm_levels(mc_ctx *ctx, int levels);
+
 /* Work of the last thermal update: {net-rate evaluations spent on the
  * bracketed leaves (those not pinned at te_min / te_max, whose two edge
  * evaluations settle them), bracketed leaves that fell back to bisection,
@@ -364,14 +374,15 @@ int mc_info(mc_ctx *ctx, int32_t *nleaf, int32_t *nnu_band, double *Ltot, double
 int mc_set_packet_order(mc_ctx *ctx, int order);
 
 /* Energy-moment tallies of the MC_MIXED metal walk (on by default): a scored
- * segment adds its path weight times the Chebyshev polynomials T_0..T_5 of
- * its packet's position in an ln-E sub-bin of its band bin to six per-leaf
+ * segment adds its path weight times the Chebyshev polynomials T_0..T_3 of
+ * its packet's position in an ln-E sub-bin of its band bin to four per-leaf
  * moments instead of one atomic per nonzero rate coefficient; after the walk
  * each coefficient sum is rebuilt from the moments with the coefficient's
  * interpolant on the sub-bin, which matches it to a relative 1e-7 (the
  * sub-bins are chosen for that on the host).  Off, or when a band bin cannot
- * be resolved, the walk keeps the per-coefficient sums; the MC_FP64 form
- * always does.  No counterpart in the reference. */
+ * be resolved, or when the moment rows would not fit the device memory
+ * budget, the walk keeps the per-coefficient sums; the MC_FP64 form always
+ * does.  No counterpart in the reference. */
 int mc_set_tally_moments(mc_ctx *ctx, int on);
 
 /* Sub-bins of the energy-moment tallies over the band (0: not used with this
@@ -440,6 +451,21 @@ int mc_dust_ir_grid(mc_ctx *ctx, int32_t *nl, double *lam, double *sabs, double 
 /* Device times of the last mc_transport_dust_emission [ms]: {launch, walk,
  * other}, and its segment counts {direct, scattered}. */
 int mc_dust_ir_timing(mc_ctx *ctx, double *out3, int64_t *segments2);
+
+/* qmc_uniforms_stream (src/qmc_mod.f90:200-260): the launch's Owen-scrambled
+ * Sobol uniforms u(d) = (code + 0.5) 2^-32 of stream 1 (stellar) or 2
+ * (diffuse) for the n consecutive 0-based Sobol indices first_index ..,
+ * dims 0 .. ndim-1 (ndim <= 12), computed on the device by the function the
+ * launch kernels use; out is row-major n x ndim.  Diagnostic. */
+int mc_qmc_uniforms(mc_ctx *ctx, int stream, int64_t first_index, int n, int ndim,
+                    double *out);
+
+/* gen_ion_photon_qmc (src/ion_band_mod.f90:1880-1935) for the stellar packet
+ * ids first_id .. first_id + n - 1 (1-based, launch order) as the pass's
+ * launch kernel computes them: rec is n x 7 {x, y, z, kx, ky, kz, E [eV]},
+ * bin_leaf n x 2 {0-based band bin, 0-based start leaf or -1 outside}.
+ * n is at most the launch batch (4,194,304).  Diagnostic. */
+int mc_launch_records(mc_ctx *ctx, int64_t first_id, int n, double *rec, int32_t *bin_leaf);
 
 /* Diagnostic: leaves crossed by the direct walk of one ray to the box edge
  * (geometry only), in the given precision form.  Writes up to maxn leaf

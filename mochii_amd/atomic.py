@@ -127,6 +127,50 @@ def read_cooling_fit(path):
     return [tuple(_floats(r[:2])) for r in rows[1:1 + n]]
 
 
+def read_nlevel(path):
+    """nlevel_load (src/nlevel_mod.f90:61-108): one n-level atom file
+    (data/atomic/nlevel_<el>_<stage>.txt) -> dict with the level list
+    (g, E [cm^-1]), the radiative rows (l, u, A [s^-1]) and the collision
+    strength fits (l, u, type, C, dE [Ry], st_lo, st_hi, logf, Chebyshev
+    coefficients).  Level indices stay 1-based as in the file."""
+    with open(path) as f:
+        rows = [ln.split() for ln in f if ln.strip() and not ln.lstrip().startswith("#")]
+    at = {"g": [], "e_cm1": [], "rad": [], "ups": []}
+    i = 0
+    while i < len(rows):
+        key, n = rows[i][0], int(rows[i][1])
+        body = rows[i + 1:i + 1 + n]
+        if key == "NLEV":
+            at["g"] = [float(r[1]) for r in body]
+            at["e_cm1"] = [float(r[2]) for r in body]
+        elif key == "NRAD":
+            at["rad"] = [[int(r[0]), int(r[1]), float(r[2])] for r in body]
+        elif key == "NUPS":
+            for r in body:
+                nc = int(r[8])
+                at["ups"].append({"l": int(r[0]), "u": int(r[1]), "type": int(r[2]),
+                                  "C": float(r[3]), "dE": float(r[4]), "st_lo": float(r[5]),
+                                  "st_hi": float(r[6]), "logf": int(r[7]),
+                                  "coef": [float(v) for v in r[9:9 + nc]]})
+        i += 1 + n
+    return at
+
+
+def nlevel_files(atomic_dir):
+    """The n-level atoms of a data directory: {'<el>_<stage>': atom}."""
+    out = {}
+    for fn in sorted(os.listdir(atomic_dir)):
+        if fn.startswith("nlevel_") and fn.endswith(".txt") and not fn.endswith("_full.txt"):
+            out[fn[len("nlevel_"):-len(".txt")]] = read_nlevel(os.path.join(atomic_dir, fn))
+    return out
+
+
+def load_nlevel(path=SNAPSHOT):
+    """The n-level atoms of the data snapshot: {'<el>_<stage>': atom}."""
+    with open(path) as f:
+        return json.load(f).get("nlevel", {})
+
+
 def read_registry(atomic_dir):
     """Every element file of a data directory with its stage cooling fits:
     {name: element dict with 'cooling': {stage (1-based): fit}}."""

@@ -318,6 +318,7 @@ extern "C" int mc_init(const mc_params *p, mc_ctx **out)
     // launch-record buffers: batches of up to 4M packets
     c->packet_order = MC_ORDER_DIRECTION;
     c->mom_on = 1;
+    c->warm_levels = MC_WARM_LEVELS_DEFAULT;
     if (mc_packet_capacity(c, p->nphotons)) {
         mc_finalize(c);
         return -100;
@@ -707,6 +708,17 @@ extern "C" int mc_set_warm_start(mc_ctx *c, int mode)
     return 0;
 }
 
+extern "C" int mc_set_warm_levels(mc_ctx *c, int levels)
+{
+    CHECK_CTX(c);
+    if (levels < 0 || levels > 24) {
+        mc_set_error("mc_set_warm_levels: levels must be in 0..24");
+        return -1;
+    }
+    c->warm_levels = levels;
+    return 0;
+}
+
 extern "C" int mc_thermal_stats(mc_ctx *c, int64_t *out4)
 {
     CHECK_CTX(c);
@@ -802,6 +814,37 @@ extern "C" int mc_timing(mc_ctx *c, double *out4)
     CHECK_CTX(c);
     for (int k = 0; k < 4; ++k) out4[k] = c->times[k];
     return 0;
+}
+
+extern "C" int mc_qmc_uniforms(mc_ctx *c, int stream, int64_t first_index, int n, int ndim,
+                               double *out)
+{
+    CHECK_CTX(c);
+    if ((stream != 1 && stream != 2) || first_index < 0 || n < 1 || ndim < 1 ||
+        ndim > MC_QMC_DIM || !out || (int64_t)n * ndim > (int64_t)1 << 26) {
+        mc_set_error("mc_qmc_uniforms: stream 1 or 2, 1 <= ndim <= 12, 1 <= n, n ndim <= 2^26");
+        return -1;
+    }
+    return mc_run_qmc_dump(c, stream, first_index, n, ndim, out);
+}
+
+extern "C" int mc_launch_records(mc_ctx *c, int64_t first_id, int n, double *rec,
+                                 int32_t *bin_leaf)
+{
+    CHECK_CTX(c);
+    if (first_id < 1 || n < 1 || n > c->pk.cap || !rec || !bin_leaf) {
+        mc_set_error("mc_launch_records: need first_id >= 1, 1 <= n <= the launch batch, outputs");
+        return -1;
+    }
+    if (!c->state_ready) {
+        mc_set_error("mc_launch_records: gas state not set up");
+        return -1;
+    }
+    if (c->p.ion_add_dust && c->dn < 2) {
+        mc_set_error("ion_add_dust needs mc_ion_dust_table first");
+        return -1;
+    }
+    return mc_run_launch_dump(c, first_id, n, rec, bin_leaf);
 }
 
 extern "C" int mc_trace_ray(mc_ctx *c, int precision, const double *pos, const double *dir,

@@ -312,6 +312,7 @@ struct mc_ctx {
     // learned warm start of the thermal solve
     MlpState mlp;
     int warm_mode;            // MC_WARM_*
+    int warm_levels;          // bisection levels before a warm start (mc_set_warm_levels)
     float *d_feat, *d_pred;   // nleaf x 8 features, nleaf x 4 predictions
     int64_t thermal_stats[4]; // evaluations (bracketed leaves), fallbacks, bracketed, leaves
     // direction-sorted walk order of stellar batches
@@ -341,6 +342,7 @@ struct mc_ctx {
     double *d_mom, *d_mom_coef, *d_mom_tab, *d_mom_bseg;
     int32_t *d_mom_band;
     size_t mom_cap;
+    int mom_warned;                 // fallback to per-coefficient sums reported
     // last pass results
     double ledger[3];
     int64_t segs[3];          // direct, scattered, metal transitions over scored segments
@@ -373,6 +375,8 @@ int mc_sort_records(mc_ctx *c, int64_t n);
 int mc_run_trace(mc_ctx *c, int precision, const double *pos, const double *dir, int32_t *cells,
                  int maxn, int32_t *n);
 int mc_run_dust_temp(mc_ctx *c, const double *d_heat);
+int mc_run_qmc_dump(mc_ctx *c, int stream, int64_t first, int n, int ndim, double *out);
+int mc_run_launch_dump(mc_ctx *c, int64_t first_id, int n, double *rec, int32_t *bin_leaf);
 int mc_run_dust_emission(mc_ctx *c, double report[MC_DUSTRT_NREPORT]);
 
 // host set-up of the infrared band (mc_dustir.cpp)

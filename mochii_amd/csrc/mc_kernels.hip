@@ -587,8 +587,14 @@ __device__ __forceinline__ void tally_one(const KArgs &a, const Lane &L, int il,
                 pm2 = pm1;
                 pm1 = pq;
             }
-            if (a.add_dust && L.dabs != 0.0)
-                unsafeAtomicAdd(a.s.hhe + (size_t)il * 8 + 6, pl * L.dabs);
+            if (a.add_dust && L.dabs != 0.0) {
+                // dust absorption keeps its per-leaf sum: into the lane's
+                // replica row of a hot leaf, which k_fold_hot folds into
+                // the leaf's row (a direct write there would be replaced)
+                double *h, *j;
+                tally_rows(a, il, h, j);
+                unsafeAtomicAdd(h + 6, pl * L.dabs);
+            }
             return;
         }
     }
@@ -683,7 +689,11 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
                         const double v = in ? L.s_pl * L.dabs : 0.0;
                         if (__ballot(v != 0.0)) {
                             sum = wave_sum(v);
-                            if (lane == leader) unsafeAtomicAdd(a.s.hhe + (size_t)lil * 8 + 6, sum);
+                            if (lane == leader) {
+                                double *h, *j;
+                                tally_rows(a, lil, h, j);
+                                unsafeAtomicAdd(h + 6, sum);
+                            }
                         }
                     }
                     if (in) has = false;
@@ -2122,7 +2132,8 @@ constexpr int kNewtonMax = 24;
 template <int W, int MT, int LAY>
 __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCool cl, bool caseA,
                                                     double y, double w, double tmin, double tmax,
-                                                    double d2cm, const float *pred, MetalSw ms)
+                                                    double d2cm, const float *pred, MetalSw ms,
+                                                    int levels)
 {
     const int il = blockIdx.x * kBlock + threadIdx.x;
     double v[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -2187,6 +2198,16 @@ __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCo
                 te = sqrt(tlo * thi);
             } else {
                 bracketed = 1;
+                // the reference's bisection for the first `levels` levels:
+                // its root lies in the level's cell [tlo, thi], so the warm
+                // root found inside that cell is the bisection's, or another
+                // crossing within the cell's width ln(te_max/te_min) / 2^levels
+                for (int it = 0; it < levels && thi / tlo - 1.0 >= 1.0e-5; ++it) {
+                    const double tm = sqrt(tlo * thi);
+                    ++evals;
+                    if (net_rate(cl, c, tm, caseA, y, xHI, xHeI, xHeII, ne) > 0.0) tlo = tm;
+                    else thi = tm;
+                }
                 double t0 = s.Te[il];
                 if constexpr (W == MC_WARM_MLP) {
                     const float *pr = pred + (size_t)il * 4;
@@ -2720,6 +2741,19 @@ __global__ __launch_bounds__(kBlock) void k_ir_heat(DevGrid g, const double *ira
     block_reduce<3>(v, ismax, partials);
 }
 
+// The launch's Sobol/Owen uniforms u(index, d) of one stream (0 stellar,
+// 1 diffuse) for n consecutive 0-based indices, row-major n x ndim: the
+// same qmc_u as k_launch / k_launch_diffuse, for comparison with the
+// reference's tabulated sequences.
+__global__ __launch_bounds__(kBlock) void k_qmc_dump(const DevTables *T, uint32_t first, int n,
+                                                     int ndim, int stream, double *out)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= (int64_t)n * ndim) return;
+    const int r = (int)(i / ndim), d = (int)(i % ndim);
+    out[i] = qmc_u(T, first + (uint32_t)r, d, stream);
+}
+
 template <int P, int G>
 __global__ void k_trace(KArgs a, double x, double y, double z, double kx, double ky, double kz,
                         int32_t *cells, int maxn, int32_t *n)
@@ -3099,7 +3133,8 @@ int mc_run_thermal(mc_ctx *c, double out[4])
     const bool caseA = c->p.case_A != 0;
     if (metal_sync(c)) return -100;
 #define MC_THERMAL_ARGS c->grid, c->st, c->cool, caseA, c->p.He_abund, c->p.ion_relax, \
-                        c->p.te_min, c->p.te_max, c->p.distance2cm, c->d_pred, make_sw(c, true)
+                        c->p.te_min, c->p.te_max, c->p.distance2cm, c->d_pred, make_sw(c, true), \
+                        c->warm_levels
 #define MC_THERMAL_W(MT_, LAY_)                                                               \
     do {                                                                                    \
         if (mode == MC_WARM_MLP)                                                            \
@@ -3395,4 +3430,54 @@ int mc_run_dust_emission(mc_ctx *c, double rep[MC_DUSTRT_NREPORT])
     b.times[1] = t_walk;
     b.times[2] = t_rest;
     return rc;
+}
+
+// Sobol/Owen uniforms of the launch (k_qmc_dump) into host memory.
+int mc_run_qmc_dump(mc_ctx *c, int stream, int64_t first, int n, int ndim, double *out)
+{
+    double *d = nullptr;
+    const size_t cnt = (size_t)n * (size_t)ndim;
+    MC_HIP(hipMalloc(&d, sizeof(double) * cnt));
+    k_qmc_dump<<<(unsigned)((cnt + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(
+        c->d_tab, (uint32_t)first, n, ndim, stream - 1, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                                            c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    MC_HIP(e);
+    return 0;
+}
+
+// Launch records of stellar packets first_id .. first_id + n - 1 (k_launch
+// into the context's packet buffers, launch order): rec n x 7 {x, y, z, kx,
+// ky, kz, E}, bin_leaf n x 2 {0-based band bin, 0-based leaf or -1}.
+int mc_run_launch_dump(mc_ctx *c, int64_t first_id, int n, double *rec, int32_t *bin_leaf)
+{
+    KArgs a = make_args(c);
+    a.first_id = first_id;
+    a.npk = n;
+    a.stream = 1;
+    a.pk.key = nullptr;
+    const int nlds = c->car ? 0 : c->n_lds_nodes;
+    k_launch<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock,
+               (size_t)nlds * (sizeof(double4) + sizeof(int32_t)), c->stream>>>(a, nlds);
+    MC_HIP(hipGetLastError());
+    std::vector<double> col((size_t)n);
+    const double *src[7] = {c->pk.x, c->pk.y, c->pk.z, c->pk.kx, c->pk.ky, c->pk.kz, c->pk.E};
+    for (int k = 0; k < 7; ++k) {
+        MC_HIP(hipMemcpyAsync(col.data(), src[k], sizeof(double) * n, hipMemcpyDeviceToHost,
+                              c->stream));
+        MC_HIP(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < n; ++i) rec[(size_t)i * 7 + k] = col[(size_t)i];
+    }
+    std::vector<int32_t> ib((size_t)n);
+    const int32_t *isrc[2] = {c->pk.inu, c->pk.leaf};
+    for (int k = 0; k < 2; ++k) {
+        MC_HIP(hipMemcpyAsync(ib.data(), isrc[k], sizeof(int32_t) * n, hipMemcpyDeviceToHost,
+                              c->stream));
+        MC_HIP(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < n; ++i) bin_leaf[(size_t)i * 2 + k] = ib[(size_t)i];
+    }
+    return 0;
 }

@@ -20,6 +20,7 @@
 // Ar band), which costs only moment memory; degree 2 does not reach 1e-7
 // within 512 sub-bins per band bin.  The FP64 form keeps the reference's
 // per-coefficient sums.
+#include <cstdio>
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -220,9 +221,22 @@ int mc_mom_ready(mc_ctx *c)
         if (c->d_mom) (void)hipFree(c->d_mom);
         c->d_mom = nullptr;
         c->mom_cap = 0;
-        if (hipMalloc(&c->d_mom, need) != hipSuccess) {
-            mc_set_error("energy-moment tallies: hipMalloc failed");
-            return -1;
+        // the rows are zeroed and read once per pass (2 x need bytes of
+        // traffic): beyond half the free device memory, or when the
+        // allocation fails, the walk keeps the per-coefficient sums
+        size_t fr = 0, tot = 0;
+        const bool fits = hipMemGetInfo(&fr, &tot) == hipSuccess && need <= fr / 2;
+        if (!fits || hipMalloc(&c->d_mom, need) != hipSuccess) {
+            (void)hipGetLastError();
+            c->d_mom = nullptr;
+            if (!c->mom_warned)
+                std::fprintf(stderr,
+                             "mochii_gpu: energy-moment tallies need %.3g GB (%zu rows x %d "
+                             "sub-bins), more than half the free device memory (%.3g GB); the "
+                             "walk keeps the per-coefficient sums\n",
+                             (double)need * 1e-9, rows, c->mom.nsub, (double)fr * 1e-9);
+            c->mom_warned = 1;
+            return 0;
         }
         c->mom_cap = need;
     }

@@ -1,9 +1,10 @@
 """Write mochii_amd/data/atomic_registry.json from a MoCHII data/atomic
 directory (element_<el>.txt and cooling_<el>_<i>.txt in the reference's text
-formats).  The JSON is a data snapshot of the published coefficients those
+formats) and the n-level atoms nlevel_<el>_<stage>.txt.  The JSON is a data snapshot of the published coefficients those
 files hold (VFKY96 / VY95 photoionisation, Voronov collisional ionisation,
 Badnell / CHIANTI recombination, Huang et al. 2023 and Kingdon & Ferland 1996
-charge exchange, CHIANTI line-cooling fits), with each file's provenance
+charge exchange, CHIANTI line-cooling fits, CHIANTI level energies,
+A-values and Burgess-Tully fits of the collision strengths), with each file's provenance
 header kept beside it.
 
     python mochii_amd/data/make_atomic_json.py /path/to/MoCHII/data/atomic
@@ -13,7 +14,7 @@ import os
 import sys
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", ".."))
-from mochii_amd.atomic import ORDER, SNAPSHOT, read_registry  # noqa: E402
+from mochii_amd.atomic import ORDER, SNAPSHOT, nlevel_files, read_registry  # noqa: E402
 
 
 def provenance(path):
@@ -31,13 +32,17 @@ def main(atomic_dir, out=SNAPSHOT):
         for st in reg[name]["cooling"]:
             fn = f"cooling_{name}_{st}.txt"
             prov[fn] = provenance(os.path.join(atomic_dir, fn))[:4]
-    doc = {"source": "MoCHII data/atomic (element and cooling-fit files)",
+    nlev = nlevel_files(atomic_dir)
+    for key in nlev:
+        prov[f"nlevel_{key}.txt"] = provenance(os.path.join(atomic_dir, f"nlevel_{key}.txt"))[:3]
+    doc = {"source": "MoCHII data/atomic (element, cooling-fit and n-level atom files)",
            "elements": {k: dict(v, cooling={str(s): f for s, f in v["cooling"].items()})
                         for k, v in reg.items()},
+           "nlevel": nlev,
            "provenance": prov}
     with open(out, "w") as f:
         json.dump(doc, f, indent=1)
-    print(f"wrote {out}: {len(reg)} elements")
+    print(f"wrote {out}: {len(reg)} elements, {len(nlev)} n-level atoms")
 
 
 if __name__ == "__main__":

@@ -125,6 +125,9 @@ def library():
         "mc_hydrogen_balance_residual_grid": ([vp, dp, dp], C.c_int),
         "mc_diffuse_info": ([vp, dp, i64p], C.c_int),
         "mc_set_warm_start": ([vp, C.c_int], C.c_int),
+        "mc_set_warm_levels": ([vp, C.c_int], C.c_int),
+        "mc_qmc_uniforms": ([vp, C.c_int, C.c_int64, C.c_int, C.c_int, dp], C.c_int),
+        "mc_launch_records": ([vp, C.c_int64, C.c_int, dp, ip], C.c_int),
         "mc_set_packet_order": ([vp, C.c_int], C.c_int),
         "mc_set_tally_moments": ([vp, C.c_int], C.c_int),
         "mc_tally_moment_info": ([vp, C.POINTER(C.c_int32), dp], C.c_int),

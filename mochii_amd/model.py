@@ -231,6 +231,28 @@ class Model:
     def set_warm_start(self, mode):
         check(self._L.mc_set_warm_start(self._h, int(mode)), "mc_set_warm_start")
 
+    def set_warm_levels(self, levels):
+        """Bisection levels a warm-started thermal solve runs first."""
+        check(self._L.mc_set_warm_levels(self._h, int(levels)), "mc_set_warm_levels")
+
+    def qmc_uniforms(self, stream, first_index, n, ndim):
+        """The launch's Sobol/Owen uniforms of stream 1 (stellar) or 2
+        (diffuse), computed on the device: (n, ndim)."""
+        out = np.zeros((n, ndim))
+        check(self._L.mc_qmc_uniforms(self._h, int(stream), int(first_index), int(n), int(ndim),
+                                      _dp(out)), "mc_qmc_uniforms")
+        return out
+
+    def launch_records(self, first_id, n):
+        """Stellar launch records of ids first_id .. first_id+n-1 as the
+        pass computes them: (rec (n, 7) x y z kx ky kz E, bin_leaf (n, 2))."""
+        rec = np.zeros((n, 7))
+        bl = np.zeros((n, 2), dtype=np.int32)
+        check(self._L.mc_launch_records(self._h, int(first_id), int(n), _dp(rec),
+                                        bl.ctypes.data_as(C.POINTER(C.c_int32))),
+              "mc_launch_records")
+        return rec, bl
+
     def thermal_stats(self):
         a = np.zeros(4, dtype=np.int64)
         check(self._L.mc_thermal_stats(self._h, a.ctypes.data_as(C.POINTER(C.c_int64))),

@@ -265,7 +265,63 @@ typedef struct {
     double *mraw, *mgam, *mheat, *mfrac;
     /* dust thermal-infrared band, NULL until its setup */
     orc_dustir *ir;
+    /* worker threads of the per-leaf thermal update (1 = serial) */
+    int     solve_threads;
 } orc_ctx;
+
+/* ---- n-level atoms (src/nlevel_mod.f90) ---------------------------------- */
+#define ORC_NL_MAXLEV 40
+#define ORC_NL_MAXTR  600
+typedef struct {
+    int    nlev;
+    double g[ORC_NL_MAXLEV], e_cm1[ORC_NL_MAXLEV];
+    int    nrad;
+    int    rl[ORC_NL_MAXTR], ru[ORC_NL_MAXTR];        /* 1-based levels */
+    double A[ORC_NL_MAXTR];
+    int    nups;
+    int    ul_l[ORC_NL_MAXTR], ul_u[ORC_NL_MAXTR], ttype[ORC_NL_MAXTR];
+    int    logf[ORC_NL_MAXTR], nc[ORC_NL_MAXTR];
+    double cups[ORC_NL_MAXTR], dery[ORC_NL_MAXTR], st_lo[ORC_NL_MAXTR], st_hi[ORC_NL_MAXTR];
+    double coef[ORC_NL_MAXTR][16];
+} orc_nlatom;
+
+/* (log T, log n_e) cooling table of one ion (src/nlevel_cooling_mod.f90:59-85):
+ * log10 T = 2 .. 6 step 0.01, log10 n_e = -6 .. 10 step 0.1; Ssup[j][i] */
+#define ORC_NLC_NT  401
+#define ORC_NLC_NNE 161
+#define ORC_NLC_TLO 2.0
+#define ORC_NLC_DT  0.01
+#define ORC_NLC_NELO (-6.0)
+#define ORC_NLC_DNE 0.1
+typedef struct {
+    int    ready;
+    double lamref[ORC_NLC_NT];
+    double Ssup[ORC_NLC_NNE * ORC_NLC_NT];
+} orc_nlcool;
+
+/* a recombination-line table (src/sh95_mod.f90:24-31): etab[line][T][ne] */
+#define ORC_SH95_MAXG 32
+#define ORC_SH95_MAXL 12
+typedef struct {
+    int    nT, nD, nlin;
+    double tgrid[ORC_SH95_MAXG], dgrid[ORC_SH95_MAXG];
+    double wls[ORC_SH95_MAXL];
+    double etab[ORC_SH95_MAXL * ORC_SH95_MAXG * ORC_SH95_MAXG];
+} orc_sh95;
+
+double orc_ups_eval(const orc_nlatom *a, int k, double T);
+void   orc_emitting_level(const orc_nlatom *a, int i1, int i2, int *lo, int *hi, double *de);
+void   orc_nlevel_prepT(const orc_nlatom *a, double T, double *Rcoll, double *Rrad);
+double orc_nlevel_cooling_solve(const orc_nlatom *a, const double *Rcoll, const double *Rrad,
+                                double ne);
+double orc_nlevel_cooling_at(const orc_nlatom *a, double T, double ne);
+void   orc_nlevel_emissivities(const orc_nlatom *a, double T, double ne, double *emis);
+double orc_nlevel_line_ident(const orc_nlatom *a, int k);
+void   orc_nlcool_build(const orc_nlatom *a, orc_nlcool *t);
+double orc_nlcool_coeff(const orc_nlcool *t, double T, double ne);
+double orc_nlcool_low_density(const orc_nlcool *t, double T);
+double orc_nlcool_apply(const orc_nlcool *t, double lam_fit, double T, double ne);
+double orc_sh95_emis(const orc_sh95 *t, int k, double T, double ne);
 
 orc_ctx *orc_create(const orc_params *p);
 void     orc_destroy(orc_ctx *c);
@@ -294,6 +350,10 @@ double   orc_net_rate(const orc_ctx *c, int il, double T, double *xHI, double *x
                       double *xHeII, double *ne);
 void     orc_thermal_update(orc_ctx *c, double *max_dx, double *max_dte, double *dx_vol,
                             double *dte_vol);
+/* threads of the per-leaf solves of orc_thermal_update; the leaves are
+ * independent and the metrics are summed in leaf order afterwards, so the
+ * result does not depend on the count */
+void     orc_set_solve_threads(orc_ctx *c, int n);
 
 /* trace metals (src/species_mod.f90) */
 int      orc_species_setup(orc_ctx *c, int nel, const orc_element *els);

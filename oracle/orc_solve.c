@@ -16,6 +16,8 @@
  */
 #include <math.h>
 #include <float.h>
+#include <pthread.h>
+#include <stdlib.h>
 #include "mc_oracle.h"
 
 void orc_solve_ion_cell(double gH, double gHe1, double gHe2, double nH, double T, int caseA,
@@ -146,53 +148,98 @@ double orc_net_rate(const orc_ctx *c, int il, double T, double *xHI, double *xHe
     return net;
 }
 
-/* gas_thermal_update (src/thermal_mod.f90:124-249) on one process */
+/* gas_thermal_update (src/thermal_mod.f90:124-249): the per-leaf
+ * bisection of T_e with the ionization solved at each trial T, then the
+ * relaxation of x.  Leaves are solved by solve_threads workers into
+ * per-leaf results; the metrics are then summed in leaf order, as the
+ * reference's loop sums them on one process. */
+typedef struct {
+    orc_ctx *c;
+    int lo, hi;
+    double *te, *x3, *ne;            /* per leaf: T_e, (xHI xHeI xHeII), ne */
+} thermal_work;
+
+static void thermal_leaf(const orc_ctx *c, int il, double *te_out, double *x_out, double *ne_out)
+{
+    double xHI = c->xHI[il], xHeI = c->xHeI[il], xHeII = c->xHeII[il], ne, te;
+    double tlo = c->p.te_min, thi = c->p.te_max;
+    double net_lo = orc_net_rate(c, il, tlo, &xHI, &xHeI, &xHeII, &ne);
+    double net_hi = orc_net_rate(c, il, thi, &xHI, &xHeI, &xHeII, &ne);
+    if (net_lo <= 0.0) {
+        te = tlo;
+    } else if (net_hi >= 0.0) {
+        te = thi;
+    } else {
+        for (int it = 1; it <= 60; ++it) {
+            double tmid = sqrt(tlo * thi);
+            double net_mid = orc_net_rate(c, il, tmid, &xHI, &xHeI, &xHeII, &ne);
+            if (net_mid > 0.0) tlo = tmid; else thi = tmid;
+            if (thi / tlo - 1.0 < 1.0e-5) break;
+        }
+        te = sqrt(tlo * thi);
+    }
+    orc_net_rate(c, il, te, &xHI, &xHeI, &xHeII, &ne);
+    *te_out = te;
+    x_out[0] = xHI; x_out[1] = xHeI; x_out[2] = xHeII;
+    *ne_out = ne;
+}
+
+static void *thermal_worker(void *arg)
+{
+    thermal_work *w = (thermal_work *)arg;
+    for (int il = w->lo; il < w->hi; ++il)
+        if (w->c->nH[il] > 0.0) thermal_leaf(w->c, il, &w->te[il], &w->x3[3 * il], &w->ne[il]);
+    return NULL;
+}
+
+void orc_set_solve_threads(orc_ctx *c, int n) { c->solve_threads = n < 1 ? 1 : n; }
+
 void orc_thermal_update(orc_ctx *c, double *max_dx, double *max_dte, double *dx_vol,
                         double *dte_vol)
 {
     const double w = c->p.ion_relax, y = c->p.He_abund;
+    const int nl = c->nleaf;
+    double *te = (double *)malloc(sizeof(double) * (nl + 1));
+    double *x3 = (double *)malloc(sizeof(double) * 3 * (nl + 1));
+    double *nes = (double *)malloc(sizeof(double) * (nl + 1));
+    int nt = c->solve_threads < 1 ? 1 : c->solve_threads;
+    if (nt > nl) nt = nl > 0 ? nl : 1;
+    thermal_work *wk = (thermal_work *)calloc(nt, sizeof(thermal_work));
+    pthread_t *th = (pthread_t *)calloc(nt, sizeof(pthread_t));
+    for (int t = 0; t < nt; ++t) {
+        wk[t].c = c; wk[t].te = te; wk[t].x3 = x3; wk[t].ne = nes;
+        wk[t].lo = (int)((int64_t)nl * t / nt);
+        wk[t].hi = (int)((int64_t)nl * (t + 1) / nt);
+    }
+    for (int t = 1; t < nt; ++t) pthread_create(&th[t], NULL, thermal_worker, &wk[t]);
+    thermal_worker(&wk[0]);
+    for (int t = 1; t < nt; ++t) pthread_join(th[t], NULL);
     double mdx = 0.0, mdt = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
-    for (int il = 0; il < c->nleaf; ++il) {
+    for (int il = 0; il < nl; ++il) {
         const double nH = c->nH[il];
         if (nH <= 0.0) { c->ne[il] = 0.0; continue; }
-        double xHI = c->xHI[il], xHeI = c->xHeI[il], xHeII = c->xHeII[il], ne, te;
-        double tlo = c->p.te_min, thi = c->p.te_max;
-        double net_lo = orc_net_rate(c, il, tlo, &xHI, &xHeI, &xHeII, &ne);
-        double net_hi = orc_net_rate(c, il, thi, &xHI, &xHeI, &xHeII, &ne);
-        if (net_lo <= 0.0) {
-            te = tlo;
-        } else if (net_hi >= 0.0) {
-            te = thi;
-        } else {
-            for (int it = 1; it <= 60; ++it) {
-                double tmid = sqrt(tlo * thi);
-                double net_mid = orc_net_rate(c, il, tmid, &xHI, &xHeI, &xHeII, &ne);
-                if (net_mid > 0.0) tlo = tmid; else thi = tmid;
-                if (thi / tlo - 1.0 < 1.0e-5) break;
-            }
-            te = sqrt(tlo * thi);
-        }
-        orc_net_rate(c, il, te, &xHI, &xHeI, &xHeII, &ne);
-        double nHI = (1.0 - w) * c->xHI[il] + w * xHI;
-        double nHeI = (1.0 - w) * c->xHeI[il] + w * xHeI;
-        double nHeII = (1.0 - w) * c->xHeII[il] + w * xHeII;
+        const double t_e = te[il], ne = nes[il];
+        double nHI = (1.0 - w) * c->xHI[il] + w * x3[3 * il];
+        double nHeI = (1.0 - w) * c->xHeI[il] + w * x3[3 * il + 1];
+        double nHeII = (1.0 - w) * c->xHeII[il] + w * x3[3 * il + 2];
         double xHeIII = 1.0 - nHeI - nHeII;
         if (xHeIII < 0.0) xHeIII = 0.0;
         double ne_new = nH * ((1.0 - nHI) + y * (nHeII + 2.0 * xHeIII));
         if (c->p.metal_ne && c->use_metals && c->nel > 0)
-            ne_new = orc_charge_neutrality_ne(c, il, te, nH, ne_new, nHI, ne);
+            ne_new = orc_charge_neutrality_ne(c, il, t_e, nH, ne_new, nHI, ne);
         double d = fabs(nHI - c->xHI[il]);
         if (d > mdx) mdx = d;
-        double dt = fabs(te - c->Te[il]) / c->Te[il];
+        double dt = fabs(t_e - c->Te[il]) / c->Te[il];
         if (dt > mdt) mdt = dt;
         double v = leaf_vol(c, il);
         s1 = s1 + v * d;
         s2 = s2 + v * (1.0 - nHI);
-        s3 = s3 + v * ne_new * fabs(te - c->Te[il]);
-        s4 = s4 + v * ne_new * te;
+        s3 = s3 + v * ne_new * fabs(t_e - c->Te[il]);
+        s4 = s4 + v * ne_new * t_e;
         c->xHI[il] = nHI; c->xHeI[il] = nHeI; c->xHeII[il] = nHeII;
-        c->ne[il] = ne_new; c->Te[il] = te;
+        c->ne[il] = ne_new; c->Te[il] = t_e;
     }
+    free(te); free(x3); free(nes); free(wk); free(th);
     *max_dx = mdx; *max_dte = mdt;
     *dx_vol = s1 / (s2 > DBL_MIN ? s2 : DBL_MIN);
     *dte_vol = s3 / (s4 > DBL_MIN ? s4 : DBL_MIN);

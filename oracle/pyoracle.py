@@ -133,6 +133,8 @@ def _load():
         f.argtypes = [C.c_double] * (2 if name == "orc_planck_shape" else 1)
     lib.orc_cooling_setup.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     lib.orc_thermal_update.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+    lib.orc_set_solve_threads.argtypes = [C.c_void_p, C.c_int]
+    lib.orc_set_solve_threads.restype = None
     lib.orc_net_rate.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
     lib.orc_net_rate.restype = C.c_double
     lib.orc_gaunt_ff.argtypes = [C.c_float, C.c_float, C.POINTER(C.c_float),
@@ -476,7 +478,11 @@ class Model:
         Ti = np.ascontiguousarray(Ti, dtype=np.float64)
         lib().orc_cooling_setup(self.c, A.size, ptr(A), ptr(Ti))
 
-    def thermal_update(self):
+    def thermal_update(self, nthreads=None):
+        """gas_thermal_update; the per-leaf solves on nthreads threads (default
+        all host cores, capped at 16); the result does not depend on it."""
+        n = nthreads or min(os.cpu_count() or 1, 16)
+        lib().orc_set_solve_threads(self.c, int(n))
         a, b, c, d = C.c_double(), C.c_double(), C.c_double(), C.c_double()
         lib().orc_thermal_update(self.c, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
         return a.value, b.value, c.value, d.value

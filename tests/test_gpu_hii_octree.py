@@ -1,0 +1,208 @@
+"""The benchmarked path against the FP64 oracle, through the C-ABI, on a
+reduced form of the octree H II region workload: n_H lognormal (mean 100,
+sigma_ln 0.5, k^-11/3) on a 32^3 field, octree to 5 levels where the
+density varies by more than 0.3 dex, He/H = 0.1, trace metals C N O Ne S Ar
+(15 transitions), explicit diffuse recombination packets (case A), thermal
+balance.  Covers the octree walk with the metal path (src/raytrace_amr.f90:
+422-486, src/species_mod.f90:291-343), the thermal update on leaves of
+unequal volume (src/thermal_mod.f90:124-249), in both precision forms; the
+mixed form's energy-moment tallies with grain absorption on hot leaves; and
+the dusty two-source nebula (HG scattering)."""
+import numpy as np
+import pytest
+
+from mochii_amd import MC_FP64, MC_MIXED
+from mochii_amd.workloads import (HI_COOLING_FIT, dusty_nebula, elements_of, hii_octree,
+                                  synthetic_kext)
+from tests.cases import gpu_model, oracle_model, rel_close
+
+pytestmark = pytest.mark.gpu
+
+PC2CM = 3.0856776e18
+
+
+def _state(octree, nH, seed=5):
+    """A partly ionised H + He state: a warm ionised core of radius 0.45 of
+    the box half width around the star, neutral outside."""
+    x, y, z, lev, half = octree
+    r = np.sqrt(x**2 + y**2 + z**2) / half
+    rng = np.random.default_rng(seed)
+    n = r.size
+    core = r < 0.45
+    xHI = np.where(core, 10 ** rng.uniform(-4, -2.5, n), 1.0)
+    xHeI = np.where(core, 10 ** rng.uniform(-3, -1, n), 1.0)
+    xHeII = np.where(core, rng.uniform(0.3, 0.9, n), 0.0)
+    Te = np.where(core, rng.uniform(8e3, 1.5e4, n), 1e3)
+    ne = nH * ((1 - xHI) + 0.1 * (xHeII + 2 * np.maximum(0, 1 - xHeI - xHeII)))
+    return dict(xHI=xHI, xHeI=xHeI, xHeII=xHeII, Te=Te, ne=ne)
+
+
+def _pair(prec, nphot=20000, dust=False):
+    params, nH, octree = hii_octree(n_field=32, nphotons=nphot, precision=prec, lmax=5,
+                                    diffuse=True, metals=True)
+    extra = {}
+    if dust:
+        params.update(ion_add_dust=1, ion_dust_scatter=1, lambda_ref=0.55)
+        extra = dict(rhokap=nH * 4.868e-22 * PC2CM, dust_table=synthetic_kext())
+    orc = oracle_model(params, nH, octree=octree, **extra)
+    gm = gpu_model(params, nH, octree=octree, **extra)
+    st = _state(octree, nH)
+    orc.set_state(**st)
+    orc.opacity_fill()
+    gm.set_state(**st)
+    gm.gas_opacity_fill()
+    orc.cooling_setup([a for a, _ in HI_COOLING_FIT], [t for _, t in HI_COOLING_FIT])
+    gm.cooling_setup(HI_COOLING_FIT)
+    return params, nH, octree, orc, gm
+
+
+def _pass(orc, gm):
+    orc.transport_pass(nthreads=8)
+    orc.rates_compute()
+    gm.ionizing_field_and_rates()
+
+
+def _check_pass(params, orc, gm, tol, floor):
+    h = orc._head()
+    Ld, nd = gm.diffuse_info()
+    assert nd == h.diffuse_nphot and nd > 2000, (nd, h.diffuse_nphot)
+    to, tg = orc.tallies(), gm.tallies()
+    for k in ("hhe", "jt"):
+        ok, err = rel_close(tg[k], to[k], tol, floor_frac=floor)
+        assert ok, (k, err)
+    ro, rg = orc.rates(), gm.rates()
+    for k in ("gHI", "gHeI", "gHeII", "hHI", "hHeI", "hHeII"):
+        ok, err = rel_close(rg[k], ro[k], tol, floor_frac=floor)
+        assert ok, (k, err)
+    ntested = 0
+    for ie, el in enumerate(elements_of(params)):
+        ok, err = rel_close(gm.species_tallies(ie), orc.species_tallies(ie), tol, floor_frac=floor)
+        assert ok, (el["name"], "raw", err)
+        (go, ho), (gg, hg) = orc.species_rates(ie), gm.species_rates(ie)
+        ok, err = rel_close(gg, go, tol, floor_frac=floor)
+        assert ok, (el["name"], "gamma", err)
+        ok, err = rel_close(hg, ho, tol, floor_frac=floor)
+        assert ok, (el["name"], "heat", err)
+        ntested += int((go > 0).sum())
+    assert ntested > 5000
+    led = gm.energy_ledger()
+    assert abs(led[1] + led[2] - led[0]) <= 5e-13 * led[0]
+    return to, tg
+
+
+def test_hii_octree_metals_fp64_pass_and_thermal_update_match_oracle():
+    params, nH, octree, orc, gm = _pair(MC_FP64)
+    assert 4000 < nH.size < 40000 and np.unique(octree[3]).size >= 2
+    _pass(orc, gm)
+    to, tg = _check_pass(params, orc, gm, 1e-9, 1e-6)
+    assert tg["seg_direct"] == to["seg_direct"]
+    np.testing.assert_allclose(gm.energy_ledger(), to["ledger"], rtol=1e-12)
+    # thermal update: bisection with metal cooling / heating on leaves of
+    # unequal volume (the volume-weighted metrics)
+    mo = orc.thermal_update()
+    orc.opacity_fill()
+    mg = gm.gas_thermal_update()
+    so, sg = orc.state(), gm.state()
+    rel = np.abs(sg["Te"] / so["Te"] - 1.0)
+    print("T_e: max rel", rel.max(), "bit-equal", np.mean(rel == 0.0))
+    assert np.mean(rel == 0.0) >= 0.98
+    assert rel.max() < 1e-3
+    ion = so["xHI"] < 0.5
+    assert ion.sum() > 100
+    ok, err = rel_close(1.0 - sg["xHI"][ion], 1.0 - so["xHI"][ion], 1e-5)
+    assert ok, err
+    np.testing.assert_allclose(mg, mo, rtol=2e-3, atol=5e-5)
+    vols = (2.0 * octree[4] / 2.0 ** octree[3]) ** 3
+    assert vols.max() / vols.min() >= 8           # unequal leaf volumes enter dx_vol / dte_vol
+    for ie, el in enumerate(elements_of(params)):
+        ok, err = rel_close(gm.species_fractions(ie)[ion], orc.species_fractions(ie)[ion], 1e-4,
+                            floor_frac=1e-8)
+        assert ok, (el["name"], err)
+
+
+def test_hii_octree_metals_mixed_pass_matches_oracle():
+    """The mixed form of the benchmarked walk (FP32 geometry and opacity,
+    energy-moment tallies) against the FP64 oracle."""
+    params, nH, octree, orc, gm = _pair(MC_MIXED)
+    nsub, ferr = gm.tally_moment_info()
+    assert nsub > 16 and ferr <= 1e-7
+    _pass(orc, gm)
+    _check_pass(params, orc, gm, 2e-4, 1e-4)
+    mo = orc.thermal_update()
+    mg = gm.gas_thermal_update()
+    so, sg = orc.state(), gm.state()
+    rel = np.abs(sg["Te"] / so["Te"] - 1.0)
+    print("mixed T_e: max rel", rel.max(), "within 1e-3", np.mean(rel < 1e-3))
+    assert np.mean(rel < 1e-3) >= 0.99
+    np.testing.assert_allclose(mg[:2], mo[:2], rtol=5e-2, atol=1e-3)
+
+
+def test_moment_tallies_keep_grain_absorption_of_hot_leaves():
+    """Mixed metal walk with grains (scattering on): the dust absorption of
+    segments in hot leaves (those next to the star, whose tallies go to
+    replica rows) survives the replica fold, with the moment tallies on and
+    off, and matches the oracle."""
+    params, nH, octree, orc, gm = _pair(MC_MIXED, dust=True)
+    off = gpu_model(params, nH, octree=octree, rhokap=nH * 4.868e-22 * PC2CM,
+                    dust_table=synthetic_kext())
+    off.set_tally_moments(False)
+    off.set_state(**_state(octree, nH))
+    off.gas_opacity_fill()
+    _pass(orc, gm)
+    off.ionizing_field_and_rates()
+    to, ton, toff = orc.tallies(), gm.tallies(), off.tallies()
+    assert ton["seg_direct"] == toff["seg_direct"]
+    # hot leaves: centres within 6 leaf widths of the star (their tallies
+    # go to replica rows); the lit ones carry grain absorption
+    x, y, z, lev, half = octree
+    w = 2.0 * half / 2.0 ** lev
+    hot = np.sqrt(x**2 + y**2 + z**2) < 6.0 * w
+    lit = hot & (to["dust_abs"] > 1e-6 * to["dust_abs"].max())
+    assert lit.sum() >= 8, lit.sum()
+    ok, err = rel_close(ton["dust_abs"][lit], to["dust_abs"][lit], 2e-4)
+    assert ok, err
+    ok, err = rel_close(ton["dust_abs"], toff["dust_abs"], 1e-9, floor_frac=1e-9)
+    assert ok, err
+    ok, err = rel_close(ton["dust_abs"], to["dust_abs"], 2e-4, floor_frac=1e-5)
+    assert ok, err
+    ho = orc.dust_heat()
+    for m in (gm, off):
+        ok, err = rel_close(m.dust("heat_dust"), ho, 2e-4, floor_frac=1e-5)
+        assert ok, err
+    ok, err = rel_close(ton["hhe"], toff["hhe"], 1e-6, floor_frac=1e-9)
+    assert ok, err
+
+
+@pytest.mark.parametrize("prec", [MC_FP64, MC_MIXED])
+def test_dusty_nebula_matches_oracle(prec):
+    """The dusty nebula at 16^3: two seeded sources, dust with albedo 0.4 and
+    HG g = 0.6 scattering; scattered flights and the Lucy estimator against
+    the oracle."""
+    params, nH, rhokap = dusty_nebula(n=16, nphotons=20000, precision=prec)
+    tab = synthetic_kext()
+    orc = oracle_model(params, nH, rhokap=rhokap, dust_table=tab)
+    gm = gpu_model(params, nH, rhokap=rhokap, dust_table=tab)
+    c = (np.arange(16) + 0.5) / 16 * 8.0 - 4.0
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    xHI = np.ones(16 ** 3)
+    for k in range(2):
+        r = np.sqrt((X - params["src_x"][k])**2 + (Y - params["src_y"][k])**2
+                    + (Z - params["src_z"][k])**2).transpose(2, 1, 0).ravel()
+        xHI = np.where(r < 1.2, 1e-3, xHI)
+    orc.set_state(xHI=xHI)
+    gm.set_state(xHI=xHI)
+    _pass(orc, gm)
+    to, tg = orc.tallies(), gm.tallies()
+    assert to["seg_scattered"] > 0.1 * to["seg_direct"]
+    tol, floor = (1e-9, 1e-6) if prec == MC_FP64 else (2e-4, 1e-4)
+    for k in ("hhe", "jt", "dust_abs"):
+        ok, err = rel_close(tg[k], to[k], tol, floor_frac=floor)
+        assert ok, (k, err)
+    if prec == MC_FP64:
+        assert (tg["seg_direct"], tg["seg_scattered"]) == (to["seg_direct"], to["seg_scattered"])
+    led = gm.energy_ledger()
+    np.testing.assert_allclose(led, to["ledger"], rtol=1e-12 if prec == MC_FP64 else 1e-5,
+                               atol=1e-12 * led[0])
+    assert abs(led[1] + led[2] - led[0]) <= 5e-13 * led[0]
+    ok, err = rel_close(gm.dust("heat_dust"), orc.dust_heat(), tol, floor_frac=floor)
+    assert ok, err
