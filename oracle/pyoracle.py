@@ -134,6 +134,18 @@ def _load():
     lib.orc_cooling_setup.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     lib.orc_thermal_update.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
     lib.orc_set_solve_threads.argtypes = [C.c_void_p, C.c_int]
+    for name, args, res in (
+            ("orc_ups_eval", [C.c_void_p, C.c_int, C.c_double], C.c_double),
+            ("orc_nlevel_cooling_at", [C.c_void_p, C.c_double, C.c_double], C.c_double),
+            ("orc_nlevel_emissivities", [C.c_void_p, C.c_double, C.c_double, _dp], None),
+            ("orc_nlevel_line_ident", [C.c_void_p, C.c_int], C.c_double),
+            ("orc_nlcool_build", [C.c_void_p, C.c_void_p], None),
+            ("orc_nlcool_coeff", [C.c_void_p, C.c_double, C.c_double], C.c_double),
+            ("orc_nlcool_low_density", [C.c_void_p, C.c_double], C.c_double),
+            ("orc_nlcool_apply", [C.c_void_p, C.c_double, C.c_double, C.c_double], C.c_double),
+            ("orc_sh95_emis", [C.c_void_p, C.c_int, C.c_double, C.c_double], C.c_double)):
+        f = getattr(lib, name)
+        f.argtypes, f.restype = args, res
     lib.orc_set_solve_threads.restype = None
     lib.orc_net_rate.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
     lib.orc_net_rate.restype = C.c_double
@@ -760,7 +772,90 @@ def ref_lib():
     L.ref_vy95.restype = C.c_double
     L.ref_ci_dere_ratio.argtypes = [C.c_int, C.c_int]
     L.ref_ci_dere_ratio.restype = C.c_double
+    L.ref_nlevel_load.argtypes = [C.c_char_p, C.c_int]
+    L.ref_nlevel_load.restype = C.c_int
+    L.ref_nlevel_cooling_at.argtypes = [C.c_double, C.c_double]
+    L.ref_nlevel_cooling_at.restype = C.c_double
+    L.ref_nlevel_emissivities.argtypes = [C.c_double, C.c_double, _dp]
+    L.ref_nlevel_line_ident.argtypes = [C.c_int]
+    L.ref_nlevel_line_ident.restype = C.c_double
+    L.ref_nlevel_cooling_add.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_int]
+    L.ref_nlevel_cooling_add.restype = C.c_int
+    L.ref_nlevel_cooling_apply.argtypes = [C.c_int] + [C.c_double] * 3
+    L.ref_nlevel_cooling_apply.restype = C.c_double
+    L.ref_nlevel_cooling_lam_se.argtypes = [C.c_int] + [C.c_double] * 2
+    L.ref_nlevel_cooling_lam_se.restype = C.c_double
     return L
+
+
+# ---- n-level atoms, local-n_e cooling tables, recombination lines ---------
+NL_MAXLEV, NL_MAXTR, NLC_NT, NLC_NNE = 40, 600, 401, 161
+
+
+class NlAtom(C.Structure):
+    """Mirror of orc_nlatom."""
+    _fields_ = [("nlev", C.c_int), ("g", C.c_double * NL_MAXLEV), ("e_cm1", C.c_double * NL_MAXLEV),
+                ("nrad", C.c_int), ("rl", C.c_int * NL_MAXTR), ("ru", C.c_int * NL_MAXTR),
+                ("A", C.c_double * NL_MAXTR), ("nups", C.c_int),
+                ("ul_l", C.c_int * NL_MAXTR), ("ul_u", C.c_int * NL_MAXTR),
+                ("ttype", C.c_int * NL_MAXTR), ("logf", C.c_int * NL_MAXTR),
+                ("nc", C.c_int * NL_MAXTR), ("cups", C.c_double * NL_MAXTR),
+                ("dery", C.c_double * NL_MAXTR), ("st_lo", C.c_double * NL_MAXTR),
+                ("st_hi", C.c_double * NL_MAXTR), ("coef", (C.c_double * 16) * NL_MAXTR)]
+
+    @classmethod
+    def make(cls, atom):
+        """From an atom dict (mochii_amd.atomic.read_nlevel)."""
+        a = cls()
+        a.nlev = len(atom["g"])
+        for i, (g, e) in enumerate(zip(atom["g"], atom["e_cm1"])):
+            a.g[i], a.e_cm1[i] = g, e
+        a.nrad = len(atom["rad"])
+        for k, (l, u, A) in enumerate(atom["rad"]):
+            a.rl[k], a.ru[k], a.A[k] = l, u, A
+        a.nups = len(atom["ups"])
+        for k, r in enumerate(atom["ups"]):
+            a.ul_l[k], a.ul_u[k], a.ttype[k], a.logf[k] = r["l"], r["u"], r["type"], r["logf"]
+            a.nc[k], a.cups[k], a.dery[k] = len(r["coef"]), r["C"], r["dE"]
+            a.st_lo[k], a.st_hi[k] = r["st_lo"], r["st_hi"]
+            for j, v in enumerate(r["coef"]):
+                a.coef[k][j] = v
+        return a
+
+    def cooling_at(self, T, ne):
+        return lib().orc_nlevel_cooling_at(C.byref(self), T, ne)
+
+    def emis(self, T, ne):
+        out = np.zeros(NL_MAXTR)
+        lib().orc_nlevel_emissivities(C.byref(self), T, ne, ptr(out))
+        return out[:self.nrad].copy()
+
+    def line_ident(self):
+        return np.array([lib().orc_nlevel_line_ident(C.byref(self), k) for k in range(self.nrad)])
+
+    def ups(self, k, T):
+        return lib().orc_ups_eval(C.byref(self), k, T)
+
+
+class NlCool(C.Structure):
+    """Mirror of orc_nlcool: the (T, n_e) table of one ion."""
+    _fields_ = [("ready", C.c_int), ("lamref", C.c_double * NLC_NT),
+                ("Ssup", C.c_double * (NLC_NNE * NLC_NT))]
+
+    @classmethod
+    def build(cls, atom):
+        t = cls()
+        lib().orc_nlcool_build(C.byref(atom), C.byref(t))
+        return t
+
+    def coeff(self, T, ne):
+        return lib().orc_nlcool_coeff(C.byref(self), T, ne)
+
+    def apply(self, lam_fit, T, ne):
+        return lib().orc_nlcool_apply(C.byref(self), lam_fit, T, ne)
+
+    def low_density(self, T):
+        return lib().orc_nlcool_low_density(C.byref(self), T)
 
 
 class Milne:

@@ -1,6 +1,6 @@
 ! C-callable entry points into the reference's own Fortran modules
 ! (qmc_mod, energy_sampler_mod, photo_xsec, recomb_mod, gaunt,
-! milne_recomb_spectrum_mod), compiled from
+! milne_recomb_spectrum_mod, nlevel_mod, nlevel_cooling_mod), compiled from
 ! their sources under the reference checkout by oracle/Makefile.  Test
 ! infrastructure only: the tests use it to check the C oracle against the
 ! reference code itself.
@@ -14,8 +14,13 @@ module ref_bind
   use recomb_mod
   use milne_recomb_spectrum_mod, only : milne_setup, milne_sample_energy, &
        milne_energy_per_recomb
+  use nlevel_mod, only : nlevel_atom_type, nlevel_load, nlevel_emissivities, &
+       nlevel_cooling_at, nlevel_nlines, nlevel_line_ident
+  use nlevel_cooling_mod, only : nlevel_cooling_add, nlevel_cooling_apply, &
+       nlevel_cooling_lam_se
   implicit none
   type(energy_sampler_type), save :: smp
+  type(nlevel_atom_type), save :: nla
 
   interface
      subroutine getGauntFF(z, log10Te, xlf, g, iflag)
@@ -160,5 +165,63 @@ contains
     real(c_double), value :: T
     ref_milne_eper = milne_energy_per_recomb(int(ich), T)
   end function ref_milne_eper
+
+  ! ---- n-level atoms and the local-n_e cooling tables ----------------------
+  function c_to_f(s, n) result(f)
+    character(kind=c_char), intent(in) :: s(*)
+    integer, intent(in) :: n
+    character(len=n) :: f
+    integer :: i
+    do i = 1, n
+       f(i:i) = s(i)
+    end do
+  end function c_to_f
+
+  integer(c_int) function ref_nlevel_load(path, n) bind(C, name='ref_nlevel_load')
+    character(kind=c_char), intent(in) :: path(*)
+    integer(c_int), value :: n
+    logical :: ok
+    call nlevel_load(c_to_f(path, int(n)), nla, ok)
+    ref_nlevel_load = merge(nlevel_nlines(nla), -1, ok)
+  end function ref_nlevel_load
+
+  real(c_double) function ref_nlevel_cooling_at(T, ne) bind(C, name='ref_nlevel_cooling_at')
+    real(c_double), value :: T, ne
+    ref_nlevel_cooling_at = nlevel_cooling_at(nla, T, ne)
+  end function ref_nlevel_cooling_at
+
+  subroutine ref_nlevel_emissivities(T, ne, emis) bind(C, name='ref_nlevel_emissivities')
+    real(c_double), value :: T, ne
+    real(c_double), intent(out) :: emis(600)
+    call nlevel_emissivities(nla, T, ne, emis)
+  end subroutine ref_nlevel_emissivities
+
+  real(c_double) function ref_nlevel_line_ident(k) bind(C, name='ref_nlevel_line_ident')
+    integer(c_int), value :: k
+    ref_nlevel_line_ident = nlevel_line_ident(nla, int(k))
+  end function ref_nlevel_line_ident
+
+  ! nlevel_cooling_add(name, stage) reading par%atomic_dir
+  integer(c_int) function ref_nlevel_cooling_add(dir, nd, name, nn, stage) &
+       bind(C, name='ref_nlevel_cooling_add')
+    character(kind=c_char), intent(in) :: dir(*), name(*)
+    integer(c_int), value :: nd, nn, stage
+    par%atomic_dir = c_to_f(dir, int(nd))
+    ref_nlevel_cooling_add = nlevel_cooling_add(c_to_f(name, int(nn)), int(stage))
+  end function ref_nlevel_cooling_add
+
+  real(c_double) function ref_nlevel_cooling_apply(idx, lam_fit, T, ne) &
+       bind(C, name='ref_nlevel_cooling_apply')
+    integer(c_int), value :: idx
+    real(c_double), value :: lam_fit, T, ne
+    ref_nlevel_cooling_apply = nlevel_cooling_apply(int(idx), lam_fit, T, ne)
+  end function ref_nlevel_cooling_apply
+
+  real(c_double) function ref_nlevel_cooling_lam_se(idx, T, ne) &
+       bind(C, name='ref_nlevel_cooling_lam_se')
+    integer(c_int), value :: idx
+    real(c_double), value :: T, ne
+    ref_nlevel_cooling_lam_se = nlevel_cooling_lam_se(int(idx), T, ne)
+  end function ref_nlevel_cooling_lam_se
 
 end module ref_bind
