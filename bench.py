@@ -217,6 +217,8 @@ def make_model(params, nH, octree, precision, warm=None, mlp=None):
     m.gas_state_setup(nH)
     if params.get("solve_te"):
         m.cooling_setup(HI_COOLING_FIT)
+        if params.get("cooling_model") == mc.lib.MC_COOL_LOCAL_NE:
+            m.nlevel_setup()
         if mlp is not None:
             m.mlp_set(mlp)
         if warm is not None:

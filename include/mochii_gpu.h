@@ -77,8 +77,9 @@ extern "C" {
 
 /* par%cooling_model (src/define.f90:781-789) */
 #define MC_COOL_LOW_DENSITY 0   /* line-cooling fits unsuppressed             */
-#define MC_COOL_LOCAL_NE    1   /* fits x the n-level (T, n_e) suppression:
-                                   needs the n-level tables, not built here  */
+#define MC_COOL_LOCAL_NE    1   /* fits x the n-level (T, n_e) suppression
+                                   (the reference default): needs the atoms
+                                   of mc_nlevel_setup                        */
 
 /* dust thermal-infrared band: per-leaf fields of mc_dust_get / mc_dust_set */
 #define MC_DUST_HEAT    0   /* Heat_dust [erg/s/cm^3] (gas_rates_mod heat_dust)     */
@@ -195,6 +196,29 @@ typedef struct mc_element {
     double  cool_A[MC_MAX_ST][MC_MAX_COOLFIT], cool_T[MC_MAX_ST][MC_MAX_COOLFIT];
 } mc_element;
 
+/* One n-level atom, the data nlevel_load reads from
+ * data/atomic/nlevel_<name>_<stage>.txt (src/nlevel_mod.f90:61-108): levels
+ * (statistical weight g, energy [cm^-1]), radiative rows (lower / upper
+ * level, 1-based, A [s^-1]) and Burgess-Tully Chebyshev fits of the
+ * collision strengths (levels, type, C, dE [Ry], st_lo, st_hi, logf, nc
+ * coefficients). */
+#define MC_NL_MAXLEV 40
+#define MC_NL_MAXTR  600
+typedef struct mc_nlevel_atom {
+    char    name[4];           /* element: "h", "c", "o", ... (lower case)     */
+    int32_t stage;             /* 1 = neutral                                  */
+    int32_t nlev;
+    double  g[MC_NL_MAXLEV], e_cm1[MC_NL_MAXLEV];
+    int32_t nrad;
+    int32_t rl[MC_NL_MAXTR], ru[MC_NL_MAXTR];
+    double  A[MC_NL_MAXTR];
+    int32_t nups;
+    int32_t ul_l[MC_NL_MAXTR], ul_u[MC_NL_MAXTR], ttype[MC_NL_MAXTR];
+    int32_t logf[MC_NL_MAXTR], nc[MC_NL_MAXTR];
+    double  cups[MC_NL_MAXTR], dery[MC_NL_MAXTR], st_lo[MC_NL_MAXTR], st_hi[MC_NL_MAXTR];
+    double  coef[MC_NL_MAXTR][16];
+} mc_nlevel_atom;
+
 typedef struct mc_ctx mc_ctx;
 
 /* Last error message of the calling thread ("" if none). */
@@ -253,8 +277,20 @@ int mc_gas_equilibrium_update(mc_ctx *ctx, double *max_dx, double *dx_vol);
 /* cooling_setup (src/cooling_mod.f90:67-92): the H I line-cooling fit
  * Lambda(T) = T^-1/2 sum_i A_i exp(-T_i/T) (nfit <= 8 terms, the rows of
  * data/atomic/cooling_h_1.txt) and the free-free Gaunt table, which the
- * library builds itself.  cooling_model 'low_density' (fit unsuppressed). */
+ * library builds itself.  With cooling_model MC_COOL_LOCAL_NE the fit is
+ * suppressed by the H I atom's (T, n_e) table (mc_nlevel_setup). */
 int mc_cooling_setup(mc_ctx *ctx, int nfit, const double *A, const double *Ti);
+
+/* The n-level atoms the library may use (nlevel_load for every ion,
+ * src/nlevel_mod.f90:61-108).  With cooling_model MC_COOL_LOCAL_NE the
+ * thermal update suppresses the H I line-cooling fit and each metal stage's
+ * fit by the ion's (T, n_e) table, built here as nlevel_cooling_add builds
+ * it (src/nlevel_cooling_mod.f90:105-152: 401 x 161 statistical-equilibrium
+ * solves per ion, cooling_setup and species_setup call it for H I and for
+ * every stage with a cooling fit); an ion without an atom keeps its fit
+ * unsuppressed, as in the reference.  The atoms also define the lines of
+ * mc_lines_compute.  Replaces any earlier set. */
+int mc_nlevel_setup(mc_ctx *ctx, int n, const mc_nlevel_atom *atoms);
 
 /* gas_thermal_update (src/thermal_mod.f90:124-249): per-leaf T_e by
  * bisection on log T with the ionisation state solved at each trial T. */

@@ -156,6 +156,47 @@ def read_nlevel(path):
     return at
 
 
+MC_NL_MAXLEV, MC_NL_MAXTR = 40, 600
+
+
+class NlevelAtom(C.Structure):
+    """Mirror of mc_nlevel_atom (include/mochii_gpu.h)."""
+    _fields_ = [("name", C.c_char * 4), ("stage", C.c_int32), ("nlev", C.c_int32),
+                ("g", C.c_double * MC_NL_MAXLEV), ("e_cm1", C.c_double * MC_NL_MAXLEV),
+                ("nrad", C.c_int32), ("rl", C.c_int32 * MC_NL_MAXTR),
+                ("ru", C.c_int32 * MC_NL_MAXTR), ("A", C.c_double * MC_NL_MAXTR),
+                ("nups", C.c_int32), ("ul_l", C.c_int32 * MC_NL_MAXTR),
+                ("ul_u", C.c_int32 * MC_NL_MAXTR), ("ttype", C.c_int32 * MC_NL_MAXTR),
+                ("logf", C.c_int32 * MC_NL_MAXTR), ("nc", C.c_int32 * MC_NL_MAXTR),
+                ("cups", C.c_double * MC_NL_MAXTR), ("dery", C.c_double * MC_NL_MAXTR),
+                ("st_lo", C.c_double * MC_NL_MAXTR), ("st_hi", C.c_double * MC_NL_MAXTR),
+                ("coef", (C.c_double * 16) * MC_NL_MAXTR)]
+
+
+def nlevel_struct(atoms):
+    """ctypes array of NlevelAtom for mc_nlevel_setup from {'<el>_<stage>':
+    atom dict} (read_nlevel / load_nlevel)."""
+    keys = sorted(atoms)
+    arr = (NlevelAtom * max(len(keys), 1))()
+    for n, key in enumerate(keys):
+        name, stage = key.rsplit("_", 1)
+        at, a = atoms[key], arr[n]
+        a.name, a.stage, a.nlev = name.encode()[:3], int(stage), len(at["g"])
+        for i, (g, e) in enumerate(zip(at["g"], at["e_cm1"])):
+            a.g[i], a.e_cm1[i] = g, e
+        a.nrad = len(at["rad"])
+        for k, (l, u, A) in enumerate(at["rad"]):
+            a.rl[k], a.ru[k], a.A[k] = l, u, A
+        a.nups = len(at["ups"])
+        for k, r in enumerate(at["ups"]):
+            a.ul_l[k], a.ul_u[k], a.ttype[k], a.logf[k] = r["l"], r["u"], r["type"], r["logf"]
+            a.nc[k], a.cups[k], a.dery[k] = len(r["coef"]), r["C"], r["dE"]
+            a.st_lo[k], a.st_hi[k] = r["st_lo"], r["st_hi"]
+            for j, v in enumerate(r["coef"]):
+                a.coef[k][j] = v
+    return arr, len(keys)
+
+
 def nlevel_files(atomic_dir):
     """The n-level atoms of a data directory: {'<el>_<stage>': atom}."""
     out = {}
@@ -165,10 +206,16 @@ def nlevel_files(atomic_dir):
     return out
 
 
+_NLEVEL = {}
+
+
 def load_nlevel(path=SNAPSHOT):
-    """The n-level atoms of the data snapshot: {'<el>_<stage>': atom}."""
-    with open(path) as f:
-        return json.load(f).get("nlevel", {})
+    """The n-level atoms of the data snapshot: {'<el>_<stage>': atom}, read
+    once per path (the same dict on every call; treat it as read-only)."""
+    if path not in _NLEVEL:
+        with open(path) as f:
+            _NLEVEL[path] = json.load(f).get("nlevel", {})
+    return _NLEVEL[path]
 
 
 def read_registry(atomic_dir):

@@ -275,9 +275,8 @@ extern "C" int mc_init(const mc_params *p, mc_ctx **out)
         mc_set_error("diffuse_model must be 0 (threshold), 1 (exponential) or 2 (milne)");
         return -1;
     }
-    if (p->cooling_model != MC_COOL_LOW_DENSITY) {
-        mc_set_error("cooling_model: only MC_COOL_LOW_DENSITY is built (the local_ne "
-                     "suppression needs the n-level atom tables)");
+    if (p->cooling_model != MC_COOL_LOW_DENSITY && p->cooling_model != MC_COOL_LOCAL_NE) {
+        mc_set_error("cooling_model: MC_COOL_LOW_DENSITY or MC_COOL_LOCAL_NE");
         return -1;
     }
     if (p->ci_model != MC_CI_VORONOV) {
@@ -371,7 +370,7 @@ extern "C" int mc_finalize(mc_ctx *c)
                     (void *)c->d_node_gd, (void *)c->d_node_gf, (void *)c->d_node_link,
                     (void *)c->d_node_nbr, (void *)c->d_leaf_node, (void *)c->d_milne,
                     (void *)c->mlp.P, (void *)c->d_feat, (void *)c->d_pred, (void *)c->pk.key,
-                    (void *)c->d_key_sorted, (void *)c->d_perm, c->d_sort_tmp})
+                    (void *)c->d_key_sorted, (void *)c->d_perm, c->d_sort_tmp, (void *)c->d_nlt})
         if (p) (void)hipFree(p);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);

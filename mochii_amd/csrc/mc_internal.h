@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "../../include/mochii_gpu.h"
@@ -110,6 +111,7 @@ struct DevMetalTab {
     double  flat_eth[MC_GPU_MAXTR];
     double  el_ab[MC_MAX_EL];        // abundance of each element
     int32_t s_ncool[MC_GPU_MAXST];
+    int32_t s_nl[MC_GPU_MAXST];      // (T, n_e) table slot of the stage, -1: none
     int32_t s_charge[MC_GPU_MAXST];  // stage j of its element has charge j - 1
     double  s_A[MC_GPU_MAXST][MC_MAX_COOLFIT], s_T[MC_GPU_MAXST][MC_MAX_COOLFIT];
 };
@@ -181,6 +183,10 @@ struct DevCool {
     double logT0, dlogT;
     int32_t nfit;
     double A[8], Ti[8];
+    // cooling_model 'local_ne': (T, n_e) tables (mc_nlevel.h slots) and the
+    // slot of H I (-1: the fit unsuppressed)
+    const double *nlt;
+    int32_t nl_hi;
 };
 
 // Packet launch records (SoA), written by the launch kernel.
@@ -343,6 +349,13 @@ struct mc_ctx {
     int32_t *d_mom_band;
     size_t mom_cap;
     int mom_warned;                 // fallback to per-coefficient sums reported
+    // n-level atoms (mc_nlevel_setup) and the (T, n_e) cooling tables built
+    // from them: table per atom index, the device slots in use
+    std::vector<mc_nlevel_atom> nl_atoms;
+    int nl_given, nl_dirty;
+    std::map<int, std::vector<double>> nl_built;
+    std::vector<int> nl_slots;
+    double *d_nlt;
     // last pass results
     double ledger[3];
     int64_t segs[3];          // direct, scattered, metal transitions over scored segments
@@ -375,6 +388,7 @@ int mc_sort_records(mc_ctx *c, int64_t n);
 int mc_run_trace(mc_ctx *c, int precision, const double *pos, const double *dir, int32_t *cells,
                  int maxn, int32_t *n);
 int mc_run_dust_temp(mc_ctx *c, const double *d_heat);
+int mc_nl_ready(mc_ctx *c);
 int mc_run_qmc_dump(mc_ctx *c, int stream, int64_t first, int n, int ndim, double *out);
 int mc_run_launch_dump(mc_ctx *c, int64_t first_id, int n, double *rec, int32_t *bin_leaf);
 int mc_run_dust_emission(mc_ctx *c, double report[MC_DUSTRT_NREPORT]);

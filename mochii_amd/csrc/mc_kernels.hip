@@ -2042,7 +2042,11 @@ __device__ double cooling_total(const DevCool &cl, const RateSet &r, double T, d
             MC_EV2ERG;
     double lam = 0.0;
     for (int i = 0; i < cl.nfit; ++i) lam += cl.A[i] * exp(-cl.Ti[i] / T);
-    return cool + ne * nHI * (lam / sqrt(T));
+    lam = lam / sqrt(T);
+    // cooling_model 'local_ne': the fit suppressed by the H I (T, n_e) table
+    // (nlevel_cooling_apply, src/cooling_mod.f90:271-275)
+    if (cl.nl_hi >= 0) lam = nl_apply(cl.nlt + (long)cl.nl_hi * kNlSlot, nl_pos(T, ne), lam);
+    return cool + ne * nHI * lam;
 }
 
 struct CellIn { double nH, gH, gHe1, gHe2, hH, hHe1, hHe2; };
@@ -2878,6 +2882,7 @@ MetalSw make_sw(mc_ctx *c, bool thermal = false)
     m.metal_heat = c->p.metal_heat != 0;
     m.metal_ff = c->p.metal_freefree != 0;
     m.noise_cap = thermal && c->warm_mode != MC_WARM_NONE ? 2 : 0;
+    m.nlt = thermal ? c->cool.nlt : nullptr;
     return m;
 }
 
@@ -3131,6 +3136,7 @@ int mc_run_thermal(mc_ctx *c, double out[4])
         if (rc) return rc;
     }
     const bool caseA = c->p.case_A != 0;
+    if (mc_nl_ready(c)) return -1;        // local_ne tables, slots into cool / mtab
     if (metal_sync(c)) return -100;
 #define MC_THERMAL_ARGS c->grid, c->st, c->cool, caseA, c->p.He_abund, c->p.ion_relax, \
                         c->p.te_min, c->p.te_max, c->p.distance2cm, c->d_pred, make_sw(c, true), \

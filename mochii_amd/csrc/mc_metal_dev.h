@@ -7,6 +7,7 @@
 // registry size), so the cache arrays are indexed by compile-time constants
 // and the per-transition registry fields are scalar loads.
 #pragma once
+#include "mc_nlevel.h"
 
 // photoionisation cross sections m_vfky96 / m_vy95 / m_sigma: mc_internal.h
 
@@ -314,6 +315,9 @@ struct MetalSw {
     // iteration after this many consecutive passes whose hydrogen is below
     // resolution (x_HII < 1e-10); see solve_cell_m
     int32_t noise_cap;
+    // cooling_model 'local_ne' tables of the thermal update (null: the
+    // stage fits unsuppressed)
+    const double *nlt;
 };
 
 // solve_ion_cell with the metal-electron and two-way charge-exchange terms
@@ -412,6 +416,15 @@ __device__ __noinline__ double m_cool_h(int cII, double nion, double nHI, double
     return nion * nHI * qlu * (227.7 * 1.380649e-16) / (1.0 + nHI * (qul + qlu) / 8.865e-5);
 }
 
+// a stage's line-cooling coefficient: its fit, suppressed by the stage's
+// (T, n_e) table under cooling_model 'local_ne' (src/species_mod.f90:836-840)
+__device__ __forceinline__ double m_cool_coef(const MetalSw &ms, const NlPos &np, int st, double T)
+{
+    const double f = m_fit(st, T);
+    const int sl = c_mt.s_nl[st];
+    return (ms.nlt && sl >= 0) ? nl_apply(ms.nlt + (long)sl * kNlSlot, np, f) : f;
+}
+
 template <int MT>
 __device__ __forceinline__ void m_thermal_terms(const MetalSw &ms, const double (&lo)[MT],
                                                 const double (&up)[MT], const double *heat,
@@ -423,16 +436,18 @@ __device__ __forceinline__ void m_thermal_terms(const MetalSw &ms, const double 
     const double sqT = sqrt(T);
     const int ntr = mt->ntr;
     double c1 = 0.0, c2 = 0.0, sff = 0.0, h = 0.0;
+    NlPos np{};
+    if (ms.nlt) np = nl_pos(T, ne);
 #pragma unroll
     for (int u = 0; u < MT; ++u) {
         if (u < ntr) {
             const double nel = mt->t_ab[u] * nH;
             const int sl = mt->t_st[u], su = sl + 1;
             if (mt->t_first[u] == u) {        // the element's neutral stage
-                if (mt->s_ncool[sl] > 0) c1 = c1 + ne * nel * lo[u] * m_fit(sl, T);
+                if (mt->s_ncool[sl] > 0) c1 = c1 + ne * nel * lo[u] * m_cool_coef(ms, np, sl, T);
                 if (mt->t_oI[u]) c2 = c2 + m_cool_h(0, nel * lo[u], nHI, T);
             }
-            if (mt->s_ncool[su] > 0) c1 = c1 + ne * nel * up[u] * m_fit(su, T);
+            if (mt->s_ncool[su] > 0) c1 = c1 + ne * nel * up[u] * m_cool_coef(ms, np, su, T);
             if (mt->t_cII[u]) c2 = c2 + m_cool_h(1, nel * up[u], nHI, T);
             const int z = mt->s_charge[su];
             const double gz = z <= 1 ? gff[0] : (z == 2 ? gff[1] : (z == 3 ? gff[2] : gff[3]));

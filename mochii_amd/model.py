@@ -209,6 +209,15 @@ class Model:
         Ti = np.ascontiguousarray([f[1] for f in fit], dtype=np.float64)
         check(self._L.mc_cooling_setup(self._h, A.size, _dp(A), _dp(Ti)), "mc_cooling_setup")
 
+    def nlevel_setup(self, atoms=None):
+        """The n-level atoms ({'<el>_<stage>': atom}, default: every atom of
+        the data snapshot): the (T, n_e) cooling tables of cooling_model
+        'local_ne' and the lines of lines_compute."""
+        from .atomic import load_nlevel, nlevel_struct
+        self.atoms = dict(load_nlevel() if atoms is None else atoms)
+        self._nla, n = nlevel_struct(self.atoms)
+        check(self._L.mc_nlevel_setup(self._h, n, self._nla), "mc_nlevel_setup")
+
     def gas_thermal_update(self):
         a, b, c, d = C.c_double(), C.c_double(), C.c_double(), C.c_double()
         check(self._L.mc_gas_thermal_update(self._h, C.byref(a), C.byref(b), C.byref(c),

@@ -3,7 +3,7 @@ seeded inputs).  Units as in the reference: code length = parsec
 (par%distance_unit = 'pc'), densities in cm^-3, luminosities in erg/s."""
 import numpy as np
 
-from .lib import MC_FP64, MC_GRID_CAR
+from .lib import MC_COOL_LOCAL_NE, MC_FP64, MC_GRID_CAR
 
 PC2CM = 3.0856776e18
 
@@ -50,12 +50,14 @@ def elements_of(params):
 
 def hii_thermal(n=32, nphotons=200_000, precision=MC_FP64, seed=12345, niter=100,
                 tol=2e-3, tol_te=2e-3, half=8.0, octree=None, nH=None, diffuse=False,
-                metals=False):
+                metals=False, cooling_model=MC_COOL_LOCAL_NE):
     """H + He (He/H = 0.1) H II region with thermal balance: one 45000 K
     blackbody at the box centre with Q(H) = 3e49 s^-1, case A, T_e solved in
     [1e3, 5e4] K, volume convergence criterion; with diffuse=True explicit
     diffuse ground-recombination packets (Milne spectra).  Uniform car grid
-    unless a leaf list (octree) with per-leaf nH is given."""
+    unless a leaf list (octree) with per-leaf nH is given.  Line cooling:
+    the reference default 'local_ne' (the fits suppressed by the n-level
+    (T, n_e) tables; the model needs nlevel_setup) or 'low_density'."""
     lum = 3.0 * L_Q49_T40K * _mean_energy_ratio(4.5e4, 4.0e4)
     params = dict(nphotons=nphotons, qmc_seed=seed, precision=precision, eion_min=13.598,
                   eion_max=100.0, nnu_ion=16, ion_align_edges=1, nsource=1,
@@ -65,7 +67,7 @@ def hii_thermal(n=32, nphotons=200_000, precision=MC_FP64, seed=12345, niter=100
                   te_fixed=1.0e4, ion_relax=1.0, te_min=1.0e3, te_max=5.0e4, gas_tol=tol,
                   gas_tol_te=tol_te, conv_vol=1, gas_niter=niter, case_A=1, solve_te=1,
                   xHI_init=1.0, xHeI_init=1.0, xHeII_init=0.0, diffuse_field=int(diffuse),
-                  diffuse_model=2)
+                  diffuse_model=2, cooling_model=int(cooling_model))
     if metals:
         from .atomic import metal_thresholds, registry
         params["metal_eth"] = metal_thresholds(registry(METAL_ABUND))
@@ -179,7 +181,7 @@ def octree_from_field(field, half, lmax=None, thresh_dex=0.3, lmin=1):
 
 def hii_octree(n_field=128, nphotons=10_000_000, precision=MC_FP64, seed=12345, field_seed=1,
                half=8.0, lmax=7, thresh_dex=0.3, lmin=1, niter=100, tol=2e-3, tol_te=2e-3,
-               diffuse=True, metals=True):
+               diffuse=True, metals=True, cooling_model=MC_COOL_LOCAL_NE):
     """H + He H II region with thermal balance on an adaptive octree: n_H
     lognormal (mean 100 cm^-3, sigma_ln 0.5) from a seeded Gaussian random
     field with P(k) ~ k^-11/3 on n_field^3, leaves refined to at most lmax
@@ -193,5 +195,6 @@ def hii_octree(n_field=128, nphotons=10_000_000, precision=MC_FP64, seed=12345, 
                                              lmin=lmin)
     params, _ = hii_thermal(n=n_field, nphotons=nphotons, precision=precision, seed=seed,
                             niter=niter, tol=tol, tol_te=tol_te, half=half, octree=True,
-                            nH=nHl, diffuse=diffuse, metals=metals)
+                            nH=nHl, diffuse=diffuse, metals=metals,
+                            cooling_model=cooling_model)
     return params, nHl, (xl, yl, zl, lev, half)

@@ -81,10 +81,13 @@ double orc_ci_HeI(double T);
 double orc_ci_HeII(double T);
 
 /* ---- cooling (src/cooling_mod.f90, src/gaunt.f90) ----------------------- */
+struct orc_nlcool_s;
 typedef struct {
     int nfit;                 /* H I line-cooling fit terms */
     double A[16], Ti[16];
     double logT[41], gff[4 * 41];
+    /* cooling_model 'local_ne': the H I (T, n_e) table, NULL = fit unsuppressed */
+    const struct orc_nlcool_s *hi_tab;
 } orc_cooling;
 
 void   orc_gaunt_ff(float z, float log10Te, const float *xlf, float *g, int n);
@@ -267,6 +270,9 @@ typedef struct {
     orc_dustir *ir;
     /* worker threads of the per-leaf thermal update (1 = serial) */
     int     solve_threads;
+    /* cooling_model 'local_ne': (T, n_e) table of each registry stage,
+     * NULL = the stage's fit unsuppressed */
+    const struct orc_nlcool_s *st_tab[ORC_MAX_EL][ORC_MAX_ST];
 } orc_ctx;
 
 /* ---- n-level atoms (src/nlevel_mod.f90) ---------------------------------- */
@@ -293,7 +299,7 @@ typedef struct {
 #define ORC_NLC_DT  0.01
 #define ORC_NLC_NELO (-6.0)
 #define ORC_NLC_DNE 0.1
-typedef struct {
+typedef struct orc_nlcool_s {
     int    ready;
     double lamref[ORC_NLC_NT];
     double Ssup[ORC_NLC_NNE * ORC_NLC_NT];
@@ -321,6 +327,9 @@ void   orc_nlcool_build(const orc_nlatom *a, orc_nlcool *t);
 double orc_nlcool_coeff(const orc_nlcool *t, double T, double ne);
 double orc_nlcool_low_density(const orc_nlcool *t, double T);
 double orc_nlcool_apply(const orc_nlcool *t, double lam_fit, double T, double ne);
+/* the stage table of registry element ie (0-based), stage 1-based */
+void   orc_set_stage_table(orc_ctx *c, int ie, int stage, const orc_nlcool *t);
+void   orc_set_hi_table(orc_ctx *c, const orc_nlcool *t);
 double orc_sh95_emis(const orc_sh95 *t, int k, double T, double ne);
 
 orc_ctx *orc_create(const orc_params *p);

@@ -22,9 +22,9 @@
  *   hydrogen_neutral_fraction, charge_neutrality_ne    ion_balance_mod :53-254
  * ci_model 'voronov' on the model path (ci_dere_ratio is restated and pinned,
  * but the model keeps ci_dere = 0, as the H/He rates do).
- * cooling_model 'low_density' only: the line-cooling fits enter unsuppressed
- * (nlevel_cooling_apply with idx = 0, src/nlevel_cooling_mod.f90:168-183);
- * the (T, n_e) suppression tables need the n-level atom solve.
+ * cooling_model 'local_ne': each stage's fit is suppressed by its (T, n_e)
+ * table when one is set (orc_set_stage_table; nlevel_cooling_apply,
+ * src/species_mod.f90:836-840), else it enters unsuppressed ('low_density').
  *
  * Pinned by (tests/test_oracle_metals.py): sigma_vfky96 / sigma_vy95,
  * rr_badnell, voronov and ci_dere_ratio against the reference's
@@ -395,7 +395,7 @@ double orc_metal_cooling(const orc_ctx *c, int il, double T, double nH, double n
             double lam = 0.0;
             for (int k = 0; k < el->ncool[i]; ++k) lam = lam + el->cool_A[i][k] * exp(-el->cool_T[i][k] / T);
             lam = lam / sqrt(T);
-            cool = cool + ne * nel * frac[i] * lam;
+            cool = cool + ne * nel * frac[i] * orc_nlcool_apply(c->st_tab[ie][i], lam, T, ne);
         }
     }
     return cool;

@@ -308,3 +308,10 @@ double orc_sh95_emis(const orc_sh95 *t, int k, double T, double ne)
                log(E(iT, iD + 1)) * (1.0 - wT) * wD + log(E(iT + 1, iD + 1)) * wT * wD);
 #undef E
 }
+
+void orc_set_stage_table(orc_ctx *c, int ie, int stage, const orc_nlcool *t)
+{
+    if (ie >= 0 && ie < ORC_MAX_EL && stage >= 1 && stage <= ORC_MAX_ST) c->st_tab[ie][stage - 1] = t;
+}
+
+void orc_set_hi_table(orc_ctx *c, const orc_nlcool *t) { c->cool.hi_tab = t; }

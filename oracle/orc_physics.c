@@ -296,6 +296,8 @@ double orc_cooling_total(const orc_cooling *cl, double T, double nH, double ne, 
     double lam = 0.0;
     for (int i = 0; i < cl->nfit; ++i) lam = lam + cl->A[i] * exp(-cl->Ti[i] / T);
     lam = lam / sqrt(T);
-    cool = cool + ne * nHI * lam;
+    /* cooling_model 'local_ne': suppressed by the H I table
+     * (src/cooling_mod.f90:271-275) */
+    cool = cool + ne * nHI * orc_nlcool_apply(cl->hi_tab, lam, T, ne);
     return cool;
 }

@@ -53,7 +53,8 @@ class OrcParams(C.Structure):
 class Cooling(C.Structure):
     """Mirror of orc_cooling (H I line fit + free-free Gaunt table)."""
     _fields_ = [("nfit", C.c_int), ("A", C.c_double * 16), ("Ti", C.c_double * 16),
-                ("logT", C.c_double * 41), ("gff", C.c_double * (4 * 41))]
+                ("logT", C.c_double * 41), ("gff", C.c_double * (4 * 41)),
+                ("hi_tab", C.c_void_p)]
 
     @classmethod
     def make(cls, fit=(), gaunt=True):
@@ -143,7 +144,9 @@ def _load():
             ("orc_nlcool_coeff", [C.c_void_p, C.c_double, C.c_double], C.c_double),
             ("orc_nlcool_low_density", [C.c_void_p, C.c_double], C.c_double),
             ("orc_nlcool_apply", [C.c_void_p, C.c_double, C.c_double, C.c_double], C.c_double),
-            ("orc_sh95_emis", [C.c_void_p, C.c_int, C.c_double, C.c_double], C.c_double)):
+            ("orc_sh95_emis", [C.c_void_p, C.c_int, C.c_double, C.c_double], C.c_double),
+            ("orc_set_stage_table", [C.c_void_p, C.c_int, C.c_int, C.c_void_p], None),
+            ("orc_set_hi_table", [C.c_void_p, C.c_void_p], None)):
         f = getattr(lib, name)
         f.argtypes, f.restype = args, res
     lib.orc_set_solve_threads.restype = None
@@ -490,6 +493,30 @@ class Model:
         Ti = np.ascontiguousarray(Ti, dtype=np.float64)
         lib().orc_cooling_setup(self.c, A.size, ptr(A), ptr(Ti))
 
+    def nlevel_setup(self, atoms):
+        """cooling_model 'local_ne': the (T, n_e) tables of H I and of every
+        registry stage with a cooling fit whose atom is in ``atoms``
+        ({'<el>_<stage>': atom dict}); tables are built once per atom."""
+        L = lib()
+        self._nlc = []
+
+        def table(key):
+            if key not in atoms:
+                return None
+            t = _NLC_CACHE.get((key, id(atoms[key])))
+            if t is None:
+                t = NlCool.build(NlAtom.make(atoms[key]))
+                _NLC_CACHE[(key, id(atoms[key]))] = t
+            self._nlc.append(t)
+            return t
+
+        t = table("h_1")
+        L.orc_set_hi_table(self.c, C.byref(t) if t is not None else None)
+        for ie, el in enumerate(self.elements):
+            for st in range(1, el["nstage"] + 1):
+                t = table(f"{el['name']}_{st}") if st in el.get("cooling", {}) else None
+                L.orc_set_stage_table(self.c, ie, st, C.byref(t) if t is not None else None)
+
     def thermal_update(self, nthreads=None):
         """gas_thermal_update; the per-leaf solves on nthreads threads (default
         all host cores, capped at 16); the result does not depend on it."""
@@ -835,6 +862,9 @@ class NlAtom(C.Structure):
 
     def ups(self, k, T):
         return lib().orc_ups_eval(C.byref(self), k, T)
+
+
+_NLC_CACHE = {}        # (atom key, atom dict id) -> table: built once per process
 
 
 class NlCool(C.Structure):

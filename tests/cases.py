@@ -45,6 +45,9 @@ def oracle_model(params, nH, rhokap=None, dust_table=None, octree=None, elements
                  dust_table=dust_table, elements=els)
     m.params.update(op)
     m.state_init(nH, rhokap=rhokap)
+    if params.get("cooling_model") == 1:          # 'local_ne': the (T, n_e) tables
+        from mochii_amd.atomic import load_nlevel
+        m.nlevel_setup(load_nlevel())
     return m
 
 
@@ -62,6 +65,8 @@ def gpu_model(params, nH, rhokap=None, dust_table=None, octree=None, elements=No
     m.gas_state_setup(nH)
     if rhokap is not None:
         m.set_state(rhokap=rhokap)
+    if params.get("cooling_model") == 1:          # 'local_ne': the n-level atoms
+        m.nlevel_setup()
     return m
 
 

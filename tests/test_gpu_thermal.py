@@ -90,3 +90,42 @@ def test_thermal_update_requires_cooling_setup():
     from mochii_amd.lib import MochiiError
     with pytest.raises(MochiiError, match="cooling_setup"):
         gm.gas_thermal_update()
+
+
+def test_local_ne_cooling_matches_oracle_and_suppresses_line_cooling():
+    """cooling_model 'local_ne' (the reference default): the H I and metal
+    line-cooling fits suppressed by the n-level (T, n_e) tables.  At
+    n_H = 1500 cm^-3 the metal lines are partly collisionally de-excited:
+    the thermal update matches the oracle with the tables, and runs hotter
+    than with the unsuppressed 'low_density' fits, on both sides."""
+    from mochii_amd.lib import MC_COOL_LOW_DENSITY
+    n = 10
+    res = {}
+    for model in (1, MC_COOL_LOW_DENSITY):
+        params, nH = hii_thermal(n=n, nphotons=20000, metals=True, cooling_model=model,
+                                 nH=np.full(n ** 3, 1500.0), half=2.0)
+        orc, gm = oracle_model(params, nH), gpu_model(params, nH)
+        orc.cooling_setup([a for a, _ in HI_COOLING_FIT], [t for _, t in HI_COOLING_FIT])
+        gm.cooling_setup(HI_COOLING_FIT)
+        c = (np.arange(n) + 0.5) / n * 2 - 1
+        X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+        xHI = np.where(np.sqrt(X**2 + Y**2 + Z**2).ravel() < 0.35, 1e-3, 1.0)
+        orc.set_state(xHI=xHI)
+        orc.opacity_fill()
+        gm.set_state(xHI=xHI)
+        orc.transport_pass(nthreads=8)
+        orc.rates_compute()
+        gm.ionizing_field_and_rates()
+        orc.thermal_update()
+        gm.gas_thermal_update()
+        so, sg = orc.state(), gm.state()
+        rel = np.abs(sg["Te"] / so["Te"] - 1.0)
+        print(model, "T_e max rel", rel.max(), "bit-equal", np.mean(rel == 0.0))
+        assert np.mean(rel == 0.0) >= 0.98 and rel.max() < 1e-3
+        res[model] = (so["Te"], sg["Te"], so["xHI"])
+    lit = res[1][2] < 0.5
+    assert lit.sum() >= 5
+    for side in (0, 1):
+        dT = res[1][side][lit] / res[MC_COOL_LOW_DENSITY][side][lit] - 1.0
+        print("local_ne / low_density - 1 on lit leaves:", np.median(dT), dT.max())
+        assert np.median(dT) > 1e-3 and np.all(dT > -1e-4)
