@@ -292,6 +292,38 @@ int mc_cooling_setup(mc_ctx *ctx, int nfit, const double *A, const double *Ti);
  * mc_lines_compute.  Replaces any earlier set. */
 int mc_nlevel_setup(mc_ctx *ctx, int n, const mc_nlevel_atom *atoms);
 
+/* One recombination-line table as sh95_load reads it (src/sh95_mod.f90:93-136):
+ * nT temperatures and nD electron densities (ascending), nlin lines with
+ * vacuum wavelengths [A] and 4 pi j / (n_e n_ion) [erg cm^3 s^-1] on the
+ * grid, etab[line][iT][iD]. */
+#define MC_LT_MAXG 32
+#define MC_LT_MAXL 12
+typedef struct mc_line_table {
+    int32_t nT, nD, nlin;
+    double  tgrid[MC_LT_MAXG], dgrid[MC_LT_MAXG];
+    double  wl[MC_LT_MAXL];
+    double  etab[MC_LT_MAXL][MC_LT_MAXG][MC_LT_MAXG];
+} mc_line_table;
+
+/* sh95_setup (src/sh95_mod.f90:66-90): the H I and He II recombination-line
+ * tables of the case in use and the He I (Porter, case B) table; hi is
+ * required by mc_lines_compute, heii and hei may be NULL (no such lines). */
+int mc_lines_setup(mc_ctx *ctx, const mc_line_table *hi, const mc_line_table *heii,
+                   const mc_line_table *hei);
+
+/* lines_write (src/lines_mod.f90:37-352) on the current state: the line
+ * luminosities L = sum over leaves of j n_ion V [erg/s], in the reference's
+ * table order: the H I recombination lines (n_e n_p, ion code 101), the He II
+ * lines (n_e n_HeIII, 202), the He I lines (n_e n_HeII, 201), then for every
+ * registry element and stage with an n-level atom (mc_nlevel_setup) its
+ * radiative rows (n-level solve per leaf at (T_e, n_e), n_ion from the stage
+ * fractions at the leaf's state and last metal rates; code 100 Z + stage).
+ * Writes at most maxn entries of L, the wavelengths wl [A] and the ion codes,
+ * and the number of lines to *n (any output pointer may be NULL).  The
+ * collisional H I Balmer rows (par%h_coll_effects, off by default) are not
+ * computed. */
+int mc_lines_compute(mc_ctx *ctx, int maxn, double *L, double *wl, int32_t *ion, int32_t *n);
+
 /* gas_thermal_update (src/thermal_mod.f90:124-249): per-leaf T_e by
  * bisection on log T with the ionisation state solved at each trial T. */
 int mc_gas_thermal_update(mc_ctx *ctx, double *max_dx, double *max_dte, double *dx_vol,

@@ -206,6 +206,74 @@ def nlevel_files(atomic_dir):
     return out
 
 
+def read_line_table(path):
+    """sh95_load (src/sh95_mod.f90:93-136): a recombination-line table
+    (GRID nT nD, T row, NE row, then LINE label nu nl lambda[A] blocks of
+    nT rows of nD emissivities 4 pi j / (n_e n_ion) [erg cm^3 s^-1])."""
+    with open(path) as f:
+        rows = [ln.split() for ln in f if ln.strip() and not ln.lstrip().startswith("#")]
+    t = {"T": [], "NE": [], "lines": []}
+    i = 0
+    while i < len(rows):
+        r = rows[i]
+        if r[0] == "GRID":
+            nT, nD = int(r[1]), int(r[2])
+            i += 1
+        elif r[0] == "T":
+            t["T"] = [float(v) for v in r[1:]]
+            i += 1
+        elif r[0] == "NE":
+            t["NE"] = [float(v) for v in r[1:]]
+            i += 1
+        elif r[0] == "LINE":
+            e = [[float(v) for v in rows[i + 1 + a]] for a in range(nT)]
+            t["lines"].append({"label": r[1], "nu": int(r[2]), "nl": int(r[3]),
+                               "wl": float(r[4]), "e": e})
+            i += 1 + nT
+        else:
+            i += 1
+    return t
+
+
+LINE_TABLES = ("sh95_hi_caseA", "sh95_hi_caseB", "sh95_heii_caseA", "sh95_heii_caseB",
+               "hei_porter_caseB")
+
+
+def line_table_files(atomic_dir):
+    return {k: read_line_table(os.path.join(atomic_dir, k + ".txt")) for k in LINE_TABLES
+            if os.path.exists(os.path.join(atomic_dir, k + ".txt"))}
+
+
+_LINES = {}
+
+
+def load_line_tables(path=SNAPSHOT):
+    """The recombination-line tables of the data snapshot (read once)."""
+    if path not in _LINES:
+        with open(path) as f:
+            _LINES[path] = json.load(f).get("line_tables", {})
+    return _LINES[path]
+
+
+class LineTable(C.Structure):
+    """Mirror of mc_line_table (include/mochii_gpu.h)."""
+    _fields_ = [("nT", C.c_int32), ("nD", C.c_int32), ("nlin", C.c_int32),
+                ("tgrid", C.c_double * 32), ("dgrid", C.c_double * 32), ("wl", C.c_double * 12),
+                ("etab", ((C.c_double * 32) * 32) * 12)]
+
+    @classmethod
+    def make(cls, tab):
+        t = cls()
+        t.nT, t.nD, t.nlin = len(tab["T"]), len(tab["NE"]), len(tab["lines"])
+        t.tgrid[:t.nT] = tab["T"]
+        t.dgrid[:t.nD] = tab["NE"]
+        for k, ln in enumerate(tab["lines"]):
+            t.wl[k] = ln["wl"]
+            for a, row in enumerate(ln["e"]):
+                t.etab[k][a][:len(row)] = row
+        return t
+
+
 _NLEVEL = {}
 
 

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/mochii_gpu.h"
+#include "mc_nlevel.h"
 
 #define MC_MAXBINS 64
 #define MC_QMC_DIM 12
@@ -356,6 +357,9 @@ struct mc_ctx {
     std::map<int, std::vector<double>> nl_built;
     std::vector<int> nl_slots;
     double *d_nlt;
+    // recombination-line tables (mc_lines_setup): H I, He II, He I
+    mc_line_table lt[3];
+    int lt_on[3];
     // last pass results
     double ledger[3];
     int64_t segs[3];          // direct, scattered, metal transitions over scored segments
@@ -389,6 +393,11 @@ int mc_run_trace(mc_ctx *c, int precision, const double *pos, const double *dir,
                  int maxn, int32_t *n);
 int mc_run_dust_temp(mc_ctx *c, const double *d_heat);
 int mc_nl_ready(mc_ctx *c);
+int find_atom(const mc_ctx *c, const char *name, int stage);
+int mc_lines_ions(mc_ctx *c, int out0, std::vector<DevLineIon> &ions, std::vector<double> &wl,
+                  std::vector<int32_t> &code);
+int mc_run_lines(mc_ctx *c, std::vector<double> &L, std::vector<double> &wl,
+                 std::vector<int32_t> &code);
 int mc_run_qmc_dump(mc_ctx *c, int stream, int64_t first, int n, int ndim, double *out);
 int mc_run_launch_dump(mc_ctx *c, int64_t first_id, int n, double *rec, int32_t *bin_leaf);
 int mc_run_dust_emission(mc_ctx *c, double report[MC_DUSTRT_NREPORT]);

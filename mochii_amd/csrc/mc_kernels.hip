@@ -2745,6 +2745,8 @@ __global__ __launch_bounds__(kBlock) void k_ir_heat(DevGrid g, const double *ira
     block_reduce<3>(v, ismax, partials);
 }
 
+#include "mc_lines_dev.h"
+
 // The launch's Sobol/Owen uniforms u(index, d) of one stream (0 stellar,
 // 1 diffuse) for n consecutive 0-based indices, row-major n x ndim: the
 // same qmc_u as k_launch / k_launch_diffuse, for comparison with the
@@ -3486,4 +3488,88 @@ int mc_run_launch_dump(mc_ctx *c, int64_t first_id, int n, double *rec, int32_t 
         for (int i = 0; i < n; ++i) bin_leaf[(size_t)i * 2 + k] = ib[(size_t)i];
     }
     return 0;
+}
+
+// lines_write's luminosities (mc_lines_dev.h): recombination tables, then
+// the metal ions of the registry with an n-level atom
+int mc_run_lines(mc_ctx *c, std::vector<double> &L, std::vector<double> &wl,
+                 std::vector<int32_t> &code)
+{
+    hipStream_t st = c->stream;
+    std::vector<DevLineTab> tabs;
+    static const int kCode[3] = {101, 202, 201};      // H I, He II, He I
+    int nline = 0;
+    for (int k = 0; k < 3; ++k) {
+        if (!c->lt_on[k] || c->lt[k].nlin == 0) continue;
+        const mc_line_table &t = c->lt[k];
+        DevLineTab d;
+        std::memset(&d, 0, sizeof(d));
+        d.nT = t.nT; d.nD = t.nD; d.nlin = t.nlin; d.out0 = nline;
+        for (int i = 0; i < t.nT; ++i) d.tgrid[i] = t.tgrid[i];
+        for (int j = 0; j < t.nD; ++j) d.dgrid[j] = t.dgrid[j];
+        for (int l = 0; l < t.nlin; ++l) {
+            for (int i = 0; i < MC_LT_MAXG; ++i)
+                for (int j = 0; j < MC_LT_MAXG; ++j)
+                    d.etab[((size_t)l * MC_LT_MAXG + i) * MC_LT_MAXG + j] = t.etab[l][i][j];
+            wl.push_back(t.wl[l]);
+            code.push_back(kCode[k]);
+        }
+        nline += t.nlin;
+        tabs.push_back(d);
+    }
+    std::vector<DevLineIon> ions;
+    if (c->ntr > 0) {
+        const int rc = mc_lines_ions(c, nline, ions, wl, code);
+        if (rc < 0) return -1;
+        nline = rc;
+    }
+    if (nline > kLnMaxLines) {
+        mc_set_error("mc_lines_compute: more than 512 lines");
+        return -1;
+    }
+    L.assign((size_t)nline, 0.0);
+    if (nline == 0) return 0;
+    double *dL = nullptr;
+    DevLineTab *dT = nullptr;
+    DevLineIon *dI = nullptr;
+    MC_HIP(hipMalloc(&dL, sizeof(double) * nline));
+    int rc = 0;
+    hipError_t e = hipMemsetAsync(dL, 0, sizeof(double) * nline, st);
+    if (e == hipSuccess && !tabs.empty()) e = hipMalloc(&dT, sizeof(DevLineTab) * tabs.size());
+    if (e == hipSuccess && !tabs.empty())
+        e = hipMemcpy(dT, tabs.data(), sizeof(DevLineTab) * tabs.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !ions.empty()) e = hipMalloc(&dI, sizeof(DevLineIon) * ions.size());
+    if (e == hipSuccess && !ions.empty())
+        e = hipMemcpy(dI, ions.data(), sizeof(DevLineIon) * ions.size(), hipMemcpyHostToDevice);
+    const unsigned nb = (unsigned)((c->nleaf + kBlock - 1) / kBlock);
+    if (e == hipSuccess) {
+        int it = 0;
+        for (int k = 0; k < 3; ++k) {
+            if (!c->lt_on[k] || c->lt[k].nlin == 0) continue;
+            k_lines_rec<<<nb, kBlock, 0, st>>>(c->grid, c->st, dT + it, k, c->p.He_abund,
+                                               c->p.distance2cm, dL);
+            ++it;
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && !ions.empty()) {
+        rc = metal_sync(c);
+        int nlmax = 0;
+        for (const DevLineIon &d : ions) nlmax = std::max(nlmax, d.nlev);
+        if (nlmax <= 6)
+            k_lines_metal<6><<<nb, kBlock, 0, st>>>(c->grid, c->st, make_sw(c), dI,
+                                                    (int)ions.size(), c->p.distance2cm, dL);
+        else
+            k_lines_metal<kLnLev><<<nb, kBlock, 0, st>>>(c->grid, c->st, make_sw(c), dI,
+                                                         (int)ions.size(), c->p.distance2cm, dL);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(L.data(), dL, sizeof(double) * nline, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(dL);
+    if (dT) (void)hipFree(dT);
+    if (dI) (void)hipFree(dI);
+    MC_HIP(e);
+    return rc ? -100 : 0;
 }

@@ -162,6 +162,8 @@ void to_slot(const std::vector<double> &lam, double *slot)
             slot[kNlT + (size_t)j * kNlT + i] = lam[(size_t)j * kNlT + i] / std::fmax(slot[i], DBL_MIN);
 }
 
+}  // namespace
+
 int find_atom(const mc_ctx *c, const char *name, int stage)
 {
     for (size_t k = 0; k < c->nl_atoms.size(); ++k)
@@ -169,8 +171,6 @@ int find_atom(const mc_ctx *c, const char *name, int stage)
             return (int)k;
     return -1;
 }
-
-}  // namespace
 
 extern "C" int mc_nlevel_setup(mc_ctx *c, int n, const mc_nlevel_atom *atoms)
 {
@@ -283,5 +283,120 @@ int mc_nl_ready(mc_ctx *c)
     c->cool.nlt = c->d_nlt;
     c->cool.nl_hi = hi_slot;
     for (int s = 0; s < MC_GPU_MAXST; ++s) c->mtab.s_nl[s] = st_slot[s];
+    return 0;
+}
+
+extern "C" int mc_lines_setup(mc_ctx *c, const mc_line_table *hi, const mc_line_table *heii,
+                              const mc_line_table *hei)
+{
+    if (!c) {
+        mc_set_error("null context");
+        return -1;
+    }
+    const mc_line_table *t3[3] = {hi, heii, hei};
+    for (int k = 0; k < 3; ++k) {
+        const mc_line_table *t = t3[k];
+        if (!t) continue;
+        bool ok = t->nT >= 2 && t->nT <= MC_LT_MAXG && t->nD >= 2 && t->nD <= MC_LT_MAXG &&
+                  t->nlin >= 0 && t->nlin <= MC_LT_MAXL;
+        for (int i = 1; ok && i < t->nT; ++i) ok = t->tgrid[i] > t->tgrid[i - 1];
+        for (int i = 1; ok && i < t->nD; ++i) ok = t->dgrid[i] > t->dgrid[i - 1];
+        for (int l = 0; ok && l < t->nlin; ++l)
+            for (int i = 0; ok && i < t->nT; ++i)
+                for (int j = 0; ok && j < t->nD; ++j) ok = t->etab[l][i][j] > 0.0;
+        if (!ok) {
+            mc_set_error("mc_lines_setup: a table needs 2..32 ascending T and n_e nodes, at most "
+                         "12 lines, positive emissivities");
+            return -1;
+        }
+    }
+    if (!hi) {
+        mc_set_error("mc_lines_setup: the H I table is required");
+        return -1;
+    }
+    for (int k = 0; k < 3; ++k) {
+        c->lt_on[k] = t3[k] != nullptr;
+        if (t3[k]) c->lt[k] = *t3[k];
+    }
+    return 0;
+}
+
+// the metal ions of the line list: every registry (element, stage) with an
+// n-level atom, in the reference's order, oriented for the device
+int mc_lines_ions(mc_ctx *c, int out0, std::vector<DevLineIon> &ions, std::vector<double> &wl,
+                  std::vector<int32_t> &code)
+{
+    int out = out0;
+    for (int ie = 0, t0 = 0; ie < c->nel; t0 += c->elements[ie].nstage - 1, ++ie) {
+        const mc_element &e = c->elements[ie];
+        for (int st = 1; st <= e.nstage; ++st) {
+            const int ia = find_atom(c, e.name, st);
+            if (ia < 0) continue;
+            const mc_nlevel_atom &a = c->nl_atoms[ia];
+            if (a.nlev > kLnLev || a.nrad > kLnRad || a.nups > kLnUps) {
+                mc_set_error(std::string("mc_lines_compute: atom ") + e.name + " " +
+                             std::to_string(st) + " exceeds the device line solve (16 levels, 48 "
+                             "radiative and 96 collision rows)");
+                return -1;
+            }
+            DevLineIon d;
+            std::memset(&d, 0, sizeof(d));
+            d.nlev = a.nlev; d.nrad = a.nrad; d.nups = a.nups; d.out0 = out;
+            // stage st of the element: the lower stage of its first
+            // transition (st = 1) or the upper stage of transition st - 2
+            d.t = st == 1 ? t0 : t0 + st - 2;
+            d.up = st == 1 ? 0 : 1;
+            d.abund = e.abund;
+            for (int k = 0; k < a.nrad; ++k) {
+                int lo, hi;
+                double de;
+                emitting(a, a.rl[k], a.ru[k], lo, hi, de);
+                d.rlo[k] = lo; d.rhi[k] = hi; d.A[k] = a.A[k]; d.rde[k] = de;
+                const double dcm = std::fabs(a.e_cm1[a.ru[k] - 1] - a.e_cm1[a.rl[k] - 1]);
+                wl.push_back(dcm > 0.0 ? 1.0e8 / dcm : 0.0);
+                code.push_back(100 * e.Z + st);
+            }
+            for (int k = 0; k < a.nups; ++k) {
+                int lo, hi;
+                double de;
+                emitting(a, a.ul_l[k], a.ul_u[k], lo, hi, de);
+                d.clo[k] = lo; d.chi[k] = hi; d.cde[k] = de;
+                d.gu[k] = a.g[hi];
+                d.gr[k] = a.g[hi] / a.g[lo];
+                d.ttype[k] = a.ttype[k]; d.logf[k] = a.logf[k]; d.nc[k] = a.nc[k];
+                d.cups[k] = a.cups[k]; d.dery[k] = a.dery[k];
+                d.st_lo[k] = a.st_lo[k]; d.st_hi[k] = a.st_hi[k];
+                for (int j = 0; j < 16; ++j) d.coef[k][j] = a.coef[k][j];
+            }
+            ions.push_back(d);
+            out += a.nrad;
+        }
+    }
+    return out;
+}
+
+extern "C" int mc_lines_compute(mc_ctx *c, int maxn, double *L, double *wl, int32_t *ion,
+                                int32_t *n)
+{
+    if (!c) {
+        mc_set_error("null context");
+        return -1;
+    }
+    if (!c->state_ready || !c->lt_on[0]) {
+        mc_set_error("mc_lines_compute: needs the gas state and the H I line table "
+                     "(mc_lines_setup)");
+        return -1;
+    }
+    std::vector<double> Lv, wv;
+    std::vector<int32_t> cv;
+    const int rc = mc_run_lines(c, Lv, wv, cv);
+    if (rc) return rc;
+    const int m = std::min<int>(maxn, (int)Lv.size());
+    for (int k = 0; k < m; ++k) {
+        if (L) L[k] = Lv[k];
+        if (wl) wl[k] = wv[k];
+        if (ion) ion[k] = cv[k];
+    }
+    if (n) *n = (int32_t)Lv.size();
     return 0;
 }

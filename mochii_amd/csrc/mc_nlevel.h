@@ -95,3 +95,29 @@ MC_HD inline double nl_apply(const double *tab, const NlPos &p, double lam_fit)
     const double fcov = fmin(1.0, lamref / fmax(lam_fit, DBL_MIN));
     return lam_fit * (fcov * S + (1.0 - fcov));
 }
+
+// ---- line luminosities (mc_lines_compute) ----------------------------------
+constexpr int kLnLev = 16, kLnRad = 48, kLnUps = 96, kLnMaxLines = 512;
+
+// one metal ion's atom, oriented on the host: radiative rows as (lower,
+// upper) 0-based with dE [erg]; collision rows as (lower, upper), g_u,
+// g_u / g_l, dE [erg] and the Upsilon fit
+struct DevLineIon {
+    int32_t nlev, nrad, nups, out0;    // out0: index of the ion's first line
+    int32_t t, up;                     // stage fraction: up ? up[t] : lo[t] of m_fracs
+    double abund;
+    int32_t rlo[kLnRad], rhi[kLnRad];
+    double A[kLnRad], rde[kLnRad];
+    int32_t clo[kLnUps], chi[kLnUps], ttype[kLnUps], logf[kLnUps], nc[kLnUps];
+    double gu[kLnUps], gr[kLnUps], cde[kLnUps];
+    double cups[kLnUps], dery[kLnUps], st_lo[kLnUps], st_hi[kLnUps];
+    double coef[kLnUps][16];
+};
+
+// one recombination-line table, etab[line][iT][iD]
+struct DevLineTab {
+    int32_t nT, nD, nlin, out0;
+    double tgrid[MC_LT_MAXG], dgrid[MC_LT_MAXG];
+    double etab[MC_LT_MAXL * MC_LT_MAXG * MC_LT_MAXG];
+};
+

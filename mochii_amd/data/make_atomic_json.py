@@ -1,6 +1,7 @@
 """Write mochii_amd/data/atomic_registry.json from a MoCHII data/atomic
 directory (element_<el>.txt and cooling_<el>_<i>.txt in the reference's text
-formats) and the n-level atoms nlevel_<el>_<stage>.txt.  The JSON is a data snapshot of the published coefficients those
+formats) the n-level atoms nlevel_<el>_<stage>.txt and the recombination-line tables
+(Storey & Hummer 1995 H I / He II, Porter et al. 2012 He I).  The JSON is a data snapshot of the published coefficients those
 files hold (VFKY96 / VY95 photoionisation, Voronov collisional ionisation,
 Badnell / CHIANTI recombination, Huang et al. 2023 and Kingdon & Ferland 1996
 charge exchange, CHIANTI line-cooling fits, CHIANTI level energies,
@@ -14,7 +15,8 @@ import os
 import sys
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", ".."))
-from mochii_amd.atomic import ORDER, SNAPSHOT, nlevel_files, read_registry  # noqa: E402
+from mochii_amd.atomic import (ORDER, SNAPSHOT, line_table_files, nlevel_files,  # noqa: E402
+                               read_registry)
 
 
 def provenance(path):
@@ -35,10 +37,15 @@ def main(atomic_dir, out=SNAPSHOT):
     nlev = nlevel_files(atomic_dir)
     for key in nlev:
         prov[f"nlevel_{key}.txt"] = provenance(os.path.join(atomic_dir, f"nlevel_{key}.txt"))[:3]
-    doc = {"source": "MoCHII data/atomic (element, cooling-fit and n-level atom files)",
+    lt = line_table_files(atomic_dir)
+    for key in lt:
+        prov[f"{key}.txt"] = provenance(os.path.join(atomic_dir, f"{key}.txt"))[:3]
+    doc = {"source": "MoCHII data/atomic (element, cooling-fit, n-level atom and "
+                     "recombination-line files)",
            "elements": {k: dict(v, cooling={str(s): f for s, f in v["cooling"].items()})
                         for k, v in reg.items()},
            "nlevel": nlev,
+           "line_tables": lt,
            "provenance": prov}
     with open(out, "w") as f:
         json.dump(doc, f, indent=1)

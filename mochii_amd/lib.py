@@ -127,6 +127,8 @@ def library():
         "mc_set_warm_start": ([vp, C.c_int], C.c_int),
         "mc_set_warm_levels": ([vp, C.c_int], C.c_int),
         "mc_nlevel_setup": ([vp, C.c_int, vp], C.c_int),
+        "mc_lines_setup": ([vp, vp, vp, vp], C.c_int),
+        "mc_lines_compute": ([vp, C.c_int, dp, dp, ip, ip], C.c_int),
         "mc_qmc_uniforms": ([vp, C.c_int, C.c_int64, C.c_int, C.c_int, dp], C.c_int),
         "mc_launch_records": ([vp, C.c_int64, C.c_int, dp, ip], C.c_int),
         "mc_set_packet_order": ([vp, C.c_int], C.c_int),

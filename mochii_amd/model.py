@@ -218,6 +218,38 @@ class Model:
         self._nla, n = nlevel_struct(self.atoms)
         check(self._L.mc_nlevel_setup(self._h, n, self._nla), "mc_nlevel_setup")
 
+    def lines_setup(self, line_case="B", case_ab=None):
+        """sh95_setup: the H I / He II recombination-line tables of the line
+        case (par%line_case 'A', 'B' or 'follow' = the continuum case
+        case_ab) and the He I (Porter, case B) table of the data snapshot."""
+        from .atomic import LineTable, load_line_tables
+        tabs = load_line_tables()
+        if case_ab is None:
+            case_ab = "A" if self.params.case_A else "B"
+        case = case_ab if line_case == "follow" else line_case
+        self.line_case = case
+        self._line_src = [tabs[f"sh95_hi_case{case}"], tabs.get(f"sh95_heii_case{case}"),
+                          tabs.get("hei_porter_caseB")]
+        self._lts = [LineTable.make(t) if t is not None else None for t in self._line_src]
+        args = [C.byref(t) if t is not None else None for t in self._lts]
+        check(self._L.mc_lines_setup(self._h, *args), "mc_lines_setup")
+
+    def lines_compute(self):
+        """lines_write's luminosities on the current state: dict of arrays
+        L [erg/s], wl [A], ion (100 Z + stage) and labels (recombination
+        line names, '' for the n-level rows), in the reference's order."""
+        n = C.c_int32()
+        check(self._L.mc_lines_compute(self._h, 0, None, None, None, C.byref(n)),
+              "mc_lines_compute")
+        L, wl = np.zeros(n.value), np.zeros(n.value)
+        ion = np.zeros(n.value, dtype=np.int32)
+        check(self._L.mc_lines_compute(self._h, n.value, _dp(L), _dp(wl),
+                                       ion.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(n)),
+              "mc_lines_compute")
+        labels = [ln["label"] for t in self._line_src if t is not None for ln in t["lines"]]
+        labels += [""] * (n.value - len(labels))
+        return dict(L=L, wl=wl, ion=ion, labels=labels)
+
     def gas_thermal_update(self):
         a, b, c, d = C.c_double(), C.c_double(), C.c_double(), C.c_double()
         check(self._L.mc_gas_thermal_update(self._h, C.byref(a), C.byref(b), C.byref(c),

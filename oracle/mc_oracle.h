@@ -330,6 +330,8 @@ double orc_nlcool_apply(const orc_nlcool *t, double lam_fit, double T, double ne
 /* the stage table of registry element ie (0-based), stage 1-based */
 void   orc_set_stage_table(orc_ctx *c, int ie, int stage, const orc_nlcool *t);
 void   orc_set_hi_table(orc_ctx *c, const orc_nlcool *t);
+int    orc_lines(const orc_ctx *c, const orc_sh95 *const *tab, const orc_nlatom *const *atom,
+                 double *L, int nthreads);
 double orc_sh95_emis(const orc_sh95 *t, int k, double T, double ne);
 
 orc_ctx *orc_create(const orc_params *p);

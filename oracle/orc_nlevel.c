@@ -315,3 +315,120 @@ void orc_set_stage_table(orc_ctx *c, int ie, int stage, const orc_nlcool *t)
 }
 
 void orc_set_hi_table(orc_ctx *c, const orc_nlcool *t) { c->cool.hi_tab = t; }
+
+/* ---- lines_write (src/lines_mod.f90:37-352) ------------------------------ *
+ * Line luminosities on the current state, in the reference's table order:
+ * the rows of the recombination tables tab[0] (H I, n_e n_p), tab[1]
+ * (He II, n_e n_HeIII) and tab[2] (He I, n_e n_HeII), any of them NULL,
+ * then for every registry element and stage with an atom
+ * (atom[ie * ORC_MAX_ST + stage - 1], NULL = none) its radiative rows.
+ * Leaves are split over nthreads threads with private sums, added in
+ * thread order.  Returns the number of lines written to L. */
+#include <pthread.h>
+#include <stdlib.h>
+
+typedef struct {
+    const orc_ctx *c;
+    const orc_sh95 *const *tab;
+    const orc_nlatom *const *atom;
+    int lo, hi, nline;
+    double *L;
+} lines_work;
+
+static double leaf_vol_cm(const orc_ctx *c, int il)
+{
+    double h;
+    if (c->car) h = 0.5 * c->dcell;
+    else h = c->ch[c->icell_of_leaf[il] - 1];
+    const double w = 2.0 * h * c->p.distance2cm;
+    return (w * w) * w;
+}
+
+static void *lines_worker(void *arg)
+{
+    lines_work *w = (lines_work *)arg;
+    const orc_ctx *c = w->c;
+    double emis[ORC_NL_MAXTR], frac[ORC_MAX_ST];
+    for (int il = w->lo; il < w->hi; ++il) {
+        const double nH = c->nH[il];
+        if (nH <= 0.0) continue;
+        const double T = c->Te[il], ne = c->ne[il];
+        const double vol = leaf_vol_cm(c, il);
+        int out = 0;
+        for (int t = 0; t < 3; ++t) {
+            const orc_sh95 *tb = w->tab[t];
+            if (!tb) continue;
+            double x;
+            int on = 1;
+            if (t == 0) x = 1.0 - c->xHI[il];
+            else if (t == 1) {
+                x = 1.0 - c->xHeI[il] - c->xHeII[il];
+                if (x < 0.0) x = 0.0;
+                on = x > 1.0e-30;
+            } else {
+                x = c->xHeII[il];
+                on = x > 1.0e-30;
+            }
+            if (on) {
+                const double pre = t == 0 ? ne * nH * x : ne * nH * c->p.He_abund * x;
+                for (int k = 0; k < tb->nlin; ++k) {
+                    const double e = pre * orc_sh95_emis(tb, k, T, ne);
+                    w->L[out + k] = w->L[out + k] + e * vol;
+                }
+            }
+            out += tb->nlin;
+        }
+        const double nHI = nH * c->xHI[il], nHII = nH - nHI;
+        for (int ie = 0; ie < c->nel; ++ie) {
+            const orc_element *el = &c->el[ie];
+            int have = 0;
+            for (int st = 1; st <= el->nstage; ++st) have |= w->atom[ie * ORC_MAX_ST + st - 1] != NULL;
+            if (!have) continue;
+            orc_species_fractions(c, ie, il, T, ne, nHI, nHII, frac);
+            for (int st = 1; st <= el->nstage; ++st) {
+                const orc_nlatom *a = w->atom[ie * ORC_MAX_ST + st - 1];
+                if (!a) continue;
+                const double nion = el->abund * nH * frac[st - 1];
+                if (nion > 1.0e-30) {
+                    orc_nlevel_emissivities(a, T, ne, emis);
+                    for (int k = 0; k < a->nrad; ++k)
+                        w->L[out + k] = w->L[out + k] + emis[k] * nion * vol;
+                }
+                out += a->nrad;
+            }
+        }
+    }
+    return NULL;
+}
+
+int orc_lines(const orc_ctx *c, const orc_sh95 *const *tab, const orc_nlatom *const *atom,
+              double *L, int nthreads)
+{
+    int nline = 0;
+    for (int t = 0; t < 3; ++t)
+        if (tab[t]) nline += tab[t]->nlin;
+    for (int ie = 0; ie < c->nel; ++ie)
+        for (int st = 1; st <= c->el[ie].nstage; ++st)
+            if (atom[ie * ORC_MAX_ST + st - 1]) nline += atom[ie * ORC_MAX_ST + st - 1]->nrad;
+    if (nthreads < 1) nthreads = 1;
+    lines_work *wk = (lines_work *)calloc(nthreads, sizeof(lines_work));
+    pthread_t *th = (pthread_t *)calloc(nthreads, sizeof(pthread_t));
+    for (int t = 0; t < nthreads; ++t) {
+        wk[t].c = c; wk[t].tab = tab; wk[t].atom = atom; wk[t].nline = nline;
+        wk[t].lo = (int)((int64_t)c->nleaf * t / nthreads);
+        wk[t].hi = (int)((int64_t)c->nleaf * (t + 1) / nthreads);
+        wk[t].L = (double *)calloc(nline + 1, sizeof(double));
+    }
+    for (int t = 1; t < nthreads; ++t) pthread_create(&th[t], NULL, lines_worker, &wk[t]);
+    lines_worker(&wk[0]);
+    for (int t = 1; t < nthreads; ++t) pthread_join(th[t], NULL);
+    for (int k = 0; k < nline; ++k) {
+        double s = 0.0;
+        for (int t = 0; t < nthreads; ++t) s = s + wk[t].L[k];
+        L[k] = s;
+    }
+    for (int t = 0; t < nthreads; ++t) free(wk[t].L);
+    free(wk);
+    free(th);
+    return nline;
+}

@@ -146,7 +146,8 @@ def _load():
             ("orc_nlcool_apply", [C.c_void_p, C.c_double, C.c_double, C.c_double], C.c_double),
             ("orc_sh95_emis", [C.c_void_p, C.c_int, C.c_double, C.c_double], C.c_double),
             ("orc_set_stage_table", [C.c_void_p, C.c_int, C.c_int, C.c_void_p], None),
-            ("orc_set_hi_table", [C.c_void_p, C.c_void_p], None)):
+            ("orc_set_hi_table", [C.c_void_p, C.c_void_p], None),
+            ("orc_lines", [C.c_void_p, C.c_void_p, C.c_void_p, _dp, C.c_int], C.c_int)):
         f = getattr(lib, name)
         f.argtypes, f.restype = args, res
     lib.orc_set_solve_threads.restype = None
@@ -517,6 +518,25 @@ class Model:
                 t = table(f"{el['name']}_{st}") if st in el.get("cooling", {}) else None
                 L.orc_set_stage_table(self.c, ie, st, C.byref(t) if t is not None else None)
 
+    def lines(self, tables, atoms, nthreads=None):
+        """lines_write's luminosities [erg/s]: tables = (H I, He II, He I)
+        recombination tables (dicts or None), atoms = {'<el>_<stage>': atom};
+        the reference's table order."""
+        self._lt = [Sh95.make(t) if t is not None else None for t in tables]
+        tp = (C.c_void_p * 3)(*[C.addressof(t) if t is not None else None for t in self._lt])
+        self._la = []
+        ap = (C.c_void_p * (ORC_MAX_EL_PY * ORC_MAX_ST_PY))()
+        for ie, el in enumerate(self.elements):
+            for st in range(1, el["nstage"] + 1):
+                key = f"{el['name']}_{st}"
+                if key in atoms:
+                    a = NlAtom.make(atoms[key])
+                    self._la.append(a)
+                    ap[ie * ORC_MAX_ST_PY + st - 1] = C.addressof(a)
+        L = np.zeros(4096)
+        n = lib().orc_lines(self.c, tp, ap, ptr(L), int(nthreads or min(os.cpu_count() or 1, 16)))
+        return L[:n].copy()
+
     def thermal_update(self, nthreads=None):
         """gas_thermal_update; the per-leaf solves on nthreads threads (default
         all host cores, capped at 16); the result does not depend on it."""
@@ -817,6 +837,7 @@ def ref_lib():
 
 # ---- n-level atoms, local-n_e cooling tables, recombination lines ---------
 NL_MAXLEV, NL_MAXTR, NLC_NT, NLC_NNE = 40, 600, 401, 161
+ORC_MAX_EL_PY, ORC_MAX_ST_PY = 11, 6
 
 
 class NlAtom(C.Structure):
@@ -862,6 +883,29 @@ class NlAtom(C.Structure):
 
     def ups(self, k, T):
         return lib().orc_ups_eval(C.byref(self), k, T)
+
+
+class Sh95(C.Structure):
+    """Mirror of orc_sh95 (etab compact: [line][T][n_e])."""
+    _fields_ = [("nT", C.c_int), ("nD", C.c_int), ("nlin", C.c_int),
+                ("tgrid", C.c_double * 32), ("dgrid", C.c_double * 32), ("wls", C.c_double * 12),
+                ("etab", C.c_double * (12 * 32 * 32))]
+
+    @classmethod
+    def make(cls, tab):
+        """From a table dict (mochii_amd.atomic.read_line_table)."""
+        t = cls()
+        t.nT, t.nD, t.nlin = len(tab["T"]), len(tab["NE"]), len(tab["lines"])
+        t.tgrid[:t.nT] = tab["T"]
+        t.dgrid[:t.nD] = tab["NE"]
+        for k, ln in enumerate(tab["lines"]):
+            t.wls[k] = ln["wl"]
+            e = np.asarray(ln["e"], dtype=np.float64).ravel()
+            t.etab[k * t.nT * t.nD:(k + 1) * t.nT * t.nD] = e
+        return t
+
+    def emis(self, k, T, ne):
+        return lib().orc_sh95_emis(C.byref(self), k, T, ne)
 
 
 _NLC_CACHE = {}        # (atom key, atom dict id) -> table: built once per process
