@@ -1,6 +1,7 @@
 // Host-side set-up of a transport context: band edges, the continuous
 // source samplers, the Sobol/Owen tables and the octree build.  Runs once per
 // context (or per grid), so it is plain FP64 C++ on the host.
+#include <thread>
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -229,8 +230,8 @@ int mc_setup_samplers(mc_ctx *c)
 {
     const mc_params &p = c->p;
     c->ncomp = p.nsource + (p.ext_on ? 1 : 0);
-    if (c->ncomp < 1 || c->ncomp > MC_MAXSRC) {
-        mc_set_error("need 1..16 source components");
+    if (c->ncomp < 1 || p.nsource < 0 || p.nsource > MC_MAXSRC) {
+        mc_set_error("need 0..16 point sources and at least one source component");
         return -1;
     }
     c->multi = c->ncomp >= 2;
@@ -353,18 +354,28 @@ int mc_build_octree_host(mc_ctx *c, int nleaf, const double *xl, const double *y
         return false;
     };
     c->hneighbor.assign((size_t)nc * 6, -1);
-    for (int i = 0; i < nc; ++i) {
-        const double x = cx[i], y = cy[i], z = cz[i], hp = 2.0 * ch[i];
-        int *nb = &c->hneighbor[(size_t)i * 6];
-        if (x + hp <= half) nb[0] = at_level(x + hp, y, z, lv[i]);
-        if (x - hp >= -half) nb[1] = at_level(x - hp, y, z, lv[i]);
-        if (y + hp <= half) nb[2] = at_level(x, y + hp, z, lv[i]);
-        if (y - hp >= -half) nb[3] = at_level(x, y - hp, z, lv[i]);
-        if (z + hp <= half) nb[4] = at_level(x, y, z + hp, lv[i]);
-        if (z - hp >= -half) nb[5] = at_level(x, y, z - hp, lv[i]);
-        for (int f = 0; f < 6; ++f)
-            if (nb[f] >= 0 && nb[f] != i && ancestor(nb[f], i)) nb[f] = -1;
-    }
+    // the cells' face neighbours are independent lookups: host threads
+    auto nbr_range = [&](int i0, int i1) {
+        for (int i = i0; i < i1; ++i) {
+            const double x = cx[i], y = cy[i], z = cz[i], hp = 2.0 * ch[i];
+            int *nb = &c->hneighbor[(size_t)i * 6];
+            if (x + hp <= half) nb[0] = at_level(x + hp, y, z, lv[i]);
+            if (x - hp >= -half) nb[1] = at_level(x - hp, y, z, lv[i]);
+            if (y + hp <= half) nb[2] = at_level(x, y + hp, z, lv[i]);
+            if (y - hp >= -half) nb[3] = at_level(x, y - hp, z, lv[i]);
+            if (z + hp <= half) nb[4] = at_level(x, y, z + hp, lv[i]);
+            if (z - hp >= -half) nb[5] = at_level(x, y, z - hp, lv[i]);
+            for (int f = 0; f < 6; ++f)
+                if (nb[f] >= 0 && nb[f] != i && ancestor(nb[f], i)) nb[f] = -1;
+        }
+    };
+    const int nth = nc < 65536 ? 1
+                               : (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int t = 1; t < nth; ++t)
+        th.emplace_back(nbr_range, (int)((int64_t)nc * t / nth), (int)((int64_t)nc * (t + 1) / nth));
+    nbr_range(0, (int)((int64_t)nc / nth));
+    for (auto &t : th) t.join();
     c->ncells = nc;
     c->nleaf = nleaf;
     return 0;

@@ -198,3 +198,159 @@ def hii_octree(n_field=128, nphotons=10_000_000, precision=MC_FP64, seed=12345, 
                             nH=nHl, diffuse=diffuse, metals=metals,
                             cooling_model=cooling_model)
     return params, nHl, (xl, yl, zl, lev, half)
+
+
+# ---- large workloads ---------------------------------------------------------
+def _hash_normal(seed, level, ix, iy, iz):
+    """Seeded standard normals keyed by (level, cell index): two splitmix64
+    finaliser rounds over the packed key, Box-Muller on the two halves."""
+    M = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+    def mix(z):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9) & M
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB) & M
+        return z ^ (z >> np.uint64(31))
+
+    with np.errstate(over="ignore"):
+        k = (np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15) + np.uint64(level) * np.uint64(0x632BE59BD9B4E019)) & M
+        k = mix(k ^ ix.astype(np.uint64))
+        k = mix(k ^ (iy.astype(np.uint64) << np.uint64(21)))
+        k = mix(k ^ (iz.astype(np.uint64) << np.uint64(42)))
+    u1 = ((k >> np.uint64(40)).astype(np.float64) + 0.5) / 2.0**24
+    u2 = (((k >> np.uint64(16)) & np.uint64(0xFFFFFF)).astype(np.float64) + 0.5) / 2.0**24
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+# the cascade levels and per-level log-widths of the multifractal density
+# (three lognormal multipliers, on cells of levels 3, 6 and 9): the leaf
+# densities of the default field span 1.2e5; MF_MASS_THR refines it to
+# ~2.0e7 leaves on levels 5-10
+CASCADE_LEVELS = (3, 6, 9)
+CASCADE_SIGMA = (0.85, 0.65, 0.5)
+MF_MASS_THR = 1.65e-7
+
+
+def _cascade_at(ix, iy, iz, level, seed, mean):
+    """Point density [cm^-3] at the centres of cells (ix, iy, iz) of a level
+    of the 3-level lognormal cascade: n_H = mean x prod_k exp(s_k g_k -
+    s_k^2 / 2) with g_k the seeded standard normal of the level-
+    CASCADE_LEVELS[k] cell holding the point (E[n_H] = mean)."""
+    rho = np.full(ix.shape, float(mean))
+    for lc, s in zip(CASCADE_LEVELS, CASCADE_SIGMA):
+        d = lc - level
+        if d <= 0:
+            jx, jy, jz = ix >> -d, iy >> -d, iz >> -d
+        else:          # a coarser cell: the cascade cell at its centre (x >= c convention)
+            h = 1 << (d - 1)
+            jx, jy, jz = (ix << d) + h, (iy << d) + h, (iz << d) + h
+        rho *= np.exp(s * _hash_normal(seed, lc, jx, jy, jz) - 0.5 * s * s)
+    return rho
+
+
+def multifractal_octree(lmax=10, lmin=3, half=50.0, mean=10.0, seed=5, mass_thr=None,
+                        target=2.0e7):
+    """Leaf list of an octree over [-half, half]^3 pc refined on the
+    multifractal density: a cell is split while its point density times its
+    volume (in units of the root volume) exceeds mass_thr, down to lmax and
+    never coarser than lmin; leaf n_H is the density at the leaf centre.
+    mass_thr defaults to the value that gives ~target leaves for the default
+    field.  Returns (x, y, z, level, nH)."""
+    thr = MF_MASS_THR if mass_thr is None else mass_thr
+    xs, ys, zs, lv, nh = [], [], [], [], []
+    cand = np.zeros((1, 3), dtype=np.int64)
+    off = np.array([[a, b, c] for c in (0, 1) for b in (0, 1) for a in (0, 1)], dtype=np.int64)
+    for l in range(0, lmax + 1):
+        ix, iy, iz = cand[:, 0], cand[:, 1], cand[:, 2]
+        rho = _cascade_at(ix, iy, iz, l, seed, mean)
+        refine = (l < lmin) | ((rho * 8.0 ** (-l) > thr * mean) & (l < lmax))
+        leaf = ~refine
+        if leaf.any():
+            w = 2.0 * half / 2 ** l
+            xs.append(-half + (ix[leaf] + 0.5) * w)
+            ys.append(-half + (iy[leaf] + 0.5) * w)
+            zs.append(-half + (iz[leaf] + 0.5) * w)
+            lv.append(np.full(int(leaf.sum()), l, dtype=np.int32))
+            nh.append(rho[leaf])
+        r = cand[refine]
+        if r.size == 0:
+            break
+        if 8 * r.shape[0] > 8 * target:
+            raise ValueError(f"multifractal_octree: {8 * r.shape[0]} cells at level {l + 1}; "
+                             f"raise mass_thr")
+        cand = (2 * r[:, None, :] + off[None, :, :]).reshape(-1, 3)
+    x, y, z, lev, nh = (np.concatenate(xs), np.concatenate(ys), np.concatenate(zs),
+                        np.concatenate(lv), np.concatenate(nh))
+    # the volume-weighted mean density is the given mean
+    nh *= mean / (np.sum(nh * 8.0 ** (-lev.astype(np.float64))))
+    return x, y, z, lev, nh
+
+
+def multifractal_cloud(nphotons=1_000_000_000, precision=MC_FP64, seed=12345, field_seed=5,
+                       half=50.0, lmax=10, nsource=16, dust=True, ir_packets=100_000_000,
+                       niter=100, tol=2e-3, tol_te=2e-3, mass_thr=None,
+                       cooling_model=MC_COOL_LOCAL_NE):
+    """Large AMR model: a 10-level octree (~2e7 leaves) over a (2 half pc)^3
+    box refined on the seeded multifractal density (3-level lognormal
+    cascade, volume-weighted mean n_H 10 cm^-3, dynamic range ~1e5),
+    nsource blackbody point sources at seeded positions in the central 80%
+    of the box with T in [30000, 55000] K and Q(H) log-uniform in
+    [3e48, 3e49] s^-1, plus an isotropic 30000 K external field entering
+    through the inscribed sphere (10% of the stellar band luminosity); H + He
+    (He/H = 0.1), thermal balance, case B; dust on (dust-to-gas 0.01, the
+    synthetic power-law table to 1e4 um, HG scattering) with the dust
+    thermal-infrared band (ir_packets per pass); nphotons packets per
+    iteration.  Returns (params, nH, octree, rhokap, dust table)."""
+    x, y, z, lev, nH = multifractal_octree(lmax=lmax, half=half, seed=field_seed,
+                                           mass_thr=mass_thr)
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-0.8 * half, 0.8 * half, size=(nsource, 3))
+    T = rng.uniform(3.0e4, 5.5e4, nsource)
+    Q = 10 ** rng.uniform(np.log10(3e48), np.log10(3e49), nsource)
+    lum = [float(q / 1e49 * L_Q49_T40K * _mean_energy_ratio(t, 4.0e4)) for q, t in zip(Q, T)]
+    rmax = 0.98 * half
+    r_cm = rmax * PC2CM
+    ext_I = 0.1 * sum(lum) / (np.pi * 4.0 * np.pi * r_cm ** 2)
+    params = dict(nphotons=nphotons, qmc_seed=seed, precision=precision, eion_min=13.598,
+                  eion_max=100.0, nnu_ion=16, ion_align_edges=1, nsource=nsource,
+                  tstar=list(T), src_lum=lum, src_x=list(pos[:, 0]), src_y=list(pos[:, 1]),
+                  src_z=list(pos[:, 2]), ext_on=1, ext_geometry=1, ext_tstar=3.0e4,
+                  ext_intensity=float(ext_I), rmax=rmax, grid_type=1, nx=1, ny=1, nz=1,
+                  xmax=half, ymax=half, zmax=half, distance2cm=PC2CM, He_abund=0.1,
+                  te_fixed=1.0e4, ion_relax=1.0, te_min=1.0e3, te_max=5.0e4, gas_tol=tol,
+                  gas_tol_te=tol_te, conv_vol=1, gas_niter=niter, case_A=0, solve_te=1,
+                  xHI_init=1.0, xHeI_init=1.0, xHeII_init=0.0, cooling_model=int(cooling_model),
+                  ion_add_dust=int(dust), ion_dust_scatter=int(dust), lambda_ref=0.55,
+                  dust_no_photons=float(ir_packets), dust_niter=2, dust_tol=1e-3)
+    rhokap = nH * 4.868e-22 * PC2CM if dust else None
+    return params, nH, (x, y, z, lev, half), rhokap, synthetic_kext(lam_max=1.0e4)
+
+
+def turbulent_cloud(n=256, nphotons=100_000_000, precision=MC_FP64, seed=12345, field_seed=4,
+                    half=16.0, nsource=4, metals=True, niter=100, tol=2e-3, tol_te=2e-3,
+                    cooling_model=MC_COOL_LOCAL_NE):
+    """Turbulent cloud on a uniform n^3 Cartesian grid over a (2 half pc)^3
+    box: lognormal n_H (mean 50 cm^-3, sigma_ln 1.0) from a seeded Gaussian
+    random field with P(k) ~ k^-11/3, nsource blackbody sources at seeded
+    positions in the central half of the box with T in [35000, 50000] K and
+    Q(H) log-uniform in [1e48, 1e49] s^-1, H/He (He/H = 0.1) with the trace
+    metals C N O Ne S Ar, thermal balance, case B; nphotons packets per
+    iteration.  Returns (params, nH)."""
+    rho = lognormal_field(n, mean=50.0, sigma_ln=1.0, seed=field_seed)
+    nH = np.ascontiguousarray(rho.ravel())        # [iz, iy, ix]: the car leaf order
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-0.5 * half, 0.5 * half, size=(nsource, 3))
+    T = rng.uniform(3.5e4, 5.0e4, nsource)
+    Q = 10 ** rng.uniform(48.0, 49.0, nsource)
+    lum = [float(q / 1e49 * L_Q49_T40K * _mean_energy_ratio(t, 4.0e4)) for q, t in zip(Q, T)]
+    params = dict(nphotons=nphotons, qmc_seed=seed, precision=precision, eion_min=13.598,
+                  eion_max=100.0, nnu_ion=16, ion_align_edges=1, nsource=nsource,
+                  tstar=list(T), src_lum=lum, src_x=list(pos[:, 0]), src_y=list(pos[:, 1]),
+                  src_z=list(pos[:, 2]), grid_type=MC_GRID_CAR, nx=n, ny=n, nz=n,
+                  xmax=half, ymax=half, zmax=half, distance2cm=PC2CM, He_abund=0.1,
+                  te_fixed=1.0e4, ion_relax=1.0, te_min=1.0e3, te_max=5.0e4, gas_tol=tol,
+                  gas_tol_te=tol_te, conv_vol=1, gas_niter=niter, case_A=0, solve_te=1,
+                  xHI_init=1.0, xHeI_init=1.0, xHeII_init=0.0, cooling_model=int(cooling_model))
+    if metals:
+        from .atomic import metal_thresholds, registry
+        params["metal_eth"] = metal_thresholds(registry(METAL_ABUND))
+    return params, nH

@@ -46,3 +46,36 @@ def test_synthetic_kext_table():
     assert lam[0] < 1.23984 / 100.0 and lam[-1] > 0.55
     assert np.allclose(alb, 0.4) and np.allclose(g, 0.6)
     assert np.allclose(np.diff(np.log(cext)) / np.diff(np.log(lam)), -1.2)
+
+
+def test_multifractal_octree_small_tiles_box_and_refines_on_mass():
+    """The cascade octree at a coarse threshold: leaves tile the box, the
+    volume-weighted mean is the given mean, denser leaves sit deeper, and
+    the field is seeded."""
+    from mochii_amd.workloads import multifractal_octree
+    x, y, z, lev, nh = multifractal_octree(lmax=6, lmin=2, half=8.0, mean=10.0, seed=3,
+                                           mass_thr=1e-4, target=1e6)
+    w = 16.0 / 2.0 ** lev
+    assert abs((w ** 3).sum() - 16.0 ** 3) < 1e-6
+    assert abs((nh * w ** 3).sum() / 16.0 ** 3 - 10.0) < 1e-9
+    assert lev.min() >= 2 and lev.max() == 6 and np.unique(lev).size >= 3
+    x2 = multifractal_octree(lmax=6, lmin=2, half=8.0, mean=10.0, seed=3, mass_thr=1e-4,
+                             target=1e6)[0]
+    assert np.array_equal(x, x2)
+    x3 = multifractal_octree(lmax=6, lmin=2, half=8.0, mean=10.0, seed=4, mass_thr=1e-4,
+                             target=1e6)[0]
+    assert x3.size != x.size or not np.array_equal(x, x3)
+    # denser leaves sit deeper: the median density grows with level
+    med = [np.median(nh[lev == l]) for l in range(3, 7) if np.any(lev == l)]
+    assert all(b > a for a, b in zip(med, med[1:]))
+
+
+def test_large_workload_parameters():
+    """The turbulent cloud's field moments and sources at a small grid size
+    (the full-size cascade octree is built by the GPU tests and the bench)."""
+    from mochii_amd.workloads import turbulent_cloud
+    p, nH = turbulent_cloud(n=32, nphotons=1000)
+    assert nH.size == 32 ** 3 and abs(nH.mean() / 50.0 - 1) < 1e-12
+    assert abs(np.log(nH).std() - 1.0) < 1e-12
+    assert p["nsource"] == 4 and all(3.5e4 <= t <= 5.0e4 for t in p["tstar"])
+    assert len(p["metal_eth"]) == 15
