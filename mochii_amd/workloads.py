@@ -247,14 +247,40 @@ def _cascade_at(ix, iy, iz, level, seed, mean):
     return rho
 
 
+def _spread3(v):
+    """Bits of v (< 2^21) spread to every third bit."""
+    v = v.astype(np.uint64) & np.uint64(0x1FFFFF)
+    v = (v | (v << np.uint64(32))) & np.uint64(0x1F00000000FFFF)
+    v = (v | (v << np.uint64(16))) & np.uint64(0x1F0000FF0000FF)
+    v = (v | (v << np.uint64(8))) & np.uint64(0x100F00F00F00F00F)
+    v = (v | (v << np.uint64(4))) & np.uint64(0x10C30C30C30C30C3)
+    v = (v | (v << np.uint64(2))) & np.uint64(0x1249249249249249)
+    return v
+
+
+def morton_order(x, y, z, level, half, lmax):
+    """Permutation that lists leaves in depth-first (Morton) order of their
+    lower corners on the level-lmax grid: leaves close in space are close in
+    the list."""
+    n = 2 ** lmax
+    w = 2.0 * half / n
+    ix = np.floor((x + half) / w - 0.5 * 2.0 ** (lmax - level) + 0.5).astype(np.int64)
+    iy = np.floor((y + half) / w - 0.5 * 2.0 ** (lmax - level) + 0.5).astype(np.int64)
+    iz = np.floor((z + half) / w - 0.5 * 2.0 ** (lmax - level) + 0.5).astype(np.int64)
+    key = _spread3(ix) | (_spread3(iy) << np.uint64(1)) | (_spread3(iz) << np.uint64(2))
+    return np.argsort(key, kind="stable")
+
+
 def multifractal_octree(lmax=10, lmin=3, half=50.0, mean=10.0, seed=5, mass_thr=None,
-                        target=2.0e7):
+                        target=2.0e7, order="morton"):
     """Leaf list of an octree over [-half, half]^3 pc refined on the
     multifractal density: a cell is split while its point density times its
     volume (in units of the root volume) exceeds mass_thr, down to lmax and
     never coarser than lmin; leaf n_H is the density at the leaf centre.
     mass_thr defaults to the value that gives ~target leaves for the default
-    field.  Returns (x, y, z, level, nH)."""
+    field.  order 'morton' lists the leaves depth-first (spatial neighbours
+    near each other in every per-leaf array), 'level' coarse to fine.
+    Returns (x, y, z, level, nH)."""
     thr = MF_MASS_THR if mass_thr is None else mass_thr
     xs, ys, zs, lv, nh = [], [], [], [], []
     cand = np.zeros((1, 3), dtype=np.int64)
@@ -282,13 +308,16 @@ def multifractal_octree(lmax=10, lmin=3, half=50.0, mean=10.0, seed=5, mass_thr=
                         np.concatenate(lv), np.concatenate(nh))
     # the volume-weighted mean density is the given mean
     nh *= mean / (np.sum(nh * 8.0 ** (-lev.astype(np.float64))))
+    if order == "morton":
+        p = morton_order(x, y, z, lev, half, lmax)
+        x, y, z, lev, nh = x[p], y[p], z[p], lev[p], nh[p]
     return x, y, z, lev, nh
 
 
 def multifractal_cloud(nphotons=1_000_000_000, precision=MC_FP64, seed=12345, field_seed=5,
                        half=50.0, lmax=10, nsource=16, dust=True, ir_packets=100_000_000,
                        niter=100, tol=2e-3, tol_te=2e-3, mass_thr=None,
-                       cooling_model=MC_COOL_LOCAL_NE):
+                       cooling_model=MC_COOL_LOCAL_NE, order="morton"):
     """Large AMR model: a 10-level octree (~2e7 leaves) over a (2 half pc)^3
     box refined on the seeded multifractal density (3-level lognormal
     cascade, volume-weighted mean n_H 10 cm^-3, dynamic range ~1e5),
@@ -301,7 +330,7 @@ def multifractal_cloud(nphotons=1_000_000_000, precision=MC_FP64, seed=12345, fi
     thermal-infrared band (ir_packets per pass); nphotons packets per
     iteration.  Returns (params, nH, octree, rhokap, dust table)."""
     x, y, z, lev, nH = multifractal_octree(lmax=lmax, half=half, seed=field_seed,
-                                           mass_thr=mass_thr)
+                                           mass_thr=mass_thr, order=order)
     rng = np.random.default_rng(seed)
     pos = rng.uniform(-0.8 * half, 0.8 * half, size=(nsource, 3))
     T = rng.uniform(3.0e4, 5.5e4, nsource)
