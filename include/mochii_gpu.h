@@ -541,6 +541,19 @@ int mc_launch_records(mc_ctx *ctx, int64_t first_id, int n, double *rec, int32_t
 int mc_trace_ray(mc_ctx *ctx, int precision, const double *pos3, const double *dir3,
                  int32_t *cells, int maxn, int32_t *n);
 
+/* mc_trace_ray for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments and result as
+ * mc_trace_ray, but the cell list is staged in a library-owned static
+ * device array of MC_TRACE_NOALLOC_MAX cells (one per GPU, held until the
+ * process ends) instead of a device buffer allocated per call: the call
+ * makes no hipMalloc / hipFree.  maxn above MC_TRACE_NOALLOC_MAX is an
+ * error.  Calls that overlap on one GPU share that array and race; the
+ * array keeps the last call's cells. */
+#define MC_TRACE_NOALLOC_MAX 65536
+int mc_trace_ray_noalloc(mc_ctx *ctx, int precision, const double *pos3, const double *dir3,
+                         int32_t *cells, int maxn, int32_t *n);
+
 #ifdef __cplusplus
 }
 #endif

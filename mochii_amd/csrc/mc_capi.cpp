@@ -815,6 +815,22 @@ extern "C" int mc_timing(mc_ctx *c, double *out4)
     return 0;
 }
 
+extern "C" int mc_trace_ray_noalloc(mc_ctx *c, int precision, const double *pos,
+                                    const double *dir, int32_t *cells, int maxn, int32_t *n)
+{
+    CHECK_CTX(c);
+    if (!pos || !dir || !cells || !n || maxn < 1) {
+        mc_set_error("mc_trace_ray_noalloc: bad arguments");
+        return -1;
+    }
+    if (maxn > MC_TRACE_NOALLOC_MAX) {
+        mc_set_error("mc_trace_ray_noalloc: maxn " + std::to_string(maxn) + " exceeds the "
+                     "device scratch of " + std::to_string(MC_TRACE_NOALLOC_MAX) + " cells");
+        return -1;
+    }
+    return mc_run_trace_noalloc(c, precision, pos, dir, cells, maxn, n);
+}
+
 extern "C" int mc_qmc_uniforms(mc_ctx *c, int stream, int64_t first_index, int n, int ndim,
                                double *out)
 {

@@ -392,6 +392,8 @@ int mc_sort_records(mc_ctx *c, int64_t n);
 int mc_run_trace(mc_ctx *c, int precision, const double *pos, const double *dir, int32_t *cells,
                  int maxn, int32_t *n);
 int mc_run_dust_temp(mc_ctx *c, const double *d_heat);
+int mc_run_trace_noalloc(mc_ctx *c, int precision, const double *pos, const double *dir,
+                         int32_t *cells, int maxn, int32_t *n);
 int mc_nl_ready(mc_ctx *c);
 int find_atom(const mc_ctx *c, const char *name, int stage);
 int mc_lines_ions(mc_ctx *c, int out0, std::vector<DevLineIon> &ions, std::vector<double> &wl,

@@ -19,6 +19,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <mutex>
+
 #include "mc_internal.h"
 
 // The trace-metal registry of the calling context, copied in before every
@@ -3251,6 +3253,65 @@ int mc_run_trace(mc_ctx *c, int precision, const double *pos, const double *dir,
     MC_HIP(hipMemcpy(cells, dcells, sizeof(int32_t) * (size_t)(*n), hipMemcpyDeviceToHost));
     (void)hipFree(dcells);
     (void)hipFree(dn);
+    return 0;
+}
+
+// ---- mc_trace_ray without device allocation -------------------------------
+// The cell list and its count go to a file-scope device array of fixed size
+// (each GPU holds its own copy until the process ends); host code reaches it
+// only through the address hipGetSymbolAddress returns for the current
+// device, looked up once per device.  Nothing in it must start at zero: the
+// kernel writes the count and the cells before they are read.  Calls that
+// overlap on one GPU share the array and race.
+static __device__ __align__(256) int32_t scratch[MC_TRACE_NOALLOC_MAX + 64];
+
+namespace {
+constexpr int kMaxDevices = 64;
+int32_t *g_scratch[kMaxDevices];
+std::mutex g_scratch_mu;
+
+int scratch_ptr(int32_t **out)
+{
+    int dev = 0;
+    MC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        mc_set_error("mc_trace_ray_noalloc: device index out of range");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            mc_set_error(std::string("mc_trace_ray_noalloc: hipGetSymbolAddress: ") +
+                         hipGetErrorString(e));
+            return -1;
+        }
+        g_scratch[dev] = static_cast<int32_t *>(p);
+    }
+    *out = g_scratch[dev];
+    return 0;
+}
+}  // namespace
+
+int mc_run_trace_noalloc(mc_ctx *c, int precision, const double *pos, const double *dir,
+                         int32_t *cells, int maxn, int32_t *n)
+{
+    int32_t *buf = nullptr;
+    if (scratch_ptr(&buf)) return -1;
+    int32_t *dcells = buf, *dn = buf + MC_TRACE_NOALLOC_MAX;     // count after the cells
+    KArgs a = make_args(c);
+    const bool face = !c->car || c->p.car_walk_amr;
+    auto kern = face ? (precision == MC_MIXED ? k_trace<MC_MIXED, kFaceExit> : k_trace<MC_FP64, kFaceExit>)
+                     : (precision == MC_MIXED ? k_trace<MC_MIXED, kCarDDA> : k_trace<MC_FP64, kCarDDA>);
+    kern<<<1, 1, 0, c->stream>>>(a, pos[0], pos[1], pos[2], dir[0], dir[1], dir[2], dcells, maxn, dn);
+    MC_HIP(hipGetLastError());
+    MC_HIP(hipMemcpyAsync(n, dn, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    MC_HIP(hipStreamSynchronize(c->stream));
+    if (*n > 0)
+        MC_HIP(hipMemcpyAsync(cells, dcells, sizeof(int32_t) * (size_t)(*n), hipMemcpyDeviceToHost,
+                              c->stream));
+    MC_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
 

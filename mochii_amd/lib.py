@@ -151,6 +151,7 @@ def library():
         "mc_set_precision": ([vp, C.c_int], C.c_int),
         "mc_timing": ([vp, dp], C.c_int),
         "mc_trace_ray": ([vp, C.c_int, dp, dp, ip, C.c_int, ip], C.c_int),
+        "mc_trace_ray_noalloc": ([vp, C.c_int, dp, dp, ip, C.c_int, ip], C.c_int),
         "mc_species_setup": ([vp, C.c_int, vp], C.c_int),
         "mc_species_fractions_get": ([vp, C.c_int, dp], C.c_int),
         "mc_species_rates_get": ([vp, C.c_int, dp, dp], C.c_int),
