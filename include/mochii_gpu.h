@@ -441,10 +441,12 @@ int mc_info(mc_ctx *ctx, int32_t *nleaf, int32_t *nnu_band, double *Ltot, double
  * packet at a time). */
 int mc_set_packet_order(mc_ctx *ctx, int order);
 
-/* Energy-moment tallies of the MC_MIXED metal walk (on by default): a scored
+/* Energy-moment tallies of the MC_MIXED walk (on by default): a scored
  * segment adds its path weight times the Chebyshev polynomials T_0..T_3 of
  * its packet's position in an ln-E sub-bin of its band bin to four per-leaf
- * moments instead of one atomic per nonzero rate coefficient; after the walk
+ * moments instead of one atomic per nonzero rate coefficient (the H/He
+ * coefficients, every metal transition's two and, with grains, the grain
+ * absorption cross section, whose table rows are sub-bin edges); after the walk
  * each coefficient sum is rebuilt from the moments with the coefficient's
  * interpolant on the sub-bin, which matches it to a relative 1e-7 (the
  * sub-bins are chosen for that on the host).  Off, or when a band bin cannot

@@ -687,12 +687,13 @@ extern "C" int mc_set_tally_moments(mc_ctx *c, int on)
 extern "C" int mc_tally_moment_info(mc_ctx *c, int32_t *nsub, double *fit_err)
 {
     CHECK_CTX(c);
-    if (c->ntr > 0 && c->mom.built_ntr != c->ntr) {
+    const int ndust = (c->p.ion_add_dust && c->dn >= 2) ? 1 : 0;
+    if (c->mom.built_ntr != c->ntr || c->mom.built_dust != ndust) {
         mc_mom_free(c);
         if (mc_mom_build(c)) return -1;
     }
-    if (nsub) *nsub = c->ntr > 0 ? c->mom.nsub : 0;
-    if (fit_err) *fit_err = c->ntr > 0 ? c->mom.fit_err : 0.0;
+    if (nsub) *nsub = c->mom.nsub;
+    if (fit_err) *fit_err = c->mom.fit_err;
     return 0;
 }
 

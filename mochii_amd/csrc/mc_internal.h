@@ -239,6 +239,7 @@ struct DevState {
 #define MC_MOM_MAXSUB 512    // sub-bins over the band at most
 struct MomTab {
     int nsub = 0, nq = 0, built_ntr = -1;
+    int ndust = 0, built_dust = -1;     // grain absorption among the coefficients (0/1)
     double fit_err = 0.0;
     std::vector<double> seg, bseg, coef;
     std::vector<int32_t> band;

@@ -79,6 +79,7 @@ struct KArgs {
     const double *mom_tab, *mom_bseg;
     int32_t mom_ns;
     int64_t mom_rows;
+    int32_t mom_dust;        // grain absorption is among the moments' coefficients
 };
 
 // H/He recombination and collisional-ionisation coefficients at one T
@@ -589,10 +590,11 @@ __device__ __forceinline__ void tally_one(const KArgs &a, const Lane &L, int il,
                 pm2 = pm1;
                 pm1 = pq;
             }
-            if (a.add_dust && L.dabs != 0.0) {
-                // dust absorption keeps its per-leaf sum: into the lane's
-                // replica row of a hot leaf, which k_fold_hot folds into
-                // the leaf's row (a direct write there would be replaced)
+            if (a.add_dust && !a.mom_dust && L.dabs != 0.0) {
+                // dust absorption outside the moments keeps its per-leaf
+                // sum: into the lane's replica row of a hot leaf, which
+                // k_fold_hot folds into the leaf's row (a direct write
+                // there would be replaced)
                 double *h, *j;
                 tally_rows(a, il, h, j);
                 unsafeAtomicAdd(h + 6, pl * L.dabs);
@@ -653,7 +655,7 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
     }
     bool has = L.s_il >= 0;
     const int nnu = a.tab->nnu;
-    constexpr bool kMom = Prec<P>::mixed && M == 1;
+    constexpr bool kMom = Prec<P>::mixed && M <= 1;
     if (a.coherent) {
         unsigned long long pend = __ballot(has);
         for (int r = 0; r < (kMom ? kAggRoundsMom : kAggRounds) && pend; ++r) {
@@ -687,7 +689,7 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
                         pm2 = pm1;
                         pm1 = pq;
                     }
-                    if (a.add_dust) {
+                    if (a.add_dust && !a.mom_dust) {
                         const double v = in ? L.s_pl * L.dabs : 0.0;
                         if (__ballot(v != 0.0)) {
                             sum = wave_sum(v);
@@ -753,7 +755,7 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
             if (in) has = false;
         }
     }
-    if (has) tally_one<M, Prec<P>::mixed && M == 1>(a, L, L.s_il, L.s_pl);
+    if (has) tally_one<M, Prec<P>::mixed && M <= 1>(a, L, L.s_il, L.s_pl);
     L.s_il = -1;
 }
 
@@ -1357,6 +1359,8 @@ __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, 
         L.nmet = n;
         L.rhe = 1.0 / he;
         L.rE = 1.0 / L.E;
+    }
+    if constexpr (M != 2) {
         // the packet's energy-moment sub-bin and coordinate t in [-1, 1];
         // a packet outside its band bin keeps the per-coefficient sums
         L.ms = -1;
@@ -1613,12 +1617,13 @@ __global__ __launch_bounds__(kBlock) void k_mom_fold_hot(DevState s, double *mom
 __global__ __launch_bounds__(kBlock) void k_mom_fold(DevState s, const double *__restrict__ mom,
                                                      int64_t rows, const double *__restrict__ coef,
                                                      const int32_t *__restrict__ band, int ns,
-                                                     int ntr, int nnu, double *mraw, int nleaf)
+                                                     int ntr, int ndust, int nnu, double *mraw,
+                                                     int nleaf)
 {
     const int il = blockIdx.x * kBlock + threadIdx.x;
     if (il >= nleaf) return;
-    constexpr int kQ = 6 + 2 * MC_GPU_MAXTR;
-    const int nq = 6 + 2 * ntr;
+    constexpr int kQ = 7 + 2 * MC_GPU_MAXTR;
+    const int nq = 6 + 2 * ntr + ndust;
     double acc[kQ];
 #pragma unroll
     for (int q = 0; q < kQ; ++q) acc[q] = 0.0;
@@ -1647,7 +1652,11 @@ __global__ __launch_bounds__(kBlock) void k_mom_fold(DevState s, const double *_
     double *r = mraw + (size_t)il * ntr * 2;
 #pragma unroll
     for (int q = 6; q < kQ; ++q)
-        if (q < nq) r[q - 6] += acc[q];
+        if (q < 6 + 2 * ntr) r[q - 6] += acc[q];
+    // grain absorption (the coefficient after the metal ones)
+#pragma unroll
+    for (int q = 6; q < kQ; ++q)
+        if (ndust && q == nq - 1) h[6] += acc[q];
 }
 
 // leaf half width (car: constant)
@@ -2856,6 +2865,7 @@ KArgs make_args(mc_ctx *c)
     a.ir_wave = nullptr;
     a.mom = nullptr;
     a.mom_rows = 0;
+    a.mom_dust = 0;
     a.mom_tab = nullptr;
     a.mom_bseg = nullptr;
     a.mom_ns = 0;
@@ -2927,11 +2937,12 @@ int mc_run_pass(mc_ctx *c)
     const bool metals = c->ntr > 0;
     // energy-moment tallies in the mixed-form metal walk
     int use_mom = 0;
-    if (metals && c->p.precision == MC_MIXED) {
+    if (c->p.precision == MC_MIXED) {
         use_mom = mc_mom_ready(c);
         if (use_mom < 0) return -100;
     }
     if (use_mom) {
+        a.mom_dust = c->mom.ndust;
         a.mom = c->d_mom;
         a.mom_rows = (int64_t)nl + (int64_t)c->st.nrep * c->st.nhot;
         a.mom_tab = c->d_mom_tab;
@@ -3051,8 +3062,8 @@ int mc_run_pass(mc_ctx *c)
             MC_HIP(hipGetLastError());
         }
         k_mom_fold<<<(c->nleaf + kBlock - 1) / kBlock, kBlock, 0, st>>>(
-            c->st, c->d_mom, mrows, c->d_mom_coef, c->d_mom_band, c->mom.nsub, c->ntr, c->nnu,
-            c->d_mraw, c->nleaf);
+            c->st, c->d_mom, mrows, c->d_mom_coef, c->d_mom_band, c->mom.nsub, c->ntr,
+            c->mom.ndust, c->nnu, c->d_mraw, c->nleaf);
         MC_HIP(hipGetLastError());
     }
     double led[3];

@@ -20,6 +20,7 @@
 // Ar band), which costs only moment memory; degree 2 does not reach 1e-7
 // within 512 sub-bins per band bin.  The FP64 form keeps the reference's
 // per-coefficient sums.
+#include <cfloat>
 #include <cstdio>
 #include <algorithm>
 #include <cmath>
@@ -37,6 +38,34 @@ constexpr double kMomTol = 1.0e-7;       // relative to the sub-bin's max |c|
 // the walk's coefficient q of a packet of energy E, in its own arithmetic:
 // q = 0..5 H/He (coef rows), q = 6 + 2k + {0,1} metal transition k in
 // threshold order (metal_coeff)
+// grain absorption cross section per H at E, in the launch's arithmetic
+// (ion_dust_*_at, src/gas_opacity_mod.f90:217-290: linear in lambda for the
+// albedo, log-log for C_ext, between the sorted table rows)
+double dust_sabs(const mc_ctx *c, double E)
+{
+    const int n = c->dn;
+    const double *lam = c->dlam.data(), *alb = c->dalb.data(), *ce = c->dcext.data();
+    const double x0 = 1.23984 / E;
+    double al, cx;
+    if (x0 <= lam[0]) {
+        al = alb[0]; cx = ce[0];
+    } else if (x0 >= lam[n - 1]) {
+        al = alb[n - 1]; cx = ce[n - 1];
+    } else {
+        int lo = 0, hi = n - 1;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (lam[mid] <= x0) lo = mid; else hi = mid;
+        }
+        const double w = (x0 - lam[lo]) / (lam[lo + 1] - lam[lo]);
+        al = alb[lo] * (1.0 - w) + alb[lo + 1] * w;
+        const double wl = std::log(x0 / lam[lo]) / std::log(lam[lo + 1] / lam[lo]);
+        cx = std::exp(std::log(std::fmax(ce[lo], DBL_MIN)) * (1.0 - wl) +
+                      std::log(std::fmax(ce[lo + 1], DBL_MIN)) * wl);
+    }
+    return (1.0 - al) * cx / c->dcref;
+}
+
 void coeffs(const mc_ctx *c, double E, double *out)
 {
     const double he = E * MC_EV2ERG;
@@ -60,6 +89,7 @@ void coeffs(const mc_ctx *c, double E, double *out)
         out[6 + 2 * k] = c1;
         out[7 + 2 * k] = c2;
     }
+    if (c->mom.ndust) out[6 + 2 * c->ntr] = dust_sabs(c, E);
 }
 
 // Chebyshev T_0..T_deg at t
@@ -118,14 +148,19 @@ int mc_mom_build(mc_ctx *c)
     m.nsub = 0;
     m.fit_err = 0.0;
     m.built_ntr = c->ntr;
+    m.ndust = (c->p.ion_add_dust && c->dn >= 2) ? 1 : 0;
+    m.built_dust = m.ndust;
     m.seg.clear();
     m.bseg.assign((size_t)c->nnu * 2, 0.0);
     m.band.clear();
     m.coef.clear();
-    const int nq = 6 + 2 * c->ntr;
+    const int nq = 6 + 2 * c->ntr + m.ndust;
     m.nq = nq;
     std::vector<double> thr = {MC_ETH_HI, MC_ETH_HEI, MC_ETH_HEII};
     for (int k = 0; k < c->ntr; ++k) thr.push_back(c->mtab.flat_eth[k]);
+    // grain absorption is smooth in ln E between the rows of its table only
+    if (m.ndust)
+        for (double l : c->dlam) thr.push_back(1.23984 / l);
     std::sort(thr.begin(), thr.end());
     std::vector<double> a, coef;
     auto fail = [&]() {
@@ -178,6 +213,7 @@ void mc_mom_free(mc_ctx *c)
     c->d_mom_band = nullptr;
     c->mom_cap = 0;
     c->mom.built_ntr = -1;
+    c->mom.built_dust = -1;
     c->mom.nsub = 0;
 }
 
@@ -186,8 +222,9 @@ void mc_mom_free(mc_ctx *c)
 // it keeps the per-coefficient sums, < 0 on a device error.
 int mc_mom_ready(mc_ctx *c)
 {
-    if (c->ntr <= 0 || !c->mom_on) return 0;
-    if (c->mom.built_ntr != c->ntr) {
+    if (!c->mom_on) return 0;
+    const int ndust = (c->p.ion_add_dust && c->dn >= 2) ? 1 : 0;
+    if (c->mom.built_ntr != c->ntr || c->mom.built_dust != ndust) {
         mc_mom_free(c);
         if (mc_mom_build(c)) return -1;
     }

@@ -206,3 +206,27 @@ def test_dusty_nebula_matches_oracle(prec):
     assert abs(led[1] + led[2] - led[0]) <= 5e-13 * led[0]
     ok, err = rel_close(gm.dust("heat_dust"), orc.dust_heat(), tol, floor_frac=floor)
     assert ok, err
+
+
+def test_moment_tallies_metal_free_walk_with_grains():
+    """The mixed-form walk without metals (the dusty nebula, scattering on)
+    takes the energy-moment tallies too, with grain absorption as one more
+    interpolated coefficient: every rate sum, band J and the dust
+    absorption agree with the per-coefficient sums of the same walk."""
+    params, nH, rhokap = dusty_nebula(n=16, nphotons=40000, precision=MC_MIXED)
+    tab = synthetic_kext()
+    on = gpu_model(params, nH, rhokap=rhokap, dust_table=tab)
+    off = gpu_model(params, nH, rhokap=rhokap, dust_table=tab)
+    off.set_tally_moments(False)
+    for m in (on, off):
+        m.ionizing_field_and_rates()
+    nsub, err = on.tally_moment_info()
+    print("metal-free sub-bins", nsub, "interpolation error", err)
+    assert nsub > 16 and err <= 1e-7
+    ta, tb = on.tallies(), off.tallies()
+    assert (ta["seg_direct"], ta["seg_scattered"]) == (tb["seg_direct"], tb["seg_scattered"])
+    for k in ("hhe", "jt", "dust_abs"):
+        ok, e = rel_close(ta[k], tb[k], 1e-6, floor_frac=1e-9)
+        assert ok, (k, e)
+    ok, e = rel_close(on.dust("heat_dust"), off.dust("heat_dust"), 1e-6, floor_frac=1e-9)
+    assert ok, e

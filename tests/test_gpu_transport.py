@@ -138,24 +138,35 @@ def test_dust_scattering_pass_matches_oracle():
     np.testing.assert_allclose(gm.energy_ledger(), to["ledger"], rtol=1e-12)
 
 
-@pytest.mark.parametrize("prec", [MC_FP64, MC_MIXED])
-def test_walk_order_does_not_change_tallies(prec):
+@pytest.mark.parametrize("prec,mom", [(MC_FP64, False), (MC_MIXED, False), (MC_MIXED, True)])
+def test_walk_order_does_not_change_tallies(prec, mom):
     """Direction-sorted walk with wave-summed tallies against launch order:
-    the same sums up to FP64 summation order."""
+    the same sums up to FP64 summation order.  With the energy-moment
+    tallies the two orders also route different segments through the
+    moments (a wave's leaf group that spans sub-bins keeps per-coefficient
+    sums), so they differ by the interpolation error: 1e-7 of the sub-bin's
+    largest coefficient per segment, a few 1e-7 of a leaf's own sum where
+    packets just above a threshold dominate it."""
     from mochii_amd.lib import MC_ORDER_LAUNCH
     n = 16
     params, nH = stromgren(n=n, nphotons=50000, precision=prec)
     a, b = gpu_model(params, nH), gpu_model(params, nH)
     xHI = _partial_state(n)
-    a.set_state(xHI=xHI)
-    b.set_state(xHI=xHI)
+    for m in (a, b):
+        m.set_state(xHI=xHI)
+        if prec == MC_MIXED:
+            m.set_tally_moments(mom)
     b.set_packet_order(MC_ORDER_LAUNCH)
     a.ionizing_field_and_rates()
     b.ionizing_field_and_rates()
+    if mom:
+        assert a.tally_moment_info()[0] > 0
     ta, tb = a.tallies(), b.tallies()
+    tol = 2e-6 if mom else 1e-12
     for k in ("hhe", "jt"):
-        ok, err = rel_close(ta[k], tb[k], 1e-12)
-        assert ok, (k, err)
+        for col in range(ta[k].shape[1]):
+            ok, err = rel_close(ta[k][:, col], tb[k][:, col], tol)
+            assert ok, (k, col, err)
     assert ta["seg_direct"] == tb["seg_direct"]
     np.testing.assert_allclose(a.energy_ledger(), b.energy_ledger(), rtol=1e-13)
 
