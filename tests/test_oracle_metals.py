@@ -11,10 +11,14 @@ gate), the stage fractions against a dense linear solve of the stage
 balance, the metal sums against the fractions, the charge-exchange solve
 against the hydrogen residual it zeroes and its own fixed point
 (tests/charge_exchange/check_cell_convergence.f90), the two-level H-impact
-cooling's low/high density limits, and the aligned band edges with metal
-thresholds against the merge / drop rules.
+cooling's low/high density limits, metal_cooling one element and one
+cooling stage at a time against the data files' fits times that stage's
+fraction, metal_freefree against the Gaunt-factor vectors of the reference's
+Python mirror, and the aligned band edges with metal thresholds against the
+merge / drop rules.
 """
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -278,6 +282,86 @@ def test_cell_solve_reaches_its_fixed_point(two_way):
             loss = nH * xHI * (gam[0] + cH * ne + rout)
             gain = (aH * ne + rin) * nH * (1 - xHI)
             assert abs(loss - gain) <= 1e-6 * loss + 1e-300
+
+
+def _element_model(els, seed=13):
+    """A 4^3 oracle model with the given registry and random metal
+    photoionisation rates; cooling_model 'low_density' (no n-level tables)."""
+    params, nH = hii_thermal(n=4, nphotons=1000)
+    op = oracle_params(params)
+    m = po.Model({k: v for k, v in op.items() if k not in ("xHI_init", "xHeI_init", "xHeII_init")},
+                 elements=els)
+    m.params.update(op)
+    m.state_init(nH)
+    m.cooling_setup(np.zeros(0), np.zeros(0))
+    rng = np.random.default_rng(seed)
+    for ie, el in enumerate(els):
+        nt = el["nstage"] - 1
+        g = 10.0 ** rng.uniform(-11, -8, size=(m.nleaf, nt))
+        m.set_species_rates(ie, g, np.zeros((m.nleaf, nt)))
+    return m
+
+
+def _lam_fit(fit, T):
+    """cooling_eval (src/cooling_mod.f90:125-134): T^-1/2 sum A_i exp(-T_i/T)."""
+    return sum(a * np.exp(-t / T) for a, t in fit) / np.sqrt(T)
+
+
+@pytest.mark.parametrize("name", sorted(DATA))
+def test_metal_cooling_one_stage_in_isolation(name):
+    """metal_cooling (src/species_mod.f90:825-846) with one element of the
+    registry and its cooling fit on one stage only: n_e n_el f_s
+    Lambda_fit(T) with f_s that stage's fraction (pinned by the dense solve
+    above) and Lambda_fit the data file's fit; every draw also checks that
+    the neighbouring stage's fraction would give another value, so a stage
+    index, abundance or n_e factor slip fails."""
+    L = po.lib()
+    rng = np.random.default_rng(len(name))
+    for st, fit in sorted(DATA[name]["cooling"].items()):
+        el = dict(DATA[name], abund=3.0e-5, cooling={st: fit})
+        m = _element_model([el])
+        nH = m.arr("nH")
+        discriminating = 0
+        for _ in range(6):
+            il = int(rng.integers(m.nleaf))
+            T, ne, x = 10 ** rng.uniform(3.4, 4.4), 10 ** rng.uniform(0, 2), rng.uniform(0.05, 0.95)
+            f = m.species_fractions_at(0, il, T, ne, x)
+            want = ne * (3.0e-5 * nH[il]) * f[st - 1] * _lam_fit(fit, T)
+            got = L.orc_metal_cooling(m.c, il, T, nH[il], ne, nH[il] * x, nH[il] * (1 - x))
+            assert got == pytest.approx(want, rel=1e-13, abs=1e-300), (name, st, T, ne)
+            other = [f[j] for j in (st - 2, st) if 0 <= j < len(f)]
+            discriminating += all(abs(o - f[st - 1]) > 1e-3 * f[st - 1] for o in other)
+        assert discriminating >= 3, (name, st)
+
+
+def test_metal_freefree_against_gaunt_vectors():
+    """metal_freefree (src/species_mod.f90:852-868) for every registry
+    element at the temperatures of the reference's Gaunt-factor vectors
+    (tests/golden/rates/cooling_rates.txt, columns Z = 1..4): 1.42554e-27
+    T^1/2 n_e n_H sum over elements of abund sum_{j>=2} f_j (j-1)^2
+    gbar_ff(T, j-1), with the stage fractions of the oracle."""
+    tab = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "rates",
+                                  "cooling_rates.txt"))
+    rows = tab[(tab[:, 0] >= 2e3) & (tab[:, 0] <= 5e4)]
+    L = po.lib()
+    els = registry({k: 0.0 for k in DATA} | {n: 1e-5 * (1 + i) for i, n in enumerate(sorted(DATA))},
+                   data=DATA)
+    m = _element_model(els)
+    nH = m.arr("nH")
+    rng = np.random.default_rng(17)
+    for row in rows:
+        T = row[0]
+        il = int(rng.integers(m.nleaf))
+        ne, x = 10 ** rng.uniform(0, 2), rng.uniform(0.05, 0.95)
+        s = 0.0
+        for ie, el in enumerate(els):
+            f = m.species_fractions_at(ie, il, T, ne, x)
+            for j in range(2, el["nstage"] + 1):
+                s += el["abund"] * f[j - 1] * (j - 1) ** 2 * row[6 + (j - 1)]
+        want = 1.42554e-27 * np.sqrt(T) * ne * nH[il] * s
+        got = L.orc_metal_freefree(m.c, il, T, nH[il], ne, nH[il] * x, nH[il] * (1 - x))
+        # the Gaunt nodes agree with the vectors to a few 1e-8 (single-precision fit)
+        assert got == pytest.approx(want, rel=2e-7), T
 
 
 def test_h_impact_cooling_two_level_limits():
