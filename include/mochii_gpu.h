@@ -232,6 +232,22 @@ const char *mc_last_error(void);
 int mc_init(const mc_params *p, mc_ctx **ctx);
 int mc_finalize(mc_ctx *ctx);
 
+/* Leaf list of an AMR input file, generic_amr_read
+ * (src/read_generic_amr.f90:34-237) with the file-name resolution of
+ * io_resolve_filename (src/iofile_mod.f90:138-170): the text form ("nleaf
+ * boxlen" then rows "x y z level nH [...]") or a FITS binary table (columns
+ * by name x, y, z, level, nH|dens|gasDen with the reference's index
+ * fallback; keywords BOXLEN, ORIGINX/Y/Z), gzip-compressed or not; HDF5 is
+ * refused.  Coordinates come back recentred on the box centre as
+ * grid_create_amr does (src/grid_mod_amr.f90:51-61), *half = BOXLEN / 2,
+ * ready for mc_grid_octree.  With x null or maxn < the leaf count the call
+ * only reports *nleaf, *half and has_opt; otherwise it fills the arrays.
+ * has_opt[0..2] (may be null): the optional metallicity / xHI / ndust
+ * columns are present (their arrays may be null).  Host only, no device. */
+int mc_leaf_list_read(const char *path, int64_t maxn, int64_t *nleaf, double *half,
+                      double *x, double *y, double *z, int32_t *level, double *nH,
+                      double *metallicity, double *xHI, double *ndust, int32_t *has_opt);
+
 /* Octree from a leaf list: amr_build_tree + amr_build_neighbors
  * (src/octree_mod.f90:347-497) as called from grid_create_amr
  * (src/grid_mod_amr.f90:216-219).  Leaf coordinates are box-centred. */
