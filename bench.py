@@ -76,7 +76,8 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", default=None)
     ap.add_argument("--kernel", default="k_walk<1, 1, 0>")
-    ap.add_argument("--per-pass", type=int, default=250)
+    ap.add_argument("--per-pass", type=int, default=0,
+                    help="dispatches of the kernel in one pass (0: all of them)")
     ap.add_argument("--skip-last", type=int, default=0)
     return ap.parse_args()
 
@@ -137,7 +138,7 @@ def pmc_summary(path, kernel, per_pass):
     expect = COPY_BYTES / 1024.0
     scale = expect / float(np.mean(cp))
     v = [float(r["Counter_Value"]) for r in rows if kernel in r["Kernel_Name"]]
-    last = v[-per_pass:]
+    last = v[-per_pass:] if per_pass > 0 else v
     total = sum(last) * 1024.0 * scale
     print(f"counter {ctr}: calibration copy of {COPY_BYTES} B reads {np.round(cp, 1)} KiB "
           f"(expected {expect:.0f}), scale {scale:.4f}")
@@ -263,8 +264,22 @@ def timed_steps(name, params, nH, octree, rhokap, tab, args, mlp):
                segments_scattered=int(np.mean([s[1] for s in segs])),
                max_energy_closure=float(np.max(clos)),
                net_rate_evals_per_bracketed_leaf=float(np.sum(evals) / max(np.sum(brk), 1)),
-               setup_s=t_setup)
+               setup_s=t_setup, moment_sub_bins=int(m.tally_moment_info()[0]))
     return m, out
+
+
+def scattered_rmw_probe(rows, width, n_ops):
+    """The device's scattered FP64 read-modify-write rate (mc_probe_scattered_rmw):
+    n_ops updates of `width` doubles at hashed rows of a rows x width array."""
+    import ctypes as C
+    from mochii_amd.lib import check, library
+    ms = C.c_double()
+    check(library().mc_probe_scattered_rmw(int(rows), int(width), int(n_ops), C.byref(ms), None),
+          "mc_probe_scattered_rmw")
+    t = ms.value * 1e-3
+    return dict(rows=int(rows), width=int(width), n_ops=int(n_ops), ms=ms.value,
+                rows_per_s=n_ops / t, atomics_per_s=n_ops * width / t,
+                bytes_per_s=n_ops * width * 8 / t)
 
 
 def ir_band(m, name, packets):
@@ -486,6 +501,13 @@ def main():
         m.close()
     head = forms.get("mixed_mlp") or forms[list(forms)[0]]
     f64 = forms.get("fp64")
+    # the walk's tally pattern on the device without the walk: one 4-double
+    # moment row per update over the model's moment array (leaves x
+    # sub-bins), and single doubles over the same bytes
+    nsub = max(head.get("moment_sub_bins", 0), 1)
+    probe = [scattered_rmw_probe(nH.size * nsub, 4, 4e9),
+             scattered_rmw_probe(nH.size * nsub * 4, 1, 8e9)]
+    log(f"[probe] scattered RMW: {probe}")
     conv = None if args.no_converge else converge_leg(mlp, train_info)
     turb = None if args.no_turbulent else turbulent_leg(args, mlp)
     cpu = None if args.no_cpu else cpu_baseline(params, nH, octree, rhokap, tab, state,
@@ -493,9 +515,17 @@ def main():
     nseg = head["segments_direct"] + head["segments_scattered"]
     bps = bytes_per_segment("mixed", True, True, 0)
     achieved = nseg * bps / (head["walk_ms"] * 1e-3) / 1e9
-    fetch = profile_value("k_walk_multifractal_1e9_traffic.md", "fetch_size_bytes_per_pass")
-    write = profile_value("k_walk_multifractal_1e9_traffic.md", "write_size_bytes_per_pass")
-    traffic = (fetch + write) if fetch is not None and write is not None else None
+    # HBM bytes per launch of the walk from the committed counter runs of
+    # the same workload (one pass = dispatches_per_pass launches)
+    prof = "k_walk_multifractal_1e9_traffic.md"
+    fetch = profile_value(prof, "fetch_size_bytes_per_pass")
+    write = profile_value(prof, "write_size_bytes_per_pass")
+    ndisp = profile_value(prof, "dispatches_per_pass")
+    pseg = profile_value(prof, "segments_per_pass")
+    traffic = traffic_seg = None
+    if None not in (fetch, write, ndisp, pseg):
+        traffic = (fetch + write) / ndisp
+        traffic_seg = (fetch + write) / pseg
     large_conv = {k: profile_value("converge_multifractal.md", k) for k in
                   ("time_to_converged_fp64_s", "time_to_converged_mixed_mlp_s",
                    "speedup_vs_fp64")}
@@ -536,7 +566,16 @@ def main():
                      "kernel": "k_walk<MC_MIXED, kFaceExit, 0>",
                      "bytes_per_segment": bps, "segments_per_pass": nseg,
                      "walk_ms_per_pass": head["walk_ms"],
-                     "traffic_basis": "HBM bytes per pass of the walk, rocprofv3 FETCH_SIZE + "
+                     "traffic_bytes_per_segment": traffic_seg,
+                     "scattered_rmw": {
+                         "probe_rows_per_s": probe[0]["rows_per_s"],
+                         "probe_atomics_per_s_single": probe[1]["atomics_per_s"],
+                         "walk_segments_per_s": nseg / (head["walk_ms"] * 1e-3),
+                         "walk_frac_of_probe": nseg / (head["walk_ms"] * 1e-3) /
+                                               probe[0]["rows_per_s"],
+                         "probes": probe},
+                     "traffic_basis": "HBM bytes per walk launch (one pass = "
+                                      f"{ndisp and int(ndisp)} launches), rocprofv3 FETCH_SIZE + "
                                       "WRITE_SIZE scaled by a 1 GiB copy "
                                       "(profiles/k_walk_multifractal_1e9_traffic.md)"},
         "cpu_baseline": cpu,

@@ -232,6 +232,17 @@ const char *mc_last_error(void);
 int mc_init(const mc_params *p, mc_ctx **ctx);
 int mc_finalize(mc_ctx *ctx);
 
+/* Measured ceiling of the walk's tally traffic: the device rate of
+ * scattered FP64 read-modify-writes.  n_ops lanes each add 1.0 to the
+ * `width` consecutive doubles (1..16) of a row chosen by a splitmix64 hash
+ * of the lane's index among `rows` rows (a rows x width array allocated,
+ * zeroed and freed by the call on the current device); one untimed run, then
+ * *ms = the timed run's kernel time (HIP events).  checksum (may be null:
+ * skips the host read-back) receives the array's sum, 2 n_ops width exactly.
+ * No counterpart in the reference. */
+int mc_probe_scattered_rmw(int64_t rows, int width, int64_t n_ops, double *ms,
+                           double *checksum);
+
 /* Leaf list of an AMR input file, generic_amr_read
  * (src/read_generic_amr.f90:34-237) with the file-name resolution of
  * io_resolve_filename (src/iofile_mod.f90:138-170): the text form ("nleaf

@@ -13,12 +13,18 @@ program drive_main
   use ion_balance_mod,    only : gas_equilibrium_update, hydrogen_balance_residual_grid
   use thermal_mod,        only : gas_thermal_update
   use gas_opacity_mod,    only : gas_opacity_setup, gas_opacity_fill
-  use mochii_dropin,      only : mochii_bind_state, mochii_bind_rates, mochii_bind_cooling
+  use mochii_dropin,      only : mochii_bind_state, mochii_bind_rates, mochii_bind_cooling, &
+                                 mochii_bind_band_tally, mochii_ctx
+  use mochii_gpu,         only : mc_info
+  use, intrinsic :: iso_c_binding, only : c_int32_t, c_double, c_null_ptr
   implicit none
 
   real(kind=wp), allocatable, target :: nH(:), xHI(:), xHeI(:), xHeII(:), ne(:), Te(:)
   real(kind=wp), allocatable, target :: gHI(:), gHeI(:), gHeII(:), hHI(:), hHeI(:), hHeII(:)
   real(kind=wp), allocatable :: fitA(:), fitT(:)
+  real(kind=wp), allocatable, target :: jt_ion(:, :)
+  integer(c_int32_t) :: nl32, nnu_band
+  real(c_double) :: ltot
   real(kind=wp) :: max_dx, max_dte, dx_vol, dte_vol, eps_hmax, eps_hvol
   logical :: converged, use_vol
   integer :: iter, niter, nleaf, il
@@ -46,6 +52,10 @@ program drive_main
   call gas_opacity_setup()
   niter = max(par%gas_niter, 1)
   call ionizing_field_setup()
+  ! jtally_mod's band tally, sized by the band ion_setup built
+  if (mc_info(mochii_ctx, nl32, nnu_band, ltot, c_null_ptr) /= 0) stop 3
+  allocate(jt_ion(nnu_band, nleaf))
+  call mochii_bind_band_tally(jt_ion)
   converged = .false.
   max_dx = 0.0_wp;  max_dte = 0.0_wp;  dx_vol = 0.0_wp;  dte_vol = 0.0_wp
   do iter = 1, niter
@@ -80,6 +90,7 @@ program drive_main
   write(*, '(a,6es24.16)') 'sum_rates', sum(gHI), sum(gHeI), sum(gHeII), sum(hHI), &
      sum(hHeI), sum(hHeII)
   write(*, '(a,5es24.16)') 'sum_state', sum(xHI), sum(xHeI), sum(xHeII), sum(ne), sum(Te)
+  write(*, '(a,es24.16)') 'sum_jt', sum(jt_ion)
 
 contains
 

@@ -11,7 +11,8 @@
 ! gas_state_setup and before ionizing_field_setup:
 !     call mochii_bind_state(gas_nH, gas_xHI, gas_xHeI, gas_xHeII, gas_ne, gas_Te)
 !     call mochii_bind_rates(gamma_HI, gamma_HeI, gamma_HeII, heat_HI, heat_HeI, heat_HeII)
-! (plus mochii_bind_octree, mochii_bind_dust and mochii_bind_registry where
+! (plus mochii_bind_octree, mochii_bind_dust, mochii_bind_registry,
+! mochii_bind_cooling and mochii_bind_band_tally where
 ! they apply).  Every replaced procedure leaves the bound arrays as the
 ! reference's would: the rates after a pass, the gas state after an update.
 module mochii_dropin
@@ -39,6 +40,7 @@ module mochii_dropin
   end type rate_ptr
   type(rate_ptr), save :: rate(6)
   real(wp), pointer, save :: s_heat_dust(:) => null(), s_t_dust(:) => null()
+  real(wp), pointer, save :: s_jt(:, :) => null()
   ! octree leaf list (box-centred leaf centres, levels, root half-width)
   real(c_double), allocatable, save :: o_x(:), o_y(:), o_z(:)
   integer(c_int32_t), allocatable, save :: o_level(:)
@@ -51,7 +53,7 @@ module mochii_dropin
   real(c_double), allocatable, save :: c_A(:), c_T(:)
 
   public :: mochii_bind_state, mochii_bind_rates, mochii_bind_octree, mochii_bind_dust
-  public :: mochii_bind_registry, mochii_bind_cooling
+  public :: mochii_bind_registry, mochii_bind_cooling, mochii_bind_band_tally
   public :: mochii_params_from_par, mochii_open, mochii_check
   public :: mochii_pull_state, mochii_pull_rates, mochii_pull_dust, mochii_push_state
 
@@ -72,6 +74,13 @@ contains
     rate(1)%a => gHI;  rate(2)%a => gHeI;  rate(3)%a => gHeII
     rate(4)%a => hHI;  rate(5)%a => hHeI;  rate(6)%a => hHeII
   end subroutine mochii_bind_rates
+
+  ! jtally_mod's band tally jt_ion(nnu_band, nleaf) (src/jtally_mod.f90:
+  ! 330-369), which the reference's gas_rates_write turns into J_nu
+  subroutine mochii_bind_band_tally(jt_ion)
+    real(wp), target, intent(inout) :: jt_ion(:, :)
+    s_jt => jt_ion
+  end subroutine mochii_bind_band_tally
 
   ! The leaf list of amr_grid (octree_mod): leaf il's centre relative to the
   ! box centre, its level and the root cell's half-width (the reader's
@@ -277,10 +286,17 @@ contains
     end subroutine get
   end subroutine mochii_pull_state
 
-  ! the pass's rates -> the bound gas_rates_mod arrays
+  ! the pass's rates -> the bound gas_rates_mod arrays (and the band tally
+  ! -> the bound jt_ion)
   subroutine mochii_pull_rates()
-    real(c_double), allocatable :: r(:)
+    real(c_double), allocatable, target :: r(:), jt(:, :)
     integer :: n, k
+    if (associated(s_jt)) then
+       allocate(jt(size(s_jt, 1), size(s_jt, 2)))
+       call mochii_check(mc_tallies_get(mochii_ctx, c_null_ptr, c_loc(jt), c_null_ptr, &
+                                        c_null_ptr), 'mc_tallies_get')
+       s_jt = real(jt, wp)
+    end if
     if (.not. associated(rate(1)%a)) return
     n = size(s_nH)
     allocate(r(6 * n))

@@ -152,6 +152,7 @@ def library():
         "mc_timing": ([vp, dp], C.c_int),
         "mc_trace_ray": ([vp, C.c_int, dp, dp, ip, C.c_int, ip], C.c_int),
         "mc_trace_ray_noalloc": ([vp, C.c_int, dp, dp, ip, C.c_int, ip], C.c_int),
+        "mc_probe_scattered_rmw": ([C.c_int64, C.c_int, C.c_int64, dp, dp], C.c_int),
         "mc_leaf_list_read": ([C.c_char_p, C.c_int64, i64p, dp, dp, dp, dp, ip, dp, dp, dp, dp,
                                ip], C.c_int),
         "mc_species_setup": ([vp, C.c_int, vp], C.c_int),

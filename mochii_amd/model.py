@@ -51,6 +51,7 @@ class Model:
         check(self._L.mc_grid_octree(self._h, lev.size, _dp(x), _dp(y), _dp(z),
                                      lev.ctypes.data_as(C.POINTER(C.c_int32)), float(half)),
               "mc_grid_octree")
+        self.leaves = (x, y, z, lev, float(half))
         self._refresh_info()
 
     def gas_state_setup(self, nH, xHI=None):
@@ -103,6 +104,7 @@ class Model:
     def dust_temp_setup(self):
         """dust_temp_setup: femit(T) of the loaded extinction table."""
         check(self._L.mc_dust_temp_setup(self._h), "mc_dust_temp_setup")
+        self.dust_temp_ready = True
 
     def dust_temp_compute(self):
         """dust_temp_compute: T_dust of every leaf from the current Heat_dust."""

@@ -79,3 +79,44 @@ def rel_close(a, b, rtol, floor_frac=1e-6):
         return np.allclose(a, b), 0.0
     err = np.abs(a[big] - b[big]) / np.abs(b[big])
     return bool(err.max() <= rtol), float(err.max())
+
+
+def read_fits_images(path):
+    """Every HDU of a FITS file of images: [(header dict, numpy array)],
+    header values parsed (T/F, ints, floats, quoted strings), the array in C
+    order (NAXISn reversed).  Written here independently of the package."""
+    import gzip as _gz
+    b = _gz.open(path).read() if str(path).endswith(".gz") else open(path, "rb").read()
+    out, p = [], 0
+    while p < len(b):
+        h, order = {}, []
+        while True:
+            c = b[p:p + 80].decode("ascii")
+            p += 80
+            if c.startswith("END"):
+                break
+            if c[8:10] != "= ":
+                continue
+            k, v = c[:8].strip(), c[10:]
+            if v.strip().startswith("'"):
+                val = v.strip()[1:].split("'")[0].rstrip()
+            else:
+                t = v.split("/")[0].strip()
+                if t in ("T", "F"):
+                    val = t == "T"
+                else:
+                    try:
+                        val = int(t)
+                    except ValueError:
+                        val = float(t)
+            h[k] = val
+            order.append(k)
+        p = -(-p // 2880) * 2880
+        shape = tuple(h[f"NAXIS{i}"] for i in range(h["NAXIS"], 0, -1))
+        n = int(np.prod(shape)) if shape else 0
+        assert h["BITPIX"] == -64
+        a = np.frombuffer(b, dtype=">f8", count=n, offset=p).reshape(shape).astype(np.float64)
+        p += -(-(8 * n) // 2880) * 2880
+        h["_order"] = order
+        out.append((h, a))
+    return out

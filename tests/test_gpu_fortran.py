@@ -124,3 +124,5 @@ def test_dropin_modules_under_reference_names_match_python_driver(tmp_path):
     s = g.state()
     np.testing.assert_allclose(f_state, [s[k].sum() for k in ("xHI", "xHeI", "xHeII", "ne", "Te")],
                                rtol=1e-6)
+    f_jt = float(next(l for l in lines if l.startswith("sum_jt")).split()[1])
+    assert f_jt > 0 and abs(f_jt / g.tallies()["jt"].sum() - 1.0) < 1e-6
