@@ -1683,24 +1683,28 @@ __global__ __launch_bounds__(kBlock) void k_rates(DevGrid g, DevState s, double 
 __device__ __forceinline__ double rr_badnell(double T, double A, double B, double T0, double T1, double Cc,
                                     double T2)
 {
+#pragma clang fp contract(off)
     const double bp = B + Cc * exp(-T2 / T), tt = sqrt(T / T0);
     return A / (tt * pow(1.0 + tt, 1.0 - bp) * pow(1.0 + sqrt(T / T1), 1.0 + bp));
 }
 __device__ __forceinline__ double rr_mao(double T, double a0, double b0, double c0, double a1, double b1,
                                 double a2, double b2)
 {
+#pragma clang fp contract(off)
     const double te = T * (MC_KBOLTZ / MC_EV2ERG);
     return a0 * 1.0e-10 * pow(te, -b0 - c0 * log(te)) * (1.0 + a2 * pow(te, -b2)) /
            (1.0 + a1 * pow(te, -b1));
 }
 __device__ __forceinline__ double voronov(double T, double dE, double P, double A, double X, double K)
 {
+#pragma clang fp contract(off)
     const double U = dE * MC_EV2ERG / (MC_KBOLTZ * T);
     return A * (1.0 + P * sqrt(U)) * pow(U, K) * exp(-U) / (X + U);
 }
 
 __device__ RateSet rates_at(double T, bool caseA)
 {
+#pragma clang fp contract(off)
     RateSet r;
     const double aA_H = rr_badnell(T, 8.318e-11, 0.7472, 2.965, 7.001e5, 0.0, 0.0);
     const double aA_He2 = rr_badnell(T, 5.235e-11, 0.6988, 7.301, 4.475e6, 0.0829, 1.682e5) +
@@ -1727,6 +1731,7 @@ __device__ RateSet rates_at(double T, bool caseA)
 __device__ double solve_cell(const RateSet &r, double gH, double gHe1, double gHe2, double nH,
                              double yHe, double &xHI, double &xHeI, double &xHeII)
 {
+#pragma clang fp contract(off)
     double xHeIII = fmax(0.0, 1.0 - xHeI - xHeII);
     double ne = fmax(nH * ((1.0 - xHI) + yHe * (xHeII + 2.0 * xHeIII)), 1.0e-12 * nH);
     for (int it = 0; it < 200; ++it) {
@@ -1974,6 +1979,7 @@ __global__ __launch_bounds__(kBlock) void k_equilibrium(DevGrid g, DevState s, d
                                                         double yHe, double w, double d2cm,
                                                         MetalSw ms)
 {
+#pragma clang fp contract(off)
     const int il = blockIdx.x * kBlock + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
     bool ismax[3] = {true, false, false};
@@ -2020,6 +2026,7 @@ __global__ __launch_bounds__(kBlock) void k_equilibrium(DevGrid g, DevState s, d
 // ---- thermal balance -------------------------------------------------------
 __device__ __forceinline__ double gbar_ff(const DevCool &cl, double T, int Z)
 {
+#pragma clang fp contract(off)
     const int iz = min(max(Z, 1), 4) - 1;
     const double lt = fmin(fmax(log10(T), cl.logT0), cl.logT0 + 40.0 * cl.dlogT);
     const int i = min((int)((lt - cl.logT0) / cl.dlogT), 39);
@@ -2031,6 +2038,7 @@ __device__ __forceinline__ double gbar_ff(const DevCool &cl, double T, int Z)
 // recombination cooling fits (Hui & Gnedin 1997) for H II and He III
 __device__ __forceinline__ double beta_hg(double T, double ttr, double z, bool caseA)
 {
+#pragma clang fp contract(off)
     const double lam = 2.0 * ttr / T;
     return caseA ? z * 1.778e-29 * T * pow(lam, 1.965) / pow(1.0 + pow(lam / 0.541, 0.502), 2.697)
                  : z * 3.435e-30 * T * pow(lam, 1.970) / pow(1.0 + pow(lam / 2.250, 0.376), 3.720);
@@ -2042,6 +2050,7 @@ __device__ double cooling_total(const DevCool &cl, const RateSet &r, double T, d
                                 double ne, double xHI, double xHeI, double xHeII, double y,
                                 bool caseA)
 {
+#pragma clang fp contract(off)
     const double xHeIII = fmax(0.0, 1.0 - xHeI - xHeII);
     const double nHI = nH * xHI, nHII = nH * (1.0 - xHI);
     const double nHeI = nH * y * xHeI, nHeII = nH * y * xHeII, nHeIII = nH * y * xHeIII;
@@ -2067,6 +2076,7 @@ struct CellIn { double nH, gH, gHe1, gHe2, hH, hHe1, hHe2; };
 __device__ double net_rate(const DevCool &cl, const CellIn &c, double T, bool caseA, double y,
                            double &xHI, double &xHeI, double &xHeII, double &ne)
 {
+#pragma clang fp contract(off)
     const RateSet r = rates_at(T, caseA);
     ne = solve_cell(r, c.gH, c.gHe1, c.gHe2, c.nH, y, xHI, xHeI, xHeII);
     const double heat = c.nH * (xHI * c.hH + y * (xHeI * c.hHe1 + xHeII * c.hHe2));
@@ -2080,6 +2090,7 @@ __device__ double net_rate(const DevCool &cl, const CellIn &c, double T, bool ca
 __device__ double net_rate_lr(const DevCool &cl, const CellIn &c, double T, bool caseA, double y,
                               double &xHI, double &xHeI, double &xHeII, double &ne, double &lr)
 {
+#pragma clang fp contract(off)
     const RateSet r = rates_at(T, caseA);
     ne = solve_cell(r, c.gH, c.gHe1, c.gHe2, c.nH, y, xHI, xHeI, xHeII);
     const double heat = c.nH * (xHI * c.hH + y * (xHeI * c.hHe1 + xHeII * c.hHe2));
@@ -2099,6 +2110,7 @@ __device__ __noinline__ NetOut net_rate_mo(const DevCool &cl, const MetalSw ms, 
                                            const CellIn c, double T, bool caseA, double y,
                                            double xHI, double xHeI, double xHeII)
 {
+#pragma clang fp contract(off)
     double ne;
     const RateSet r = rates_at(T, caseA);
     double lo[MT], up[MT];
@@ -2150,6 +2162,7 @@ __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCo
                                                     double d2cm, const float *pred, MetalSw ms,
                                                     int levels)
 {
+#pragma clang fp contract(off)
     const int il = blockIdx.x * kBlock + threadIdx.x;
     double v[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     bool ismax[9] = {true, true, false, false, false, false, false, false, false};
@@ -2315,6 +2328,7 @@ __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCo
 template <int MT, int LAY>
 __global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, DevState s, bool caseA, MetalSw ms)
 {
+#pragma clang fp contract(off)
     const int il = blockIdx.x * kBlock + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
     bool ismax[3] = {true, false, false};
@@ -2351,6 +2365,7 @@ __global__ __launch_bounds__(kBlock) void k_opacity_fill(DevGrid g, DevState s, 
 __global__ __launch_bounds__(kBlock) void k_metal_rates(DevGrid g, const double *mraw, double *mgam,
                                                         double *mheat, int ntr, double d2cm)
 {
+#pragma clang fp contract(off)
     const int il = blockIdx.x * kBlock + threadIdx.x;
     if (il >= g.nleaf) return;
     const double w = 2.0 * leaf_half(g, il);
@@ -2372,6 +2387,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_refresh(DevGrid g, DevState s,
                                                           double *mfrac, double *mop, float *mopf,
                                                           int nst, double d2cm)
 {
+#pragma clang fp contract(off)
     const int il = blockIdx.x * kBlock + threadIdx.x;
     if (il >= g.nleaf) return;
     const double nH = s.nH[il];

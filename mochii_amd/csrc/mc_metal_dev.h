@@ -14,6 +14,7 @@
 // voronov_ci x ci_dere_ratio (src/species_mod.f90:815-821, 524-525)
 __device__ __forceinline__ double m_ci(const DevMetalTab *mt, int t, double T)
 {
+#pragma clang fp contract(off)
     const double *c = mt->ci[t];
     const double U = c[0] * MC_EV2ERG / (MC_KBOLTZ * T);
     return c[2] * (1.0 + c[1] * sqrt(U)) * pow(U, c[4]) * exp(-U) / (c[3] + U) * mt->ci_ratio[t];
@@ -23,6 +24,7 @@ __device__ __forceinline__ double m_ci(const DevMetalTab *mt, int t, double T)
 // Shull & Van Steenberg DR2 form (src/species_mod.f90:428-447)
 __device__ __forceinline__ double m_alpha_rec(const DevMetalTab *mt, int t, double T, double tm15)
 {
+#pragma clang fp contract(off)
     double a;
     if (mt->has_rr2[t]) {
         a = mt->rr2[t][0] * pow(T / 1.0e4, -mt->rr2[t][1]);
@@ -44,6 +46,7 @@ __device__ __forceinline__ double m_alpha_rec(const DevMetalTab *mt, int t, doub
 // cx_rate forms 1-7 (src/species_mod.f90:452-486)
 __device__ __forceinline__ double m_cx(int form, const double *c, double T)
 {
+#pragma clang fp contract(off)
     double t4 = T / 1.0e4, lnT, k;
     switch (form) {
     case 1:
@@ -79,6 +82,7 @@ struct MCoef { double ci, rec, cxi, cxr; };
 
 __device__ __noinline__ MCoef m_coef(int t, double T, double tm15)
 {
+#pragma clang fp contract(off)
     const DevMetalTab *mt = &c_mt;
     MCoef q;
     q.ci = m_ci(mt, t, T);
@@ -94,6 +98,7 @@ template <int MT>
 __device__ __forceinline__ void m_prepare(const DevMetalTab *mt, const double *gam, double T,
                                           MCache<MT> &cc)
 {
+#pragma clang fp contract(off)
     const int ntr = mt->ntr;
     const double tm15 = pow(T, -1.5);
 #pragma unroll
@@ -117,6 +122,7 @@ template <int MT>
 __device__ __forceinline__ double m_ratio(const MCache<MT> &cc, int t, double ne, double nHI,
                                           double nHII)
 {
+#pragma clang fp contract(off)
     double rion = cc.gam[t] + ne * cc.ci[t];
     if (cc.cxi[t] > 0.0) rion = rion + nHII * cc.cxi[t];
     double rrec = ne * cc.rec[t];
@@ -135,6 +141,7 @@ __device__ __forceinline__ void m_fracs(const DevMetalTab *mt, const MCache<MT> 
                                         double nHI, double nHII, double (&lo)[MT],
                                         double (&up)[MT])
 {
+#pragma clang fp contract(off)
     const int ntr = mt->ntr;
     double s = 1.0, prod = 1.0;
 #pragma unroll
@@ -177,6 +184,7 @@ __device__ __forceinline__ void m_sums(const DevMetalTab *mt, const MCache<MT> &
                                        double ne, double nHI, double nHII, double &nem,
                                        double &rin, double &rout)
 {
+#pragma clang fp contract(off)
     double lo[MT], up[MT];
     m_fracs<MT>(mt, cc, ne, nHI, nHII, lo, up);
     const int ntr = mt->ntr;
@@ -218,6 +226,7 @@ __device__ __forceinline__ void m_sums_fixed(const MCache<MT> &cc, double nH, do
                                              double nHI, double nHII, double &nem, double &rin,
                                              double &rout)
 {
+#pragma clang fp contract(off)
     using L = MLay<LAY>;
     double a = 0.0, b = 0.0, c = 0.0;
 #pragma unroll
@@ -254,6 +263,7 @@ template <int MT, int LAY>
 __device__ __forceinline__ void m_sums_l(const MCache<MT> &cc, double nH, double ne, double nHI,
                                          double nHII, double &nem, double &rin, double &rout)
 {
+#pragma clang fp contract(off)
     if constexpr (LAY == MC_LAYOUT_GENERIC) m_sums<MT>(&c_mt, cc, nH, ne, nHI, nHII, nem, rin, rout);
     else m_sums_fixed<LAY, MT>(cc, nH, ne, nHI, nHII, nem, rin, rout);
 }
@@ -263,6 +273,7 @@ template <int MT, int LAY>
 __device__ double m_hydrogen_neutral_fraction(const MCache<MT> &cc, double nH, double ne,
                                               double A, double B, double xw = -1.0)
 {
+#pragma clang fp contract(off)
     double nem, rin, rout;
     double xa = 0.0, Ra, xb = 1.0, Rb, scale = 0.0;
     bool warm = false;
@@ -327,6 +338,7 @@ __device__ double solve_cell_m(const MetalSw &ms, const MCache<MT> &cc, const Ra
                                double gH, double gHe1, double gHe2, double nH, double yHe,
                                double &xHI, double &xHeI, double &xHeII)
 {
+#pragma clang fp contract(off)
     const bool use = ms.metal_ne || ms.cx_two_way;
     double xHeIII = fmax(0.0, 1.0 - xHeI - xHeII);
     double ne = fmax(nH * ((1.0 - xHI) + yHe * (xHeII + 2.0 * xHeIII)), 1.0e-12 * nH);
@@ -374,6 +386,7 @@ template <int MT, int LAY>
 __device__ double charge_neutrality_m(const MCache<MT> &cc, double nH, double ne_HHe, double xHI,
                                       double ne_seed)
 {
+#pragma clang fp contract(off)
     const double nHI = nH * xHI, nHII = nH * (1.0 - xHI);
     double ne = fmax(ne_seed, 1.0e-12 * nH), nem, rin, rout;
     for (int it = 0; it < 200; ++it) {
@@ -395,6 +408,7 @@ __device__ double charge_neutrality_m(const MCache<MT> &cc, double nH, double ne
 // src/cooling_mod.f90:125-134)
 __device__ __noinline__ double m_fit(int st, double T)
 {
+#pragma clang fp contract(off)
     const DevMetalTab *mt = &c_mt;
     double lam = 0.0;
     const int nf = mt->s_ncool[st];
@@ -406,6 +420,7 @@ __device__ __noinline__ double m_fit(int st, double T)
 // (src/species_mod.f90:713-730)
 __device__ __noinline__ double m_cool_h(int cII, double nion, double nHI, double T)
 {
+#pragma clang fp contract(off)
     if (cII) {
         const double qul = 7.6e-10 * pow(T / 100.0, 0.14);
         const double qlu = 2.0 * qul * exp(-91.25 / T);
@@ -420,6 +435,7 @@ __device__ __noinline__ double m_cool_h(int cII, double nion, double nHI, double
 // (T, n_e) table under cooling_model 'local_ne' (src/species_mod.f90:836-840)
 __device__ __forceinline__ double m_cool_coef(const MetalSw &ms, const NlPos &np, int st, double T)
 {
+#pragma clang fp contract(off)
     const double f = m_fit(st, T);
     const int sl = c_mt.s_nl[st];
     return (ms.nlt && sl >= 0) ? nl_apply(ms.nlt + (long)sl * kNlSlot, np, f) : f;
@@ -432,6 +448,7 @@ __device__ __forceinline__ void m_thermal_terms(const MetalSw &ms, const double 
                                                 const double gff[4], double &mcool,
                                                 double &mcoolH, double &mff, double &mheat)
 {
+#pragma clang fp contract(off)
     const DevMetalTab *mt = &c_mt;
     const double sqT = sqrt(T);
     const int ntr = mt->ntr;

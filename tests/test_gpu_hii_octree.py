@@ -105,8 +105,7 @@ def test_hii_octree_metals_fp64_pass_and_thermal_update_match_oracle():
     so, sg = orc.state(), gm.state()
     rel = np.abs(sg["Te"] / so["Te"] - 1.0)
     print("T_e: max rel", rel.max(), "bit-equal", np.mean(rel == 0.0))
-    assert np.mean(rel == 0.0) >= 0.98
-    assert rel.max() < 1e-3
+    assert np.all(rel == 0.0)             # the solve's arithmetic is the oracle's
     ion = so["xHI"] < 0.5
     assert ion.sum() > 100
     ok, err = rel_close(1.0 - sg["xHI"][ion], 1.0 - so["xHI"][ion], 1e-5)

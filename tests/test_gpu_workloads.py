@@ -66,7 +66,10 @@ def _check_thermal(orc, gm, near):
     so, sg = orc.state(), gm.state()
     rel = np.abs(sg["Te"] / so["Te"] - 1.0)
     print("T_e: max rel", rel.max(), "bit-equal", np.mean(rel == 0.0))
-    assert np.mean(rel == 0.0) >= 0.98 and rel.max() < 1e-3
+    # the cell solve and the balance run without FMA contraction, as the
+    # oracle and the reference compute them: T_e is bit-equal leaf for leaf,
+    # also in faint neutral leaves where the last bit of x_HI decides the sign
+    assert np.all(rel == 0.0)
     ion = so["xHI"] < 0.5
     assert ion.sum() > 50
     ok, err = rel_close(1.0 - sg["xHI"][ion], 1.0 - so["xHI"][ion], 1e-5)
