@@ -279,8 +279,8 @@ extern "C" int mc_init(const mc_params *p, mc_ctx **out)
         mc_set_error("cooling_model: MC_COOL_LOW_DENSITY or MC_COOL_LOCAL_NE");
         return -1;
     }
-    if (p->ci_model != MC_CI_VORONOV) {
-        mc_set_error("ci_model: only MC_CI_VORONOV is built");
+    if (p->ci_model != MC_CI_VORONOV && p->ci_model != MC_CI_DERE_HYBRID) {
+        mc_set_error("ci_model must be MC_CI_VORONOV or MC_CI_DERE_HYBRID");
         return -1;
     }
     if (p->charge_exchange != MC_CX_TWO_WAY && p->charge_exchange != MC_CX_METAL_ONLY) {
@@ -875,6 +875,25 @@ extern "C" int mc_trace_ray(mc_ctx *c, int precision, const double *pos, const d
 }
 
 // ---- trace metals -----------------------------------------------------------
+namespace {
+// ci_dere_ratio (src/recomb_mod.f90:288-311): the Dere (2007) / Voronov
+// ratio of stage `stage` (1 = X0 -> X+) of element Z, 1 where not tabulated
+double dere_ratio(int Z, int stage)
+{
+    struct Row { int Z, n; double r[4]; };
+    static const Row tab[] = {
+        {1, 1, {0.9063}}, {2, 2, {1.0389, 1.0686}}, {6, 3, {1.0499, 0.913, 1.0377}},
+        {7, 2, {1.0421, 1.1966}}, {8, 2, {1.041, 1.1181}}, {10, 2, {0.8089, 1.1395}},
+        {12, 2, {0.3793, 0.9857}}, {14, 4, {0.7328, 0.8798, 0.4492, 0.8221}},
+        {16, 3, {1.3572, 0.8925, 0.8119}}, {17, 4, {0.5412, 0.8428, 0.9237, 0.819}},
+        {18, 3, {0.9242, 0.8644, 0.9752}}, {20, 4, {0.7652, 1.1668, 1.0422, 0.8705}},
+        {26, 3, {0.9904, 1.0568, 1.824}}};
+    for (const Row &row : tab)
+        if (row.Z == Z) return stage >= 1 && stage <= row.n ? row.r[stage - 1] : 1.0;
+    return 1.0;
+}
+}  // namespace
+
 extern "C" int mc_species_setup(mc_ctx *c, int n, const mc_element *els)
 {
     CHECK_CTX(c);
@@ -939,7 +958,7 @@ extern "C" int mc_species_setup(mc_ctx *c, int n, const mc_element *els)
             m.photo_form[t] = e.photo_form[it];
             for (int k = 0; k < 8; ++k) m.photo[t][k] = e.photo[it][k];
             for (int k = 0; k < 5; ++k) m.ci[t][k] = e.ci[it][k];
-            m.ci_ratio[t] = 1.0;                       // ci_model 'voronov'
+            m.ci_ratio[t] = c->p.ci_model == MC_CI_DERE_HYBRID ? dere_ratio(e.Z, it + 1) : 1.0;
             m.has_rr2[t] = e.has_rr2[it];
             for (int k = 0; k < 6; ++k) m.rr[t][k] = e.rr[it][k];
             for (int k = 0; k < 2; ++k) m.rr2[t][k] = e.rr2[it][k];

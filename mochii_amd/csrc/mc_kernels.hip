@@ -1702,7 +1702,7 @@ __device__ __forceinline__ double voronov(double T, double dE, double P, double 
     return A * (1.0 + P * sqrt(U)) * pow(U, K) * exp(-U) / (X + U);
 }
 
-__device__ RateSet rates_at(double T, bool caseA)
+__device__ RateSet rates_at(double T, bool caseA, bool dere)
 {
 #pragma clang fp contract(off)
     RateSet r;
@@ -1724,6 +1724,12 @@ __device__ RateSet rates_at(double T, bool caseA)
     r.cH = voronov(T, 13.6, 0.0, 2.91e-8, 0.232, 0.39);
     r.cHe1 = voronov(T, 24.6, 0.0, 1.75e-8, 0.180, 0.35);
     r.cHe2 = voronov(T, 54.4, 1.0, 2.05e-9, 0.265, 0.25);
+    if (dere) {
+        // ci_dere_ratio(1, 1), (2, 1), (2, 2) (src/recomb_mod.f90:296-297)
+        r.cH = r.cH * 0.9063;
+        r.cHe1 = r.cHe1 * 1.0389;
+        r.cHe2 = r.cHe2 * 1.0686;
+    }
     return r;
 }
 
@@ -1855,7 +1861,7 @@ __global__ __launch_bounds__(kBlock) void k_diffuse_build(DevGrid g, DevState s,
     const double x3 = fmax(0.0, (1.0 - s.xHeI[il]) - s.xHeII[il]);
     const double nHII = nH * (1.0 - s.xHI[il]);
     const double nHeII = (nH * y) * s.xHeII[il], nHeIII = (nH * y) * x3;
-    const RateSet rA = rates_at(T, true), rB = rates_at(T, false);
+    const RateSet rA = rates_at(T, true, false), rB = rates_at(T, false, false);
     const double a1 = rA.aH - rB.aH, a2 = rA.aHe2 - rB.aHe2, a3 = rA.aHe3 - rB.aHe3;
     o[0] = ((((ne * nHII) * a1) * ground_mean_energy(M, 0, T)) * MC_EV2ERG) * vol;
     o[1] = ((((ne * nHeII) * a2) * ground_mean_energy(M, 1, T)) * MC_EV2ERG) * vol;
@@ -1988,7 +1994,7 @@ __global__ __launch_bounds__(kBlock) void k_equilibrium(DevGrid g, DevState s, d
         if (nH <= 0.0) {
             s.ne[il] = 0.0;
         } else {
-            const RateSet r = rates_at(T, caseA);
+            const RateSet r = rates_at(T, caseA, ms.ci_dere != 0);
             const size_t n = g.nleaf;
             double xHI = s.xHI[il], xHeI = s.xHeI[il], xHeII = s.xHeII[il], ne;
             MCache<(MT > 0 ? MT : 1)> cc;
@@ -2073,11 +2079,11 @@ struct CellIn { double nH, gH, gHe1, gHe2, hH, hHe1, hHe2; };
 
 // net heating - cooling at trial T, ionisation state solved at T (warm start
 // from the state passed in)
-__device__ double net_rate(const DevCool &cl, const CellIn &c, double T, bool caseA, double y,
-                           double &xHI, double &xHeI, double &xHeII, double &ne)
+__device__ double net_rate(const DevCool &cl, const CellIn &c, double T, bool caseA, bool dere,
+                           double y, double &xHI, double &xHeI, double &xHeII, double &ne)
 {
 #pragma clang fp contract(off)
-    const RateSet r = rates_at(T, caseA);
+    const RateSet r = rates_at(T, caseA, dere);
     ne = solve_cell(r, c.gH, c.gHe1, c.gHe2, c.nH, y, xHI, xHeI, xHeII);
     const double heat = c.nH * (xHI * c.hH + y * (xHeI * c.hHe1 + xHeII * c.hHe2));
     return heat - cooling_total(cl, r, T, c.nH, ne, xHI, xHeI, xHeII, y, caseA);
@@ -2087,11 +2093,12 @@ __device__ double net_rate(const DevCool &cl, const CellIn &c, double T, bool ca
 // of the warm-started solve.  Near a root where one side varies
 // exponentially with 1/T (collisional line cooling) the log ratio is close
 // to linear in ln T while the difference is not.
-__device__ double net_rate_lr(const DevCool &cl, const CellIn &c, double T, bool caseA, double y,
-                              double &xHI, double &xHeI, double &xHeII, double &ne, double &lr)
+__device__ double net_rate_lr(const DevCool &cl, const CellIn &c, double T, bool caseA, bool dere,
+                              double y, double &xHI, double &xHeI, double &xHeII, double &ne,
+                              double &lr)
 {
 #pragma clang fp contract(off)
-    const RateSet r = rates_at(T, caseA);
+    const RateSet r = rates_at(T, caseA, dere);
     ne = solve_cell(r, c.gH, c.gHe1, c.gHe2, c.nH, y, xHI, xHeI, xHeII);
     const double heat = c.nH * (xHI * c.hH + y * (xHeI * c.hHe1 + xHeII * c.hHe2));
     const double cool = cooling_total(cl, r, T, c.nH, ne, xHI, xHeI, xHeII, y, caseA);
@@ -2112,7 +2119,7 @@ __device__ __noinline__ NetOut net_rate_mo(const DevCool &cl, const MetalSw ms, 
 {
 #pragma clang fp contract(off)
     double ne;
-    const RateSet r = rates_at(T, caseA);
+    const RateSet r = rates_at(T, caseA, ms.ci_dere != 0);
     double lo[MT], up[MT];
     {
         MCache<MT> cc;
@@ -2178,7 +2185,7 @@ __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCo
             auto net_rate = [&](const DevCool &cl_, const CellIn &c_, double T, bool cA, double y_,
                                 double &a1, double &a2, double &a3, double &ne_) -> double {
                 if constexpr (MT == 0) {
-                    return ::net_rate(cl_, c_, T, cA, y_, a1, a2, a3, ne_);
+                    return ::net_rate(cl_, c_, T, cA, ms.ci_dere != 0, y_, a1, a2, a3, ne_);
                 } else {
                     const NetOut o = net_rate_mo<MT, LAY>(cl_, ms, il, c_, T, cA, y_, a1, a2, a3);
                     a1 = o.xHI; a2 = o.xHeI; a3 = o.xHeII; ne_ = o.ne;
@@ -2189,7 +2196,7 @@ __global__ __launch_bounds__(kBlock) void k_thermal(DevGrid g, DevState s, DevCo
                                    double y_, double &a1, double &a2, double &a3, double &ne_,
                                    double &lr) -> double {
                 if constexpr (MT == 0) {
-                    return ::net_rate_lr(cl_, c_, T, cA, y_, a1, a2, a3, ne_, lr);
+                    return ::net_rate_lr(cl_, c_, T, cA, ms.ci_dere != 0, y_, a1, a2, a3, ne_, lr);
                 } else {
                     const NetOut o = net_rate_mo<MT, LAY>(cl_, ms, il, c_, T, cA, y_, a1, a2, a3);
                     a1 = o.xHI; a2 = o.xHeI; a3 = o.xHeII; ne_ = o.ne; lr = o.lr;
@@ -2333,7 +2340,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(DevGrid g, DevState s, bool
     double v[3] = {0.0, 0.0, 0.0};
     bool ismax[3] = {true, false, false};
     if (il < g.nleaf && s.nH[il] > 0.0 && 1.0 - s.xHI[il] >= 1.0e-10) {
-        const RateSet r = rates_at(s.Te[il], caseA);
+        const RateSet r = rates_at(s.Te[il], caseA, ms.ci_dere != 0);
         const double nHI = s.nH[il] * s.xHI[il], nHII = s.nH[il] * (1.0 - s.xHI[il]);
         double rin = 0.0, rout = 0.0;
         if constexpr (MT != 0) {
@@ -2911,6 +2918,7 @@ MetalSw make_sw(mc_ctx *c, bool thermal = false)
     m.cx_two_way = c->p.charge_exchange == MC_CX_TWO_WAY;
     m.metal_heat = c->p.metal_heat != 0;
     m.metal_ff = c->p.metal_freefree != 0;
+    m.ci_dere = c->p.ci_model == MC_CI_DERE_HYBRID;
     m.noise_cap = thermal && c->warm_mode != MC_WARM_NONE ? 2 : 0;
     m.nlt = thermal ? c->cool.nlt : nullptr;
     return m;

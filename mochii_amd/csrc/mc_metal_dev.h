@@ -322,6 +322,10 @@ __device__ double m_hydrogen_neutral_fraction(const MCache<MT> &cc, double nH, d
 struct MetalSw {
     const double *gam, *heat;    // nleaf x ntr, element order
     int32_t ntr, metal_ne, cx_two_way, metal_heat, metal_ff;
+    // ci_model 'dere_hybrid': the H / He collisional ionisation rates carry
+    // the Dere ratio (ci_HI / ci_HeI / ci_HeII, src/recomb_mod.f90:263-279;
+    // the metal ones hold it in the registry's ci_ratio)
+    int32_t ci_dere;
     // warm-started solve only (0 = the reference's rule): stop the n_e
     // iteration after this many consecutive passes whose hydrogen is below
     // resolution (x_HII < 1e-10); see solve_cell_m

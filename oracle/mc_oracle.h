@@ -76,6 +76,7 @@ double orc_alphaA_HeII(double T);
 double orc_alphaB_HeII(double T);
 double orc_alphaA_HeIII(double T);
 double orc_alphaB_HeIII(double T);
+int    orc_ci_model_set(int ci_dere);
 double orc_ci_HI(double T);
 double orc_ci_HeI(double T);
 double orc_ci_HeII(double T);

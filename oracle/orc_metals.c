@@ -20,8 +20,8 @@
  *   ci_dere_ratio                                      recomb_mod :288-311
  *   solve_ion_cell (metal branches),
  *   hydrogen_neutral_fraction, charge_neutrality_ne    ion_balance_mod :53-254
- * ci_model 'voronov' on the model path (ci_dere_ratio is restated and pinned,
- * but the model keeps ci_dere = 0, as the H/He rates do).
+ * ci_model 'dere_hybrid' multiplies every Voronov rate, H and He included,
+ * by ci_dere_ratio (the model's ci_dere switch, set for each model-level call).
  * cooling_model 'local_ne': each stage's fit is suppressed by its (T, n_e)
  * table when one is set (orc_set_stage_table; nlevel_cooling_apply,
  * src/species_mod.f90:836-840), else it enters unsuppressed ('low_density').
@@ -321,6 +321,7 @@ void orc_solve_ion_cell_leaf(const orc_ctx *c, int il, double gH, double gHe1, d
                              double nH, double T, int caseA, double yHe, double *xHI,
                              double *xHeI, double *xHeII, double *ne_out)
 {
+    const int ci_prev = orc_ci_model_set(c->p.ci_dere);
     int with_ne = 0, with_cx = 0;
     if (il >= 0 && c->use_metals && c->nel > 0) {
         with_ne = c->p.metal_ne;
@@ -360,6 +361,7 @@ void orc_solve_ion_cell_leaf(const orc_ctx *c, int il, double gH, double gHe1, d
         if (fabs(ne - ne_old) <= 1.0e-10 * ne) break;
     }
     *ne_out = ne;
+    orc_ci_model_set(ci_prev);
 }
 
 /* charge_neutrality_ne (src/ion_balance_mod.f90:235-254) */

@@ -122,11 +122,31 @@ static double voronov(double T, double dE, double P, double A, double X, double 
     return A * (1.0 + P * sqrt(U)) * pow(U, K) * exp(-U) / (X + U);
 }
 
-/* ci_* (src/recomb_mod.f90:263-279); the Dere ratio is 1 for ci_model
- * 'voronov' */
-double orc_ci_HI(double T) { return voronov(T, 13.6, 0.0, 2.91e-8, 0.232, 0.39); }
-double orc_ci_HeI(double T) { return voronov(T, 24.6, 0.0, 1.75e-8, 0.180, 0.35); }
-double orc_ci_HeII(double T) { return voronov(T, 54.4, 1.0, 2.05e-9, 0.265, 0.25); }
+/* ci_model of the model being solved: 1 for 'dere_hybrid'.  The model-level
+ * entry points set it from their context for the duration of the call
+ * (orc_ci_model_set returns the previous value to restore); 0 otherwise. */
+static int g_ci_dere = 0;
+int orc_ci_model_set(int ci_dere)
+{
+    const int prev = g_ci_dere;
+    g_ci_dere = ci_dere;
+    return prev;
+}
+
+/* ci_* (src/recomb_mod.f90:263-279): Voronov times ci_dere_ratio, which is 1
+ * for ci_model 'voronov' */
+double orc_ci_HI(double T)
+{
+    return voronov(T, 13.6, 0.0, 2.91e-8, 0.232, 0.39) * orc_ci_dere_ratio(g_ci_dere, 1, 1);
+}
+double orc_ci_HeI(double T)
+{
+    return voronov(T, 24.6, 0.0, 1.75e-8, 0.180, 0.35) * orc_ci_dere_ratio(g_ci_dere, 2, 1);
+}
+double orc_ci_HeII(double T)
+{
+    return voronov(T, 54.4, 1.0, 2.05e-9, 0.265, 0.25) * orc_ci_dere_ratio(g_ci_dere, 2, 2);
+}
 
 /* ---- cooling (src/cooling_mod.f90, src/gaunt.f90) ---------------------- */
 

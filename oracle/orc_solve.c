@@ -59,6 +59,7 @@ static double leaf_vol(const orc_ctx *c, int i)
 /* gas_equilibrium_update (src/ion_balance_mod.f90:352-434) */
 void orc_equilibrium_update(orc_ctx *c, double *max_dx, double *dx_vol)
 {
+    const int ci_prev = orc_ci_model_set(c->p.ci_dere);
     const double T = c->p.te_fixed, yHe = c->p.He_abund, w = c->p.ion_relax;
     double mdx = 0.0, sdx = 0.0, sx = 0.0;
     for (int i = 0; i < c->nleaf; ++i) {
@@ -86,11 +87,13 @@ void orc_equilibrium_update(orc_ctx *c, double *max_dx, double *dx_vol)
     }
     *max_dx = mdx;
     *dx_vol = sdx / (sx > DBL_MIN ? sx : DBL_MIN);
+    orc_ci_model_set(ci_prev);
 }
 
 /* hydrogen_balance_residual_grid (src/ion_balance_mod.f90:280-343) */
 void orc_residual_grid(orc_ctx *c, double *eps_max, double *eps_vol)
 {
+    const int ci_prev = orc_ci_model_set(c->p.ci_dere);
     double emax = 0.0, sev = 0.0, sv = 0.0;
     for (int i = 0; i < c->nleaf; ++i) {
         if (c->nH[i] <= 0.0) continue;
@@ -116,6 +119,7 @@ void orc_residual_grid(orc_ctx *c, double *eps_max, double *eps_vol)
     }
     *eps_max = emax;
     *eps_vol = sev / (sv > DBL_MIN ? sv : DBL_MIN);
+    orc_ci_model_set(ci_prev);
 }
 
 /* cooling_setup (src/cooling_mod.f90:67-92): H I fit terms (from
@@ -132,6 +136,7 @@ void orc_cooling_setup(orc_ctx *c, int nfit, const double *A, const double *Ti)
 double orc_net_rate(const orc_ctx *c, int il, double T, double *xHI, double *xHeI,
                     double *xHeII, double *ne)
 {
+    const int ci_prev = orc_ci_model_set(c->p.ci_dere);
     const double nH = c->nH[il], y = c->p.He_abund;
     orc_solve_ion_cell_leaf(c, il, c->gHI[il], c->gHeI[il], c->gHeII[il], nH, T, c->p.caseA, y,
                             xHI, xHeI, xHeII, ne);
@@ -145,6 +150,7 @@ double orc_net_rate(const orc_ctx *c, int il, double T, double *xHI, double *xHe
         if (c->p.metal_freefree) net = net - orc_metal_freefree(c, il, T, nH, *ne, nHI, nHII);
         if (c->p.metal_heat) net = net + orc_metal_heating(c, il, T, nH, *ne, nHI, nHII);
     }
+    orc_ci_model_set(ci_prev);
     return net;
 }
 
@@ -197,6 +203,7 @@ void orc_set_solve_threads(orc_ctx *c, int n) { c->solve_threads = n < 1 ? 1 : n
 void orc_thermal_update(orc_ctx *c, double *max_dx, double *max_dte, double *dx_vol,
                         double *dte_vol)
 {
+    const int ci_prev = orc_ci_model_set(c->p.ci_dere);
     const double w = c->p.ion_relax, y = c->p.He_abund;
     const int nl = c->nleaf;
     double *te = (double *)malloc(sizeof(double) * (nl + 1));
@@ -243,4 +250,5 @@ void orc_thermal_update(orc_ctx *c, double *max_dx, double *max_dte, double *dx_
     *max_dx = mdx; *max_dte = mdt;
     *dx_vol = s1 / (s2 > DBL_MIN ? s2 : DBL_MIN);
     *dte_vol = s3 / (s4 > DBL_MIN ? s4 : DBL_MIN);
+    orc_ci_model_set(ci_prev);
 }

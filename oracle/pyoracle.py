@@ -132,6 +132,8 @@ def _load():
         f = getattr(lib, name)
         f.restype = C.c_double
         f.argtypes = [C.c_double] * (2 if name == "orc_planck_shape" else 1)
+    lib.orc_ci_model_set.argtypes = [C.c_int]
+    lib.orc_ci_model_set.restype = C.c_int
     lib.orc_cooling_setup.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     lib.orc_thermal_update.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
     lib.orc_set_solve_threads.argtypes = [C.c_void_p, C.c_int]
@@ -817,6 +819,8 @@ def ref_lib():
     L.ref_vfky96.restype = C.c_double
     L.ref_vy95.argtypes = [C.c_double, _dp]
     L.ref_vy95.restype = C.c_double
+    L.ref_rate_dere.argtypes = [C.c_int, C.c_double]
+    L.ref_rate_dere.restype = C.c_double
     L.ref_ci_dere_ratio.argtypes = [C.c_int, C.c_int]
     L.ref_ci_dere_ratio.restype = C.c_double
     L.ref_nlevel_load.argtypes = [C.c_char_p, C.c_int]

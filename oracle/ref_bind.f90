@@ -117,6 +117,19 @@ contains
     end select
   end function ref_rate
 
+  ! ci_HI / ci_HeI / ci_HeII (which 6, 7, 8) with par%ci_model = 'dere_hybrid'
+  real(c_double) function ref_rate_dere(which, T) bind(C, name='ref_rate_dere')
+    integer(c_int), value :: which
+    real(c_double), value :: T
+    par%ci_model = 'dere_hybrid'
+    select case (which)
+    case (6); ref_rate_dere = ci_HI(T)
+    case (7); ref_rate_dere = ci_HeI(T)
+    case default; ref_rate_dere = ci_HeII(T)
+    end select
+    par%ci_model = 'voronov'
+  end function ref_rate_dere
+
   real(c_double) function ref_vfky96(e, p) bind(C, name='ref_vfky96')
     real(c_double), value :: e
     real(c_double), intent(in) :: p(8)

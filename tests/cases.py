@@ -33,6 +33,8 @@ def oracle_params(params):
         out.setdefault("diffuse_model", 2)
     if "charge_exchange" in params:
         out["cx_two_way"] = int(params["charge_exchange"] == 0)
+    if "ci_model" in params:
+        out["ci_dere"] = int(params["ci_model"] == 1)          # MC_CI_DERE_HYBRID
     return out
 
 

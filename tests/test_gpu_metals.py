@@ -150,6 +150,48 @@ def test_metal_thermal_update_matches_oracle():
     assert hot.sum() > 20
 
 
+def test_dere_hybrid_thermal_update_matches_oracle():
+    """ci_model 'dere_hybrid' (src/recomb_mod.f90:263-311): the H / He and
+    every metal Voronov rate times the Dere ratio, in the pass's stage cache,
+    the cell solves and the collisional-ionisation cooling of the thermal
+    update -- T_e bit-equal to the oracle under the same switch, and the
+    stage fractions of the collisionally ionised hot gas moved against the
+    Voronov model."""
+    n = 8
+    out = {}
+    for ci in (0, 1):
+        params, nH = hii_thermal(n=n, nphotons=20000, metals=True)
+        params["ci_model"] = ci
+        orc, gm = oracle_model(params, nH), gpu_model(params, nH)
+        orc.cooling_setup([a for a, _ in HI_COOLING_FIT], [t for _, t in HI_COOLING_FIT])
+        gm.cooling_setup(HI_COOLING_FIT)
+        x = _core(n)
+        orc.set_state(xHI=x)
+        orc.opacity_fill()
+        gm.set_state(xHI=x)
+        _pass(orc, gm)
+        mo = orc.thermal_update()
+        orc.opacity_fill()
+        mg = gm.gas_thermal_update()
+        so, sg = orc.state(), gm.state()
+        rel = np.abs(sg["Te"] / so["Te"] - 1.0)
+        print("ci_model", ci, "T_e: max rel diff", rel.max(), "bit-equal", np.mean(rel == 0.0))
+        assert np.all(rel == 0.0)
+        np.testing.assert_allclose(mg, mo, rtol=1e-3, atol=5e-5)
+        els = elements_of(params)
+        # the stage fractions follow n_e, which is resolved where n_HII is
+        # (the dark leaves' n_e is noise-limited, as in the test above)
+        resolved = 1.0 - so["xHI"] >= 1e-8
+        assert resolved.sum() > 50
+        for ie, el in enumerate(els):
+            ok, err = rel_close(gm.species_fractions(ie)[resolved],
+                                orc.species_fractions(ie)[resolved], 1e-4, floor_frac=1e-6)
+            assert ok, (ci, el["name"], err)
+        out[ci] = (sg["Te"], [gm.species_fractions(ie) for ie in range(len(els))])
+    assert not np.array_equal(out[0][0], out[1][0])
+    assert any(not np.allclose(a, b, rtol=1e-6, atol=0) for a, b in zip(out[0][1], out[1][1]))
+
+
 def test_metal_warm_start_matches_bisection():
     """The safeguarded Newton warm start with the metal terms accepts the
     bisection's bracket: same T_e to the bracket width on nearly all leaves."""

@@ -155,6 +155,26 @@ GW = {"c": (9, 6), "n": (4, 9), "o": (9, 4), "ne": (1, 6), "mg": (1, 2), "si": (
       "s": (9, 4), "ar": (1, 6), "fe": (25, 30), "cl": (6, 9), "ca": (1, 2)}
 
 
+def test_dere_hybrid_h_he_rates_match_reference():
+    """ci_model 'dere_hybrid' scales ci_HI / ci_HeI / ci_HeII by the Dere
+    ratio (src/recomb_mod.f90:263-279): the oracle under the switch against
+    the reference module with par%ci_model = 'dere_hybrid', and the switch
+    restores the Voronov rates."""
+    R, L = _ref(), po.lib()
+    T = np.geomspace(3e3, 1e6, 25)
+    plain = [[L.orc_ci_HI(t), L.orc_ci_HeI(t), L.orc_ci_HeII(t)] for t in T]
+    prev = L.orc_ci_model_set(1)
+    try:
+        for t in T:
+            for which, f in ((6, L.orc_ci_HI), (7, L.orc_ci_HeI), (8, L.orc_ci_HeII)):
+                assert f(t) == pytest.approx(R.ref_rate_dere(which, t), rel=1e-13, abs=1e-300)
+    finally:
+        L.orc_ci_model_set(prev)
+    again = [[L.orc_ci_HI(t), L.orc_ci_HeI(t), L.orc_ci_HeII(t)] for t in T]
+    assert again == plain
+    np.testing.assert_allclose(R.ref_rate_dere(6, 2e4) / R.ref_rate(6, 2e4), 0.9063, rtol=1e-14)
+
+
 def test_charge_exchange_detailed_balance():
     L = po.lib()
     tested = 0
