@@ -468,6 +468,17 @@ int mc_info(mc_ctx *ctx, int32_t *nleaf, int32_t *nnu_band, double *Ltot, double
  * packet at a time). */
 int mc_set_packet_order(mc_ctx *ctx, int order);
 
+/* Launch records per batch of a pass (default MC_LAUNCH_BATCH_DEFAULT =
+ * 64M, 124 B of device memory each; halved down to 1M when the device cannot
+ * hold them).  A pass launches and walks its packets in batches of this many
+ * records; with direction order a batch is sorted as a whole, so larger
+ * batches give each wavefront packets of one emitter and direction when many
+ * emitters share a pass.  The tallies are the same sums for any batch size
+ * (to FP64 summation order).  Lower it to bound the records' memory.
+ * max_records >= 1024.  No counterpart in the reference. */
+#define MC_LAUNCH_BATCH_DEFAULT ((int64_t)1 << 26)
+int mc_set_launch_batch(mc_ctx *ctx, int64_t max_records);
+
 /* Energy-moment tallies of the MC_MIXED walk (on by default): a scored
  * segment adds its path weight times the Chebyshev polynomials T_0..T_3 of
  * its packet's position in an ln-E sub-bin of its band bin to four per-leaf
@@ -561,7 +572,7 @@ int mc_qmc_uniforms(mc_ctx *ctx, int stream, int64_t first_index, int n, int ndi
  * ids first_id .. first_id + n - 1 (1-based, launch order) as the pass's
  * launch kernel computes them: rec is n x 7 {x, y, z, kx, ky, kz, E [eV]},
  * bin_leaf n x 2 {0-based band bin, 0-based start leaf or -1 outside}.
- * n is at most the launch batch (4,194,304).  Diagnostic. */
+ * n is at most the launch batch (mc_set_launch_batch).  Diagnostic. */
 int mc_launch_records(mc_ctx *ctx, int64_t first_id, int n, double *rec, int32_t *bin_leaf);
 
 /* Diagnostic: leaves crossed by the direct walk of one ray to the box edge

@@ -221,11 +221,15 @@ static int alloc_leaf_arrays(mc_ctx *c)
 }
 
 // launch-record buffers and the direction-sort workspace for batches of
-// min(n, 4M) packets; grown (never shrunk) when a band needs larger batches
-int mc_packet_capacity(mc_ctx *c, int64_t n)
+// min(n, batch_max) packets (124 B of records each; 64M by default,
+// mc_set_launch_batch); grown (never shrunk) when a
+// band needs larger batches.  Large batches keep the direction-sorted waves
+// coherent when many emitters share a pass: a batch holds every emitter's
+// share of its packet ids, so the packets of one emitter and direction cell
+// per batch grow with the batch.  Halved down to 1M when the device cannot
+// hold them.
+static void free_packet_buffers(mc_ctx *c)
 {
-    const int64_t cap = std::min<int64_t>(std::max<int64_t>(n, 1), (int64_t)1 << 22);
-    if (c->pk.x && c->pk.cap >= cap) return 0;
     PacketBuf &pk = c->pk;
     for (void *q : {(void *)pk.x, (void *)pk.y, (void *)pk.z, (void *)pk.kx, (void *)pk.ky,
                     (void *)pk.kz, (void *)pk.E, (void *)pk.sig, (void *)pk.inu, (void *)pk.leaf,
@@ -236,6 +240,12 @@ int mc_packet_capacity(mc_ctx *c, int64_t n)
     pk.key = c->d_key_sorted = nullptr;
     c->d_perm = nullptr;
     c->d_sort_tmp = nullptr;
+    pk.cap = 0;
+}
+
+static int alloc_packet_buffers(mc_ctx *c, int64_t cap)
+{
+    PacketBuf &pk = c->pk;
     pk.cap = cap;
     if (dalloc(&pk.x, cap) || dalloc(&pk.y, cap) || dalloc(&pk.z, cap) || dalloc(&pk.kx, cap) ||
         dalloc(&pk.ky, cap) || dalloc(&pk.kz, cap) || dalloc(&pk.E, cap) ||
@@ -249,6 +259,22 @@ int mc_packet_capacity(mc_ctx *c, int64_t n)
         return -100;
     }
     return 0;
+}
+
+int mc_packet_capacity(mc_ctx *c, int64_t n)
+{
+    int64_t cap = std::min<int64_t>(std::max<int64_t>(n, 1), c->batch_max);
+    if (c->pk.x && c->pk.cap >= cap) return 0;
+    for (;;) {
+        free_packet_buffers(c);
+        if (alloc_packet_buffers(c, cap) == 0) return 0;
+        (void)hipGetLastError();
+        if (cap <= ((int64_t)1 << 20)) {
+            free_packet_buffers(c);
+            return -100;
+        }
+        cap = std::max<int64_t>(cap / 2, (int64_t)1 << 20);
+    }
 }
 
 extern "C" int mc_init(const mc_params *p, mc_ctx **out)
@@ -314,8 +340,9 @@ extern "C" int mc_init(const mc_params *p, mc_ctx **out)
         mc_finalize(c);
         return -100;
     }
-    // launch-record buffers: batches of up to 4M packets
+    // launch-record buffers: batches of up to 64M packets
     c->packet_order = MC_ORDER_DIRECTION;
+    c->batch_max = MC_LAUNCH_BATCH_DEFAULT;
     c->mom_on = 1;
     c->warm_levels = MC_WARM_LEVELS_DEFAULT;
     if (mc_packet_capacity(c, p->nphotons)) {
@@ -675,6 +702,18 @@ extern "C" int mc_set_packet_order(mc_ctx *c, int order)
     }
     c->packet_order = order;
     return 0;
+}
+
+extern "C" int mc_set_launch_batch(mc_ctx *c, int64_t max_records)
+{
+    CHECK_CTX(c);
+    if (max_records < 1024) {
+        mc_set_error("mc_set_launch_batch: max_records must be >= 1024");
+        return -1;
+    }
+    c->batch_max = max_records;
+    free_packet_buffers(c);
+    return mc_packet_capacity(c, c->p.nphotons);
 }
 
 extern "C" int mc_set_tally_moments(mc_ctx *c, int on)

@@ -325,6 +325,7 @@ struct mc_ctx {
     int64_t thermal_stats[4]; // evaluations (bracketed leaves), fallbacks, bracketed, leaves
     // direction-sorted walk order of stellar batches
     int packet_order;         // MC_ORDER_*
+    int64_t batch_max;        // launch records per batch (mc_set_launch_batch)
     uint32_t *d_key_sorted;
     int32_t *d_perm;
     void *d_sort_tmp;

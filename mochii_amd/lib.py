@@ -133,6 +133,7 @@ def library():
         "mc_launch_records": ([vp, C.c_int64, C.c_int, dp, ip], C.c_int),
         "mc_set_packet_order": ([vp, C.c_int], C.c_int),
         "mc_set_tally_moments": ([vp, C.c_int], C.c_int),
+        "mc_set_launch_batch": ([vp, C.c_int64], C.c_int),
         "mc_tally_moment_info": ([vp, C.POINTER(C.c_int32), dp], C.c_int),
         "mc_thermal_stats": ([vp, i64p], C.c_int),
         "mc_mlp_set": ([vp, fp, C.c_int], C.c_int),

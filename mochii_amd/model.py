@@ -184,6 +184,10 @@ class Model:
         """MC_ORDER_DIRECTION (default) or MC_ORDER_LAUNCH walk order."""
         check(self._L.mc_set_packet_order(self._h, int(order)), "mc_set_packet_order")
 
+    def set_launch_batch(self, max_records):
+        """Launch records per batch of a pass (default 64M)."""
+        check(self._L.mc_set_launch_batch(self._h, int(max_records)), "mc_set_launch_batch")
+
     def set_precision(self, precision):
         check(self._L.mc_set_precision(self._h, int(precision)), "mc_set_precision")
         self.params.precision = int(precision)

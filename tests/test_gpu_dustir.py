@@ -193,11 +193,13 @@ def test_ir_transport_mixed_form_matches_oracle():
 
 
 def test_ir_transport_batches_and_convergence():
-    """5e6 packets walk in two launch-record batches (4M + the rest) with the
-    oracle's counts and ledger; the second pass changes the emitted total by
-    about the reabsorbed fraction squared."""
+    """5e6 packets walk in two launch-record batches (4M + the rest, the
+    batch lowered from its default) with the oracle's counts and ledger; the
+    second pass changes the emitted total by about the reabsorbed fraction
+    squared."""
     N = 5_000_000
     params, orc, gm = _pair(n=8, N=N)
+    gm.set_launch_batch(1 << 22)
     ho = _heat(orc)
     _setup(orc, gm, ho)
     _, ro = orc.transport_dust_emission(ho, npacket=N, niter=2, tol=1e-12, nthreads=8)
