@@ -673,13 +673,19 @@ constexpr int kAggMin = 3;      // smallest group worth a wave reduction
 // per pass of the H II region workload at 0 / 1 / 2 / 8 rounds
 constexpr int kAggRoundsMom = 1;
 
-// The metal-free moment walk merges every set of lanes that scored the same
-// (leaf, sub-bin) in one step: a per-wave LDS table of kAggSlots slots, each
-// MC_MOM_NM moments + the grain absorption kept outside the moments + a
-// claiming lane.  A global FP64 atomic costs far more device throughput than
-// an LDS one, so each distinct (leaf, sub-bin) of the step goes out as one
-// set of atomics from the lane that claimed its slot; lanes that lose their
-// slot to another key, and segments without a sub-bin, issue their own.
+// The metal-free moment walk keeps a per-wave LDS cache of (leaf, sub-bin)
+// moment rows: kAggSlots slots, each a tag, MC_MOM_NM moments and the grain
+// absorption kept outside the moments.  A global FP64 atomic costs far more
+// device throughput than an LDS one (the device completes ~2.3e10 FP64
+// atomics per second wherever they land), and the lanes of a wave walk
+// nearby leaves in one sub-bin, so a lane adds its segment to the slot that
+// holds its (leaf, sub-bin) and the slot's sums go out as one set of atomics
+// when another key takes the slot or the wave ends.  Per step: a lane whose
+// key is in its slot adds to it (a hit, and the slot stays); the lanes whose
+// key is not claim its slot, one of them (the owner) writes the old entry
+// out and installs its key, and the lanes with the owner's key add to it;
+// a lane whose slot is held by a hit of another key, or taken by another
+// key, and segments without a sub-bin, issue their own atomics.
 constexpr int kAggSlotBits = 7;
 constexpr int kAggSlots = 1 << kAggSlotBits;
 constexpr int kAggVals = MC_MOM_NM + 1;
@@ -691,22 +697,61 @@ __device__ __forceinline__ void wave_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// tv: this wave's kAggSlots x kAggVals doubles, zero between calls; tc: its
-// kAggSlots claims
-__device__ __forceinline__ void flush_mom_agg(const KArgs &a, Lane &L, int lane, double *tv,
-                                              int *tc)
+// the moment-row key of a (leaf, sub-bin), never 0 (0 marks an empty slot)
+__device__ __forceinline__ unsigned long long mom_key(int il, int ms)
+{
+    return ((((unsigned long long)(unsigned)il) << 20) | (unsigned)ms) + 1ull;
+}
+
+// write one cached entry's sums to its row (the issuing lane's replica of a
+// hot leaf) and clear them
+__device__ __forceinline__ void mom_cache_write(const KArgs &a, unsigned long long key, double *t)
+{
+    const unsigned long long k = key - 1ull;
+    const int il = (int)(k >> 20), ms = (int)(k & 0xFFFFFull);
+    double *m = mom_row(a, il, ms);
+#pragma unroll
+    for (int q = 0; q < MC_MOM_NM; ++q) {
+        unsafeAtomicAdd(m + q, t[q]);
+        t[q] = 0.0;
+    }
+    const double d = t[MC_MOM_NM];
+    if (d != 0.0) {
+        double *h, *j;
+        tally_rows(a, il, h, j);
+        unsafeAtomicAdd(h + 6, d);
+        t[MC_MOM_NM] = 0.0;
+    }
+}
+
+// tv: this wave's kAggSlots x kAggVals sums (zero in an empty slot), tt its
+// tags, tc its claims
+__device__ __forceinline__ void flush_mom_cache(const KArgs &a, Lane &L, int lane, double *tv,
+                                                unsigned long long *tt, int *tc)
 {
     const bool has = L.s_il >= 0;
     if (!__ballot(has)) return;
     const bool mom = has && L.ms >= 0;
-    const unsigned long long key =
-        mom ? (((unsigned long long)(unsigned)L.s_il << 20) | (unsigned)L.ms) : 0ull;
+    const unsigned long long key = mom ? mom_key(L.s_il, L.ms) : 0ull;
     const int slot = (int)((key * 0x9E3779B97F4A7C15ull) >> (64 - kAggSlotBits));
-    if (mom) tc[slot] = lane;
+    const bool hit = mom && tt[slot] == key;
+    const bool miss = mom && !hit;
+    // claims: the missing lanes first, then a hit marks its slot as held
+    if (miss) tc[slot] = lane;
     wave_sync_lds();
-    const int owner = mom ? tc[slot] : lane;
+    if (hit) tc[slot] = 64;
+    wave_sync_lds();
+    const int claim = miss ? tc[slot] : 64;
+    const int owner = claim < 64 ? claim : lane;
     const unsigned long long okey = (unsigned long long)__shfl((long long)key, owner);
-    const bool in = mom && okey == key;
+    const bool in = hit || (miss && claim < 64 && okey == key);
+    if (miss && claim == lane) {
+        double *t = tv + slot * kAggVals;
+        const unsigned long long old = tt[slot];
+        if (old != 0ull) mom_cache_write(a, old, t);
+        tt[slot] = key;
+    }
+    wave_sync_lds();
     if (in) {
         double *t = tv + slot * kAggVals;
         const double pl = L.s_pl;
@@ -726,24 +771,19 @@ __device__ __forceinline__ void flush_mom_agg(const KArgs &a, Lane &L, int lane,
         tally_one<0, true>(a, L, L.s_il, L.s_pl);
     }
     wave_sync_lds();
-    if (in && owner == lane) {
-        double *t = tv + slot * kAggVals;
-        double *m = mom_row(a, L.s_il, L.ms);
-#pragma unroll
-        for (int q = 0; q < MC_MOM_NM; ++q) {
-            unsafeAtomicAdd(m + q, t[q]);
-            t[q] = 0.0;
-        }
-        const double d = t[MC_MOM_NM];
-        if (d != 0.0) {
-            double *h, *j;
-            tally_rows(a, L.s_il, h, j);
-            unsafeAtomicAdd(h + 6, d);
-            t[MC_MOM_NM] = 0.0;
+    L.s_il = -1;
+}
+
+// the wave's cached entries out, at the end of its walk
+__device__ __forceinline__ void mom_cache_drain(const KArgs &a, int lane, double *tv,
+                                                unsigned long long *tt)
+{
+    for (int s = lane; s < kAggSlots; s += 64) {
+        if (tt[s] != 0ull) {
+            mom_cache_write(a, tt[s], tv + s * kAggVals);
+            tt[s] = 0ull;
         }
     }
-    wave_sync_lds();
-    L.s_il = -1;
 }
 
 // Issue the tallies of the step's segments, called by all 64 lanes.  With
@@ -755,11 +795,11 @@ __device__ __forceinline__ void flush_mom_agg(const KArgs &a, Lane &L, int lane,
 // per-lane atomics.  jbin: this wave's nnu LDS bins, zero between calls.
 template <int P, int M>
 __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, double *jbin,
-                                            double *aggv, int *aggc)
+                                            double *aggv, unsigned long long *aggt, int *aggc)
 {
     if constexpr (Prec<P>::mixed && M == 0) {
         if (a.mom) {
-            flush_mom_agg(a, L, lane, aggv, aggc);
+            flush_mom_cache(a, L, lane, aggv, aggt, aggc);
             return;
         }
     }
@@ -1537,13 +1577,16 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
     __shared__ double jbins[kBlock / 64][MC_MAXBINS];
     double *jbin = jbins[threadIdx.x >> 6];
     for (int b = threadIdx.x & 63; b < MC_MAXBINS; b += 64) jbin[b] = 0.0;
-    // the moment walk's per-wave (leaf, sub-bin) table (metal-free mixed form)
+    // the moment walk's per-wave (leaf, sub-bin) cache (metal-free mixed form)
     constexpr int kSlots = (Prec<P>::mixed && M == 0) ? kAggSlots : 1;
     __shared__ double aggvs[kBlock / 64][kSlots * kAggVals];
+    __shared__ unsigned long long aggts[kBlock / 64][kSlots];
     __shared__ int aggcs[kBlock / 64][kSlots];
     double *aggv = aggvs[threadIdx.x >> 6];
+    unsigned long long *aggt = aggts[threadIdx.x >> 6];
     int *aggc = aggcs[threadIdx.x >> 6];
     for (int b = threadIdx.x & 63; b < kSlots * kAggVals; b += 64) aggv[b] = 0.0;
+    for (int b = threadIdx.x & 63; b < kSlots; b += 64) aggt[b] = 0ull;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1610,7 +1653,13 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
             if constexpr (G == kCarDDA) car_step<P, M>(a, L, cn);
             else oct_step<P, M>(a, L, cn);
         }
-        flush_tally<P, M>(a, L, lane, jbin, aggv, aggc);
+        flush_tally<P, M>(a, L, lane, jbin, aggv, aggt, aggc);
+    }
+    if constexpr (Prec<P>::mixed && M == 0) {
+        if (a.mom) {
+            wave_sync_lds();
+            mom_cache_drain(a, lane, aggv, aggt);
+        }
     }
     if constexpr (M == 2) {
         // compensated wave sums of the infrared ledger into this wave's slot

@@ -100,6 +100,37 @@ def test_multifractal_reduced_matches_oracle(prec):
         _check_thermal(orc, gm, near)
 
 
+def test_multifractal_moment_cache_matches_per_coefficient_sums():
+    """The mixed-form walk of the reduced multifractal model (16 sources, the
+    external sphere field, grains with scattering) with its energy-moment
+    tallies merged in the per-wave cache agrees with the per-coefficient sums
+    of the same walk in every rate sum, band J bin and the grain absorption,
+    in one launch batch and in batches of 4096 records."""
+    params, nH, octree, rhokap, tab = multifractal_cloud(nphotons=200_000, precision=MC_MIXED,
+                                                         lmax=6, mass_thr=3e-4, ir_packets=1e5)
+    st, _ = _ionised_near_sources(params, *octree[:3], nH, 6.0)
+    out = {}
+    for mom, batch in ((False, None), (True, None), (True, 4096)):
+        gm = gpu_model(params, nH, rhokap=rhokap, dust_table=tab, octree=octree)
+        gm.set_state(**st)
+        gm.gas_opacity_fill()
+        gm.set_tally_moments(mom)
+        if batch:
+            gm.set_launch_batch(batch)
+        gm.ionizing_field_and_rates()
+        if mom:
+            assert gm.tally_moment_info()[0] > 16
+        out[(mom, batch)] = gm.tallies()
+        gm.close()
+    ref = out[(False, None)]
+    for key in ((True, None), (True, 4096)):
+        t = out[key]
+        assert (t["seg_direct"], t["seg_scattered"]) == (ref["seg_direct"], ref["seg_scattered"])
+        for k in ("hhe", "jt", "dust_abs"):
+            ok, e = rel_close(t[k], ref[k], 1e-6, floor_frac=1e-9)
+            assert ok, (key, k, e)
+
+
 def test_turbulent_reduced_matches_oracle():
     n = 24
     params, nH = turbulent_cloud(n=n, nphotons=20000, half=4.0)
