@@ -259,7 +259,8 @@ def timed_steps(name, params, nH, octree, rhokap, tab, args, mlp):
     dt = time.perf_counter() - t0
     N = int(params["nphotons"])
     out = dict(packets_per_s=N * args.steps / dt, ms_per_step=1e3 * dt / args.steps,
-               packets_per_step=N, walk_ms=float(np.mean(walk)), solve_ms=float(np.mean(solve)),
+               packets_per_step=N, steps=args.steps, warmup=args.warmup,
+               walk_ms=float(np.mean(walk)), solve_ms=float(np.mean(solve)),
                segments_direct=int(np.mean([s[0] for s in segs])),
                segments_scattered=int(np.mean([s[1] for s in segs])),
                max_energy_closure=float(np.max(clos)),
@@ -322,7 +323,10 @@ def cell_errors(a, b):
     return out
 
 
-def converge(name, params, nH, octree, mlp, seed=None, rhokap=None, tab=None):
+def converge(name, params, nH, octree, mlp, seed=None, rhokap=None, tab=None, ir_packets=0):
+    """run_model (src/main.f90:124-227) of one form to convergence; with
+    ir_packets, the dust thermal-infrared band after it (main.f90:244-258),
+    timed on its own."""
     import mochii_amd as mc
     from mochii_amd import run_model
     prec, warm = form_setup(name, mlp)
@@ -334,11 +338,15 @@ def converge(name, params, nH, octree, mlp, seed=None, rhokap=None, tab=None):
                     int(s.split()[1].rstrip(":")) % 10 == 0 else None)
     t = time.perf_counter() - t0
     st = m.state()
-    m.close()
     log(f"[converge {name}{'' if seed is None else f' seed {seed}'}] {res['iterations']} "
         f"iterations, converged={res['converged']}, {t:.2f} s")
-    return dict(time_to_converged_s=t, iterations=res["iterations"],
-                converged=bool(res["converged"])), st
+    out = dict(time_to_converged_s=t, iterations=res["iterations"],
+               converged=bool(res["converged"]))
+    if ir_packets:
+        out["ir_band"] = ir_band(m, name, ir_packets)
+        out["time_to_converged_with_ir_s"] = t + out["ir_band"]["s_per_call"]
+    m.close()
+    return out, st
 
 
 def converge_leg(mlp, train_info):
@@ -448,18 +456,31 @@ def converge_large(which, mlp, train_info):
     else:
         params, nH, octree, rhokap, tab = multifractal_cloud()
     rows, states = {}, {}
+    irp = int(params.get("dust_no_photons", 0)) if params.get("ion_add_dust") else 0
     for name in ("mixed_mlp", "fp64"):
-        rows[name], states[name] = converge(name, params, nH, octree, mlp, rhokap=rhokap, tab=tab)
-    out = dict(workload=which, leaves=int(nH.size), forms=rows,
+        rows[name], states[name] = converge(name, params, nH, octree, mlp, rhokap=rhokap, tab=tab,
+                                            ir_packets=irp)
+    out = dict(workload=which, leaves=int(nH.size), packets_per_iteration=int(params["nphotons"]),
+               forms=rows,
                speedup_vs_fp64=rows["fp64"]["time_to_converged_s"] /
                rows["mixed_mlp"]["time_to_converged_s"],
                error_vs_fp64=cell_errors(states["mixed_mlp"], states["fp64"]),
                surrogate_training=train_info)
+    if train_info:
+        out["speedup_vs_fp64_with_training"] = rows["fp64"]["time_to_converged_s"] / (
+            rows["mixed_mlp"]["time_to_converged_s"] + train_info["collect_s"] +
+            train_info["train_s"])
     print(json.dumps(out))
     for k in ("fp64", "mixed_mlp"):
         print(f"time_to_converged_{k}_s: {rows[k]['time_to_converged_s']:.6e}")
         print(f"iterations_{k}: {rows[k]['iterations']}")
+        if "time_to_converged_with_ir_s" in rows[k]:
+            print(f"time_to_converged_with_ir_{k}_s: {rows[k]['time_to_converged_with_ir_s']:.6e}")
     print(f"speedup_vs_fp64: {out['speedup_vs_fp64']:.6e}")
+    if "speedup_vs_fp64_with_training" in out:
+        print(f"speedup_vs_fp64_with_training: {out['speedup_vs_fp64_with_training']:.6e}")
+    for k, v in out["error_vs_fp64"].items():
+        print(f"{k}: {v:.6e}")
 
 
 def main():
@@ -492,21 +513,33 @@ def main():
                                                          ir_packets=args.ir_packets)
     log(f"[large model] {nH.size} leaves generated in {time.perf_counter() - t0:.1f} s")
     forms, ir, state = {}, {}, None
-    for name in args.forms.split(","):
-        m, forms[name] = timed_steps(name, params, nH, octree, rhokap, tab, args, mlp)
+    names = args.forms.split(",")
+    head_name = "mixed_mlp" if "mixed_mlp" in names else names[0]
+    for name in names:
+        # the value is the head form's K timed steps; a comparison form takes
+        # at most 5 (and 1 warm-up), which bounds the run when K is large
+        a2 = args
+        if name != head_name:
+            a2 = argparse.Namespace(**vars(args))
+            a2.steps, a2.warmup = min(args.steps, 5), min(args.warmup, 1)
+        m, forms[name] = timed_steps(name, params, nH, octree, rhokap, tab, a2, mlp)
         if not args.no_ir:
             ir[name] = ir_band(m, name, args.ir_packets)
         if name == "mixed_mlp" or state is None:
             state = m.state()
         m.close()
-    head = forms.get("mixed_mlp") or forms[list(forms)[0]]
+    head = forms[head_name]
     f64 = forms.get("fp64")
     # the walk's tally pattern on the device without the walk: one 4-double
     # moment row per update over the model's moment array (leaves x
-    # sub-bins), and single doubles over the same bytes
+    # sub-bins), single doubles over the same bytes, and 4-double rows over
+    # 3.2 MB (inside one XCD's L2) and 128 MB (inside the MALL): the FP64
+    # atomic rate does not depend on where the rows live
     nsub = max(head.get("moment_sub_bins", 0), 1)
     probe = [scattered_rmw_probe(nH.size * nsub, 4, 4e9),
-             scattered_rmw_probe(nH.size * nsub * 4, 1, 8e9)]
+             scattered_rmw_probe(nH.size * nsub * 4, 1, 8e9),
+             scattered_rmw_probe(100_000, 4, 4e9),
+             scattered_rmw_probe(4_000_000, 4, 4e9)]
     log(f"[probe] scattered RMW: {probe}")
     conv = None if args.no_converge else converge_leg(mlp, train_info)
     turb = None if args.no_turbulent else turbulent_leg(args, mlp)
@@ -570,6 +603,8 @@ def main():
                      "scattered_rmw": {
                          "probe_rows_per_s": probe[0]["rows_per_s"],
                          "probe_atomics_per_s_single": probe[1]["atomics_per_s"],
+                         "probe_atomics_per_s_3MB": probe[2]["atomics_per_s"],
+                         "probe_atomics_per_s_128MB": probe[3]["atomics_per_s"],
                          "walk_segments_per_s": nseg / (head["walk_ms"] * 1e-3),
                          "walk_frac_of_probe": nseg / (head["walk_ms"] * 1e-3) /
                                                probe[0]["rows_per_s"],

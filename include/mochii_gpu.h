@@ -439,6 +439,13 @@ int mc_species_tallies_get(mc_ctx *ctx, int ie, double *raw);
  * counterpart in the reference. */
 int mc_metal_segment_transitions(mc_ctx *ctx, int64_t *sum_ntr);
 
+/* Global FP64 atomics the walk issued for its tallies in the last pass
+ * (stellar and diffuse batches: rate sums, band J, grain absorption, metal
+ * path tallies, energy moments; after the wave merges and the moment cache),
+ * counted per lane on the device.  The walk's achieved atomic rate is this
+ * over the walk time.  No counterpart in the reference. */
+int mc_walk_atomics(mc_ctx *ctx, int64_t *n);
+
 /* hydrogen_balance_residual_grid (src/ion_balance_mod.f90:313-343). */
 int mc_hydrogen_balance_residual_grid(mc_ctx *ctx, double *eps_max, double *eps_vol);
 

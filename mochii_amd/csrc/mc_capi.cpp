@@ -332,7 +332,7 @@ extern "C" int mc_init(const mc_params *p, mc_ctx **out)
         mc_finalize(c);
         return -100;
     }
-    if (dalloc(&c->st.ledger, 3) || dalloc(&c->st.segs, 3) || dalloc(&c->st.counter, 1)) {
+    if (dalloc(&c->st.ledger, 3) || dalloc(&c->st.segs, 4) || dalloc(&c->st.counter, 1)) {
         mc_finalize(c);
         return -100;
     }
@@ -1108,6 +1108,17 @@ extern "C" int mc_species_tallies_get(mc_ctx *c, int ie, double *raw)
             for (int q = 0; q < 2; ++q)
                 raw[((size_t)il * nt + it) * 2 + q] = h[((size_t)il * ntr + k) * 2 + q];
         }
+    return 0;
+}
+
+extern "C" int mc_walk_atomics(mc_ctx *c, int64_t *n)
+{
+    CHECK_CTX(c);
+    if (!n) {
+        mc_set_error("mc_walk_atomics: null output");
+        return -1;
+    }
+    *n = c->segs[3];
     return 0;
 }
 

@@ -209,7 +209,7 @@ struct DevState {
     double *jt;        // nleaf * nnu
     double *rates;     // 6 * nleaf
     double *ledger;    // 3
-    unsigned long long *segs;   // 3: direct, scattered, sum of metal transitions
+    unsigned long long *segs;   // 4: direct, scattered, sum of metal transitions, tally atomics
     unsigned long long *counter;// work counter for persistent kernels
     double *partials;  // per-block metric partials
     // leaves next to the point sources take many updates per pass; their
@@ -364,7 +364,8 @@ struct mc_ctx {
     int lt_on[3];
     // last pass results
     double ledger[3];
-    int64_t segs[3];          // direct, scattered, metal transitions over scored segments
+    int64_t segs[4];          // direct, scattered, metal transitions over scored segments,
+                              // global FP64 tally atomics of the walk
     double times[4];
 };
 

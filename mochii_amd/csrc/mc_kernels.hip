@@ -472,6 +472,7 @@ struct Lane {
     double s_pl;
     // octree walk: current node, current position (FP64 form), node geometry
     int node, leaf0;
+    uint32_t nat;            // global FP64 tally atomics this lane issued
     double ox, oy, oz;
     float4 gf;
 };
@@ -569,6 +570,13 @@ __device__ __forceinline__ int hot_rep(const DevState &s)
     return (int)(((blockIdx.x * kBlock + threadIdx.x) * 0x9E3779B1u) >> 16) % s.nrep;
 }
 
+// one global FP64 tally atomic, counted for the pass's atomics report
+__device__ __forceinline__ void tally_add(double *p, double v, uint32_t &n)
+{
+    unsafeAtomicAdd(p, v);
+    ++n;
+}
+
 __device__ __forceinline__ void tally_rows(const KArgs &a, int il, double *&h, double *&j)
 {
     const DevState &s = a.s;
@@ -611,19 +619,19 @@ __device__ __forceinline__ double *mom_row(const KArgs &a, int il, int ms)
 // tallies (MOM) the MC_MOM_NM moments of the packet's sub-bin instead,
 // pl T_m(t) by the Chebyshev recurrence
 template <int M, bool MOM>
-__device__ __forceinline__ void tally_one(const KArgs &a, const Lane &L, int il, double pl)
+__device__ __forceinline__ void tally_one(const KArgs &a, Lane &L, int il, double pl)
 {
     if constexpr (MOM) {
         if (L.ms >= 0) {
             double *m = mom_row(a, il, L.ms);
             const double t2 = 2.0 * mom_t();
             double pm2 = pl, pm1 = 0.5 * t2 * pl;
-            unsafeAtomicAdd(m + 0, pm2);
-            unsafeAtomicAdd(m + 1, pm1);
+            tally_add(m + 0, pm2, L.nat);
+            tally_add(m + 1, pm1, L.nat);
 #pragma unroll
             for (int q = 2; q < MC_MOM_NM; ++q) {
                 const double pq = t2 * pm1 - pm2;
-                unsafeAtomicAdd(m + q, pq);
+                tally_add(m + q, pq, L.nat);
                 pm2 = pm1;
                 pm1 = pq;
             }
@@ -634,27 +642,27 @@ __device__ __forceinline__ void tally_one(const KArgs &a, const Lane &L, int il,
                 // there would be replaced)
                 double *h, *j;
                 tally_rows(a, il, h, j);
-                unsafeAtomicAdd(h + 6, pl * L.dabs);
+                tally_add(h + 6, pl * L.dabs, L.nat);
             }
             return;
         }
     }
     double *h, *j;
     tally_rows(a, il, h, j);
-    unsafeAtomicAdd(j + L.inu, pl);
+    tally_add(j + L.inu, pl, L.nat);
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         const double ck = coef(k);
-        if (ck != 0.0) unsafeAtomicAdd(h + k, pl * ck);
+        if (ck != 0.0) tally_add(h + k, pl * ck, L.nat);
     }
-    if (a.add_dust && L.dabs != 0.0) unsafeAtomicAdd(h + 6, pl * L.dabs);
+    if (a.add_dust && L.dabs != 0.0) tally_add(h + 6, pl * L.dabs, L.nat);
     if constexpr (M == 1) {
         double *r = a.mraw + (size_t)il * a.ntr * 2;
         for (int k = 0; k < L.nmet; ++k) {
             double c1, c2;
             metal_coeff(k, L, c1, c2);
-            unsafeAtomicAdd(r + 2 * k, pl * c1);
-            if (c2 != 0.0) unsafeAtomicAdd(r + 2 * k + 1, pl * c2);
+            tally_add(r + 2 * k, pl * c1, L.nat);
+            if (c2 != 0.0) tally_add(r + 2 * k + 1, pl * c2, L.nat);
         }
     }
 }
@@ -705,21 +713,22 @@ __device__ __forceinline__ unsigned long long mom_key(int il, int ms)
 
 // write one cached entry's sums to its row (the issuing lane's replica of a
 // hot leaf) and clear them
-__device__ __forceinline__ void mom_cache_write(const KArgs &a, unsigned long long key, double *t)
+__device__ __forceinline__ void mom_cache_write(const KArgs &a, unsigned long long key, double *t,
+                                                uint32_t &nat)
 {
     const unsigned long long k = key - 1ull;
     const int il = (int)(k >> 20), ms = (int)(k & 0xFFFFFull);
     double *m = mom_row(a, il, ms);
 #pragma unroll
     for (int q = 0; q < MC_MOM_NM; ++q) {
-        unsafeAtomicAdd(m + q, t[q]);
+        tally_add(m + q, t[q], nat);
         t[q] = 0.0;
     }
     const double d = t[MC_MOM_NM];
     if (d != 0.0) {
         double *h, *j;
         tally_rows(a, il, h, j);
-        unsafeAtomicAdd(h + 6, d);
+        tally_add(h + 6, d, nat);
         t[MC_MOM_NM] = 0.0;
     }
 }
@@ -748,7 +757,7 @@ __device__ __forceinline__ void flush_mom_cache(const KArgs &a, Lane &L, int lan
     if (miss && claim == lane) {
         double *t = tv + slot * kAggVals;
         const unsigned long long old = tt[slot];
-        if (old != 0ull) mom_cache_write(a, old, t);
+        if (old != 0ull) mom_cache_write(a, old, t, L.nat);
         tt[slot] = key;
     }
     wave_sync_lds();
@@ -776,11 +785,11 @@ __device__ __forceinline__ void flush_mom_cache(const KArgs &a, Lane &L, int lan
 
 // the wave's cached entries out, at the end of its walk
 __device__ __forceinline__ void mom_cache_drain(const KArgs &a, int lane, double *tv,
-                                                unsigned long long *tt)
+                                                unsigned long long *tt, uint32_t &nat)
 {
     for (int s = lane; s < kAggSlots; s += 64) {
         if (tt[s] != 0ull) {
-            mom_cache_write(a, tt[s], tv + s * kAggVals);
+            mom_cache_write(a, tt[s], tv + s * kAggVals, nat);
             tt[s] = 0ull;
         }
     }
@@ -835,14 +844,14 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
                     const double t2 = in ? 2.0 * mom_t() : 0.0;
                     double pm2 = pl, pm1 = 0.5 * t2 * pl;
                     double sum = wave_sum(pl);
-                    if (lane == leader) unsafeAtomicAdd(m + 0, sum);
+                    if (lane == leader) tally_add(m + 0, sum, L.nat);
                     sum = wave_sum(pm1);
-                    if (lane == leader) unsafeAtomicAdd(m + 1, sum);
+                    if (lane == leader) tally_add(m + 1, sum, L.nat);
 #pragma unroll
                     for (int q = 2; q < MC_MOM_NM; ++q) {
                         const double pq = t2 * pm1 - pm2;
                         sum = wave_sum(pq);
-                        if (lane == leader) unsafeAtomicAdd(m + q, sum);
+                        if (lane == leader) tally_add(m + q, sum, L.nat);
                         pm2 = pm1;
                         pm1 = pq;
                     }
@@ -853,7 +862,7 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
                             if (lane == leader) {
                                 double *h, *j;
                                 tally_rows(a, lil, h, j);
-                                unsafeAtomicAdd(h + 6, sum);
+                                tally_add(h + 6, sum, L.nat);
                             }
                         }
                     }
@@ -868,14 +877,14 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
                 const double v = in ? L.s_pl * coef(k) : 0.0;
                 if (__ballot(v != 0.0)) {
                     const double sum = wave_sum(v);
-                    if (lane == leader) unsafeAtomicAdd(h + k, sum);
+                    if (lane == leader) tally_add(h + k, sum, L.nat);
                 }
             }
             if (a.add_dust) {
                 const double v = in ? L.s_pl * L.dabs : 0.0;
                 if (__ballot(v != 0.0)) {
                     const double sum = wave_sum(v);
-                    if (lane == leader) unsafeAtomicAdd(h + 6, sum);
+                    if (lane == leader) tally_add(h + 6, sum, L.nat);
                 }
             }
             if constexpr (M == 1) {
@@ -888,8 +897,8 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
                     const double s1 = wave_sum(on ? L.s_pl * c1 : 0.0);
                     const double s2 = wave_sum(on ? L.s_pl * c2 : 0.0);
                     if (lane == leader) {
-                        unsafeAtomicAdd(r + 2 * k, s1);
-                        if (s2 != 0.0) unsafeAtomicAdd(r + 2 * k + 1, s2);
+                        tally_add(r + 2 * k, s1, L.nat);
+                        if (s2 != 0.0) tally_add(r + 2 * k + 1, s2, L.nat);
                     }
                 }
             }
@@ -902,7 +911,7 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
             for (int b = lane; b < nnu; b += 64) {
                 const double v = jbin[b];
                 if (v != 0.0) {
-                    unsafeAtomicAdd(j + b, v);
+                    tally_add(j + b, v, L.nat);
                     jbin[b] = 0.0;
                 }
             }
@@ -1594,6 +1603,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
     L.phase = 0;
     L.s_il = -1;
     L.nmet = 0;
+    L.nat = 0u;
     Counters cn = {0.0, 0.0, 0.0, 0u, 0u, 0u, 0ull, 0.0, 0.0};
     if constexpr (M != 2) {
         led(0) = 0.0;
@@ -1658,7 +1668,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
     if constexpr (Prec<P>::mixed && M == 0) {
         if (a.mom) {
             wave_sync_lds();
-            mom_cache_drain(a, lane, aggv, aggt);
+            mom_cache_drain(a, lane, aggv, aggt, L.nat);
         }
     }
     if constexpr (M == 2) {
@@ -1685,6 +1695,12 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
         cn.e_emit = led(0);
         cn.e_abs = led(1);
         cn.e_esc = led(2);
+    }
+    if constexpr (M != 2) {
+        // the lanes' tally atomics, one counter atomic per wave
+        unsigned long long nat = L.nat;
+        for (int off = 32; off > 0; off >>= 1) nat += __shfl_down(nat, off);
+        if (lane == 0 && nat) atomicAdd(a.s.segs + 3, nat);
     }
     // wave reduction of the ledger and segment counts, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
@@ -1720,7 +1736,7 @@ __global__ void k_reset_counter(unsigned long long *p) { *p = 0ull; }
 __global__ void k_zero_counters(DevState s)
 {
     if (threadIdx.x < 3) s.ledger[threadIdx.x] = 0.0;
-    if (threadIdx.x < 3) s.segs[threadIdx.x] = 0ull;
+    if (threadIdx.x < 4) s.segs[threadIdx.x] = 0ull;
     if (threadIdx.x == 0) *s.counter = 0ull;
 }
 
@@ -3252,14 +3268,12 @@ int mc_run_pass(mc_ctx *c)
         MC_HIP(hipGetLastError());
     }
     double led[3];
-    unsigned long long sg[3];
+    unsigned long long sg[4];
     MC_HIP(hipMemcpyAsync(led, c->st.ledger, sizeof(led), hipMemcpyDeviceToHost, st));
     MC_HIP(hipMemcpyAsync(sg, c->st.segs, sizeof(sg), hipMemcpyDeviceToHost, st));
     MC_HIP(hipStreamSynchronize(st));
     for (int k = 0; k < 3; ++k) c->ledger[k] = led[k];
-    c->segs[0] = (int64_t)sg[0];
-    c->segs[1] = (int64_t)sg[1];
-    c->segs[2] = (int64_t)sg[2];
+    for (int k = 0; k < 4; ++k) c->segs[k] = (int64_t)sg[k];
     c->times[0] = t_launch;
     c->times[1] = t_walk;
     return 0;

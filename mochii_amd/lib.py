@@ -161,6 +161,7 @@ def library():
         "mc_species_rates_get": ([vp, C.c_int, dp, dp], C.c_int),
         "mc_species_tallies_get": ([vp, C.c_int, dp], C.c_int),
         "mc_metal_segment_transitions": ([vp, i64p], C.c_int),
+        "mc_walk_atomics": ([vp, i64p], C.c_int),
         "mc_dust_temp_setup": ([vp], C.c_int),
         "mc_dust_temp_compute": ([vp], C.c_int),
         "mc_dust_emis_transport_setup": ([vp], C.c_int),

@@ -78,6 +78,12 @@ class Model:
               "mc_metal_segment_transitions")
         return v.value
 
+    def walk_atomics(self):
+        """Global FP64 tally atomics the walk issued in the last pass."""
+        v = C.c_int64()
+        check(self._L.mc_walk_atomics(self._h, C.byref(v)), "mc_walk_atomics")
+        return v.value
+
     def species_fractions(self, ie):
         """stage_frac of registry element ie: (nleaf, nstage)."""
         ns = self.elements[ie]["nstage"]

@@ -199,3 +199,35 @@ def test_pass_sources_and_external_field_matches_oracle(geom, prec):
     led = gm.energy_ledger()
     np.testing.assert_allclose(led, to["ledger"], rtol=1e-12 if prec == MC_FP64 else 1e-5)
     assert abs(led[1] + led[2] - led[0]) <= 5e-13 * led[0]
+
+
+def test_walk_atomics_count():
+    """The walk's count of global FP64 tally atomics: in launch order (no
+    wave merges) an H-only band below the He I edge issues exactly three per
+    scored segment in the FP64 form (band J, Gamma_HI, heating); the
+    direction-sorted walk merges same-leaf lanes and issues fewer; the
+    energy-moment walk issues at most four per segment (fewer after its
+    per-wave cache)."""
+    from mochii_amd.lib import MC_ORDER_LAUNCH
+    n = 16
+    params, nH = stromgren(n=n, nphotons=40000)
+    params["eion_max"] = 24.0
+    xHI = _partial_state(n)
+    counts = {}
+    for prec, order in ((MC_FP64, MC_ORDER_LAUNCH), (MC_FP64, None), (MC_MIXED, None)):
+        g = gpu_model(dict(params, precision=prec), nH)
+        g.set_state(xHI=xHI)
+        if order is not None:
+            g.set_packet_order(order)
+        g.ionizing_field_and_rates()
+        nd, ns = g.segment_counts()
+        counts[(prec, order)] = (g.walk_atomics(), nd + ns)
+        g.close()
+    nat, nseg = counts[(MC_FP64, MC_ORDER_LAUNCH)]
+    print("atomics per segment:", {k: v[0] / v[1] for k, v in counts.items()})
+    assert nseg > 100_000
+    assert 3 * nseg - 16 <= nat <= 3 * nseg
+    nat_dir, nseg_dir = counts[(MC_FP64, None)]
+    assert nseg_dir == nseg and nat_dir < nat
+    nat_mom, nseg_mom = counts[(MC_MIXED, None)]
+    assert 0 < nat_mom <= 4 * nseg_mom
