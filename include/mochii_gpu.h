@@ -476,14 +476,15 @@ int mc_info(mc_ctx *ctx, int32_t *nleaf, int32_t *nnu_band, double *Ltot, double
 int mc_set_packet_order(mc_ctx *ctx, int order);
 
 /* Launch records per batch of a pass (default MC_LAUNCH_BATCH_DEFAULT =
- * 64M, 124 B of device memory each; halved down to 1M when the device cannot
- * hold them).  A pass launches and walks its packets in batches of this many
+ * 2^30, 124 B of device memory each, so a pass of up to 1.07e9 packets is
+ * one batch of at most 133 GB; halved down to 1M when the device cannot hold
+ * them).  A pass launches and walks its packets in batches of this many
  * records; with direction order a batch is sorted as a whole, so larger
- * batches give each wavefront packets of one emitter and direction when many
- * emitters share a pass.  The tallies are the same sums for any batch size
- * (to FP64 summation order).  Lower it to bound the records' memory.
- * max_records >= 1024.  No counterpart in the reference. */
-#define MC_LAUNCH_BATCH_DEFAULT ((int64_t)1 << 26)
+ * batches give each wavefront packets of one emitter, sub-bin and direction
+ * when many emitters share a pass.  The tallies are the same sums for any
+ * batch size (to FP64 summation order).  Lower it to bound the records'
+ * memory.  max_records >= 1024.  No counterpart in the reference. */
+#define MC_LAUNCH_BATCH_DEFAULT ((int64_t)1 << 30)
 int mc_set_launch_batch(mc_ctx *ctx, int64_t max_records);
 
 /* Energy-moment tallies of the MC_MIXED walk (on by default): a scored

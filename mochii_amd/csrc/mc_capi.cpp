@@ -221,7 +221,7 @@ static int alloc_leaf_arrays(mc_ctx *c)
 }
 
 // launch-record buffers and the direction-sort workspace for batches of
-// min(n, batch_max) packets (124 B of records each; 64M by default,
+// min(n, batch_max) packets (124 B of records each; 2^30 by default,
 // mc_set_launch_batch); grown (never shrunk) when a
 // band needs larger batches.  Large batches keep the direction-sorted waves
 // coherent when many emitters share a pass: a batch holds every emitter's
@@ -340,7 +340,7 @@ extern "C" int mc_init(const mc_params *p, mc_ctx **out)
         mc_finalize(c);
         return -100;
     }
-    // launch-record buffers: batches of up to 64M packets
+    // launch-record buffers: batches of up to MC_LAUNCH_BATCH_DEFAULT packets
     c->packet_order = MC_ORDER_DIRECTION;
     c->batch_max = MC_LAUNCH_BATCH_DEFAULT;
     c->mom_on = 1;

@@ -734,7 +734,8 @@ __device__ __forceinline__ void mom_cache_write(const KArgs &a, unsigned long lo
 }
 
 // tv: this wave's kAggSlots x kAggVals sums (zero in an empty slot), tt its
-// tags, tc its claims
+// tags, tc its claims; M the walk's metal path
+template <int M>
 __device__ __forceinline__ void flush_mom_cache(const KArgs &a, Lane &L, int lane, double *tv,
                                                 unsigned long long *tt, int *tc)
 {
@@ -777,7 +778,7 @@ __device__ __forceinline__ void flush_mom_cache(const KArgs &a, Lane &L, int lan
         }
         if (a.add_dust && !a.mom_dust && L.dabs != 0.0) atomicAdd(t + MC_MOM_NM, pl * L.dabs);
     } else if (has) {
-        tally_one<0, true>(a, L, L.s_il, L.s_pl);
+        tally_one<M, true>(a, L, L.s_il, L.s_pl);
     }
     wave_sync_lds();
     L.s_il = -1;
@@ -806,9 +807,9 @@ template <int P, int M>
 __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, double *jbin,
                                             double *aggv, unsigned long long *aggt, int *aggc)
 {
-    if constexpr (Prec<P>::mixed && M == 0) {
+    if constexpr (Prec<P>::mixed && M <= 1) {
         if (a.mom) {
-            flush_mom_cache(a, L, lane, aggv, aggt, aggc);
+            flush_mom_cache<M>(a, L, lane, aggv, aggt, aggc);
             return;
         }
     }
@@ -1586,8 +1587,8 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
     __shared__ double jbins[kBlock / 64][MC_MAXBINS];
     double *jbin = jbins[threadIdx.x >> 6];
     for (int b = threadIdx.x & 63; b < MC_MAXBINS; b += 64) jbin[b] = 0.0;
-    // the moment walk's per-wave (leaf, sub-bin) cache (metal-free mixed form)
-    constexpr int kSlots = (Prec<P>::mixed && M == 0) ? kAggSlots : 1;
+    // the moment walk's per-wave (leaf, sub-bin) cache (mixed form)
+    constexpr int kSlots = (Prec<P>::mixed && M <= 1) ? kAggSlots : 1;
     __shared__ double aggvs[kBlock / 64][kSlots * kAggVals];
     __shared__ unsigned long long aggts[kBlock / 64][kSlots];
     __shared__ int aggcs[kBlock / 64][kSlots];
@@ -1665,7 +1666,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
         }
         flush_tally<P, M>(a, L, lane, jbin, aggv, aggt, aggc);
     }
-    if constexpr (Prec<P>::mixed && M == 0) {
+    if constexpr (Prec<P>::mixed && M <= 1) {
         if (a.mom) {
             wave_sync_lds();
             mom_cache_drain(a, lane, aggv, aggt, L.nat);
