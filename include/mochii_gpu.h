@@ -475,16 +475,21 @@ int mc_info(mc_ctx *ctx, int32_t *nleaf, int32_t *nnu_band, double *Ltot, double
  * packet at a time). */
 int mc_set_packet_order(mc_ctx *ctx, int order);
 
-/* Launch records per batch of a pass (default MC_LAUNCH_BATCH_DEFAULT =
- * 2^30, 124 B of device memory each, so a pass of up to 1.07e9 packets is
- * one batch of at most 133 GB; halved down to 1M when the device cannot hold
- * them).  A pass launches and walks its packets in batches of this many
- * records; with direction order a batch is sorted as a whole, so larger
- * batches give each wavefront packets of one emitter, sub-bin and direction
- * when many emitters share a pass.  The tallies are the same sums for any
- * batch size (to FP64 summation order).  Lower it to bound the records'
- * memory.  max_records >= 1024.  No counterpart in the reference. */
-#define MC_LAUNCH_BATCH_DEFAULT ((int64_t)1 << 30)
+/* Launch records per batch of a pass, 124 B of device memory each (halved
+ * down to 1M when the device cannot hold them).  A pass launches and walks
+ * its packets in batches of this many records; with direction order a batch
+ * is sorted as a whole, so larger batches give each wavefront packets of one
+ * emitter, sub-bin and direction when many emitters share a pass.  Defaults,
+ * set by mc_init from the precision form: MC_MIXED 2^30 (a pass of up to
+ * 1.07e9 packets is one batch, at most 133 GB: its energy-moment walk merges
+ * a wave's tallies in LDS, so tighter waves issue fewer atomics), MC_FP64
+ * 2^26 (its per-coefficient atomics slow down when many waves update the
+ * same rows at once: measured 5.18 s per pass at 2^26 against 6.17 s at
+ * 2^30 on the 2e7-leaf multifractal model).  The tallies are the same sums
+ * for any batch size (to FP64 summation order).  Lower it to bound the
+ * records' memory.  max_records >= 1024.  No counterpart in the reference. */
+#define MC_LAUNCH_BATCH_MIXED ((int64_t)1 << 30)
+#define MC_LAUNCH_BATCH_FP64 ((int64_t)1 << 26)
 int mc_set_launch_batch(mc_ctx *ctx, int64_t max_records);
 
 /* Energy-moment tallies of the MC_MIXED walk (on by default): a scored

@@ -221,8 +221,8 @@ static int alloc_leaf_arrays(mc_ctx *c)
 }
 
 // launch-record buffers and the direction-sort workspace for batches of
-// min(n, batch_max) packets (124 B of records each; 2^30 by default,
-// mc_set_launch_batch); grown (never shrunk) when a
+// min(n, batch_max) packets (124 B of records each; the default depends on
+// the precision form, mc_set_launch_batch); grown (never shrunk) when a
 // band needs larger batches.  Large batches keep the direction-sorted waves
 // coherent when many emitters share a pass: a batch holds every emitter's
 // share of its packet ids, so the packets of one emitter and direction cell
@@ -340,9 +340,9 @@ extern "C" int mc_init(const mc_params *p, mc_ctx **out)
         mc_finalize(c);
         return -100;
     }
-    // launch-record buffers: batches of up to MC_LAUNCH_BATCH_DEFAULT packets
+    // launch-record buffers: batches of up to MC_LAUNCH_BATCH_MIXED / _FP64 packets
     c->packet_order = MC_ORDER_DIRECTION;
-    c->batch_max = MC_LAUNCH_BATCH_DEFAULT;
+    c->batch_max = p->precision == MC_MIXED ? MC_LAUNCH_BATCH_MIXED : MC_LAUNCH_BATCH_FP64;
     c->mom_on = 1;
     c->warm_levels = MC_WARM_LEVELS_DEFAULT;
     if (mc_packet_capacity(c, p->nphotons)) {

@@ -682,7 +682,7 @@ constexpr int kAggMin = 3;      // smallest group worth a wave reduction
 constexpr int kAggRoundsMom = 1;
 
 // The metal-free moment walk keeps a per-wave LDS cache of (leaf, sub-bin)
-// moment rows: kAggSlots slots, each a tag, MC_MOM_NM moments and the grain
+// moment rows: 128 slots, each a tag, MC_MOM_NM moments and the grain
 // absorption kept outside the moments.  A global FP64 atomic costs far more
 // device throughput than an LDS one (the device completes ~2.3e10 FP64
 // atomics per second wherever they land), and the lanes of a wave walk
@@ -694,8 +694,12 @@ constexpr int kAggRoundsMom = 1;
 // out and installs its key, and the lanes with the owner's key add to it;
 // a lane whose slot is held by a hit of another key, or taken by another
 // key, and segments without a sub-bin, issue their own atomics.
-constexpr int kAggSlotBits = 7;
-constexpr int kAggSlots = 1 << kAggSlotBits;
+#ifndef MC_AGG_BITS0
+#define MC_AGG_BITS0 7
+#endif
+// slots per wave: 2^7 (three blocks of four waves per CU fit the LDS); the
+// metal walk's LDS also holds its packets' metal cross sections
+template <int M> constexpr int agg_slot_bits() { return M == 0 ? MC_AGG_BITS0 : 7; }
 constexpr int kAggVals = MC_MOM_NM + 1;
 
 __device__ __forceinline__ void wave_sync_lds()
@@ -733,7 +737,7 @@ __device__ __forceinline__ void mom_cache_write(const KArgs &a, unsigned long lo
     }
 }
 
-// tv: this wave's kAggSlots x kAggVals sums (zero in an empty slot), tt its
+// tv: this wave's slots x kAggVals sums (zero in an empty slot), tt its
 // tags, tc its claims; M the walk's metal path
 template <int M>
 __device__ __forceinline__ void flush_mom_cache(const KArgs &a, Lane &L, int lane, double *tv,
@@ -743,7 +747,7 @@ __device__ __forceinline__ void flush_mom_cache(const KArgs &a, Lane &L, int lan
     if (!__ballot(has)) return;
     const bool mom = has && L.ms >= 0;
     const unsigned long long key = mom ? mom_key(L.s_il, L.ms) : 0ull;
-    const int slot = (int)((key * 0x9E3779B97F4A7C15ull) >> (64 - kAggSlotBits));
+    const int slot = (int)((key * 0x9E3779B97F4A7C15ull) >> (64 - agg_slot_bits<M>()));
     const bool hit = mom && tt[slot] == key;
     const bool miss = mom && !hit;
     // claims: the missing lanes first, then a hit marks its slot as held
@@ -785,10 +789,11 @@ __device__ __forceinline__ void flush_mom_cache(const KArgs &a, Lane &L, int lan
 }
 
 // the wave's cached entries out, at the end of its walk
+template <int M>
 __device__ __forceinline__ void mom_cache_drain(const KArgs &a, int lane, double *tv,
                                                 unsigned long long *tt, uint32_t &nat)
 {
-    for (int s = lane; s < kAggSlots; s += 64) {
+    for (int s = lane; s < (1 << agg_slot_bits<M>()); s += 64) {
         if (tt[s] != 0ull) {
             mom_cache_write(a, tt[s], tv + s * kAggVals, nat);
             tt[s] = 0ull;
@@ -1588,7 +1593,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
     double *jbin = jbins[threadIdx.x >> 6];
     for (int b = threadIdx.x & 63; b < MC_MAXBINS; b += 64) jbin[b] = 0.0;
     // the moment walk's per-wave (leaf, sub-bin) cache (mixed form)
-    constexpr int kSlots = (Prec<P>::mixed && M <= 1) ? kAggSlots : 1;
+    constexpr int kSlots = (Prec<P>::mixed && M <= 1) ? (1 << agg_slot_bits<M>()) : 1;
     __shared__ double aggvs[kBlock / 64][kSlots * kAggVals];
     __shared__ unsigned long long aggts[kBlock / 64][kSlots];
     __shared__ int aggcs[kBlock / 64][kSlots];
@@ -1669,7 +1674,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
     if constexpr (Prec<P>::mixed && M <= 1) {
         if (a.mom) {
             wave_sync_lds();
-            mom_cache_drain(a, lane, aggv, aggt, L.nat);
+            mom_cache_drain<M>(a, lane, aggv, aggt, L.nat);
         }
     }
     if constexpr (M == 2) {

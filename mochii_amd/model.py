@@ -191,7 +191,7 @@ class Model:
         check(self._L.mc_set_packet_order(self._h, int(order)), "mc_set_packet_order")
 
     def set_launch_batch(self, max_records):
-        """Launch records per batch of a pass (default 2^30)."""
+        """Launch records per batch of a pass (default 2^30 mixed, 2^26 FP64)."""
         check(self._L.mc_set_launch_batch(self._h, int(max_records)), "mc_set_launch_batch")
 
     def set_precision(self, precision):
