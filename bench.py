@@ -238,7 +238,7 @@ def timed_steps(name, params, nH, octree, rhokap, tab, args, mlp):
         m.ionizing_field_and_rates()
         m.gas_thermal_update()
         log(f"  warm-up step {k + 1}: {m.timing()}")
-    walk, solve, segs, evals, brk, step_s, clos = [], [], [], [], [], [], []
+    walk, solve, segs, evals, brk, step_s, clos, natom = [], [], [], [], [], [], [], []
     t0 = time.perf_counter()
     for k in range(args.steps):
         ts = time.perf_counter()
@@ -246,6 +246,7 @@ def timed_steps(name, params, nH, octree, rhokap, tab, args, mlp):
         tm = m.timing()
         walk.append(tm["walk_ms"])
         segs.append(m.segment_counts())
+        natom.append(m.walk_atomics())
         led = m.energy_ledger()
         clos.append(abs(led[1] + led[2] - led[0]) / led[0])
         m.gas_thermal_update()
@@ -263,6 +264,8 @@ def timed_steps(name, params, nH, octree, rhokap, tab, args, mlp):
                walk_ms=float(np.mean(walk)), solve_ms=float(np.mean(solve)),
                segments_direct=int(np.mean([s[0] for s in segs])),
                segments_scattered=int(np.mean([s[1] for s in segs])),
+               walk_atomics=int(np.mean(natom)),
+               walk_atomics_per_s=float(np.sum(natom) / (np.sum(walk) * 1e-3)),
                max_energy_closure=float(np.max(clos)),
                net_rate_evals_per_bracketed_leaf=float(np.sum(evals) / max(np.sum(brk), 1)),
                setup_s=t_setup, moment_sub_bins=int(m.tally_moment_info()[0]))
@@ -600,6 +603,16 @@ def main():
                      "bytes_per_segment": bps, "segments_per_pass": nseg,
                      "walk_ms_per_pass": head["walk_ms"],
                      "traffic_bytes_per_segment": traffic_seg,
+                     "atomics": {
+                         "walk_atomics_per_pass": head["walk_atomics"],
+                         "walk_atomics_per_segment": head["walk_atomics"] / max(nseg, 1),
+                         "walk_atomics_per_s": head["walk_atomics_per_s"],
+                         "ceiling_atomics_per_s": max(p["atomics_per_s"] for p in probe),
+                         "frac_of_ceiling": head["walk_atomics_per_s"] /
+                                            max(p["atomics_per_s"] for p in probe),
+                         "basis": "global FP64 tally atomics counted by the walk "
+                                  "(mc_walk_atomics) over its event-timed duration, against "
+                                  "the scattered-RMW probe's highest FP64 atomic rate"},
                      "scattered_rmw": {
                          "probe_rows_per_s": probe[0]["rows_per_s"],
                          "probe_atomics_per_s_single": probe[1]["atomics_per_s"],
