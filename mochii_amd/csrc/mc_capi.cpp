@@ -965,8 +965,9 @@ extern "C" int mc_species_setup(mc_ctx *c, int n, const mc_element *els)
     }
     if (ntr > MC_GPU_MAXTR) {
         mc_set_error("mc_species_setup: " + std::to_string(ntr) +
-                     " metal transitions; the device cell solve holds at most 16 "
-                     "(C N O Ne S Ar have 15)");
+                     " metal transitions; the device tables hold at most " +
+                     std::to_string(MC_GPU_MAXTR) +
+                     " (the reference's eleven elements have 32)");
         return -1;
     }
     DevMetalTab &m = c->mtab;

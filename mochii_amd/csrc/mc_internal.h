@@ -80,8 +80,12 @@ struct MlpState {
 // are in element order (element by element, stage by stage), stages s
 // likewise; flat_t lists the transitions by threshold (the packet cache
 // order of species_packet_cross_sections).  The cell-solve kernels keep the
-// (leaf, T) rate cache in registers with loops unrolled to MC_GPU_MAXTR.
-#define MC_GPU_MAXTR 16
+// (leaf, T) rate cache in registers with loops unrolled to MC_GPU_MT_FAST
+// transitions for registries up to that size (C N O Ne S Ar have 15), and to
+// MC_GPU_MAXTR for larger ones (the reference's eleven elements C N O Ne S Ar
+// Mg Fe Si Cl Ca have 32; that cache spills to scratch).
+#define MC_GPU_MAXTR 32
+#define MC_GPU_MT_FAST 16
 // Registry layouts with compiled-in element structure (transitions per
 // element in registry order); any other registry takes the generic path
 #define MC_LAYOUT_GENERIC 0

@@ -1797,6 +1797,7 @@ __global__ __launch_bounds__(kBlock) void k_mom_fold_hot(DevState s, double *mom
 // sub-bins in step across the wave: the sub-bin-major moments are read
 // coalesced and the interpolation coefficients are wave-uniform loads (the
 // hot leaves' replica rows are already in their own rows, k_mom_fold_hot).
+template <int KQ>
 __global__ __launch_bounds__(kBlock) void k_mom_fold(DevState s, const double *__restrict__ mom,
                                                      int64_t rows, const double *__restrict__ coef,
                                                      const int32_t *__restrict__ band, int ns,
@@ -1805,7 +1806,7 @@ __global__ __launch_bounds__(kBlock) void k_mom_fold(DevState s, const double *_
 {
     const int il = blockIdx.x * kBlock + threadIdx.x;
     if (il >= nleaf) return;
-    constexpr int kQ = 7 + 2 * MC_GPU_MAXTR;
+    constexpr int kQ = KQ;
     const int nq = 6 + 2 * ntr + ndust;
     double acc[kQ];
 #pragma unroll
@@ -3268,9 +3269,15 @@ int mc_run_pass(mc_ctx *c)
                 c->st, c->d_mom, mrows, c->mom.nsub, c->nleaf);
             MC_HIP(hipGetLastError());
         }
-        k_mom_fold<<<(c->nleaf + kBlock - 1) / kBlock, kBlock, 0, st>>>(
-            c->st, c->d_mom, mrows, c->d_mom_coef, c->d_mom_band, c->mom.nsub, c->ntr,
-            c->mom.ndust, c->nnu, c->d_mraw, c->nleaf);
+        const unsigned nbf = (unsigned)((c->nleaf + kBlock - 1) / kBlock);
+        if (c->ntr > MC_GPU_MT_FAST)
+            k_mom_fold<7 + 2 * MC_GPU_MAXTR><<<nbf, kBlock, 0, st>>>(
+                c->st, c->d_mom, mrows, c->d_mom_coef, c->d_mom_band, c->mom.nsub, c->ntr,
+                c->mom.ndust, c->nnu, c->d_mraw, c->nleaf);
+        else
+            k_mom_fold<7 + 2 * MC_GPU_MT_FAST><<<nbf, kBlock, 0, st>>>(
+                c->st, c->d_mom, mrows, c->d_mom_coef, c->d_mom_band, c->mom.nsub, c->ntr,
+                c->mom.ndust, c->nnu, c->d_mraw, c->nleaf);
         MC_HIP(hipGetLastError());
     }
     double led[3];
@@ -3313,13 +3320,15 @@ int mc_run_equilibrium(mc_ctx *c, double *max_dx, double *dx_vol)
     MC_HIP(hipEventRecord(c->ev[6], st));
 #define MC_EQ_ARGS c->grid, c->st, c->p.te_fixed, c->p.case_A != 0, c->p.He_abund, \
                    c->p.ion_relax, c->p.distance2cm, make_sw(c)
-    if (c->ntr > 0) {
+    if (c->ntr > MC_GPU_MT_FAST) {
+        k_equilibrium<MC_GPU_MAXTR, MC_LAYOUT_GENERIC><<<nblk, kBlock, 0, st>>>(MC_EQ_ARGS);
+    } else if (c->ntr > 0) {
         if (c->metal_layout == MC_LAYOUT_CNONeSAr)
-            k_equilibrium<MC_GPU_MAXTR, MC_LAYOUT_CNONeSAr><<<nblk, kBlock, 0, st>>>(MC_EQ_ARGS);
+            k_equilibrium<MC_GPU_MT_FAST, MC_LAYOUT_CNONeSAr><<<nblk, kBlock, 0, st>>>(MC_EQ_ARGS);
         else if (c->metal_layout == MC_LAYOUT_CNONeS)
-            k_equilibrium<MC_GPU_MAXTR, MC_LAYOUT_CNONeS><<<nblk, kBlock, 0, st>>>(MC_EQ_ARGS);
+            k_equilibrium<MC_GPU_MT_FAST, MC_LAYOUT_CNONeS><<<nblk, kBlock, 0, st>>>(MC_EQ_ARGS);
         else
-            k_equilibrium<MC_GPU_MAXTR, MC_LAYOUT_GENERIC><<<nblk, kBlock, 0, st>>>(MC_EQ_ARGS);
+            k_equilibrium<MC_GPU_MT_FAST, MC_LAYOUT_GENERIC><<<nblk, kBlock, 0, st>>>(MC_EQ_ARGS);
     } else {
         k_equilibrium<0, MC_LAYOUT_GENERIC><<<nblk, kBlock, 0, st>>>(MC_EQ_ARGS);
     }
@@ -3370,10 +3379,12 @@ int mc_run_thermal(mc_ctx *c, double out[4])
         else                                                                                \
             k_thermal<MC_WARM_NONE, MT_, LAY_><<<nblk, kBlock, 0, st>>>(MC_THERMAL_ARGS);   \
     } while (0)
-    if (c->ntr > 0) {
-        if (c->metal_layout == MC_LAYOUT_CNONeSAr) MC_THERMAL_W(MC_GPU_MAXTR, MC_LAYOUT_CNONeSAr);
-        else if (c->metal_layout == MC_LAYOUT_CNONeS) MC_THERMAL_W(MC_GPU_MAXTR, MC_LAYOUT_CNONeS);
-        else MC_THERMAL_W(MC_GPU_MAXTR, MC_LAYOUT_GENERIC);
+    if (c->ntr > MC_GPU_MT_FAST) {
+        MC_THERMAL_W(MC_GPU_MAXTR, MC_LAYOUT_GENERIC);
+    } else if (c->ntr > 0) {
+        if (c->metal_layout == MC_LAYOUT_CNONeSAr) MC_THERMAL_W(MC_GPU_MT_FAST, MC_LAYOUT_CNONeSAr);
+        else if (c->metal_layout == MC_LAYOUT_CNONeS) MC_THERMAL_W(MC_GPU_MT_FAST, MC_LAYOUT_CNONeS);
+        else MC_THERMAL_W(MC_GPU_MT_FAST, MC_LAYOUT_GENERIC);
     } else {
         MC_THERMAL_W(0, MC_LAYOUT_GENERIC);
     }
@@ -3409,15 +3420,18 @@ int mc_run_residual(mc_ctx *c, double *eps_max, double *eps_vol)
 {
     const int nblk = (c->nleaf + kBlock - 1) / kBlock;
     if (metal_sync(c)) return -100;
-    if (c->ntr > 0) {
+    if (c->ntr > MC_GPU_MT_FAST) {
+        k_residual<MC_GPU_MAXTR, MC_LAYOUT_GENERIC><<<nblk, kBlock, 0, c->stream>>>(
+            c->grid, c->st, c->p.case_A != 0, make_sw(c));
+    } else if (c->ntr > 0) {
         if (c->metal_layout == MC_LAYOUT_CNONeSAr)
-            k_residual<MC_GPU_MAXTR, MC_LAYOUT_CNONeSAr><<<nblk, kBlock, 0, c->stream>>>(
+            k_residual<MC_GPU_MT_FAST, MC_LAYOUT_CNONeSAr><<<nblk, kBlock, 0, c->stream>>>(
                 c->grid, c->st, c->p.case_A != 0, make_sw(c));
         else if (c->metal_layout == MC_LAYOUT_CNONeS)
-            k_residual<MC_GPU_MAXTR, MC_LAYOUT_CNONeS><<<nblk, kBlock, 0, c->stream>>>(
+            k_residual<MC_GPU_MT_FAST, MC_LAYOUT_CNONeS><<<nblk, kBlock, 0, c->stream>>>(
                 c->grid, c->st, c->p.case_A != 0, make_sw(c));
         else
-            k_residual<MC_GPU_MAXTR, MC_LAYOUT_GENERIC><<<nblk, kBlock, 0, c->stream>>>(
+            k_residual<MC_GPU_MT_FAST, MC_LAYOUT_GENERIC><<<nblk, kBlock, 0, c->stream>>>(
                 c->grid, c->st, c->p.case_A != 0, make_sw(c));
     } else {
         k_residual<0, MC_LAYOUT_GENERIC><<<nblk, kBlock, 0, c->stream>>>(c->grid, c->st,
@@ -3445,8 +3459,13 @@ int mc_run_stage_refresh(mc_ctx *c)
 {
     if (c->ntr <= 0) return 0;
     if (metal_sync(c)) return -100;
-    k_stage_refresh<MC_GPU_MAXTR><<<(c->nleaf + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
-        c->grid, c->st, make_sw(c), c->d_mfrac, c->d_mop, c->d_mopf, c->nst, c->p.distance2cm);
+    const unsigned nb = (unsigned)((c->nleaf + kBlock - 1) / kBlock);
+    if (c->ntr > MC_GPU_MT_FAST)
+        k_stage_refresh<MC_GPU_MAXTR><<<nb, kBlock, 0, c->stream>>>(
+            c->grid, c->st, make_sw(c), c->d_mfrac, c->d_mop, c->d_mopf, c->nst, c->p.distance2cm);
+    else
+        k_stage_refresh<MC_GPU_MT_FAST><<<nb, kBlock, 0, c->stream>>>(
+            c->grid, c->st, make_sw(c), c->d_mfrac, c->d_mop, c->d_mopf, c->nst, c->p.distance2cm);
     MC_HIP(hipGetLastError());
     MC_HIP(hipStreamSynchronize(c->stream));
     return 0;

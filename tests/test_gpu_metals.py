@@ -80,13 +80,17 @@ def test_metal_pass_matches_oracle(prec):
 
 
 # registries: C N O Ne S Ar and C N O Ne S take the cell-solve kernels with
-# the element layout compiled in, C O S the generic registry path
-REGISTRIES = {"CNONeSAr": {"ar": 2.5e-6}, "CNONeS": {}, "COS": {"n": 0.0, "ne": 0.0}}
+# the element layout compiled in, C O S the generic registry path, and the
+# reference's eleven elements (32 transitions, solar abundances of Asplund et
+# al. 2009 for those whose reference default is 0) the 32-transition kernels
+REGISTRIES = {"CNONeSAr": {"ar": 2.5e-6}, "CNONeS": {}, "COS": {"n": 0.0, "ne": 0.0},
+              "ALL": {"ar": 2.5e-6, "mg": 3.98e-5, "fe": 3.16e-5, "si": 3.24e-5,
+                      "cl": 3.16e-7, "ca": 2.19e-6}}
 
 
 @pytest.mark.parametrize("cx,metal_ne,reg", [(0, 1, "CNONeSAr"), (MC_CX_METAL_ONLY, 1, "CNONeSAr"),
                                              (0, 0, "CNONeSAr"), (0, 1, "CNONeS"),
-                                             (0, 1, "COS")])
+                                             (0, 1, "COS"), (0, 1, "ALL")])
 def test_metal_equilibrium_update_matches_oracle(cx, metal_ne, reg):
     """Fixed-T update: solve_ion_cell with the metal electrons and
     (two_way) charge exchange, relaxation, charge neutrality of the written
@@ -115,6 +119,52 @@ def test_metal_equilibrium_update_matches_oracle(cx, metal_ne, reg):
         assert ok, (el["name"], err)
     eo, eg = orc.residual_grid(), gm.hydrogen_balance_residual_grid()
     np.testing.assert_allclose(eg, eo, rtol=1e-5, atol=1e-12)
+
+
+@pytest.mark.parametrize("prec", [MC_FP64, MC_MIXED])
+def test_full_registry_pass_and_thermal_update_match_oracle(prec):
+    """The reference's eleven elements, 32 metal transitions (past the
+    16-transition register cache: the 32-transition cell-solve kernels and a
+    walk with 32 cross sections per packet in LDS): H/He rates, every
+    element's raw tallies, Gammas and photoheating after one pass; in the
+    FP64 form the thermal update's T_e bit-equal to the oracle."""
+    els = registry(REGISTRIES["ALL"])
+    assert len(els) == 11 and sum(e["nstage"] - 1 for e in els) == 32
+    params, nH = hii_thermal(n=8, nphotons=20000, metals=True, precision=prec)
+    params["metal_eth"] = metal_thresholds(els)
+    orc, gm = oracle_model(params, nH, elements=els), gpu_model(params, nH, elements=els)
+    orc.cooling_setup([a for a, _ in HI_COOLING_FIT], [t for _, t in HI_COOLING_FIT])
+    gm.cooling_setup(HI_COOLING_FIT)
+    x = _core(8)
+    orc.set_state(xHI=x)
+    orc.opacity_fill()
+    gm.set_state(xHI=x)
+    _pass(orc, gm)
+    tol, floor = (1e-9, 1e-6) if prec == MC_FP64 else (2e-4, 1e-4)
+    ro, rg = orc.rates(), gm.rates()
+    for k in ("gHI", "gHeI", "hHI", "hHeI"):
+        ok, err = rel_close(rg[k], ro[k], tol, floor_frac=floor)
+        assert ok, (k, err)
+    for ie, el in enumerate(els):
+        ok, err = rel_close(gm.species_tallies(ie), orc.species_tallies(ie), tol,
+                            floor_frac=floor)
+        assert ok, (el["name"], "raw", err)
+        (go, ho), (gg, hg) = orc.species_rates(ie), gm.species_rates(ie)
+        ok, err = rel_close(gg, go, tol, floor_frac=floor)
+        assert ok, (el["name"], "gamma", err)
+    if prec != MC_FP64:
+        return
+    orc.thermal_update()
+    orc.opacity_fill()
+    gm.gas_thermal_update()
+    so, sg = orc.state(), gm.state()
+    rel = np.abs(sg["Te"] / so["Te"] - 1.0)
+    print("full registry T_e: max rel diff", rel.max())
+    assert np.all(rel == 0.0)
+    for ie, el in enumerate(els):
+        ok, err = rel_close(gm.species_fractions(ie), orc.species_fractions(ie), 1e-7,
+                            floor_frac=1e-10)
+        assert ok, (el["name"], err)
 
 
 def test_metal_thermal_update_matches_oracle():
