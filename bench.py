@@ -562,12 +562,17 @@ def main():
     if None not in (fetch, write, ndisp, pseg):
         traffic = (fetch + write) / ndisp
         traffic_seg = (fetch + write) / pseg
-    large_conv = {k: profile_value("converge_multifractal.md", k) for k in
-                  ("time_to_converged_fp64_s", "time_to_converged_mixed_mlp_s",
-                   "speedup_vs_fp64")}
-    turb_conv = {k: profile_value("converge_turbulent.md", k) for k in
-                 ("time_to_converged_fp64_s", "time_to_converged_mixed_mlp_s",
-                  "speedup_vs_fp64")}
+    conv_keys = ("time_to_converged_fp64_s", "time_to_converged_mixed_mlp_s", "iterations_fp64",
+                 "iterations_mixed_mlp", "speedup_vs_fp64", "speedup_vs_fp64_with_training",
+                 "frac_cells_xHII_within_1pct", "frac_cells_Te_within_1pct", "p99_rel_Te",
+                 "p99_rel_xHII_ionised")
+    large_conv = {k: profile_value("converge_multifractal.md", k) for k in conv_keys}
+    turb_conv = {k: profile_value("converge_turbulent.md", k) for k in conv_keys}
+    for cv in (large_conv, turb_conv):
+        if cv["frac_cells_Te_within_1pct"] is not None:
+            cv["meets_1pct_target"] = bool(cv["frac_cells_Te_within_1pct"] >= 0.99 and
+                                           cv["frac_cells_xHII_within_1pct"] >= 0.99)
+        cv["source"] = "committed summary of python3 bench.py --converge-large (profiles/)"
     line = {
         "metric": "packets/s + time-to-converged model, surrogate+FP32 vs plain FP64, "
                   "<=1% T_e/x_HII",
