@@ -1586,8 +1586,24 @@ __device__ __forceinline__ unsigned long long wave_first(unsigned long long v)
     return ((unsigned long long)hi << 32) | lo;
 }
 
+#ifndef MC_WALK_MINB_0
+#define MC_WALK_MINB_0 3
+#endif
+#ifndef MC_WALK_MINB_1
+#define MC_WALK_MINB_1 3
+#endif
+#ifndef MC_WALK_MINB_2
+#define MC_WALK_MINB_2 3
+#endif
+// blocks per CU the walk's registers are sized for: ionizing walk without and
+// with the metal path, infrared walk
+constexpr int walk_min_blocks(int M)
+{
+    return M == 2 ? MC_WALK_MINB_2 : (M == 1 ? MC_WALK_MINB_1 : MC_WALK_MINB_0);
+}
+
 template <int P, int G, int M>
-__global__ __launch_bounds__(kBlock, 3) void k_walk(KArgs a)
+__global__ __launch_bounds__(kBlock, walk_min_blocks(M)) void k_walk(KArgs a)
 {
     __shared__ double jbins[kBlock / 64][MC_MAXBINS];
     double *jbin = jbins[threadIdx.x >> 6];
