@@ -164,6 +164,10 @@ typedef struct mc_params {
     int32_t dust_niter;        /* par%dust_niter: self-heating passes           */
     double  dust_no_photons;   /* par%dust_no_photons: packets per pass         */
     double  dust_tol;          /* par%dust_tol: relative change that stops it   */
+    /* par%hei_diffuse (src/define.f90:634): with diffuse_field, the He I
+     * excited-level recombination channel (src/diffuse_mod.f90:95-105);
+     * needs mc_diffuse_hei_setup before the first pass */
+    int32_t hei_diffuse;
 } mc_params;
 
 /* One element of the trace-metal registry, the fields read_element fills
@@ -297,6 +301,22 @@ int mc_ionizing_field_and_rates(mc_ctx *ctx, int with_peel);
  * count diffuse_nphot (src/diffuse_mod.f90:38-39, printed by
  * ionizing_field_and_rates at src/ionizing_field_mod.f90:141-143). */
 int mc_diffuse_info(mc_ctx *ctx, double *lum, int64_t *nphot);
+
+/* hei_2ph_setup (src/hei_twophoton_mod.f90:98-183), called by src/main.f90:
+ * 107-112 when diffuse_field and hei_diffuse are on: the He I 2^1S two-photon
+ * shape A(y), y = E / 20.62 eV, as the n rows (4 <= n <= 128, ascending y) of
+ * data/HeI2phot.dat.  Builds the photon and energy moments of the H-ionizing
+ * window [13.598 eV, min(20.62 eV, eion_max)] and the energy-weighted sampling
+ * CDF on 1024 nodes that channel 4 of the diffuse field draws from.  The
+ * caller reads the file (the rows up to the first that is not two numbers). */
+int mc_diffuse_hei_setup(mc_ctx *ctx, int n, const double *y, const double *A);
+
+/* The two-photon moments of the last mc_diffuse_hei_setup: photons and
+ * energy [eV] per 2^1S decay inside the window (hei_2ph_photons_per_decay,
+ * hei_2ph_energy_per_decay, src/hei_twophoton_mod.f90:67-76), and, when
+ * grid_y / cdf_e are not NULL, the 1024 sampling nodes (y) and their CDF. */
+int mc_diffuse_hei_info(mc_ctx *ctx, double *n_per_decay, double *e_per_decay, double *grid_y,
+                        double *cdf_e);
 
 /* gas_equilibrium_update (src/ion_balance_mod.f90:352-434). */
 int mc_gas_equilibrium_update(mc_ctx *ctx, double *max_dx, double *dx_vol);

@@ -13,6 +13,7 @@ of the eleven registry elements is kept as data in
 import ctypes as C
 import json
 import os
+import numpy as np
 
 MC_MAX_EL, MC_MAX_ST, MC_MAX_METAL_TR, MC_MAX_COOLFIT = 11, 6, 64, 8
 MC_CX_TWO_WAY, MC_CX_METAL_ONLY = 0, 1
@@ -284,6 +285,16 @@ def load_nlevel(path=SNAPSHOT):
         with open(path) as f:
             _NLEVEL[path] = json.load(f).get("nlevel", {})
     return _NLEVEL[path]
+
+
+def load_hei_two_photon(path=SNAPSHOT):
+    """The He I 2^1S two-photon shape of the data snapshot (data/HeI2phot.dat,
+    Drake, Victor & Dalgarno 1969): (y, A) float64 arrays, y = E / 20.62 eV."""
+    with open(path) as f:
+        t = json.load(f).get("hei_two_photon")
+    if not t:
+        raise KeyError(f"{path} holds no He I two-photon table")
+    return np.asarray(t["y"], dtype=np.float64), np.asarray(t["A"], dtype=np.float64)
 
 
 def read_registry(atomic_dir):

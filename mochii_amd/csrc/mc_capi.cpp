@@ -203,7 +203,7 @@ static int alloc_leaf_arrays(mc_ctx *c)
         return -100;
     if (c->p.diffuse_field) {
         const size_t nb = (n + 1023) / 1024;        // scan tiles of 1024 leaves
-        if (dalloc(&c->d_dif, 4 * n) || dalloc(&c->d_dcdf, n) || dalloc(&c->d_bsum, nb + 1))
+        if (dalloc(&c->d_dif, 5 * n) || dalloc(&c->d_dcdf, n) || dalloc(&c->d_bsum, nb + 1))
             return -100;
     }
     if (c->p.ion_add_dust) {
@@ -396,7 +396,7 @@ extern "C" int mc_finalize(mc_ctx *c)
                     (void *)c->pk.E, (void *)c->pk.sig, (void *)c->pk.inu, (void *)c->pk.leaf,
                     (void *)c->d_node_gd, (void *)c->d_node_gf, (void *)c->d_node_link,
                     (void *)c->d_node_nbr, (void *)c->d_leaf_node, (void *)c->d_milne,
-                    (void *)c->mlp.P, (void *)c->d_feat, (void *)c->d_pred, (void *)c->pk.key,
+                    (void *)c->d_hei, (void *)c->mlp.P, (void *)c->d_feat, (void *)c->d_pred, (void *)c->pk.key,
                     (void *)c->d_key_sorted, (void *)c->d_perm, c->d_sort_tmp, (void *)c->d_nlt})
         if (p) (void)hipFree(p);
     for (auto &e : c->ev)
@@ -774,6 +774,60 @@ extern "C" int mc_diffuse_info(mc_ctx *c, double *lum, int64_t *nphot)
     CHECK_CTX(c);
     if (lum) *lum = c->diffuse_lum;
     if (nphot) *nphot = c->diffuse_nphot;
+    return 0;
+}
+
+extern "C" int mc_diffuse_hei_setup(mc_ctx *c, int n, const double *y, const double *A)
+{
+    CHECK_CTX(c);
+    if (!y || !A) {
+        mc_set_error("mc_diffuse_hei_setup: null table");
+        return -1;
+    }
+    std::vector<double> h(2 * MC_HEI2PH_NCDF);
+    double nper = 0.0, eper = 0.0;
+    const int rc = mc_hei2ph_tables(n, y, A, c->p.eion_max, h.data(), h.data() + MC_HEI2PH_NCDF,
+                                    &nper, &eper);
+    if (rc == -1) {
+        mc_set_error("mc_diffuse_hei_setup: the two-photon table needs 4 to 128 rows, got " +
+                     std::to_string(n));
+        return -1;
+    }
+    if (rc == -2) {
+        mc_set_error("mc_diffuse_hei_setup: the two-photon table's y must ascend");
+        return -1;
+    }
+    if (rc) {
+        mc_set_error("mc_diffuse_hei_setup: the two-photon table integrates to zero");
+        return -1;
+    }
+    if (!c->d_hei && dalloc(&c->d_hei, h.size())) return -100;
+    MC_HIP(hipMemcpy(c->d_hei, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    c->hei_nper = nper;
+    c->hei_eper = eper;
+    c->hei_ready = true;
+    DevMilne &M = c->milne;
+    M.hei_y = c->d_hei;
+    M.hei_cdf = c->d_hei + MC_HEI2PH_NCDF;
+    M.hei_eper = eper;
+    M.hei = c->p.diffuse_field && c->p.hei_diffuse;
+    return 0;
+}
+
+extern "C" int mc_diffuse_hei_info(mc_ctx *c, double *n_per_decay, double *e_per_decay,
+                                   double *grid_y, double *cdf_e)
+{
+    CHECK_CTX(c);
+    if (!c->hei_ready) {
+        mc_set_error("mc_diffuse_hei_info: no two-photon table (mc_diffuse_hei_setup)");
+        return -1;
+    }
+    if (n_per_decay) *n_per_decay = c->hei_nper;
+    if (e_per_decay) *e_per_decay = c->hei_eper;
+    if (grid_y) MC_HIP(hipMemcpy(grid_y, c->d_hei, sizeof(double) * MC_HEI2PH_NCDF,
+                                 hipMemcpyDeviceToHost));
+    if (cdf_e) MC_HIP(hipMemcpy(cdf_e, c->d_hei + MC_HEI2PH_NCDF, sizeof(double) * MC_HEI2PH_NCDF,
+                                hipMemcpyDeviceToHost));
     return 0;
 }
 

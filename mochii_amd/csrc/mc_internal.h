@@ -63,7 +63,22 @@ struct DevMilne {
     double eth[3];
     double eion_max;
     int32_t model;           // 0 threshold, 1 exponential, 2 milne
+    // channel 3, He I excited-level recombination (par%hei_diffuse): the
+    // two-photon sampling nodes y and their energy CDF, and the energy per
+    // decay inside the H-ionizing window [eV]; hei = 0 leaves the channel off
+    const double *hei_y;     // [MC_HEI2PH_NCDF]
+    const double *hei_cdf;   // [MC_HEI2PH_NCDF]
+    double hei_eper;
+    int32_t hei;
 };
+
+// He I 2^1S two-photon spectrum (src/hei_twophoton_mod.f90): table rows at
+// most, sampling nodes over the ionizing window, line and pair energies [eV]
+#define MC_HEI2PH_MAXPTS 128
+#define MC_HEI2PH_NCDF 1024
+#define MC_HEI_E_TOT 20.62
+#define MC_HEI_E3S 19.82
+#define MC_HEI_E1P 21.22
 
 // Learned warm start: network sizes (the FP32 parameter blob length
 // MC_MLP_NPARAM is part of the C-ABI)
@@ -313,10 +328,13 @@ struct mc_ctx {
     bool state_ready;
     DevCool cool;
     bool cool_ready;
-    // diffuse field: Milne tables, per-leaf emission {ch0, ch1, ch2, total},
-    // leaf CDF, block sums of the scan, totals of the last build
+    // diffuse field: Milne tables, per-leaf emission {ch0, ch1, ch2, ch3,
+    // total}, leaf CDF, block sums of the scan, totals of the last build
     double *d_milne;          // cdf | xbar | fin | xg
     DevMilne milne;
+    double *d_hei;            // two-photon grid_y | cdf_e
+    double hei_nper, hei_eper;
+    bool hei_ready;
     double *d_dif, *d_dcdf, *d_bsum;
     int nbsum;
     double diffuse_lum;
@@ -383,6 +401,8 @@ void mc_set_error(const std::string &msg);
 void mc_gff_table(double logT[41], double gff[4][41]);
 void mc_milne_tables(double eion_max, double *cdf, double *xbar, double *fin, double *xg,
                      double *lgT);
+int mc_hei2ph_tables(int n, const double *y, const double *A, double eion_max, double *grid_y,
+                     double *cdf_e, double *n_per, double *e_per);
 
 // device work (mc_kernels.hip)
 int mc_run_pass(mc_ctx *c);

@@ -2045,17 +2045,74 @@ __device__ double ground_photon_energy(const DevMilne &M, int ich, double T, dou
     return fmin(E, M.eion_max * 0.999);
 }
 
-// diffuse_build, channels 1-3: per-leaf energy luminosity of the ground
-// continua n_e n_ion alpha_1(T) E_rec(T) V [erg/s] and the leaf total
+// ---- He I excited-level recombination (par%hei_diffuse) --------------------
+// Every case-B He II recombination cascades to n = 2 and decays through
+// 2^3S (19.82 eV line), 2^1P (21.22 eV, 584 A) or 2^1S (two-photon pair);
+// only the H-ionizing part of each branch enters the band.
+
+// alpha_fit and hei_branch_fractions (src/hei_cascade_mod.f90:62-81): the
+// effective recombination coefficients into each branch, power laws in T
+// clamped to [3e3, 2e4] K, as fractions of their sum
+__device__ void hei_branch_fractions(double T, double &f3s, double &f1s, double &f1p)
+{
+#pragma clang fp contract(off)
+    const double t = fmin(fmax(T, 3.0e3), 2.0e4) / 1.0e4;
+    const double a3 = 2.10e-13 * pow(t, -0.381);
+    const double a1s = 2.06e-14 * pow(t, -0.451);
+    const double a1p = 4.17e-14 * pow(t, -0.695);
+    const double sum = (a3 + a1s) + a1p;
+    f3s = a3 / sum;
+    f1s = a1s / sum;
+    f1p = a1p / sum;
+}
+
+// hei_branch_energies (src/diffuse_mod.f90:308-324) with hei_584_ionizes_H
+// (src/hei_cascade_mod.f90:115-119): energy radiated inside the band per He II
+// recombination by each branch [eV].  The trapped 584 A photon ionizes H with
+// probability 1 / (1 + 77 T^-1/2 x_HeI / x_HI); otherwise it is converted to
+// 2^1S and leaves as the pair continuum.
+__device__ void hei_branch_energies(const DevMilne &M, double T, double xHI, double xHeI,
+                                    double &e3s, double &e1p, double &e1s)
+{
+#pragma clang fp contract(off)
+    double f3s, f1s, f1p;
+    hei_branch_fractions(T, f3s, f1s, f1p);
+    const double p584 = 1.0 / (1.0 + ((77.0 / sqrt(fmax(T, DBL_MIN))) * fmax(xHeI, 0.0)) /
+                                         fmax(xHI, DBL_MIN));
+    e3s = f3s * MC_HEI_E3S;
+    e1p = (f1p * p584) * MC_HEI_E1P;
+    e1s = (f1s + f1p * (1.0 - p584)) * M.hei_eper;
+}
+
+// hei_2ph_sample_energy (src/hei_twophoton_mod.f90:190-211): inverse of the
+// energy-weighted pair CDF on its 1024 nodes, linear inside a node [eV]
+__device__ double hei_pair_energy(const DevMilne &M, double u)
+{
+#pragma clang fp contract(off)
+    const double uu = fmin(fmax(u, 0.0), 1.0);
+    int lo = 0, hi = MC_HEI2PH_NCDF - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (uu > M.hei_cdf[mid]) lo = mid; else hi = mid;
+    }
+    const double ca = M.hei_cdf[lo], cb = M.hei_cdf[hi], ya = M.hei_y[lo];
+    const double yv = cb > ca ? ya + ((M.hei_y[hi] - ya) * (uu - ca)) / (cb - ca) : ya;
+    return yv * MC_HEI_E_TOT;
+}
+
+// diffuse_build, channels 1-4 (src/diffuse_mod.f90:60-121): per-leaf energy
+// luminosity of the ground continua n_e n_ion alpha_1(T) E_rec(T) V and of the
+// He I excited channel n_e n_HeII alpha_B(T) V sum(E_branch) [erg/s], and the
+// leaf total; 5 values per leaf
 __global__ __launch_bounds__(kBlock) void k_diffuse_build(DevGrid g, DevState s, DevMilne M,
                                                           double y, double d2cm, double *dif)
 {
 #pragma clang fp contract(off)
     const int il = blockIdx.x * kBlock + threadIdx.x;
     if (il >= g.nleaf) return;
-    double *o = dif + (size_t)il * 4;
+    double *o = dif + (size_t)il * 5;
     const double nH = s.nH[il];
-    if (nH <= 0.0) { o[0] = o[1] = o[2] = o[3] = 0.0; return; }
+    if (nH <= 0.0) { o[0] = o[1] = o[2] = o[3] = o[4] = 0.0; return; }
     const double T = s.Te[il], ne = s.ne[il];
     const double sd = (2.0 * leaf_half(g, il)) * d2cm, vol = (sd * sd) * sd;
     const double x3 = fmax(0.0, (1.0 - s.xHeI[il]) - s.xHeII[il]);
@@ -2066,7 +2123,14 @@ __global__ __launch_bounds__(kBlock) void k_diffuse_build(DevGrid g, DevState s,
     o[0] = ((((ne * nHII) * a1) * ground_mean_energy(M, 0, T)) * MC_EV2ERG) * vol;
     o[1] = ((((ne * nHeII) * a2) * ground_mean_energy(M, 1, T)) * MC_EV2ERG) * vol;
     o[2] = ((((ne * nHeIII) * a3) * ground_mean_energy(M, 2, T)) * MC_EV2ERG) * vol;
-    o[3] = ((o[0] + o[1]) + o[2]) + 0.0;
+    double c3 = 0.0;
+    if (M.hei) {
+        double e3s, e1p, e1s;
+        hei_branch_energies(M, T, s.xHI[il], s.xHeI[il], e3s, e1p, e1s);
+        c3 = ((((ne * nHeII) * rB.aHe2) * vol) * MC_EV2ERG) * ((e3s + e1p) + e1s);
+    }
+    o[3] = c3;
+    o[4] = ((o[0] + o[1]) + o[2]) + c3;
 }
 
 // leaf CDF of the diffuse emission: inclusive scan of the leaf totals in
@@ -2138,6 +2202,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(double *cdf, int n, const
 // gen_diffuse_photon_qmc (src/diffuse_mod.f90:221-296): 9 Sobol uniforms of
 // the diffuse stream; emitting leaf from the leaf CDF, uniform position in
 // it, isotropic direction, channel by luminosity, Milne energy at the leaf T
+// or, in the He I channel, the decay branch by energy and the line or pair
+// energy
 __global__ __launch_bounds__(kBlock) void k_launch_diffuse(KArgs a)
 {
 #pragma clang fp contract(off)
@@ -2171,10 +2237,19 @@ __global__ __launch_bounds__(kBlock) void k_launch_diffuse(KArgs a)
     const double cost = 2.0 * u[4] - 1.0, sint = sqrt(1.0 - cost * cost);
     double sp, cp;
     sincos(MC_TWOPI * u[5], &sp, &cp);
-    const double *w = a.dif + (size_t)il * 4;
-    const double uu = u[6] * (((w[0] + w[1]) + w[2]) + 0.0);
-    const int ch = uu <= w[0] ? 0 : (uu <= w[0] + w[1] ? 1 : 2);
-    const double E = ground_photon_energy(a.milne, ch, a.s.Te[il], u[7]);
+    const double *w = a.dif + (size_t)il * 5;
+    const double uu = u[6] * (((w[0] + w[1]) + w[2]) + w[3]);
+    const int ch = uu <= w[0] ? 0 : (uu <= w[0] + w[1] ? 1 : (uu <= (w[0] + w[1]) + w[2] ? 2 : 3));
+    const double Te = a.s.Te[il];
+    double E;
+    if (ch < 3) {
+        E = ground_photon_energy(a.milne, ch, Te, u[7]);
+    } else {
+        double e3s, e1p, e1s;
+        hei_branch_energies(a.milne, Te, a.s.xHI[il], a.s.xHeI[il], e3s, e1p, e1s);
+        const double ub = u[7] * ((e3s + e1p) + e1s);
+        E = ub <= e3s ? MC_HEI_E3S : (ub <= e3s + e1p ? MC_HEI_E1P : hei_pair_energy(a.milne, u[8]));
+    }
     store_packet(a, i, x, y, z, sint * cp, sint * sp, cost, E, il);
 }
 
@@ -3153,6 +3228,10 @@ int sum_partials(mc_ctx *c, int nblk, double out[3])
 // ---------------------------------------------------------------------------
 int mc_run_pass(mc_ctx *c)
 {
+    if (c->p.diffuse_field && c->p.hei_diffuse && !c->hei_ready) {
+        mc_set_error("hei_diffuse needs the He I two-photon table (mc_diffuse_hei_setup)");
+        return -1;
+    }
     KArgs a = make_args(c);
     const size_t nl = (size_t)c->nleaf;
     hipStream_t st = c->stream;
@@ -3253,7 +3332,7 @@ int mc_run_pass(mc_ctx *c)
             c->grid, c->st, c->milne, c->p.He_abund, c->p.distance2cm, c->d_dif);
         MC_HIP(hipGetLastError());
         const int nb = (nl1 + kScanTile - 1) / kScanTile;
-        k_scan_tiles<<<nb, kBlock, 0, st>>>(c->d_dif + 3, 4, nl1, c->d_dcdf, c->d_bsum);
+        k_scan_tiles<<<nb, kBlock, 0, st>>>(c->d_dif + 4, 5, nl1, c->d_dcdf, c->d_bsum);
         k_scan_sums<<<1, 1024, 0, st>>>(c->d_bsum, nb);
         k_scan_apply<<<(nl1 + kBlock - 1) / kBlock, kBlock, 0, st>>>(c->d_dcdf, nl1, c->d_bsum, nb);
         MC_HIP(hipGetLastError());

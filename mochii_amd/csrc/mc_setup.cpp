@@ -549,3 +549,71 @@ void mc_milne_tables(double eion_max, double *cdf, double *xbar, double *fin, do
         }
     }
 }
+
+// hei_2ph_setup (src/hei_twophoton_mod.f90:98-183): the He I 2^1S pair
+// spectrum A(y), y = E / 20.62 eV, linear between table rows and zero
+// outside them (shape_A, :81-92).  Simpson over each table interval gives the
+// normalisation (photons per decay = 2 over the whole pair spectrum); Simpson
+// over 1023 equal panels of the H-ionizing window gives the in-window photon
+// and energy moments and the cumulative energy-weighted density y A(y) that
+// the sampler inverts.  Same operation order as the reference, contraction
+// off, so the moments agree with it to the last bit.
+namespace {
+struct Hei2phShape {
+    int n;
+    const double *y, *A;
+    double operator()(double v) const
+    {
+        if (v <= y[0] || v >= y[n - 1]) return 0.0;
+        int i = 0;
+        while (i < n - 2 && y[i + 1] < v) ++i;
+        return A[i] + ((A[i + 1] - A[i]) * (v - y[i])) / (y[i + 1] - y[i]);
+    }
+};
+}  // namespace
+
+int mc_hei2ph_tables(int n, const double *y, const double *A, double eion_max, double *grid_y,
+                     double *cdf_e, double *n_per, double *e_per)
+{
+    if (n < 4 || n > MC_HEI2PH_MAXPTS) return -1;
+    for (int i = 0; i + 1 < n; ++i)
+        if (!(y[i + 1] > y[i])) return -2;
+    const Hei2phShape f{n, y, A};
+    double tot = 0.0;
+    for (int i = 0; i + 1 < n; ++i) {
+        const double ya = y[i], yb = y[i + 1], ym = 0.5 * (ya + yb);
+        tot = tot + ((yb - ya) / 6.0) * ((A[i] + 4.0 * f(ym)) + A[i + 1]);
+    }
+    if (!(tot > 0.0)) return -3;
+    const int N = MC_HEI2PH_NCDF;
+    const double y_lo = MC_ETH_HI / MC_HEI_E_TOT;
+    const double y_hi = std::min(MC_HEI_E_TOT, eion_max) / MC_HEI_E_TOT;
+    *n_per = 0.0;
+    *e_per = 0.0;
+    if (y_hi <= y_lo) {            // window closed by the band edge
+        for (int i = 0; i < N; ++i) {
+            grid_y[i] = 0.0;
+            cdf_e[i] = 1.0;
+        }
+        return 0;
+    }
+    const double span = y_hi - y_lo;
+    auto node = [&](int i) { return y_lo + (span * (double)i) / (double)(N - 1); };
+    double acc_n = 0.0, acc_e = 0.0, seg = 0.0;
+    grid_y[0] = y_lo;
+    cdf_e[0] = 0.0;
+    for (int i = 1; i < N; ++i) {
+        const double ya = node(i - 1), yb = node(i), ym = 0.5 * (ya + yb), h6 = (yb - ya) / 6.0;
+        const double fa = f(ya), fm = f(ym), fb = f(yb);
+        acc_n = acc_n + h6 * ((fa + 4.0 * fm) + fb);
+        acc_e = acc_e + (h6 * (((fa * ya) + (4.0 * fm) * ym) + fb * yb)) * MC_HEI_E_TOT;
+        seg = seg + h6 * (((fa * ya) + 4.0 * (fm * ym)) + fb * yb);
+        grid_y[i] = yb;
+        cdf_e[i] = seg;
+    }
+    *n_per = (2.0 * acc_n) / tot;
+    *e_per = (2.0 * acc_e) / tot;
+    for (int i = 0; i < N; ++i) cdf_e[i] = seg > 0.0 ? cdf_e[i] / seg : 1.0;
+    cdf_e[N - 1] = 1.0;
+    return 0;
+}

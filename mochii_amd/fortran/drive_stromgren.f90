@@ -150,6 +150,7 @@ contains
     call off('ion_metal_abs', c_loc(p%ion_metal_abs))
     call off('cooling_model', c_loc(p%cooling_model))
     call off('dust_niter', c_loc(p%dust_niter));  call off('dust_tol', c_loc(p%dust_tol))
+    call off('hei_diffuse', c_loc(p%hei_diffuse))
   contains
     subroutine off(name, q)
       character(len=*), intent(in) :: name

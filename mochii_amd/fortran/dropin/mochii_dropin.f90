@@ -201,6 +201,7 @@ contains
     end if
     p%dust_niter = par%dust_niter;  p%dust_no_photons = par%dust_no_photons
     p%dust_tol = par%dust_tol
+    p%hei_diffuse = lg(par%diffuse_field .and. par%hei_diffuse)
     ! metal thresholds of the active registry, in registry order (src/main.f90:58-82)
     p%nmetal = 0
     if (par%use_metals .and. allocated(m_el)) then
@@ -246,6 +247,7 @@ contains
        call mochii_check(mc_species_setup(mochii_ctx, int(size(m_el), c_int), m_el), &
                          'mc_species_setup')
     end if
+    if (p%hei_diffuse /= 0) call hei_two_photon_load()
     call mochii_check(mc_gas_state_setup(mochii_ctx, real(s_nH, c_double), c_null_ptr), &
                       'mc_gas_state_setup')
     call mochii_push_state()
@@ -318,6 +320,27 @@ contains
        s_t_dust = real(t, wp)
     end if
   end subroutine mochii_pull_dust
+
+  ! hei_2ph_setup's read (src/hei_twophoton_mod.f90:105-121): the rows of
+  ! par%data_dir/HeI2phot.dat up to the first that is not two numbers
+  subroutine hei_two_photon_load()
+    integer, parameter :: maxpts = 128
+    real(c_double) :: ty(maxpts), ta(maxpts), ya, fa
+    integer :: unit, ios, n
+    open(newunit=unit, file=trim(par%data_dir)//'/HeI2phot.dat', status='old', iostat=ios)
+    if (ios /= 0) call fatal('cannot open '//trim(par%data_dir)//'/HeI2phot.dat (par%data_dir)')
+    n = 0
+    do
+       read(unit, *, iostat=ios) ya, fa
+       if (ios /= 0) exit
+       n = n + 1
+       if (n > maxpts) call fatal('HeI2phot.dat has more than 128 rows')
+       ty(n) = ya;  ta(n) = fa
+    end do
+    close(unit)
+    call mochii_check(mc_diffuse_hei_setup(mochii_ctx, int(n, c_int), ty, ta), &
+                      'mc_diffuse_hei_setup')
+  end subroutine hei_two_photon_load
 
   subroutine mochii_check(rc, what)
     integer(c_int), intent(in) :: rc

@@ -53,6 +53,7 @@ module mochii_gpu
      integer(c_int32_t) :: charge_exchange, ci_model, cooling_model
      integer(c_int32_t) :: dust_niter
      real(c_double)     :: dust_no_photons, dust_tol
+     integer(c_int32_t) :: hei_diffuse
   end type mc_params
 
   ! one registry element (read_element + cooling_load); C arrays [stage][k]
@@ -166,6 +167,14 @@ module mochii_gpu
        integer(c_int64_t), intent(out) :: nphot
        integer(c_int) :: rc
      end function mc_diffuse_info
+
+     function mc_diffuse_hei_setup(ctx, n, y, A) bind(C, name='mc_diffuse_hei_setup') result(rc)
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n
+       real(c_double), intent(in) :: y(*), A(*)
+       integer(c_int) :: rc
+     end function mc_diffuse_hei_setup
 
      function mc_gas_equilibrium_update(ctx, max_dx, dx_vol) &
           bind(C, name='mc_gas_equilibrium_update') result(rc)
@@ -337,6 +346,7 @@ module mochii_gpu
   public :: mc_last_error, mc_init, mc_finalize, mc_grid_octree, mc_gas_state_setup
   public :: mc_state_set, mc_state_get, mc_ion_dust_table, mc_gas_opacity_fill
   public :: mc_ionizing_field_and_rates, mc_diffuse_info, mc_gas_equilibrium_update
+  public :: mc_diffuse_hei_setup
   public :: mc_cooling_setup, mc_gas_thermal_update, mc_set_warm_start
   public :: mc_hydrogen_balance_residual_grid, mc_tallies_get, mc_rates_get
   public :: mc_energy_ledger, mc_info, mc_set_precision
@@ -371,6 +381,7 @@ contains
     p%charge_exchange = MC_CX_TWO_WAY;  p%ci_model = MC_CI_VORONOV
     p%cooling_model = MC_COOL_LOW_DENSITY
     p%dust_niter = 2;  p%dust_no_photons = 1d6;  p%dust_tol = 1d-3
+    p%hei_diffuse = 0
   end function mc_params_default
 
   ! mc_last_error() as a Fortran string (for fatal_error).

@@ -55,6 +55,7 @@ class Params(C.Structure):
         ("metal_freefree", C.c_int32), ("charge_exchange", C.c_int32), ("ci_model", C.c_int32),
         ("cooling_model", C.c_int32),
         ("dust_niter", C.c_int32), ("dust_no_photons", C.c_double), ("dust_tol", C.c_double),
+        ("hei_diffuse", C.c_int32),
     ]
 
     # reference defaults (src/define.f90) for the fields this path reads
@@ -70,7 +71,7 @@ class Params(C.Structure):
                     diffuse_field=0, diffuse_model=MC_DIFFUSE_MILNE, nmetal=0, ion_metal_abs=1,
                     metal_ne=1, metal_heat=1, metal_freefree=1, charge_exchange=MC_CX_TWO_WAY,
                     ci_model=MC_CI_VORONOV, cooling_model=MC_COOL_LOW_DENSITY,
-                    dust_niter=2, dust_no_photons=1.0e6, dust_tol=1.0e-3)
+                    dust_niter=2, dust_no_photons=1.0e6, dust_tol=1.0e-3, hei_diffuse=0)
 
     @classmethod
     def make(cls, **kw):
@@ -124,6 +125,8 @@ def library():
         "mc_gas_equilibrium_update": ([vp, dp, dp], C.c_int),
         "mc_hydrogen_balance_residual_grid": ([vp, dp, dp], C.c_int),
         "mc_diffuse_info": ([vp, dp, i64p], C.c_int),
+        "mc_diffuse_hei_setup": ([vp, C.c_int, dp, dp], C.c_int),
+        "mc_diffuse_hei_info": ([vp, dp, dp, dp, dp], C.c_int),
         "mc_set_warm_start": ([vp, C.c_int], C.c_int),
         "mc_set_warm_levels": ([vp, C.c_int], C.c_int),
         "mc_nlevel_setup": ([vp, C.c_int, vp], C.c_int),

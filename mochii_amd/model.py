@@ -37,6 +37,8 @@ class Model:
         check(L.mc_init(C.byref(self.params), C.byref(self._h)), "mc_init")
         self._L = L
         self._refresh_info()
+        if self.params.diffuse_field and self.params.hei_diffuse:
+            self.diffuse_hei_setup()       # src/main.f90:107-112
 
     def _refresh_info(self):
         n, nnu, lt = C.c_int32(), C.c_int32(), C.c_double()
@@ -208,6 +210,29 @@ class Model:
         L, n = C.c_double(), C.c_int64()
         check(self._L.mc_diffuse_info(self._h, C.byref(L), C.byref(n)), "mc_diffuse_info")
         return L.value, n.value
+
+    def diffuse_hei_setup(self, y=None, A=None):
+        """hei_2ph_setup: the He I 2^1S two-photon shape (default: the data
+        snapshot's HeI2phot.dat rows) for channel 4 of the diffuse field."""
+        if y is None or A is None:
+            from .atomic import load_hei_two_photon
+            y, A = load_hei_two_photon()
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        if y.shape != A.shape:
+            raise ValueError("diffuse_hei_setup: y and A differ in length")
+        check(self._L.mc_diffuse_hei_setup(self._h, y.size, _dp(y), _dp(A)),
+              "mc_diffuse_hei_setup")
+
+    def diffuse_hei_info(self):
+        """(photons per decay, eV per decay, grid_y[1024], cdf_e[1024]) of the
+        two-photon window."""
+        g = np.zeros(1024)
+        c = np.zeros(1024)
+        n, e = C.c_double(), C.c_double()
+        check(self._L.mc_diffuse_hei_info(self._h, C.byref(n), C.byref(e), _dp(g), _dp(c)),
+              "mc_diffuse_hei_info")
+        return n.value, e.value, g, c
 
     def gas_equilibrium_update(self):
         a, b = C.c_double(), C.c_double()

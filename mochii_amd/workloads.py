@@ -200,20 +200,28 @@ def hii_octree(n_field=128, nphotons=10_000_000, precision=MC_FP64, seed=12345, 
     return params, nHl, (xl, yl, zl, lev, half)
 
 
-def lexington_hii40(nphotons=40_000_000, precision=MC_FP64, seed=1234, level=6):
-    """The reference's Lexington HII40 benchmark input (tests/g2_hii/hii40_bench.in,
-    grid of tests/g2_hii/make_hii_grid.py): a uniform level-`level` octree over
-    [-R_out, R_out]^3 pc with n_H = 100 cm^-3 in the hollow shell
-    R_in = 3e18 cm <= r < R_out = 1.46e19 cm and 0 elsewhere, a 40000 K
-    blackbody of band luminosity 1.2734e39 erg/s (13.598-100 eV) at the
-    centre, 32 threshold-aligned band bins, He/H = 0.1 and C N O Ne S at
-    2.2e-4 / 4e-5 / 3.3e-4 / 5e-5 / 9e-6 (the reference defaults) without the
-    metal electrons and photoheating, thermal balance in [3e3, 3e4] K with
-    'local_ne' cooling, the explicit diffuse field (case A, Milne spectra),
-    ionised start, 'vol' convergence at gas_tol 5e-3 / gas_tol_te 1e-2.
-    Returns (params, nH, octree, elements)."""
+# the two Lexington benchmark inputs (tests/g2_hii/hii20_bench.in and
+# hii40_bench.in): blackbody T [K], band luminosity 13.598-100 eV [erg/s],
+# outer radius of the shell [cm]
+LEXINGTON = {"hii20": (2.0e4, 2.5326e38, 9.5e18), "hii40": (4.0e4, 1.2734e39, 1.46e19)}
+
+
+def lexington(model="hii40", nphotons=40_000_000, precision=MC_FP64, seed=1234, level=6):
+    """The reference's Lexington benchmark inputs (tests/g2_hii/hii20_bench.in,
+    hii40_bench.in; grid of tests/g2_hii/make_hii_grid.py): a uniform
+    level-`level` octree over [-R_out, R_out]^3 pc with n_H = 100 cm^-3 in the
+    hollow shell R_in = 3e18 cm <= r < R_out and 0 elsewhere, a blackbody at
+    the centre (HII20: 20000 K, 2.5326e38 erg/s in 13.598-100 eV, R_out =
+    9.5e18 cm; HII40: 40000 K, 1.2734e39 erg/s, R_out = 1.46e19 cm), 32
+    threshold-aligned band bins, He/H = 0.1 and C N O Ne S at 2.2e-4 / 4e-5 /
+    3.3e-4 / 5e-5 / 9e-6 (the reference defaults) without the metal electrons
+    and photoheating, thermal balance in [3e3, 3e4] K with 'local_ne'
+    cooling, the explicit diffuse field (case A, Milne spectra) with the He I
+    excited-level channel (par%hei_diffuse), ionised start, 'vol' convergence
+    at gas_tol 5e-3 / gas_tol_te 1e-2.  Returns (params, nH, octree, elements)."""
     from .atomic import metal_thresholds, registry
-    rin, rout = 3.0e18 / PC2CM, 1.46e19 / PC2CM
+    tstar, lum, rout_cm = LEXINGTON[model]
+    rin, rout = 3.0e18 / PC2CM, rout_cm / PC2CM
     n = 2 ** level
     c = -rout + (np.arange(n) + 0.5) * (2.0 * rout / n)
     X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
@@ -223,15 +231,20 @@ def lexington_hii40(nphotons=40_000_000, precision=MC_FP64, seed=1234, level=6):
     lev = np.full(x.size, level, dtype=np.int32)
     els = registry({})
     params = dict(nphotons=nphotons, qmc_seed=seed, precision=precision, eion_min=13.598,
-                  eion_max=100.0, nnu_ion=32, ion_align_edges=1, nsource=1, tstar=[4.0e4],
-                  src_lum=[1.2734e39], src_x=[0.0], src_y=[0.0], src_z=[0.0], grid_type=1,
+                  eion_max=100.0, nnu_ion=32, ion_align_edges=1, nsource=1, tstar=[tstar],
+                  src_lum=[lum], src_x=[0.0], src_y=[0.0], src_z=[0.0], grid_type=1,
                   nx=1, ny=1, nz=1, xmax=rout, ymax=rout, zmax=rout, distance2cm=PC2CM,
                   He_abund=0.1, te_fixed=1.0e4, ion_relax=1.0, te_min=3.0e3, te_max=3.0e4,
                   gas_tol=5.0e-3, gas_tol_te=1.0e-2, conv_vol=1, gas_niter=100, case_A=1,
                   solve_te=1, xHI_init=1.0e-3, xHeI_init=0.5, xHeII_init=0.5, diffuse_field=1,
-                  diffuse_model=2, cooling_model=int(MC_COOL_LOCAL_NE),
+                  diffuse_model=2, hei_diffuse=1, cooling_model=int(MC_COOL_LOCAL_NE),
                   metal_eth=metal_thresholds(els), metal_ne=0, metal_heat=0)
     return params, nH, (x, y, z, lev, rout), els
+
+
+def lexington_hii40(nphotons=40_000_000, precision=MC_FP64, seed=1234, level=6):
+    """lexington("hii40", ...)."""
+    return lexington("hii40", nphotons=nphotons, precision=precision, seed=seed, level=level)
 
 
 def ionised_weighted_te(state, nH):

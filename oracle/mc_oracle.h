@@ -139,7 +139,20 @@ typedef struct {
     int     ion_metal_abs, metal_ne, metal_heat, metal_freefree;
     int     cx_two_way;              /* par%charge_exchange == 'two_way' */
     int     ci_dere;                 /* par%ci_model == 'dere_hybrid'    */
+    int     hei_diffuse;             /* par%hei_diffuse: He I excited channel */
 } orc_params;
+
+/* He I 2^1S two-photon continuum above the H I edge
+ * (src/hei_twophoton_mod.f90:44-56): the tabulated shape, photons and energy
+ * per decay in the ionizing window, the energy-weighted sampling CDF */
+#define ORC_HEI2PH_MAXPTS 128
+#define ORC_HEI2PH_NCDF 1024
+typedef struct {
+    int    ready, n_pts;
+    double tab_y[ORC_HEI2PH_MAXPTS], tab_A[ORC_HEI2PH_MAXPTS];
+    double n_per_decay, e_per_decay, y_lo, y_hi;
+    double grid_y[ORC_HEI2PH_NCDF], cdf_e[ORC_HEI2PH_NCDF];
+} orc_hei2ph;
 
 /* ---- trace-metal registry (src/species_mod.f90:54-128) -------------------
  * One element as read_element fills it, laid out exactly like mc_element of
@@ -251,8 +264,10 @@ typedef struct {
     /* cooling tables */
     orc_cooling cool;
     /* diffuse field: Milne tables, per-leaf channel luminosities dif_ch
-     * (leaf-major, 3 per leaf), leaf CDF, totals of the last build */
+     * (leaf-major, 4 per leaf: H I, He I, He II ground continua, He I
+     * excited cascade), leaf CDF, totals of the last build */
     orc_milne *milne;
+    orc_hei2ph *hei;                 /* He I two-photon shape (par%hei_diffuse) */
     double *dif_ch, *dif_cdf, diffuse_lum;
     int64_t diffuse_nphot;
     /* trace metals (species_mod module state): registry, element-order
@@ -425,6 +440,15 @@ int      orc_launch(orc_ctx *c, int64_t id, double *out9);
 int      orc_diffuse_setup(orc_ctx *c);
 void     orc_diffuse_build(orc_ctx *c);
 int      orc_launch_diffuse(orc_ctx *c, int64_t id, double *out9);
+/* He I excited channel of the diffuse field (par%hei_diffuse) */
+void     orc_hei_branch_fractions(double T, double *f3s, double *f1s, double *f1p);
+double   orc_hei_584_ionizes_H(double T, double xHI, double xHeI);
+int      orc_hei2ph_setup(orc_hei2ph *h, int n, const double *y, const double *A,
+                          double eion_max);
+double   orc_hei2ph_sample_energy(const orc_hei2ph *h, double u);
+void     orc_hei_branch_energies(const orc_hei2ph *h, double T, double xHI, double xHeI,
+                                 double *e3s, double *e1p, double *e1s);
+int      orc_diffuse_hei_setup(orc_ctx *c, int n, const double *y, const double *A);
 
 #ifdef __cplusplus
 }

@@ -42,7 +42,7 @@ void orc_destroy(orc_ctx *c)
     free(c->dlam); free(c->dalb); free(c->dg); free(c->dcext);
     free(c->hhe); free(c->jt); free(c->dust_abs);
     free(c->gHI); free(c->gHeI); free(c->gHeII); free(c->hHI); free(c->hHeI); free(c->hHeII);
-    free(c->milne); free(c->dif_ch); free(c->dif_cdf);
+    free(c->milne); free(c->hei); free(c->dif_ch); free(c->dif_cdf);
     free(c->mraw); free(c->mgam); free(c->mheat); free(c->mfrac);
     orc_dustir_free(c->ir);
     free(c);

@@ -629,7 +629,7 @@ static void launch(const orc_ctx *c, int64_t id, pkt_t *ph)
 }
 
 /* gen_diffuse_photon_qmc (src/diffuse_mod.f90:221-296) with
- * ground_photon_energy (:341-355); channels 1-3, 9 uniforms on stream 2 */
+ * ground_photon_energy (:341-355); channels 1-4, 9 uniforms on stream 2 */
 static void launch_diffuse(const orc_ctx *c, int64_t id, pkt_t *ph)
 {
     double u[ORC_QMC_MAXDIM];
@@ -649,21 +649,33 @@ static void launch_diffuse(const orc_ctx *c, int64_t id, pkt_t *ph)
     ph->kx = sint * cos(phi);
     ph->ky = sint * sin(phi);
     ph->kz = cost;
-    const double *w = c->dif_ch + (size_t)(il - 1) * 3;
-    double uu = u[6] * (((w[0] + w[1]) + w[2]) + 0.0);
+    const double *w = c->dif_ch + (size_t)(il - 1) * 4;
+    double uu = u[6] * (((w[0] + w[1]) + w[2]) + w[3]);
     int ch;
     double eph;
     if (uu <= w[0]) { ch = 0; eph = ORC_ETH_HI; }
     else if (uu <= w[0] + w[1]) { ch = 1; eph = ORC_ETH_HEI; }
-    else { ch = 2; eph = ORC_ETH_HEII; }
+    else if (uu <= (w[0] + w[1]) + w[2]) { ch = 2; eph = ORC_ETH_HEII; }
+    else { ch = 3; eph = 0.0; }
     const double T = c->Te[il - 1];
-    if (c->p.diffuse_model == 2) {
-        eph = orc_milne_sample_energy(c->milne, ch, T, u[7]);
-    } else if (c->p.diffuse_model == 1) {
-        double kT = ORC_KBOLTZ * T / ORC_EV2ERG;
-        eph = eph + kT * (-log(u[7] > DBL_MIN ? u[7] : DBL_MIN));
+    if (ch <= 2) {
+        if (c->p.diffuse_model == 2) {
+            eph = orc_milne_sample_energy(c->milne, ch, T, u[7]);
+        } else if (c->p.diffuse_model == 1) {
+            double kT = ORC_KBOLTZ * T / ORC_EV2ERG;
+            eph = eph + kT * (-log(u[7] > DBL_MIN ? u[7] : DBL_MIN));
+        }
+        if (eph > c->p.eion_max * 0.999) eph = c->p.eion_max * 0.999;
+    } else {
+        /* the He I cascade: decay branch by energy luminosity, fixed line
+         * energies, the two-photon energy from the tabulated shape */
+        double be3s, be1p, be1s;
+        orc_hei_branch_energies(c->hei, T, c->xHI[il - 1], c->xHeI[il - 1], &be3s, &be1p, &be1s);
+        const double ub = u[7] * ((be3s + be1p) + be1s);
+        if (ub <= be3s) eph = 19.82;
+        else if (ub <= be3s + be1p) eph = 21.22;
+        else eph = orc_hei2ph_sample_energy(c->hei, u[8]);
     }
-    if (eph > c->p.eion_max * 0.999) eph = c->p.eion_max * 0.999;
     ph->inu = orc_bin_of(c, eph);
     ph->wgt = 1.0;
     ph->Lpacket = c->Ltot / (double)c->p.nphotons;

@@ -43,6 +43,7 @@ class OrcParams(C.Structure):
         ("nmetal", C.c_int), ("metal_eth", C.c_double * 64),
         ("ion_metal_abs", C.c_int), ("metal_ne", C.c_int), ("metal_heat", C.c_int),
         ("metal_freefree", C.c_int), ("cx_two_way", C.c_int), ("ci_dere", C.c_int),
+        ("hei_diffuse", C.c_int),
     ]
 
     # metal switches at the reference defaults (src/define.f90:593-652, 801)
@@ -173,6 +174,18 @@ def _load():
     lib.orc_diffuse_setup.restype = C.c_int
     lib.orc_diffuse_build.argtypes = [C.c_void_p]
     lib.orc_launch_diffuse.argtypes = [C.c_void_p, C.c_int64, _dp]
+    lib.orc_diffuse_hei_setup.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    lib.orc_diffuse_hei_setup.restype = C.c_int
+    lib.orc_hei_branch_fractions.argtypes = [C.c_double, _dp, _dp, _dp]
+    lib.orc_hei_branch_fractions.restype = None
+    lib.orc_hei_584_ionizes_H.argtypes = [C.c_double] * 3
+    lib.orc_hei_584_ionizes_H.restype = C.c_double
+    lib.orc_hei2ph_setup.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_double]
+    lib.orc_hei2ph_setup.restype = C.c_int
+    lib.orc_hei2ph_sample_energy.argtypes = [C.c_void_p, C.c_double]
+    lib.orc_hei2ph_sample_energy.restype = C.c_double
+    lib.orc_hei_branch_energies.argtypes = [C.c_void_p] + [C.c_double] * 3 + [_dp] * 3
+    lib.orc_hei_branch_energies.restype = None
     lib.orc_transport_pass.restype = C.c_int
     vp = C.c_void_p
     lib.orc_species_setup.argtypes = [vp, C.c_int, vp]
@@ -618,8 +631,16 @@ class Model:
 
     def diffuse_tables(self):
         h = self._head()
-        return (np.ctypeslib.as_array(h.dif_ch, (self.nleaf * 3,)).reshape(self.nleaf, 3).copy(),
+        return (np.ctypeslib.as_array(h.dif_ch, (self.nleaf * 4,)).reshape(self.nleaf, 4).copy(),
                 np.ctypeslib.as_array(h.dif_cdf, (self.nleaf,)).copy())
+
+    def diffuse_hei_setup(self, y, A):
+        """The He I two-photon shape for the diffuse field's excited channel
+        (hei_2ph_setup, src/hei_twophoton_mod.f90:98-183)."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        if lib().orc_diffuse_hei_setup(self.c, y.size, ptr(y), ptr(A)):
+            raise RuntimeError("orc_diffuse_hei_setup failed")
 
     def launch_diffuse(self, pid):
         out = np.zeros(9)
@@ -736,7 +757,7 @@ class _CtxHead(C.Structure):
         ("e_emit", C.c_double), ("e_abs", C.c_double), ("e_esc", C.c_double),
         ("gHI", _dp), ("gHeI", _dp), ("gHeII", _dp), ("hHI", _dp), ("hHeI", _dp), ("hHeII", _dp),
         ("cool", Cooling),
-        ("milne", C.c_void_p), ("dif_ch", _dp), ("dif_cdf", _dp),
+        ("milne", C.c_void_p), ("hei", C.c_void_p), ("dif_ch", _dp), ("dif_cdf", _dp),
         ("diffuse_lum", C.c_double), ("diffuse_nphot", C.c_int64),
         ("use_metals", C.c_int), ("nel", C.c_int), ("ntr", C.c_int), ("nst", C.c_int),
         ("el", C.c_byte * (11 * 3648)),
@@ -787,6 +808,46 @@ def rng_uniform(seed, stream, pid, k):
 
 
 # ---- the reference's own modules compiled from source (optional) ----------
+class Hei2ph(C.Structure):
+    """Mirror of orc_hei2ph (the He I two-photon shape and its moments)."""
+    _fields_ = [("ready", C.c_int), ("n_pts", C.c_int),
+                ("tab_y", C.c_double * 128), ("tab_A", C.c_double * 128),
+                ("n_per_decay", C.c_double), ("e_per_decay", C.c_double),
+                ("y_lo", C.c_double), ("y_hi", C.c_double),
+                ("grid_y", C.c_double * 1024), ("cdf_e", C.c_double * 1024)]
+
+
+def hei_branch_fractions(T):
+    """(f_3S, f_1S, f_1P) of the He I cascade at T (src/hei_cascade_mod.f90:72-81)."""
+    f = np.zeros(3)
+    lib().orc_hei_branch_fractions(T, ptr(f[0:]), ptr(f[1:]), ptr(f[2:]))
+    return tuple(float(v) for v in f)
+
+
+def hei_584_ionizes_H(T, xHI, xHeI):
+    return float(lib().orc_hei_584_ionizes_H(T, xHI, xHeI))
+
+
+def hei_two_photon(y, A, eion_max=100.0):
+    """hei_2ph_setup of the given table: the Hei2ph record."""
+    h = Hei2ph()
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    if lib().orc_hei2ph_setup(C.byref(h), y.size, ptr(y), ptr(A), eion_max):
+        raise RuntimeError("orc_hei2ph_setup failed")
+    return h
+
+
+def hei2ph_sample_energy(h, u):
+    return float(lib().orc_hei2ph_sample_energy(C.byref(h), u))
+
+
+def hei_branch_energies(h, T, xHI, xHeI):
+    e = np.zeros(3)
+    lib().orc_hei_branch_energies(C.byref(h), T, xHI, xHeI, ptr(e[0:]), ptr(e[1:]), ptr(e[2:]))
+    return tuple(float(v) for v in e)
+
+
 def ref_lib():
     """The reference Fortran modules built into oracle/_ref, or None."""
     if not os.path.exists(REF_PATH):
@@ -810,6 +871,10 @@ def ref_lib():
     L.ref_rate.restype = C.c_double
     L.ref_gaunt_ff.argtypes = [C.c_float, C.c_float, C.POINTER(C.c_float),
                                C.POINTER(C.c_float), C.c_int]
+    L.ref_hei_branch_fractions.argtypes = [C.c_double, _dp, _dp, _dp]
+    L.ref_hei_branch_fractions.restype = None
+    L.ref_hei_584_ionizes_H.argtypes = [C.c_double] * 3
+    L.ref_hei_584_ionizes_H.restype = C.c_double
     L.ref_milne_setup.argtypes = [C.c_double] * 3
     L.ref_milne_sample.argtypes = [C.c_int, C.c_double, C.c_double]
     L.ref_milne_sample.restype = C.c_double

@@ -1,6 +1,7 @@
 ! C-callable entry points into the reference's own Fortran modules
 ! (qmc_mod, energy_sampler_mod, photo_xsec, recomb_mod, gaunt,
-! milne_recomb_spectrum_mod, nlevel_mod, nlevel_cooling_mod), compiled from
+! milne_recomb_spectrum_mod, nlevel_mod, nlevel_cooling_mod, hei_cascade_mod),
+! compiled from
 ! their sources under the reference checkout by oracle/Makefile.  Test
 ! infrastructure only: the tests use it to check the C oracle against the
 ! reference code itself.
@@ -18,6 +19,7 @@ module ref_bind
        nlevel_cooling_at, nlevel_nlines, nlevel_line_ident
   use nlevel_cooling_mod, only : nlevel_cooling_add, nlevel_cooling_apply, &
        nlevel_cooling_lam_se
+  use hei_cascade_mod, only : hei_branch_fractions, hei_584_ionizes_H
   implicit none
   type(energy_sampler_type), save :: smp
   type(nlevel_atom_type), save :: nla
@@ -236,5 +238,16 @@ contains
     real(c_double), value :: T, ne
     ref_nlevel_cooling_lam_se = nlevel_cooling_lam_se(int(idx), T, ne)
   end function ref_nlevel_cooling_lam_se
+
+  subroutine ref_hei_branch_fractions(T, f3s, f1s, f1p) bind(C, name='ref_hei_branch_fractions')
+    real(c_double), value :: T
+    real(c_double), intent(out) :: f3s, f1s, f1p
+    call hei_branch_fractions(T, f3s, f1s, f1p)
+  end subroutine ref_hei_branch_fractions
+
+  real(c_double) function ref_hei_584_ionizes_H(T, xHI, xHeI) bind(C, name='ref_hei_584_ionizes_H')
+    real(c_double), value :: T, xHI, xHeI
+    ref_hei_584_ionizes_H = hei_584_ionizes_H(T, xHI, xHeI)
+  end function ref_hei_584_ionizes_H
 
 end module ref_bind

@@ -18,6 +18,7 @@ _MAP = {  # library Params name -> oracle param name
     "diffuse_field": "diffuse_field", "diffuse_model": "diffuse_model",
     "metal_eth": "metal_eth", "ion_metal_abs": "ion_metal_abs", "metal_ne": "metal_ne",
     "metal_heat": "metal_heat", "metal_freefree": "metal_freefree",
+    "hei_diffuse": "hei_diffuse",
 }
 _ORC_DEFAULTS = dict(source_cdf_tol=1e-8, align_edges=1, lambda_ref=0.55, te_min=1e3,
                      te_max=5e4, ext_geometry=2)
@@ -47,6 +48,9 @@ def oracle_model(params, nH, rhokap=None, dust_table=None, octree=None, elements
                  dust_table=dust_table, elements=els)
     m.params.update(op)
     m.state_init(nH, rhokap=rhokap)
+    if op.get("diffuse_field") and op.get("hei_diffuse"):
+        from mochii_amd.atomic import load_hei_two_photon
+        m.diffuse_hei_setup(*load_hei_two_photon())
     if params.get("cooling_model") == 1:          # 'local_ne': the (T, n_e) tables
         from mochii_amd.atomic import load_nlevel
         m.nlevel_setup(load_nlevel())

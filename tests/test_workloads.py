@@ -1,5 +1,6 @@
 """Seeded synthetic inputs: the lognormal field and the octree leaf lists."""
 import numpy as np
+import pytest
 
 from mochii_amd.workloads import lognormal_field, octree_from_field, synthetic_kext
 
@@ -79,3 +80,28 @@ def test_large_workload_parameters():
     assert abs(np.log(nH).std() - 1.0) < 1e-12
     assert p["nsource"] == 4 and all(3.5e4 <= t <= 5.0e4 for t in p["tstar"])
     assert len(p["metal_eth"]) == 15
+
+
+@pytest.mark.parametrize("model,q_h", [("hii20", 1.004e49), ("hii40", 4.265e49)])
+def test_lexington_inputs(model, q_h):
+    """The band luminosities of tests/g2_hii/hii{20,40}_bench.in carry the
+    benchmark's ionizing photon rate (the inputs' own comments: Q_H = 1.004e49
+    and 4.265e49 s^-1): Q = L_band Int B_nu / h nu / Int B_nu over
+    13.598-100 eV; the hollow shell sits between 3e18 cm and R_out."""
+    from scipy.integrate import quad
+    from mochii_amd.workloads import LEXINGTON, PC2CM, lexington
+    params, nH, (x, y, z, lev, half), els = lexington(model)
+    T, L = params["tstar"][0], params["src_lum"][0]
+    kT = 1.380649e-16 * T / 1.602176634e-12
+
+    def planck(E, k):
+        return E ** (3 - k) / np.expm1(E / kT)
+    Q = L * quad(planck, 13.598, 100.0, args=(1,))[0] / quad(planck, 13.598, 100.0, args=(0,))[0]
+    Q /= 1.602176634e-12
+    assert abs(Q / q_h - 1.0) < 2e-3, Q
+    r = np.sqrt(x * x + y * y + z * z) * PC2CM
+    gas = nH > 0
+    assert half * PC2CM == pytest.approx(LEXINGTON[model][2], rel=1e-15)
+    assert r[gas].min() >= 3.0e18 and r[gas].max() < LEXINGTON[model][2]
+    assert np.all(nH[gas] == 100.0) and lev.min() == lev.max() == 6
+    assert params["hei_diffuse"] == 1 and params["diffuse_field"] == 1
