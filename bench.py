@@ -273,18 +273,22 @@ def timed_steps(name, params, nH, octree, rhokap, tab, args, mlp):
     return m, out
 
 
-def scattered_rmw_probe(rows, width, n_ops):
-    """The device's scattered FP64 read-modify-write rate (mc_probe_scattered_rmw):
-    n_ops updates of `width` doubles at hashed rows of a rows x width array."""
+def scattered_rmw_probe(rows, width, n_ops, together=False):
+    """The device's scattered FP64 read-modify-write rate: n_ops updates of
+    `width` doubles at hashed rows of a rows x width array, one lane adding
+    a row's doubles in turn (mc_probe_scattered_rmw: `width` atomic requests
+    per row) or `width` lanes adding them in one instruction
+    (mc_probe_row_atomics: one request per row)."""
     import ctypes as C
     from mochii_amd.lib import check, library
     ms = C.c_double()
-    check(library().mc_probe_scattered_rmw(int(rows), int(width), int(n_ops), C.byref(ms), None),
-          "mc_probe_scattered_rmw")
+    fn = library().mc_probe_row_atomics if together else library().mc_probe_scattered_rmw
+    check(fn(int(rows), int(width), int(n_ops), C.byref(ms), None), "scattered-RMW probe")
     t = ms.value * 1e-3
-    return dict(rows=int(rows), width=int(width), n_ops=int(n_ops), ms=ms.value,
-                rows_per_s=n_ops / t, atomics_per_s=n_ops * width / t,
-                bytes_per_s=n_ops * width * 8 / t)
+    req = n_ops if together else n_ops * width
+    return dict(rows=int(rows), width=int(width), n_ops=int(n_ops), together=bool(together),
+                ms=ms.value, rows_per_s=n_ops / t, atomics_per_s=req / t,
+                doubles_per_s=n_ops * width / t, bytes_per_s=n_ops * width * 8 / t)
 
 
 def ir_band(m, name, packets):
@@ -546,7 +550,8 @@ def main():
     probe = [scattered_rmw_probe(nH.size * nsub, 4, 4e9),
              scattered_rmw_probe(nH.size * nsub * 4, 1, 8e9),
              scattered_rmw_probe(100_000, 4, 4e9),
-             scattered_rmw_probe(4_000_000, 4, 4e9)]
+             scattered_rmw_probe(4_000_000, 4, 4e9),
+             scattered_rmw_probe(nH.size * nsub, 4, 4e9, together=True)]
     log(f"[probe] scattered RMW: {probe}")
     conv = None if args.no_converge else converge_leg(mlp, train_info)
     turb = None if args.no_turbulent else turbulent_leg(args, mlp)
@@ -622,14 +627,16 @@ def main():
                          "ceiling_atomics_per_s": max(p["atomics_per_s"] for p in probe),
                          "frac_of_ceiling": head["walk_atomics_per_s"] /
                                             max(p["atomics_per_s"] for p in probe),
-                         "basis": "global FP64 tally atomics counted by the walk "
+                         "basis": "global FP64 tally atomic requests (one per 64-byte line "
+                                  "an atomic instruction touches) counted by the walk "
                                   "(mc_walk_atomics) over its event-timed duration, against "
-                                  "the scattered-RMW probe's highest FP64 atomic rate"},
+                                  "the scattered-RMW probes' highest request rate"},
                      "scattered_rmw": {
                          "probe_rows_per_s": probe[0]["rows_per_s"],
                          "probe_atomics_per_s_single": probe[1]["atomics_per_s"],
                          "probe_atomics_per_s_3MB": probe[2]["atomics_per_s"],
                          "probe_atomics_per_s_128MB": probe[3]["atomics_per_s"],
+                         "probe_rows_per_s_four_lanes": probe[4]["rows_per_s"],
                          "walk_segments_per_s": nseg / (head["walk_ms"] * 1e-3),
                          "walk_frac_of_probe": nseg / (head["walk_ms"] * 1e-3) /
                                                probe[0]["rows_per_s"],

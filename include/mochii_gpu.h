@@ -247,6 +247,15 @@ int mc_finalize(mc_ctx *ctx);
 int mc_probe_scattered_rmw(int64_t rows, int width, int64_t n_ops, double *ms,
                            double *checksum);
 
+/* The same rows updated the way the walk's tally writers issue them: n_rows
+ * row updates, the `width` doubles of a row (1, 2, 4 or 8, so a row lies in
+ * one 64-byte line) added by `width` lanes of one instruction.  The device's
+ * atomic unit counts requests, one per line an instruction touches, so this
+ * completes about `width` times the doubles per second of
+ * mc_probe_scattered_rmw at the same row rate.  *ms, checksum (2 n_rows width)
+ * as there.  No counterpart in the reference. */
+int mc_probe_row_atomics(int64_t rows, int width, int64_t n_rows, double *ms, double *checksum);
+
 /* Leaf list of an AMR input file, generic_amr_read
  * (src/read_generic_amr.f90:34-237) with the file-name resolution of
  * io_resolve_filename (src/iofile_mod.f90:138-170): the text form ("nleaf
@@ -459,11 +468,13 @@ int mc_species_tallies_get(mc_ctx *ctx, int ie, double *raw);
  * counterpart in the reference. */
 int mc_metal_segment_transitions(mc_ctx *ctx, int64_t *sum_ntr);
 
-/* Global FP64 atomics the walk issued for its tallies in the last pass
- * (stellar and diffuse batches: rate sums, band J, grain absorption, metal
- * path tallies, energy moments; after the wave merges and the moment cache),
- * counted per lane on the device.  The walk's achieved atomic rate is this
- * over the walk time.  No counterpart in the reference. */
+/* Global FP64 atomic requests the walk issued for its tallies in the last
+ * pass (stellar and diffuse batches: rate sums, band J, grain absorption,
+ * metal path tallies, energy moments; after the wave merges and the moment
+ * cache): one per 64-byte line an atomic instruction touches, the unit the
+ * device's atomic throughput is counted in (the lanes of one instruction
+ * that add into one line cost one request).  The walk's achieved atomic rate
+ * is this over the walk time.  No counterpart in the reference. */
 int mc_walk_atomics(mc_ctx *ctx, int64_t *n);
 
 /* hydrogen_balance_residual_grid (src/ion_balance_mod.f90:313-343). */
@@ -511,6 +522,25 @@ int mc_set_packet_order(mc_ctx *ctx, int order);
 #define MC_LAUNCH_BATCH_MIXED ((int64_t)1 << 30)
 #define MC_LAUNCH_BATCH_FP64 ((int64_t)1 << 26)
 int mc_set_launch_batch(mc_ctx *ctx, int64_t max_records);
+
+/* Issue form of the mixed walk's energy-moment rows.  The device's atomic
+ * unit counts requests, one per 64-byte line an atomic instruction touches.
+ * MC_ROWS_LANE: each lane adds its row's MC_MOM_NM sums with atomics of its
+ * own (MC_MOM_NM requests a row; the fewest instructions and registers).
+ * MC_ROWS_TOGETHER: the rows of the wave's lanes go out transposed, several
+ * lanes adding one row in one instruction (one request a row).
+ * MC_ROWS_AUTO (default): per-lane until a pass's request rate comes near the
+ * device's ceiling, then transposed (the host decides between passes from the
+ * measured rate).  The sums are the same in every form (to FP64 summation
+ * order).  No counterpart in the reference. */
+#define MC_ROWS_AUTO 0
+#define MC_ROWS_LANE 1
+#define MC_ROWS_TOGETHER 2
+int mc_set_row_issue(mc_ctx *ctx, int mode);
+
+/* The caller's MC_ROWS_* mode and the form the last pass used: 1 transposed,
+ * 0 per lane, -1 no moment rows (FP64 form, or moments off).  Diagnostic. */
+int mc_row_issue(mc_ctx *ctx, int *mode, int *last);
 
 /* Energy-moment tallies of the MC_MIXED walk (on by default): a scored
  * segment adds its path weight times the Chebyshev polynomials T_0..T_3 of

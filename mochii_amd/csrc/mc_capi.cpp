@@ -344,6 +344,9 @@ extern "C" int mc_init(const mc_params *p, mc_ctx **out)
     c->packet_order = MC_ORDER_DIRECTION;
     c->batch_max = p->precision == MC_MIXED ? MC_LAUNCH_BATCH_MIXED : MC_LAUNCH_BATCH_FP64;
     c->mom_on = 1;
+    c->row_issue = MC_ROWS_AUTO;
+    c->rows_together = false;
+    c->rows_together_last = -1;
     c->warm_levels = MC_WARM_LEVELS_DEFAULT;
     if (mc_packet_capacity(c, p->nphotons)) {
         mc_finalize(c);
@@ -714,6 +717,26 @@ extern "C" int mc_set_launch_batch(mc_ctx *c, int64_t max_records)
     c->batch_max = max_records;
     free_packet_buffers(c);
     return mc_packet_capacity(c, c->p.nphotons);
+}
+
+extern "C" int mc_set_row_issue(mc_ctx *c, int mode)
+{
+    CHECK_CTX(c);
+    if (mode != MC_ROWS_AUTO && mode != MC_ROWS_LANE && mode != MC_ROWS_TOGETHER) {
+        mc_set_error("mc_set_row_issue: mode must be MC_ROWS_AUTO, MC_ROWS_LANE or MC_ROWS_TOGETHER");
+        return -1;
+    }
+    c->row_issue = mode;
+    if (mode != MC_ROWS_AUTO) c->rows_together = mode == MC_ROWS_TOGETHER;
+    return 0;
+}
+
+extern "C" int mc_row_issue(mc_ctx *c, int *mode, int *last)
+{
+    CHECK_CTX(c);
+    if (mode) *mode = c->row_issue;
+    if (last) *last = c->rows_together_last;
+    return 0;
 }
 
 extern "C" int mc_set_tally_moments(mc_ctx *c, int on)

@@ -80,6 +80,12 @@ struct DevMilne {
 #define MC_HEI_E3S 19.82
 #define MC_HEI_E1P 21.22
 
+// the device's atomic request rate: one request per 64-byte line an atomic
+// instruction touches, ~2.37e10 per second on MI355X for FP64, FP32 and
+// integer adds alike wherever the lines live (L2, MALL or HBM: 1.8e10 for a
+// 4 GB footprint), measured with the scattered-RMW probes
+#define MC_ATOMIC_REQ_PER_S 2.3e10
+
 // Learned warm start: network sizes (the FP32 parameter blob length
 // MC_MLP_NPARAM is part of the C-ABI)
 #define MC_MLP_IN 8
@@ -332,6 +338,12 @@ struct mc_ctx {
     // total}, leaf CDF, block sums of the scan, totals of the last build
     double *d_milne;          // cdf | xbar | fin | xg
     DevMilne milne;
+    // issue form of the mixed walk's moment rows: the caller's choice
+    // (MC_ROWS_*), the form for the next pass, the form of the last pass
+    // (1 transposed, 0 per lane, -1 no moment rows)
+    int row_issue;
+    bool rows_together;
+    int rows_together_last;
     double *d_hei;            // two-photon grid_y | cdf_e
     double hei_nper, hei_eper;
     bool hei_ready;

@@ -113,6 +113,19 @@ __device__ __forceinline__ double &mom_t()
     return mc_msig[9 * kBlock + threadIdx.x];
 }
 
+// the coefficient and sub-bin rows of another thread of the block
+__device__ __forceinline__ double coef_of(int k, uint32_t tid)
+{
+    extern __shared__ double mc_msig[];
+    return mc_msig[k * kBlock + tid];
+}
+
+__device__ __forceinline__ double mom_t_of(uint32_t tid)
+{
+    extern __shared__ double mc_msig[];
+    return mc_msig[9 * kBlock + tid];
+}
+
 __device__ __forceinline__ double &msig(int k)
 {
     extern __shared__ double mc_msig[];
@@ -563,21 +576,34 @@ __device__ __forceinline__ void score(const KArgs &, Lane &L, int il, double pl,
     cn.smet += (uint32_t)L.nmet;
 }
 
-// tally destination of a leaf: its own rows, or one of the replicas of a hot
-// leaf chosen by the issuing lane
-__device__ __forceinline__ int hot_rep(const DevState &s)
+// entries of the per-wave moment cache: MC_MOM_NM moments and the grain
+// absorption kept outside them
+constexpr int kAggVals = MC_MOM_NM + 1;
+
+__device__ __forceinline__ void wave_sync_lds()
 {
-    return (int)(((blockIdx.x * kBlock + threadIdx.x) * 0x9E3779B1u) >> 16) % s.nrep;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// one global FP64 tally atomic, counted for the pass's atomics report
+// tally destination of a leaf: its own rows, or one of the replicas of a hot
+// leaf chosen by a thread of the block (tid: the issuing lane, or the first
+// lane of the lanes that issue one row together)
+__device__ __forceinline__ int hot_rep(const DevState &s, uint32_t tid)
+{
+    return (int)(((blockIdx.x * kBlock + tid) * 0x9E3779B1u) >> 16) % s.nrep;
+}
+
+// one global FP64 tally atomic issued by one lane alone: one request
 __device__ __forceinline__ void tally_add(double *p, double v, uint32_t &n)
 {
     unsafeAtomicAdd(p, v);
     ++n;
 }
 
-__device__ __forceinline__ void tally_rows(const KArgs &a, int il, double *&h, double *&j)
+__device__ __forceinline__ void tally_rows(const KArgs &a, int il, double *&h, double *&j,
+                                           uint32_t tid)
 {
     const DevState &s = a.s;
     const int nnu = a.tab->nnu;
@@ -586,7 +612,7 @@ __device__ __forceinline__ void tally_rows(const KArgs &a, int il, double *&h, d
         h = s.hhe + (size_t)il * 8;
         j = s.jt + (size_t)il * nnu;
     } else {
-        const size_t slot = (size_t)hot_rep(s) * s.nhot + hs;
+        const size_t slot = (size_t)hot_rep(s, tid) * s.nhot + hs;
         h = s.hot_hhe + slot * 8;
         j = s.hot_jt + slot * nnu;
     }
@@ -603,66 +629,190 @@ __device__ __forceinline__ void metal_coeff(int k, const Lane &L, double &c1, do
 }
 
 // energy moments of sub-bin ms of a leaf: its own row, or one of the
-// replicas of a hot leaf chosen by the issuing lane
-__device__ __forceinline__ double *mom_row(const KArgs &a, int il, int ms)
+// replicas of a hot leaf
+__device__ __forceinline__ double *mom_row(const KArgs &a, int il, int ms, uint32_t tid)
 {
     const DevState &s = a.s;
     const int hs = s.nhot > 0 ? s.hot_slot[il] : -1;
     // replica rows follow the nleaf = mom_rows - nrep nhot leaf rows
     const size_t row = hs < 0 ? (size_t)il
-                              : (size_t)a.mom_rows - (size_t)(s.nrep - hot_rep(s)) * s.nhot + hs;
+                              : (size_t)a.mom_rows - (size_t)(s.nrep - hot_rep(s, tid)) * s.nhot + hs;
     return a.mom + ((size_t)ms * a.mom_rows + row) * MC_MOM_NM;
 }
 
-// one lane's segment: 1 band-J atomic + one per nonzero rate coefficient
-// (+ 2 per metal transition below the packet's energy); with energy-moment
-// tallies (MOM) the MC_MOM_NM moments of the packet's sub-bin instead,
-// pl T_m(t) by the Chebyshev recurrence
-template <int M, bool MOM>
-__device__ __forceinline__ void tally_one(const KArgs &a, Lane &L, int il, double pl)
+// The device's atomic unit serves one request per 64-byte line that an
+// atomic instruction touches, whatever the number of the instruction's lanes
+// in that line: eight lanes adding into eight doubles of one 64-byte row
+// complete as fast as one lane adding one double (~2.4e10 requests per
+// second device-wide, measured, for FP64, FP32 and integer adds alike).  A
+// lane that issues its row's sums alone costs one request per sum; the
+// writers below issue the rows of several lanes transposed, W lanes per row
+// in one instruction, so that a row costs one request per line it spans.
+// Entries are the lanes with `ent` set, in lane order; es is this wave's
+// table of 64 source lanes; the rows go out 64 / W per instruction.
+
+constexpr int kRowsLane = 0, kRowsTogether = 1;
+
+#ifndef MC_COOP_MIN
+#define MC_COOP_MIN 4
+#endif
+// rows below which the entry lanes issue their own: with fewer the
+// transposed issue costs more instructions than the requests it saves
+constexpr int kCoopMin = MC_COOP_MIN;
+
+// rank of each entry lane and the table of source lanes; returns the count
+__device__ __forceinline__ int entry_table(bool ent, int lane, int *es)
 {
-    if constexpr (MOM) {
-        if (L.ms >= 0) {
-            double *m = mom_row(a, il, L.ms);
+    const unsigned long long em = __ballot(ent);
+    if (ent) es[__popcll(em & ((1ull << lane) - 1ull))] = lane;
+    wave_sync_lds();
+    return __popcll(em);
+}
+
+// moment q of a segment of path luminosity pl at sub-bin coordinate t:
+// pl T_q(t) by the Chebyshev recurrence
+__device__ __forceinline__ double seg_moment(double pl, double t, int q)
+{
+    const double t2 = 2.0 * t;
+    double pm2 = pl, pm1 = 0.5 * t2 * pl;
+    for (int j = 2; j <= q; ++j) {
+        const double pq = t2 * pm1 - pm2;
+        pm2 = pm1;
+        pm1 = pq;
+    }
+    return q == 0 ? pm2 : pm1;
+}
+
+// Moment rows.  Entry of a lane: its key (il, sub-bin) as mom_key and
+// either the cache slot whose sums go out (eslot >= 0, cleared here) or, with
+// eslot < 0, the lane's own segment (its path luminosity and sub-bin
+// coordinate).  A row is MC_MOM_NM moments, 32 bytes in one line, and, with
+// the grain absorption kept outside the moments, that term in the leaf's
+// rate row.  The writing lanes take what they add from the slot or from the
+// entry lane, so no lane holds a row in registers.
+__device__ __forceinline__ void mom_rows_out(const KArgs &a, const Lane &L, int lane, bool ent,
+                                             unsigned long long key, int eslot, double *tv,
+                                             int *es, uint32_t &nat)
+{
+    if (!__ballot(ent)) return;
+    const bool dust = a.add_dust && !a.mom_dust;
+    if (kCoopMin > 0 && __popcll(__ballot(ent)) < kCoopMin) {
+        // few rows: each entry lane issues its own
+        if (ent) {
+            double *p = mom_row(a, (int)((key - 1ull) >> 20), (int)((key - 1ull) & 0xFFFFFull), threadIdx.x);
             const double t2 = 2.0 * mom_t();
-            double pm2 = pl, pm1 = 0.5 * t2 * pl;
-            tally_add(m + 0, pm2, L.nat);
-            tally_add(m + 1, pm1, L.nat);
+            double pm2 = L.s_pl, pm1 = 0.5 * t2 * L.s_pl;
 #pragma unroll
-            for (int q = 2; q < MC_MOM_NM; ++q) {
-                const double pq = t2 * pm1 - pm2;
-                tally_add(m + q, pq, L.nat);
-                pm2 = pm1;
-                pm1 = pq;
+            for (int q = 0; q < MC_MOM_NM; ++q) {
+                double val;
+                if (eslot >= 0) {
+                    val = tv[eslot * kAggVals + q];
+                    tv[eslot * kAggVals + q] = 0.0;
+                } else if (q < 2) {
+                    val = q == 0 ? pm2 : pm1;
+                } else {
+                    val = t2 * pm1 - pm2;
+                    pm2 = pm1;
+                    pm1 = val;
+                }
+                tally_add(p + q, val, nat);
             }
-            if (a.add_dust && !a.mom_dust && L.dabs != 0.0) {
-                // dust absorption outside the moments keeps its per-leaf
-                // sum: into the lane's replica row of a hot leaf, which
-                // k_fold_hot folds into the leaf's row (a direct write
-                // there would be replaced)
+            if (dust) {
+                double val = eslot >= 0 ? tv[eslot * kAggVals + MC_MOM_NM] : L.s_pl * L.dabs;
+                if (eslot >= 0) tv[eslot * kAggVals + MC_MOM_NM] = 0.0;
+                if (val != 0.0) {
+                    double *h, *j;
+                    tally_rows(a, (int)((key - 1ull) >> 20), h, j, threadIdx.x);
+                    tally_add(h + 6, val, nat);
+                }
+            }
+        }
+        wave_sync_lds();
+        return;
+    }
+    const int n = entry_table(ent, lane, es);
+    const int W = dust ? MC_MOM_NM + 1 : MC_MOM_NM;
+    const int R = 64 / W;
+    const int r = lane / W, q = lane - r * W;
+    const uint32_t tid = (threadIdx.x & ~63u) + (uint32_t)(r * W);
+    for (int base = 0; base < n; base += R) {
+        const bool on = r < R && base + r < n;
+        const int src = on ? es[base + r] : lane;
+        const unsigned long long k = (unsigned long long)__shfl((long long)key, src) - 1ull;
+        const int sl = __shfl(eslot, src);
+        const double pl = __shfl(L.s_pl, src);
+        const double da = dust ? __shfl(L.dabs, src) : 0.0;
+        if (on) {
+            double val;
+            if (sl >= 0) {
+                double *t = tv + sl * kAggVals + q;
+                val = *t;
+                *t = 0.0;
+            } else if (q < MC_MOM_NM) {
+                val = seg_moment(pl, mom_t_of((threadIdx.x & ~63u) + (uint32_t)src), q);
+            } else {
+                val = pl * da;
+            }
+            const int il = (int)(k >> 20), ms = (int)(k & 0xFFFFFull);
+            if (q < MC_MOM_NM) {
+                unsafeAtomicAdd(mom_row(a, il, ms, tid) + q, val);
+                if (q == 0) ++nat;
+            } else if (val != 0.0) {
                 double *h, *j;
-                tally_rows(a, il, h, j);
-                tally_add(h + 6, pl * L.dabs, L.nat);
+                tally_rows(a, il, h, j, tid);
+                tally_add(h + 6, val, nat);
             }
-            return;
         }
     }
-    double *h, *j;
-    tally_rows(a, il, h, j);
-    tally_add(j + L.inu, pl, L.nat);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        const double ck = coef(k);
-        if (ck != 0.0) tally_add(h + k, pl * ck, L.nat);
+    wave_sync_lds();
+}
+
+// Per-coefficient rows of the lanes with `ent` set (score_ion_path,
+// src/ion_score_mod.f90:137-231): the six rate coefficients and the grain
+// absorption into the leaf's 64-byte rate row (skipping exact zeros: He
+// terms below their thresholds), the path luminosity into the packet's
+// band-J bin; then, with the metal path, each lane's own metal transitions
+// (species_rate_path_score, src/species_mod.f90:329-343).
+template <int M>
+__device__ __forceinline__ void coef_rows_out(const KArgs &a, Lane &L, int lane, bool ent,
+                                              int *es)
+{
+    if (!__ballot(ent)) return;
+    const int n = entry_table(ent, lane, es);
+    const int r = lane >> 3, q = lane & 7;
+    const uint32_t tid = (threadIdx.x & ~63u) + (uint32_t)(r * 8);
+    for (int base = 0; base < n; base += 8) {
+        const bool on = base + r < n;
+        const int src = on ? es[base + r] : lane;
+        const int il = __shfl(L.s_il, src), inu = __shfl(L.inu, src);
+        const double pl = __shfl(L.s_pl, src);
+        const double da = __shfl(L.dabs, src);
+        if (on) {
+            double *h, *j;
+            tally_rows(a, il, h, j, tid);
+            if (q == 7) {
+                tally_add(j + inu, pl, L.nat);
+            } else {
+                const double val = q < 6 ? pl * coef_of(q, (threadIdx.x & ~63u) + (uint32_t)src)
+                                         : (a.add_dust ? pl * da : 0.0);
+                if (val != 0.0) {
+                    unsafeAtomicAdd(h + q, val);
+                    if (q == 0) ++L.nat;
+                }
+            }
+        }
     }
-    if (a.add_dust && L.dabs != 0.0) tally_add(h + 6, pl * L.dabs, L.nat);
+    wave_sync_lds();
     if constexpr (M == 1) {
-        double *r = a.mraw + (size_t)il * a.ntr * 2;
-        for (int k = 0; k < L.nmet; ++k) {
-            double c1, c2;
-            metal_coeff(k, L, c1, c2);
-            tally_add(r + 2 * k, pl * c1, L.nat);
-            if (c2 != 0.0) tally_add(r + 2 * k + 1, pl * c2, L.nat);
+        if (ent) {
+            const double pl = L.s_pl;
+            double *r2 = a.mraw + (size_t)L.s_il * a.ntr * 2;
+            for (int k = 0; k < L.nmet; ++k) {
+                double c1, c2;
+                metal_coeff(k, L, c1, c2);
+                tally_add(r2 + 2 * k, pl * c1, L.nat);
+                if (c2 != 0.0) tally_add(r2 + 2 * k + 1, pl * c2, L.nat);
+            }
         }
     }
 }
@@ -683,31 +833,23 @@ constexpr int kAggRoundsMom = 1;
 
 // The metal-free moment walk keeps a per-wave LDS cache of (leaf, sub-bin)
 // moment rows: 128 slots, each a tag, MC_MOM_NM moments and the grain
-// absorption kept outside the moments.  A global FP64 atomic costs far more
-// device throughput than an LDS one (the device completes ~2.3e10 FP64
-// atomics per second wherever they land), and the lanes of a wave walk
-// nearby leaves in one sub-bin, so a lane adds its segment to the slot that
-// holds its (leaf, sub-bin) and the slot's sums go out as one set of atomics
-// when another key takes the slot or the wave ends.  Per step: a lane whose
-// key is in its slot adds to it (a hit, and the slot stays); the lanes whose
-// key is not claim its slot, one of them (the owner) writes the old entry
-// out and installs its key, and the lanes with the owner's key add to it;
-// a lane whose slot is held by a hit of another key, or taken by another
-// key, and segments without a sub-bin, issue their own atomics.
+// absorption kept outside the moments.  Global atomics cost far more device
+// throughput than LDS ones, and the lanes of a wave walk nearby leaves in one
+// sub-bin, so a lane adds its segment to the slot that holds its (leaf,
+// sub-bin) and the slot's sums go out as one row when another key takes the
+// slot or the wave ends.  Per step: a lane whose key is in its slot adds to
+// it (a hit, and the slot stays); the lanes whose key is not claim its slot,
+// one of them (the owner) takes the old entry out and installs its key, and
+// the lanes with the owner's key add to it; a lane whose slot is held by a
+// hit of another key, or taken by another key, sends its segment's moments
+// out as a row of its own.  The evicted rows and those go out together
+// (mom_rows_out); segments without a sub-bin keep the per-coefficient sums.
 #ifndef MC_AGG_BITS0
 #define MC_AGG_BITS0 7
 #endif
 // slots per wave: 2^7 (three blocks of four waves per CU fit the LDS); the
 // metal walk's LDS also holds its packets' metal cross sections
 template <int M> constexpr int agg_slot_bits() { return M == 0 ? MC_AGG_BITS0 : 7; }
-constexpr int kAggVals = MC_MOM_NM + 1;
-
-__device__ __forceinline__ void wave_sync_lds()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // the moment-row key of a (leaf, sub-bin), never 0 (0 marks an empty slot)
 __device__ __forceinline__ unsigned long long mom_key(int il, int ms)
@@ -715,33 +857,11 @@ __device__ __forceinline__ unsigned long long mom_key(int il, int ms)
     return ((((unsigned long long)(unsigned)il) << 20) | (unsigned)ms) + 1ull;
 }
 
-// write one cached entry's sums to its row (the issuing lane's replica of a
-// hot leaf) and clear them
-__device__ __forceinline__ void mom_cache_write(const KArgs &a, unsigned long long key, double *t,
-                                                uint32_t &nat)
-{
-    const unsigned long long k = key - 1ull;
-    const int il = (int)(k >> 20), ms = (int)(k & 0xFFFFFull);
-    double *m = mom_row(a, il, ms);
-#pragma unroll
-    for (int q = 0; q < MC_MOM_NM; ++q) {
-        tally_add(m + q, t[q], nat);
-        t[q] = 0.0;
-    }
-    const double d = t[MC_MOM_NM];
-    if (d != 0.0) {
-        double *h, *j;
-        tally_rows(a, il, h, j);
-        tally_add(h + 6, d, nat);
-        t[MC_MOM_NM] = 0.0;
-    }
-}
-
 // tv: this wave's slots x kAggVals sums (zero in an empty slot), tt its
-// tags, tc its claims; M the walk's metal path
+// tags, tc its claims, es its entry table; M the walk's metal path
 template <int M>
 __device__ __forceinline__ void flush_mom_cache(const KArgs &a, Lane &L, int lane, double *tv,
-                                                unsigned long long *tt, int *tc)
+                                                unsigned long long *tt, int *tc, int *es)
 {
     const bool has = L.s_il >= 0;
     if (!__ballot(has)) return;
@@ -759,10 +879,152 @@ __device__ __forceinline__ void flush_mom_cache(const KArgs &a, Lane &L, int lan
     const int owner = claim < 64 ? claim : lane;
     const unsigned long long okey = (unsigned long long)__shfl((long long)key, owner);
     const bool in = hit || (miss && claim < 64 && okey == key);
+    // rows out this step: the entry an owner evicts (its slot's sums, the
+    // old tag) and the segment of a lane that got no slot; the owner
+    // installs its key at once, the sums follow once the old ones are out
+    unsigned long long ekey = key;
+    int eslot = -1;
+    bool ent = mom && !in;
+    if (miss && claim == lane) {
+        const unsigned long long old = tt[slot];
+        if (old != 0ull) {
+            ekey = old;
+            eslot = slot;
+            ent = true;
+        }
+        tt[slot] = key;
+    }
+    mom_rows_out(a, L, lane, ent, ekey, eslot, tv, es, L.nat);
+    if (in) {
+        double *t = tv + slot * kAggVals;
+        const double pl = L.s_pl;
+        const double t2 = 2.0 * mom_t();
+        double pm2 = pl, pm1 = 0.5 * t2 * pl;
+        atomicAdd(t + 0, pm2);
+        atomicAdd(t + 1, pm1);
+#pragma unroll
+        for (int q = 2; q < MC_MOM_NM; ++q) {
+            const double pq = t2 * pm1 - pm2;
+            atomicAdd(t + q, pq);
+            pm2 = pm1;
+            pm1 = pq;
+        }
+        if (a.add_dust && !a.mom_dust && L.dabs != 0.0) atomicAdd(t + MC_MOM_NM, pl * L.dabs);
+    }
+    // segments without a sub-bin: the per-coefficient sums
+    coef_rows_out<M>(a, L, lane, has && !mom, es);
+    wave_sync_lds();
+    L.s_il = -1;
+}
+
+// the wave's cached entries out, at the end of its walk
+template <int M>
+__device__ __forceinline__ void mom_cache_drain(const KArgs &a, const Lane &L, int lane,
+                                                double *tv, unsigned long long *tt, int *es,
+                                                uint32_t &nat)
+{
+    for (int s0 = 0; s0 < (1 << agg_slot_bits<M>()); s0 += 64) {
+        const int s = s0 + lane;
+        const unsigned long long key = tt[s];
+        mom_rows_out(a, L, lane, key != 0ull, key, s, tv, es, nat);
+        tt[s] = 0ull;
+    }
+}
+
+// The per-lane issue of the same cache: an owner writes the entry it evicts
+// and a lane without a slot its segment's moments, each sum by one atomic
+// of its own.  Fewer instructions and registers than the transposed issue,
+// so the faster form while the device's atomic unit has room; the host
+// switches between the two on the measured request rate (mc_run_pass).
+__device__ __forceinline__ void mom_row_lane(const KArgs &a, unsigned long long key, double *t,
+                                             uint32_t &nat)
+{
+    const unsigned long long k = key - 1ull;
+    const int il = (int)(k >> 20), ms = (int)(k & 0xFFFFFull);
+    double *m = mom_row(a, il, ms, threadIdx.x);
+#pragma unroll
+    for (int q = 0; q < MC_MOM_NM; ++q) {
+        tally_add(m + q, t[q], nat);
+        t[q] = 0.0;
+    }
+    const double d = t[MC_MOM_NM];
+    if (d != 0.0) {
+        double *h, *j;
+        tally_rows(a, il, h, j, threadIdx.x);
+        tally_add(h + 6, d, nat);
+        t[MC_MOM_NM] = 0.0;
+    }
+}
+
+// one lane's segment: its moments, or without a sub-bin the per-coefficient
+// sums (band J, nonzero rate coefficients, grain absorption, metals)
+template <int M>
+__device__ __forceinline__ void segment_lane(const KArgs &a, Lane &L)
+{
+    const double pl = L.s_pl;
+    if (L.ms >= 0) {
+        double *m = mom_row(a, L.s_il, L.ms, threadIdx.x);
+        const double t2 = 2.0 * mom_t();
+        double pm2 = pl, pm1 = 0.5 * t2 * pl;
+        tally_add(m + 0, pm2, L.nat);
+        tally_add(m + 1, pm1, L.nat);
+#pragma unroll
+        for (int q = 2; q < MC_MOM_NM; ++q) {
+            const double pq = t2 * pm1 - pm2;
+            tally_add(m + q, pq, L.nat);
+            pm2 = pm1;
+            pm1 = pq;
+        }
+        if (a.add_dust && !a.mom_dust && L.dabs != 0.0) {
+            double *h, *j;
+            tally_rows(a, L.s_il, h, j, threadIdx.x);
+            tally_add(h + 6, pl * L.dabs, L.nat);
+        }
+        return;
+    }
+    double *h, *j;
+    tally_rows(a, L.s_il, h, j, threadIdx.x);
+    tally_add(j + L.inu, pl, L.nat);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const double ck = coef(k);
+        if (ck != 0.0) tally_add(h + k, pl * ck, L.nat);
+    }
+    if (a.add_dust && L.dabs != 0.0) tally_add(h + 6, pl * L.dabs, L.nat);
+    if constexpr (M == 1) {
+        double *r = a.mraw + (size_t)L.s_il * a.ntr * 2;
+        for (int k = 0; k < L.nmet; ++k) {
+            double c1, c2;
+            metal_coeff(k, L, c1, c2);
+            tally_add(r + 2 * k, pl * c1, L.nat);
+            if (c2 != 0.0) tally_add(r + 2 * k + 1, pl * c2, L.nat);
+        }
+    }
+}
+
+template <int M>
+__device__ __forceinline__ void flush_mom_cache_lane(const KArgs &a, Lane &L, int lane, double *tv,
+                                                     unsigned long long *tt, int *tc)
+{
+    const bool has = L.s_il >= 0;
+    if (!__ballot(has)) return;
+    const bool mom = has && L.ms >= 0;
+    const unsigned long long key = mom ? mom_key(L.s_il, L.ms) : 0ull;
+    const int slot = (int)((key * 0x9E3779B97F4A7C15ull) >> (64 - agg_slot_bits<M>()));
+    const bool hit = mom && tt[slot] == key;
+    const bool miss = mom && !hit;
+    if (miss) tc[slot] = lane;
+    wave_sync_lds();
+    if (hit) tc[slot] = 64;
+    wave_sync_lds();
+    const int claim = miss ? tc[slot] : 64;
+    const int owner = claim < 64 ? claim : lane;
+    const unsigned long long okey = (unsigned long long)__shfl((long long)key, owner);
+    const bool in = hit || (miss && claim < 64 && okey == key);
     if (miss && claim == lane) {
         double *t = tv + slot * kAggVals;
         const unsigned long long old = tt[slot];
-        if (old != 0ull) mom_cache_write(a, old, t, L.nat);
+        if (old != 0ull) mom_row_lane(a, old, t, L.nat);
         tt[slot] = key;
     }
     wave_sync_lds();
@@ -782,20 +1044,19 @@ __device__ __forceinline__ void flush_mom_cache(const KArgs &a, Lane &L, int lan
         }
         if (a.add_dust && !a.mom_dust && L.dabs != 0.0) atomicAdd(t + MC_MOM_NM, pl * L.dabs);
     } else if (has) {
-        tally_one<M, true>(a, L, L.s_il, L.s_pl);
+        segment_lane<M>(a, L);
     }
     wave_sync_lds();
     L.s_il = -1;
 }
 
-// the wave's cached entries out, at the end of its walk
 template <int M>
-__device__ __forceinline__ void mom_cache_drain(const KArgs &a, int lane, double *tv,
-                                                unsigned long long *tt, uint32_t &nat)
+__device__ __forceinline__ void mom_cache_drain_lane(const KArgs &a, int lane, double *tv,
+                                                     unsigned long long *tt, uint32_t &nat)
 {
     for (int s = lane; s < (1 << agg_slot_bits<M>()); s += 64) {
         if (tt[s] != 0ull) {
-            mom_cache_write(a, tt[s], tv + s * kAggVals, nat);
+            mom_row_lane(a, tt[s], tv + s * kAggVals, nat);
             tt[s] = 0ull;
         }
     }
@@ -803,18 +1064,20 @@ __device__ __forceinline__ void mom_cache_drain(const KArgs &a, int lane, double
 
 // Issue the tallies of the step's segments, called by all 64 lanes.  With
 // direction-coherent packets a leaf group (lanes that scored the same leaf)
-// is summed across the wave and its rate sums go out as one atomic per
-// coefficient from the group's first lane; its band-J contributions (keyed
-// by each packet's energy bin) are binned in the wave's LDS row first and go
-// out as one atomic per nonzero bin.  Lanes outside a large group use
-// per-lane atomics.  jbin: this wave's nnu LDS bins, zero between calls.
-template <int P, int M>
+// is summed across the wave and its rate sums go out as one row from the
+// group's lanes; its band-J contributions (keyed by each packet's energy
+// bin) are binned in the wave's LDS row first and go out as one atomic per
+// nonzero bin.  Lanes outside a large group go out as rows of their own.
+// jbin: this wave's nnu LDS bins, zero between calls.
+template <int P, int M, int T>
 __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, double *jbin,
-                                            double *aggv, unsigned long long *aggt, int *aggc)
+                                            double *aggv, unsigned long long *aggt, int *aggc,
+                                            int *es)
 {
     if constexpr (Prec<P>::mixed && M <= 1) {
         if (a.mom) {
-            flush_mom_cache<M>(a, L, lane, aggv, aggt, aggc);
+            if constexpr (T == kRowsTogether) flush_mom_cache<M>(a, L, lane, aggv, aggt, aggc, es);
+            else flush_mom_cache_lane<M>(a, L, lane, aggv, aggt, aggc);
             return;
         }
     }
@@ -837,64 +1100,29 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
             const unsigned long long grp = __ballot(in);
             pend &= ~grp;
             if (__popcll(grp) < kAggMin) continue;
-            if constexpr (kMom) {
-                // energy-moment group: when every lane of the leaf group is
-                // in the leader's moment sub-bin the group sums the
-                // Chebyshev moments of its segments, one atomic per moment
-                // (into the leader's replica of a hot leaf), instead of the
-                // ~20 per-coefficient sums below
-                const int lms = __shfl(L.ms, leader);
-                if (a.mom && lms >= 0 && __ballot(in && L.ms == lms) == grp) {
-                    double *m = mom_row(a, lil, lms);
-                    const double pl = in ? L.s_pl : 0.0;
-                    const double t2 = in ? 2.0 * mom_t() : 0.0;
-                    double pm2 = pl, pm1 = 0.5 * t2 * pl;
-                    double sum = wave_sum(pl);
-                    if (lane == leader) tally_add(m + 0, sum, L.nat);
-                    sum = wave_sum(pm1);
-                    if (lane == leader) tally_add(m + 1, sum, L.nat);
-#pragma unroll
-                    for (int q = 2; q < MC_MOM_NM; ++q) {
-                        const double pq = t2 * pm1 - pm2;
-                        sum = wave_sum(pq);
-                        if (lane == leader) tally_add(m + q, sum, L.nat);
-                        pm2 = pm1;
-                        pm1 = pq;
-                    }
-                    if (a.add_dust && !a.mom_dust) {
-                        const double v = in ? L.s_pl * L.dabs : 0.0;
-                        if (__ballot(v != 0.0)) {
-                            sum = wave_sum(v);
-                            if (lane == leader) {
-                                double *h, *j;
-                                tally_rows(a, lil, h, j);
-                                tally_add(h + 6, sum, L.nat);
-                            }
-                        }
-                    }
-                    if (in) has = false;
-                    continue;
-                }
-            }
+            const uint32_t ltid = (threadIdx.x & ~63u) + (uint32_t)leader;
             double *h, *j;
-            tally_rows(a, lil, h, j);
+            tally_rows(a, lil, h, j, ltid);
+            // the group's rate row: sums of the six coefficients and the
+            // grain absorption, lane k issuing sum k
+            double mine = 0.0;
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
                 const double v = in ? L.s_pl * coef(k) : 0.0;
-                if (__ballot(v != 0.0)) {
-                    const double sum = wave_sum(v);
-                    if (lane == leader) tally_add(h + k, sum, L.nat);
-                }
+                const double sum = __ballot(v != 0.0) ? wave_sum(v) : 0.0;
+                if (lane == k) mine = sum;
             }
             if (a.add_dust) {
                 const double v = in ? L.s_pl * L.dabs : 0.0;
-                if (__ballot(v != 0.0)) {
-                    const double sum = wave_sum(v);
-                    if (lane == leader) tally_add(h + 6, sum, L.nat);
-                }
+                const double sum = __ballot(v != 0.0) ? wave_sum(v) : 0.0;
+                if (lane == 6) mine = sum;
+            }
+            if (lane < 7 && mine != 0.0) {
+                unsafeAtomicAdd(h + lane, mine);
+                if (lane == 0) ++L.nat;
             }
             if constexpr (M == 1) {
-                double *r = a.mraw + (size_t)lil * a.ntr * 2;
+                double *r2 = a.mraw + (size_t)lil * a.ntr * 2;
                 for (int k = 0; k < a.ntr; ++k) {
                     const bool on = in && k < L.nmet;
                     if (!__ballot(on)) break;
@@ -902,32 +1130,32 @@ __device__ __forceinline__ void flush_tally(const KArgs &a, Lane &L, int lane, d
                     if (on) metal_coeff(k, L, c1, c2);
                     const double s1 = wave_sum(on ? L.s_pl * c1 : 0.0);
                     const double s2 = wave_sum(on ? L.s_pl * c2 : 0.0);
-                    if (lane == leader) {
-                        tally_add(r + 2 * k, s1, L.nat);
-                        if (s2 != 0.0) tally_add(r + 2 * k + 1, s2, L.nat);
-                    }
+                    // lanes 0 and 1: the pair (c1, c2) of the transition, one line
+                    if (lane == 0 || (lane == 1 && s2 != 0.0))
+                        unsafeAtomicAdd(r2 + 2 * k + lane, lane == 0 ? s1 : s2);
+                    if (lane == 0) ++L.nat;
                 }
             }
             // band J of the group: per-wave bins in LDS, then one atomic
             // per nonzero bin
             if (in) atomicAdd(&jbin[L.inu], L.s_pl);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            for (int b = lane; b < nnu; b += 64) {
-                const double v = jbin[b];
-                if (v != 0.0) {
-                    tally_add(j + b, v, L.nat);
-                    jbin[b] = 0.0;
-                }
+            wave_sync_lds();
+            static_assert(MC_MAXBINS <= 64, "one band bin per lane");
+            const double jv = lane < nnu ? jbin[lane] : 0.0;
+            if (jv != 0.0) {
+                unsafeAtomicAdd(j + lane, jv);
+                jbin[lane] = 0.0;
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // requests: one per 8-bin line of the row holding a nonzero bin
+            const unsigned long long nz = __ballot(jv != 0.0);
+            if (lane == 0)
+                for (int g = 0; g < 64; g += 8) L.nat += ((nz >> g) & 0xFFull) != 0ull ? 1u : 0u;
+            wave_sync_lds();
             if (in) has = false;
         }
     }
-    if (has) tally_one<M, Prec<P>::mixed && M <= 1>(a, L, L.s_il, L.s_pl);
+    if constexpr (T == kRowsTogether) coef_rows_out<M>(a, L, lane, has, es);
+    else if (has) segment_lane<M>(a, L);
     L.s_il = -1;
 }
 
@@ -1577,6 +1805,10 @@ __device__ __forceinline__ void load_packet(const KArgs &a, Lane &L, int64_t r, 
 }
 
 constexpr int kWalkChunk = 256;   // records a launch-order wave takes at once
+#ifndef MC_COHERENT_REFILL
+#define MC_COHERENT_REFILL 64
+#endif
+constexpr int kCoherentRefill = MC_COHERENT_REFILL;   // idle lanes a sorted wave refills at
 
 // lane 0's value as a wave-uniform (scalar) value
 __device__ __forceinline__ unsigned long long wave_first(unsigned long long v)
@@ -1602,7 +1834,9 @@ constexpr int walk_min_blocks(int M)
     return M == 2 ? MC_WALK_MINB_2 : (M == 1 ? MC_WALK_MINB_1 : MC_WALK_MINB_0);
 }
 
-template <int P, int G, int M>
+// T: issue form of the energy-moment rows (mixed form with moments only):
+// kRowsTogether (transposed rows) or kRowsLane (per-lane atomics)
+template <int P, int G, int M, int T = kRowsTogether>
 __global__ __launch_bounds__(kBlock, walk_min_blocks(M)) void k_walk(KArgs a)
 {
     __shared__ double jbins[kBlock / 64][MC_MAXBINS];
@@ -1613,6 +1847,8 @@ __global__ __launch_bounds__(kBlock, walk_min_blocks(M)) void k_walk(KArgs a)
     __shared__ double aggvs[kBlock / 64][kSlots * kAggVals];
     __shared__ unsigned long long aggts[kBlock / 64][kSlots];
     __shared__ int aggcs[kBlock / 64][kSlots];
+    __shared__ int ess[kBlock / 64][64];
+    int *es = ess[threadIdx.x >> 6];
     double *aggv = aggvs[threadIdx.x >> 6];
     unsigned long long *aggt = aggts[threadIdx.x >> 6];
     int *aggc = aggcs[threadIdx.x >> 6];
@@ -1643,13 +1879,18 @@ __global__ __launch_bounds__(kBlock, walk_min_blocks(M)) void k_walk(KArgs a)
         unsigned long long r = 0;
         bool take = false;
         if (a.coherent) {
-            // direction-sorted records: the wave takes 64 consecutive records
-            // at once, when every lane is idle, so its packets stay together
-            if (need == ~0ull) {
+            // direction-sorted records: the wave takes consecutive records
+            // for its idle lanes once kCoherentRefill of them are idle (or no
+            // lane walks), so that its packets stay together
+            const int idle = __popcll(need);
+            if (idle > 0 && (idle >= kCoherentRefill || !__ballot(L.phase == 1 || L.phase == 2))) {
                 unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(a.s.counter, 64ull);
-                r = wave_first(base) + (unsigned long long)lane;
-                take = true;
+                if (lane == 0) base = atomicAdd(a.s.counter, (unsigned long long)idle);
+                base = wave_first(base);
+                if (L.phase == 0) {
+                    r = base + (unsigned long long)__popcll(need & ((1ull << lane) - 1ull));
+                    take = true;
+                }
             }
         } else if (need != 0ull) {
             if (!dry && qn >= qe) {
@@ -1685,12 +1926,13 @@ __global__ __launch_bounds__(kBlock, walk_min_blocks(M)) void k_walk(KArgs a)
             if constexpr (G == kCarDDA) car_step<P, M>(a, L, cn);
             else oct_step<P, M>(a, L, cn);
         }
-        flush_tally<P, M>(a, L, lane, jbin, aggv, aggt, aggc);
+        flush_tally<P, M, T>(a, L, lane, jbin, aggv, aggt, aggc, es);
     }
     if constexpr (Prec<P>::mixed && M <= 1) {
         if (a.mom) {
             wave_sync_lds();
-            mom_cache_drain<M>(a, lane, aggv, aggt, L.nat);
+            if constexpr (T == kRowsTogether) mom_cache_drain<M>(a, L, lane, aggv, aggt, es, L.nat);
+            else mom_cache_drain_lane<M>(a, lane, aggv, aggt, L.nat);
         }
     }
     if constexpr (M == 2) {
@@ -3226,6 +3468,12 @@ int sum_partials(mc_ctx *c, int nblk, double out[3])
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// fractions of the device's atomic request rate at which the moment rows'
+// issue form switches (measured on the 20.4 M-leaf multifractal model: the
+// per-lane form's first pass runs at 53% of the rate and is faster there, an
+// ionised pass at 90%, where the transposed form is 26% faster)
+constexpr double kRowsUpFrac = 0.7, kRowsDownFrac = 0.4;
+
 int mc_run_pass(mc_ctx *c)
 {
     if (c->p.diffuse_field && c->p.hei_diffuse && !c->hei_ready) {
@@ -3278,6 +3526,16 @@ int mc_run_pass(mc_ctx *c)
                            : (const void *)k_walk<MC_FP64, kFaceExit, 0>)
                   : (mixed ? (const void *)k_walk<MC_MIXED, kCarDDA, 0>
                            : (const void *)k_walk<MC_FP64, kCarDDA, 0>);
+    // the moment rows' issue form for this pass (mixed form with moments)
+    const bool rows_lane = mixed && use_mom && !c->rows_together;
+    if (rows_lane) {
+        if (metals)
+            kw = face ? (const void *)k_walk<MC_MIXED, kFaceExit, 1, kRowsLane>
+                      : (const void *)k_walk<MC_MIXED, kCarDDA, 1, kRowsLane>;
+        else
+            kw = face ? (const void *)k_walk<MC_MIXED, kFaceExit, 0, kRowsLane>
+                      : (const void *)k_walk<MC_MIXED, kCarDDA, 0, kRowsLane>;
+    }
     const size_t dlds = (size_t)(kCoefRows + (metals ? c->ntr : 0)) * kBlock * sizeof(double);
     const int nblk = persistent_blocks(kw, dlds);
     const int nlds = c->car ? 0 : c->n_lds_nodes;
@@ -3384,6 +3642,19 @@ int mc_run_pass(mc_ctx *c)
     for (int k = 0; k < 4; ++k) c->segs[k] = (int64_t)sg[k];
     c->times[0] = t_launch;
     c->times[1] = t_walk;
+    c->rows_together_last = mixed && use_mom ? (rows_lane ? 0 : 1) : -1;
+    if (mixed && use_mom && c->row_issue == MC_ROWS_AUTO && t_walk > 0.0f) {
+        // the next pass's issue form, from this pass's request rate: per-lane
+        // rows until the walk drives the atomic unit near its ceiling, then
+        // transposed rows (about MC_MOM_NM lane atomics per request) until
+        // the per-lane form would run well below it; the gap between the two
+        // thresholds keeps the choice from flapping
+        const double rate = (double)sg[3] / (t_walk * 1e-3);
+        if (!c->rows_together && rate > kRowsUpFrac * MC_ATOMIC_REQ_PER_S)
+            c->rows_together = true;
+        else if (c->rows_together && MC_MOM_NM * rate < kRowsDownFrac * MC_ATOMIC_REQ_PER_S)
+            c->rows_together = false;
+    }
     return 0;
 }
 

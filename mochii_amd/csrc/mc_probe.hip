@@ -35,6 +35,19 @@ __global__ __launch_bounds__(kProbeBlock) void k_probe_rmw(double *a, int64_t ro
     }
 }
 
+// the transposed form: lane-op i adds 1 to double (i mod width) of row
+// hash(i / width), so that the `width` lanes of a row issue its adds in one
+// instruction (width divides 64 and 8: a row never straddles a 64-byte line)
+__global__ __launch_bounds__(kProbeBlock) void k_probe_rows(double *a, int64_t rows, int width,
+                                                            int64_t n_lane_ops)
+{
+    const int64_t stride = (int64_t)gridDim.x * kProbeBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kProbeBlock + threadIdx.x; i < n_lane_ops; i += stride) {
+        const uint64_t r = probe_mix((uint64_t)(i / width)) % (uint64_t)rows;
+        unsafeAtomicAdd(a + r * (uint64_t)width + (uint64_t)(i % width), 1.0);
+    }
+}
+
 __global__ __launch_bounds__(kProbeBlock) void k_probe_fill(double *a, int64_t n)
 {
     const int64_t stride = (int64_t)gridDim.x * kProbeBlock;
@@ -44,20 +57,16 @@ __global__ __launch_bounds__(kProbeBlock) void k_probe_fill(double *a, int64_t n
 
 }  // namespace
 
-extern "C" int mc_probe_scattered_rmw(int64_t rows, int width, int64_t n_ops, double *ms,
-                                      double *checksum)
+static int probe_run(int64_t rows, int width, int64_t n_ops, bool together, double *ms,
+                     double *checksum)
 {
-    if (rows <= 0 || width < 1 || width > 16 || n_ops <= 0 || !ms) {
-        mc_set_error("mc_probe_scattered_rmw: rows > 0, 1 <= width <= 16, n_ops > 0");
-        return -1;
-    }
     const size_t n = (size_t)rows * (size_t)width;
     double *a = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipMalloc(&a, n * sizeof(double));
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        mc_set_error("mc_probe_scattered_rmw: hipMalloc of " + std::to_string(n * 8) +
+        mc_set_error("scattered-RMW probe: hipMalloc of " + std::to_string(n * 8) +
                      " bytes failed");
         return -1;
     }
@@ -71,12 +80,14 @@ extern "C" int mc_probe_scattered_rmw(int64_t rows, int width, int64_t n_ops, do
     if (e == hipSuccess) e = hipEventCreate(&e1);
     // one untimed run touches the pages, then the timed one
     if (e == hipSuccess) {
-        k_probe_rmw<<<grid, kProbeBlock>>>(a, rows, width, n_ops);
+        if (together) k_probe_rows<<<grid, kProbeBlock>>>(a, rows, width, n_ops * width);
+        else k_probe_rmw<<<grid, kProbeBlock>>>(a, rows, width, n_ops);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
     if (e == hipSuccess) {
-        k_probe_rmw<<<grid, kProbeBlock>>>(a, rows, width, n_ops);
+        if (together) k_probe_rows<<<grid, kProbeBlock>>>(a, rows, width, n_ops * width);
+        else k_probe_rmw<<<grid, kProbeBlock>>>(a, rows, width, n_ops);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
@@ -102,9 +113,30 @@ extern "C" int mc_probe_scattered_rmw(int64_t rows, int width, int64_t n_ops, do
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(a);
     if (e != hipSuccess) {
-        mc_set_error(std::string("mc_probe_scattered_rmw: ") + hipGetErrorString(e));
+        mc_set_error(std::string("scattered-RMW probe: ") + hipGetErrorString(e));
         return -1;
     }
     *ms = (double)t;
     return 0;
+}
+
+extern "C" int mc_probe_scattered_rmw(int64_t rows, int width, int64_t n_ops, double *ms,
+                                      double *checksum)
+{
+    if (rows <= 0 || width < 1 || width > 16 || n_ops <= 0 || !ms) {
+        mc_set_error("mc_probe_scattered_rmw: rows > 0, 1 <= width <= 16, n_ops > 0");
+        return -1;
+    }
+    return probe_run(rows, width, n_ops, false, ms, checksum);
+}
+
+extern "C" int mc_probe_row_atomics(int64_t rows, int width, int64_t n_rows, double *ms,
+                                    double *checksum)
+{
+    if (rows <= 0 || (width != 1 && width != 2 && width != 4 && width != 8) || n_rows <= 0 ||
+        n_rows > ((int64_t)1 << 40) || !ms) {
+        mc_set_error("mc_probe_row_atomics: rows > 0, width 1, 2, 4 or 8, 0 < n_rows <= 2^40");
+        return -1;
+    }
+    return probe_run(rows, width, n_rows, true, ms, checksum);
 }

@@ -10,6 +10,7 @@ MC_EXT_SPH, MC_EXT_REC = 1, 2
 MC_DIFFUSE_THRESHOLD, MC_DIFFUSE_EXPONENTIAL, MC_DIFFUSE_MILNE = 0, 1, 2
 MC_WARM_NONE, MC_WARM_NEWTON, MC_WARM_MLP = 0, 1, 2
 MC_ORDER_LAUNCH, MC_ORDER_DIRECTION = 0, 1
+MC_ROWS_AUTO, MC_ROWS_LANE, MC_ROWS_TOGETHER = 0, 1, 2
 MC_MLP_NPARAM = 9180
 MC_MAX_METAL_TR = 64
 MC_CX_TWO_WAY, MC_CX_METAL_ONLY = 0, 1
@@ -137,6 +138,8 @@ def library():
         "mc_set_packet_order": ([vp, C.c_int], C.c_int),
         "mc_set_tally_moments": ([vp, C.c_int], C.c_int),
         "mc_set_launch_batch": ([vp, C.c_int64], C.c_int),
+        "mc_set_row_issue": ([vp, C.c_int], C.c_int),
+        "mc_row_issue": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
         "mc_tally_moment_info": ([vp, C.POINTER(C.c_int32), dp], C.c_int),
         "mc_thermal_stats": ([vp, i64p], C.c_int),
         "mc_mlp_set": ([vp, fp, C.c_int], C.c_int),
@@ -157,6 +160,7 @@ def library():
         "mc_trace_ray": ([vp, C.c_int, dp, dp, ip, C.c_int, ip], C.c_int),
         "mc_trace_ray_noalloc": ([vp, C.c_int, dp, dp, ip, C.c_int, ip], C.c_int),
         "mc_probe_scattered_rmw": ([C.c_int64, C.c_int, C.c_int64, dp, dp], C.c_int),
+        "mc_probe_row_atomics": ([C.c_int64, C.c_int, C.c_int64, dp, dp], C.c_int),
         "mc_leaf_list_read": ([C.c_char_p, C.c_int64, i64p, dp, dp, dp, dp, ip, dp, dp, dp, dp,
                                ip], C.c_int),
         "mc_species_setup": ([vp, C.c_int, vp], C.c_int),

@@ -196,6 +196,17 @@ class Model:
         """Launch records per batch of a pass (default 2^30 mixed, 2^26 FP64)."""
         check(self._L.mc_set_launch_batch(self._h, int(max_records)), "mc_set_launch_batch")
 
+    def set_row_issue(self, mode):
+        """Issue form of the mixed walk's moment rows: MC_ROWS_AUTO (default),
+        MC_ROWS_LANE or MC_ROWS_TOGETHER."""
+        check(self._L.mc_set_row_issue(self._h, int(mode)), "mc_set_row_issue")
+
+    def row_issue(self):
+        """(mode, form of the last pass: 1 transposed, 0 per lane, -1 none)."""
+        mode, last = C.c_int(), C.c_int()
+        check(self._L.mc_row_issue(self._h, C.byref(mode), C.byref(last)), "mc_row_issue")
+        return mode.value, last.value
+
     def set_precision(self, precision):
         check(self._L.mc_set_precision(self._h, int(precision)), "mc_set_precision")
         self.params.precision = int(precision)

@@ -280,14 +280,18 @@ def test_metal_warm_start_matches_bisection():
     assert np.all(sb["ne"][dark] < 1e-6 * sb["nH"][dark])
 
 
-def test_energy_moment_tallies_match_per_coefficient_sums():
+@pytest.mark.parametrize("rows", [1, 2])
+def test_energy_moment_tallies_match_per_coefficient_sums(rows):
     """MC_MIXED metal walk: the energy-moment tallies (four Chebyshev moments
     of the packet's ln-E sub-bin per segment, every coefficient sum rebuilt
     from its interpolant on the sub-bin) against the per-coefficient sums of
-    the same walk.  The walk is the same (same segments); H/He rate sums,
-    band J and the metal raw tallies agree to the interpolation tolerance."""
+    the same walk, with the moment rows issued per lane (MC_ROWS_LANE) and
+    transposed (MC_ROWS_TOGETHER).  The walk is the same (same segments);
+    H/He rate sums, band J and the metal raw tallies agree to the
+    interpolation tolerance."""
     params, nH = hii_thermal(n=10, nphotons=40000, metals=True, precision=MC_MIXED, diffuse=True)
     on, off = gpu_model(params, nH), gpu_model(params, nH)
+    on.set_row_issue(rows)
     off.set_tally_moments(False)
     nsub, err = on.tally_moment_info()
     print("sub-bins", nsub, "interpolation error", err)

@@ -77,13 +77,15 @@ def _check_thermal(orc, gm, near):
     np.testing.assert_allclose(mg, mo, rtol=2e-3, atol=5e-5)
 
 
-@pytest.mark.parametrize("prec", [MC_FP64, MC_MIXED])
-def test_multifractal_reduced_matches_oracle(prec):
+@pytest.mark.parametrize("prec,rows", [(MC_FP64, 0), (MC_MIXED, 1), (MC_MIXED, 2)])
+def test_multifractal_reduced_matches_oracle(prec, rows):
+    """rows: the mixed walk's moment rows issued per lane (1) or transposed (2)."""
     params, nH, octree, rhokap, tab = multifractal_cloud(nphotons=20000, precision=prec, lmax=6,
                                                          mass_thr=3e-4, ir_packets=1e5)
     assert 5000 < nH.size < 30000 and params["nsource"] == 16 and params["ext_on"]
     orc = oracle_model(params, nH, rhokap=rhokap, dust_table=tab, octree=octree)
     gm = gpu_model(params, nH, rhokap=rhokap, dust_table=tab, octree=octree)
+    gm.set_row_issue(rows)
     st, near = _ionised_near_sources(params, *octree[:3], nH, 6.0)
     for m in (orc, gm):
         m.set_state(**st)
@@ -105,25 +107,31 @@ def test_multifractal_moment_cache_matches_per_coefficient_sums():
     external sphere field, grains with scattering) with its energy-moment
     tallies merged in the per-wave cache agrees with the per-coefficient sums
     of the same walk in every rate sum, band J bin and the grain absorption,
-    in one launch batch and in batches of 4096 records."""
+    in one launch batch and in batches of 4096 records, with the cache's rows
+    issued per lane and transposed."""
+    from mochii_amd.lib import MC_ROWS_LANE, MC_ROWS_TOGETHER
     params, nH, octree, rhokap, tab = multifractal_cloud(nphotons=200_000, precision=MC_MIXED,
                                                          lmax=6, mass_thr=3e-4, ir_packets=1e5)
     st, _ = _ionised_near_sources(params, *octree[:3], nH, 6.0)
     out = {}
-    for mom, batch in ((False, None), (True, None), (True, 4096)):
+    cases = [(False, None, MC_ROWS_LANE)] + [(True, b, r) for b in (None, 4096)
+                                             for r in (MC_ROWS_LANE, MC_ROWS_TOGETHER)]
+    for mom, batch, rows in cases:
         gm = gpu_model(params, nH, rhokap=rhokap, dust_table=tab, octree=octree)
         gm.set_state(**st)
         gm.gas_opacity_fill()
         gm.set_tally_moments(mom)
+        gm.set_row_issue(rows)
         if batch:
             gm.set_launch_batch(batch)
         gm.ionizing_field_and_rates()
         if mom:
             assert gm.tally_moment_info()[0] > 16
-        out[(mom, batch)] = gm.tallies()
+            assert gm.row_issue()[1] == (1 if rows == MC_ROWS_TOGETHER else 0)
+        out[(mom, batch, rows)] = gm.tallies()
         gm.close()
-    ref = out[(False, None)]
-    for key in ((True, None), (True, 4096)):
+    ref = out[cases[0]]
+    for key in cases[1:]:
         t = out[key]
         assert (t["seg_direct"], t["seg_scattered"]) == (ref["seg_direct"], ref["seg_scattered"])
         for k in ("hhe", "jt", "dust_abs"):
