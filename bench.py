@@ -75,7 +75,7 @@ def parse():
     ap.add_argument("--trace-summary", default=None)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", default=None)
-    ap.add_argument("--kernel", default="k_walk<1, 1, 0>")
+    ap.add_argument("--kernel", default="k_walk<1, 1, 0,")
     ap.add_argument("--per-pass", type=int, default=0,
                     help="dispatches of the kernel in one pass (0: all of them)")
     ap.add_argument("--skip-last", type=int, default=0)
@@ -238,7 +238,7 @@ def timed_steps(name, params, nH, octree, rhokap, tab, args, mlp):
         m.ionizing_field_and_rates()
         m.gas_thermal_update()
         log(f"  warm-up step {k + 1}: {m.timing()}")
-    walk, solve, segs, evals, brk, step_s, clos, natom = [], [], [], [], [], [], [], []
+    walk, solve, segs, evals, brk, step_s, clos, natom, rows = [], [], [], [], [], [], [], [], []
     t0 = time.perf_counter()
     for k in range(args.steps):
         ts = time.perf_counter()
@@ -247,6 +247,7 @@ def timed_steps(name, params, nH, octree, rhokap, tab, args, mlp):
         walk.append(tm["walk_ms"])
         segs.append(m.segment_counts())
         natom.append(m.walk_atomics())
+        rows.append(m.row_issue()[1])
         led = m.energy_ledger()
         clos.append(abs(led[1] + led[2] - led[0]) / led[0])
         m.gas_thermal_update()
@@ -269,7 +270,10 @@ def timed_steps(name, params, nH, octree, rhokap, tab, args, mlp):
                max_energy_closure=float(np.max(clos)),
                net_rate_evals_per_bracketed_leaf=float(np.sum(evals) / max(np.sum(brk), 1)),
                setup_s=t_setup, moment_sub_bins=int(m.tally_moment_info()[0]),
-               step_ms=[1e3 * x for x in step_s])
+               step_ms=[1e3 * x for x in step_s],
+               # issue form of the moment rows per timed step (1 transposed,
+               # 0 per lane, -1 none)
+               row_issue=rows)
     return m, out
 
 
@@ -427,30 +431,36 @@ def cpu_baseline(params, nH, octree, rhokap, tab, state, seconds):
     t_setup = time.perf_counter() - t0
     log(f"[cpu] oracle set up in {t_setup:.1f} s, {nthr} threads")
     N = int(params["nphotons"])
+    # the pass's fixed cost (per-thread tally set-up and reduction) from an
+    # empty pass, a first rate from 20000 packets, then a sample sized to
+    # ~0.6 `seconds` of packet work, whose rate is the value's
+    t0 = time.perf_counter()
+    orc.transport_pass(1, 1, nthr)
+    fixed = time.perf_counter() - t0
     n1 = 20_000
     t0 = time.perf_counter()
     orc.transport_pass(1, n1 + 1, nthr)
     t1 = time.perf_counter() - t0
-    n2 = int(min(N, max(4 * n1, n1 * (0.6 * seconds) / max(t1, 1e-3))))
+    rate1 = n1 / max(t1 - fixed, 0.05)
+    n2 = int(min(N, max(4 * n1, rate1 * 0.6 * seconds)))
     t0 = time.perf_counter()
     orc.transport_pass(1, n2 + 1, nthr)
     t2 = time.perf_counter() - t0
-    rate = (n2 - n1) / max(t2 - t1, 1e-9)
-    fixed = max(t1 - n1 / rate, 0.0)
+    rate = n2 / max(t2 - fixed, 1e-9)
     orc.rates_compute()
     t0 = time.perf_counter()
     orc.thermal_update(nthreads=nthr)
     t_th = time.perf_counter() - t0
     step = N / rate + fixed + t_th
-    log(f"[cpu] {n1} packets {t1:.1f} s, {n2} packets {t2:.1f} s -> {rate:.3e} packets/s, "
-        f"fixed {fixed:.1f} s, thermal {t_th:.1f} s")
+    log(f"[cpu] empty pass {fixed:.1f} s, {n1} packets {t1:.1f} s, {n2} packets {t2:.1f} s -> "
+        f"{rate:.3e} packets/s, thermal {t_th:.1f} s")
     return {"value": N / step, "unit": "packets/s", "cores": nthr, "kind": "port",
             "nproc": ncpu, "threads_used": nthr,
             "transport_packets_per_s": rate, "step_s_projected": step,
-            "sample": f"FP64 C oracle on the same model and state: transport passes of the first "
-                      f"{n1} and {n2} of the {N} packet ids ({t1:.1f} s, {t2:.1f} s; their "
-                      f"difference gives the per-packet rate, the remainder the pass's fixed "
-                      f"tally cost) and one thermal update of all {nH.size} leaves "
+            "sample": f"FP64 C oracle on the same model and state: an empty transport pass "
+                      f"(its fixed tally set-up and reduction, {fixed:.1f} s), a pass of the "
+                      f"first {n2} of the {N} packet ids ({t2:.1f} s; less the fixed cost, the "
+                      f"per-packet rate) and one thermal update of all {nH.size} leaves "
                       f"({t_th:.1f} s); a whole step projected to {N:.0e} packets, {nthr} "
                       f"threads (the box's CPU share)"}
 
@@ -616,7 +626,11 @@ def main():
         "surrogate_training": train_info,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_walk<MC_MIXED, kFaceExit, 0>",
+                     "kernel": "k_walk<MC_MIXED, kFaceExit, 0, %s>" % (
+                         "kRowsTogether" if head.get("row_issue") and
+                         min(head["row_issue"]) == 1 else
+                         "kRowsLane" if head.get("row_issue") and
+                         max(head["row_issue"]) == 0 else "kRowsLane/kRowsTogether"),
                      "bytes_per_segment": bps, "segments_per_pass": nseg,
                      "walk_ms_per_pass": head["walk_ms"],
                      "traffic_bytes_per_segment": traffic_seg,

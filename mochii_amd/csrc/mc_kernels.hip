@@ -3469,10 +3469,12 @@ int sum_partials(mc_ctx *c, int nblk, double out[3])
 
 // ---------------------------------------------------------------------------
 // fractions of the device's atomic request rate at which the moment rows'
-// issue form switches (measured on the 20.4 M-leaf multifractal model: the
-// per-lane form's first pass runs at 53% of the rate and is faster there, an
-// ionised pass at 90%, where the transposed form is 26% faster)
-constexpr double kRowsUpFrac = 0.7, kRowsDownFrac = 0.4;
+// issue form switches.  Measured on the 20.4 M-leaf multifractal model
+// converging from the neutral start, both forms pass by pass: the per-lane
+// form is faster up to 69% of the rate (its first pass, at 53%, by 6%),
+// the transposed form from 70% on (by 10% at the converged state, where the
+// per-lane form runs at 80%; by 26% with every leaf ionised, at 90%)
+constexpr double kRowsUpFrac = 0.68, kRowsDownFrac = 0.4;
 
 int mc_run_pass(mc_ctx *c)
 {
