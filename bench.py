@@ -76,6 +76,12 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", default=None)
     ap.add_argument("--kernel", default="k_walk<1, 1, 0,")
+    ap.add_argument("--state", choices=("start", "ionised"), default="start",
+                    help="state of the timed steps: the workload's own start, or every leaf "
+                         "ionised (x_HI 1e-3, He singly ionised, 1e4 K: the walk's regime late "
+                         "in a converging run)")
+    ap.add_argument("--row-issue", choices=("auto", "lane", "together"), default="auto",
+                    help="issue form of the mixed walk's moment rows (mc_set_row_issue)")
     ap.add_argument("--per-pass", type=int, default=0,
                     help="dispatches of the kernel in one pass (0: all of them)")
     ap.add_argument("--skip-last", type=int, default=0)
@@ -234,6 +240,14 @@ def timed_steps(name, params, nH, octree, rhokap, tab, args, mlp):
     m = make_model(params, nH, octree, prec, warm, mlp, rhokap, tab)
     t_setup = time.perf_counter() - t0
     log(f"[{name}] set up in {t_setup:.1f} s ({nH.size} leaves)")
+    from mochii_amd.lib import MC_ROWS_AUTO, MC_ROWS_LANE, MC_ROWS_TOGETHER
+    m.set_row_issue({"auto": MC_ROWS_AUTO, "lane": MC_ROWS_LANE,
+                     "together": MC_ROWS_TOGETHER}[args.row_issue])
+    if args.state == "ionised":
+        n = nH.size
+        m.set_state(xHI=np.full(n, 1e-3), xHeI=np.full(n, 1e-3), xHeII=np.full(n, 0.9),
+                    ne=nH * (1.0 - 1e-3 + 0.1 * (0.9 + 2 * 0.099)), Te=np.full(n, 1e4))
+        m.gas_opacity_fill()
     for k in range(args.warmup):
         m.ionizing_field_and_rates()
         m.gas_thermal_update()
@@ -613,7 +627,10 @@ def main():
                                "thermal balance (local_ne cooling), dust with HG scattering and "
                                "the dust-IR band, synthetic data",
                    "leaves": int(nH.size), "packets_per_iteration": int(params["nphotons"]),
-                   "ir_packets_per_pass": int(args.ir_packets)},
+                   "ir_packets_per_pass": int(args.ir_packets),
+                   "state": ("every leaf ionised" if args.state == "ionised" else
+                             "the workload's start (neutral gas)"),
+                   "row_issue": args.row_issue},
         "forms": forms,
         "step_speedup_vs_fp64": (sum(f64["step_ms"]) / sum(head["step_ms"][:len(f64["step_ms"])]))
                                 if f64 else None,
