@@ -431,9 +431,10 @@ def turbulent_leg(args, mlp):
 # ---- the CPU baseline -------------------------------------------------------------
 def cpu_baseline(params, nH, octree, rhokap, tab, state, seconds):
     """The FP64 oracle on the host cores, a whole step in the value's unit:
-    its transport rate from two packet samples of the same pass (the
-    difference removes the pass's fixed tally set-up and reduction, which is
-    then added once) and its thermal update over every leaf."""
+    an empty pass gives the pass's fixed cost (per-thread tally set-up and
+    reduction), a sized sample of the same pass, less that fixed cost, gives
+    the per-packet transport rate; the step is the full packet count at that
+    rate, plus the fixed cost once, plus one thermal update over every leaf."""
     from tests.cases import oracle_model
     ncpu = os.cpu_count() or 1
     nthr = min(ncpu, int(os.environ.get("OMP_NUM_THREADS", ncpu)))
