@@ -356,8 +356,12 @@ __global__ __launch_bounds__(kBlock) void k_launch(KArgs a, int nlds)
     const DevTables *T = a.tab;
     const int64_t id = a.first_id + i;
     const uint32_t q = (uint32_t)(id - 1);   // index bits beyond 32 carry no direction number
+    // a loop bounded by the run-time dimension count would leave u in
+    // per-lane scratch memory (56 B written and read back per record):
+    // unrolled over the fixed maximum, it stays in registers
     double u[7];
-    for (int d = 0; d < T->qmc_ndim; ++d) u[d] = qmc_u(T, q, d, 0);
+#pragma unroll
+    for (int d = 0; d < 7; ++d) u[d] = d < T->qmc_ndim ? qmc_u(T, q, d, 0) : 0.0;
     int comp = 0;
     double umu, uphi, ue;
     if (!T->multi && T->nsource == 1) {
